@@ -1,0 +1,159 @@
+/*
+ * niftyreg_hip.h -- C ABI of libnrhip.so, the MI355X (gfx950) implementation of NiftyReg's
+ * F3D objective/gradient step and Aladin's block-matching + LTS step.
+ *
+ * Plain C types only: images are planar float arrays (x fastest; a vector image holds all x
+ * components, then all y, then all z, as the reference's host nifti layout), geometries are
+ * nrh_geom values, handles are opaque.  Every call returns NRH_OK (0) or an error code; the
+ * message of the last error of the calling thread is returned by nrh_last_error().
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * NiftyReg source tree).  A handle owns one HIP stream; calls on different handles may run
+ * concurrently from different host threads, calls on one handle must be serialised.
+ */
+#ifndef NIFTYREG_HIP_H
+#define NIFTYREG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRH_OK 0
+#define NRH_ERR_ARG 1          /* invalid argument (NR_FATAL_ERROR in the reference) */
+#define NRH_ERR_HIP 2          /* HIP runtime error */
+#define NRH_ERR_STATE 3        /* call out of order (e.g. gradient before histogram) */
+#define NRH_ERR_NOMEM 4        /* device allocation failed */
+#define NRH_ERR_UNSUPPORTED 5  /* configuration the library does not implement */
+#define NRH_ERR_NUMERIC 6      /* degenerate data (no active voxel / block, singular fit) */
+
+/* Geometry of a 3-D image as the reference reads it from a nifti header.
+ * xyz / ijk: the matrix pair the reference selects (sform if sform_code > 0, else qform;
+ * reg-lib/Content.h:31-36), row-major, voxel->mm and mm->voxel.  dx, dy, dz: pixdim[1..3].
+ * sform: 1 when xyz/ijk are the sform (changes the node-spacing ratio of
+ * reg_voxelCentricToNodeCentric, reg-lib/cpu/_reg_localTrans.cpp:1595-1604). */
+typedef struct nrh_geom {
+    int nx, ny, nz;
+    float dx, dy, dz;
+    float xyz[16];
+    float ijk[16];
+    int sform;
+} nrh_geom;
+
+/* ------------------------------------------------------------------ library */
+const char *nrh_last_error(void);
+int nrh_version(void);
+/* Device selection; replaces CudaContext::SetCudaIdx (reg-lib/cuda/CudaContext.cpp:17-25) and
+ * Platform::SetGpuIdx (reg-lib/Platform.cpp:66-93). */
+int nrh_device_count(int *count);
+int nrh_init(int device);
+
+/* ------------------------------------------------------------------ host helpers (no device use) */
+/* nifti_mat44_inverse (reg-io/niftilib/nifti1_io.c:1680-1717) */
+int nrh_mat44_inverse(const float in[16], float out[16]);
+/* reg_createControlPointGrid<float> (reg-lib/cpu/_reg_localTrans.cpp:21-128): grid geometry and
+ * the identity control-point positions (planar, 3*G floats; may be NULL to query the size). */
+int nrh_create_control_point_grid(const nrh_geom *ref, const float spacing_mm[3], nrh_geom *grid,
+                                  float *identity);
+
+/* ------------------------------------------------------------------ F3D */
+typedef struct nrh_f3d nrh_f3d;
+
+typedef struct nrh_f3d_params {
+    int ref_bins;             /* internal NMI bins = user bins + 4 (reg-lib/_reg_base.cpp:567-585) */
+    int flo_bins;
+    float padding;            /* warped padding value (NaN by default, reg-lib/_reg_base.cpp:46) */
+    float similarity_weight;  /* reg_base::similarityWeight after CheckParameters */
+    float be_weight;          /* bending-energy weight of the current level */
+    float le_weight;          /* linear-energy weight; must be 0 (not implemented) */
+    int image_storage;        /* 0: float32 images in HBM, 1: float16 storage, float32 compute */
+} nrh_f3d_params;
+
+/* F3dContent + DefContent + Content construction (reg-lib/F3dContent.cpp:4-19,
+ * reg-lib/DefContent.cpp:5-15, reg-lib/Content.cpp:5-22) and the device upload of
+ * CudaF3dContent (reg-lib/cuda/CudaF3dContent.cpp:4-30).  mask: int per reference voxel,
+ * > -1 active (NULL = all active).  cpp: grid geometry; the positions are set with
+ * nrh_f3d_set_control_points. */
+int nrh_f3d_create(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
+                   const int *mask, const nrh_geom *cpp, const nrh_f3d_params *params, nrh_f3d **out);
+int nrh_f3d_destroy(nrh_f3d *h);
+int nrh_f3d_set_params(nrh_f3d *h, const nrh_f3d_params *params);
+/* F3dContent::UpdateControlPointGrid / GetControlPointGrid (reg-lib/F3dContent.h:17-25) */
+int nrh_f3d_set_control_points(nrh_f3d *h, const float *planar);
+int nrh_f3d_get_control_points(nrh_f3d *h, float *planar);
+/* reg_nmi::InitialiseMeasure's reg_intensityRescale of both images (reg-lib/cpu/_reg_nmi.cpp:91-158,
+ * reg-lib/cpu/_reg_tools.cpp:93-208): rescale reference and floating to [lo, hi] on the device. */
+int nrh_f3d_rescale_intensities(nrh_f3d *h, float ref_lo, float ref_hi, float flo_lo, float flo_hi);
+int nrh_f3d_get_images(nrh_f3d *h, float *ref, float *flo);
+
+/* Per-operation entry points, one per Compute / reg_nmi virtual (reg-lib/Compute.h:11-44,
+ * reg-lib/cpu/_reg_measure.h:59-86).  They read and write device-resident buffers of the
+ * handle; the nrh_f3d_download_* calls copy those buffers to the host. */
+int nrh_f3d_deformation_field(nrh_f3d *h);                 /* Compute::GetDeformationField(false, true) */
+int nrh_f3d_resample(nrh_f3d *h);                          /* Compute::ResampleImage(1, padding) */
+int nrh_f3d_image_gradient(nrh_f3d *h);                    /* Compute::GetImageGradient(1, padding, 0) */
+int nrh_f3d_nmi_value(nrh_f3d *h, double *nmi, double entropies[4]); /* reg_nmi::GetSimilarityMeasureValueFw */
+int nrh_f3d_zero_voxel_gradient(nrh_f3d *h);               /* DefContent::ZeroVoxelBasedMeasureGradient */
+int nrh_f3d_nmi_voxel_gradient(nrh_f3d *h, double weight); /* reg_nmi::GetVoxelBasedSimilarityMeasureGradientFw */
+int nrh_f3d_convolve_voxel_gradient(nrh_f3d *h);           /* Compute::ConvolveImage (private) */
+int nrh_f3d_voxel_to_node(nrh_f3d *h, float weight);       /* Compute::VoxelCentricToNodeCentric */
+int nrh_f3d_zero_node_gradient(nrh_f3d *h);                /* F3dContent::ZeroTransformationGradient */
+int nrh_f3d_bending_energy(nrh_f3d *h, double *value);     /* Compute::ApproxBendingEnergy */
+int nrh_f3d_bending_energy_gradient(nrh_f3d *h, float weight); /* Compute::ApproxBendingEnergyGradient */
+int nrh_f3d_max_length(nrh_f3d *h, double *value);         /* Compute::GetMaximalLength */
+int nrh_f3d_normalise_gradient(nrh_f3d *h, double max_length); /* Compute::NormaliseGradient */
+
+int nrh_f3d_download_field(nrh_f3d *h, float *planar);          /* 3*N */
+int nrh_f3d_download_warped(nrh_f3d *h, float *warped);         /* N */
+int nrh_f3d_download_warped_gradient(nrh_f3d *h, float *planar);/* 3*N */
+int nrh_f3d_download_voxel_gradient(nrh_f3d *h, float *planar); /* 3*N */
+int nrh_f3d_download_node_gradient(nrh_f3d *h, float *planar);  /* 3*G */
+int nrh_f3d_upload_node_gradient(nrh_f3d *h, const float *planar);
+/* reg_nmi's jointHistogramPro / jointHistogramLog (rb*fb + rb + fb doubles each) */
+int nrh_f3d_download_histogram(nrh_f3d *h, double *pro, double *logh);
+
+/* Fused objective: reg_f3d::GetObjectiveFunctionValue (reg-lib/_reg_f3d.cpp:443-463) in one
+ * device sweep (field, warp and joint histogram fused; no field or warped image is written).
+ * terms: wMeasure, wBE, wLE (may be NULL). */
+int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]);
+/* Fused gradient: reg_f3d::GetObjectiveFunctionGradient (reg-lib/_reg_f3d.cpp:553-577) with the
+ * similarity gradient, convolution and node projection in one device sweep.  The histogram
+ * is recomputed unless the last nrh_f3d_objective ran on the same control points.  When
+ * max_length is non-NULL the gradient is then normalised as reg_f3d::NormaliseGradient
+ * (reg-lib/_reg_f3d.cpp:395-411) and the maximal length returned. */
+int nrh_f3d_gradient(nrh_f3d *h, double *max_length);
+
+/* Device-side optimiser state (replaces CudaOptimiser / CudaConjugateGradient,
+ * reg-lib/cuda/CudaOptimiser.cu:72-86,198-304): best DOF copy, PR conjugate gradient update
+ * (first != 0 initialises), DOF update cur = best + scale * grad. */
+int nrh_f3d_store_best(nrh_f3d *h);
+int nrh_f3d_restore_best(nrh_f3d *h);
+int nrh_f3d_conjugate_gradient(nrh_f3d *h, int first);
+int nrh_f3d_update_control_points(nrh_f3d *h, float scale);
+
+/* Full F3D registration on one pair: reg_base<float>::Run (reg-lib/_reg_base.cpp:727-819) with
+ * reg_f3d's level set-up (reg-lib/_reg_f3d.cpp:92-133,142-290), NMI, bending energy, CG and
+ * line search.  The pyramid and the control-point grid are built on the host; all
+ * evaluations run on the device.  cpp_out receives the final grid (3*G floats, G from
+ * grid_out), final_nmi the NMI of the last level. */
+typedef struct nrh_f3d_config {
+    int bins;              /* user bins (64) */
+    float be_weight;       /* -be (0.001) */
+    float le_weight;       /* -le: must be 0 */
+    float spacing;         /* -sx: negative = voxels (-5) */
+    int levels;            /* -ln (3) */
+    int levels_to_perform; /* -lp (0 = all) */
+    int max_iterations;    /* -maxit: final-level budget (150) */
+    float padding;         /* -pad (NaN) */
+    int use_pyramid;       /* 1 unless -nopy */
+    int image_storage;     /* 0 float32, 1 float16 */
+} nrh_f3d_config;
+int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
+                     const nrh_f3d_config *cfg, nrh_geom *grid_out, float *cpp_out, size_t cpp_capacity,
+                     double *final_nmi, int *iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

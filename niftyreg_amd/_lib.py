@@ -1,0 +1,89 @@
+"""ctypes binding of libnrhip.so (include/niftyreg_hip.h).
+
+The library is loaded from this package directory only; a missing library raises
+ImportError-style errors on first use so that no caller can silently run without the HIP path.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnrhip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "niftyreg_hip.h")
+
+NRH_OK = 0
+ERR_NAMES = {1: "ARG", 2: "HIP", 3: "STATE", 4: "NOMEM", 5: "UNSUPPORTED", 6: "NUMERIC"}
+
+
+class NiftyRegError(RuntimeError):
+    """Raised for a non-zero status of the C ABI (the reference raises std::runtime_error)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{ERR_NAMES.get(code, code)}] {message}")
+        self.code = code
+
+
+class nrh_geom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float),
+                ("xyz", ctypes.c_float * 16), ("ijk", ctypes.c_float * 16), ("sform", ctypes.c_int)]
+
+
+class nrh_f3d_params(ctypes.Structure):
+    _fields_ = [("ref_bins", ctypes.c_int), ("flo_bins", ctypes.c_int), ("padding", ctypes.c_float),
+                ("similarity_weight", ctypes.c_float), ("be_weight", ctypes.c_float),
+                ("le_weight", ctypes.c_float), ("image_storage", ctypes.c_int)]
+
+
+class nrh_f3d_config(ctypes.Structure):
+    _fields_ = [("bins", ctypes.c_int), ("be_weight", ctypes.c_float), ("le_weight", ctypes.c_float),
+                ("spacing", ctypes.c_float), ("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int),
+                ("max_iterations", ctypes.c_int), ("padding", ctypes.c_float), ("use_pyramid", ctypes.c_int),
+                ("image_storage", ctypes.c_int)]
+
+
+_lib = None
+
+
+def declared_symbols():
+    """Function names declared in include/niftyreg_hip.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(nrh_[a-z0-9_]+)\s*\(", text)))
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.nrh_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def check(rc):
+    if rc != NRH_OK:
+        msg = load().nrh_last_error().decode(errors="replace")
+        raise NiftyRegError(rc, msg)
+
+
+def call(name, *args):
+    fn = getattr(load(), name)
+    check(fn(*args))
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))

@@ -1,0 +1,83 @@
+// Objective and gradient entry points of the F3D handle: the fused sweep when the geometry
+// allows it, otherwise the per-operation chain of the reference's Compute calls.
+
+#include "f3d_handle.hpp"
+
+namespace nrh {
+
+// reg_f3d::WarpFloatingImage + reg_nmi value (reg-lib/_reg_base.cpp:688-709, reg-lib/cpu/_reg_nmi.cpp:160-398)
+static void histogram_at_current_points(nrh_f3d *h) {
+    if (fused_supported(h)) {
+        fused_histogram(h);
+    } else {
+        op_spline_field(h);
+        op_resample(h);
+        op_joint_histogram(h, h->warped.p);
+        op_nmi_finalize(h);
+        h->histVersion = h->cppVersion;
+    }
+}
+
+}  // namespace nrh
+
+using namespace nrh;
+
+extern "C" {
+
+int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
+    return guard([&] {
+        NRH_REQUIRE(h && value, NRH_ERR_ARG, "null argument");
+        const nrh_f3d_params& p = h->params;
+        if (p.similarity_weight > 0) histogram_at_current_points(h);
+        if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
+        NRH_HIP(hipMemcpyAsync(h->pinned, h->ent.p, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        NRH_HIP(hipMemcpyAsync(h->pinned + 5, h->scalars.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        NRH_HIP(hipStreamSynchronize(h->stream));
+        const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * h->pinned[4] : 0.0;
+        const double wBE = p.be_weight > 0 ? (double)p.be_weight * h->pinned[5] : 0.0;
+        *value = wMeasure - wBE;
+        if (terms) {
+            terms[0] = wMeasure;
+            terms[1] = wBE;
+            terms[2] = 0.0;
+        }
+    });
+}
+
+int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
+    return guard([&] {
+        NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
+        const nrh_f3d_params& p = h->params;
+        if (p.similarity_weight > 0) {
+            if (fused_supported(h)) {
+                if (h->histVersion != h->cppVersion) fused_histogram(h);
+                fused_gradient(h, p.similarity_weight);
+            } else {
+                // reg_f3d::GetSimilarityMeasureGradient chain (reg-lib/_reg_f3d.cpp:353-362)
+                op_spline_field(h);
+                op_resample(h);
+                op_joint_histogram(h, h->warped.p);
+                op_nmi_finalize(h);
+                h->histVersion = h->cppVersion;
+                op_image_gradient(h);
+                ensure_voxel_grad(h);
+                NRH_HIP(hipMemsetAsync(h->voxelGrad.p, 0, h->voxelGrad.bytes(), h->stream));
+                op_nmi_voxel_gradient(h, 1.0);
+                op_convolve_voxel_gradient(h);
+                op_voxel_to_node(h, p.similarity_weight);
+            }
+        } else {
+            op_fill_f32(h->nodeGrad.p, 3 * h->G, 0.f, h->stream);
+        }
+        if (p.be_weight > 0) op_bending_energy_gradient(h, p.be_weight);
+        if (max_length) {
+            op_max_length(h);
+            op_normalise_device(h);
+            *max_length = read_scalar(h, h->scalars.p + 1);
+        } else {
+            NRH_HIP(hipStreamSynchronize(h->stream));
+        }
+    });
+}
+
+}  // extern "C"

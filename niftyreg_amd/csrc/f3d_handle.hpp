@@ -1,0 +1,88 @@
+// State of one F3D registration pair on the device (the HIP counterpart of F3dContent).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include "common.hpp"
+
+struct nrh_f3d {
+    nrh_geom ref{}, flo{}, cpp{};
+    nrh_f3d_params params{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    size_t N = 0, Nf = 0, G = 0;
+
+    // images (float32 storage, or float16 storage when params.image_storage == 1)
+    nrh::DevBuf<float> refImg, floImg;
+    nrh::DevBuf<__half> refH, floH;
+    nrh::DevBuf<int> mask;  // empty: every voxel active
+
+    // control points (planar [3][G]) and optimiser arrays
+    nrh::DevBuf<float> cppPos, best, nodeGrad, cgG, cgH;
+
+    // per-operation buffers, allocated on first use
+    nrh::DevBuf<float> field, warped, warpedGrad, voxelGrad;
+    nrh::DevBuf<float> convTmp, convDens, convDensTmp, convKernel;
+    nrh::DevBuf<uint8_t> convNan;
+    nrh::DevBuf<float> beDeriv;
+
+    // joint histogram: integer counts, reg_nmi's Pro / Log arrays, entropies
+    nrh::DevBuf<uint32_t> histCount;
+    nrh::DevBuf<double> histPro, histLog;
+    nrh::DevBuf<double> ent;       // H_r, H_f, H_rf, total, nmi
+    nrh::DevBuf<float> histLogF;   // rb*fb joint log + rb + fb marginal logs, float
+
+    // reduction scratch
+    nrh::DevBuf<double> partials;  // per-block partial sums
+    nrh::DevBuf<double> scalars;   // device-side scalar results
+    nrh::DevBuf<uint32_t> maxBits;
+
+    // fused node-gradient partial sums (one window per tile)
+    nrh::DevBuf<float> nodePartials;
+
+    double *pinned = nullptr;  // host-pinned scalar read-back
+
+    uint64_t cppVersion = 1;   // bumped on every control-point change
+    uint64_t histVersion = 0;  // cppVersion the histogram was computed at
+    bool halfDirty = true;     // float16 copies of the images need refreshing
+
+    int totalBins() const { return params.ref_bins * params.flo_bins + params.ref_bins + params.flo_bins; }
+};
+
+namespace nrh {
+
+void ensure_field(nrh_f3d *h);
+void ensure_warped(nrh_f3d *h);
+void ensure_warped_grad(nrh_f3d *h);
+void ensure_voxel_grad(nrh_f3d *h);
+const float *ref_ptr(nrh_f3d *h);  // float32 view (converted from float16 storage if needed)
+
+// per-operation launches (f3d_ops.hip)
+void op_spline_field(nrh_f3d *h);
+void op_resample(nrh_f3d *h);
+void op_image_gradient(nrh_f3d *h);
+void op_joint_histogram(nrh_f3d *h, const float *warped);
+void op_nmi_finalize(nrh_f3d *h);
+void op_nmi_voxel_gradient(nrh_f3d *h, double weight);
+void op_convolve_voxel_gradient(nrh_f3d *h);
+void op_voxel_to_node(nrh_f3d *h, float weight);
+void op_bending_energy(nrh_f3d *h, double *out_device);
+void op_bending_energy_gradient(nrh_f3d *h, float weight);
+void op_max_length(nrh_f3d *h);
+void op_normalise(nrh_f3d *h, double max_length);
+void op_rescale(nrh_f3d *h, float *data, size_t n, float lo, float hi);
+void op_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+void op_fill_f32(float *p, size_t n, float v, hipStream_t s);
+void op_cg(nrh_f3d *h, int first);
+void op_update_dof(nrh_f3d *h, float scale);
+double read_scalar(nrh_f3d *h, const double *dev);
+
+float node_weight_ratio(const nrh_f3d *h, float weight);
+void op_normalise_device(nrh_f3d *h);
+
+// fused evaluation (f3d_fused.hip)
+bool fused_supported(nrh_f3d *h);
+void fused_release(const nrh_f3d *h);
+void fused_histogram(nrh_f3d *h);
+void fused_gradient(nrh_f3d *h, float weight);
+
+}  // namespace nrh

@@ -1,0 +1,899 @@
+// Per-operation F3D kernels: one launch per reference Compute / reg_nmi virtual.
+//
+// These kernels keep the reference's rounding (float/double split, summation order, no fused
+// multiply-adds) so that each stage can be compared element by element with the CPU path.
+// The hot loop of a registration uses the fused kernels of f3d_fused.hip instead.
+
+#include "device.hpp"
+#include "f3d_handle.hpp"
+#include <cmath>
+
+namespace nrh {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned blocks_for(size_t n, int block = kBlock) {
+    size_t b = (n + block - 1) / block;
+    return (unsigned)(b > 0 ? b : 1);
+}
+
+// ------------------------------------------------------------------ deformation field
+// reg_cubic_spline_getDeformationField3D, non-composition (reg-lib/cpu/_reg_localTrans.cpp:1080-1453)
+__global__ void __launch_bounds__(kBlock)
+k_spline_field(int nx, int ny, int nz, int gnx, int gny, size_t G, const float *__restrict__ cpp,
+               const int *__restrict__ mask, float gx, float gy, float gz, int lut,
+               float *__restrict__ field, size_t N) {
+#pragma clang fp contract(off)
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    const int x = (int)(idx % nx);
+    const int y = (int)((idx / nx) % ny);
+    const int z = (int)(idx / ((size_t)nx * ny));
+    if (mask && mask[idx] < 0) {
+        if (!lut) { field[idx] = 0.f; field[idx + N] = 0.f; field[idx + 2 * N] = 0.f; }
+        return;
+    }
+    int px, py, pz;
+    float tx, ty, tz, xb[4], yb[4], zb[4];
+    cell_of(x, gx, lut, px, tx);
+    cell_of(y, gy, lut, py, ty);
+    cell_of(z, gz, lut, pz, tz);
+    bspline_basis(tx, xb);
+    bspline_basis(ty, yb);
+    bspline_basis(tz, zb);
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    for (int c = 0; c < 4; ++c)
+        for (int b = 0; b < 4; ++b) {
+            const float yzw = yb[b] * zb[c];
+            const size_t row = ((size_t)(pz + c) * gny + (py + b)) * gnx + px;
+            for (int a = 0; a < 4; ++a) {
+                const float w = xb[a] * yzw;
+                r0 += cpp[row + a] * w;
+                r1 += cpp[row + a + G] * w;
+                r2 += cpp[row + a + 2 * G] * w;
+            }
+        }
+    field[idx] = r0;
+    field[idx + N] = r1;
+    field[idx + 2 * N] = r2;
+}
+
+// ------------------------------------------------------------------ resampling
+// ResampleImage3D, linear kernel (reg-lib/cpu/_reg_resampling.cpp:316-464)
+__global__ void __launch_bounds__(kBlock)
+k_resample_linear(const float *__restrict__ field, size_t N, const int *__restrict__ mask,
+                  const float *__restrict__ flo, int fnx, int fny, int fnz, Affine ijk, float pad,
+                  float *__restrict__ warped) {
+#pragma clang fp contract(off)
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    if (mask && mask[idx] < 0) return;
+    const float3 p = affine_apply_d(ijk, field[idx], field[idx + N], field[idx + 2 * N]);
+    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
+    float rx = p.x - (float)px, ry = p.y - (float)py, rz = p.z - (float)pz;
+    if (rx < 0) rx = 0;
+    if (ry < 0) ry = 0;
+    if (rz < 0) rz = 0;
+    const float xb[2] = { 1.f - rx, rx }, yb[2] = { 1.f - ry, ry }, zb[2] = { 1.f - rz, rz };
+    const bool inside = px > -1 && px + 1 < fnx && py > -1 && py + 1 < fny && pz > -1 && pz + 1 < fnz;
+    float intensity = 0.f;
+    for (int c = 0; c < 2; ++c) {
+        const int Z = pz + c;
+        float yT = 0.f;
+        for (int b = 0; b < 2; ++b) {
+            const int Y = py + b;
+            float xT = 0.f;
+            for (int a = 0; a < 2; ++a) {
+                const int X = px + a;
+                float v;
+                if (inside || (X > -1 && X < fnx && Y > -1 && Y < fny && Z > -1 && Z < fnz))
+                    v = flo[((size_t)Z * fny + Y) * fnx + X];
+                else v = pad;
+                xT += v * xb[a];
+            }
+            yT += xT * yb[b];
+        }
+        intensity += yT * zb[c];
+    }
+    warped[idx] = intensity;
+}
+
+// TrilinearImageGradient (reg-lib/cpu/_reg_resampling.cpp:1797-1915)
+__global__ void __launch_bounds__(kBlock)
+k_image_gradient_linear(const float *__restrict__ field, size_t N, const int *__restrict__ mask,
+                        const float *__restrict__ flo, int fnx, int fny, int fnz, Affine ijk, float pad,
+                        float *__restrict__ grad) {
+#pragma clang fp contract(off)
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    if (!mask || mask[idx] > -1) {
+        const float3 p = affine_apply_d(ijk, field[idx], field[idx + N], field[idx + 2 * N]);
+        const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
+        float r = p.x - (float)px;
+        const float xb[2] = { (float)(1.0 - (double)r), r };
+        r = p.y - (float)py;
+        const float yb[2] = { (float)(1.0 - (double)r), r };
+        r = p.z - (float)pz;
+        const float zb[2] = { (float)(1.0 - (double)r), r };
+        const float deriv[2] = { -1.f, 1.f };
+        for (int c = 0; c < 2; ++c) {
+            const int Z = pz + c;
+            const bool zIn = Z > -1 && Z < fnz;
+            float xx = 0.f, yy = 0.f, zz = 0.f;
+            for (int b = 0; b < 2; ++b) {
+                const int Y = py + b;
+                const bool yzIn = zIn && Y > -1 && Y < fny;
+                float xT = 0.f, yT = 0.f;
+                for (int a = 0; a < 2; ++a) {
+                    const int X = px + a;
+                    const float coeff = (yzIn && X > -1 && X < fnx) ? flo[((size_t)Z * fny + Y) * fnx + X] : pad;
+                    xT += coeff * deriv[a];
+                    yT += coeff * xb[a];
+                }
+                xx += xT * yb[b];
+                yy += yT * deriv[b];
+                zz += yT * yb[b];
+            }
+            g0 += xx * zb[c];
+            g1 += yy * zb[c];
+            g2 += zz * deriv[c];
+        }
+        if (g0 != g0) g0 = 0.f;
+        if (g1 != g1) g1 = 0.f;
+        if (g2 != g2) g2 = 0.f;
+    }
+    grad[idx] = g0;
+    grad[idx + N] = g1;
+    grad[idx + 2 * N] = g2;
+}
+
+// ------------------------------------------------------------------ joint histogram
+// Approximate-Parzen count fill of reg_getNmiValue (reg-lib/cpu/_reg_nmi.cpp:211-248):
+// integer counts, privatised per workgroup in LDS, merged with integer atomics (exact in
+// any order).
+__global__ void __launch_bounds__(kBlock)
+k_joint_histogram(const float *__restrict__ ref, const float *__restrict__ war, const int *__restrict__ mask,
+                  size_t N, int rb, int fb, uint32_t *__restrict__ counts) {
+    extern __shared__ uint32_t lh[];
+    const int nb = rb * fb;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) lh[i] = 0;
+    __syncthreads();
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += (size_t)gridDim.x * blockDim.x) {
+        if (mask && mask[idx] < 0) continue;
+        const float r = ref[idx], w = war[idx];
+        if (r == r && w == w && 0.f <= r && r < (float)rb && 0.f <= w && w < (float)fb)
+            atomicAdd(&lh[(int)r + (int)w * rb], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if (lh[i]) atomicAdd(&counts[i], lh[i]);
+}
+
+// Smoothing, normalisation, marginals and entropies of reg_getNmiValue (reg-lib/cpu/_reg_nmi.cpp:249-344)
+// in FP64, one workgroup.  ent: H_r, H_f, H_rf, total, nmi.
+__global__ void __launch_bounds__(1024)
+k_nmi_finalize(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+               double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent) {
+    __shared__ double red[16];
+    const int nb = rb * fb, total = nb + rb + fb;
+    const double k0 = 1.0 / 6.0, k1 = (double)(2.f / 3.f);  // GetBasisSplineValue(-1), (0)
+    for (int i = threadIdx.x; i < total; i += blockDim.x) pro[i] = i < nb ? (double)counts[i] : 0.0;
+    __syncthreads();
+    // along the reference axis into logh
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        const int r = i % rb, f = i / rb;
+        double v = 0;
+        if (r - 1 > -1) v += pro[i - 1] * k0;
+        v += pro[i] * k1;
+        if (r + 1 < rb) v += pro[i + 1] * k0;
+        logh[r + rb * f] = v;
+    }
+    __syncthreads();
+    // along the floating axis back into pro
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        const int r = i % rb, f = i / rb;
+        double v = 0;
+        if (f - 1 > -1) v += logh[i - rb] * k0;
+        v += logh[i] * k1;
+        if (f + 1 < fb) v += logh[i + rb] * k0;
+        pro[i] = v;
+    }
+    __syncthreads();
+    double s = 0;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) s += pro[i];
+    s = block_sum_double(s, red);
+    __shared__ double totalS;
+    if (threadIdx.x == 0) totalS = s;
+    __syncthreads();
+    const double act = totalS;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) pro[i] /= act;
+    __syncthreads();
+    for (int i = threadIdx.x; i < rb + fb; i += blockDim.x) {
+        double m = 0;
+        if (i < rb) {
+            for (int f = 0; f < fb; ++f) m += pro[i + rb * f];
+        } else {
+            const int f = i - rb;
+            for (int r = 0; r < rb; ++r) m += pro[rb * f + r];
+        }
+        pro[nb + i] = m;
+    }
+    __syncthreads();
+    double hr = 0, hf = 0, hj = 0;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const double p = pro[i];
+        double l = 0;
+        if (p > 0) {
+            l = log(p);
+            if (i < nb) hj -= p * l;
+            else if (i < nb + rb) hr -= p * l;
+            else hf -= p * l;
+        }
+        logh[i] = l;
+        logf[i] = (float)l;
+    }
+    hr = block_sum_double(hr, red);
+    __syncthreads();
+    hf = block_sum_double(hf, red);
+    __syncthreads();
+    hj = block_sum_double(hj, red);
+    if (threadIdx.x == 0) {
+        ent[0] = hr;
+        ent[1] = hf;
+        ent[2] = hj;
+        ent[3] = act;
+        ent[4] = (hr + hf) / hj;
+    }
+}
+
+// reg_getVoxelBasedNmiGradient3d<float> (reg-lib/cpu/_reg_nmi.cpp:501-593); grad +=
+__global__ void __launch_bounds__(kBlock)
+k_nmi_voxel_gradient(const float *__restrict__ ref, const float *__restrict__ war, const float *__restrict__ wg,
+                     const int *__restrict__ mask, size_t N, int rb, int fb, const double *__restrict__ logh,
+                     const double *__restrict__ ent, double weight, float *__restrict__ grad) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (mask && mask[i] < 0) return;
+    const float rv = ref[i], wv = war[i];
+    if (!(rv == rv && wv == wv)) return;
+    const float g[3] = { wg[i], wg[i + N], wg[i + 2 * N] };
+    const double nmi = (ent[0] + ent[1]) / ent[2];
+    const size_t refOff = (size_t)rb * fb, floOff = refOff + rb;
+    double jd[3] = { 0, 0, 0 }, rd[3] = { 0, 0, 0 }, wd[3] = { 0, 0, 0 };
+    for (int r = (int)(rv - 1.f); r < (int)(rv + 3.f); ++r) {
+        if (!(-1 < r && r < rb)) continue;
+        for (int w = (int)(wv - 1.f); w < (int)(wv + 3.f); ++w) {
+            if (!(-1 < w && w < fb)) continue;
+            const double common = nmi_basis((double)(rv - (float)r)) * nmi_basis_deriv((double)(wv - (float)w));
+            const double jl = logh[r + w * rb], rl = logh[r + refOff], wl = logh[w + floOff];
+            for (int d = 0; d < 3; ++d) {
+                if (g[d] == g[d]) {
+                    rd[d] += common * (double)g[d] * rl;
+                    wd[d] += common * (double)g[d] * wl;
+                    jd[d] += common * (double)g[d] * jl;
+                }
+            }
+        }
+    }
+    const double den = ent[2] * ent[3];
+    for (int d = 0; d < 3; ++d) grad[i + d * N] += (float)(weight * (rd[d] + wd[d] - nmi * jd[d]) / den);
+}
+
+// ------------------------------------------------------------------ convolution
+// Density preparation of reg_tools_kernelConvolution (reg-lib/cpu/_reg_tools.cpp:860-866).
+__global__ void __launch_bounds__(kBlock)
+k_conv_prepare(float *__restrict__ data, const int *__restrict__ mask, size_t N, float *__restrict__ dens,
+               uint8_t *__restrict__ isnan_) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float v = data[i];
+    const float d = ((!mask || mask[i] >= 0) && v == v) ? 1.f : 0.f;
+    dens[i] = d;
+    isnan_[i] = d == 0.f;
+    if (d == 0.f) data[i] = 0.f;
+}
+
+// One axis of the density-normalised line convolution (reg-lib/cpu/_reg_tools.cpp:977-1093,
+// scalar path: kernel-weighted sums in double over the in-range taps).
+__global__ void __launch_bounds__(kBlock)
+k_conv_axis(const float *__restrict__ data, const float *__restrict__ dens, int nx, int ny, int nz, int axis,
+            const float *__restrict__ kernel, int radius, float *__restrict__ outData, float *__restrict__ outDens) {
+#pragma clang fp contract(off)
+    const size_t N = (size_t)nx * ny * nz;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    const int L = axis == 0 ? nx : axis == 1 ? ny : nz;
+    const int l = axis == 0 ? x : axis == 1 ? y : z;
+    const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)nx : (size_t)nx * ny;
+    const size_t base = i - (size_t)l * stride;
+    int pre = l - radius, pst = l + radius + 1;
+    int k0 = 0;
+    if (pre < 0) { k0 = -pre; pre = 0; }
+    if (pst > L) pst = L;
+    double is = 0, ds = 0;
+    for (int k = pre, kk = k0; k < pst; ++k, ++kk) {
+        const float kv = kernel[kk];
+        is += (double)(kv * data[base + (size_t)k * stride]);
+        ds += (double)(kv * dens[base + (size_t)k * stride]);
+    }
+    outData[i] = (float)is;
+    outDens[i] = (float)ds;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_conv_normalise(float *__restrict__ data, const float *__restrict__ dens, const uint8_t *__restrict__ isnan_, size_t N) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    data[i] = isnan_[i] ? __builtin_nanf("") : data[i] / dens[i];
+}
+
+// ------------------------------------------------------------------ voxel -> node
+// reg_voxelCentricToNodeCentric, update = false (reg-lib/cpu/_reg_localTrans.cpp:1540-1684)
+__global__ void __launch_bounds__(kBlock)
+k_voxel_to_node(const float *__restrict__ vg, int nx, int ny, int nz, int gnx, int gny, int gnz, Affine T,
+                Affine reorient, float weight, float *__restrict__ ng) {
+#pragma clang fp contract(off)
+    const size_t G = (size_t)gnx * gny * gnz, N = (size_t)nx * ny * nz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= G) return;
+    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+    const float3 vc = affine_apply_d(T, (float)x, (float)y, (float)z);
+    const int pre[3] = { floor_int(vc.x), floor_int(vc.y), floor_int(vc.z) };
+    float bx[2], by[2], bz[2];
+    bx[1] = vc.x - (float)pre[0]; bx[0] = 1.f - bx[1];
+    by[1] = vc.y - (float)pre[1]; by[0] = 1.f - by[1];
+    bz[1] = vc.z - (float)pre[2]; bz[0] = 1.f - bz[1];
+    float iv0 = 0.f, iv1 = 0.f, iv2 = 0.f;
+    for (int c = 0; c < 2; ++c) {
+        const int Z = pre[2] + c;
+        if (Z < 0 || Z >= nz) continue;
+        for (int b = 0; b < 2; ++b) {
+            const int Y = pre[1] + b;
+            if (Y < 0 || Y >= ny) continue;
+            for (int a = 0; a < 2; ++a) {
+                const int X = pre[0] + a;
+                if (X < 0 || X >= nx) continue;
+                const size_t idx = ((size_t)Z * ny + Y) * nx + X;
+                float w = bx[a] * by[b];
+                w *= bz[c];
+                iv0 += w * vg[idx];
+                iv1 += w * vg[idx + N];
+                iv2 += w * vg[idx + 2 * N];
+            }
+        }
+    }
+    const float *R = reorient.m;
+    const float r0 = R[0] * iv0 + R[4] * iv1 + R[8] * iv2;
+    const float r1 = R[1] * iv0 + R[5] * iv1 + R[9] * iv2;
+    const float r2 = R[2] * iv0 + R[6] * iv1 + R[10] * iv2;
+    ng[n] = r0 * weight;
+    ng[n + G] = r1 * weight;
+    ng[n + 2 * G] = r2 * weight;
+}
+
+// ------------------------------------------------------------------ bending energy
+struct BeBasis {
+    float b[6][27];  // xx, yy, zz, xy, yz, xz
+};
+
+// reg_spline_approxBendingEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:73-153): per-block
+// FP64 partial sums over interior nodes.
+__global__ void __launch_bounds__(kBlock)
+k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BeBasis B, double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    __shared__ double red[16];
+    const size_t G = (size_t)gnx * gny * gnz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0;
+    if (n < G) {
+        const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+        if (x > 0 && y > 0 && z > 0 && x < gnx - 1 && y < gny - 1 && z < gnz - 1) {
+            float d[3][6] = {};
+            int i = 0;
+            for (int c = -1; c < 2; ++c)
+                for (int b = -1; b < 2; ++b)
+                    for (int a = -1; a < 2; ++a, ++i) {
+                        const size_t idx = ((size_t)(z + c) * gny + y + b) * gnx + x + a;
+                        for (int k = 0; k < 3; ++k) {
+                            const float cv = cpp[idx + k * G];
+                            for (int t = 0; t < 6; ++t) d[k][t] += B.b[t][i] * cv;
+                        }
+                    }
+            s = (double)(d[0][0] * d[0][0] + d[0][1] * d[0][1] + d[0][2] * d[0][2]) +
+                2.0 * (double)(d[0][3] * d[0][3] + d[0][4] * d[0][4] + d[0][5] * d[0][5]);
+            for (int k = 1; k < 3; ++k) {
+                s += (double)(d[k][0] * d[k][0]);
+                s += (double)(d[k][1] * d[k][1]);
+                s += (double)(d[k][2] * d[k][2]);
+                s += 2.0 * (double)(d[k][3] * d[k][3] + d[k][4] * d[k][4] + d[k][5] * d[k][5]);
+            }
+        }
+    }
+    s = block_sum_double(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(1024)
+k_reduce_partials(const double *__restrict__ partials, int n, double scale, double *__restrict__ out) {
+    __shared__ double red[16];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+    s = block_sum_double(s, red);
+    if (threadIdx.x == 0) *out = s * scale;
+}
+
+// Node position in float, reg_getDisplacementFromDeformation_3D (reg-lib/cpu/_reg_tools.cpp:2004-2019)
+__device__ __forceinline__ float node_pos(const Affine& m, int r, int x, int y, int z) {
+#pragma clang fp contract(off)
+    return m.m[r * 4 + 0] * (float)x + m.m[r * 4 + 1] * (float)y + m.m[r * 4 + 2] * (float)z + m.m[r * 4 + 3];
+}
+
+// First half of reg_spline_approxBendingEnergyGradient3D (regul.cpp:317-379): 18 second
+// derivatives of the displacement at every node.
+__global__ void __launch_bounds__(kBlock)
+k_be_deriv(const float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz, BeBasis B, float *__restrict__ der) {
+#pragma clang fp contract(off)
+    const size_t G = (size_t)gnx * gny * gnz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= G) return;
+    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+    float d[3][6] = {};
+    int i = 0;
+    for (int c = -1; c < 2; ++c)
+        for (int b = -1; b < 2; ++b)
+            for (int a = -1; a < 2; ++a, ++i) {
+                const int X = x + a, Y = y + b, Z = z + c;
+                if (X < 0 || Y < 0 || Z < 0 || X >= gnx || Y >= gny || Z >= gnz) continue;
+                const size_t idx = ((size_t)Z * gny + Y) * gnx + X;
+                for (int k = 0; k < 3; ++k) {
+                    const float cv = cpp[idx + k * G] - node_pos(xyz, k, X, Y, Z);
+                    for (int t = 0; t < 6; ++t) d[k][t] += B.b[t][i] * cv;
+                }
+            }
+    float *o = der + 18 * n;
+    for (int t = 0; t < 3; ++t)
+        for (int k = 0; k < 3; ++k) o[t * 3 + k] = d[k][t];
+    for (int t = 3; t < 6; ++t)
+        for (int k = 0; k < 3; ++k) o[t * 3 + k] = (float)(2.0 * (double)d[k][t]);
+}
+
+// Second half (regul.cpp:381-448): adjoint 27-point gather, grad += weight / G * value.
+__global__ void __launch_bounds__(kBlock)
+k_be_grad(const float *__restrict__ der, int gnx, int gny, int gnz, BeBasis B, float ratio, float *__restrict__ grad) {
+#pragma clang fp contract(off)
+    const size_t G = (size_t)gnx * gny * gnz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= G) return;
+    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+    float gv[3] = { 0.f, 0.f, 0.f };
+    int a = 0;
+    for (int Z = z - 1; Z < z + 2; ++Z)
+        for (int Y = y - 1; Y < y + 2; ++Y)
+            for (int X = x - 1; X < x + 2; ++X, ++a) {
+                if (X < 0 || Y < 0 || Z < 0 || X >= gnx || Y >= gny || Z >= gnz) continue;
+                const float *dp = der + 18 * (((size_t)Z * gny + Y) * gnx + X);
+                for (int t = 0; t < 6; ++t)
+                    for (int k = 0; k < 3; ++k) gv[k] += dp[t * 3 + k] * B.b[t][a];
+            }
+    grad[n] += ratio * gv[0];
+    grad[n + G] += ratio * gv[1];
+    grad[n + 2 * G] += ratio * gv[2];
+}
+
+// ------------------------------------------------------------------ optimiser helpers
+// reg_getMaximalLength<float> (reg-lib/cpu/_reg_tools.cpp:335-353): lengths are >= 0, so their
+// IEEE bit patterns order like unsigned integers.
+__global__ void __launch_bounds__(kBlock)
+k_max_length(const float *__restrict__ g, size_t G, uint32_t *__restrict__ out) {
+#pragma clang fp contract(off)
+    uint32_t best = 0;
+    for (size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x; n < G; n += (size_t)gridDim.x * blockDim.x) {
+        const float x = g[n], y = g[n + G], z = g[2 * G + n];
+        const float l = sqrtf(x * x + y * y + z * z);
+        const uint32_t b = __float_as_uint(l);
+        best = b > best ? b : best;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_down(best, off, 64);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+}
+
+__global__ void k_max_to_scalar(const uint32_t *__restrict__ bits, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = (double)__uint_as_float(*bits);
+}
+
+// Compute::NormaliseGradient (reg-lib/Compute.cpp:152-183)
+__global__ void __launch_bounds__(kBlock)
+k_normalise(float *__restrict__ g, size_t n, double inv) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g[i] = (float)((double)g[i] * inv);
+}
+
+// Normalisation by a device-resident max length (fused gradient path): skips when it is 0.
+__global__ void __launch_bounds__(kBlock)
+k_normalise_dev(float *__restrict__ g, size_t n, const double *__restrict__ maxLen) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = *maxLen;
+    if (i < n && m != 0.0) g[i] = (float)((double)g[i] * (1.0 / m));
+}
+
+// ConjugateGradient<float>::UpdateGradientValues (reg-lib/Optimiser.cpp:231-317)
+__global__ void __launch_bounds__(kBlock)
+k_cg_init(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) H[i] = G_[i] = -grad[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_cg_dots(const float *__restrict__ grad, const float *__restrict__ G_, const float *__restrict__ H, size_t n,
+          double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    __shared__ double red[16];
+    double gg = 0, dgg = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        gg += (double)(H[i] * G_[i]);
+        dgg += (double)((grad[i] + G_[i]) * grad[i]);
+    }
+    gg = block_sum_double(gg, red);
+    __syncthreads();
+    dgg = block_sum_double(dgg, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = gg;
+        partials[2 * blockIdx.x + 1] = dgg;
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_cg_gamma(const double *__restrict__ partials, int nb, double *__restrict__ gam) {
+    __shared__ double red[16];
+    double gg = 0, dgg = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        gg += partials[2 * i];
+        dgg += partials[2 * i + 1];
+    }
+    gg = block_sum_double(gg, red);
+    __syncthreads();
+    dgg = block_sum_double(dgg, red);
+    if (threadIdx.x == 0) *gam = dgg / gg;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_cg_apply(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H, size_t n, const double *__restrict__ gam) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double gm = *gam;
+    G_[i] = -grad[i];
+    H[i] = (float)((double)G_[i] + gm * (double)H[i]);
+    grad[i] = -H[i];
+}
+
+// Compute::UpdateControlPointPosition (reg-lib/Compute.cpp:91-127)
+__global__ void __launch_bounds__(kBlock)
+k_update_dof(float *__restrict__ cur, const float *__restrict__ best, const float *__restrict__ grad, float scale, size_t n) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cur[i] = best[i] + scale * grad[i];
+}
+
+// ------------------------------------------------------------------ intensity rescale
+// reg_intensityRescale_core<float> (reg-lib/cpu/_reg_tools.cpp:93-183): min/max over finite values.
+__global__ void __launch_bounds__(kBlock)
+k_minmax(const float *__restrict__ d, size_t n, uint32_t *__restrict__ mm) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = d[i];
+        if (v == v) { lo = fminf(lo, v); hi = fmaxf(hi, v); }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_down(lo, off, 64));
+        hi = fmaxf(hi, __shfl_down(hi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        // order-preserving integer keys of the floats
+        auto key = [](float f) {
+            uint32_t u = __float_as_uint(f);
+            return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        };
+        atomicMin(&mm[0], key(lo));
+        atomicMax(&mm[1], key(hi));
+    }
+}
+
+__device__ __forceinline__ float unkey(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_rescale(float *__restrict__ d, size_t n, const uint32_t *__restrict__ mm, float newMin, float newMax) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float lo = unkey(mm[0]), hi = unkey(mm[1]);
+    const double diff = (double)(hi - lo), newDiff = (double)(newMax - newMin);
+    double v = (double)d[i];
+    if (v == v) {
+        v = (v - (double)lo) / diff;
+        v = v * newDiff + (double)newMin;
+    }
+    d[i] = (float)v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_f32(float *p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Round to the 6-decimal literals the reference tabulates (reg-lib/cpu/_reg_splineBasis.cpp:292-457).
+float round6(double v) {
+    const double r = std::floor(std::fabs(v) * 1e6 + 0.5) / 1e6;
+    return (float)(v < 0 ? -r : r);
+}
+
+BeBasis make_be_basis() {
+    // Tensor products of the cubic B-spline at a node: value {1/6, 2/3, 1/6}, first
+    // derivative {-1/2, 0, 1/2}, second derivative {1, -2, 1}.
+    const double v[3] = { 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0 }, d1[3] = { -0.5, 0.0, 0.5 }, d2[3] = { 1.0, -2.0, 1.0 };
+    BeBasis B;
+    int i = 0;
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a, ++i) {
+                B.b[0][i] = round6(d2[a] * v[b] * v[c]);
+                B.b[1][i] = round6(v[a] * d2[b] * v[c]);
+                B.b[2][i] = round6(v[a] * v[b] * d2[c]);
+                B.b[3][i] = round6(d1[a] * d1[b] * v[c]);
+                B.b[4][i] = round6(v[a] * d1[b] * d1[c]);
+                B.b[5][i] = round6(d1[a] * v[b] * d1[c]);
+            }
+    return B;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launches
+void op_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
+    const unsigned b = std::min(blocks_for(n), 4096u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(kBlock), 0, s, p, n, v);
+}
+
+void op_fill_f32(float *p, size_t n, float v, hipStream_t s) {
+    const unsigned b = std::min(blocks_for(n), 4096u);
+    hipLaunchKernelGGL(k_fill_f32, dim3(b), dim3(kBlock), 0, s, p, n, v);
+}
+
+static bool lut_rule(const nrh_f3d *h, float g[3]) {
+    g[0] = h->cpp.dx / h->ref.dx;
+    g[1] = h->cpp.dy / h->ref.dy;
+    g[2] = h->cpp.dz / h->ref.dz;
+    return g[0] == 5.f && g[1] == 5.f && g[2] == 5.f;
+}
+
+void op_spline_field(nrh_f3d *h) {
+    ensure_field(h);
+    float g[3];
+    const int lut = lut_rule(h, g) ? 1 : 0;
+    hipLaunchKernelGGL(k_spline_field, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, h->ref.nx, h->ref.ny,
+                       h->ref.nz, h->cpp.nx, h->cpp.ny, h->G, h->cppPos.p, h->mask.p, g[0], g[1], g[2], lut,
+                       h->field.p, h->N);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_resample(nrh_f3d *h) {
+    ensure_warped(h);
+    NRH_REQUIRE(!h->field.empty(), NRH_ERR_STATE, "resample before the deformation field");
+    NRH_REQUIRE(!h->floImg.empty(), NRH_ERR_UNSUPPORTED, "per-operation resampling needs float32 image storage");
+    hipLaunchKernelGGL(k_resample_linear, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, h->field.p, h->N,
+                       h->mask.p, h->floImg.p, h->flo.nx, h->flo.ny, h->flo.nz, to_affine(h->flo.ijk),
+                       h->params.padding, h->warped.p);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_image_gradient(nrh_f3d *h) {
+    ensure_warped_grad(h);
+    NRH_REQUIRE(!h->field.empty(), NRH_ERR_STATE, "image gradient before the deformation field");
+    NRH_REQUIRE(!h->floImg.empty(), NRH_ERR_UNSUPPORTED, "per-operation gradient needs float32 image storage");
+    hipLaunchKernelGGL(k_image_gradient_linear, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, h->field.p,
+                       h->N, h->mask.p, h->floImg.p, h->flo.nx, h->flo.ny, h->flo.nz, to_affine(h->flo.ijk),
+                       h->params.padding, h->warpedGrad.p);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_joint_histogram(nrh_f3d *h, const float *warped) {
+    const int rb = h->params.ref_bins, fb = h->params.flo_bins;
+    op_fill_u32(h->histCount.p, (size_t)rb * fb, 0u, h->stream);
+    const unsigned b = std::min(blocks_for(h->N), 2048u);
+    hipLaunchKernelGGL(k_joint_histogram, dim3(b), dim3(kBlock), (size_t)rb * fb * sizeof(uint32_t), h->stream,
+                       ref_ptr(h), warped, h->mask.p, h->N, rb, fb, h->histCount.p);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_nmi_finalize(nrh_f3d *h) {
+    hipLaunchKernelGGL(k_nmi_finalize, dim3(1), dim3(1024), 0, h->stream, h->histCount.p, h->params.ref_bins,
+                       h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_nmi_voxel_gradient(nrh_f3d *h, double weight) {
+    ensure_voxel_grad(h);
+    NRH_REQUIRE(!h->warped.empty() && !h->warpedGrad.empty(), NRH_ERR_STATE,
+                "NMI voxel gradient needs the warped image and its gradient");
+    hipLaunchKernelGGL(k_nmi_voxel_gradient, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, ref_ptr(h),
+                       h->warped.p, h->warpedGrad.p, h->mask.p, h->N, h->params.ref_bins, h->params.flo_bins,
+                       h->histLog.p, h->ent.p, weight, h->voxelGrad.p);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_convolve_voxel_gradient(nrh_f3d *h) {
+    NRH_REQUIRE(!h->voxelGrad.empty(), NRH_ERR_STATE, "no voxel-based gradient to convolve");
+    const size_t N = h->N;
+    h->convTmp.alloc(N);
+    h->convDens.alloc(N);
+    h->convDensTmp.alloc(N);
+    h->convNan.alloc(N);
+    const int dims[3] = { h->ref.nx, h->ref.ny, h->ref.nz };
+    const float pix[3] = { h->ref.dx, h->ref.dy, h->ref.dz };
+    const float sp[3] = { h->cpp.dx, h->cpp.dy, h->cpp.dz };
+    for (int axis = 0; axis < 3; ++axis) {
+        // Cubic B-spline kernel of reg_tools_kernelConvolution (reg-lib/cpu/_reg_tools.cpp:868-904)
+        const double temp = (double)(sp[axis] / pix[axis]);
+        const int radius = (int)(temp * 2.0f);
+        std::vector<float> kern(2 * radius + 1);
+        for (int i = -radius; i <= radius; ++i) {
+            const double rel = std::fabs(i / temp);
+            float kv = 0.f;
+            if (rel < 1.0) kv = (float)(2.0 / 3.0 - rel * rel + 0.5 * rel * rel * rel);
+            else if (rel < 2.0) kv = (float)(-((rel - 2.0) * (rel - 2.0) * (rel - 2.0)) / 6.0);
+            kern[i + radius] = kv;
+        }
+        if (radius > 0) {
+            h->convKernel.alloc(std::max<size_t>(kern.size(), 64));
+            NRH_HIP(hipMemcpyAsync(h->convKernel.p, kern.data(), kern.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        }
+        for (int comp = 0; comp < 3; ++comp) {
+            float *d = h->voxelGrad.p + comp * N;
+            hipLaunchKernelGGL(k_conv_prepare, dim3(blocks_for(N)), dim3(kBlock), 0, h->stream, d, (const int *)nullptr,
+                               N, h->convDens.p, h->convNan.p);
+            if (radius > 0 && dims[axis] > 1) {
+                hipLaunchKernelGGL(k_conv_axis, dim3(blocks_for(N)), dim3(kBlock), 0, h->stream, d, h->convDens.p,
+                                   dims[0], dims[1], dims[2], axis, h->convKernel.p, radius, h->convTmp.p, h->convDensTmp.p);
+                NRH_HIP(hipMemcpyAsync(d, h->convTmp.p, N * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+                hipLaunchKernelGGL(k_conv_normalise, dim3(blocks_for(N)), dim3(kBlock), 0, h->stream, d, h->convDensTmp.p,
+                                   h->convNan.p, N);
+            } else {
+                hipLaunchKernelGGL(k_conv_normalise, dim3(blocks_for(N)), dim3(kBlock), 0, h->stream, d, h->convDens.p,
+                                   h->convNan.p, N);
+            }
+        }
+    }
+    NRH_HIP(hipGetLastError());
+}
+
+static Affine mat_mul_affine(const float *a, const float *b) {
+    // mat44 product in double stored in float (reg-lib/cpu/Maths.hpp:235-245)
+    Affine r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j)
+            r.m[i * 4 + j] = (float)((double)a[i * 4 + 0] * (double)b[0 * 4 + j] + (double)a[i * 4 + 1] * (double)b[1 * 4 + j] +
+                                     (double)a[i * 4 + 2] * (double)b[2 * 4 + j] + (double)a[i * 4 + 3] * (double)b[3 * 4 + j]);
+    return r;
+}
+
+float node_weight_ratio(const nrh_f3d *h, float weight) {
+    // weight *= node spacing / voxel spacing per axis (reg-lib/cpu/_reg_localTrans.cpp:1595-1604)
+    const float pd[3] = { h->ref.dx, h->ref.dy, h->ref.dz };
+    const float gd[3] = { h->cpp.dx, h->cpp.dy, h->cpp.dz };
+    for (int i = 0; i < 3; ++i) {
+        float r = gd[i];
+        if (h->cpp.sform) {
+            const float *m = h->cpp.xyz;
+            r = std::sqrt(m[i * 4 + 0] * m[i * 4 + 0] + m[i * 4 + 1] * m[i * 4 + 1] + m[i * 4 + 2] * m[i * 4 + 2]);
+        }
+        r /= pd[i];
+        weight *= r;
+    }
+    return weight;
+}
+
+void op_voxel_to_node(nrh_f3d *h, float weight) {
+    NRH_REQUIRE(!h->voxelGrad.empty(), NRH_ERR_STATE, "no voxel-based gradient");
+    const Affine T = mat_mul_affine(h->ref.ijk, h->cpp.xyz);
+    const float w = node_weight_ratio(h, weight);
+    hipLaunchKernelGGL(k_voxel_to_node, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->voxelGrad.p, h->ref.nx,
+                       h->ref.ny, h->ref.nz, h->cpp.nx, h->cpp.ny, h->cpp.nz, T, to_affine(h->flo.ijk), w, h->nodeGrad.p);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_bending_energy(nrh_f3d *h, double *out_device) {
+    static const BeBasis B = make_be_basis();
+    const unsigned nb = blocks_for(h->G);
+    h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)nb + 64));
+    hipLaunchKernelGGL(k_be_value, dim3(nb), dim3(kBlock), 0, h->stream, h->cppPos.p, h->cpp.nx, h->cpp.ny, h->cpp.nz,
+                       B, h->partials.p);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, h->partials.p, (int)nb,
+                       1.0 / (double)(3 * h->G), out_device);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_bending_energy_gradient(nrh_f3d *h, float weight) {
+    static const BeBasis B = make_be_basis();
+    h->beDeriv.alloc(18 * h->G);
+    hipLaunchKernelGGL(k_be_deriv, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->cppPos.p, h->cpp.nx,
+                       h->cpp.ny, h->cpp.nz, to_affine(h->cpp.xyz), B, h->beDeriv.p);
+    const float ratio = weight / (float)h->G;
+    hipLaunchKernelGGL(k_be_grad, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->beDeriv.p, h->cpp.nx,
+                       h->cpp.ny, h->cpp.nz, B, ratio, h->nodeGrad.p);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_max_length(nrh_f3d *h) {
+    op_fill_u32(h->maxBits.p, 1, 0u, h->stream);
+    const unsigned b = std::min(blocks_for(h->G), 1024u);
+    hipLaunchKernelGGL(k_max_length, dim3(b), dim3(kBlock), 0, h->stream, h->nodeGrad.p, h->G, h->maxBits.p);
+    hipLaunchKernelGGL(k_max_to_scalar, dim3(1), dim3(64), 0, h->stream, h->maxBits.p, h->scalars.p + 1);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_normalise(nrh_f3d *h, double max_length) {
+    if (max_length == 0) return;
+    hipLaunchKernelGGL(k_normalise, dim3(blocks_for(3 * h->G)), dim3(kBlock), 0, h->stream, h->nodeGrad.p, 3 * h->G,
+                       1.0 / max_length);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_normalise_device(nrh_f3d *h) {
+    hipLaunchKernelGGL(k_normalise_dev, dim3(blocks_for(3 * h->G)), dim3(kBlock), 0, h->stream, h->nodeGrad.p,
+                       3 * h->G, (const double *)(h->scalars.p + 1));
+    NRH_HIP(hipGetLastError());
+}
+
+void op_rescale(nrh_f3d *h, float *data, size_t n, float lo, float hi) {
+    const uint32_t init[2] = { 0xffffffffu, 0u };
+    NRH_HIP(hipMemcpyAsync(h->maxBits.p + 2, init, sizeof init, hipMemcpyHostToDevice, h->stream));
+    const unsigned b = std::min(blocks_for(n), 2048u);
+    hipLaunchKernelGGL(k_minmax, dim3(b), dim3(kBlock), 0, h->stream, data, n, h->maxBits.p + 2);
+    hipLaunchKernelGGL(k_rescale, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, data, n, h->maxBits.p + 2, lo, hi);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_cg(nrh_f3d *h, int first) {
+    const size_t n = 3 * h->G;
+    if (first) {
+        hipLaunchKernelGGL(k_cg_init, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->nodeGrad.p, h->cgG.p, h->cgH.p, n);
+    } else {
+        const unsigned nb = std::min(blocks_for(n), 512u);
+        h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)nb + 64));
+        hipLaunchKernelGGL(k_cg_dots, dim3(nb), dim3(kBlock), 0, h->stream, h->nodeGrad.p, h->cgG.p, h->cgH.p, n, h->partials.p);
+        hipLaunchKernelGGL(k_cg_gamma, dim3(1), dim3(1024), 0, h->stream, h->partials.p, (int)nb, h->scalars.p + 2);
+        hipLaunchKernelGGL(k_cg_apply, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->nodeGrad.p, h->cgG.p, h->cgH.p,
+                           n, (const double *)(h->scalars.p + 2));
+    }
+    NRH_HIP(hipGetLastError());
+}
+
+void op_update_dof(nrh_f3d *h, float scale) {
+    const size_t n = 3 * h->G;
+    hipLaunchKernelGGL(k_update_dof, dim3(blocks_for(n)), dim3(kBlock), 0, h->stream, h->cppPos.p, h->best.p,
+                       h->nodeGrad.p, scale, n);
+    NRH_HIP(hipGetLastError());
+    h->cppVersion++;
+}
+
+double read_scalar(nrh_f3d *h, const double *dev) {
+    NRH_HIP(hipMemcpyAsync(h->pinned, dev, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    NRH_HIP(hipStreamSynchronize(h->stream));
+    return h->pinned[0];
+}
+
+}  // namespace nrh

@@ -1,0 +1,194 @@
+"""F3D objective / gradient on the GPU through libnrhip's C ABI.
+
+F3dContent mirrors the reference's F3dContent + Compute pair for one image pair
+(reg-lib/F3dContent.h, reg-lib/Compute.h): the images, control points and every
+intermediate stay on the device; methods named after the reference's Compute / reg_nmi
+virtuals run one operation, objective() / gradient() run the fused sweeps.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .geometry import Geom
+
+
+def default_params(bins=64, padding=float("nan"), similarity_weight=0.999, be_weight=0.001, image_storage=0):
+    p = _lib.nrh_f3d_params()
+    p.ref_bins = bins + 4          # the reference adds 4 bins for the spline support
+    p.flo_bins = bins + 4
+    p.padding = padding
+    p.similarity_weight = similarity_weight
+    p.be_weight = be_weight
+    p.le_weight = 0.0
+    p.image_storage = image_storage
+    return p
+
+
+class F3dContent:
+    def __init__(self, ref_geom: Geom, ref, flo_geom: Geom, flo, cpp_geom: Geom, cpp=None, mask=None, params=None):
+        self.ref_geom, self.flo_geom, self.cpp_geom = ref_geom, flo_geom, cpp_geom
+        self.N, self.Nf, self.G = ref_geom.nvox, flo_geom.nvox, cpp_geom.nvox
+        self.params = params if params is not None else default_params()
+        ref = np.ascontiguousarray(ref, np.float32).reshape(-1)
+        flo = np.ascontiguousarray(flo, np.float32).reshape(-1)
+        assert ref.size == self.N and flo.size == self.Nf
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, np.int32).reshape(-1)
+            assert m.size == self.N
+        self._rg, self._fg, self._cg = ref_geom.to_c(), flo_geom.to_c(), cpp_geom.to_c()
+        h = ctypes.c_void_p()
+        _lib.call("nrh_f3d_create", ctypes.byref(self._rg), _lib.fptr(ref), ctypes.byref(self._fg), _lib.fptr(flo),
+                  _lib.iptr(m) if m is not None else None, ctypes.byref(self._cg), ctypes.byref(self.params),
+                  ctypes.byref(h))
+        self.h = h
+        if cpp is not None:
+            self.set_control_points(cpp)
+
+    def close(self):
+        if self.h:
+            _lib.call("nrh_f3d_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _op(self, name, *args):
+        _lib.call(name, self.h, *args)
+
+    def set_params(self, params):
+        self.params = params
+        self._op("nrh_f3d_set_params", ctypes.byref(params))
+
+    # --- control points
+    def set_control_points(self, cpp):
+        cpp = np.ascontiguousarray(cpp, np.float32).reshape(-1)
+        assert cpp.size == 3 * self.G
+        self._op("nrh_f3d_set_control_points", _lib.fptr(cpp))
+
+    def get_control_points(self):
+        out = np.empty(3 * self.G, np.float32)
+        self._op("nrh_f3d_get_control_points", _lib.fptr(out))
+        return out
+
+    def rescale_intensities(self, ref_range, flo_range):
+        self._op("nrh_f3d_rescale_intensities", ctypes.c_float(ref_range[0]), ctypes.c_float(ref_range[1]),
+                 ctypes.c_float(flo_range[0]), ctypes.c_float(flo_range[1]))
+
+    def get_images(self):
+        r = np.empty(self.N, np.float32)
+        f = np.empty(self.Nf, np.float32)
+        self._op("nrh_f3d_get_images", _lib.fptr(r), _lib.fptr(f))
+        return r, f
+
+    # --- per-operation (Compute / reg_nmi)
+    def GetDeformationField(self):
+        self._op("nrh_f3d_deformation_field")
+
+    def ResampleImage(self):
+        self._op("nrh_f3d_resample")
+
+    def GetImageGradient(self):
+        self._op("nrh_f3d_image_gradient")
+
+    def GetSimilarityMeasureValue(self):
+        nmi = ctypes.c_double()
+        ent = np.zeros(4, np.float64)
+        self._op("nrh_f3d_nmi_value", ctypes.byref(nmi), _lib.dptr(ent))
+        return nmi.value, ent
+
+    def ZeroVoxelBasedMeasureGradient(self):
+        self._op("nrh_f3d_zero_voxel_gradient")
+
+    def GetVoxelBasedSimilarityMeasureGradient(self, weight=1.0):
+        self._op("nrh_f3d_nmi_voxel_gradient", ctypes.c_double(weight))
+
+    def ConvolveImage(self):
+        self._op("nrh_f3d_convolve_voxel_gradient")
+
+    def VoxelCentricToNodeCentric(self, weight):
+        self._op("nrh_f3d_voxel_to_node", ctypes.c_float(weight))
+
+    def ZeroTransformationGradient(self):
+        self._op("nrh_f3d_zero_node_gradient")
+
+    def ApproxBendingEnergy(self):
+        v = ctypes.c_double()
+        self._op("nrh_f3d_bending_energy", ctypes.byref(v))
+        return v.value
+
+    def ApproxBendingEnergyGradient(self, weight):
+        self._op("nrh_f3d_bending_energy_gradient", ctypes.c_float(weight))
+
+    def GetMaximalLength(self):
+        v = ctypes.c_double()
+        self._op("nrh_f3d_max_length", ctypes.byref(v))
+        return v.value
+
+    def NormaliseGradient(self, max_length):
+        self._op("nrh_f3d_normalise_gradient", ctypes.c_double(max_length))
+
+    # --- downloads
+    def _dl(self, name, n, dtype=np.float32):
+        out = np.empty(n, dtype)
+        self._op(name, _lib.fptr(out))
+        return out
+
+    def deformation_field(self):
+        return self._dl("nrh_f3d_download_field", 3 * self.N)
+
+    def warped(self):
+        return self._dl("nrh_f3d_download_warped", self.N)
+
+    def warped_gradient(self):
+        return self._dl("nrh_f3d_download_warped_gradient", 3 * self.N)
+
+    def voxel_gradient(self):
+        return self._dl("nrh_f3d_download_voxel_gradient", 3 * self.N)
+
+    def node_gradient(self):
+        return self._dl("nrh_f3d_download_node_gradient", 3 * self.G)
+
+    def set_node_gradient(self, g):
+        g = np.ascontiguousarray(g, np.float32).reshape(-1)
+        self._op("nrh_f3d_upload_node_gradient", _lib.fptr(g))
+
+    def histogram(self):
+        rb, fb = self.params.ref_bins, self.params.flo_bins
+        tot = rb * fb + rb + fb
+        pro = np.empty(tot, np.float64)
+        logh = np.empty(tot, np.float64)
+        self._op("nrh_f3d_download_histogram", _lib.dptr(pro), _lib.dptr(logh))
+        return pro, logh
+
+    # --- fused evaluation
+    def objective(self):
+        v = ctypes.c_double()
+        terms = np.zeros(3, np.float64)
+        self._op("nrh_f3d_objective", ctypes.byref(v), _lib.dptr(terms))
+        return v.value, terms
+
+    def gradient(self, normalise=False):
+        if normalise:
+            m = ctypes.c_double()
+            self._op("nrh_f3d_gradient", ctypes.byref(m))
+            return m.value
+        self._op("nrh_f3d_gradient", None)
+        return None
+
+    # --- optimiser state
+    def store_best(self):
+        self._op("nrh_f3d_store_best")
+
+    def restore_best(self):
+        self._op("nrh_f3d_restore_best")
+
+    def conjugate_gradient(self, first):
+        self._op("nrh_f3d_conjugate_gradient", ctypes.c_int(1 if first else 0))
+
+    def update_control_points(self, scale):
+        self._op("nrh_f3d_update_control_points", ctypes.c_float(scale))

@@ -1,0 +1,790 @@
+/*
+ * nr_oracle.c -- CPU restatement of the NiftyReg F3D hot path (test infrastructure only).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library, and
+ * only as the checker.  Each function cites the reference function it restates; the
+ * arithmetic follows the reference's scalar (non-SSE) path: float products and sums where the
+ * reference uses float, double where it promotes to double, no fused multiply-add (this file
+ * is built with -ffp-contract=off, as the reference is on x86-64 without FMA).
+ */
+#include "nr_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define M(m, i, j) ((m)[(i) * 4 + (j)])
+
+/* Floor<int>(float) (reg-lib/cpu/Maths.hpp:63-66): truncate, then step down for negatives. */
+static int floor_int(float x) {
+    int i = (int)x;
+    return i - (x < (float)i);
+}
+
+/* Mat44Mul<float> (reg-lib/cpu/Maths.hpp:272-285): double accumulation, float result. */
+static void mat44_mul_f(const float *m, const float in[3], float out[3]) {
+    for (int r = 0; r < 3; ++r)
+        out[r] = (float)((double)M(m, r, 0) * (double)in[0] + (double)M(m, r, 1) * (double)in[1] +
+                         (double)M(m, r, 2) * (double)in[2] + (double)M(m, r, 3));
+}
+
+/* Mat44Mul<double> */
+static void mat44_mul_d(const float *m, const double in[3], double out[3]) {
+    for (int r = 0; r < 3; ++r)
+        out[r] = (double)M(m, r, 0) * in[0] + (double)M(m, r, 1) * in[1] + (double)M(m, r, 2) * in[2] +
+                 (double)M(m, r, 3);
+}
+
+/* mat44 product A*B in double, stored in float (reg-lib/cpu/Maths.hpp:235-245) */
+static void mat44_mm(const float *a, const float *b, float *out) {
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            out[i * 4 + j] = (float)((double)M(a, i, 0) * (double)M(b, 0, j) + (double)M(a, i, 1) * (double)M(b, 1, j) +
+                                     (double)M(a, i, 2) * (double)M(b, 2, j) + (double)M(a, i, 3) * (double)M(b, 3, j));
+}
+
+void orc_mat44_inverse(const float in[16], float out[16]) {
+    /* Inverse of an affine 4x4 by cofactors in double (the textbook adjugate formula). */
+    double a[3][3], t[3];
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) a[i][j] = M(in, i, j);
+        t[i] = M(in, i, 3);
+    }
+    double c[3][3];
+    c[0][0] = a[1][1] * a[2][2] - a[2][1] * a[1][2];
+    c[0][1] = -(a[0][1] * a[2][2] - a[2][1] * a[0][2]);
+    c[0][2] = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+    c[1][0] = -(a[1][0] * a[2][2] - a[2][0] * a[1][2]);
+    c[1][1] = a[0][0] * a[2][2] - a[2][0] * a[0][2];
+    c[1][2] = -(a[0][0] * a[1][2] - a[1][0] * a[0][2]);
+    c[2][0] = a[1][0] * a[2][1] - a[2][0] * a[1][1];
+    c[2][1] = -(a[0][0] * a[2][1] - a[2][0] * a[0][1]);
+    c[2][2] = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+    double det = a[0][0] * c[0][0] + a[0][1] * c[1][0] + a[0][2] * c[2][0];
+    double inv = det != 0.0 ? 1.0 / det : 0.0;
+    for (int i = 0; i < 3; ++i) {
+        double s = 0;
+        for (int j = 0; j < 3; ++j) {
+            out[i * 4 + j] = (float)(c[i][j] * inv);
+            s += c[i][j] * t[j];
+        }
+        out[i * 4 + 3] = (float)(-s * inv);
+    }
+    out[12] = out[13] = out[14] = 0.f;
+    out[15] = det != 0.0 ? 1.f : 0.f;
+}
+
+/* ------------------------------------------------------------------ spline basis */
+void orc_bspline_basis(float t, float v[4]) {
+    float ff = t * t;
+    float fff = ff * t;
+    float mf = (float)(1.0 - t);
+    v[0] = (float)((double)(mf * mf * mf) / 6.0);
+    v[1] = (float)((3.0 * fff - 6.0 * ff + 4.0) / 6.0);
+    v[2] = (float)((-3.0 * fff + 3.0 * ff + 3.0 * t + 1.0) / 6.0);
+    v[3] = (float)(fff / 6.0);
+}
+
+void orc_bspline_basis_d1(float t, float v[4], float d[4]) {
+    orc_bspline_basis(t, v);
+    d[3] = (float)(t * t / 2.0);
+    d[0] = (float)(t - 1.0 / 2.0 - d[3]);
+    d[2] = (float)(1.0 + d[0] - 2.0 * d[3]);
+    d[1] = -d[0] - d[2] - d[3];
+}
+
+/* ------------------------------------------------------------------ deformation field */
+static void grid_values(const orc_geom *cpp, const float *cppData, int x0, int y0, int z0,
+                        float *gx, float *gy, float *gz) {
+    /* get_GridValues (reg-lib/cpu/_reg_splineBasis.cpp:603-665) for an in-range window. */
+    const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    int coord = 0;
+    for (int Z = z0; Z < z0 + 4; ++Z)
+        for (int Y = y0; Y < y0 + 4; ++Y)
+            for (int X = x0; X < x0 + 4; ++X, ++coord) {
+                size_t idx = ((size_t)Z * cpp->ny + Y) * cpp->nx + X;
+                gx[coord] = cppData[idx];
+                gy[coord] = cppData[idx + g];
+                gz[coord] = cppData[idx + 2 * g];
+            }
+}
+
+void orc_spline_field(const orc_geom *ref, const orc_geom *cpp, const float *cppData,
+                      const int *mask, float *field) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    float *fx = field, *fy = field + n, *fz = field + 2 * n;
+    const float gvs[3] = { cpp->dx / ref->dx, cpp->dy / ref->dy, cpp->dz / ref->dz };
+
+    if (gvs[0] == 5.f && gvs[1] == 5.f && gvs[2] == 5.f) {
+        /* Look-up table of the 125 x 64 products x*(y*z) (reg-lib/cpu/_reg_localTrans.cpp:1112-1175) */
+        float *coef = (float *)malloc(125 * 64 * sizeof(float));
+        for (int z = 0; z < 5; ++z) {
+            float zb[4], yb[4], xb[4], yz[16];
+            orc_bspline_basis((float)z / 5.f, zb);
+            for (int y = 0; y < 5; ++y) {
+                orc_bspline_basis((float)y / 5.f, yb);
+                int c = 0;
+                for (int a = 0; a < 4; ++a)
+                    for (int j = 0; j < 4; ++j) yz[c++] = yb[j] * zb[a];
+                for (int x = 0; x < 5; ++x) {
+                    orc_bspline_basis((float)x / 5.f, xb);
+                    float *cf = coef + ((z * 5 + y) * 5 + x) * 64;
+                    int k = 0;
+                    for (int a = 0; a < 16; ++a)
+                        for (int i = 0; i < 4; ++i) cf[k++] = xb[i] * yz[a];
+                }
+            }
+        }
+        /* Loop over 5x5x5 voxel blocks (1178-1300) */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int zp = 0; zp < cpp->nz - 3; ++zp)
+            for (int yp = 0; yp < cpp->ny - 3; ++yp) {
+                float gx[64], gy[64], gz[64];
+                for (int xp = 0; xp < cpp->nx - 3; ++xp) {
+                    grid_values(cpp, cppData, xp, yp, zp, gx, gy, gz);
+                    for (int c = 0; c < 5; ++c) {
+                        int z = zp * 5 + c;
+                        if (z >= ref->nz) break;
+                        for (int b = 0; b < 5; ++b) {
+                            int y = yp * 5 + b;
+                            if (y >= ref->ny) break;
+                            for (int a = 0; a < 5; ++a) {
+                                int x = xp * 5 + a;
+                                size_t idx = ((size_t)z * ref->ny + y) * ref->nx + x;
+                                if (x >= ref->nx || (mask && mask[idx] < 0)) continue;
+                                const float *cf = coef + ((c * 5 + b) * 5 + a) * 64;
+                                float r0 = 0, r1 = 0, r2 = 0;
+                                for (int k = 0; k < 64; ++k) {
+                                    r0 += gx[k] * cf[k];
+                                    r1 += gy[k] * cf[k];
+                                    r2 += gz[k] * cf[k];
+                                }
+                                fx[idx] = r0; fy[idx] = r1; fz[idx] = r2;
+                            }
+                        }
+                    }
+                }
+            }
+        free(coef);
+        return;
+    }
+
+    /* General spacing (reg-lib/cpu/_reg_localTrans.cpp:1305-1450) */
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < ref->nz; ++z) {
+        float zb[4], yb[4], xb[4], yz[16], xyz[64], gx[64], gy[64], gz[64];
+        int zPre = (int)((float)z / gvs[2]);
+        float basis = (float)z / gvs[2] - (float)zPre;
+        if (basis < 0) basis = 0;
+        orc_bspline_basis(basis, zb);
+        for (int y = 0; y < ref->ny; ++y) {
+            int yPre = (int)((float)y / gvs[1]);
+            basis = (float)y / gvs[1] - (float)yPre;
+            if (basis < 0) basis = 0;
+            orc_bspline_basis(basis, yb);
+            int c = 0;
+            for (int a = 0; a < 4; ++a)
+                for (int j = 0; j < 4; ++j) yz[c++] = yb[j] * zb[a];
+            for (int x = 0; x < ref->nx; ++x) {
+                int xPre = (int)((float)x / gvs[0]);
+                basis = (float)x / gvs[0] - (float)xPre;
+                if (basis < 0) basis = 0;
+                orc_bspline_basis(basis, xb);
+                c = 0;
+                for (int a = 0; a < 16; ++a)
+                    for (int i = 0; i < 4; ++i) xyz[c++] = xb[i] * yz[a];
+                grid_values(cpp, cppData, xPre, yPre, zPre, gx, gy, gz);
+                size_t idx = ((size_t)z * ref->ny + y) * ref->nx + x;
+                float r0 = 0, r1 = 0, r2 = 0;
+                if (!mask || mask[idx] > -1) {
+                    for (int a = 0; a < 64; ++a) {
+                        r0 += gx[a] * xyz[a];
+                        r1 += gy[a] * xyz[a];
+                        r2 += gz[a] * xyz[a];
+                    }
+                }
+                fx[idx] = r0; fy[idx] = r1; fz[idx] = r2;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ resampling */
+void orc_resample_linear(const orc_geom *flo, const float *floData, const orc_geom *ref,
+                         const float *field, const int *mask, float pad, float *warped) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+#pragma omp parallel for schedule(static)
+    for (size_t idx = 0; idx < n; ++idx) {
+        if (mask && mask[idx] < 0) continue;
+        float world[3] = { field[idx], field[idx + n], field[idx + 2 * n] }, pos[3];
+        mat44_mul_f(flo->ijk, world, pos);
+        int prev[3];
+        float xb[2], yb[2], zb[2];
+        float *bases[3] = { xb, yb, zb };
+        for (int i = 0; i < 3; ++i) {
+            prev[i] = floor_int(pos[i]);
+            float rel = pos[i] - (float)prev[i];
+            if (rel < 0) rel = 0;
+            bases[i][1] = rel;
+            bases[i][0] = 1.f - rel;
+        }
+        float intensity = 0;
+        int inside = prev[0] > -1 && prev[0] + 1 < flo->nx && prev[1] > -1 && prev[1] + 1 < flo->ny &&
+                     prev[2] > -1 && prev[2] + 1 < flo->nz;
+        for (int c = 0; c < 2; ++c) {
+            int Z = prev[2] + c;
+            float yTemp = 0;
+            for (int b = 0; b < 2; ++b) {
+                int Y = prev[1] + b;
+                float xTemp = 0;
+                for (int a = 0; a < 2; ++a) {
+                    int X = prev[0] + a;
+                    float v;
+                    if (inside || (X > -1 && X < flo->nx && Y > -1 && Y < flo->ny && Z > -1 && Z < flo->nz))
+                        v = floData[((size_t)Z * flo->ny + Y) * flo->nx + X];
+                    else v = pad;
+                    xTemp += v * xb[a];
+                }
+                yTemp += xTemp * yb[b];
+            }
+            intensity += yTemp * zb[c];
+        }
+        warped[idx] = intensity;
+    }
+}
+
+void orc_image_gradient_linear(const orc_geom *flo, const float *floData, const orc_geom *ref,
+                               const float *field, const int *mask, float pad, float *grad) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    const float deriv[2] = { -1.f, 1.f };
+#pragma omp parallel for schedule(static)
+    for (size_t idx = 0; idx < n; ++idx) {
+        float g[3] = { 0, 0, 0 };
+        if (!mask || mask[idx] > -1) {
+            float world[3] = { field[idx], field[idx + n], field[idx + 2 * n] }, pos[3];
+            mat44_mul_f(flo->ijk, world, pos);
+            int prev[3] = { floor_int(pos[0]), floor_int(pos[1]), floor_int(pos[2]) };
+            float rel, xb[2], yb[2], zb[2];
+            rel = pos[0] - (float)prev[0]; xb[0] = (float)(1.0 - rel); xb[1] = rel;
+            rel = pos[1] - (float)prev[1]; yb[0] = (float)(1.0 - rel); yb[1] = rel;
+            rel = pos[2] - (float)prev[2]; zb[0] = (float)(1.0 - rel); zb[1] = rel;
+            for (int c = 0; c < 2; ++c) {
+                int Z = prev[2] + c;
+                int zIn = Z > -1 && Z < flo->nz;
+                float xx = 0, yy = 0, zz = 0;
+                for (int b = 0; b < 2; ++b) {
+                    int Y = prev[1] + b;
+                    int yzIn = zIn && Y > -1 && Y < flo->ny;
+                    float xT = 0, yT = 0;
+                    for (int a = 0; a < 2; ++a) {
+                        int X = prev[0] + a;
+                        float coeff = (yzIn && X > -1 && X < flo->nx) ?
+                            floData[((size_t)Z * flo->ny + Y) * flo->nx + X] : pad;
+                        xT += coeff * deriv[a];
+                        yT += coeff * xb[a];
+                    }
+                    xx += xT * yb[b];
+                    yy += yT * deriv[b];
+                    zz += yT * yb[b];
+                }
+                g[0] += xx * zb[c];
+                g[1] += yy * zb[c];
+                g[2] += zz * deriv[c];
+            }
+            for (int i = 0; i < 3; ++i)
+                if (g[i] != g[i]) g[i] = 0;
+        }
+        grad[idx] = g[0];
+        grad[idx + n] = g[1];
+        grad[idx + 2 * n] = g[2];
+    }
+}
+
+/* ------------------------------------------------------------------ NMI */
+void orc_intensity_rescale(float *data, size_t n, float newMin, float newMax) {
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (size_t i = 0; i < n; ++i) {
+        float v = data[i];
+        if (v == v) {
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
+    }
+    double diff = (double)(mx - mn);
+    double newDiff = (double)(newMax - newMin);
+    for (size_t i = 0; i < n; ++i) {
+        double v = (double)data[i];
+        if (v == v) {
+            v = (v - (double)mn) / diff;
+            v = v * newDiff + newMin;
+        }
+        data[i] = (float)v;
+    }
+}
+
+/* GetBasisSplineValue<double> (reg-lib/cpu/_reg_nmi.h:116-129); note the float constant 2.f/3.f. */
+static double bspline_d(double x) {
+    x = x < 0 ? -x : x;
+    double v = 0;
+    if (x < 2.f) {
+        if (x < 1.f) v = (double)(2.f / 3.f) + (0.5f * x - 1.f) * x * x;
+        else {
+            x -= 2.f;
+            v = -x * x * x / 6.f;
+        }
+    }
+    return v;
+}
+
+/* GetBasisSplineDerivativeValue<double> (reg-lib/cpu/_reg_nmi.h:131-145) */
+static double bspline_deriv_d(double ox) {
+    double x = ox < 0 ? -ox : ox;
+    double v = 0;
+    if (x < 2.f) {
+        if (x < 1.f) v = (1.5f * x - 2.f) * ox;
+        else {
+            x -= 2.f;
+            v = -0.5f * x * x;
+            if (ox < 0) v = -v;
+        }
+    }
+    return v;
+}
+
+double orc_nmi(const float *ref, const float *war, const int *mask, size_t n, int rb, int fb,
+               double *pro, double *logh, double *ent) {
+    const int total = rb * fb + rb + fb;
+    memset(pro, 0, total * sizeof(double));
+    /* integer counts (the reference counts in per-thread double arrays; counts are exact) */
+    unsigned long long *counts = (unsigned long long *)calloc((size_t)rb * fb, sizeof(unsigned long long));
+    for (size_t i = 0; i < n; ++i) {
+        if (mask && mask[i] < 0) continue;
+        float r = ref[i], w = war[i];
+        if (r == r && w == w && 0 <= r && r < rb && 0 <= w && w < fb)
+            counts[(int)r + (int)w * rb]++;
+    }
+    for (int i = 0; i < rb * fb; ++i) pro[i] = (double)counts[i];
+    free(counts);
+    const double kernel[3] = { bspline_d(-1.0), bspline_d(0.0), bspline_d(-1.0) };
+    memset(logh, 0, total * sizeof(double));
+    for (int f = 0; f < fb; ++f)
+        for (int r = 0; r < rb; ++r) {
+            double v = 0;
+            for (int it = 0; it < 3; ++it) {
+                int idx = r - 1 + it;
+                if (-1 < idx && idx < rb) v += pro[idx + rb * f] * kernel[it];
+            }
+            logh[r + rb * f] = v;
+        }
+    for (int r = 0; r < rb; ++r)
+        for (int f = 0; f < fb; ++f) {
+            double v = 0;
+            for (int it = 0; it < 3; ++it) {
+                int idx = f - 1 + it;
+                if (-1 < idx && idx < fb) v += logh[r + rb * idx] * kernel[it];
+            }
+            pro[r + rb * f] = v;
+        }
+    double active = 0;
+    for (int i = 0; i < total; ++i) active += pro[i];
+    ent[3] = active;
+    for (int i = 0; i < total; ++i) pro[i] /= active;
+    for (int r = 0; r < rb; ++r) {
+        double s = 0;
+        for (int f = 0; f < fb; ++f) s += pro[r + rb * f];
+        pro[rb * fb + r] = s;
+    }
+    for (int f = 0; f < fb; ++f) {
+        double s = 0;
+        for (int r = 0; r < rb; ++r) s += pro[rb * f + r];
+        pro[rb * fb + rb + f] = s;
+    }
+    memset(logh, 0, total * sizeof(double));
+    double hr = 0, hf = 0, hj = 0;
+    for (int r = 0; r < rb; ++r) {
+        double p = pro[rb * fb + r];
+        if (p > 0) { double l = log(p); hr -= p * l; logh[rb * fb + r] = l; }
+    }
+    for (int f = 0; f < fb; ++f) {
+        double p = pro[rb * fb + rb + f];
+        if (p > 0) { double l = log(p); hf -= p * l; logh[rb * fb + rb + f] = l; }
+    }
+    for (int i = 0; i < rb * fb; ++i) {
+        double p = pro[i];
+        if (p > 0) { double l = log(p); hj -= p * l; logh[i] = l; }
+    }
+    ent[0] = hr; ent[1] = hf; ent[2] = hj;
+    return (hr + hf) / hj;
+}
+
+void orc_nmi_voxel_gradient(const float *ref, const float *war, const float *warGrad,
+                            const int *mask, size_t n, int rb, int fb, const double *logh,
+                            const double *ent, double weight, float *grad) {
+    const double nmi = (ent[0] + ent[1]) / ent[2];
+    const size_t refOff = (size_t)rb * fb, floOff = refOff + rb;
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i) {
+        if (mask && mask[i] < 0) continue;
+        float rv = ref[i], wv = war[i];
+        if (!(rv == rv && wv == wv)) continue;
+        float g[3] = { warGrad[i], warGrad[i + n], warGrad[i + 2 * n] };
+        double jd[3] = { 0, 0, 0 }, rd[3] = { 0, 0, 0 }, wd[3] = { 0, 0, 0 };
+        for (int r = (int)(rv - 1.f); r < (int)(rv + 3.f); ++r) {
+            if (!(-1 < r && r < rb)) continue;
+            for (int w = (int)(wv - 1.f); w < (int)(wv + 3.f); ++w) {
+                if (!(-1 < w && w < fb)) continue;
+                double common = bspline_d((double)(rv - (float)r)) * bspline_deriv_d((double)(wv - (float)w));
+                double jl = logh[r + w * rb], rl = logh[r + refOff], wl = logh[w + floOff];
+                for (int d = 0; d < 3; ++d) {
+                    if (g[d] == g[d]) {
+                        rd[d] += common * g[d] * rl;
+                        wd[d] += common * g[d] * wl;
+                        jd[d] += common * g[d] * jl;
+                    }
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d)
+            grad[i + d * n] += (float)(weight * (rd[d] + wd[d] - nmi * jd[d]) / (ent[2] * ent[3]));
+    }
+}
+
+/* ------------------------------------------------------------------ convolution */
+void orc_kernel_convolution_axis(const orc_geom *g, float *data, float sigma, int kernelType,
+                                 const int *mask, int axis) {
+    const int dims[3] = { g->nx, g->ny, g->nz };
+    const float pix[3] = { g->dx, g->dy, g->dz };
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    unsigned char *isnan_ = (unsigned char *)malloc(n);
+    float *dens = (float *)malloc(n * sizeof(float));
+    for (size_t i = 0; i < n; ++i) {
+        dens[i] = ((!mask || mask[i] >= 0) && data[i] == data[i]) ? 1.f : 0.f;
+        isnan_[i] = dens[i] == 0.f;
+        if (isnan_[i]) data[i] = 0;
+    }
+    if (dims[axis] > 1) {
+        double temp;
+        if (sigma > 0) temp = (double)(sigma / pix[axis]);
+        else temp = fabs((double)sigma);
+        int radius = 0;
+        if (kernelType == 2) radius = (int)(temp * 3.0f);
+        else if (kernelType == 3) radius = (int)(temp * 2.0f);
+        if (radius > 0) {
+            float *kernel = (float *)malloc((2 * radius + 1) * sizeof(float));
+            double ksum = 0;
+            for (int i = -radius; i <= radius; ++i) {
+                if (kernelType == 3) {
+                    double rel = fabs(i / temp);
+                    if (rel < 1.0) kernel[i + radius] = (float)(2.0 / 3.0 - rel * rel + 0.5 * rel * rel * rel);
+                    else if (rel < 2.0) kernel[i + radius] = (float)(-((rel - 2.0) * (rel - 2.0) * (rel - 2.0)) / 6.0);
+                    else kernel[i + radius] = 0;
+                } else {
+                    kernel[i + radius] = (float)(exp(-(double)(i * i) / (2.0 * temp * temp)) / (temp * 2.506628274631));
+                }
+                ksum += kernel[i + radius];
+            }
+            (void)ksum;
+            int planes, lineOff;
+            if (axis == 0) { planes = dims[1] * dims[2]; lineOff = 1; }
+            else if (axis == 1) { planes = dims[0] * dims[2]; lineOff = dims[0]; }
+            else { planes = dims[0] * dims[1]; lineOff = planes; }
+#pragma omp parallel for schedule(static)
+            for (int p = 0; p < planes; ++p) {
+                size_t start;
+                if (axis == 0) start = (size_t)p * dims[0];
+                else if (axis == 1) start = (size_t)(p / dims[0]) * dims[0] * dims[1] + p % dims[0];
+                else start = p;
+                int L = dims[axis];
+                float *bi = (float *)malloc(L * sizeof(float));
+                float *bd = (float *)malloc(L * sizeof(float));
+                for (int l = 0; l < L; ++l) {
+                    bi[l] = data[start + (size_t)l * lineOff];
+                    bd[l] = dens[start + (size_t)l * lineOff];
+                }
+                for (int l = 0; l < L; ++l) {
+                    int pre = l - radius, pst = l + radius + 1;
+                    const float *kp = kernel;
+                    if (pre < 0) { kp = kernel - pre; pre = 0; }
+                    if (pst > L) pst = L;
+                    double is = 0, ds = 0;
+                    for (int k = pre; k < pst; ++k) {
+                        float kv = *kp++;
+                        is += kv * bi[k];
+                        ds += kv * bd[k];
+                    }
+                    data[start + (size_t)l * lineOff] = (float)is;
+                    dens[start + (size_t)l * lineOff] = (float)ds;
+                }
+                free(bi);
+                free(bd);
+            }
+            free(kernel);
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (isnan_[i]) data[i] = NAN;
+        else data[i] = data[i] / dens[i];
+    }
+    free(isnan_);
+    free(dens);
+}
+
+void orc_convolve_voxel_gradient(const orc_geom *ref, const orc_geom *cpp, float *voxelGrad) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    const float sp[3] = { cpp->dx, cpp->dy, cpp->dz };
+    for (int axis = 0; axis < 3; ++axis)
+        for (int comp = 0; comp < 3; ++comp)
+            orc_kernel_convolution_axis(ref, voxelGrad + comp * n, sp[axis], 3, NULL, axis);
+}
+
+void orc_voxel_to_node(const orc_geom *cpp, const orc_geom *ref, const float *voxelGrad,
+                       const float reorient_ijk[16], float weight, float *nodeGrad) {
+    const size_t nv = (size_t)ref->nx * ref->ny * ref->nz;
+    const size_t ng = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    float T[16];
+    mat44_mm(ref->ijk, cpp->xyz, T);
+    /* ratio = node spacing (sform column norm) / voxel pixdim, folded into the weight */
+    for (int i = 0; i < 3; ++i) {
+        float r = sqrtf(M(cpp->xyz, i, 0) * M(cpp->xyz, i, 0) + M(cpp->xyz, i, 1) * M(cpp->xyz, i, 1) +
+                        M(cpp->xyz, i, 2) * M(cpp->xyz, i, 2));
+        const float pd[3] = { ref->dx, ref->dy, ref->dz };
+        r /= pd[i];
+        weight *= r;
+    }
+    const float *vx = voxelGrad, *vy = voxelGrad + nv, *vz = voxelGrad + 2 * nv;
+    for (int z = 0; z < cpp->nz; ++z)
+        for (int y = 0; y < cpp->ny; ++y)
+            for (int x = 0; x < cpp->nx; ++x) {
+                float nc[3] = { (float)x, (float)y, (float)z }, vc[3];
+                mat44_mul_f(T, nc, vc);
+                int pre[3] = { floor_int(vc[0]), floor_int(vc[1]), floor_int(vc[2]) };
+                float bx[2], by[2], bz[2];
+                bx[1] = vc[0] - (float)pre[0]; bx[0] = 1.f - bx[1];
+                by[1] = vc[1] - (float)pre[1]; by[0] = 1.f - by[1];
+                bz[1] = vc[2] - (float)pre[2]; bz[0] = 1.f - bz[1];
+                float iv[3] = { 0, 0, 0 };
+                for (int c = 0; c < 2; ++c) {
+                    int Z = pre[2] + c;
+                    if (Z < 0 || Z >= ref->nz) continue;
+                    for (int b = 0; b < 2; ++b) {
+                        int Y = pre[1] + b;
+                        if (Y < 0 || Y >= ref->ny) continue;
+                        for (int a = 0; a < 2; ++a) {
+                            int X = pre[0] + a;
+                            if (X < 0 || X >= ref->nx) continue;
+                            size_t idx = ((size_t)Z * ref->ny + Y) * ref->nx + X;
+                            float w = bx[a] * by[b];
+                            w *= bz[c];
+                            iv[0] += w * vx[idx];
+                            iv[1] += w * vy[idx];
+                            iv[2] += w * vz[idx];
+                        }
+                    }
+                }
+                /* reorientation applied transposed (1654-1666) */
+                float ro[3];
+                for (int j = 0; j < 3; ++j)
+                    ro[j] = M(reorient_ijk, 0, j) * iv[0] + M(reorient_ijk, 1, j) * iv[1] + M(reorient_ijk, 2, j) * iv[2];
+                size_t nidx = ((size_t)z * cpp->ny + y) * cpp->nx + x;
+                nodeGrad[nidx] = ro[0] * weight;
+                nodeGrad[nidx + ng] = ro[1] * weight;
+                nodeGrad[nidx + 2 * ng] = ro[2] * weight;
+            }
+}
+
+/* ------------------------------------------------------------------ bending energy */
+static float round6(double v) {
+    /* The reference tabulates these stencils as 6-decimal literals; reproduce the literal. */
+    double r = floor(fabs(v) * 1e6 + 0.5) / 1e6;
+    return (float)(v < 0 ? -r : r);
+}
+
+void orc_second_order_basis(float xx[27], float yy[27], float zz[27], float xy[27], float yz[27], float xz[27]) {
+    /* Tensor products of the cubic B-spline at a node: value {1/6, 2/3, 1/6},
+     * first derivative {-1/2, 0, 1/2}, second derivative {1, -2, 1}. */
+    const double v[3] = { 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0 };
+    const double d1[3] = { -0.5, 0.0, 0.5 };
+    const double d2[3] = { 1.0, -2.0, 1.0 };
+    int i = 0;
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a, ++i) {
+                xx[i] = round6(d2[a] * v[b] * v[c]);
+                yy[i] = round6(v[a] * d2[b] * v[c]);
+                zz[i] = round6(v[a] * v[b] * d2[c]);
+                xy[i] = round6(d1[a] * d1[b] * v[c]);
+                yz[i] = round6(v[a] * d1[b] * d1[c]);
+                xz[i] = round6(d1[a] * v[b] * d1[c]);
+            }
+}
+
+double orc_bending_energy(const orc_geom *cpp, const float *cppData) {
+    const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    float bxx[27], byy[27], bzz[27], bxy[27], byz[27], bxz[27];
+    orc_second_order_basis(bxx, byy, bzz, bxy, byz, bxz);
+    double total = 0;
+#pragma omp parallel for collapse(2) reduction(+ : total) schedule(static)
+    for (int z = 1; z < cpp->nz - 1; ++z)
+        for (int y = 1; y < cpp->ny - 1; ++y)
+            for (int x = 1; x < cpp->nx - 1; ++x) {
+                float d[3][6];
+                memset(d, 0, sizeof d);
+                int i = 0;
+                for (int c = -1; c < 2; ++c)
+                    for (int b = -1; b < 2; ++b)
+                        for (int a = -1; a < 2; ++a, ++i) {
+                            size_t idx = ((size_t)(z + c) * cpp->ny + y + b) * cpp->nx + x + a;
+                            for (int k = 0; k < 3; ++k) {
+                                float cv = cppData[idx + k * g];
+                                d[k][0] += bxx[i] * cv;
+                                d[k][1] += byy[i] * cv;
+                                d[k][2] += bzz[i] * cv;
+                                d[k][3] += bxy[i] * cv;
+                                d[k][4] += byz[i] * cv;
+                                d[k][5] += bxz[i] * cv;
+                            }
+                        }
+                /* same association as the reference expression (regul.cpp:146-149) */
+                double s = (double)(d[0][0] * d[0][0] + d[0][1] * d[0][1] + d[0][2] * d[0][2]) +
+                           2.0 * (double)(d[0][3] * d[0][3] + d[0][4] * d[0][4] + d[0][5] * d[0][5]);
+                for (int k = 1; k < 3; ++k) {
+                    s += (double)(d[k][0] * d[k][0]);
+                    s += (double)(d[k][1] * d[k][1]);
+                    s += (double)(d[k][2] * d[k][2]);
+                    s += 2.0 * (double)(d[k][3] * d[k][3] + d[k][4] * d[k][4] + d[k][5] * d[k][5]);
+                }
+                total += s;
+            }
+    return total / (double)(3 * g);
+}
+
+/* Identity node position in float, as reg_getDisplacementFromDeformation_3D (reg-lib/cpu/_reg_tools.cpp:1988-2031) */
+static void node_position(const orc_geom *cpp, int x, int y, int z, float out[3]) {
+    for (int r = 0; r < 3; ++r)
+        out[r] = M(cpp->xyz, r, 0) * (float)x + M(cpp->xyz, r, 1) * (float)y + M(cpp->xyz, r, 2) * (float)z +
+                 M(cpp->xyz, r, 3);
+}
+
+void orc_bending_energy_gradient(const orc_geom *cpp, const float *cppData, float weight, float *grad) {
+    const int nx = cpp->nx, ny = cpp->ny, nz = cpp->nz;
+    const size_t g = (size_t)nx * ny * nz;
+    float bxx[27], byy[27], bzz[27], bxy[27], byz[27], bxz[27];
+    orc_second_order_basis(bxx, byy, bzz, bxy, byz, bxz);
+    float *disp = (float *)malloc(3 * g * sizeof(float));
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                size_t idx = ((size_t)z * ny + y) * nx + x;
+                float p[3];
+                node_position(cpp, x, y, z, p);
+                for (int k = 0; k < 3; ++k) disp[idx + k * g] = cppData[idx + k * g] - p[k];
+            }
+    float *der = (float *)calloc(18 * g, sizeof(float));
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                float d[3][6];
+                memset(d, 0, sizeof d);
+                int i = 0;
+                for (int c = -1; c < 2; ++c)
+                    for (int b = -1; b < 2; ++b)
+                        for (int a = -1; a < 2; ++a, ++i) {
+                            if (x + a < 0 || y + b < 0 || z + c < 0 || x + a >= nx || y + b >= ny || z + c >= nz) continue;
+                            size_t idx = ((size_t)(z + c) * ny + y + b) * nx + x + a;
+                            for (int k = 0; k < 3; ++k) {
+                                float cv = disp[idx + k * g];
+                                d[k][0] += bxx[i] * cv;
+                                d[k][1] += byy[i] * cv;
+                                d[k][2] += bzz[i] * cv;
+                                d[k][3] += bxy[i] * cv;
+                                d[k][4] += byz[i] * cv;
+                                d[k][5] += bxz[i] * cv;
+                            }
+                        }
+                float *o = der + 18 * (((size_t)z * ny + y) * nx + x);
+                for (int t = 0; t < 3; ++t)
+                    for (int k = 0; k < 3; ++k) o[t * 3 + k] = d[k][t];
+                for (int t = 3; t < 6; ++t)
+                    for (int k = 0; k < 3; ++k) o[t * 3 + k] = (float)(2.0 * d[k][t]);
+            }
+    const float ratio = weight / (float)g;
+    const float *basis[6] = { bxx, byy, bzz, bxy, byz, bxz };
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                float gv[3] = { 0, 0, 0 };
+                int a = 0;
+                for (int Z = z - 1; Z < z + 2; ++Z)
+                    for (int Y = y - 1; Y < y + 2; ++Y)
+                        for (int X = x - 1; X < x + 2; ++X, ++a) {
+                            if (X < 0 || Y < 0 || Z < 0 || X >= nx || Y >= ny || Z >= nz) continue;
+                            const float *dp = der + 18 * (((size_t)Z * ny + Y) * nx + X);
+                            for (int t = 0; t < 6; ++t)
+                                for (int k = 0; k < 3; ++k) gv[k] += dp[t * 3 + k] * basis[t][a];
+                        }
+                size_t idx = ((size_t)z * ny + y) * nx + x;
+                for (int k = 0; k < 3; ++k) grad[idx + k * g] += ratio * gv[k];
+            }
+    free(der);
+    free(disp);
+}
+
+/* ------------------------------------------------------------------ optimiser helpers */
+float orc_max_length(const float *grad, size_t nodes) {
+    float mx = 0;
+    for (size_t i = 0; i < nodes; ++i) {
+        float x = grad[i], y = grad[i + nodes], z = grad[i + 2 * nodes];
+        float l = (float)sqrt(x * x + y * y + z * z);
+        mx = l > mx ? l : mx;
+    }
+    return mx;
+}
+
+void orc_normalise_gradient(float *grad, size_t nodes, double maxLength) {
+    if (maxLength == 0) return;
+    double inv = 1.0 / maxLength;
+    for (size_t i = 0; i < 3 * nodes; ++i) grad[i] = (float)((double)grad[i] * inv);
+}
+
+double orc_cg_update(float *grad, float *g, float *h, size_t n, int first) {
+    if (first) {
+        for (size_t i = 0; i < n; ++i) h[i] = g[i] = -grad[i];
+        return 0;
+    }
+    double dgg = 0, gg = 0;
+    for (size_t i = 0; i < n; ++i) {
+        gg += h[i] * g[i];
+        dgg += (grad[i] + g[i]) * grad[i];
+    }
+    double gam = dgg / gg;
+    for (size_t i = 0; i < n; ++i) {
+        g[i] = -grad[i];
+        h[i] = (float)(g[i] + gam * h[i]);
+        grad[i] = -h[i];
+    }
+    return gam;
+}
+
+void orc_update_dof(float *cur, const float *best, const float *grad, float scale, size_t n) {
+    for (size_t i = 0; i < n; ++i) cur[i] = best[i] + scale * grad[i];
+}
+
+/* ------------------------------------------------------------------ affine field */
+void orc_affine_field(const orc_geom *ref, const float affine[16], const int *mask, float *field) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    float T[16];
+    mat44_mm(affine, ref->xyz, T);
+    for (int z = 0; z < ref->nz; ++z)
+        for (int y = 0; y < ref->ny; ++y)
+            for (int x = 0; x < ref->nx; ++x) {
+                size_t idx = ((size_t)z * ref->ny + y) * ref->nx + x;
+                if (mask && mask[idx] < 0) continue;
+                double v[3] = { (double)x, (double)y, (double)z }, p[3];
+                mat44_mul_d(T, v, p);
+                field[idx] = (float)p[0];
+                field[idx + n] = (float)p[1];
+                field[idx + 2 * n] = (float)p[2];
+            }
+}

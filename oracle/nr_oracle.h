@@ -1,0 +1,99 @@
+/*
+ * nr_oracle.h -- CPU restatement of the NiftyReg F3D / Aladin hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only the tests,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product library
+ * (libnrhip.so) never links or calls it.
+ *
+ * Every function restates the reference CPU algorithm in plain C, following the scalar
+ * (non-SSE) summation order of the reference file cited beside it.  Images are planar float
+ * arrays, x fastest; vector images hold all x components, then all y, then all z.
+ */
+#ifndef NR_ORACLE_H
+#define NR_ORACLE_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Image geometry: dims, voxel size, voxel->mm (xyz) and mm->voxel (ijk), row-major 4x4. */
+typedef struct {
+    int nx, ny, nz;
+    float dx, dy, dz;
+    float xyz[16];
+    float ijk[16];
+} orc_geom;
+
+/* get_BSplineBasisValues<float> (reg-lib/cpu/_reg_splineBasis.cpp:17-26) */
+void orc_bspline_basis(float t, float out[4]);
+/* ... with first derivative (reg-lib/cpu/_reg_splineBasis.cpp:29-37) */
+void orc_bspline_basis_d1(float t, float val[4], float first[4]);
+
+/* reg_cubic_spline_getDeformationField3D, non-composition (reg-lib/cpu/_reg_localTrans.cpp:1080-1453).
+ * The 5-voxel look-up-table path (1110-1304) is used when cpp.d / ref.d == 5 on every axis.
+ * mask may be NULL (all active); masked voxels are left untouched on the LUT path and set to 0
+ * otherwise, as in the reference. */
+void orc_spline_field(const orc_geom *ref, const orc_geom *cpp, const float *cppData,
+                      const int *mask, float *field);
+
+/* ResampleImage3D with linear interpolation (reg-lib/cpu/_reg_resampling.cpp:316-464). */
+void orc_resample_linear(const orc_geom *flo, const float *floData, const orc_geom *ref,
+                         const float *field, const int *mask, float pad, float *warped);
+
+/* TrilinearImageGradient (reg-lib/cpu/_reg_resampling.cpp:1797-1915). */
+void orc_image_gradient_linear(const orc_geom *flo, const float *floData, const orc_geom *ref,
+                               const float *field, const int *mask, float pad, float *grad);
+
+/* reg_intensityRescale_core<float> for scl_slope 1, scl_inter 0 (reg-lib/cpu/_reg_tools.cpp:93-183). */
+void orc_intensity_rescale(float *data, size_t n, float newMin, float newMax);
+
+/* reg_getNmiValue<float>, approximate Parzen window (reg-lib/cpu/_reg_nmi.cpp:160-344).
+ * pro / logh: rb*fb + rb + fb doubles; ent: 4 doubles (H_r, H_f, H_rf, total).  Returns NMI. */
+double orc_nmi(const float *ref, const float *war, const int *mask, size_t n, int rb, int fb,
+               double *pro, double *logh, double *ent);
+
+/* reg_getVoxelBasedNmiGradient3d<float> (reg-lib/cpu/_reg_nmi.cpp:501-593); grad is += . */
+void orc_nmi_voxel_gradient(const float *ref, const float *war, const float *warGrad,
+                            const int *mask, size_t n, int rb, int fb, const double *logh,
+                            const double *ent, double weight, float *grad);
+
+/* reg_tools_kernelConvolution<float> for one volume and one axis (reg-lib/cpu/_reg_tools.cpp:832-1112),
+ * kernel types: 2 Gaussian, 3 cubic B-spline.  sigma > 0 is in mm, < 0 in voxels. */
+void orc_kernel_convolution_axis(const orc_geom *g, float *data, float sigma, int kernel,
+                                 const int *mask, int axis);
+
+/* Compute::ConvolveImage on a 3-component voxel gradient (reg-lib/Compute.cpp:230-264): cubic
+ * kernel, x pass with the grid dx, y pass with dy, z pass with dz. */
+void orc_convolve_voxel_gradient(const orc_geom *ref, const orc_geom *cpp, float *voxelGrad);
+
+/* reg_voxelCentricToNodeCentric, update = false (reg-lib/cpu/_reg_localTrans.cpp:1540-1684). */
+void orc_voxel_to_node(const orc_geom *cpp, const orc_geom *ref, const float *voxelGrad,
+                       const float reorient_ijk[16], float weight, float *nodeGrad);
+
+/* reg_spline_approxBendingEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:73-153). */
+double orc_bending_energy(const orc_geom *cpp, const float *cppData);
+/* reg_spline_approxBendingEnergyGradient3D (reg-lib/cpu/_reg_localTrans_regul.cpp:287-451); grad += */
+void orc_bending_energy_gradient(const orc_geom *cpp, const float *cppData, float weight, float *grad);
+/* set_second_order_bspline_basis_values (reg-lib/cpu/_reg_splineBasis.cpp:292-457) */
+void orc_second_order_basis(float xx[27], float yy[27], float zz[27], float xy[27], float yz[27], float xz[27]);
+
+/* reg_getMaximalLength<float> (reg-lib/cpu/_reg_tools.cpp:335-353) */
+float orc_max_length(const float *grad, size_t nodes);
+/* Compute::NormaliseGradient (reg-lib/Compute.cpp:152-183) */
+void orc_normalise_gradient(float *grad, size_t nodes, double maxLength);
+/* ConjugateGradient<float>::UpdateGradientValues (reg-lib/Optimiser.cpp:231-317).
+ * first != 0 initialises; returns gamma (0 on the first call). */
+double orc_cg_update(float *grad, float *g, float *h, size_t n, int first);
+/* Compute::UpdateControlPointPosition (reg-lib/Compute.cpp:91-127) */
+void orc_update_dof(float *cur, const float *best, const float *grad, float scale, size_t n);
+
+/* reg_affine_getDeformationField3D (reg-lib/cpu/_reg_globalTrans.cpp:74-132) */
+void orc_affine_field(const orc_geom *ref, const float affine[16], const int *mask, float *field);
+
+/* mat44 inverse in double, stored in float (nifti_mat44_inverse semantics). */
+void orc_mat44_inverse(const float in[16], float out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,535 @@
+// C-ABI shim over the reference CPU path (test infrastructure only).
+//
+// Compiled together with the reference's own sources into oracle/_ref/libnrref.so by
+// oracle/Makefile.  It lets the tests and the golden-vector generator call the reference
+// functions on plain arrays: every image is described by an nr_geom (dims + 4x4 sform,
+// row-major) and passed as a planar float array.  Nothing here re-implements reference
+// arithmetic; each entry point builds the nifti images and calls the named reference
+// function.
+
+#include "_reg_localTrans.h"
+#include "_reg_localTrans_regul.h"
+#include "_reg_resampling.h"
+#include "_reg_nmi.h"
+#include "_reg_lncc.h"
+#include "_reg_tools.h"
+#include "_reg_globalTrans.h"
+#include "_reg_blockMatching.h"
+#include "_reg_f3d.h"
+#include "_reg_aladin.h"
+#include "F3dContent.h"
+#include <cstring>
+#include <vector>
+#include <memory>
+
+extern "C" {
+typedef struct {
+    int nx, ny, nz;
+    float pixdim[3];  // pixdim[1..3] of the header (0: derive from the matrix columns)
+    float sform[16];  // row-major voxel -> mm
+} nr_geom;
+}
+
+namespace {
+
+mat44 to_mat44(const float *m) {
+    mat44 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = m[i * 4 + j];
+    return r;
+}
+
+void from_mat44(const mat44& r, float *m) {
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) m[i * 4 + j] = r.m[i][j];
+}
+
+// Set sform and qform of an image from a 4x4 voxel->mm matrix.
+void set_orientation(nifti_image *img, const mat44& m) {
+    img->sform_code = 1;
+    img->sto_xyz = m;
+    img->sto_ijk = nifti_mat44_inverse(m);
+    float qb, qc, qd, qx, qy, qz, dx, dy, dz, qfac;
+    nifti_mat44_to_quatern(m, &qb, &qc, &qd, &qx, &qy, &qz, &dx, &dy, &dz, &qfac);
+    img->qform_code = 1;
+    img->quatern_b = qb; img->quatern_c = qc; img->quatern_d = qd;
+    img->qoffset_x = qx; img->qoffset_y = qy; img->qoffset_z = qz;
+    img->qfac = qfac;
+    img->qto_xyz = nifti_quatern_to_mat44(qb, qc, qd, qx, qy, qz, dx, dy, dz, qfac);
+    img->qto_ijk = nifti_mat44_inverse(img->qto_xyz);
+    img->pixdim[1] = img->dx = dx;
+    img->pixdim[2] = img->dy = dy;
+    img->pixdim[3] = img->dz = dz;
+}
+
+NiftiImage make_image(const nr_geom *g, const float *data, int nt = 1, int nu = 1) {
+    std::vector<NiftiImage::dim_t> dims{ g->nx, g->ny, g->nz };
+    if (nt > 1 || nu > 1) dims.push_back(nt);
+    if (nu > 1) dims.push_back(nu);
+    NiftiImage img(dims, NIFTI_TYPE_FLOAT32);
+    img->pixdim[0] = 1.f;
+    set_orientation(img, to_mat44(g->sform));
+    if (g->pixdim[0] > 0) {
+        img->pixdim[1] = img->dx = g->pixdim[0];
+        img->pixdim[2] = img->dy = g->pixdim[1];
+        img->pixdim[3] = img->dz = g->pixdim[2];
+    }
+    if (nt > 1 || nu > 1) { img->pixdim[4] = img->dt = 1.f; }
+    if (nu > 1) { img->pixdim[5] = img->du = 1.f; }
+    img->scl_slope = 1.f;
+    img->scl_inter = 0.f;
+    const size_t n = (size_t)g->nx * g->ny * g->nz * nt * nu;
+    if (data) std::memcpy(img->data, data, n * sizeof(float));
+    else std::memset(img->data, 0, n * sizeof(float));
+    return img;
+}
+
+// A vector image (field / gradient) laid out like the reference's deformation field.
+NiftiImage make_vector_image(const nr_geom *g, const float *data) {
+    NiftiImage img = make_image(g, data, 1, 3);
+    img->intent_code = NIFTI_INTENT_VECTOR;
+    img->intent_p1 = DEF_FIELD;
+    return img;
+}
+
+// A control point grid image with the reference's header conventions.
+NiftiImage make_grid(const nr_geom *g, const float *data) {
+    NiftiImage img = make_vector_image(g, data);
+    img->intent_p1 = CUB_SPLINE_GRID;
+    img.setIntentName("NREG_TRANS"s);
+    return img;
+}
+
+size_t nvox(const nr_geom *g) { return (size_t)g->nx * g->ny * g->nz; }
+
+// Geometry of an existing image (sform if set, else qform).
+void geom_of(const nifti_image *img, nr_geom *g) {
+    g->nx = img->nx; g->ny = img->ny; g->nz = img->nz;
+    g->pixdim[0] = img->dx; g->pixdim[1] = img->dy; g->pixdim[2] = img->dz;
+    from_mat44(img->sform_code > 0 ? img->sto_xyz : img->qto_xyz, g->sform);
+}
+
+thread_local std::string last_error;
+
+// reg_nmi with its per-time-point histogram buffers exposed for comparison.
+class RefNmi: public reg_nmi {
+public:
+    double* Pro() { return this->jointHistogramPro[0]; }
+    double* Log() { return this->jointHistogramLog[0]; }
+    double* Ent() { return this->entropyValues[0]; }
+    int Total() { return this->totalBinNumber[0]; }
+};
+
+template<class F> int guarded(F&& f) {
+    try { f(); return 0; }
+    catch (const std::exception& e) { last_error = e.what(); return 1; }
+}
+
+} // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return last_error.c_str(); }
+
+// ---------------------------------------------------------------- control point grid
+// reg_createControlPointGrid<float> (cpu/_reg_localTrans.cpp:21-128)
+int ref_create_cpp(const nr_geom *ref, const float spacing_mm[3], nr_geom *out, float *data) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, nullptr);
+        NiftiImage cpp;
+        reg_createControlPointGrid<float>(cpp, refImg, spacing_mm);
+        geom_of(cpp, out);
+        if (data) std::memcpy(data, cpp->data, cpp.nVoxels() * sizeof(float));
+    });
+}
+
+// reg_spline_refineControlPointGrid (cpu/_reg_localTrans.cpp:1809-2120); out grid has the
+// refined size which the caller obtains by a first call with data_out == nullptr.
+int ref_refine_cpp(const nr_geom *ref, const nr_geom *cpp, const float *cppData, nr_geom *out, float *data_out) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, nullptr);
+        NiftiImage grid = make_grid(cpp, cppData);
+        reg_spline_refineControlPointGrid(grid, refImg);
+        geom_of(grid, out);
+        if (data_out) std::memcpy(data_out, grid->data, grid.nVoxels() * sizeof(float));
+    });
+}
+
+// ---------------------------------------------------------------- deformation field
+// reg_spline_getDeformationField (cpu/_reg_localTrans.cpp:1457-1538)
+int ref_spline_field(const nr_geom *ref, const nr_geom *cpp, const float *cppData, const int *mask,
+                     int forceNoLut, float *field) {
+    return guarded([&] {
+        NiftiImage grid = make_grid(cpp, cppData);
+        NiftiImage def = make_vector_image(ref, field);  // masked voxels keep their input value
+        std::vector<int> m(mask ? mask : nullptr, mask ? mask + nvox(ref) : nullptr);
+        if (!mask) m.assign(nvox(ref), 0);
+        reg_spline_getDeformationField(grid, def, m.data(), false, true, forceNoLut != 0);
+        std::memcpy(field, def->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// reg_affine_getDeformationField (cpu/_reg_globalTrans.cpp:135-175)
+int ref_affine_field(const nr_geom *ref, const float affine[16], const int *mask, float *field) {
+    return guarded([&] {
+        NiftiImage def = make_vector_image(ref, field);
+        mat44 a = to_mat44(affine);
+        std::vector<int> m(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, nvox(ref) * sizeof(int));
+        reg_affine_getDeformationField(&a, def, false, m.data());
+        std::memcpy(field, def->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// ---------------------------------------------------------------- resampling
+// reg_resampleImage (cpu/_reg_resampling.cpp:598-691); warped holds the values kept for masked voxels
+int ref_resample(const nr_geom *flo, const float *floData, const nr_geom *ref, const float *field,
+                 const int *mask, int interp, float pad, float *warped) {
+    return guarded([&] {
+        NiftiImage floImg = make_image(flo, floData);
+        NiftiImage def = make_vector_image(ref, field);
+        NiftiImage war = make_image(ref, warped);
+        std::vector<int> m(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, nvox(ref) * sizeof(int));
+        reg_resampleImage(floImg, war, def, m.data(), interp, pad);
+        std::memcpy(warped, war->data, nvox(ref) * sizeof(float));
+    });
+}
+
+// reg_getImageGradient (cpu/_reg_resampling.cpp:2269-2384)
+int ref_image_gradient(const nr_geom *flo, const float *floData, const nr_geom *ref, const float *field,
+                       const int *mask, int interp, float pad, float *grad) {
+    return guarded([&] {
+        NiftiImage floImg = make_image(flo, floData);
+        NiftiImage def = make_vector_image(ref, field);
+        NiftiImage g = make_vector_image(ref, nullptr);
+        std::vector<int> m(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, nvox(ref) * sizeof(int));
+        reg_getImageGradient(floImg, g, def, m.data(), interp, pad, 0);
+        std::memcpy(grad, g->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// ---------------------------------------------------------------- NMI
+// reg_intensityRescale (cpu/_reg_tools.cpp:93-208)
+int ref_intensity_rescale(const nr_geom *g, float *data, float newMin, float newMax) {
+    return guarded([&] {
+        NiftiImage img = make_image(g, data);
+        reg_intensityRescale(img, 0, newMin, newMax);
+        std::memcpy(data, img->data, nvox(g) * sizeof(float));
+    });
+}
+
+// reg_nmi value (cpu/_reg_nmi.cpp:160-398) on already-rescaled images.
+// histPro / histLog: totalBins doubles each (bins_r*bins_f + bins_r + bins_f); ent: 4 doubles.
+int ref_nmi_value(const nr_geom *ref, const float *refData, const float *warped, const int *mask,
+                  int refBins, int floBins, int approx, double *value, double *histPro, double *histLog,
+                  double *ent) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, refData);
+        NiftiImage floImg = make_image(ref, refData);
+        NiftiImage war = make_image(ref, warped);
+        NiftiImage wg = make_vector_image(ref, nullptr);
+        NiftiImage vg = make_vector_image(ref, nullptr);
+        std::vector<int> m(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, nvox(ref) * sizeof(int));
+        // Bypass the in-place rescale of InitialiseMeasure: the caller passes rescaled data,
+        // so the measure is initialised on images whose range is already [2, bins-3].
+        RefNmi nmi;
+        nmi.SetTimePointWeight(0, 1.0);
+        nmi.SetReferenceBinNumber(refBins, 0);
+        nmi.SetFloatingBinNumber(floBins, 0);
+        if (!approx) nmi.DoNotApproximatePw();
+        nmi.InitialiseMeasure(refImg, floImg, m.data(), war, wg, vg);
+        // InitialiseMeasure rescaled refImg; restore the caller's values
+        std::memcpy(refImg->data, refData, nvox(ref) * sizeof(float));
+        *value = nmi.GetSimilarityMeasureValue();
+        if (histPro) std::memcpy(histPro, nmi.Pro(), nmi.Total() * sizeof(double));
+        if (histLog) std::memcpy(histLog, nmi.Log(), nmi.Total() * sizeof(double));
+        if (ent) std::memcpy(ent, nmi.Ent(), 4 * sizeof(double));
+    });
+}
+
+// NMI value + voxel-based gradient (cpu/_reg_nmi.cpp:501-638).  voxelGrad is accumulated (+=).
+int ref_nmi_gradient(const nr_geom *ref, const float *refData, const float *warped, const float *warpedGrad,
+                     const int *mask, int refBins, int floBins, double *value, float *voxelGrad) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, refData);
+        NiftiImage floImg = make_image(ref, refData);
+        NiftiImage war = make_image(ref, warped);
+        NiftiImage wg = make_vector_image(ref, warpedGrad);
+        NiftiImage vg = make_vector_image(ref, voxelGrad);
+        std::vector<int> m(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, nvox(ref) * sizeof(int));
+        reg_nmi nmi;
+        nmi.SetTimePointWeight(0, 1.0);
+        nmi.SetReferenceBinNumber(refBins, 0);
+        nmi.SetFloatingBinNumber(floBins, 0);
+        nmi.InitialiseMeasure(refImg, floImg, m.data(), war, wg, vg);
+        std::memcpy(refImg->data, refData, nvox(ref) * sizeof(float));
+        *value = nmi.GetSimilarityMeasureValue();
+        nmi.GetVoxelBasedSimilarityMeasureGradient(0);
+        std::memcpy(voxelGrad, vg->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// ---------------------------------------------------------------- convolution / node gradient
+// reg_tools_kernelConvolution (cpu/_reg_tools.cpp:832-1341) on an image with nt volumes.
+// kernel: 0 mean, 1 linear, 2 gaussian, 3 cubic (ConvKernelType order).
+int ref_kernel_convolution(const nr_geom *g, int nt, float *data, const float *sigma, int kernel,
+                           const int *mask, const int axes[3]) {
+    return guarded([&] {
+        NiftiImage img = make_image(g, data, nt, 1);
+        std::unique_ptr<bool[]> tp(new bool[nt]);
+        for (int i = 0; i < nt; ++i) tp[i] = true;
+        bool ax[3] = { axes[0] != 0, axes[1] != 0, axes[2] != 0 };
+        reg_tools_kernelConvolution(img, sigma, static_cast<ConvKernelType>(kernel), mask, tp.get(), ax);
+        std::memcpy(data, img->data, nvox(g) * nt * sizeof(float));
+    });
+}
+
+// Compute::ConvolveImage (Compute.cpp:230-264) followed by Compute::VoxelCentricToNodeCentric
+// (Compute.cpp:266-274 -> cpu/_reg_localTrans.cpp:1540-1705).  voxelGrad is convolved in place.
+int ref_node_gradient(const nr_geom *ref, float *voxelGrad, const nr_geom *cpp, const nr_geom *flo,
+                      float weight, float *nodeGrad) {
+    return guarded([&] {
+        NiftiImage vg = make_vector_image(ref, voxelGrad);
+        NiftiImage grid = make_grid(cpp, nullptr);
+        NiftiImage floImg = make_image(flo, nullptr);
+        const int axesOrder[3][3] = { { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+        const float sp[3] = { grid->dx, grid->dy, grid->dz };
+        for (int a = 0; a < 3; ++a) {
+            float s[3] = { sp[a], sp[a], sp[a] };
+            bool ax[3] = { axesOrder[a][0] != 0, axesOrder[a][1] != 0, axesOrder[a][2] != 0 };
+            reg_tools_kernelConvolution(vg, s, ConvKernelType::Cubic, nullptr, nullptr, ax);
+        }
+        std::memcpy(voxelGrad, vg->data, 3 * nvox(ref) * sizeof(float));
+        NiftiImage ng = make_grid(cpp, nullptr);
+        mat44 *reorient = Content::GetIJKMatrix(*floImg);
+        reg_voxelCentricToNodeCentric(ng, vg, weight, false, reorient);
+        std::memcpy(nodeGrad, ng->data, 3 * nvox(cpp) * sizeof(float));
+    });
+}
+
+// ---------------------------------------------------------------- bending energy
+// reg_spline_approxBendingEnergy (cpu/_reg_localTrans_regul.cpp:73-177)
+int ref_be_value(const nr_geom *cpp, const float *cppData, double *value) {
+    return guarded([&] {
+        NiftiImage grid = make_grid(cpp, cppData);
+        *value = reg_spline_approxBendingEnergy(grid);
+    });
+}
+
+// reg_spline_approxBendingEnergyGradient (cpu/_reg_localTrans_regul.cpp:287-482); grad is += .
+// The reference converts the grid to displacement and back in place; the round-tripped grid is
+// returned in cppOut when non-null.
+int ref_be_gradient(const nr_geom *cpp, const float *cppData, float weight, float *grad, float *cppOut) {
+    return guarded([&] {
+        NiftiImage grid = make_grid(cpp, cppData);
+        NiftiImage g = make_grid(cpp, grad);
+        reg_spline_approxBendingEnergyGradient(grid, g, weight);
+        std::memcpy(grad, g->data, 3 * nvox(cpp) * sizeof(float));
+        if (cppOut) std::memcpy(cppOut, grid->data, 3 * nvox(cpp) * sizeof(float));
+    });
+}
+
+// reg_spline_approxLinearEnergy / Gradient (cpu/_reg_localTrans_regul.cpp:560-857)
+int ref_le_value(const nr_geom *cpp, const float *cppData, double *value) {
+    return guarded([&] {
+        NiftiImage grid = make_grid(cpp, cppData);
+        *value = reg_spline_approxLinearEnergy(grid);
+    });
+}
+int ref_le_gradient(const nr_geom *cpp, const float *cppData, float weight, float *grad) {
+    return guarded([&] {
+        NiftiImage grid = make_grid(cpp, cppData);
+        NiftiImage g = make_grid(cpp, grad);
+        reg_spline_approxLinearEnergyGradient(grid, g, weight);
+        std::memcpy(grad, g->data, 3 * nvox(cpp) * sizeof(float));
+    });
+}
+
+// ---------------------------------------------------------------- pyramid
+// reg_downsampleImage<float> (cpu/_reg_tools.cpp:1343-1467), type 1 = smoothed + decimated.
+// First call with out data == nullptr to get the output geometry.
+int ref_downsample(const nr_geom *g, const float *data, nr_geom *out, float *outData) {
+    return guarded([&] {
+        NiftiImage img = make_image(g, data);
+        bool axes[8] = { false, true, true, true, false, false, false, false };
+        reg_downsampleImage<float>(img, 1, axes);
+        geom_of(img, out);
+        if (outData) std::memcpy(outData, img->data, img.nVoxels() * sizeof(float));
+    });
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- full F3D through reg_f3d
+namespace {
+class RefF3d: public reg_f3d<float> {
+public:
+    RefF3d(): reg_f3d<float>(1, 1) {}
+    // Prepare level `level` the way reg_base::Run does before its optimisation loop.
+    void Prepare(int level) {
+        this->Initialise();
+        this->maxIterationNumber *= 1;
+        this->InitCurrentLevel(level);
+        this->InitialiseSimilarity();
+        this->SetOptimiser();
+    }
+    double Value() { return this->GetObjectiveFunctionValue(); }
+    void Gradient() { this->GetObjectiveFunctionGradient(); }
+    double NormGrad() { return this->NormaliseGradient(); }
+    F3dContent& Con() { return dynamic_cast<F3dContent&>(*this->con); }
+    NiftiImage& Grid() { return this->controlPointGrid; }
+    double WMeasure() { return this->currentWMeasure; }
+    double WBE() { return this->currentWBE; }
+    double WLE() { return this->currentWLE; }
+    float SimWeight() { return this->similarityWeight; }
+    float BEWeight() { return this->bendingEnergyWeight; }
+    size_t Iterations() { return this->optimiser ? this->optimiser->GetCurrentIterationNumber() : 0; }
+};
+
+struct F3dHandle {
+    std::unique_ptr<RefF3d> reg;
+    nr_geom refG, floG;
+};
+} // namespace
+
+extern "C" {
+
+typedef struct {
+    int bins;            // user bin count (reference adds 4)
+    float be;            // bending energy weight
+    float le;            // linear energy weight
+    float spacing;       // control point spacing; negative = voxels
+    int levels;          // pyramid levels
+    int levels_to_perform;
+    int maxit;           // final-level iteration budget
+    int use_pyramid;
+    float padding;       // warped padding value
+} ref_f3d_params;
+
+static void configure(RefF3d& r, const ref_f3d_params *p) {
+    r.SetPlatformType(PlatformType::Cpu);
+    r.UseNMISetReferenceBinNumber(0, p->bins);
+    r.UseNMISetFloatingBinNumber(0, p->bins);
+    r.SetBendingEnergyWeight(p->be);
+    r.SetLinearEnergyWeight(p->le);
+    r.SetJacobianLogWeight(0);
+    r.SetSpacing(0, p->spacing);
+    r.SetLevelNumber(p->levels);
+    r.SetLevelToPerform(p->levels_to_perform);
+    r.SetMaximalIterationNumber(p->maxit);
+    if (!p->use_pyramid) r.DoNotUsePyramidalApproach();
+    r.SetWarpedPaddingValue(p->padding);
+    r.DoNotPrintOutInformation();
+}
+
+// Create a reg_f3d<float>, run Initialise / InitCurrentLevel(level) / InitialiseSimilarity /
+// SetOptimiser so that single objective+gradient evaluations can be timed and compared.
+void* ref_f3d_create(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                     const ref_f3d_params *p, int level) {
+    F3dHandle *h = new F3dHandle;
+    int rc = guarded([&] {
+        h->reg.reset(new RefF3d);
+        h->refG = *ref; h->floG = *flo;
+        configure(*h->reg, p);
+        h->reg->SetReferenceImage(make_image(ref, refData));
+        h->reg->SetFloatingImage(make_image(flo, floData));
+        h->reg->Prepare(level);
+    });
+    if (rc) { delete h; return nullptr; }
+    return h;
+}
+
+void ref_f3d_destroy(void *hp) { delete static_cast<F3dHandle*>(hp); }
+
+// Geometry of the current level's grid and images.
+int ref_f3d_level_geom(void *hp, nr_geom *cpp, nr_geom *ref, nr_geom *flo) {
+    F3dHandle *h = static_cast<F3dHandle*>(hp);
+    return guarded([&] {
+        F3dContent& c = h->reg->Con();
+        geom_of(c.F3dContent::GetControlPointGrid(), cpp);
+        geom_of(c.Content::GetReference(), ref);
+        geom_of(c.Content::GetFloating(), flo);
+    });
+}
+
+// Copy out the current level's (rescaled) reference / floating images.
+int ref_f3d_level_images(void *hp, float *refData, float *floData) {
+    F3dHandle *h = static_cast<F3dHandle*>(hp);
+    return guarded([&] {
+        F3dContent& c = h->reg->Con();
+        NiftiImage& r = c.Content::GetReference();
+        NiftiImage& f = c.Content::GetFloating();
+        if (refData) std::memcpy(refData, r->data, r.nVoxelsPerVolume() * sizeof(float));
+        if (floData) std::memcpy(floData, f->data, f.nVoxelsPerVolume() * sizeof(float));
+    });
+}
+
+int ref_f3d_set_cpp(void *hp, const float *cppData) {
+    F3dHandle *h = static_cast<F3dHandle*>(hp);
+    return guarded([&] {
+        NiftiImage& g = h->reg->Con().F3dContent::GetControlPointGrid();
+        std::memcpy(g->data, cppData, g.nVoxels() * sizeof(float));
+    });
+}
+
+int ref_f3d_get_cpp(void *hp, float *cppData) {
+    F3dHandle *h = static_cast<F3dHandle*>(hp);
+    return guarded([&] {
+        NiftiImage& g = h->reg->Con().F3dContent::GetControlPointGrid();
+        std::memcpy(cppData, g->data, g.nVoxels() * sizeof(float));
+    });
+}
+
+// reg_f3d::GetObjectiveFunctionValue (_reg_f3d.cpp:443-463).  terms: wMeasure, wBE, wLE.
+int ref_f3d_value(void *hp, double *value, double *terms) {
+    F3dHandle *h = static_cast<F3dHandle*>(hp);
+    return guarded([&] {
+        *value = h->reg->Value();
+        if (terms) { terms[0] = h->reg->WMeasure(); terms[1] = h->reg->WBE(); terms[2] = h->reg->WLE(); }
+    });
+}
+
+// reg_f3d::GetObjectiveFunctionGradient (_reg_f3d.cpp:553-577), optionally followed by
+// reg_f3d::NormaliseGradient (_reg_f3d.cpp:395-411).  grad receives the transformation gradient.
+int ref_f3d_gradient(void *hp, int normalise, double *maxLength, float *grad, float *voxelGrad) {
+    F3dHandle *h = static_cast<F3dHandle*>(hp);
+    return guarded([&] {
+        h->reg->Gradient();
+        if (normalise) { double m = h->reg->NormGrad(); if (maxLength) *maxLength = m; }
+        F3dContent& c = h->reg->Con();
+        NiftiImage& tg = c.GetTransformationGradient();
+        if (grad) std::memcpy(grad, tg->data, tg.nVoxels() * sizeof(float));
+        if (voxelGrad) {
+            NiftiImage& vg = c.GetVoxelBasedMeasureGradient();
+            std::memcpy(voxelGrad, vg->data, vg.nVoxels() * sizeof(float));
+        }
+    });
+}
+
+int ref_f3d_weights(void *hp, float *w) {
+    F3dHandle *h = static_cast<F3dHandle*>(hp);
+    w[0] = h->reg->SimWeight(); w[1] = h->reg->BEWeight();
+    return 0;
+}
+
+// Full registration through reg_base::Run (_reg_base.cpp:727-819).  Output: final-level grid.
+int ref_f3d_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                const ref_f3d_params *p, nr_geom *cppOut, float *cppData, int cppCapacity) {
+    return guarded([&] {
+        RefF3d r;
+        configure(r, p);
+        r.SetReferenceImage(make_image(ref, refData));
+        r.SetFloatingImage(make_image(flo, floData));
+        r.Run();
+        NiftiImage g = r.GetControlPointPositionImage();
+        geom_of(g, cppOut);
+        if (cppData && (int)g.nVoxels() <= cppCapacity)
+            std::memcpy(cppData, g->data, g.nVoxels() * sizeof(float));
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,349 @@
+"""ctypes access to the checkers: oracle/liboracle.so (the C restatement) and
+oracle/_ref/libnrref.so (the reference CPU path compiled from its own sources).
+
+Test infrastructure only.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libnrref.so")
+
+F = ctypes.POINTER(ctypes.c_float)
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+
+
+def fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(F)
+
+
+def dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(D)
+
+
+def ip(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(I)
+
+
+class orc_geom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float),
+                ("xyz", ctypes.c_float * 16), ("ijk", ctypes.c_float * 16)]
+
+
+class nr_geom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("pixdim", ctypes.c_float * 3),
+                ("sform", ctypes.c_float * 16)]
+
+
+class ref_f3d_params(ctypes.Structure):
+    _fields_ = [("bins", ctypes.c_int), ("be", ctypes.c_float), ("le", ctypes.c_float), ("spacing", ctypes.c_float),
+                ("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("maxit", ctypes.c_int),
+                ("use_pyramid", ctypes.c_int), ("padding", ctypes.c_float)]
+
+
+def og(g):
+    o = orc_geom()
+    o.nx, o.ny, o.nz = g.dims
+    o.dx, o.dy, o.dz = g.pixdim
+    o.xyz[:] = [float(v) for v in g.xyz.reshape(16)]
+    o.ijk[:] = [float(v) for v in g.ijk.reshape(16)]
+    return o
+
+
+def rg(g):
+    r = nr_geom()
+    r.nx, r.ny, r.nz = g.dims
+    r.pixdim[:] = [float(v) for v in g.pixdim]
+    r.sform[:] = [float(v) for v in g.xyz.reshape(16)]
+    return r
+
+
+_orc = None
+_ref = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        _orc = ctypes.CDLL(ORACLE)
+        _orc.orc_nmi.restype = ctypes.c_double
+        _orc.orc_bending_energy.restype = ctypes.c_double
+        _orc.orc_max_length.restype = ctypes.c_float
+        _orc.orc_cg_update.restype = ctypes.c_double
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = ctypes.CDLL(REF)
+        _ref.ref_last_error.restype = ctypes.c_char_p
+        _ref.ref_f3d_create.restype = ctypes.c_void_p
+        for n in ("ref_f3d_destroy", "ref_f3d_level_geom", "ref_f3d_level_images", "ref_f3d_set_cpp",
+                  "ref_f3d_get_cpp", "ref_f3d_value", "ref_f3d_gradient", "ref_f3d_weights"):
+            getattr(_ref, n).argtypes = None
+    return _ref
+
+
+def rcheck(rc):
+    if rc != 0:
+        raise RuntimeError(ref().ref_last_error().decode())
+
+
+# ---------------------------------------------------------------- oracle wrappers
+def orc_field(refg, cppg, cpp, mask=None, init=None):
+    n = refg.nvox
+    field = np.zeros(3 * n, np.float32) if init is None else init.astype(np.float32).copy()
+    orc().orc_spline_field(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(cpp), ip(mask), fp(field))
+    return field
+
+
+def orc_resample(flog, flo, refg, field, mask=None, pad=float("nan"), init=None):
+    w = np.zeros(refg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    orc().orc_resample_linear(ctypes.byref(og(flog)), fp(flo), ctypes.byref(og(refg)), fp(field), ip(mask),
+                              ctypes.c_float(pad), fp(w))
+    return w
+
+
+def orc_image_gradient(flog, flo, refg, field, mask=None, pad=float("nan")):
+    g = np.zeros(3 * refg.nvox, np.float32)
+    orc().orc_image_gradient_linear(ctypes.byref(og(flog)), fp(flo), ctypes.byref(og(refg)), fp(field), ip(mask),
+                                    ctypes.c_float(pad), fp(g))
+    return g
+
+
+def orc_rescale(a, lo, hi):
+    a = np.ascontiguousarray(a, np.float32).copy()
+    orc().orc_intensity_rescale(fp(a), ctypes.c_size_t(a.size), ctypes.c_float(lo), ctypes.c_float(hi))
+    return a
+
+
+def orc_nmi(refv, war, rb, fb, mask=None):
+    tot = rb * fb + rb + fb
+    pro = np.zeros(tot, np.float64)
+    logh = np.zeros(tot, np.float64)
+    ent = np.zeros(4, np.float64)
+    v = orc().orc_nmi(fp(refv), fp(war), ip(mask), ctypes.c_size_t(refv.size), rb, fb, dp(pro), dp(logh), dp(ent))
+    return v, pro, logh, ent
+
+
+def orc_nmi_grad(refv, war, wgrad, rb, fb, logh, ent, weight=1.0, mask=None, init=None):
+    g = np.zeros(3 * refv.size, np.float32) if init is None else init.astype(np.float32).copy()
+    orc().orc_nmi_voxel_gradient(fp(refv), fp(war), fp(wgrad), ip(mask), ctypes.c_size_t(refv.size), rb, fb,
+                                 dp(logh), dp(ent), ctypes.c_double(weight), fp(g))
+    return g
+
+
+def orc_convolve_voxel_gradient(refg, cppg, vg):
+    vg = vg.astype(np.float32).copy()
+    orc().orc_convolve_voxel_gradient(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vg))
+    return vg
+
+
+def orc_voxel_to_node(cppg, refg, vg, flog, weight):
+    ng = np.zeros(3 * cppg.nvox, np.float32)
+    ijk = np.ascontiguousarray(flog.ijk, np.float32).reshape(16)
+    orc().orc_voxel_to_node(ctypes.byref(og(cppg)), ctypes.byref(og(refg)), fp(vg), fp(ijk), ctypes.c_float(weight), fp(ng))
+    return ng
+
+
+def orc_be(cppg, cpp):
+    return orc().orc_bending_energy(ctypes.byref(og(cppg)), fp(cpp))
+
+
+def orc_be_grad(cppg, cpp, weight, init=None):
+    g = np.zeros(3 * cppg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    orc().orc_bending_energy_gradient(ctypes.byref(og(cppg)), fp(cpp), ctypes.c_float(weight), fp(g))
+    return g
+
+
+def orc_node_gradient_chain(refg, flog, cppg, ref_r, flo_r, cpp, rb, fb, sim_w, be_w, pad=float("nan"), mask=None):
+    """reg_f3d::GetObjectiveFunctionGradient on the oracle: (node gradient, nmi, voxel gradient)."""
+    field = orc_field(refg, cppg, cpp, mask)
+    war = orc_resample(flog, flo_r, refg, field, mask, pad)
+    nmi, pro, logh, ent = orc_nmi(ref_r, war, rb, fb, mask)
+    wg = orc_image_gradient(flog, flo_r, refg, field, mask, pad)
+    vg = orc_nmi_grad(ref_r, war, wg, rb, fb, logh, ent, 1.0, mask)
+    cv = orc_convolve_voxel_gradient(refg, cppg, vg)
+    ng = orc_voxel_to_node(cppg, refg, cv, flog, sim_w)
+    if be_w > 0:
+        ng = orc_be_grad(cppg, cpp, be_w, ng)
+    return ng, nmi, vg
+
+
+# ---------------------------------------------------------------- reference wrappers
+def ref_field(refg, cppg, cpp, mask=None, force_no_lut=False, init=None):
+    field = np.zeros(3 * refg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    rcheck(ref().ref_spline_field(ctypes.byref(rg(refg)), ctypes.byref(rg(cppg)), fp(cpp), ip(mask),
+                                  int(force_no_lut), fp(field)))
+    return field
+
+
+def ref_resample(flog, flo, refg, field, mask=None, pad=float("nan"), init=None):
+    w = np.zeros(refg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    rcheck(ref().ref_resample(ctypes.byref(rg(flog)), fp(flo), ctypes.byref(rg(refg)), fp(field), ip(mask), 1,
+                              ctypes.c_float(pad), fp(w)))
+    return w
+
+
+def ref_image_gradient(flog, flo, refg, field, mask=None, pad=float("nan")):
+    g = np.zeros(3 * refg.nvox, np.float32)
+    rcheck(ref().ref_image_gradient(ctypes.byref(rg(flog)), fp(flo), ctypes.byref(rg(refg)), fp(field), ip(mask), 1,
+                                    ctypes.c_float(pad), fp(g)))
+    return g
+
+
+def ref_rescale(g, a, lo, hi):
+    a = np.ascontiguousarray(a, np.float32).copy()
+    rcheck(ref().ref_intensity_rescale(ctypes.byref(rg(g)), fp(a), ctypes.c_float(lo), ctypes.c_float(hi)))
+    return a
+
+
+def ref_nmi(g, refv, war, rb, fb, mask=None):
+    tot = rb * fb + rb + fb
+    pro = np.zeros(tot, np.float64)
+    logh = np.zeros(tot, np.float64)
+    ent = np.zeros(4, np.float64)
+    v = ctypes.c_double()
+    rcheck(ref().ref_nmi_value(ctypes.byref(rg(g)), fp(refv), fp(war), ip(mask), rb, fb, 1, ctypes.byref(v),
+                               dp(pro), dp(logh), dp(ent)))
+    return v.value, pro, logh, ent
+
+
+def ref_nmi_grad(g, refv, war, wgrad, rb, fb, mask=None, init=None):
+    vg = np.zeros(3 * g.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    v = ctypes.c_double()
+    rcheck(ref().ref_nmi_gradient(ctypes.byref(rg(g)), fp(refv), fp(war), fp(wgrad), ip(mask), rb, fb,
+                                  ctypes.byref(v), fp(vg)))
+    return v.value, vg
+
+
+def ref_node_gradient(refg, vg, cppg, flog, weight):
+    vg = vg.astype(np.float32).copy()
+    ng = np.zeros(3 * cppg.nvox, np.float32)
+    rcheck(ref().ref_node_gradient(ctypes.byref(rg(refg)), fp(vg), ctypes.byref(rg(cppg)), ctypes.byref(rg(flog)),
+                                   ctypes.c_float(weight), fp(ng)))
+    return ng, vg
+
+
+def ref_kernel_convolution(g, data, sigma, kernel, axes, mask=None):
+    data = np.ascontiguousarray(data, np.float32).copy()
+    nt = data.size // g.nvox
+    s = np.full(nt, sigma, np.float32)
+    ax = np.asarray(axes, np.int32)
+    rcheck(ref().ref_kernel_convolution(ctypes.byref(rg(g)), nt, fp(data), fp(s), kernel, ip(mask), ip(ax)))
+    return data
+
+
+def ref_be(cppg, cpp):
+    v = ctypes.c_double()
+    rcheck(ref().ref_be_value(ctypes.byref(rg(cppg)), fp(cpp), ctypes.byref(v)))
+    return v.value
+
+
+def ref_be_grad(cppg, cpp, weight, init=None):
+    g = np.zeros(3 * cppg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    rcheck(ref().ref_be_gradient(ctypes.byref(rg(cppg)), fp(cpp), ctypes.c_float(weight), fp(g), None))
+    return g
+
+
+def ref_create_cpp(refg, spacing):
+    out = nr_geom()
+    sp = np.asarray(spacing, np.float32)
+    rcheck(ref().ref_create_cpp(ctypes.byref(rg(refg)), fp(sp), ctypes.byref(out), None))
+    n = out.nx * out.ny * out.nz
+    data = np.zeros(3 * n, np.float32)
+    rcheck(ref().ref_create_cpp(ctypes.byref(rg(refg)), fp(sp), ctypes.byref(out), fp(data)))
+    return (out.nx, out.ny, out.nz), np.array(out.sform[:], np.float32).reshape(4, 4), data, tuple(out.pixdim[:])
+
+
+def ref_affine_field(refg, affine, mask=None):
+    field = np.zeros(3 * refg.nvox, np.float32)
+    a = np.ascontiguousarray(affine, np.float32).reshape(16)
+    rcheck(ref().ref_affine_field(ctypes.byref(rg(refg)), fp(a), ip(mask), fp(field)))
+    return field
+
+
+class RefF3d:
+    """reg_f3d<float> prepared at one level for single objective / gradient evaluations."""
+
+    def __init__(self, refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=1,
+                 level=0, maxit=150, padding=float("nan")):
+        p = ref_f3d_params()
+        p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
+        p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
+        self.p = p
+        r = ref()
+        r.ref_f3d_create.argtypes = [ctypes.c_void_p] * 6
+        h = r.ref_f3d_create(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
+                             ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
+                             ctypes.byref(p), ctypes.c_void_p(level))
+        if not h:
+            raise RuntimeError(r.ref_last_error().decode())
+        self.h = ctypes.c_void_p(h)
+        cg, rgm, fgm = nr_geom(), nr_geom(), nr_geom()
+        rcheck(r.ref_f3d_level_geom(self.h, ctypes.byref(cg), ctypes.byref(rgm), ctypes.byref(fgm)))
+        self.cpp_geom, self.ref_geom, self.flo_geom = cg, rgm, fgm
+        self.G = cg.nx * cg.ny * cg.nz
+
+    def images(self):
+        nr_ = self.ref_geom.nx * self.ref_geom.ny * self.ref_geom.nz
+        nf = self.flo_geom.nx * self.flo_geom.ny * self.flo_geom.nz
+        a, b = np.zeros(nr_, np.float32), np.zeros(nf, np.float32)
+        rcheck(ref().ref_f3d_level_images(self.h, fp(a), fp(b)))
+        return a, b
+
+    def weights(self):
+        w = np.zeros(2, np.float32)
+        rcheck(ref().ref_f3d_weights(self.h, fp(w)))
+        return float(w[0]), float(w[1])
+
+    def get_cpp(self):
+        c = np.zeros(3 * self.G, np.float32)
+        rcheck(ref().ref_f3d_get_cpp(self.h, fp(c)))
+        return c
+
+    def set_cpp(self, c):
+        rcheck(ref().ref_f3d_set_cpp(self.h, fp(np.ascontiguousarray(c, np.float32))))
+
+    def value(self):
+        v = ctypes.c_double()
+        t = np.zeros(3, np.float64)
+        rcheck(ref().ref_f3d_value(self.h, ctypes.byref(v), dp(t)))
+        return v.value, t
+
+    def gradient(self, normalise=False):
+        g = np.zeros(3 * self.G, np.float32)
+        m = ctypes.c_double()
+        rcheck(ref().ref_f3d_gradient(self.h, int(normalise), ctypes.byref(m), fp(g), None))
+        return g, m.value
+
+    def close(self):
+        if self.h:
+            ref().ref_f3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def geom_from_nr(g):
+    from niftyreg_amd.geometry import Geom
+    return Geom((g.nx, g.ny, g.nz), np.array(g.sform[:], np.float32).reshape(4, 4), pixdim=tuple(g.pixdim[:]))
