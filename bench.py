@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""F3D objective+gradient evaluations per second on one MI355X.
+
+Workload: the 256 x 256 x 160 float32 shell-phantom pair (1 mm, identity orientation), NMI
+with 64 bins (68 internal), bending energy 0.001, control-point spacing 5 voxels (grid
+55 x 55 x 35), linear interpolation, NaN padding.  One step is one reg_f3d objective value
+plus one objective gradient at the same control points (the unit "obj+grad eval"), run
+through libnrhip's fused sweeps with the images and control points resident in HBM.
+
+Prints one JSON line.  The roofline entry covers the dominant kernel (the gradient sweep),
+timed with HIP events on the stream it runs on; the CPU baseline times the reference CPU
+path (oracle/_ref, OpenMP) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+DIMS = (256, 256, 160)
+BINS = 64
+BE = 0.001
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--storage", choices=["f32", "f16"], default="f32")
+    return p.parse_args()
+
+
+def cpu_baseline(ref, flo, geom):
+    """reg_f3d<float> objective + gradient from the reference CPU build, timed on the host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracles
+    if not oracles.ref_available():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    r = oracles.RefF3d(geom, ref, geom, flo, bins=BINS, be=BE, le=0.0, spacing=-5.0, levels=1)
+    r.value()
+    r.gradient()
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        r.value()
+        r.gradient()
+        n += 1
+        if time.perf_counter() - t0 > 10.0 or n >= 5:
+            break
+    dt = time.perf_counter() - t0
+    r.close()
+    return {"value": n / dt, "unit": "evals/s", "cores": threads, "kind": "reference",
+            "sample": f"{n} obj+grad evals of reg_f3d<float> at 256x256x160 (one level, -be 0.001 -le 0, 64 bins), "
+                      f"OMP_NUM_THREADS={threads}, after one warm-up eval"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): every GPU box here has a single MI355X",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from niftyreg_amd import phantoms
+    from niftyreg_amd.f3d import F3dContent, default_params
+    from niftyreg_amd.geometry import Geom, create_control_point_grid
+    from niftyreg_amd import _lib
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    _lib.call("nrh_init", 0)
+
+    geom = Geom.identity(DIMS)
+    ref, flo = phantoms.config1_pair(seed=0, dims=DIMS)
+    cppg, cpp = create_control_point_grid(geom, [5.0, 5.0, 5.0])
+    rng = np.random.default_rng(7)
+    cpp = (cpp + rng.uniform(-2.0, 2.0, cpp.size)).astype(np.float32)
+    storage = 1 if a.storage == "f16" else 0
+    h = F3dContent(geom, ref, geom, flo, cppg, cpp,
+                   params=default_params(bins=BINS, similarity_weight=1.0 - BE, be_weight=BE, image_storage=storage))
+    h.rescale_intensities((2, BINS + 1), (2, BINS + 1))
+    assert h.fused_available()
+
+    for _ in range(a.warmup):
+        h.objective()
+        h.gradient()
+    h.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        h.objective()
+        h.gradient()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms, counts = h.sweep_times()
+    value, terms = h.objective()
+    h.set_profiling(False)
+
+    N = geom.nvox
+    img_bytes = 2 if storage else 4
+    algo_bytes = 2 * img_bytes * N  # reference + floating read once per sweep
+    grad_ms = ms[1] / max(counts[1], 1)
+    hist_ms = ms[0] / max(counts[0], 1)
+    achieved = algo_bytes / (grad_ms * 1e-3) / 1e9
+    out = {
+        "metric": "F3D obj+grad evals/s at 256x256x160",
+        "value": a.steps / dt,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if not storage else "f32 compute, f16 image storage",
+        "data": "synthetic",
+        "config": {
+            "workload": "F3D NMI objective+gradient eval, 256x256x160 float32 shell phantom pair, "
+                        "64 bins, BE 0.001, LE 0, 5-voxel control grid 55x55x35, linear interpolation",
+            "voxels": N, "control_points": cppg.nvox, "final_objective": value, "nmi": terms[0] / (1.0 - BE),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "k_fused_pass<float,true> (gradient sweep)",
+            "kernel_ms": grad_ms, "histogram_sweep_ms": hist_ms, "algorithmic_bytes_per_launch": algo_bytes,
+        },
+        "cpu_baseline": None,
+    }
+    h.close()
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(ref, flo, geom)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

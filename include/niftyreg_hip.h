@@ -56,6 +56,15 @@ int nrh_mat44_inverse(const float in[16], float out[16]);
  * the identity control-point positions (planar, 3*G floats; may be NULL to query the size). */
 int nrh_create_control_point_grid(const nrh_geom *ref, const float spacing_mm[3], nrh_geom *grid,
                                   float *identity);
+/* reg_spline_refineControlPointGrid (reg-lib/cpu/_reg_localTrans.cpp:1809-2190) with the level's
+ * reference image: half the spacing, cubic B-spline subdivision of the positions.  Call with
+ * out_data NULL to obtain the refined geometry first. */
+int nrh_refine_control_point_grid(const nrh_geom *ref, const nrh_geom *grid, const float *data, nrh_geom *out,
+                                  float *out_data);
+/* reg_downsampleImage<float>(image, 1, axes) (reg-lib/cpu/_reg_tools.cpp:1343-1467): Gaussian
+ * smoothing (0.7355 voxel) and nearest-neighbour decimation by 2 along the axes flagged in
+ * axes[0..2].  Call with out_data NULL to obtain the output geometry first. */
+int nrh_downsample_image(const nrh_geom *in, const float *data, const int axes[3], nrh_geom *out, float *out_data);
 
 /* ------------------------------------------------------------------ F3D */
 typedef struct nrh_f3d nrh_f3d;
@@ -123,6 +132,19 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]);
  * max_length is non-NULL the gradient is then normalised as reg_f3d::NormaliseGradient
  * (reg-lib/_reg_f3d.cpp:395-411) and the maximal length returned. */
 int nrh_f3d_gradient(nrh_f3d *h, double *max_length);
+/* 1 when the handle's geometry takes the fused sweeps (integer grid spacing >= 4 voxels with
+ * nodes on voxels), 0 when objective / gradient run the per-operation chain. */
+int nrh_f3d_fused_available(nrh_f3d *h, int *available);
+/* The fused objective sweep with its deformation field also written to the handle's field
+ * buffer (read with nrh_f3d_download_field): checks the separable field evaluation against
+ * reg_spline_getDeformationField (reg-lib/cpu/_reg_localTrans.cpp:1457-1538). */
+int nrh_f3d_fused_field(nrh_f3d *h);
+/* HIP-event timing of the two fused sweeps on the handle's stream (no reference counterpart;
+ * the reference times whole calls with steady_clock, reg-apps/reg_benchmark.cpp:139-163).
+ * Enabling resets the totals; nrh_f3d_sweep_times returns the accumulated milliseconds and
+ * launch counts of [histogram sweep, gradient sweep]. */
+int nrh_f3d_set_profiling(nrh_f3d *h, int enable);
+int nrh_f3d_sweep_times(nrh_f3d *h, double ms[2], long counts[2]);
 
 /* Device-side optimiser state (replaces CudaOptimiser / CudaConjugateGradient,
  * reg-lib/cuda/CudaOptimiser.cu:72-86,198-304): best DOF copy, PR conjugate gradient update

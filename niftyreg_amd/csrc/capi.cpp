@@ -211,6 +211,8 @@ int nrh_f3d_destroy(nrh_f3d *h) {
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         if (h->pinned) (void)hipHostFree(h->pinned);
         fused_release(h);
+        for (int i = 0; i < 4; ++i)
+            if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
         hipStream_t s = h->stream;
         delete h;
         if (s) (void)hipStreamDestroy(s);

@@ -33,6 +33,7 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
         NRH_HIP(hipMemcpyAsync(h->pinned, h->ent.p, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         NRH_HIP(hipMemcpyAsync(h->pinned + 5, h->scalars.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         NRH_HIP(hipStreamSynchronize(h->stream));
+        collect_sweep_times(h);
         const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * h->pinned[4] : 0.0;
         const double wBE = p.be_weight > 0 ? (double)p.be_weight * h->pinned[5] : 0.0;
         *value = wMeasure - wBE;
@@ -77,6 +78,47 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
         } else {
             NRH_HIP(hipStreamSynchronize(h->stream));
         }
+        collect_sweep_times(h);
+    });
+}
+
+int nrh_f3d_set_profiling(nrh_f3d *h, int enable) {
+    return guard([&] {
+        NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
+        if (enable && !h->ev[0])
+            for (int i = 0; i < 4; ++i) NRH_HIP(hipEventCreate(&h->ev[i]));
+        h->profile = enable != 0;
+        h->msSweep[0] = h->msSweep[1] = 0.0;
+        h->nSweep[0] = h->nSweep[1] = 0;
+        h->evPending[0] = h->evPending[1] = false;
+    });
+}
+
+int nrh_f3d_sweep_times(nrh_f3d *h, double ms[2], long counts[2]) {
+    return guard([&] {
+        NRH_REQUIRE(h && ms && counts, NRH_ERR_ARG, "null argument");
+        for (int k = 0; k < 2; ++k) {
+            ms[k] = h->msSweep[k];
+            counts[k] = h->nSweep[k];
+        }
+    });
+}
+
+int nrh_f3d_fused_field(nrh_f3d *h) {
+    return guard([&] {
+        NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
+        NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry");
+        ensure_field(h);
+        fused_histogram(h, h->field.p);
+        NRH_HIP(hipStreamSynchronize(h->stream));
+        collect_sweep_times(h);
+    });
+}
+
+int nrh_f3d_fused_available(nrh_f3d *h, int *available) {
+    return guard([&] {
+        NRH_REQUIRE(h && available, NRH_ERR_ARG, "null argument");
+        *available = fused_supported(h) ? 1 : 0;
     });
 }
 

@@ -1,0 +1,400 @@
+// Host side of a full F3D registration: image pyramid, control-point grid creation and
+// refinement, and the conjugate-gradient / line-search loop of reg_base<T>::Run
+// (reg-lib/_reg_base.cpp:727-819, reg-lib/Optimiser.cpp:129-317, reg-lib/_reg_f3d.cpp:92-159).
+// Every objective and gradient evaluation runs on the device through the F3D handle.
+
+#include "f3d_handle.hpp"
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <memory>
+
+namespace nrh {
+
+void mat44_inverse(const float *R, float *Q);
+void create_grid(const nrh_geom& ref, const float spacing[3], nrh_geom& grid, float *identity);
+
+namespace {
+
+// Gaussian smoothing of reg_tools_kernelConvolution (reg-lib/cpu/_reg_tools.cpp:832-1112) with all
+// three axes in one call: intensity and density are convolved axis after axis and divided once.
+void gaussian_smooth(const nrh_geom& g, std::vector<float>& img, float sigmaVox) {
+    const int dims[3] = { g.nx, g.ny, g.nz };
+    const size_t N = voxels(g);
+    std::vector<float> dens(N);
+    std::vector<uint8_t> isnan_(N);
+    for (size_t i = 0; i < N; ++i) {
+        const bool ok = img[i] == img[i];
+        dens[i] = ok ? 1.f : 0.f;
+        isnan_[i] = !ok;
+        if (!ok) img[i] = 0.f;
+    }
+    const double temp = std::fabs((double)sigmaVox);
+    const int radius = (int)(temp * 3.0f);
+    std::vector<float> kern(2 * radius + 1);
+    for (int i = -radius; i <= radius; ++i)
+        kern[i + radius] = (float)(std::exp(-(double)(i * i) / (2.0 * temp * temp)) / (temp * 2.506628274631));
+    std::vector<float> bi, bd;
+    for (int n = 0; n < 3; ++n) {
+        if (dims[n] <= 1 || radius <= 0) continue;
+        const int L = dims[n];
+        const size_t stride = n == 0 ? 1 : n == 1 ? (size_t)dims[0] : (size_t)dims[0] * dims[1];
+        const size_t planes = N / L;
+        bi.resize(L);
+        bd.resize(L);
+        for (size_t p = 0; p < planes; ++p) {
+            size_t start;
+            if (n == 0) start = p * dims[0];
+            else if (n == 1) start = (p / dims[0]) * (size_t)dims[0] * dims[1] + p % dims[0];
+            else start = p;
+            for (int l = 0; l < L; ++l) {
+                bi[l] = img[start + l * stride];
+                bd[l] = dens[start + l * stride];
+            }
+            for (int l = 0; l < L; ++l) {
+                int pre = l - radius, pst = l + radius + 1, k0 = 0;
+                if (pre < 0) { k0 = -pre; pre = 0; }
+                if (pst > L) pst = L;
+                double is = 0, ds = 0;
+                for (int k = pre, kk = k0; k < pst; ++k, ++kk) {
+                    is += (double)(kern[kk] * bi[k]);
+                    ds += (double)(kern[kk] * bd[k]);
+                }
+                img[start + l * stride] = (float)is;
+                dens[start + l * stride] = (float)ds;
+            }
+        }
+    }
+    for (size_t i = 0; i < N; ++i) img[i] = isnan_[i] ? std::numeric_limits<float>::quiet_NaN() : img[i] / dens[i];
+}
+
+}  // namespace
+
+// reg_downsampleImage<float>(image, 1, axes) (reg-lib/cpu/_reg_tools.cpp:1343-1467): Gaussian
+// smoothing (sigma 0.7355 voxel) then nearest-neighbour decimation by 2 along the flagged axes.
+// The new voxel -> mm map doubles the flagged columns and keeps the origin.
+void downsample(const nrh_geom& g, const std::vector<float>& in, const bool axes[3], nrh_geom& out,
+                std::vector<float>& res) {
+    std::vector<float> img(in);
+    gaussian_smooth(g, img, -0.7355f);
+    const int od[3] = { g.nx, g.ny, g.nz };
+    int nd[3];
+    float pd[3] = { g.dx, g.dy, g.dz };
+    for (int i = 0; i < 3; ++i) {
+        nd[i] = od[i];
+        if (od[i] > 1 && axes[i]) nd[i] = (int)std::ceil(od[i] / 2.0);
+        if (axes[i]) pd[i] *= 2.f;
+    }
+    out = g;
+    out.nx = nd[0]; out.ny = nd[1]; out.nz = nd[2];
+    out.dx = pd[0]; out.dy = pd[1]; out.dz = pd[2];
+    for (int c = 0; c < 3; ++c)
+        if (axes[c])
+            for (int r = 0; r < 3; ++r) out.xyz[r * 4 + c] = g.xyz[r * 4 + c] * 2.f;
+    mat44_inverse(out.xyz, out.ijk);
+    const float *m = out.xyz, *r2v = g.ijk;
+    auto round_int = [](float x) { return (int)(x + (x >= 0 ? 0.5f : -0.5f)); };
+    res.assign(voxels(out), 0.f);
+    size_t idx = 0;
+    for (int z = 0; z < nd[2]; ++z)
+        for (int y = 0; y < nd[1]; ++y)
+            for (int x = 0; x < nd[0]; ++x, ++idx) {
+#pragma clang fp contract(off)
+                float real[3];
+                for (int r = 0; r < 3; ++r)
+                    real[r] = (float)x * m[r * 4] + (float)y * m[r * 4 + 1] + (float)z * m[r * 4 + 2] + m[r * 4 + 3];
+                int pos[3];
+                for (int r = 0; r < 3; ++r)
+                    pos[r] = round_int(real[0] * r2v[r * 4] + real[1] * r2v[r * 4 + 1] + real[2] * r2v[r * 4 + 2] + r2v[r * 4 + 3]);
+                if (od[2] == 1) pos[2] = 0;
+                float v = std::numeric_limits<float>::quiet_NaN();
+                if (pos[0] > -1 && pos[0] < od[0] && pos[1] > -1 && pos[1] < od[1] && pos[2] > -1 && pos[2] < od[2])
+                    v = img[((size_t)pos[2] * od[1] + pos[1]) * od[0] + pos[0]];
+                res[idx] = v;
+            }
+}
+
+// reg_spline_refineControlPointGrid3D (reg-lib/cpu/_reg_localTrans.cpp:1809-2120): cubic B-spline
+// subdivision.  Node 2x-1 of the new grid takes (1 6 1)/8 of old nodes x-1, x, x+1 and node 2x
+// takes (1 1)/2 of x, x+1 on each axis; old nodes outside the grid count as 0.  The new header is
+// the control-point grid of the level's reference image at half the spacing.
+void refine_grid(const nrh_geom& ref, const nrh_geom& old, const std::vector<float>& oldData, nrh_geom& grid,
+                 std::vector<float>& data) {
+    const float sp[3] = { old.dx / 2.0f, old.dy / 2.0f, old.dz / 2.0f };
+    create_grid(ref, sp, grid, nullptr);
+    const int on[3] = { old.nx, old.ny, old.nz };
+    const size_t OG = voxels(old), NG = voxels(grid);
+    data.assign(3 * NG, 0.f);
+    auto get = [&](const float *a, int x, int y, int z) -> float {
+        if (x < 0 || x >= on[0] || y < 0 || y >= on[1] || z < 0 || z >= on[2]) return 0.f;
+        return a[((size_t)z * on[1] + y) * on[0] + x];
+    };
+    // per-axis taps: even child (offsets -1, 0, 1; weights 1, 6, 1), odd child (offsets 0, 1; 1, 1)
+    static const int offE[3] = { -1, 0, 1 }, wE[3] = { 1, 6, 1 };
+    static const int offO[2] = { 0, 1 }, wO[2] = { 1, 1 };
+    for (int c = 0; c < 3; ++c) {
+#pragma clang fp contract(off)
+        const float *src = oldData.data() + c * OG;
+        float *dst = data.data() + c * NG;
+        for (int z = 0; z < on[2]; ++z)
+            for (int y = 0; y < on[1]; ++y)
+                for (int x = 0; x < on[0]; ++x)
+                    for (int cz = 0; cz < 2; ++cz)
+                        for (int cy = 0; cy < 2; ++cy)
+                            for (int cx = 0; cx < 2; ++cx) {
+                                const int X = 2 * x - 1 + cx, Y = 2 * y - 1 + cy, Z = 2 * z - 1 + cz;
+                                if (X < 0 || Y < 0 || Z < 0 || X >= grid.nx || Y >= grid.ny || Z >= grid.nz) continue;
+                                const int nx_ = cx ? 2 : 3, ny_ = cy ? 2 : 3, nz_ = cz ? 2 : 3;
+                                const int *ox = cx ? offO : offE, *oy = cy ? offO : offE, *oz = cz ? offO : offE;
+                                const int *wx = cx ? wO : wE, *wy = cy ? wO : wE, *wz = cz ? wO : wE;
+                                // group the taps by weight as the reference writes them (1, 6, 36, 216)
+                                float sums[4] = { 0.f, 0.f, 0.f, 0.f };
+                                for (int k = 0; k < nz_; ++k)
+                                    for (int j = 0; j < ny_; ++j)
+                                        for (int i = 0; i < nx_; ++i) {
+                                            const int w = wx[i] * wy[j] * wz[k];
+                                            const int cls = w == 1 ? 0 : w == 6 ? 1 : w == 36 ? 2 : 3;
+                                            sums[cls] += get(src, x + ox[i], y + oy[j], z + oz[k]);
+                                        }
+                                const float den = (float)((cx ? 2 : 8) * (cy ? 2 : 8) * (cz ? 2 : 8));
+                                dst[((size_t)Z * grid.ny + Y) * grid.nx + X] =
+                                    (sums[0] + 6.0f * sums[1] + 36.0f * sums[2] + 216.0f * sums[3]) / den;
+                            }
+    }
+}
+
+namespace {
+
+struct Level {
+    nrh_geom geom;
+    std::vector<float> data;
+};
+
+// reg_createImagePyramid<float> (reg-lib/cpu/_reg_tools.cpp:1705-1733): the finest level is the
+// input; each coarser level halves the axes whose half-size is still >= 32 voxels.
+std::vector<Level> pyramid(const nrh_geom& g, const float *data, int levels, int toPerform) {
+    std::vector<Level> p(toPerform);
+    Level cur{ g, std::vector<float>(data, data + voxels(g)) };
+    auto axes_of = [](const nrh_geom& q, bool ax[3]) {
+        ax[0] = q.nx / 2 >= 32;
+        ax[1] = q.ny / 2 >= 32;
+        ax[2] = q.nz / 2 >= 32;
+    };
+    for (int l = toPerform; l < levels; ++l) {
+        bool ax[3];
+        axes_of(cur.geom, ax);
+        Level nxt;
+        downsample(cur.geom, cur.data, ax, nxt.geom, nxt.data);
+        cur = std::move(nxt);
+    }
+    p[toPerform - 1] = cur;
+    for (int l = toPerform - 2; l >= 0; --l) {
+        bool ax[3];
+        axes_of(p[l + 1].geom, ax);
+        downsample(p[l + 1].geom, p[l + 1].data, ax, p[l].geom, p[l].data);
+    }
+    return p;
+}
+
+struct Handle {
+    nrh_f3d *h = nullptr;
+    ~Handle() { if (h) nrh_f3d_destroy(h); }
+};
+
+void check(int rc) {
+    if (rc != NRH_OK) throw Error(rc, nrh_last_error());
+}
+
+}  // namespace
+
+void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& floG, const float *floData,
+                   const nrh_f3d_config& cfg, nrh_geom& gridOut, std::vector<float>& cppOut, double& finalNmi,
+                   int& totalIterations) {
+    NRH_REQUIRE(cfg.le_weight == 0.f, NRH_ERR_UNSUPPORTED, "the linear-energy penalty is not implemented (use -le 0)");
+    NRH_REQUIRE(cfg.levels >= 1 && cfg.bins >= 1 && cfg.max_iterations >= 0, NRH_ERR_ARG, "invalid configuration");
+    const int levels = cfg.levels;
+    int toPerform = cfg.levels_to_perform > 0 ? std::min(cfg.levels_to_perform, levels) : levels;
+    if (!cfg.use_pyramid) toPerform = levels;
+    // reg_f3d::CheckParameters: weights of the objective (reg-lib/_reg_f3d.cpp:142-159)
+    float be = cfg.be_weight;
+    const float penalty = be + cfg.le_weight;
+    float simW;
+    if (penalty >= 1) {
+        simW = 0;
+        be /= penalty;
+    } else {
+        simW = 1 - penalty;
+    }
+    // pyramids (or the full-resolution images at every level with -nopy)
+    std::vector<Level> rp, fp;
+    if (cfg.use_pyramid) {
+        rp = pyramid(refG, refData, levels, toPerform);
+        fp = pyramid(floG, floData, levels, toPerform);
+    } else {
+        rp.assign(1, Level{ refG, std::vector<float>(refData, refData + voxels(refG)) });
+        fp.assign(1, Level{ floG, std::vector<float>(floData, floData + voxels(floG)) });
+    }
+    // control-point grid at the coarsest level (reg-lib/_reg_f3d.cpp:161-195)
+    float spmm = cfg.spacing < 0 ? -cfg.spacing * refG.dx : cfg.spacing;
+    float spy = cfg.spacing < 0 ? -cfg.spacing * refG.dy : cfg.spacing;
+    float spz = cfg.spacing < 0 ? -cfg.spacing * refG.dz : cfg.spacing;
+    const float scale = std::pow(2.f, (float)(levels - 1));
+    const float gsp[3] = { spmm * scale, spy * scale, spz * scale };
+    nrh_geom grid;
+    std::vector<float> cpp;
+    {
+        const nrh_geom& g0 = rp[0].geom;
+        create_grid(g0, gsp, grid, nullptr);
+        cpp.resize(3 * voxels(grid));
+        create_grid(g0, gsp, grid, cpp.data());
+    }
+    size_t maxIter = (size_t)cfg.max_iterations * (size_t)std::pow(2.0, toPerform - 1);
+    totalIterations = 0;
+    finalNmi = 0;
+    for (int level = 0; level < toPerform; ++level) {
+        const int idx = cfg.use_pyramid ? level : 0;
+        const nrh_geom& lr = rp[idx].geom;
+        const nrh_geom& lf = fp[idx].geom;
+        float maxStep = std::max(lr.dx, lr.dy);
+        maxStep = std::max(maxStep, lr.dz);
+        // reg_f3d::InitCurrentLevel (reg-lib/_reg_f3d.cpp:92-133)
+        if (level == 0) {
+            be = be / std::pow(16.f, (float)(levels - 1));
+        } else {
+            be = be * 16;
+            nrh_geom ng;
+            std::vector<float> nd;
+            refine_grid(lr, grid, cpp, ng, nd);
+            grid = ng;
+            cpp.swap(nd);
+        }
+        nrh_f3d_params p{};
+        p.ref_bins = cfg.bins + 4;
+        p.flo_bins = cfg.bins + 4;
+        p.padding = cfg.padding;
+        p.similarity_weight = simW;
+        p.be_weight = be;
+        p.le_weight = 0.f;
+        p.image_storage = cfg.image_storage;
+        Handle H;
+        check(nrh_f3d_create(&lr, rp[idx].data.data(), &lf, fp[idx].data.data(), nullptr, &grid, &p, &H.h));
+        nrh_f3d *h = H.h;
+        check(nrh_f3d_set_control_points(h, cpp.data()));
+        // reg_nmi::InitialiseMeasure: rescale to [2, bins - 3] (reg-lib/cpu/_reg_nmi.cpp:115-129)
+        check(nrh_f3d_rescale_intensities(h, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
+        // Optimiser::Initialise (reg-lib/Optimiser.cpp:48-90)
+        check(nrh_f3d_store_best(h));
+        double best = 0, terms[3];
+        check(nrh_f3d_objective(h, &best, terms));
+        size_t iter = 0;
+        const float smallest = maxStep / 100.f;
+        float currentSize = maxStep;
+        bool firstCall = true, justRestarted = false;
+        while (currentSize) {
+            if (iter >= maxIter) break;
+            double maxLen = 0;
+            check(nrh_f3d_gradient(h, &maxLen));  // GetObjectiveFunctionGradient + NormaliseGradient
+            ++iter;
+            currentSize = std::min(currentSize, maxStep);
+            check(nrh_f3d_conjugate_gradient(h, firstCall ? 1 : 0));
+            firstCall = false;
+            // Optimiser::Optimise line search (reg-lib/Optimiser.cpp:129-173)
+            size_t lineIter = 0;
+            float added = 0;
+            float curLen = currentSize;
+            while (curLen > smallest && lineIter < 12 && iter < maxIter) {
+                check(nrh_f3d_update_control_points(h, -curLen));
+                double value = 0;
+                check(nrh_f3d_objective(h, &value, terms));
+                if (value > best) {
+                    best = value;
+                    added += curLen;
+                    curLen *= 1.1f;
+                    curLen = std::min(curLen, maxStep);
+                    check(nrh_f3d_store_best(h));
+                } else {
+                    curLen *= 0.5;
+                }
+                ++iter;
+                ++lineIter;
+            }
+            currentSize = added;
+            check(nrh_f3d_restore_best(h));
+            // restart to steepest descent once when the line search stalls (reg-lib/_reg_base.cpp:781-793)
+            if (currentSize == 0) {
+                if (!justRestarted && iter < maxIter) {
+                    firstCall = true;
+                    currentSize = maxStep;
+                    justRestarted = true;
+                }
+            } else {
+                justRestarted = false;
+            }
+        }
+        totalIterations += (int)iter;
+        check(nrh_f3d_get_control_points(h, cpp.data()));
+        double value = 0;
+        check(nrh_f3d_objective(h, &value, terms));
+        finalNmi = simW > 0 ? terms[0] / simW : 0.0;
+        maxIter /= 2;
+    }
+    gridOut = grid;
+    cppOut = cpp;
+}
+
+}  // namespace nrh
+
+using namespace nrh;
+
+extern "C" {
+
+int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
+                     const nrh_f3d_config *cfg, nrh_geom *grid_out, float *cpp_out, size_t cpp_capacity,
+                     double *final_nmi, int *iterations) {
+    return guard([&] {
+        check_geom(ref, "reference");
+        check_geom(flo, "floating");
+        NRH_REQUIRE(ref_data && flo_data && cfg && grid_out, NRH_ERR_ARG, "null argument");
+        nrh_geom g;
+        std::vector<float> cpp;
+        double nmi = 0;
+        int it = 0;
+        register_pair(*ref, ref_data, *flo, flo_data, *cfg, g, cpp, nmi, it);
+        *grid_out = g;
+        if (cpp_out) {
+            NRH_REQUIRE(cpp.size() <= cpp_capacity, NRH_ERR_ARG, "cpp_out too small for the final grid");
+            std::memcpy(cpp_out, cpp.data(), cpp.size() * sizeof(float));
+        }
+        if (final_nmi) *final_nmi = nmi;
+        if (iterations) *iterations = it;
+    });
+}
+
+int nrh_downsample_image(const nrh_geom *in, const float *data, const int axes[3], nrh_geom *out, float *out_data) {
+    return guard([&] {
+        check_geom(in, "input");
+        NRH_REQUIRE(data && axes && out, NRH_ERR_ARG, "null argument");
+        const bool ax[3] = { axes[0] != 0, axes[1] != 0, axes[2] != 0 };
+        std::vector<float> src(data, data + voxels(*in)), res;
+        nrh_geom g;
+        downsample(*in, src, ax, g, res);
+        *out = g;
+        if (out_data) std::memcpy(out_data, res.data(), res.size() * sizeof(float));
+    });
+}
+
+int nrh_refine_control_point_grid(const nrh_geom *ref, const nrh_geom *grid, const float *data, nrh_geom *out,
+                                  float *out_data) {
+    return guard([&] {
+        check_geom(ref, "reference");
+        check_geom(grid, "grid");
+        NRH_REQUIRE(data && out, NRH_ERR_ARG, "null argument");
+        std::vector<float> src(data, data + 3 * voxels(*grid)), res;
+        nrh_geom g;
+        refine_grid(*ref, *grid, src, g, res);
+        *out = g;
+        if (out_data) std::memcpy(out_data, res.data(), res.size() * sizeof(float));
+    });
+}
+
+}  // extern "C"

@@ -52,10 +52,12 @@ struct FusedArgs {
     const int *preX, *preY, *preZ;
     const float4 *bX, *bY, *bZ;
     int tilesX, tilesY, tilesZ, numTiles;
+    Affine cppXyz;  // grid voxel -> mm
     uint32_t *hist;
     const float *logTab;
     const double *ent;
     float *partials;
+    float *fieldOut;  // optional: the deformation field of the sweep (planar [3][N]), for checking
 };
 
 template<class T> __device__ __forceinline__ float ld(const T *p, size_t i);
@@ -123,7 +125,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
     // ---- control-point window
     for (int i = tid; i < 3 * WZ * WY * WX; i += kThreads) {
         const int ix = i % WX, iy = (i / WX) % WY, iz = (i / (WX * WY)) % WZ, c = i / (WX * WY * WZ);
-        sCp[c][iz][iy][ix] = a.cpp[c * a.G + ((size_t)(pz0 + iz) * a.gny + (py0 + iy)) * a.gnx + (px0 + ix)];
+        const double node = (double)a.cppXyz.m[c * 4] * (px0 + ix) + (double)a.cppXyz.m[c * 4 + 1] * (py0 + iy) +
+                            (double)a.cppXyz.m[c * 4 + 2] * (pz0 + iz) + (double)a.cppXyz.m[c * 4 + 3];
+        sCp[c][iz][iy][ix] =
+            (float)((double)a.cpp[c * a.G + ((size_t)(pz0 + iz) * a.gny + (py0 + iy)) * a.gnx + (px0 + ix)] - node);
     }
     __syncthreads();
     if (GRAD && tid < WX) {
@@ -180,9 +185,26 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
                 if (active) {
                     const float4 b = sBx[x];
                     const int o = sPx[x] - px0;
+                    // field = identity + displacement: the B-spline sum of the node positions is
+                    // the affine map of (pre + t + 1) on each axis (cubic B-splines reproduce
+                    // linear functions), evaluated in double; the window holds displacements.
+                    const float4 by = sBy[y], bz = sBz[z];
+                    const double ux = (double)sPx[x] + (double)(b.y + 2.f * b.z + 3.f * b.w);
+                    const double uy = (double)sPy[y] + (double)(by.y + 2.f * by.z + 3.f * by.w);
+                    const double uz = (double)sPz[z] + (double)(bz.y + 2.f * bz.z + 3.f * bz.w);
                     float w[3];
-                    for (int c = 0; c < 3; ++c)
-                        w[c] = b.x * sAyz[y][o][c] + b.y * sAyz[y][o + 1][c] + b.z * sAyz[y][o + 2][c] + b.w * sAyz[y][o + 3][c];
+                    for (int c = 0; c < 3; ++c) {
+                        const float d = b.x * sAyz[y][o][c] + b.y * sAyz[y][o + 1][c] + b.z * sAyz[y][o + 2][c] +
+                                        b.w * sAyz[y][o + 3][c];
+                        const double id = (double)a.cppXyz.m[c * 4] * ux + (double)a.cppXyz.m[c * 4 + 1] * uy +
+                                          (double)a.cppXyz.m[c * 4 + 2] * uz + (double)a.cppXyz.m[c * 4 + 3];
+                        w[c] = (float)(id + (double)d);
+                    }
+                    if (a.fieldOut) {
+                        a.fieldOut[idx] = w[0];
+                        a.fieldOut[idx + (size_t)a.nx * a.ny * a.nz] = w[1];
+                        a.fieldOut[idx + 2 * (size_t)a.nx * a.ny * a.nz] = w[2];
+                    }
                     const float3 p = affine_apply_f(a.floIjk, w[0], w[1], w[2]);
                     const int fx = floor_int(p.x), fy = floor_int(p.y), fz = floor_int(p.z);
                     float rx = p.x - (float)fx, ry = p.y - (float)fy, rz = p.z - (float)fz;
@@ -526,6 +548,17 @@ static FusedPlan *plan_of(nrh_f3d *h) {
 
 bool fused_supported(nrh_f3d *h) { return plan_of(h)->ok; }
 
+void collect_sweep_times(nrh_f3d *h) {
+    for (int k = 0; k < 2; ++k) {
+        if (!h->evPending[k]) continue;
+        float ms = 0.f;
+        NRH_HIP(hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]));
+        h->msSweep[k] += ms;
+        h->nSweep[k] += 1;
+        h->evPending[k] = false;
+    }
+}
+
 static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
     FusedArgs a{};
     const bool half = h->params.image_storage == 1;
@@ -547,6 +580,7 @@ static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
     a.nx = h->ref.nx; a.ny = h->ref.ny; a.nz = h->ref.nz;
     a.fnx = h->flo.nx; a.fny = h->flo.ny; a.fnz = h->flo.nz;
     a.floIjk = to_affine(h->flo.ijk);
+    a.cppXyz = to_affine(h->cpp.xyz);
     a.pad = h->params.padding;
     a.rb = h->params.ref_bins;
     a.fb = h->params.flo_bins;
@@ -562,18 +596,24 @@ static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
 
 static unsigned grid_for(int numTiles) { return (unsigned)((numTiles + 7) / 8 * 8); }
 
-void fused_histogram(nrh_f3d *h) {
+void fused_histogram(nrh_f3d *h, float *fieldOut) {
     FusedPlan *p = plan_of(h);
     NRH_REQUIRE(p->ok, NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry");
     FusedArgs a = make_args(h, p);
     a.hist = h->histCount.p;
+    a.fieldOut = fieldOut;
     op_fill_u32(h->histCount.p, (size_t)a.rb * a.fb, 0u, h->stream);
     const size_t lds = (size_t)a.rb * a.fb * sizeof(uint32_t);
+    if (h->profile) NRH_HIP(hipEventRecord(h->ev[0], h->stream));
     if (h->params.image_storage == 1)
         hipLaunchKernelGGL((k_fused_pass<__half, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     else
         hipLaunchKernelGGL((k_fused_pass<float, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     NRH_HIP(hipGetLastError());
+    if (h->profile) {
+        NRH_HIP(hipEventRecord(h->ev[1], h->stream));
+        h->evPending[0] = true;
+    }
     op_nmi_finalize(h);
     h->histVersion = h->cppVersion;
 }
@@ -587,11 +627,16 @@ void fused_gradient(nrh_f3d *h, float weight) {
     h->nodePartials.alloc((size_t)p->numTiles * 3 * WIN);
     a.partials = h->nodePartials.p;
     const size_t lds = (size_t)h->totalBins() * sizeof(float);
+    if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
     if (h->params.image_storage == 1)
         hipLaunchKernelGGL((k_fused_pass<__half, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     else
         hipLaunchKernelGGL((k_fused_pass<float, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     NRH_HIP(hipGetLastError());
+    if (h->profile) {
+        NRH_HIP(hipEventRecord(h->ev[3], h->stream));
+        h->evPending[1] = true;
+    }
     ReduceArgs r{};
     r.partials = h->nodePartials.p;
     r.gnx = h->cpp.nx; r.gny = h->cpp.ny; r.gnz = h->cpp.nz;

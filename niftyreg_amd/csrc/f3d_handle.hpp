@@ -45,6 +45,13 @@ struct nrh_f3d {
     uint64_t histVersion = 0;  // cppVersion the histogram was computed at
     bool halfDirty = true;     // float16 copies of the images need refreshing
 
+    // optional HIP-event timing of the fused sweeps (on this handle's stream)
+    bool profile = false;
+    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
+    bool evPending[2] = { false, false };
+    double msSweep[2] = { 0.0, 0.0 };  // accumulated: histogram sweep, gradient sweep
+    long nSweep[2] = { 0, 0 };
+
     int totalBins() const { return params.ref_bins * params.flo_bins + params.ref_bins + params.flo_bins; }
 };
 
@@ -82,7 +89,8 @@ void op_normalise_device(nrh_f3d *h);
 // fused evaluation (f3d_fused.hip)
 bool fused_supported(nrh_f3d *h);
 void fused_release(const nrh_f3d *h);
-void fused_histogram(nrh_f3d *h);
+void fused_histogram(nrh_f3d *h, float *fieldOut = nullptr);
 void fused_gradient(nrh_f3d *h, float weight);
+void collect_sweep_times(nrh_f3d *h);  // after a stream synchronisation
 
 }  // namespace nrh

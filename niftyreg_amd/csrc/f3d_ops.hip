@@ -684,6 +684,14 @@ void op_spline_field(nrh_f3d *h) {
     ensure_field(h);
     float g[3];
     const int lut = lut_rule(h, g) ? 1 : 0;
+    // every voxel's 4x4x4 support must lie inside the grid (the reference would extrapolate
+    // through get_SlidedValues for a grid that does not cover the image)
+    const int dims[3] = { h->ref.nx, h->ref.ny, h->ref.nz }, gd[3] = { h->cpp.nx, h->cpp.ny, h->cpp.nz };
+    for (int i = 0; i < 3; ++i) {
+        const int c = dims[i] - 1;
+        const int pre = lut ? c / 5 : (int)((float)c / g[i]);
+        NRH_REQUIRE(pre + 3 < gd[i], NRH_ERR_UNSUPPORTED, "control point grid does not cover the reference image");
+    }
     hipLaunchKernelGGL(k_spline_field, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, h->ref.nx, h->ref.ny,
                        h->ref.nz, h->cpp.nx, h->cpp.ny, h->G, h->cppPos.p, h->mask.p, g[0], g[1], g[2], lut,
                        h->field.p, h->N);

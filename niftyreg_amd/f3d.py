@@ -166,6 +166,24 @@ class F3dContent:
         return pro, logh
 
     # --- fused evaluation
+    def fused_available(self):
+        v = ctypes.c_int()
+        self._op("nrh_f3d_fused_available", ctypes.byref(v))
+        return bool(v.value)
+
+    def set_profiling(self, enable=True):
+        self._op("nrh_f3d_set_profiling", ctypes.c_int(1 if enable else 0))
+
+    def sweep_times(self):
+        ms = np.zeros(2, np.float64)
+        cnt = (ctypes.c_long * 2)()
+        self._op("nrh_f3d_sweep_times", _lib.dptr(ms), cnt)
+        return ms, np.array(cnt[:])
+
+    def fused_field(self):
+        self._op("nrh_f3d_fused_field")
+        return self.deformation_field()
+
     def objective(self):
         v = ctypes.c_double()
         terms = np.zeros(3, np.float64)
@@ -192,3 +210,28 @@ class F3dContent:
 
     def update_control_points(self, scale):
         self._op("nrh_f3d_update_control_points", ctypes.c_float(scale))
+
+
+def register(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3,
+             levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True, image_storage=0):
+    """Full reg_f3d registration of one pair on the device (reg_base<float>::Run).
+
+    Returns (grid Geom, control points planar float32, final NMI, iterations)."""
+    cfg = _lib.nrh_f3d_config()
+    cfg.bins, cfg.be_weight, cfg.le_weight, cfg.spacing = bins, be, le, spacing
+    cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
+    cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), image_storage
+    ref = np.ascontiguousarray(ref, np.float32).reshape(-1)
+    flo = np.ascontiguousarray(flo, np.float32).reshape(-1)
+    rg, fg = ref_geom.to_c(), flo_geom.to_c()
+    out = _lib.nrh_geom()
+    # the final grid is at most the grid of the full-resolution image at the final spacing
+    cap = 3 * int(np.prod([int(np.ceil(n * d / (abs(spacing) * d if spacing < 0 else spacing) + 4))
+                           for n, d in zip(ref_geom.dims, ref_geom.pixdim)]))
+    cpp = np.zeros(cap, np.float32)
+    nmi = ctypes.c_double()
+    its = ctypes.c_int()
+    _lib.call("nrh_f3d_register", ctypes.byref(rg), _lib.fptr(ref), ctypes.byref(fg), _lib.fptr(flo), ctypes.byref(cfg),
+              ctypes.byref(out), _lib.fptr(cpp), ctypes.c_size_t(cap), ctypes.byref(nmi), ctypes.byref(its))
+    g = Geom.from_c(out)
+    return g, cpp[:3 * g.nvox].copy(), nmi.value, its.value

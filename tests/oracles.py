@@ -347,3 +347,58 @@ class RefF3d:
 def geom_from_nr(g):
     from niftyreg_amd.geometry import Geom
     return Geom((g.nx, g.ny, g.nz), np.array(g.sform[:], np.float32).reshape(4, 4), pixdim=tuple(g.pixdim[:]))
+
+
+# ---------------------------------------------------------------- FP64 references (numpy)
+def _axis_weights_fp64(n, gvs, nodes):
+    """Dense (n x nodes) cubic B-spline weights in float64 at the exact cell positions."""
+    W = np.zeros((n, nodes), np.float64)
+    for v in range(n):
+        pre = int(np.floor(v / gvs))
+        t = v / gvs - pre
+        b = ((1 - t) ** 3 / 6, (3 * t ** 3 - 6 * t ** 2 + 4) / 6, (-3 * t ** 3 + 3 * t ** 2 + 3 * t + 1) / 6, t ** 3 / 6)
+        for a in range(4):
+            W[v, pre + a] = b[a]
+    return W
+
+
+def fp64_field(refg, cppg, cpp):
+    """The cubic B-spline deformation field in float64 (separable dense contraction)."""
+    nx, ny, nz = refg.dims
+    gx, gy, gz = cppg.dims
+    gvs = [cppg.pixdim[i] / refg.pixdim[i] for i in range(3)]
+    Wx, Wy, Wz = (_axis_weights_fp64(n, g, m) for n, g, m in zip((nx, ny, nz), gvs, (gx, gy, gz)))
+    c = np.asarray(cpp, np.float64).reshape(3, gz, gy, gx)
+    out = np.empty((3, nz, ny, nx), np.float64)
+    for k in range(3):
+        t = np.einsum("kji,xi->kjx", c[k], Wx)
+        t = np.einsum("kjx,yj->kyx", t, Wy)
+        out[k] = np.einsum("kyx,zk->zyx", t, Wz)
+    return out.reshape(-1)
+
+
+def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3, maxit=150,
+                padding=float("nan")):
+    """reg_f3d<float>::Run on the reference CPU build; returns (grid geom, control points)."""
+    p = ref_f3d_params()
+    p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
+    p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
+    out = nr_geom()
+    sp = [abs(spacing) * d if spacing < 0 else spacing for d in refg.pixdim]
+    cap = 3 * int(np.prod([int(np.ceil(n * d / s + 4)) for n, d, s in zip(refg.dims, refg.pixdim, sp)]))
+    buf = np.zeros(cap, np.float32)
+    rcheck(ref().ref_f3d_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
+                             ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
+                             ctypes.byref(p), ctypes.byref(out), fp(buf), cap))
+    n = 3 * out.nx * out.ny * out.nz
+    return geom_from_nr(out), buf[:n].copy()
+
+
+def nmi_at(refg, flog, cppg, cpp, ref_img, flo_img, bins=64):
+    """NMI of the pair warped through a control-point grid, on the oracle (images rescaled here)."""
+    rb = bins + 4
+    r = orc_rescale(np.asarray(ref_img, np.float32).ravel(), 2, rb - 3)
+    f = orc_rescale(np.asarray(flo_img, np.float32).ravel(), 2, rb - 3)
+    field = orc_field(refg, cppg, cpp)
+    war = orc_resample(flog, f, refg, field)
+    return orc_nmi(r, war, rb, rb)[0]
