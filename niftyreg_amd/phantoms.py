@@ -138,3 +138,11 @@ def config1_pair(seed=0, dims=(256, 256, 160)):
     ref = shell_phantom(dims, seed)
     flo = bspline_warp(ref, 5, 4.0, seed + 1000)
     return ref, flo
+
+
+def config3_pair(seed=0, dims=(128, 128, 128)):
+    """Blob phantom and an affinely warped copy (rotations within 15 degrees per axis,
+    translations within 10 mm, scales 0.9-1.1, shears within 0.05).  Returns ref, flo, A."""
+    ref = blob_phantom(dims, seed)
+    A = random_affine(seed + 2000)
+    return ref, affine_warp(ref, A), A

@@ -788,3 +788,176 @@ void orc_affine_field(const orc_geom *ref, const float affine[16], const int *ma
                 field[idx + 2 * n] = (float)p[2];
             }
 }
+
+/* ------------------------------------------------------------------ Aladin: active blocks */
+/* Numerical-Recipes heap sort of values with a companion index array, ascending
+ * (reg-lib/cpu/Maths.cpp:114-153).  Tie order is whatever this sift-down sequence produces,
+ * so it is restated step for step (1-based heap over v[0..n-1]). */
+static void heap_sort_indexed(float *v, int *idx, int n) {
+    float *a = v - 1;
+    int *ix = idx - 1;
+    int l = (n >> 1) + 1, ir = n;
+    if (n < 2) return;
+    for (;;) {
+        float val;
+        int iv;
+        if (l > 1) {
+            --l;
+            val = a[l];
+            iv = ix[l];
+        } else {
+            val = a[ir];
+            iv = ix[ir];
+            a[ir] = a[1];
+            ix[ir] = ix[1];
+            if (--ir == 1) {
+                a[1] = val;
+                ix[1] = iv;
+                return;
+            }
+        }
+        int i = l, j = l + l;
+        while (j <= ir) {
+            if (j < ir && a[j] < a[j + 1]) ++j;
+            if (val < a[j]) {
+                a[i] = a[j];
+                ix[i] = ix[j];
+                i = j;
+                j <<= 1;
+            } else break;
+        }
+        a[i] = val;
+        ix[i] = iv;
+    }
+}
+
+int orc_block_count(const orc_geom *ref, int nb[3]) {
+    nb[0] = (ref->nx + 3) / 4;
+    nb[1] = (ref->ny + 3) / 4;
+    nb[2] = (ref->nz + 3) / 4;
+    return nb[0] * nb[1] * nb[2];
+}
+
+/* Variance of one 4x4x4 block (reg-lib/cpu/_reg_blockMatching.cpp:70-128): the mean runs over
+ * finite voxels with mask > -1, the variance over every finite voxel of the block, both as
+ * float sums in z, y, x order.  Returns -1 for an unusable block. */
+float orc_block_variance(const orc_geom *ref, const float *img, const int *mask, int bi, int bj, int bk) {
+    float mean = 0.f, count = 0.f, var = 0.f;
+    for (int z = 4 * bk; z < 4 * bk + 4 && z < ref->nz; ++z)
+        for (int y = 4 * bj; y < 4 * bj + 4 && y < ref->ny; ++y)
+            for (int x = 4 * bi; x < 4 * bi + 4 && x < ref->nx; ++x) {
+                size_t i = ((size_t)z * ref->ny + y) * ref->nx + x;
+                float v = img[i];
+                if (v == v && (!mask || mask[i] > -1)) {
+                    mean += v;
+                    count += 1.f;
+                }
+            }
+    mean /= count;
+    for (int z = 4 * bk; z < 4 * bk + 4 && z < ref->nz; ++z)
+        for (int y = 4 * bj; y < 4 * bj + 4 && y < ref->ny; ++y)
+            for (int x = 4 * bi; x < 4 * bi + 4 && x < ref->nx; ++x) {
+                float v = img[((size_t)z * ref->ny + y) * ref->nx + x];
+                if (v == v) var += (mean - v) * (mean - v);
+            }
+    var /= count;
+    return (count > 32.f && var > 0.f) ? var : -1.f;
+}
+
+/* initialise_block_matching_method + _reg_set_active_blocks<float>
+ * (reg-lib/cpu/_reg_blockMatching.cpp:213-273, 49-206).  totalBlock[b] = rank of block b among the
+ * kept blocks (0 = largest variance) or -1.  Returns the active block number. */
+int orc_active_blocks(const orc_geom *ref, const float *img, const int *mask, int percent, int *totalBlock) {
+    int nb[3];
+    const int total = orc_block_count(ref, nb);
+    float *var = (float *)malloc(sizeof(float) * total);
+    int *idx = (int *)malloc(sizeof(int) * total);
+    int unusable = 0, b = 0;
+    for (int k = 0; k < nb[2]; ++k)
+        for (int j = 0; j < nb[1]; ++j)
+            for (int i = 0; i < nb[0]; ++i, ++b) {
+                var[b] = orc_block_variance(ref, img, mask, i, j, k);
+                if (var[b] < 0.f) ++unusable;
+                idx[b] = b;
+            }
+    int active = (int)((double)total * ((double)percent / 100.0));
+    if (active > total - unusable) active = total - unusable;
+    heap_sort_indexed(var, idx, total);
+    for (int r = 0; r < total; ++r) totalBlock[idx[total - 1 - r]] = r < active ? r : -1;
+    free(var);
+    free(idx);
+    return active;
+}
+
+/* block_matching_method3D<float> (reg-lib/cpu/_reg_blockMatching.cpp:455-690) with a capture
+ * range of 3 voxels and step 1.  Positions are written at 3*rank; a block without any offset of
+ * more than 32 overlapping voxels gets a NaN warped position.  Returns the defined count. */
+int orc_block_matching(const orc_geom *ref, const float *img, const float *warped, const int *mask,
+                       const int *totalBlock, float *refPos, float *warpPos) {
+    int nb[3];
+    orc_block_count(ref, nb);
+    int defined = 0, b = 0;
+    for (int k = 0; k < nb[2]; ++k)
+        for (int j = 0; j < nb[1]; ++j)
+            for (int i = 0; i < nb[0]; ++i, ++b) {
+                if (totalBlock[b] < 0) continue;
+                float rv[64];
+                unsigned char rok[64];
+                for (int a = 0; a < 64; ++a) {
+                    int x = 4 * i + (a & 3), y = 4 * j + ((a >> 2) & 3), z = 4 * k + (a >> 4);
+                    rok[a] = 0;
+                    rv[a] = 0.f;
+                    if (x < ref->nx && y < ref->ny && z < ref->nz) {
+                        size_t q = ((size_t)z * ref->ny + y) * ref->nx + x;
+                        float v = img[q];
+                        if (v == v && (!mask || mask[q] > -1)) { rv[a] = v; rok[a] = 1; }
+                    }
+                }
+                float best = 0.f, disp[3] = { NAN, 0.f, 0.f };
+                for (int n = -3; n <= 3; ++n)
+                    for (int m = -3; m <= 3; ++m)
+                        for (int l = -3; l <= 3; ++l) {
+                            float wv[64];
+                            unsigned char wok[64];
+                            for (int a = 0; a < 64; ++a) {
+                                int x = 4 * i + l + (a & 3), y = 4 * j + m + ((a >> 2) & 3), z = 4 * k + n + (a >> 4);
+                                wok[a] = 0;
+                                wv[a] = 0.f;
+                                if (x > -1 && x < ref->nx && y > -1 && y < ref->ny && z > -1 && z < ref->nz) {
+                                    float v = warped[((size_t)z * ref->ny + y) * ref->nx + x];
+                                    if (v == v) { wv[a] = v; wok[a] = 1; }
+                                }
+                            }
+                            float rm = 0.f, wm = 0.f, cnt = 0.f;
+                            for (int a = 0; a < 64; ++a)
+                                if (rok[a] && wok[a]) { rm += rv[a]; wm += wv[a]; cnt += 1.f; }
+                            if (!(cnt > 32.f)) continue;
+                            rm /= cnt;
+                            wm /= cnt;
+                            float rvar = 0.f, wvar = 0.f, cc = 0.f;
+                            for (int a = 0; a < 64; ++a)
+                                if (rok[a] && wok[a]) {
+                                    float rt = rv[a] - rm, wt = wv[a] - wm;
+                                    rvar += rt * rt;
+                                    wvar += wt * wt;
+                                    cc += rt * wt;
+                                }
+                            cc = (rvar * wvar) > 0.f ? fabsf(cc / sqrtf(rvar * wvar)) : 0.f;
+                            if (cc > best) {
+                                best = cc + 1.0e-7f;
+                                disp[0] = (float)l;
+                                disp[1] = (float)m;
+                                disp[2] = (float)n;
+                            }
+                        }
+                float corner[3] = { (float)(4 * i), (float)(4 * j), (float)(4 * k) };
+                disp[0] += corner[0];
+                disp[1] += corner[1];
+                disp[2] += corner[2];
+                const int r = 3 * totalBlock[b];
+                mat44_mul_f(ref->xyz, corner, refPos + r);
+                mat44_mul_f(ref->xyz, disp, warpPos + r);
+                if (disp[0] == disp[0]) ++defined;
+            }
+    return defined;
+}

@@ -93,6 +93,24 @@ void orc_affine_field(const orc_geom *ref, const float affine[16], const int *ma
 /* mat44 inverse in double, stored in float (nifti_mat44_inverse semantics). */
 void orc_mat44_inverse(const float in[16], float out[16]);
 
+
+/* ---- Aladin ---------------------------------------------------------------------------- */
+/* Block grid of 4x4x4 blocks: nb = ceil(n/4) per axis; returns the total block number. */
+int orc_block_count(const orc_geom *ref, int nb[3]);
+/* Variance of one block, -1 when unusable (reg-lib/cpu/_reg_blockMatching.cpp:70-128). */
+float orc_block_variance(const orc_geom *ref, const float *img, const int *mask, int bi, int bj, int bk);
+/* _reg_set_active_blocks<float> + initialise_block_matching_method
+ * (reg-lib/cpu/_reg_blockMatching.cpp:49-273): totalBlock[b] = rank or -1; returns the active number. */
+int orc_active_blocks(const orc_geom *ref, const float *img, const int *mask, int percent, int *totalBlock);
+/* block_matching_method3D<float> (reg-lib/cpu/_reg_blockMatching.cpp:455-690); returns the defined count. */
+int orc_block_matching(const orc_geom *ref, const float *img, const float *warped, const int *mask,
+                       const int *totalBlock, float *refPos, float *warpPos);
+/* optimize + optimize_3D (reg-lib/cpu/_reg_blockMatching.cpp:722-821,
+ * reg-lib/cpu/_reg_globalTrans.cpp:328-394): LTS update of matrix (row-major, in/out).
+ * Returns 0, or -1 when there are too few defined correspondences. */
+int orc_lts(const float *refPos, const float *warpPos, int nActive, int percentToKeep, int affine,
+            float matrix[16]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -532,4 +532,85 @@ int ref_f3d_run(const nr_geom *ref, const float *refData, const nr_geom *flo, co
     });
 }
 
+// ---------------------------------------------------------------- Aladin (block matching + LTS)
+
+// initialise_block_matching_method + block_matching_method (cpu/_reg_blockMatching.cpp:213-273,
+// 455-690) on a reference image and an already-warped floating image.  Outputs are in the
+// reference's rank order; counts = {totalBlockNumber, activeBlockNumber, definedActiveBlockNumber}.
+int ref_block_matching(const nr_geom *ref, const float *refData, const float *warped, const int *mask,
+                       int blockPercent, int inlierLts, int *totalBlock, float *refPos, float *warpPos,
+                       int *counts) {
+    return guarded([&] {
+        NiftiImage r = make_image(ref, refData);
+        NiftiImage w = make_image(ref, warped);
+        std::vector<int> m(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, m.size() * sizeof(int));
+        _reg_blockMatchingParam p;
+        initialise_block_matching_method(r, &p, blockPercent, inlierLts, 1, m.data(), false);
+        counts[0] = p.totalBlockNumber;
+        counts[1] = p.activeBlockNumber;
+        if (totalBlock) std::memcpy(totalBlock, p.totalBlock, p.totalBlockNumber * sizeof(int));
+        if (warped) {
+            block_matching_method(r, w, &p, m.data());
+            counts[2] = p.definedActiveBlockNumber;
+            if (refPos) std::memcpy(refPos, p.referencePosition, 3 * p.activeBlockNumber * sizeof(float));
+            if (warpPos) std::memcpy(warpPos, p.warpedPosition, 3 * p.activeBlockNumber * sizeof(float));
+        }
+    });
+}
+
+// optimize (cpu/_reg_blockMatching.cpp:722-821): LTS update of a 4x4 matrix (row-major, in/out)
+// from nActive correspondences of which nDefined have a non-NaN warped position.
+int ref_lts(const float *refPos, const float *warpPos, int nActive, int nDefined, int percentToKeep, int affine,
+            float *matrix) {
+    return guarded([&] {
+        _reg_blockMatchingParam p;
+        p.dim = 3;
+        p.blockNumber[0] = p.blockNumber[1] = p.blockNumber[2] = 2;
+        p.activeBlockNumber = nActive;
+        p.definedActiveBlockNumber = nDefined;
+        p.percent_to_keep = percentToKeep;
+        p.referencePosition = (float *)malloc(3 * nActive * sizeof(float));
+        p.warpedPosition = (float *)malloc(3 * nActive * sizeof(float));
+        std::memcpy(p.referencePosition, refPos, 3 * nActive * sizeof(float));
+        std::memcpy(p.warpedPosition, warpPos, 3 * nActive * sizeof(float));
+        mat44 m = to_mat44(matrix);
+        optimize(&p, &m, affine != 0);
+        from_mat44(m, matrix);
+    });
+}
+
+typedef struct {
+    int levels;           // pyramid levels
+    int levels_to_perform;
+    int max_iterations;   // per level (doubled at the first level)
+    int block_percent;
+    int inlier_lts;
+    int rigid;            // perform the rigid stage
+    int affine;           // perform the affine stage
+    int align_centre;
+} ref_aladin_params;
+
+// reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364); matrix receives the final affine.
+int ref_aladin_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                   const ref_aladin_params *p, float *matrix) {
+    return guarded([&] {
+        reg_aladin<float> a;
+        a.SetPlatformType(PlatformType::Cpu);
+        a.SetInputReference(make_image(ref, refData));
+        a.SetInputFloating(make_image(flo, floData));
+        a.SetNumberOfLevels(p->levels);
+        a.SetLevelsToPerform(p->levels_to_perform);
+        a.SetMaxIterations(p->max_iterations);
+        a.SetBlockPercentage(p->block_percent);
+        a.SetInlierLts(p->inlier_lts);
+        a.SetPerformRigid(p->rigid != 0);
+        a.SetPerformAffine(p->affine != 0);
+        a.SetAlignCentre(p->align_centre != 0);
+        a.SetVerbose(false);
+        a.Run();
+        from_mat44(*a.GetTransformationMatrix(), matrix);
+    });
+}
+
 } // extern "C"

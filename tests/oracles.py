@@ -402,3 +402,65 @@ def nmi_at(refg, flog, cppg, cpp, ref_img, flo_img, bins=64):
     field = orc_field(refg, cppg, cpp)
     war = orc_resample(flog, f, refg, field)
     return orc_nmi(r, war, rb, rb)[0]
+
+
+# ---------------------------------------------------------------- Aladin
+class ref_aladin_params(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("max_iterations", ctypes.c_int),
+                ("block_percent", ctypes.c_int), ("inlier_lts", ctypes.c_int), ("rigid", ctypes.c_int),
+                ("affine", ctypes.c_int), ("align_centre", ctypes.c_int)]
+
+
+def block_count(g):
+    nb = [(n + 3) // 4 for n in g.dims]
+    return nb, nb[0] * nb[1] * nb[2]
+
+
+def orc_active_blocks(g, img, percent=50, mask=None):
+    _, total = block_count(g)
+    tb = np.zeros(total, np.int32)
+    n = orc().orc_active_blocks(ctypes.byref(og(g)), fp(img), ip(mask), percent, ip(tb))
+    return tb, n
+
+
+def orc_block_matching(g, img, warped, tb, n_active, mask=None):
+    rp = np.zeros(3 * n_active, np.float32)
+    wp = np.zeros(3 * n_active, np.float32)
+    d = orc().orc_block_matching(ctypes.byref(og(g)), fp(img), fp(warped), ip(mask), ip(tb), fp(rp), fp(wp))
+    return rp, wp, d
+
+
+def orc_lts(rp, wp, percent=50, affine=True, matrix=None):
+    m = np.eye(4, dtype=np.float32) if matrix is None else np.array(matrix, np.float32).copy()
+    rc = orc().orc_lts(fp(rp), fp(wp), len(rp) // 3, percent, int(affine), fp(m))
+    if rc != 0:
+        raise RuntimeError("too few correspondences")
+    return m
+
+
+def ref_block_matching(g, img, warped=None, percent=50, mask=None):
+    _, total = block_count(g)
+    tb = np.zeros(total, np.int32)
+    counts = np.zeros(3, np.int32)
+    n_max = total
+    rp = np.zeros(3 * n_max, np.float32)
+    wp = np.zeros(3 * n_max, np.float32)
+    rcheck(ref().ref_block_matching(ctypes.byref(rg(g)), fp(img), None if warped is None else fp(warped), ip(mask),
+                                    percent, 50, ip(tb), fp(rp), fp(wp), ip(counts)))
+    n = int(counts[1])
+    return tb, n, rp[:3 * n], wp[:3 * n], int(counts[2])
+
+
+def ref_lts(rp, wp, n_defined, percent=50, affine=True, matrix=None):
+    m = np.eye(4, dtype=np.float32) if matrix is None else np.array(matrix, np.float32).copy()
+    rcheck(ref().ref_lts(fp(rp), fp(wp), len(rp) // 3, n_defined, percent, int(affine), fp(m)))
+    return m
+
+
+def ref_aladin_run(refg, ref_img, flog, flo_img, levels=3, maxit=5, block_percent=50, inlier=50, rigid=False,
+                   affine=True, align_centre=True):
+    p = ref_aladin_params(levels, levels, maxit, block_percent, inlier, int(rigid), int(affine), int(align_centre))
+    m = np.zeros(16, np.float32)
+    rcheck(ref().ref_aladin_run(ctypes.byref(rg(refg)), fp(ref_img.ravel()), ctypes.byref(rg(flog)),
+                                fp(flo_img.ravel()), ctypes.byref(p), fp(m)))
+    return m.reshape(4, 4)
