@@ -175,6 +175,72 @@ int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom 
                      const nrh_f3d_config *cfg, nrh_geom *grid_out, float *cpp_out, size_t cpp_capacity,
                      double *final_nmi, int *iterations);
 
+/* ------------------------------------------------------------------ Aladin (affine) */
+/* One pyramid level of reg_aladin on the device: the HIP counterpart of AladinContent
+ * (reg-lib/AladinContent.cpp:6-29) with its block-matching parameters
+ * (_reg_blockMatchingParam, reg-lib/cpu/_reg_blockMatching.h:32-62).  The current affine
+ * matrix (floating mm -> reference mm direction of reg_aladin's affineTransformation) lives on
+ * the device, so warp -> block matching -> LTS iterations run without host round trips. */
+typedef struct nrh_aladin nrh_aladin;
+
+typedef struct nrh_aladin_params {
+    int block_percent;  /* percentage of blocks used (reg_aladin -%v, 50) */
+    int inlier_lts;     /* percentage of inliers kept by the LTS (reg_aladin -%i, 50) */
+    float padding;      /* warped padding value (NaN) */
+} nrh_aladin_params;
+
+/* AladinContent construction: uploads both images (and the optional reference mask, int per
+ * voxel, < 0 excluded), then selects the active blocks as initialise_block_matching_method /
+ * _reg_set_active_blocks<float> (reg-lib/cpu/_reg_blockMatching.cpp:213-273,49-206): block
+ * variances on the device, heap-sort ranking (reg-lib/cpu/Maths.cpp:114-153) on the host.
+ * Fails with NRH_ERR_NUMERIC when fewer than 2 blocks are usable. */
+int nrh_aladin_create(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
+                      const int *mask, const nrh_aladin_params *params, nrh_aladin **out);
+int nrh_aladin_destroy(nrh_aladin *h);
+/* counts = {totalBlockNumber, activeBlockNumber, definedActiveBlockNumber of the last block
+ * matching}; total_block (may be NULL) receives _reg_blockMatchingParam::totalBlock: the rank of
+ * each block among the active ones, or -1. */
+int nrh_aladin_blocks(nrh_aladin *h, int counts[3], int *total_block);
+/* The current affine (row-major 4x4; reg_aladin::affineTransformation). */
+int nrh_aladin_set_matrix(nrh_aladin *h, const float m[16]);
+int nrh_aladin_get_matrix(nrh_aladin *h, float m[16]);
+/* reg_aladin::GetWarpedImage (reg-lib/_reg_aladin.cpp:256-260): AffineDeformationFieldKernel
+ * (reg_affine_getDeformationField, reg-lib/cpu/_reg_globalTrans.cpp:74-175) fused with the
+ * trilinear ResampleImageKernel (reg-lib/cpu/_reg_resampling.cpp:316-464); the field is not
+ * stored. */
+int nrh_aladin_warp(nrh_aladin *h);
+int nrh_aladin_upload_warped(nrh_aladin *h, const float *warped);
+int nrh_aladin_download_warped(nrh_aladin *h, float *warped);
+/* BlockMatchingKernel: block_matching_method3D<float> (reg-lib/cpu/_reg_blockMatching.cpp:455-690),
+ * 4x4x4 blocks, capture range 3 voxels, step 1. */
+int nrh_aladin_block_matching(nrh_aladin *h);
+/* _reg_blockMatchingParam::referencePosition / warpedPosition (3 floats per active block, in
+ * rank order, mm; NaN warped position for a block without a match). */
+int nrh_aladin_download_positions(nrh_aladin *h, float *ref_pos, float *warp_pos, int *defined);
+/* LtsKernel: optimize (reg-lib/cpu/_reg_blockMatching.cpp:722-821) -> optimize_3D
+ * (reg-lib/cpu/_reg_globalTrans.cpp:328-394), affine (EstimateAffineLeastSquares) or rigid
+ * (EstimateRigidLeastSquares, reg-lib/cpu/Maths.cpp:325-390), in FP64 in one device workgroup;
+ * updates the handle's matrix.  Fails with NRH_ERR_NUMERIC when fewer than 8 (affine) / 4
+ * (rigid) correspondences are defined, leaving the matrix unchanged. */
+int nrh_aladin_lts(nrh_aladin *h, int affine);
+/* The same LTS on host arrays (n correspondences, NaN warped x = undefined); m is in/out. */
+int nrh_lts(const float *ref_pos, const float *warp_pos, int n, int inlier_lts, int affine, float m[16]);
+
+/* Full affine registration: reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364) with
+ * InitialiseRegistration's pyramid and centre alignment (:88-245), non-symmetric. */
+typedef struct nrh_aladin_config {
+    int levels;             /* -ln (3) */
+    int levels_to_perform;  /* -lp (0 = all) */
+    int max_iterations;     /* -maxit (5; doubled at the first level) */
+    int block_percent;      /* -%v (50) */
+    int inlier_lts;         /* -%i (50) */
+    int rigid;              /* rigid stage (1 unless -affDirect) */
+    int affine;             /* affine stage (1 unless -rigOnly) */
+    int align_centre;       /* 1 unless -noAlign */
+} nrh_aladin_config;
+int nrh_aladin_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
+                        const nrh_aladin_config *cfg, float matrix[16]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,16 @@ class nrh_f3d_config(ctypes.Structure):
                 ("image_storage", ctypes.c_int)]
 
 
+class nrh_aladin_params(ctypes.Structure):
+    _fields_ = [("block_percent", ctypes.c_int), ("inlier_lts", ctypes.c_int), ("padding", ctypes.c_float)]
+
+
+class nrh_aladin_config(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("max_iterations", ctypes.c_int),
+                ("block_percent", ctypes.c_int), ("inlier_lts", ctypes.c_int), ("rigid", ctypes.c_int),
+                ("affine", ctypes.c_int), ("align_centre", ctypes.c_int)]
+
+
 _lib = None
 
 

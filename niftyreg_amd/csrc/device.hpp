@@ -63,6 +63,41 @@ __device__ __forceinline__ void cell_of(int c, float gvs, bool lut, int& pre, fl
     }
 }
 
+// Trilinear sample of a float volume at floating voxel position p, with the padding value for
+// taps outside the volume (ResampleImage3D, reg-lib/cpu/_reg_resampling.cpp:316-464 with
+// interpLinearKernel, :79-84).
+__device__ __forceinline__ float sample_linear(const float *__restrict__ img, int nx, int ny, int nz, float3 p,
+                                               float pad) {
+#pragma clang fp contract(off)
+    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
+    float rx = p.x - (float)px, ry = p.y - (float)py, rz = p.z - (float)pz;
+    if (rx < 0) rx = 0;
+    if (ry < 0) ry = 0;
+    if (rz < 0) rz = 0;
+    const float xb[2] = { 1.f - rx, rx }, yb[2] = { 1.f - ry, ry }, zb[2] = { 1.f - rz, rz };
+    const bool inside = px > -1 && px + 1 < nx && py > -1 && py + 1 < ny && pz > -1 && pz + 1 < nz;
+    float intensity = 0.f;
+    for (int c = 0; c < 2; ++c) {
+        const int Z = pz + c;
+        float yT = 0.f;
+        for (int b = 0; b < 2; ++b) {
+            const int Y = py + b;
+            float xT = 0.f;
+            for (int a = 0; a < 2; ++a) {
+                const int X = px + a;
+                float v;
+                if (inside || (X > -1 && X < nx && Y > -1 && Y < ny && Z > -1 && Z < nz))
+                    v = img[((size_t)Z * ny + Y) * nx + X];
+                else v = pad;
+                xT += v * xb[a];
+            }
+            yT += xT * yb[b];
+        }
+        intensity += yT * zb[c];
+    }
+    return intensity;
+}
+
 // GetBasisSplineValue<double> (reg-lib/cpu/_reg_nmi.h:116-129); 2.f/3.f is a float constant there.
 __device__ __forceinline__ double nmi_basis(double x) {
     x = x < 0 ? -x : x;

@@ -4,6 +4,7 @@
 // Every objective and gradient evaluation runs on the device through the F3D handle.
 
 #include "f3d_handle.hpp"
+#include "host.hpp"
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -11,14 +12,15 @@
 
 namespace nrh {
 
-void mat44_inverse(const float *R, float *Q);
-void create_grid(const nrh_geom& ref, const float spacing[3], nrh_geom& grid, float *identity);
-
 namespace {
 
 // Gaussian smoothing of reg_tools_kernelConvolution (reg-lib/cpu/_reg_tools.cpp:832-1112) with all
 // three axes in one call: intensity and density are convolved axis after axis and divided once.
+// Each line sum follows the reference's default x86 build (USE_SSE with SSE3, :1017-1047): float
+// products accumulated in four float lanes over groups of four taps (lane j takes tap 3 - j of
+// each group), the lanes reduced as (l0 + l1) + (l2 + l3), then the remaining taps added in double.
 void gaussian_smooth(const nrh_geom& g, std::vector<float>& img, float sigmaVox) {
+#pragma clang fp contract(off)
     const int dims[3] = { g.nx, g.ny, g.nz };
     const size_t N = voxels(g);
     std::vector<float> dens(N);
@@ -33,7 +35,7 @@ void gaussian_smooth(const nrh_geom& g, std::vector<float>& img, float sigmaVox)
     const int radius = (int)(temp * 3.0f);
     std::vector<float> kern(2 * radius + 1);
     for (int i = -radius; i <= radius; ++i)
-        kern[i + radius] = (float)(std::exp(-(double)(i * i) / (2.0 * temp * temp)) / (temp * 2.506628274631));
+        kern[i + radius] = (float)(std::exp(-(double)(i * i) / (2.0 * (temp * temp))) / (temp * 2.506628274631));
     std::vector<float> bi, bd;
     for (int n = 0; n < 3; ++n) {
         if (dims[n] <= 1 || radius <= 0) continue;
@@ -55,8 +57,16 @@ void gaussian_smooth(const nrh_geom& g, std::vector<float>& img, float sigmaVox)
                 int pre = l - radius, pst = l + radius + 1, k0 = 0;
                 if (pre < 0) { k0 = -pre; pre = 0; }
                 if (pst > L) pst = L;
-                double is = 0, ds = 0;
-                for (int k = pre, kk = k0; k < pst; ++k, ++kk) {
+                float li[4] = { 0.f, 0.f, 0.f, 0.f }, ld[4] = { 0.f, 0.f, 0.f, 0.f };
+                int k = pre, kk = k0;
+                for (; k < pst - 3; k += 4, kk += 4)
+                    for (int j = 0; j < 4; ++j) {
+                        li[3 - j] = kern[kk + j] * bi[k + j] + li[3 - j];
+                        ld[3 - j] = kern[kk + j] * bd[k + j] + ld[3 - j];
+                    }
+                double is = (double)((li[0] + li[1]) + (li[2] + li[3]));
+                double ds = (double)((ld[0] + ld[1]) + (ld[2] + ld[3]));
+                for (; k < pst; ++k, ++kk) {
                     is += (double)(kern[kk] * bi[k]);
                     ds += (double)(kern[kk] * bd[k]);
                 }
@@ -163,13 +173,6 @@ void refine_grid(const nrh_geom& ref, const nrh_geom& old, const std::vector<flo
     }
 }
 
-namespace {
-
-struct Level {
-    nrh_geom geom;
-    std::vector<float> data;
-};
-
 // reg_createImagePyramid<float> (reg-lib/cpu/_reg_tools.cpp:1705-1733): the finest level is the
 // input; each coarser level halves the axes whose half-size is still >= 32 voxels.
 std::vector<Level> pyramid(const nrh_geom& g, const float *data, int levels, int toPerform) {
@@ -195,6 +198,8 @@ std::vector<Level> pyramid(const nrh_geom& g, const float *data, int levels, int
     }
     return p;
 }
+
+namespace {
 
 struct Handle {
     nrh_f3d *h = nullptr;

@@ -71,33 +71,7 @@ k_resample_linear(const float *__restrict__ field, size_t N, const int *__restri
     if (idx >= N) return;
     if (mask && mask[idx] < 0) return;
     const float3 p = affine_apply_d(ijk, field[idx], field[idx + N], field[idx + 2 * N]);
-    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
-    float rx = p.x - (float)px, ry = p.y - (float)py, rz = p.z - (float)pz;
-    if (rx < 0) rx = 0;
-    if (ry < 0) ry = 0;
-    if (rz < 0) rz = 0;
-    const float xb[2] = { 1.f - rx, rx }, yb[2] = { 1.f - ry, ry }, zb[2] = { 1.f - rz, rz };
-    const bool inside = px > -1 && px + 1 < fnx && py > -1 && py + 1 < fny && pz > -1 && pz + 1 < fnz;
-    float intensity = 0.f;
-    for (int c = 0; c < 2; ++c) {
-        const int Z = pz + c;
-        float yT = 0.f;
-        for (int b = 0; b < 2; ++b) {
-            const int Y = py + b;
-            float xT = 0.f;
-            for (int a = 0; a < 2; ++a) {
-                const int X = px + a;
-                float v;
-                if (inside || (X > -1 && X < fnx && Y > -1 && Y < fny && Z > -1 && Z < fnz))
-                    v = flo[((size_t)Z * fny + Y) * fnx + X];
-                else v = pad;
-                xT += v * xb[a];
-            }
-            yT += xT * yb[b];
-        }
-        intensity += yT * zb[c];
-    }
-    warped[idx] = intensity;
+    warped[idx] = sample_linear(flo, fnx, fny, fnz, p, pad);
 }
 
 // TrilinearImageGradient (reg-lib/cpu/_reg_resampling.cpp:1797-1915)

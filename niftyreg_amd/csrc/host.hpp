@@ -1,0 +1,22 @@
+// Host-side helpers shared by the F3D and Aladin drivers (no device use).
+#pragma once
+
+#include <vector>
+#include "common.hpp"
+
+namespace nrh {
+
+// nifti_mat44_inverse (reg-io/niftilib/nifti1_io.c:1680-1717)
+void mat44_inverse(const float *R, float *Q);
+// reg_createControlPointGrid<float> (reg-lib/cpu/_reg_localTrans.cpp:21-128)
+void create_grid(const nrh_geom& ref, const float spacing[3], nrh_geom& grid, float *identity);
+
+struct Level {
+    nrh_geom geom;
+    std::vector<float> data;
+};
+
+// reg_createImagePyramid<float> (reg-lib/cpu/_reg_tools.cpp:1705-1733)
+std::vector<Level> pyramid(const nrh_geom& g, const float *data, int levels, int toPerform);
+
+}  // namespace nrh

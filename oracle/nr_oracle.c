@@ -622,8 +622,10 @@ double orc_bending_energy(const orc_geom *cpp, const float *cppData) {
     const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
     float bxx[27], byy[27], bzz[27], bxy[27], byz[27], bxz[27];
     orc_second_order_basis(bxx, byy, bzz, bxy, byz, bxz);
-    double total = 0;
-#pragma omp parallel for collapse(2) reduction(+ : total) schedule(static)
+    /* per-node terms in parallel, then one sequential sum in node order: the reference's value
+     * at one OpenMP thread (its reduction order, and so its last bits, follow the thread count) */
+    double *terms = (double *)calloc(g, sizeof(double));
+#pragma omp parallel for collapse(2) schedule(static)
     for (int z = 1; z < cpp->nz - 1; ++z)
         for (int y = 1; y < cpp->ny - 1; ++y)
             for (int x = 1; x < cpp->nx - 1; ++x) {
@@ -653,8 +655,13 @@ double orc_bending_energy(const orc_geom *cpp, const float *cppData) {
                     s += (double)(d[k][2] * d[k][2]);
                     s += 2.0 * (double)(d[k][3] * d[k][3] + d[k][4] * d[k][4] + d[k][5] * d[k][5]);
                 }
-                total += s;
+                terms[((size_t)z * cpp->ny + y) * cpp->nx + x] = s;
             }
+    double total = 0;
+    for (int z = 1; z < cpp->nz - 1; ++z)
+        for (int y = 1; y < cpp->ny - 1; ++y)
+            for (int x = 1; x < cpp->nx - 1; ++x) total += terms[((size_t)z * cpp->ny + y) * cpp->nx + x];
+    free(terms);
     return total / (double)(3 * g);
 }
 

@@ -271,6 +271,13 @@ def ref_create_cpp(refg, spacing):
     return (out.nx, out.ny, out.nz), np.array(out.sform[:], np.float32).reshape(4, 4), data, tuple(out.pixdim[:])
 
 
+def orc_affine_field(refg, affine, mask=None):
+    field = np.zeros(3 * refg.nvox, np.float32)
+    a = np.ascontiguousarray(affine, np.float32).reshape(16)
+    orc().orc_affine_field(ctypes.byref(og(refg)), fp(a), ip(mask), fp(field))
+    return field
+
+
 def ref_affine_field(refg, affine, mask=None):
     field = np.zeros(3 * refg.nvox, np.float32)
     a = np.ascontiguousarray(affine, np.float32).reshape(16)

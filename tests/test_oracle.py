@@ -59,7 +59,9 @@ def test_golden_chain(name):
     assert np.array_equal(ng, d["scalar_node_grad"])
     sse = d["sse_node_grad"]
     assert np.abs(ng - sse).max() <= 1e-3 * np.abs(sse).max()
-    assert O.orc_be(cppg, cpp) == d["scalar_be"]
+    # the reference reduces the per-node terms with OpenMP, so its last bits follow its thread
+    # count; the oracle sums in node order
+    assert abs(O.orc_be(cppg, cpp) - d["scalar_be"]) <= 1e-13 * abs(d["scalar_be"])
     assert np.array_equal(O.orc_be_grad(cppg, cpp, np.float32(0.001)), d["scalar_be_grad"])
 
 
