@@ -77,7 +77,6 @@ template<class TImg, bool GRAD>
 __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
     // ---- LDS
     __shared__ float sCp[3][WZM][WYM][WXM];
-    __shared__ float sAz[TZ][WYM][WXM][3];
     __shared__ float sAyz[TY][WXM][3];
     __shared__ float4 sBx[TX], sBy[TY], sBz[TZ];
     __shared__ int sPx[TX], sPy[TY], sPz[TZ];
@@ -85,7 +84,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
     extern __shared__ float dyn[];  // histogram (value) or log table (gradient)
     __shared__ float sG[GRAD ? TY : 1][GRAD ? TX : 1][3];
     __shared__ float sU1[GRAD ? TY : 1][GRAD ? WXM : 1][3];
-    __shared__ float sU2[GRAD ? TZ : 1][GRAD ? WYM : 1][GRAD ? WXM : 1][3];
+    __shared__ float sOut[GRAD ? WZM : 1][GRAD ? WYM : 1][GRAD ? WXM : 1][3];
 
     const int tid = threadIdx.x;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -116,8 +115,11 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
     const int WZ = a.preZ[z0 + nZ - 1] - pz0 + 4;
 
     if (GRAD) {
-        const int nb = a.rb * a.fb + a.rb + a.fb;
-        for (int i = tid; i < nb; i += kThreads) dyn[i] = a.logTab[i];
+        // K[r + w rb] = log p_r + log p_w - NMI log p_rw: the voxel NMI derivative is then the
+        // bicubic contraction sum_r sum_w B(ref - r) B'(war - w) K[r, w] (see below)
+        const float nmiF = (float)a.ent[4];
+        const float *lj = a.logTab, *lr = a.logTab + a.rb * a.fb, *lw = lr + a.rb;
+        for (int i = tid; i < a.rb * a.fb; i += kThreads) dyn[i] = (lr[i % a.rb] + lw[i / a.rb]) - nmiF * lj[i];
     } else {
         uint32_t *h = (uint32_t *)dyn;
         for (int i = tid; i < a.rb * a.fb; i += kThreads) h[i] = 0u;
@@ -141,17 +143,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
         sXFirst[tid] = f;
         sXLast[tid] = l;
     }
-    // ---- stage A: z contraction for every z of the tile
-    for (int i = tid; i < nZ * WY * WX; i += kThreads) {
-        const int ix = i % WX, iy = (i / WX) % WY, z = i / (WX * WY);
-        const float4 b = sBz[z];
-        const int o = sPz[z] - pz0;
-        for (int c = 0; c < 3; ++c)
-            sAz[z][iy][ix][c] = b.x * sCp[c][o][iy][ix] + b.y * sCp[c][o + 1][iy][ix] + b.z * sCp[c][o + 2][iy][ix] +
-                                b.w * sCp[c][o + 3][iy][ix];
-    }
     if (GRAD) {
-        for (int i = tid; i < TZ * WYM * WXM * 3; i += kThreads) (&sU2[0][0][0][0])[i] = 0.f;
+        for (int i = tid; i < WZM * WYM * WXM * 3; i += kThreads) (&sOut[0][0][0][0])[i] = 0.f;
     }
     __syncthreads();
 
@@ -165,14 +158,22 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
     const int nb2 = a.rb * a.fb;
 
     for (int z = 0; z < nZ; ++z) {
-        // ---- stage B: y contraction for this slice
-        for (int i = tid; i < nY * WX; i += kThreads) {
-            const int ix = i % WX, y = i / WX;
-            const float4 b = sBy[y];
-            const int o = sPy[y] - py0;
-            for (int c = 0; c < 3; ++c)
-                sAyz[y][ix][c] = b.x * sAz[z][o][ix][c] + b.y * sAz[z][o + 1][ix][c] + b.z * sAz[z][o + 2][ix][c] +
-                                 b.w * sAz[z][o + 3][ix][c];
+        // ---- stage B: z then y contraction of the window for this slice
+        {
+            const float4 bz = sBz[z];
+            const int oz = sPz[z] - pz0;
+            for (int i = tid; i < nY * WX; i += kThreads) {
+                const int ix = i % WX, y = i / WX;
+                const float4 b = sBy[y];
+                const int o = sPy[y] - py0;
+                for (int c = 0; c < 3; ++c) {
+                    float az[4];
+                    for (int j = 0; j < 4; ++j)
+                        az[j] = bz.x * sCp[c][oz][o + j][ix] + bz.y * sCp[c][oz + 1][o + j][ix] +
+                                bz.z * sCp[c][oz + 2][o + j][ix] + bz.w * sCp[c][oz + 3][o + j][ix];
+                    sAyz[y][ix][c] = b.x * az[0] + b.y * az[1] + b.z * az[2] + b.w * az[3];
+                }
+            }
         }
         __syncthreads();
         // ---- per voxel: field, warp (+ gradient)
@@ -248,34 +249,39 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
                         if (gx != gx) gx = 0.f;
                         if (gy != gy) gy = 0.f;
                         if (gz != gz) gz = 0.f;
-                        // NMI derivative (reg_getVoxelBasedNmiGradient3d) in separable form
-                        const float *lj = dyn;
-                        const float *lr = dyn + nb2;
-                        const float *lw = dyn + nb2 + a.rb;
-                        float A = 0.f, sumB = 0.f, sumD = 0.f, C = 0.f, D = 0.f;
-                        const int r0 = (int)(rv - 1.f), r1 = (int)(rv + 3.f);
-                        const int w0 = (int)(war - 1.f), w1 = (int)(war + 3.f);
-                        float bw[4];
-                        int wi[4];
-                        int nw = 0;
-                        for (int ww = w0; ww < w1 && nw < 4; ++ww) {
-                            if (ww < 0 || ww >= a.fb) continue;
-                            const float d = nmi_basis_deriv_f(war - (float)ww);
-                            bw[nw] = d;
-                            wi[nw++] = ww;
-                            sumD += d;
-                            C += d * lw[ww];
+                        // NMI derivative (reg_getVoxelBasedNmiGradient3d, reg-lib/cpu/_reg_nmi.cpp:501-593):
+                        //   (sum_r B_r log p_r)(sum_w B'_w) + (sum_r B_r)(sum_w B'_w log p_w)
+                        //   - NMI sum_r sum_w B_r B'_w log p_rw  =  sum_r sum_w B_r B'_w K[r, w]
+                        // over the 4 bins around each intensity (bins outside the histogram get
+                        // weight 0, as the reference skips them).  B_r: cubic B-spline weights of the
+                        // reference intensity's fraction; B'_w: their derivative for the warped one.
+                        const float rfl = floorf(rv), wfl = floorf(war);
+                        const float f = rv - rfl, gg = war - wfl;
+                        const float f2 = f * f, f3 = f2 * f, omf = 1.f - f;
+                        float br[4] = { omf * omf * omf * (1.f / 6.f), 0.66666669f - f2 + 0.5f * f3,
+                                        (-3.f * f3 + 3.f * f2 + 3.f * f + 1.f) * (1.f / 6.f), f3 * (1.f / 6.f) };
+                        const float omg = 1.f - gg;
+                        float dw[4] = { -0.5f * omg * omg, (1.5f * gg - 2.f) * gg, (0.5f + 1.5f * gg) * omg,
+                                        0.5f * gg * gg };
+                        const int rb0 = (int)rfl - 1, wb0 = (int)wfl - 1;
+                        int ri[4], wc[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const bool okr = (unsigned)(rb0 + k) < (unsigned)a.rb;
+                            const bool okw = (unsigned)(wb0 + k) < (unsigned)a.fb;
+                            ri[k] = okr ? rb0 + k : 0;
+                            br[k] = okr ? br[k] : 0.f;
+                            wc[k] = (okw ? wb0 + k : 0) * a.rb;
+                            dw[k] = okw ? dw[k] : 0.f;
                         }
-                        for (int rr = r0; rr < r1; ++rr) {
-                            if (rr < 0 || rr >= a.rb) continue;
-                            const float bb = nmi_basis_f(rv - (float)rr);
-                            sumB += bb;
-                            A += bb * lr[rr];
-                            float s = 0.f;
-                            for (int k = 0; k < nw; ++k) s += bw[k] * lj[rr + wi[k] * a.rb];
-                            D += bb * s;
+                        float S = 0.f;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const float T = br[0] * dyn[ri[0] + wc[j]] + br[1] * dyn[ri[1] + wc[j]] +
+                                            br[2] * dyn[ri[2] + wc[j]] + br[3] * dyn[ri[3] + wc[j]];
+                            S += dw[j] * T;
                         }
-                        const float S = (A * sumD + sumB * C - nmi * D) * coef;
+                        S *= coef;
                         g0 = S * gx;
                         g1 = S * gy;
                         g2 = S * gz;
@@ -307,7 +313,10 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
                 sU1[y][ix][2] = s2;
             }
             __syncthreads();
-            // ---- stage B^T: y adjoint into the slice's (node y, node x) accumulator
+            // ---- stage B^T: y adjoint of the slice, then its z-adjoint share added to the
+            // tile's node window (slices in z order, one owner thread per (node y, node x))
+            const float4 bzz = sBz[z];
+            const int ozz = sPz[z] - pz0;
             for (int i = tid; i < WY * WX; i += kThreads) {
                 const int ix = i % WX, iy = i / WX;
                 float s0 = 0.f, s1 = 0.f, s2 = 0.f;
@@ -320,9 +329,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
                     s1 += w * sU1[y][ix][1];
                     s2 += w * sU1[y][ix][2];
                 }
-                sU2[z][iy][ix][0] = s0;
-                sU2[z][iy][ix][1] = s1;
-                sU2[z][iy][ix][2] = s2;
+                const float bzk[4] = { bzz.x, bzz.y, bzz.z, bzz.w };
+                for (int k = 0; k < 4; ++k) {
+                    sOut[ozz + k][iy][ix][0] += bzk[k] * s0;
+                    sOut[ozz + k][iy][ix][1] += bzk[k] * s1;
+                    sOut[ozz + k][iy][ix][2] += bzk[k] * s2;
+                }
             }
         }
         __syncthreads();
@@ -333,24 +345,14 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
         for (int i = tid; i < nb2; i += kThreads)
             if (h[i]) atomicAdd(&a.hist[i], h[i]);
     } else {
-        // ---- stage A^T: z adjoint, written as this tile's partial node window
+        // ---- the tile's partial node window
         float *out = a.partials + (size_t)t * 3 * WIN;
         for (int i = tid; i < WZ * WY * WX; i += kThreads) {
             const int ix = i % WX, iy = (i / WX) % WY, iz = i / (WX * WY);
-            float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-            for (int z = 0; z < nZ; ++z) {
-                const int o = iz - (sPz[z] - pz0);
-                if (o < 0 || o > 3) continue;
-                const float4 b = sBz[z];
-                const float w = o == 0 ? b.x : o == 1 ? b.y : o == 2 ? b.z : b.w;
-                s0 += w * sU2[z][iy][ix][0];
-                s1 += w * sU2[z][iy][ix][1];
-                s2 += w * sU2[z][iy][ix][2];
-            }
             const int wi = (iz * WYM + iy) * WXM + ix;
-            out[wi] = s0;
-            out[WIN + wi] = s1;
-            out[2 * WIN + wi] = s2;
+            out[wi] = sOut[iz][iy][ix][0];
+            out[WIN + wi] = sOut[iz][iy][ix][1];
+            out[2 * WIN + wi] = sOut[iz][iy][ix][2];
         }
     }
 }
@@ -626,7 +628,7 @@ void fused_gradient(nrh_f3d *h, float weight) {
     a.ent = h->ent.p;
     h->nodePartials.alloc((size_t)p->numTiles * 3 * WIN);
     a.partials = h->nodePartials.p;
-    const size_t lds = (size_t)h->totalBins() * sizeof(float);
+    const size_t lds = (size_t)a.rb * a.fb * sizeof(float);
     if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
     if (h->params.image_storage == 1)
         hipLaunchKernelGGL((k_fused_pass<__half, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);

@@ -401,12 +401,15 @@ def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing
     return geom_from_nr(out), buf[:n].copy()
 
 
-def nmi_at(refg, flog, cppg, cpp, ref_img, flo_img, bins=64):
-    """NMI of the pair warped through a control-point grid, on the oracle (images rescaled here)."""
+def nmi_at(refg, flog, cppg, cpp, ref_img, flo_img, bins=64, fp64=False):
+    """NMI of the pair warped through a control-point grid, on the oracle (images rescaled here).
+    fp64: evaluate the field in double (rounded once to float) instead of the reference's float
+    sums; the two differ in whether voxels mapped exactly onto the last voxel plane (identity
+    grids) fall outside the floating image."""
     rb = bins + 4
     r = orc_rescale(np.asarray(ref_img, np.float32).ravel(), 2, rb - 3)
     f = orc_rescale(np.asarray(flo_img, np.float32).ravel(), 2, rb - 3)
-    field = orc_field(refg, cppg, cpp)
+    field = fp64_field(refg, cppg, cpp).astype(np.float32) if fp64 else orc_field(refg, cppg, cpp)
     war = orc_resample(flog, f, refg, field)
     return orc_nmi(r, war, rb, rb)[0]
 

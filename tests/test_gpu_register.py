@@ -3,6 +3,13 @@
 Line-search accept/reject decisions depend on the last bits of each objective value, so the
 two trajectories diverge after a few iterations; parity of a whole registration is the final
 NMI within 1e-4 (the device may not end lower than the reference by more than that).
+
+Both final grids are scored with the same checker: the oracle NMI with the deformation field
+evaluated in double.  The reference's float field puts voxels that map exactly onto the last
+voxel plane (as on the identity grid every registration starts from) inside or outside the
+floating image depending on the last bit: on the 64^3 identity grid its SSE build marks 89% of
+the upper x face as outside, its scalar build 41%, moving the NMI by 2.7e-3.  The fused
+sweeps evaluate the position in double, so their NMI is checked against the double-field score.
 """
 import time
 
@@ -26,13 +33,13 @@ def test_config0_registration_matches_reference(gpu, levels, maxit):
     t0 = time.perf_counter()
     cg, cpp, nmi, its = register(g, ref, g, flo, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=levels, maxit=maxit)
     t_gpu = time.perf_counter() - t0
-    nmi_gpu = O.nmi_at(g, g, cg, cpp, ref, flo)
+    nmi_gpu = O.nmi_at(g, g, cg, cpp, ref, flo, fp64=True)
     t0 = time.perf_counter()
     rcg, rcpp = O.ref_f3d_run(g, ref, g, flo, levels=levels, maxit=maxit)
     t_cpu = time.perf_counter() - t0
-    nmi_ref = O.nmi_at(g, g, rcg, rcpp, ref, flo)
+    nmi_ref = O.nmi_at(g, g, rcg, rcpp, ref, flo, fp64=True)
     g0, ident = create_control_point_grid(g, [5.0] * 3)
-    nmi0 = O.nmi_at(g, g, g0, ident, ref, flo)
+    nmi0 = O.nmi_at(g, g, g0, ident, ref, flo, fp64=True)
     print(f"levels {levels}: start NMI {nmi0:.6f}, device {nmi_gpu:.6f} ({its} it, {t_gpu:.2f} s), "
           f"reference {nmi_ref:.6f} ({t_cpu:.2f} s)")
     assert cg.dims == rcg.dims
