@@ -968,3 +968,173 @@ int orc_block_matching(const orc_geom *ref, const float *img, const float *warpe
             }
     return defined;
 }
+
+/* ------------------------------------------------------------------ LNCC */
+/* reg_tools_kernelConvolution<float> over the three axes of one volume (reg-lib/cpu/_reg_tools.cpp:
+ * 832-1112, scalar build): density = (mask >= 0 && finite); intensity and density convolved line by
+ * line along x, then y, then z (float products, double line sums); intensity / density at the end,
+ * NaN where the density started at 0. */
+void orc_kernel_convolution(const orc_geom *g, float *data, float sigma, int kernelType, const int *mask) {
+    const int dims[3] = { g->nx, g->ny, g->nz };
+    const float pix[3] = { g->dx, g->dy, g->dz };
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    unsigned char *isnan_ = (unsigned char *)malloc(n);
+    float *dens = (float *)malloc(n * sizeof(float));
+    for (size_t i = 0; i < n; ++i) {
+        dens[i] = ((!mask || mask[i] >= 0) && data[i] == data[i]) ? 1.f : 0.f;
+        isnan_[i] = dens[i] == 0.f;
+        if (isnan_[i]) data[i] = 0;
+    }
+    for (int axis = 0; axis < 3; ++axis) {
+        if (dims[axis] <= 1) continue;
+        double temp = sigma > 0 ? (double)(sigma / pix[axis]) : fabs((double)sigma);
+        int radius = kernelType == 2 ? (int)(temp * 3.0f) : (int)(temp * 2.0f);
+        if (radius <= 0) continue;
+        float *kernel = (float *)malloc((2 * radius + 1) * sizeof(float));
+        for (int i = -radius; i <= radius; ++i) {
+            if (kernelType == 3) {
+                double rel = fabs(i / temp);
+                if (rel < 1.0) kernel[i + radius] = (float)(2.0 / 3.0 - rel * rel + 0.5 * rel * rel * rel);
+                else if (rel < 2.0) kernel[i + radius] = (float)(-((rel - 2.0) * (rel - 2.0) * (rel - 2.0)) / 6.0);
+                else kernel[i + radius] = 0;
+            } else {
+                kernel[i + radius] = (float)(exp(-(double)(i * i) / (2.0 * (temp * temp))) / (temp * 2.506628274631));
+            }
+        }
+        int planes, lineOff;
+        if (axis == 0) { planes = dims[1] * dims[2]; lineOff = 1; }
+        else if (axis == 1) { planes = dims[0] * dims[2]; lineOff = dims[0]; }
+        else { planes = dims[0] * dims[1]; lineOff = planes; }
+#pragma omp parallel for schedule(static)
+        for (int p = 0; p < planes; ++p) {
+            size_t start;
+            if (axis == 0) start = (size_t)p * dims[0];
+            else if (axis == 1) start = (size_t)(p / dims[0]) * dims[0] * dims[1] + p % dims[0];
+            else start = p;
+            const int L = dims[axis];
+            float *bi = (float *)malloc(L * sizeof(float));
+            float *bd = (float *)malloc(L * sizeof(float));
+            for (int l = 0; l < L; ++l) {
+                bi[l] = data[start + (size_t)l * lineOff];
+                bd[l] = dens[start + (size_t)l * lineOff];
+            }
+            for (int l = 0; l < L; ++l) {
+                int pre = l - radius, pst = l + radius + 1;
+                const float *kp = kernel;
+                if (pre < 0) { kp = kernel - pre; pre = 0; }
+                if (pst > L) pst = L;
+                double is = 0, ds = 0;
+                for (int k = pre; k < pst; ++k) {
+                    const float kv = *kp++;
+                    is += kv * bi[k];
+                    ds += kv * bd[k];
+                }
+                data[start + (size_t)l * lineOff] = (float)is;
+                dens[start + (size_t)l * lineOff] = (float)ds;
+            }
+            free(bi);
+            free(bd);
+        }
+        free(kernel);
+    }
+    for (size_t i = 0; i < n; ++i) data[i] = isnan_[i] ? NAN : data[i] / dens[i];
+    free(isnan_);
+    free(dens);
+}
+
+/* UpdateLocalStatImages<float> (reg-lib/cpu/_reg_lncc.cpp:195-245): combined mask (reference mask
+ * with the NaN voxels of both images removed), Gaussian-weighted local means and standard
+ * deviations of the reference and warped images. */
+static void lncc_local_stats(const orc_geom *g, const float *ref, const float *war, const int *mask, float sigma,
+                             float *mean, float *sdev, float *wmean, float *wsdev, int *combined) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    for (size_t i = 0; i < n; ++i) {
+        combined[i] = mask ? mask[i] : 0;
+        if (ref[i] != ref[i]) combined[i] = -1;
+        if (war[i] != war[i]) combined[i] = -1;
+        mean[i] = ref[i];
+        sdev[i] = ref[i] * ref[i];
+        wmean[i] = war[i];
+        wsdev[i] = war[i] * war[i];
+    }
+    orc_kernel_convolution(g, mean, sigma, 2, combined);
+    orc_kernel_convolution(g, sdev, sigma, 2, combined);
+    orc_kernel_convolution(g, wmean, sigma, 2, combined);
+    orc_kernel_convolution(g, wsdev, sigma, 2, combined);
+    for (size_t i = 0; i < n; ++i) {
+        sdev[i] = sqrtf(sdev[i] - mean[i] * mean[i]);
+        wsdev[i] = sqrtf(wsdev[i] - wmean[i] * wmean[i]);
+        if (sdev[i] < 1.e-06) sdev[i] = 0;
+        if (wsdev[i] < 1.e-06) wsdev[i] = 0;
+    }
+}
+
+/* reg_lncc value for one time point (reg-lib/cpu/_reg_lncc.cpp:247-300): mean |local correlation|
+ * over the combined mask, the sum accumulated in voxel order. */
+double orc_lncc_value(const orc_geom *g, const float *ref, const float *war, const int *mask, float sigma) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    float *mean = (float *)malloc(n * sizeof(float)), *sdev = (float *)malloc(n * sizeof(float));
+    float *wmean = (float *)malloc(n * sizeof(float)), *wsdev = (float *)malloc(n * sizeof(float));
+    float *corr = (float *)malloc(n * sizeof(float));
+    int *comb = (int *)malloc(n * sizeof(int));
+    lncc_local_stats(g, ref, war, mask, sigma, mean, sdev, wmean, wsdev, comb);
+    for (size_t i = 0; i < n; ++i) corr[i] = ref[i] * war[i];
+    orc_kernel_convolution(g, corr, sigma, 2, comb);
+    double sum = 0;
+    size_t active = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (comb[i] > -1) {
+            const double l = (corr[i] - (mean[i] * wmean[i])) / (sdev[i] * wsdev[i]);
+            if (l == l && !isinf(l)) {
+                sum += fabs(l);
+                ++active;
+            }
+        }
+    }
+    free(mean); free(sdev); free(wmean); free(wsdev); free(corr); free(comb);
+    return sum / (double)active;
+}
+
+/* reg_getVoxelBasedLnccGradient<float> (reg-lib/cpu/_reg_lncc.cpp:393-512); grad -= , then
+ * non-finite gradient entries are zeroed. */
+void orc_lncc_gradient(const orc_geom *g, const float *ref, const float *war, const float *warGrad, const int *mask,
+                       float sigma, double weight, float *grad) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    float *mean = (float *)malloc(n * sizeof(float)), *sdev = (float *)malloc(n * sizeof(float));
+    float *wmean = (float *)malloc(n * sizeof(float)), *wsdev = (float *)malloc(n * sizeof(float));
+    float *corr = (float *)malloc(n * sizeof(float));
+    int *comb = (int *)malloc(n * sizeof(int));
+    lncc_local_stats(g, ref, war, mask, sigma, mean, sdev, wmean, wsdev, comb);
+    for (size_t i = 0; i < n; ++i) corr[i] = ref[i] * war[i];
+    orc_kernel_convolution(g, corr, sigma, 2, comb);
+    size_t active = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (comb[i] > -1) {
+            const double rm = mean[i], wm = wmean[i], rs = sdev[i], ws = wsdev[i];
+            const double c = corr[i] - (rm * wm);
+            double t1 = 1.0 / (rs * ws);
+            double t2 = c / (rs * ws * ws * ws);
+            double t3 = (c * wm) / (rs * ws * ws * ws) - rm / (rs * ws);
+            if (t1 == t1 && !isinf(t1) && t2 == t2 && !isinf(t2) && t3 == t3 && !isinf(t3)) {
+                if (c < 0) { t1 = -t1; t2 = -t2; t3 = -t3; }
+                wmean[i] = (float)t1;
+                wsdev[i] = (float)t2;
+                corr[i] = (float)t3;
+                ++active;
+            } else wmean[i] = wsdev[i] = corr[i] = 0;
+        } else wmean[i] = wsdev[i] = corr[i] = 0;
+    }
+    const double adjusted = weight / (double)active;
+    orc_kernel_convolution(g, wmean, sigma, 2, comb);
+    orc_kernel_convolution(g, wsdev, sigma, 2, comb);
+    orc_kernel_convolution(g, corr, sigma, 2, comb);
+    for (size_t i = 0; i < n; ++i) {
+        if (comb[i] > -1) {
+            const double common = (wmean[i] * ref[i] - wsdev[i] * war[i] + corr[i]) * adjusted;
+            for (int c = 0; c < 3; ++c) grad[i + c * n] -= (float)(warGrad[i + c * n] * common);
+        }
+    }
+    for (size_t i = 0; i < 3 * n; ++i)
+        if (grad[i] != grad[i] || isinf(grad[i])) grad[i] = 0;
+    free(mean); free(sdev); free(wmean); free(wsdev); free(corr); free(comb);
+}

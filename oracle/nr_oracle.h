@@ -111,6 +111,17 @@ int orc_block_matching(const orc_geom *ref, const float *img, const float *warpe
 int orc_lts(const float *refPos, const float *warpPos, int nActive, int percentToKeep, int affine,
             float matrix[16]);
 
+
+/* ---- LNCC ------------------------------------------------------------------------------ */
+/* reg_tools_kernelConvolution<float> on one volume over the three axes (reg-lib/cpu/_reg_tools.cpp:
+ * 832-1112, scalar build).  kernel: 2 Gaussian, 3 cubic B-spline; sigma < 0 in voxels. */
+void orc_kernel_convolution(const orc_geom *g, float *data, float sigma, int kernel, const int *mask);
+/* reg_lncc value, one time point (reg-lib/cpu/_reg_lncc.cpp:195-300); images already in [0, 1]. */
+double orc_lncc_value(const orc_geom *g, const float *ref, const float *war, const int *mask, float sigma);
+/* reg_getVoxelBasedLnccGradient<float> (reg-lib/cpu/_reg_lncc.cpp:393-512); grad -= */
+void orc_lncc_gradient(const orc_geom *g, const float *ref, const float *war, const float *warGrad, const int *mask,
+                       float sigma, double weight, float *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,23 @@ typedef struct {
 } nr_geom;
 }
 
+// The LNCC free functions of reg-lib/cpu/_reg_lncc.cpp (defined there at namespace scope, not
+// declared in its header).
+double GetSimilarityMeasureValue(const nifti_image *referenceImage, nifti_image *meanImage, nifti_image *sdevImage,
+                                 const nifti_image *warpedImage, nifti_image *warpedMeanImage,
+                                 nifti_image *warpedSdevImage, const int *referenceMask, int *forwardMask,
+                                 const float *kernelStandardDeviation, nifti_image *correlationImage,
+                                 const ConvKernelType kernelType, const int referenceTimePoints,
+                                 const double *timePointWeights);
+void GetVoxelBasedSimilarityMeasureGradient(const nifti_image *referenceImage, nifti_image *meanImage,
+                                            nifti_image *sdevImage, const nifti_image *warpedImage,
+                                            nifti_image *warpedMeanImage, nifti_image *warpedSdevImage,
+                                            const int *referenceMask, int *forwardMask,
+                                            const float *kernelStandardDeviation, nifti_image *correlationImage,
+                                            const nifti_image *warpedGradient, nifti_image *measureGradient,
+                                            const ConvKernelType kernelType, const int currentTimePoint,
+                                            const double timePointWeight);
+
 namespace {
 
 mat44 to_mat44(const float *m) {
@@ -610,6 +627,42 @@ int ref_aladin_run(const nr_geom *ref, const float *refData, const nr_geom *flo,
         a.SetVerbose(false);
         a.Run();
         from_mat44(*a.GetTransformationMatrix(), matrix);
+    });
+}
+
+// ---------------------------------------------------------------- LNCC
+
+// reg_lncc value (reg-lib/cpu/_reg_lncc.cpp:303-345) on a reference and a warped image already
+// rescaled to [0, 1]; Gaussian kernel, sigma in voxels when negative.
+int ref_lncc_value(const nr_geom *ref, const float *refData, const float *warped, const int *mask, float sigma,
+                   double *value) {
+    return guarded([&] {
+        NiftiImage r = make_image(ref, refData), w = make_image(ref, warped);
+        NiftiImage mean = make_image(ref, refData), sdev = make_image(ref, refData);
+        NiftiImage wmean = make_image(ref, refData), wsdev = make_image(ref, refData), corr = make_image(ref, refData);
+        std::vector<int> m(nvox(ref), 0), fw(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, m.size() * sizeof(int));
+        const float sd[1] = { sigma };
+        const double tw[1] = { 1.0 };
+        *value = GetSimilarityMeasureValue(r, mean, sdev, w, wmean, wsdev, m.data(), fw.data(), sd, corr,
+                                           ConvKernelType::Gaussian, 1, tw);
+    });
+}
+
+// reg_lncc voxel-based gradient (reg-lib/cpu/_reg_lncc.cpp:514-577): grad (3N, in/out) -= ...
+int ref_lncc_gradient(const nr_geom *ref, const float *refData, const float *warped, const float *warpedGrad,
+                      const int *mask, float sigma, double weight, float *grad) {
+    return guarded([&] {
+        NiftiImage r = make_image(ref, refData), w = make_image(ref, warped);
+        NiftiImage mean = make_image(ref, refData), sdev = make_image(ref, refData);
+        NiftiImage wmean = make_image(ref, refData), wsdev = make_image(ref, refData), corr = make_image(ref, refData);
+        NiftiImage wg = make_vector_image(ref, warpedGrad), g = make_vector_image(ref, grad);
+        std::vector<int> m(nvox(ref), 0), fw(nvox(ref), 0);
+        if (mask) std::memcpy(m.data(), mask, m.size() * sizeof(int));
+        const float sd[1] = { sigma };
+        GetVoxelBasedSimilarityMeasureGradient(r, mean, sdev, w, wmean, wsdev, m.data(), fw.data(), sd, corr, wg, g,
+                                               ConvKernelType::Gaussian, 0, weight);
+        std::memcpy(grad, g->data, 3 * nvox(ref) * sizeof(float));
     });
 }
 

@@ -89,6 +89,38 @@ def main():
     m[:3, 3] = [2.0, -3.0, 5.0]
     case_f3d("oblique", None, 4.0, 3, geom=Geom((20, 18, 16), m))
     case_objective("config0_small", (40, 36, 32), 0)
+    case_lncc("blobs", (32, 28, 24), 5)
+    case_lncc("masked", (26, 24, 22), 6, masked=True)
+
+
+def case_lncc(name, dims, seed, masked=False):
+    """reg_lncc value and voxel-based gradient on a B-spline-warped pair rescaled to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    refg = Geom.identity(dims)
+    ref = phantoms.blob_phantom(refg.dims, seed).ravel()
+    flo = phantoms.blob_phantom(refg.dims, seed + 3).ravel()
+    cppg, cpp = create_control_point_grid(refg, [5.0] * 3)
+    cpp = (cpp + rng.uniform(-2, 2, cpp.size)).astype(np.float32)
+    mask = None
+    if masked:
+        mask = (rng.uniform(size=refg.nvox) > 0.1).astype(np.int32) - 1
+    out = dict(ref=ref, flo=flo, cpp=cpp, dims=np.array(dims), cpp_dims=np.array(cppg.dims), cpp_xyz=cppg.xyz,
+               mask=mask if mask is not None else np.zeros(0, np.int32))
+    for variant, lib in (("sse", "libnrref.so"), ("scalar", "libnrref_nosse.so")):
+        O._ref = None
+        O.REF = os.path.join(ROOT, "oracle", "_ref", lib)
+        field = O.ref_field(refg, cppg, cpp)
+        ref_r = O.ref_rescale(refg, ref, 0, 1)
+        flo_r = O.ref_rescale(refg, flo, 0, 1)
+        war = O.ref_resample(refg, flo_r, refg, field, mask)
+        wg = O.ref_image_gradient(refg, flo_r, refg, field, mask)
+        value = O.ref_lncc(refg, ref_r, war, mask)
+        grad = O.ref_lncc_grad(refg, ref_r, war, wg, mask, weight=1.0)
+        if variant == "scalar":
+            out.update(ref_rescaled=ref_r, warped=war, warped_grad=wg)
+        out.update({f"{variant}_value": np.float64(value), f"{variant}_grad": grad})
+    np.savez_compressed(os.path.join(HERE, f"lncc_{name}.npz"), **out)
+    print("wrote", name)
 
 
 if __name__ == "__main__":

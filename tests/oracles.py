@@ -474,3 +474,36 @@ def ref_aladin_run(refg, ref_img, flog, flo_img, levels=3, maxit=5, block_percen
     rcheck(ref().ref_aladin_run(ctypes.byref(rg(refg)), fp(ref_img.ravel()), ctypes.byref(rg(flog)),
                                 fp(flo_img.ravel()), ctypes.byref(p), fp(m)))
     return m.reshape(4, 4)
+
+
+# ---------------------------------------------------------------- LNCC
+def orc_convolution(g, data, sigma, kernel=2, mask=None):
+    out = np.ascontiguousarray(data, np.float32).copy()
+    orc().orc_kernel_convolution(ctypes.byref(og(g)), fp(out), ctypes.c_float(sigma), kernel, ip(mask))
+    return out
+
+
+def orc_lncc(g, refv, war, mask=None, sigma=-5.0):
+    orc().orc_lncc_value.restype = ctypes.c_double
+    return orc().orc_lncc_value(ctypes.byref(og(g)), fp(refv), fp(war), ip(mask), ctypes.c_float(sigma))
+
+
+def orc_lncc_grad(g, refv, war, wgrad, mask=None, sigma=-5.0, weight=1.0, init=None):
+    grad = np.zeros(3 * g.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    orc().orc_lncc_gradient(ctypes.byref(og(g)), fp(refv), fp(war), fp(wgrad), ip(mask), ctypes.c_float(sigma),
+                            ctypes.c_double(weight), fp(grad))
+    return grad
+
+
+def ref_lncc(g, refv, war, mask=None, sigma=-5.0):
+    v = ctypes.c_double()
+    rcheck(ref().ref_lncc_value(ctypes.byref(rg(g)), fp(refv), fp(war), ip(mask), ctypes.c_float(sigma),
+                                ctypes.byref(v)))
+    return v.value
+
+
+def ref_lncc_grad(g, refv, war, wgrad, mask=None, sigma=-5.0, weight=1.0, init=None):
+    grad = np.zeros(3 * g.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    rcheck(ref().ref_lncc_gradient(ctypes.byref(rg(g)), fp(refv), fp(war), fp(wgrad), ip(mask), ctypes.c_float(sigma),
+                                   ctypes.c_double(weight), fp(grad)))
+    return grad
