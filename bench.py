@@ -63,6 +63,19 @@ def cpu_baseline(ref, flo, geom):
                       f"OMP_NUM_THREADS={threads}, after one warm-up eval"}
 
 
+def measured_traffic():
+    """HBM bytes per gradient-sweep launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (scaled by the copy-kernel calibration recorded there), or None."""
+    path = os.path.join(ROOT, "profiles", "f3d_eval_256x256x160_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return int(line.split(":")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -132,7 +145,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "k_fused_pass<float,true> (gradient sweep)",
+            "traffic": measured_traffic() if not storage else None,
+            "traffic_source": "profiles/f3d_eval_256x256x160_hbm_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+            "kernel": "k_fused_pass<float,true> (gradient sweep)",
             "kernel_ms": grad_ms, "histogram_sweep_ms": hist_ms, "algorithmic_bytes_per_launch": algo_bytes,
         },
         "cpu_baseline": None,

@@ -77,7 +77,12 @@ typedef struct nrh_f3d_params {
     float be_weight;          /* bending-energy weight of the current level */
     float le_weight;          /* linear-energy weight; must be 0 (not implemented) */
     int image_storage;        /* 0: float32 images in HBM, 1: float16 storage, float32 compute */
+    int measure;              /* NRH_MEASURE_NMI or NRH_MEASURE_LNCC (reg-lib/MeasureCreator.hpp:8) */
+    float lncc_sigma;         /* LNCC Gaussian standard deviation, < 0: voxels (reg_lncc default -5) */
 } nrh_f3d_params;
+
+#define NRH_MEASURE_NMI 0
+#define NRH_MEASURE_LNCC 1
 
 /* F3dContent + DefContent + Content construction (reg-lib/F3dContent.cpp:4-19,
  * reg-lib/DefContent.cpp:5-15, reg-lib/Content.cpp:5-22) and the device upload of
@@ -105,6 +110,14 @@ int nrh_f3d_image_gradient(nrh_f3d *h);                    /* Compute::GetImageG
 int nrh_f3d_nmi_value(nrh_f3d *h, double *nmi, double entropies[4]); /* reg_nmi::GetSimilarityMeasureValueFw */
 int nrh_f3d_zero_voxel_gradient(nrh_f3d *h);               /* DefContent::ZeroVoxelBasedMeasureGradient */
 int nrh_f3d_nmi_voxel_gradient(nrh_f3d *h, double weight); /* reg_nmi::GetVoxelBasedSimilarityMeasureGradientFw */
+/* reg_lncc (reg-lib/cpu/_reg_lncc.cpp): GetSimilarityMeasureValueFw (:303-345, the local
+ * statistics of UpdateLocalStatImages :195-245 and the mean |local correlation| of
+ * reg_getLnccValue :247-300) on the handle's warped image, and
+ * GetVoxelBasedSimilarityMeasureGradientFw (:514-577, reg_getVoxelBasedLnccGradient :393-512):
+ * voxel gradient -= ... with the given time-point weight.  The images must already be rescaled
+ * to [0, 1] (reg_lncc::InitialiseMeasure, :84-105). */
+int nrh_f3d_lncc_value(nrh_f3d *h, double *value);
+int nrh_f3d_lncc_voxel_gradient(nrh_f3d *h, double weight);
 int nrh_f3d_convolve_voxel_gradient(nrh_f3d *h);           /* Compute::ConvolveImage (private) */
 int nrh_f3d_voxel_to_node(nrh_f3d *h, float weight);       /* Compute::VoxelCentricToNodeCentric */
 int nrh_f3d_zero_node_gradient(nrh_f3d *h);                /* F3dContent::ZeroTransformationGradient */
@@ -170,6 +183,8 @@ typedef struct nrh_f3d_config {
     float padding;         /* -pad (NaN) */
     int use_pyramid;       /* 1 unless -nopy */
     int image_storage;     /* 0 float32, 1 float16 */
+    int measure;           /* NRH_MEASURE_NMI (--nmi) or NRH_MEASURE_LNCC (--lncc) */
+    float lncc_sigma;      /* --lncc value: Gaussian standard deviation, < 0 voxels (-5) */
 } nrh_f3d_config;
 int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                      const nrh_f3d_config *cfg, nrh_geom *grid_out, float *cpp_out, size_t cpp_capacity,

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnrhip.so")
 
-SOURCES = ["capi.cpp", "capi_eval.cpp", "f3d_ops.hip", "f3d_fused.hip", "driver.cpp", "aladin.hip"]
+SOURCES = ["capi.cpp", "capi_eval.cpp", "f3d_ops.hip", "f3d_fused.hip", "driver.cpp", "aladin.hip", "lncc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NRH_OFFLOAD_ARCH", "gfx950")

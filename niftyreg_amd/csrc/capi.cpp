@@ -113,6 +113,8 @@ void check_params(const nrh_f3d_params *p) {
                 "bin counts must lie in [5, 260]");
     NRH_REQUIRE(p->le_weight == 0.f, NRH_ERR_UNSUPPORTED, "the linear-energy penalty is not implemented (use -le 0)");
     NRH_REQUIRE(p->image_storage == 0 || p->image_storage == 1, NRH_ERR_ARG, "image_storage must be 0 or 1");
+    NRH_REQUIRE(p->measure == NRH_MEASURE_NMI || p->measure == NRH_MEASURE_LNCC, NRH_ERR_ARG, "unknown measure");
+    NRH_REQUIRE(p->measure != NRH_MEASURE_LNCC || p->lncc_sigma != 0.f, NRH_ERR_ARG, "LNCC needs a non-zero sigma");
 }
 
 }  // namespace nrh

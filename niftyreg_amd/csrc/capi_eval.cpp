@@ -28,9 +28,20 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
     return guard([&] {
         NRH_REQUIRE(h && value, NRH_ERR_ARG, "null argument");
         const nrh_f3d_params& p = h->params;
-        if (p.similarity_weight > 0) histogram_at_current_points(h);
+        const bool lncc = p.measure == NRH_MEASURE_LNCC;
+        if (p.similarity_weight > 0) {
+            if (lncc) {
+                // reg_f3d::WarpFloatingImage then reg_lncc::GetSimilarityMeasureValueFw
+                op_spline_field(h);
+                op_resample(h);
+                op_lncc_value(h, h->scalars.p + 4);
+            } else {
+                histogram_at_current_points(h);
+            }
+        }
         if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
-        NRH_HIP(hipMemcpyAsync(h->pinned, h->ent.p, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        NRH_HIP(hipMemcpyAsync(h->pinned, lncc ? h->scalars.p : h->ent.p, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream));
         NRH_HIP(hipMemcpyAsync(h->pinned + 5, h->scalars.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         NRH_HIP(hipStreamSynchronize(h->stream));
         collect_sweep_times(h);
@@ -49,7 +60,18 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
     return guard([&] {
         NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
         const nrh_f3d_params& p = h->params;
-        if (p.similarity_weight > 0) {
+        if (p.similarity_weight > 0 && p.measure == NRH_MEASURE_LNCC) {
+            // reg_f3d::GetSimilarityMeasureGradient with reg_lncc (reg-lib/_reg_f3d.cpp:353-362,
+            // reg-lib/_reg_base.cpp:497-549)
+            op_spline_field(h);
+            op_resample(h);
+            op_image_gradient(h);
+            ensure_voxel_grad(h);
+            NRH_HIP(hipMemsetAsync(h->voxelGrad.p, 0, h->voxelGrad.bytes(), h->stream));
+            op_lncc_voxel_gradient(h, 1.0);
+            op_convolve_voxel_gradient(h);
+            op_voxel_to_node(h, p.similarity_weight);
+        } else if (p.similarity_weight > 0) {
             if (fused_supported(h)) {
                 if (h->histVersion != h->cppVersion) fused_histogram(h);
                 fused_gradient(h, p.similarity_weight);

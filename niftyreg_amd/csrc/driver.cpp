@@ -281,12 +281,16 @@ void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& f
         p.be_weight = be;
         p.le_weight = 0.f;
         p.image_storage = cfg.image_storage;
+        p.measure = cfg.measure;
+        p.lncc_sigma = cfg.lncc_sigma;
         Handle H;
         check(nrh_f3d_create(&lr, rp[idx].data.data(), &lf, fp[idx].data.data(), nullptr, &grid, &p, &H.h));
         nrh_f3d *h = H.h;
         check(nrh_f3d_set_control_points(h, cpp.data()));
-        // reg_nmi::InitialiseMeasure: rescale to [2, bins - 3] (reg-lib/cpu/_reg_nmi.cpp:115-129)
-        check(nrh_f3d_rescale_intensities(h, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
+        // reg_nmi::InitialiseMeasure rescales to [2, bins - 3] (reg-lib/cpu/_reg_nmi.cpp:115-129),
+        // reg_lncc::InitialiseMeasure to [0, 1] (reg-lib/cpu/_reg_lncc.cpp:99-104)
+        if (p.measure == NRH_MEASURE_LNCC) check(nrh_f3d_rescale_intensities(h, 0.f, 1.f, 0.f, 1.f));
+        else check(nrh_f3d_rescale_intensities(h, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
         // Optimiser::Initialise (reg-lib/Optimiser.cpp:48-90)
         check(nrh_f3d_store_best(h));
         double best = 0, terms[3];

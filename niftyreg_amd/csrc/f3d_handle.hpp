@@ -39,6 +39,13 @@ struct nrh_f3d {
     // fused node-gradient partial sums (one window per tile)
     nrh::DevBuf<float> nodePartials;
 
+    // LNCC: two ping-pong sets of planar numerators + density, combined mask, reductions
+    nrh::DevBuf<float> lnccA, lnccB, lnccKern;
+    nrh::DevBuf<uint8_t> lnccComb;
+    nrh::DevBuf<double> lnccPart, lnccScalars;  // scalars: adjusted weight, -, value, active count
+    float lnccSigma = 0.f;
+    int lnccRadius[3] = { 0, 0, 0 }, lnccKernOff[3] = { 0, 0, 0 };
+
     double *pinned = nullptr;  // host-pinned scalar read-back
 
     uint64_t cppVersion = 1;   // bumped on every control-point change
@@ -82,6 +89,10 @@ void op_fill_f32(float *p, size_t n, float v, hipStream_t s);
 void op_cg(nrh_f3d *h, int first);
 void op_update_dof(nrh_f3d *h, float scale);
 double read_scalar(nrh_f3d *h, const double *dev);
+
+// LNCC measure (lncc.hip); value = mean |local correlation| into out_device[0] (count in [1])
+void op_lncc_value(nrh_f3d *h, double *out_device);
+void op_lncc_voxel_gradient(nrh_f3d *h, double weight);
 
 float node_weight_ratio(const nrh_f3d *h, float weight);
 void op_normalise_device(nrh_f3d *h);

@@ -13,8 +13,11 @@ from . import _lib
 from .geometry import Geom
 
 
-def default_params(bins=64, padding=float("nan"), similarity_weight=0.999, be_weight=0.001, image_storage=0):
+def default_params(bins=64, padding=float("nan"), similarity_weight=0.999, be_weight=0.001, image_storage=0,
+                   measure=_lib.MEASURE_NMI, lncc_sigma=-5.0):
     p = _lib.nrh_f3d_params()
+    p.measure = measure
+    p.lncc_sigma = lncc_sigma
     p.ref_bins = bins + 4          # the reference adds 4 bins for the spline support
     p.flo_bins = bins + 4
     p.padding = padding
@@ -100,6 +103,16 @@ class F3dContent:
         ent = np.zeros(4, np.float64)
         self._op("nrh_f3d_nmi_value", ctypes.byref(nmi), _lib.dptr(ent))
         return nmi.value, ent
+
+    def GetLnccValue(self):
+        """reg_lncc::GetSimilarityMeasureValueFw on the current warped image."""
+        v = ctypes.c_double()
+        self._op("nrh_f3d_lncc_value", ctypes.byref(v))
+        return v.value
+
+    def GetVoxelBasedLnccGradient(self, weight=1.0):
+        """reg_lncc::GetVoxelBasedSimilarityMeasureGradientFw (voxel gradient -= ...)."""
+        self._op("nrh_f3d_lncc_voxel_gradient", ctypes.c_double(weight))
 
     def ZeroVoxelBasedMeasureGradient(self):
         self._op("nrh_f3d_zero_voxel_gradient")
@@ -213,14 +226,16 @@ class F3dContent:
 
 
 def register(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3,
-             levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True, image_storage=0):
+             levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True, image_storage=0,
+             measure=_lib.MEASURE_NMI, lncc_sigma=-5.0):
     """Full reg_f3d registration of one pair on the device (reg_base<float>::Run).
 
-    Returns (grid Geom, control points planar float32, final NMI, iterations)."""
+    Returns (grid Geom, control points planar float32, final similarity value, iterations)."""
     cfg = _lib.nrh_f3d_config()
     cfg.bins, cfg.be_weight, cfg.le_weight, cfg.spacing = bins, be, le, spacing
     cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
     cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), image_storage
+    cfg.measure, cfg.lncc_sigma = measure, lncc_sigma
     ref = np.ascontiguousarray(ref, np.float32).reshape(-1)
     flo = np.ascontiguousarray(flo, np.float32).reshape(-1)
     rg, fg = ref_geom.to_c(), flo_geom.to_c()

@@ -425,12 +425,18 @@ typedef struct {
     int maxit;           // final-level iteration budget
     int use_pyramid;
     float padding;       // warped padding value
+    int measure;         // 0 NMI, 1 LNCC
+    float lncc_sigma;    // LNCC kernel standard deviation (< 0: voxels)
 } ref_f3d_params;
 
 static void configure(RefF3d& r, const ref_f3d_params *p) {
     r.SetPlatformType(PlatformType::Cpu);
-    r.UseNMISetReferenceBinNumber(0, p->bins);
-    r.UseNMISetFloatingBinNumber(0, p->bins);
+    if (p->measure == 1) {
+        r.UseLNCC(0, p->lncc_sigma);
+    } else {
+        r.UseNMISetReferenceBinNumber(0, p->bins);
+        r.UseNMISetFloatingBinNumber(0, p->bins);
+    }
     r.SetBendingEnergyWeight(p->be);
     r.SetLinearEnergyWeight(p->le);
     r.SetJacobianLogWeight(0);

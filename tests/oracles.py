@@ -48,7 +48,8 @@ class nr_geom(ctypes.Structure):
 class ref_f3d_params(ctypes.Structure):
     _fields_ = [("bins", ctypes.c_int), ("be", ctypes.c_float), ("le", ctypes.c_float), ("spacing", ctypes.c_float),
                 ("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("maxit", ctypes.c_int),
-                ("use_pyramid", ctypes.c_int), ("padding", ctypes.c_float)]
+                ("use_pyramid", ctypes.c_int), ("padding", ctypes.c_float), ("measure", ctypes.c_int),
+                ("lncc_sigma", ctypes.c_float)]
 
 
 def og(g):
@@ -289,10 +290,11 @@ class RefF3d:
     """reg_f3d<float> prepared at one level for single objective / gradient evaluations."""
 
     def __init__(self, refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=1,
-                 level=0, maxit=150, padding=float("nan")):
+                 level=0, maxit=150, padding=float("nan"), measure=0, lncc_sigma=-5.0):
         p = ref_f3d_params()
         p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
         p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
+        p.measure, p.lncc_sigma = measure, lncc_sigma
         self.p = p
         r = ref()
         r.ref_f3d_create.argtypes = [ctypes.c_void_p] * 6
@@ -385,11 +387,12 @@ def fp64_field(refg, cppg, cpp):
 
 
 def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3, maxit=150,
-                padding=float("nan")):
+                padding=float("nan"), measure=0, lncc_sigma=-5.0):
     """reg_f3d<float>::Run on the reference CPU build; returns (grid geom, control points)."""
     p = ref_f3d_params()
     p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
     p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
+    p.measure, p.lncc_sigma = measure, lncc_sigma
     out = nr_geom()
     sp = [abs(spacing) * d if spacing < 0 else spacing for d in refg.pixdim]
     cap = 3 * int(np.prod([int(np.ceil(n * d / s + 4)) for n, d, s in zip(refg.dims, refg.pixdim, sp)]))
