@@ -510,3 +510,32 @@ def ref_lncc_grad(g, refv, war, wgrad, mask=None, sigma=-5.0, weight=1.0, init=N
     rcheck(ref().ref_lncc_gradient(ctypes.byref(rg(g)), fp(refv), fp(war), fp(wgrad), ip(mask), ctypes.c_float(sigma),
                                    ctypes.c_double(weight), fp(grad)))
     return grad
+
+
+# ---------------------------------------------------------------- HIP platform plugin check
+PLUGIN = os.path.join(ROOT, "oracle", "_ref", "libnrplugin.so")
+_plugin = None
+
+
+def plugin_available():
+    return os.path.exists(PLUGIN)
+
+
+def plugin_f3d_eval(hip, refg, ref_img, flog, flo_img, cppg, cpp, bins=64, lncc=False, simw=0.999, bew=0.001):
+    """One objective value + gradient through the reference's own Content / Compute / Measure
+    calls, on the reference CPU classes (hip=False) or the HIP platform plugin (hip=True)."""
+    global _plugin
+    if _plugin is None:
+        _plugin = ctypes.CDLL(PLUGIN)
+        _plugin.plugin_last_error.restype = ctypes.c_char_p
+    m, be = ctypes.c_double(), ctypes.c_double()
+    grad = np.zeros(3 * cppg.nvox, np.float32)
+    vg = np.zeros(3 * refg.nvox, np.float32)
+    rc = _plugin.plugin_f3d_eval(int(hip), ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).ravel()),
+                                 ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).ravel()),
+                                 ctypes.byref(rg(cppg)), fp(np.ascontiguousarray(cpp, np.float32)), bins, int(lncc),
+                                 ctypes.c_float(simw), ctypes.c_float(bew), ctypes.byref(m), ctypes.byref(be),
+                                 fp(grad), fp(vg))
+    if rc != 0:
+        raise RuntimeError(_plugin.plugin_last_error().decode())
+    return m.value, be.value, grad, vg

@@ -1,0 +1,59 @@
+"""The HIP platform plugin (niftyreg_amd/plugin) against the reference's own classes.
+
+oracle/_ref/libnrplugin.so drives the calls reg_f3d makes for one objective value and one
+gradient through either the reference's CPU Content / Compute / Measure classes or the plugin's
+HIP classes.  On the CPU side this pins the driver sequence against the oracle chain; on the GPU
+the two platforms must agree: the per-operation device kernels follow the scalar reference
+build, so the gradients are bit-exact and the values agree to FP64 reduction order (1e-12).
+"""
+import numpy as np
+import pytest
+
+import oracles as O
+from niftyreg_amd import phantoms
+from niftyreg_amd.geometry import Geom, create_control_point_grid
+
+needs_plugin = pytest.mark.skipif(not O.plugin_available(), reason="plugin check library not built")
+
+
+def _case(dims=(40, 36, 32), seed=0, geom=None):
+    g = geom if geom is not None else Geom.identity(dims)
+    ref = phantoms.blob_phantom(g.dims, seed).ravel()
+    flo = phantoms.blob_phantom(g.dims, seed + 3).ravel()
+    cg, cpp = create_control_point_grid(g, [5.0 * p for p in g.pixdim])
+    cpp = (cpp + np.random.default_rng(seed).uniform(-2, 2, cpp.size)).astype(np.float32)
+    return g, ref, flo, cg, cpp
+
+
+@needs_plugin
+def test_reference_classes_path_matches_oracle_chain():
+    g, ref, flo, cg, cpp = _case()
+    m, be, grad, _ = O.plugin_f3d_eval(False, g, ref, g, flo, cg, cpp)
+    r, f = O.orc_rescale(ref, 2, 65), O.orc_rescale(flo, 2, 65)
+    want, nmi, _ = O.orc_node_gradient_chain(g, g, cg, r, f, cpp, 68, 68, np.float32(0.999), np.float32(0.001))
+    assert m == nmi
+    assert np.array_equal(grad, want)
+    assert abs(be - O.orc_be(cg, cpp)) <= 1e-13 * be
+
+
+def _oblique(dims):
+    th = 0.2
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    m = np.eye(4, dtype=np.float32)
+    m[:3, :3] = (R @ np.diag([1.1, 0.9, 1.2])).astype(np.float32)
+    m[:3, 3] = [4.0, -6.0, 2.5]
+    return Geom(dims, m)
+
+
+@needs_plugin
+@pytest.mark.gpu
+@pytest.mark.parametrize("lncc", [False, True])
+@pytest.mark.parametrize("which", ["cube", "oblique"])
+def test_hip_platform_matches_reference_platform(gpu, lncc, which):
+    g, ref, flo, cg, cpp = _case(geom=_oblique((34, 30, 28)) if which == "oblique" else None)
+    cm, cbe, cgrad, cvg = O.plugin_f3d_eval(False, g, ref, g, flo, cg, cpp, lncc=lncc)
+    hm, hbe, hgrad, hvg = O.plugin_f3d_eval(True, g, ref, g, flo, cg, cpp, lncc=lncc)
+    assert abs(hm - cm) <= 1e-12 * abs(cm)
+    assert abs(hbe - cbe) <= 1e-12 * abs(cbe)
+    assert np.array_equal(hvg, cvg, equal_nan=True)
+    assert np.array_equal(hgrad, cgrad)
