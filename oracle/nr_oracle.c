@@ -1138,3 +1138,210 @@ void orc_lncc_gradient(const orc_geom *g, const float *ref, const float *war, co
         if (grad[i] != grad[i] || isinf(grad[i])) grad[i] = 0;
     free(mean); free(sdev); free(wmean); free(wsdev); free(corr); free(comb);
 }
+
+/* ------------------------------------------------------------------ linear energy */
+/* 3x3 float matrices with the arithmetic of niftilib's nifti_mat33_* helpers
+ * (reg-io/niftilib/nifti1_io.c:1807-1960): double intermediates where those use double
+ * (including the long double reciprocal of the determinant), float products and sums elsewhere. */
+typedef struct { float m[3][3]; } orc_m33;
+
+static orc_m33 m33_inverse(orc_m33 R) {
+    double r11 = R.m[0][0], r12 = R.m[0][1], r13 = R.m[0][2];
+    double r21 = R.m[1][0], r22 = R.m[1][1], r23 = R.m[1][2];
+    double r31 = R.m[2][0], r32 = R.m[2][1], r33 = R.m[2][2];
+    double deti = r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 - r31 * r22 * r13;
+    if (deti != 0.0L) deti = 1.0L / deti;
+    orc_m33 Q;
+    Q.m[0][0] = (float)(deti * (r22 * r33 - r32 * r23));
+    Q.m[0][1] = (float)(deti * (-r12 * r33 + r32 * r13));
+    Q.m[0][2] = (float)(deti * (r12 * r23 - r22 * r13));
+    Q.m[1][0] = (float)(deti * (-r21 * r33 + r31 * r23));
+    Q.m[1][1] = (float)(deti * (r11 * r33 - r31 * r13));
+    Q.m[1][2] = (float)(deti * (-r11 * r23 + r21 * r13));
+    Q.m[2][0] = (float)(deti * (r21 * r32 - r31 * r22));
+    Q.m[2][1] = (float)(deti * (-r11 * r32 + r31 * r12));
+    Q.m[2][2] = (float)(deti * (r11 * r22 - r21 * r12));
+    return Q;
+}
+
+static float m33_determ(orc_m33 R) {
+    double r11 = R.m[0][0], r12 = R.m[0][1], r13 = R.m[0][2];
+    double r21 = R.m[1][0], r22 = R.m[1][1], r23 = R.m[1][2];
+    double r31 = R.m[2][0], r32 = R.m[2][1], r33 = R.m[2][2];
+    return (float)(r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 - r31 * r22 * r13);
+}
+
+static float m33_rownorm(orc_m33 A) {
+    float r1 = (float)(fabs(A.m[0][0]) + fabs(A.m[0][1]) + fabs(A.m[0][2]));
+    float r2 = (float)(fabs(A.m[1][0]) + fabs(A.m[1][1]) + fabs(A.m[1][2]));
+    float r3 = (float)(fabs(A.m[2][0]) + fabs(A.m[2][1]) + fabs(A.m[2][2]));
+    if (r1 < r2) r1 = r2;
+    if (r1 < r3) r1 = r3;
+    return r1;
+}
+
+static float m33_colnorm(orc_m33 A) {
+    float r1 = (float)(fabs(A.m[0][0]) + fabs(A.m[1][0]) + fabs(A.m[2][0]));
+    float r2 = (float)(fabs(A.m[0][1]) + fabs(A.m[1][1]) + fabs(A.m[2][1]));
+    float r3 = (float)(fabs(A.m[0][2]) + fabs(A.m[1][2]) + fabs(A.m[2][2]));
+    if (r1 < r2) r1 = r2;
+    if (r1 < r3) r1 = r3;
+    return r1;
+}
+
+static orc_m33 m33_mul(orc_m33 A, orc_m33 B) {
+    orc_m33 C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C.m[i][j] = A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j] + A.m[i][2] * B.m[2][j];
+    return C;
+}
+
+/* Closest orthogonal matrix by Higham's scaled Newton iteration, as nifti_mat33_polar. */
+static orc_m33 m33_polar(orc_m33 A) {
+    orc_m33 X = A, Y, Z;
+    float alp, bet, gam, gmi, dif = 1.0f;
+    int k = 0;
+    gam = m33_determ(X);
+    while (gam == 0.0) {
+        gam = (float)(0.00001 * (0.001 + m33_rownorm(X)));
+        X.m[0][0] += gam;
+        X.m[1][1] += gam;
+        X.m[2][2] += gam;
+        gam = m33_determ(X);
+    }
+    for (;;) {
+        Y = m33_inverse(X);
+        if (dif > 0.3) {
+            alp = (float)(sqrt(m33_rownorm(X) * m33_colnorm(X)));
+            bet = (float)(sqrt(m33_rownorm(Y) * m33_colnorm(Y)));
+            gam = (float)(sqrt(bet / alp));
+            gmi = (float)(1.0 / gam);
+        } else {
+            gam = gmi = 1.0f;
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Z.m[i][j] = (float)(0.5 * (gam * X.m[i][j] + gmi * Y.m[j][i]));
+        double d = 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) d += fabs(Z.m[i][j] - X.m[i][j]);
+        dif = (float)d;
+        k = k + 1;
+        if (k > 100 || dif < 3.e-6) break;
+        X = Z;
+    }
+    return Z;
+}
+
+/* Six-significant-digit literal of a double (the reference's tables of first-order basis
+ * products are written that way). */
+static float sig6(double v) {
+    if (v == 0) return 0.f;
+    const double a = fabs(v);
+    const int e = (int)floor(log10(a));
+    const double scale = pow(10.0, 5 - e);
+    const double r = floor(a * scale + 0.5) / scale;
+    return (float)(v < 0 ? -r : r);
+}
+
+/* set_first_order_basis_values 3-D (reg-lib/cpu/_reg_splineBasis.cpp:182-290): tensor products of
+ * the cubic B-spline first derivative {-1/2, 0, 1/2} and value {1/6, 2/3, 1/6} at a node. */
+void orc_first_order_basis(float bx[27], float by[27], float bz[27]) {
+    const double v[3] = { 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0 }, d1[3] = { -0.5, 0.0, 0.5 };
+    int i = 0;
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a, ++i) {
+                bx[i] = sig6(d1[a] * v[b] * v[c]);
+                by[i] = sig6(v[a] * d1[b] * v[c]);
+                bz[i] = sig6(v[a] * v[b] * d1[c]);
+            }
+}
+
+/* Rotation-free displacement Jacobian at an interior node (reg-lib/cpu/_reg_localTrans_regul.cpp:
+ * 596-634): first-order basis contraction of the 27 positions, mm -> grid voxel, polar rotation
+ * removed, identity subtracted. */
+static orc_m33 le_matrix(const orc_geom *cpp, const float *d, int x, int y, int z, const float *bx, const float *by,
+                         const float *bz, orc_m33 reo) {
+    const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    orc_m33 M;
+    memset(&M, 0, sizeof M);
+    int i = 0;
+    for (int c = -1; c < 2; ++c)
+        for (int b = -1; b < 2; ++b)
+            for (int a = -1; a < 2; ++a, ++i) {
+                const size_t idx = ((size_t)(z + c) * cpp->ny + y + b) * cpp->nx + x + a;
+                const float cx = d[idx], cy = d[idx + g], cz = d[idx + 2 * g];
+                M.m[0][0] += bx[i] * cx;
+                M.m[1][0] += by[i] * cx;
+                M.m[2][0] += bz[i] * cx;
+                M.m[0][1] += bx[i] * cy;
+                M.m[1][1] += by[i] * cy;
+                M.m[2][1] += bz[i] * cy;
+                M.m[0][2] += bx[i] * cz;
+                M.m[1][2] += by[i] * cz;
+                M.m[2][2] += bz[i] * cz;
+            }
+    M = m33_mul(reo, M);
+    const orc_m33 R = m33_inverse(m33_polar(M));
+    M = m33_mul(R, M);
+    M.m[0][0]--;
+    M.m[1][1]--;
+    M.m[2][2]--;
+    return M;
+}
+
+static orc_m33 grid_reorientation(const orc_geom *cpp) {
+    orc_m33 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r.m[i][j] = cpp->ijk[i * 4 + j];
+    return r;
+}
+
+/* reg_spline_approxLinearEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:559-647); sums in
+ * node order. */
+double orc_linear_energy(const orc_geom *cpp, const float *cppData) {
+    float bx[27], by[27], bz[27];
+    orc_first_order_basis(bx, by, bz);
+    const orc_m33 reo = grid_reorientation(cpp);
+    double total = 0;
+    for (int z = 1; z < cpp->nz - 1; ++z)
+        for (int y = 1; y < cpp->ny - 1; ++y)
+            for (int x = 1; x < cpp->nx - 1; ++x) {
+                const orc_m33 M = le_matrix(cpp, cppData, x, y, z, bx, by, bz, reo);
+                double v = 0;
+                for (int b = 0; b < 3; ++b)
+                    for (int a = 0; a < 3; ++a) {
+                        const double s = 0.5 * (M.m[a][b] + M.m[b][a]);
+                        v += s * s;
+                    }
+                total += v;
+            }
+    return total / (double)(3 * (size_t)cpp->nx * cpp->ny * cpp->nz);
+}
+
+/* reg_spline_approxLinearEnergyGradient3D (reg-lib/cpu/_reg_localTrans_regul.cpp:739-824); grad += */
+void orc_linear_energy_gradient(const orc_geom *cpp, const float *cppData, float weight, float *grad) {
+    float bx[27], by[27], bz[27];
+    orc_first_order_basis(bx, by, bz);
+    const orc_m33 reo = grid_reorientation(cpp);
+    const orc_m33 inv = m33_inverse(reo);
+    const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    const float ratio = weight / (float)g;
+    for (int z = 1; z < cpp->nz - 1; ++z)
+        for (int y = 1; y < cpp->ny - 1; ++y)
+            for (int x = 1; x < cpp->nx - 1; ++x) {
+                const orc_m33 M = le_matrix(cpp, cppData, x, y, z, bx, by, bz, reo);
+                int i = 26;
+                for (int c = -1; c < 2; ++c)
+                    for (int b = -1; b < 2; ++b)
+                        for (int a = -1; a < 2; ++a, --i) {
+                            const size_t idx = ((size_t)(z + c) * cpp->ny + y + b) * cpp->nx + x + a;
+                            const float gv0 = -2.f * M.m[0][0] * bx[i];
+                            const float gv1 = -2.f * M.m[1][1] * by[i];
+                            const float gv2 = -2.f * M.m[2][2] * bz[i];
+                            grad[idx] += ratio * (inv.m[0][0] * gv0 + inv.m[0][1] * gv1 + inv.m[0][2] * gv2);
+                            grad[idx + g] += ratio * (inv.m[1][0] * gv0 + inv.m[1][1] * gv1 + inv.m[1][2] * gv2);
+                            grad[idx + 2 * g] += ratio * (inv.m[2][0] * gv0 + inv.m[2][1] * gv1 + inv.m[2][2] * gv2);
+                        }
+            }
+}

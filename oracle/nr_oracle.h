@@ -122,6 +122,15 @@ double orc_lncc_value(const orc_geom *g, const float *ref, const float *war, con
 void orc_lncc_gradient(const orc_geom *g, const float *ref, const float *war, const float *warGrad, const int *mask,
                        float sigma, double weight, float *grad);
 
+
+/* ---- linear energy ---------------------------------------------------------------------- */
+/* set_first_order_basis_values 3-D (reg-lib/cpu/_reg_splineBasis.cpp:182-290) */
+void orc_first_order_basis(float bx[27], float by[27], float bz[27]);
+/* reg_spline_approxLinearEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:559-647) */
+double orc_linear_energy(const orc_geom *cpp, const float *cppData);
+/* reg_spline_approxLinearEnergyGradient3D (reg-lib/cpu/_reg_localTrans_regul.cpp:739-824); grad += */
+void orc_linear_energy_gradient(const orc_geom *cpp, const float *cppData, float weight, float *grad);
+
 #ifdef __cplusplus
 }
 #endif

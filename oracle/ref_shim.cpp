@@ -323,6 +323,7 @@ public:
     double WLE() { return this->currentWLE; }
     float SimWeight() { return this->similarityWeight; }
     float BEWeight() { return this->bendingEnergyWeight; }
+    float LEWeight() { return this->linearEnergyWeight; }
     size_t Iterations() { return this->optimiser ? this->optimiser->GetCurrentIterationNumber() : 0; }
 };
 
@@ -454,7 +455,7 @@ int ref_f3d_gradient(void *hp, int normalise, double *maxLength, float *grad, fl
 
 int ref_f3d_weights(void *hp, float *w) {
     F3dHandle *h = static_cast<F3dHandle*>(hp);
-    w[0] = h->reg->SimWeight(); w[1] = h->reg->BEWeight();
+    w[0] = h->reg->SimWeight(); w[1] = h->reg->BEWeight(); w[2] = h->reg->LEWeight();
     return 0;
 }
 

@@ -316,10 +316,11 @@ class RefF3d:
         rcheck(ref().ref_f3d_level_images(self.h, fp(a), fp(b)))
         return a, b
 
-    def weights(self):
-        w = np.zeros(2, np.float32)
+    def weights(self, with_le=False):
+        """(similarity, bending-energy[, linear-energy]) weights of the prepared level."""
+        w = np.zeros(3, np.float32)
         rcheck(ref().ref_f3d_weights(self.h, fp(w)))
-        return float(w[0]), float(w[1])
+        return (float(w[0]), float(w[1]), float(w[2])) if with_le else (float(w[0]), float(w[1]))
 
     def get_cpp(self):
         c = np.zeros(3 * self.G, np.float32)
@@ -539,3 +540,27 @@ def plugin_f3d_eval(hip, refg, ref_img, flog, flo_img, cppg, cpp, bins=64, lncc=
     if rc != 0:
         raise RuntimeError(_plugin.plugin_last_error().decode())
     return m.value, be.value, grad, vg
+
+
+# ---------------------------------------------------------------- linear energy
+def orc_le(cppg, cpp):
+    orc().orc_linear_energy.restype = ctypes.c_double
+    return orc().orc_linear_energy(ctypes.byref(og(cppg)), fp(cpp))
+
+
+def orc_le_grad(cppg, cpp, weight, init=None):
+    g = np.zeros(3 * cppg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    orc().orc_linear_energy_gradient(ctypes.byref(og(cppg)), fp(cpp), ctypes.c_float(weight), fp(g))
+    return g
+
+
+def ref_le(cppg, cpp):
+    v = ctypes.c_double()
+    rcheck(ref().ref_le_value(ctypes.byref(rg(cppg)), fp(cpp), ctypes.byref(v)))
+    return v.value
+
+
+def ref_le_grad(cppg, cpp, weight, init=None):
+    g = np.zeros(3 * cppg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
+    rcheck(ref().ref_le_gradient(ctypes.byref(rg(cppg)), fp(cpp), ctypes.c_float(weight), fp(g)))
+    return g
