@@ -20,13 +20,14 @@ void check_geom(const nrh_geom *g, const char *what) {
 }
 
 // The inverse of an affine 4x4 with the adjugate expressions of nifti_mat44_inverse
-// (reg-io/niftilib/nifti1_io.c:1680-1717), in double, stored in float.
+// (reg-io/niftilib/nifti1_io.c:1680-1717), in double with the determinant's reciprocal in long
+// double (as niftilib), stored in float.
 void mat44_inverse(const float *R, float *Q) {
     const double r11 = R[0], r12 = R[1], r13 = R[2], v1 = R[3];
     const double r21 = R[4], r22 = R[5], r23 = R[6], v2 = R[7];
     const double r31 = R[8], r32 = R[9], r33 = R[10], v3 = R[11];
     double deti = r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 - r31 * r22 * r13;
-    if (deti != 0.0) deti = 1.0 / deti;
+    if (deti != 0.0) deti = (double)(1.0L / (long double)deti);
     Q[0] = (float)(deti * (r22 * r33 - r32 * r23));
     Q[1] = (float)(deti * (-r12 * r33 + r32 * r13));
     Q[2] = (float)(deti * (r12 * r23 - r22 * r13));
@@ -111,7 +112,7 @@ void check_params(const nrh_f3d_params *p) {
     NRH_REQUIRE(p != nullptr, NRH_ERR_ARG, "null parameters");
     NRH_REQUIRE(p->ref_bins >= 5 && p->flo_bins >= 5 && p->ref_bins <= 260 && p->flo_bins <= 260, NRH_ERR_ARG,
                 "bin counts must lie in [5, 260]");
-    NRH_REQUIRE(p->le_weight == 0.f, NRH_ERR_UNSUPPORTED, "the linear-energy penalty is not implemented (use -le 0)");
+    NRH_REQUIRE(p->le_weight >= 0.f && p->be_weight >= 0.f, NRH_ERR_ARG, "penalty weights must be >= 0");
     NRH_REQUIRE(p->image_storage == 0 || p->image_storage == 1, NRH_ERR_ARG, "image_storage must be 0 or 1");
     NRH_REQUIRE(p->measure == NRH_MEASURE_NMI || p->measure == NRH_MEASURE_LNCC, NRH_ERR_ARG, "unknown measure");
     NRH_REQUIRE(p->measure != NRH_MEASURE_LNCC || p->lncc_sigma != 0.f, NRH_ERR_ARG, "LNCC needs a non-zero sigma");
@@ -201,6 +202,7 @@ int nrh_f3d_create(const nrh_geom *ref, const float *ref_data, const nrh_geom *f
         h->ent.alloc(8);
         h->scalars.alloc(16);
         h->maxBits.alloc(8);
+        NRH_HIP(hipMemsetAsync(h->maxBits.p, 0, h->maxBits.bytes(), h->stream));  // [6]: BE reduction ticket
         h->partials.alloc(4096);
         NRH_HIP(hipStreamSynchronize(h->stream));
         *out = h.release();
@@ -309,6 +311,16 @@ int nrh_f3d_nmi_value(nrh_f3d *h, double *nmi, double entropies[4]) {
         if (entropies) for (int i = 0; i < 4; ++i) entropies[i] = h->pinned[i];
     });
 }
+
+int nrh_f3d_linear_energy(nrh_f3d *h, double *value) {
+    return guard([&] {
+        NRH_REQUIRE(h && value, NRH_ERR_ARG, "null argument");
+        op_linear_energy(h, h->scalars.p + 8);
+        *value = read_scalar(h, h->scalars.p + 8);
+    });
+}
+
+NRH_HANDLE_OP(nrh_f3d_linear_energy_gradient(nrh_f3d *h, float weight), op_linear_energy_gradient(h, weight))
 
 int nrh_f3d_bending_energy(nrh_f3d *h, double *value) {
     return guard([&] {

@@ -40,18 +40,21 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
             }
         }
         if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
+        if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
         NRH_HIP(hipMemcpyAsync(h->pinned, lncc ? h->scalars.p : h->ent.p, 5 * sizeof(double), hipMemcpyDeviceToHost,
                                h->stream));
         NRH_HIP(hipMemcpyAsync(h->pinned + 5, h->scalars.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        NRH_HIP(hipMemcpyAsync(h->pinned + 6, h->scalars.p + 9, sizeof(double), hipMemcpyDeviceToHost, h->stream));
         NRH_HIP(hipStreamSynchronize(h->stream));
         collect_sweep_times(h);
         const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * h->pinned[4] : 0.0;
         const double wBE = p.be_weight > 0 ? (double)p.be_weight * h->pinned[5] : 0.0;
-        *value = wMeasure - wBE;
+        const double wLE = p.le_weight > 0 ? (double)p.le_weight * h->pinned[6] : 0.0;
+        *value = wMeasure - wBE - wLE;
         if (terms) {
             terms[0] = wMeasure;
             terms[1] = wBE;
-            terms[2] = 0.0;
+            terms[2] = wLE;
         }
     });
 }
@@ -92,10 +95,13 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
         } else {
             op_fill_f32(h->nodeGrad.p, 3 * h->G, 0.f, h->stream);
         }
-        if (p.be_weight > 0) op_bending_energy_gradient(h, p.be_weight);
+        // reg_f3d::GetObjectiveFunctionGradient: bending energy, then linear energy
+        const bool be = p.be_weight > 0, le = p.le_weight > 0;
+        if (be) op_bending_energy_gradient(h, p.be_weight, max_length != nullptr && !le);
+        if (le) op_linear_energy_gradient(h, p.le_weight, max_length != nullptr);
         if (max_length) {
-            op_max_length(h);
-            op_normalise_device(h);
+            if (!be && !le) op_max_length_bits(h);
+            op_normalise_from_bits(h);
             *max_length = read_scalar(h, h->scalars.p + 1);
         } else {
             NRH_HIP(hipStreamSynchronize(h->stream));

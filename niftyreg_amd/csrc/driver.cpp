@@ -215,18 +215,18 @@ void check(int rc) {
 void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& floG, const float *floData,
                    const nrh_f3d_config& cfg, nrh_geom& gridOut, std::vector<float>& cppOut, double& finalNmi,
                    int& totalIterations) {
-    NRH_REQUIRE(cfg.le_weight == 0.f, NRH_ERR_UNSUPPORTED, "the linear-energy penalty is not implemented (use -le 0)");
     NRH_REQUIRE(cfg.levels >= 1 && cfg.bins >= 1 && cfg.max_iterations >= 0, NRH_ERR_ARG, "invalid configuration");
     const int levels = cfg.levels;
     int toPerform = cfg.levels_to_perform > 0 ? std::min(cfg.levels_to_perform, levels) : levels;
     if (!cfg.use_pyramid) toPerform = levels;
     // reg_f3d::CheckParameters: weights of the objective (reg-lib/_reg_f3d.cpp:142-159)
-    float be = cfg.be_weight;
-    const float penalty = be + cfg.le_weight;
+    float be = cfg.be_weight, le = cfg.le_weight;
+    const float penalty = be + le;
     float simW;
     if (penalty >= 1) {
         simW = 0;
         be /= penalty;
+        le /= penalty;
     } else {
         simW = 1 - penalty;
     }
@@ -265,8 +265,10 @@ void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& f
         // reg_f3d::InitCurrentLevel (reg-lib/_reg_f3d.cpp:92-133)
         if (level == 0) {
             be = be / std::pow(16.f, (float)(levels - 1));
+            le = le / std::pow(3.f, (float)(levels - 1));
         } else {
             be = be * 16;
+            le = le * 3;
             nrh_geom ng;
             std::vector<float> nd;
             refine_grid(lr, grid, cpp, ng, nd);
@@ -279,7 +281,7 @@ void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& f
         p.padding = cfg.padding;
         p.similarity_weight = simW;
         p.be_weight = be;
-        p.le_weight = 0.f;
+        p.le_weight = le;
         p.image_storage = cfg.image_storage;
         p.measure = cfg.measure;
         p.lncc_sigma = cfg.lncc_sigma;

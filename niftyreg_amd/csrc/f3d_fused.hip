@@ -382,16 +382,34 @@ __global__ void __launch_bounds__(256) k_node_reduce(ReduceArgs r) {
     float u0 = 0.f, u1 = 0.f, u2 = 0.f;
     if (inv != 0.f) {
         const int2 ax = r.tx[i], ay = r.ty[j], az = r.tz[k];
-        for (int c = az.x; c <= az.y; ++c)
-            for (int b = ay.x; b <= ay.y; ++b)
-                for (int a = ax.x; a <= ax.y; ++a) {
+        const int na = ax.y - ax.x + 1, nab = na * (ay.y - ay.x + 1);
+        const int cnt = nab * (az.y - az.x + 1);
+        // windows in (z, y, x) tile order, loaded four at a time so the loads overlap; the sums
+        // keep that order
+        for (int q0 = 0; q0 < cnt; q0 += 4) {
+            float v[4][3];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + u;
+                if (q < cnt) {
+                    const int a = ax.x + q % na, b = ay.x + (q % nab) / na, c = az.x + q / nab;
                     const size_t tile = ((size_t)c * r.tilesY + b) * r.tilesX + a;
                     const int wi = ((k - r.wz0[c]) * WYM + (j - r.wy0[b])) * WXM + (i - r.wx0[a]);
                     const float *p = r.partials + tile * 3 * WIN + wi;
-                    u0 += p[0];
-                    u1 += p[WIN];
-                    u2 += p[2 * WIN];
+                    v[u][0] = p[0];
+                    v[u][1] = p[WIN];
+                    v[u][2] = p[2 * WIN];
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (q0 + u < cnt) {
+                    u0 += v[u][0];
+                    u1 += v[u][1];
+                    u2 += v[u][2];
+                }
+            }
+        }
         u0 *= inv;
         u1 *= inv;
         u2 *= inv;

@@ -24,6 +24,8 @@ struct nrh_f3d {
     nrh::DevBuf<float> convTmp, convDens, convDensTmp, convKernel;
     nrh::DevBuf<uint8_t> convNan;
     nrh::DevBuf<float> beDeriv;
+    nrh::DevBuf<float> leDiag;      // per-node rotation-free displacement Jacobian diagonal [3][G]
+    uint64_t leDiagVersion = 0;     // cppVersion leDiag was computed at (0: none)
 
     // joint histogram: integer counts, reg_nmi's Pro / Log arrays, entropies
     nrh::DevBuf<uint32_t> histCount;
@@ -80,8 +82,13 @@ void op_nmi_voxel_gradient(nrh_f3d *h, double weight);
 void op_convolve_voxel_gradient(nrh_f3d *h);
 void op_voxel_to_node(nrh_f3d *h, float weight);
 void op_bending_energy(nrh_f3d *h, double *out_device);
-void op_bending_energy_gradient(nrh_f3d *h, float weight);
+// trackMax: also leave the max length of the final gradient in maxBits[0] (float bits)
+void op_bending_energy_gradient(nrh_f3d *h, float weight, bool trackMax = false);
 void op_max_length(nrh_f3d *h);
+// linear energy (value into out_device, / 3G; keepDiag marks the per-node diagonal as current) and
+// its gradient (grad +=, optionally tracking the final max length in maxBits[0])
+void op_linear_energy(nrh_f3d *h, double *out_device, bool keepDiag = true);
+void op_linear_energy_gradient(nrh_f3d *h, float weight, bool trackMax = false);
 void op_normalise(nrh_f3d *h, double max_length);
 void op_rescale(nrh_f3d *h, float *data, size_t n, float lo, float hi);
 void op_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
@@ -96,6 +103,8 @@ void op_lncc_voxel_gradient(nrh_f3d *h, double weight);
 
 float node_weight_ratio(const nrh_f3d *h, float weight);
 void op_normalise_device(nrh_f3d *h);
+void op_max_length_bits(nrh_f3d *h);     // max length of nodeGrad into maxBits[0]
+void op_normalise_from_bits(nrh_f3d *h);  // normalise by maxBits[0]; scalars[1] = the length
 
 // fused evaluation (f3d_fused.hip)
 bool fused_supported(nrh_f3d *h);

@@ -5,6 +5,7 @@
 // The hot loop of a registration uses the fused kernels of f3d_fused.hip instead.
 
 #include "device.hpp"
+#include "recip_x87.hpp"
 #include "f3d_handle.hpp"
 #include <cmath>
 
@@ -223,6 +224,111 @@ k_nmi_finalize(const uint32_t *__restrict__ counts, int rb, int fb, double *__re
     }
 }
 
+// The same finalisation with the working array in LDS (histograms of up to kFinLds entries):
+// identical arithmetic and thread partition, no global-memory round trips between phases.
+constexpr int kFinLds = 7936;
+constexpr int kFinPer = (kFinLds + 1023) / 1024;
+
+__global__ void __launch_bounds__(1024)
+k_nmi_finalize_lds(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent) {
+    __shared__ double P[kFinLds];
+    __shared__ double red[16];
+    __shared__ double totalS;
+    const int nb = rb * fb, total = nb + rb + fb;
+    const double k0 = 1.0 / 6.0, k1 = (double)(2.f / 3.f);
+    for (int i = threadIdx.x; i < total; i += blockDim.x) P[i] = i < nb ? (double)counts[i] : 0.0;
+    __syncthreads();
+    double v[kFinPer];
+    // along the reference axis
+#pragma unroll
+    for (int q = 0; q < kFinPer; ++q) {
+        const int i = threadIdx.x + q * 1024;
+        v[q] = 0;
+        if (i < nb) {
+            const int r = i % rb;
+            double t = 0;
+            if (r - 1 > -1) t += P[i - 1] * k0;
+            t += P[i] * k1;
+            if (r + 1 < rb) t += P[i + 1] * k0;
+            v[q] = t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kFinPer; ++q) {
+        const int i = threadIdx.x + q * 1024;
+        if (i < nb) P[i] = v[q];
+    }
+    __syncthreads();
+    // along the floating axis
+#pragma unroll
+    for (int q = 0; q < kFinPer; ++q) {
+        const int i = threadIdx.x + q * 1024;
+        v[q] = 0;
+        if (i < nb) {
+            const int f = i / rb;
+            double t = 0;
+            if (f - 1 > -1) t += P[i - rb] * k0;
+            t += P[i] * k1;
+            if (f + 1 < fb) t += P[i + rb] * k0;
+            v[q] = t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kFinPer; ++q) {
+        const int i = threadIdx.x + q * 1024;
+        if (i < nb) P[i] = v[q];
+    }
+    __syncthreads();
+    double s = 0;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) s += P[i];
+    s = block_sum_double(s, red);
+    if (threadIdx.x == 0) totalS = s;
+    __syncthreads();
+    const double act = totalS;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) P[i] /= act;
+    __syncthreads();
+    for (int i = threadIdx.x; i < rb + fb; i += blockDim.x) {
+        double m = 0;
+        if (i < rb) {
+            for (int f = 0; f < fb; ++f) m += P[i + rb * f];
+        } else {
+            const int f = i - rb;
+            for (int r = 0; r < rb; ++r) m += P[rb * f + r];
+        }
+        P[nb + i] = m;
+    }
+    __syncthreads();
+    double hr = 0, hf = 0, hj = 0;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const double p = P[i];
+        double l = 0;
+        if (p > 0) {
+            l = log(p);
+            if (i < nb) hj -= p * l;
+            else if (i < nb + rb) hr -= p * l;
+            else hf -= p * l;
+        }
+        pro[i] = p;
+        logh[i] = l;
+        logf[i] = (float)l;
+    }
+    hr = block_sum_double(hr, red);
+    __syncthreads();
+    hf = block_sum_double(hf, red);
+    __syncthreads();
+    hj = block_sum_double(hj, red);
+    if (threadIdx.x == 0) {
+        ent[0] = hr;
+        ent[1] = hf;
+        ent[2] = hj;
+        ent[3] = act;
+        ent[4] = (hr + hf) / hj;
+    }
+}
+
 // reg_getVoxelBasedNmiGradient3d<float> (reg-lib/cpu/_reg_nmi.cpp:501-593); grad +=
 __global__ void __launch_bounds__(kBlock)
 k_nmi_voxel_gradient(const float *__restrict__ ref, const float *__restrict__ war, const float *__restrict__ wg,
@@ -358,9 +464,11 @@ struct BeBasis {
 // reg_spline_approxBendingEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:73-153): per-block
 // FP64 partial sums over interior nodes.
 __global__ void __launch_bounds__(kBlock)
-k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BeBasis B, double *__restrict__ partials) {
+k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BeBasis B, double *__restrict__ partials,
+           uint32_t *__restrict__ ticket, double scale, double *__restrict__ out) {
 #pragma clang fp contract(off)
     __shared__ double red[16];
+    __shared__ bool last;
     const size_t G = (size_t)gnx * gny * gnz;
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0;
@@ -389,7 +497,22 @@ k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BeBasis B, 
         }
     }
     s = block_sum_double(s, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    // the last block to finish sums the per-block partials in block order
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) t += partials[i];
+    t = block_sum_double(t, red);
+    if (threadIdx.x == 0) {
+        *out = t * scale;
+        *ticket = 0u;
+    }
 }
 
 __global__ void __launch_bounds__(1024)
@@ -410,10 +533,12 @@ __device__ __forceinline__ float node_pos(const Affine& m, int r, int x, int y, 
 // First half of reg_spline_approxBendingEnergyGradient3D (regul.cpp:317-379): 18 second
 // derivatives of the displacement at every node.
 __global__ void __launch_bounds__(kBlock)
-k_be_deriv(const float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz, BeBasis B, float *__restrict__ der) {
+k_be_deriv(const float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz, BeBasis B, float *__restrict__ der,
+           uint32_t *__restrict__ maxBits) {
 #pragma clang fp contract(off)
     const size_t G = (size_t)gnx * gny * gnz;
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n == 0 && maxBits) *maxBits = 0u;  // reset for k_be_grad's max-length tracking
     if (n >= G) return;
     const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
     float d[3][6] = {};
@@ -437,26 +562,271 @@ k_be_deriv(const float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz,
 }
 
 // Second half (regul.cpp:381-448): adjoint 27-point gather, grad += weight / G * value.
+// The reference converts the control points to displacements in place and back around the
+// computation (regul.cpp:312, 450), so every point becomes (p - node) + node in float; the
+// kernel applies the same round trip to its node (k_be_deriv, the only reader, has finished).
 __global__ void __launch_bounds__(kBlock)
-k_be_grad(const float *__restrict__ der, int gnx, int gny, int gnz, BeBasis B, float ratio, float *__restrict__ grad) {
+k_be_grad(const float *__restrict__ der, int gnx, int gny, int gnz, BeBasis B, float ratio, float *__restrict__ grad,
+          uint32_t *__restrict__ maxBits, float *__restrict__ cpp, Affine xyz) {
 #pragma clang fp contract(off)
     const size_t G = (size_t)gnx * gny * gnz;
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= G) return;
-    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
-    float gv[3] = { 0.f, 0.f, 0.f };
-    int a = 0;
-    for (int Z = z - 1; Z < z + 2; ++Z)
-        for (int Y = y - 1; Y < y + 2; ++Y)
-            for (int X = x - 1; X < x + 2; ++X, ++a) {
-                if (X < 0 || Y < 0 || Z < 0 || X >= gnx || Y >= gny || Z >= gnz) continue;
-                const float *dp = der + 18 * (((size_t)Z * gny + Y) * gnx + X);
-                for (int t = 0; t < 6; ++t)
-                    for (int k = 0; k < 3; ++k) gv[k] += dp[t * 3 + k] * B.b[t][a];
+    const bool valid = n < G;
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    if (valid) {
+        const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+        float gv[3] = { 0.f, 0.f, 0.f };
+        int a = 0;
+        for (int Z = z - 1; Z < z + 2; ++Z)
+            for (int Y = y - 1; Y < y + 2; ++Y)
+                for (int X = x - 1; X < x + 2; ++X, ++a) {
+                    if (X < 0 || Y < 0 || Z < 0 || X >= gnx || Y >= gny || Z >= gnz) continue;
+                    const float *dp = der + 18 * (((size_t)Z * gny + Y) * gnx + X);
+                    for (int t = 0; t < 6; ++t)
+                        for (int k = 0; k < 3; ++k) gv[k] += dp[t * 3 + k] * B.b[t][a];
+                }
+        g0 = grad[n] + ratio * gv[0];
+        g1 = grad[n + G] + ratio * gv[1];
+        g2 = grad[n + 2 * G] + ratio * gv[2];
+        grad[n] = g0;
+        grad[n + G] = g1;
+        grad[n + 2 * G] = g2;
+        for (int k = 0; k < 3; ++k) {
+            const float node = node_pos(xyz, k, x, y, z);
+            cpp[n + k * G] = (cpp[n + k * G] - node) + node;
+        }
+    }
+    if (maxBits) {
+        // reg_getMaximalLength of the final gradient (lengths >= 0 order like their bit patterns)
+        uint32_t b = valid ? __float_as_uint(sqrtf(g0 * g0 + g1 * g1 + g2 * g2)) : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = __shfl_down(b, off, 64);
+            b = o > b ? o : b;
+        }
+        if ((threadIdx.x & 63) == 0) atomicMax(maxBits, b);
+    }
+}
+
+// ------------------------------------------------------------------ linear energy
+// 3x3 float matrices with niftilib's nifti_mat33_* arithmetic (reg-io/niftilib/nifti1_io.c:
+// 1807-1960): double intermediates where those use double, float products and sums elsewhere.
+// niftilib takes the determinant's reciprocal in long double (80-bit on x86-64); recip_x87
+// reproduces that rounding in double arithmetic.
+struct M33 {
+    float m[3][3];
+};
+
+__device__ __forceinline__ M33 m33_inverse(const M33& R) {
+#pragma clang fp contract(off)
+    const double r11 = R.m[0][0], r12 = R.m[0][1], r13 = R.m[0][2];
+    const double r21 = R.m[1][0], r22 = R.m[1][1], r23 = R.m[1][2];
+    const double r31 = R.m[2][0], r32 = R.m[2][1], r33 = R.m[2][2];
+    double deti = r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 -
+                  r31 * r22 * r13;
+    if (deti != 0.0) deti = recip_x87(deti);  // niftilib's 1.0l / deti
+    M33 Q;
+    Q.m[0][0] = (float)(deti * (r22 * r33 - r32 * r23));
+    Q.m[0][1] = (float)(deti * (-r12 * r33 + r32 * r13));
+    Q.m[0][2] = (float)(deti * (r12 * r23 - r22 * r13));
+    Q.m[1][0] = (float)(deti * (-r21 * r33 + r31 * r23));
+    Q.m[1][1] = (float)(deti * (r11 * r33 - r31 * r13));
+    Q.m[1][2] = (float)(deti * (-r11 * r23 + r21 * r13));
+    Q.m[2][0] = (float)(deti * (r21 * r32 - r31 * r22));
+    Q.m[2][1] = (float)(deti * (-r11 * r32 + r31 * r12));
+    Q.m[2][2] = (float)(deti * (r11 * r22 - r21 * r12));
+    return Q;
+}
+
+__device__ __forceinline__ float m33_determ(const M33& R) {
+#pragma clang fp contract(off)
+    const double r11 = R.m[0][0], r12 = R.m[0][1], r13 = R.m[0][2];
+    const double r21 = R.m[1][0], r22 = R.m[1][1], r23 = R.m[1][2];
+    const double r31 = R.m[2][0], r32 = R.m[2][1], r33 = R.m[2][2];
+    return (float)(r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 -
+                   r31 * r22 * r13);
+}
+
+__device__ __forceinline__ float m33_rownorm(const M33& A) {
+#pragma clang fp contract(off)
+    float r = 0.f;
+    for (int i = 0; i < 3; ++i) {
+        const float v = (float)(fabs((double)A.m[i][0]) + fabs((double)A.m[i][1]) + fabs((double)A.m[i][2]));
+        if (r < v || i == 0) r = v;
+    }
+    return r;
+}
+
+__device__ __forceinline__ float m33_colnorm(const M33& A) {
+#pragma clang fp contract(off)
+    float r = 0.f;
+    for (int j = 0; j < 3; ++j) {
+        const float v = (float)(fabs((double)A.m[0][j]) + fabs((double)A.m[1][j]) + fabs((double)A.m[2][j]));
+        if (r < v || j == 0) r = v;
+    }
+    return r;
+}
+
+__device__ __forceinline__ M33 m33_mul(const M33& A, const M33& B) {
+#pragma clang fp contract(off)
+    M33 C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C.m[i][j] = A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j] + A.m[i][2] * B.m[2][j];
+    return C;
+}
+
+// Higham's scaled Newton iteration for the closest orthogonal matrix (nifti_mat33_polar).
+__device__ M33 m33_polar(const M33& A) {
+#pragma clang fp contract(off)
+    M33 X = A, Y, Z;
+    float gam = m33_determ(X), gmi, dif = 1.0f;
+    int k = 0;
+    for (int guard = 0; gam == 0.0f && guard < 1000; ++guard) {
+        gam = (float)(0.00001 * (0.001 + (double)m33_rownorm(X)));
+        X.m[0][0] += gam;
+        X.m[1][1] += gam;
+        X.m[2][2] += gam;
+        gam = m33_determ(X);
+    }
+    for (;;) {
+        Y = m33_inverse(X);
+        if (dif > 0.3) {
+            const float alp = (float)sqrt((double)(m33_rownorm(X) * m33_colnorm(X)));
+            const float bet = (float)sqrt((double)(m33_rownorm(Y) * m33_colnorm(Y)));
+            gam = (float)sqrt((double)(bet / alp));
+            gmi = (float)(1.0 / (double)gam);
+        } else {
+            gam = gmi = 1.0f;
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Z.m[i][j] = (float)(0.5 * (double)(gam * X.m[i][j] + gmi * Y.m[j][i]));
+        double d = 0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) d += fabs((double)(Z.m[i][j] - X.m[i][j]));
+        dif = (float)d;
+        k = k + 1;
+        if (k > 100 || dif < 3.e-6) break;
+        X = Z;
+    }
+    return Z;
+}
+
+struct LeBasis {
+    float b[3][27];  // first-order tensor products along x, y, z
+};
+
+// Rotation-free displacement Jacobian at an interior node (reg-lib/cpu/_reg_localTrans_regul.cpp:596-634).
+__device__ M33 le_matrix(const float *__restrict__ cpp, size_t G, int gnx, int gny, int x, int y, int z,
+                         const LeBasis& B, const M33& reo) {
+#pragma clang fp contract(off)
+    M33 M = {};
+    int i = 0;
+    for (int c = -1; c < 2; ++c)
+        for (int b = -1; b < 2; ++b)
+            for (int a = -1; a < 2; ++a, ++i) {
+                const size_t idx = ((size_t)(z + c) * gny + y + b) * gnx + x + a;
+                const float cv[3] = { cpp[idx], cpp[idx + G], cpp[idx + 2 * G] };
+                for (int k = 0; k < 3; ++k) {
+                    M.m[0][k] += B.b[0][i] * cv[k];
+                    M.m[1][k] += B.b[1][i] * cv[k];
+                    M.m[2][k] += B.b[2][i] * cv[k];
+                }
             }
-    grad[n] += ratio * gv[0];
-    grad[n + G] += ratio * gv[1];
-    grad[n + 2 * G] += ratio * gv[2];
+    M = m33_mul(reo, M);
+    const M33 R = m33_inverse(m33_polar(M));
+    M = m33_mul(R, M);
+    M.m[0][0] -= 1.f;
+    M.m[1][1] -= 1.f;
+    M.m[2][2] -= 1.f;
+    return M;
+}
+
+// Per-node terms of reg_spline_approxLinearEnergyValue3D (:559-647) and the diagonal the gradient
+// uses; the last block to finish sums the block partials in block order into *out (/ 3G).
+__global__ void __launch_bounds__(kBlock)
+k_le_terms(const float *__restrict__ cpp, int gnx, int gny, int gnz, LeBasis B, M33 reo, float *__restrict__ diag,
+           double *__restrict__ partials, uint32_t *__restrict__ ticket, double scale, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double red[16];
+    __shared__ bool last;
+    const size_t G = (size_t)gnx * gny * gnz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0;
+    if (n < G) {
+        const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+        float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+        if (x > 0 && y > 0 && z > 0 && x < gnx - 1 && y < gny - 1 && z < gnz - 1) {
+            const M33 M = le_matrix(cpp, G, gnx, gny, x, y, z, B, reo);
+            for (int b = 0; b < 3; ++b)
+                for (int a = 0; a < 3; ++a) {
+                    const double v = 0.5 * (double)(M.m[a][b] + M.m[b][a]);
+                    s += v * v;
+                }
+            d0 = M.m[0][0];
+            d1 = M.m[1][1];
+            d2 = M.m[2][2];
+        }
+        diag[n] = d0;
+        diag[n + G] = d1;
+        diag[n + 2 * G] = d2;
+    }
+    s = block_sum_double(s, red);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) t += partials[i];
+    t = block_sum_double(t, red);
+    if (threadIdx.x == 0) {
+        *out = t * scale;
+        *ticket = 0u;
+    }
+}
+
+// reg_spline_approxLinearEnergyGradient3D (:739-824) as a gather: each node adds the
+// contributions of the interior nodes around it in the reference's source order, so every sum
+// runs in the same order as the reference's scatter.  Optionally tracks the final max length.
+__global__ void __launch_bounds__(kBlock)
+k_le_grad(const float *__restrict__ diag, int gnx, int gny, int gnz, LeBasis B, M33 inv, float ratio,
+          float *__restrict__ grad, uint32_t *__restrict__ maxBits) {
+#pragma clang fp contract(off)
+    const size_t G = (size_t)gnx * gny * gnz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = n < G;
+    const size_t m = valid ? n : 0;
+    const int x = (int)(m % gnx), y = (int)((m / gnx) % gny), z = (int)(m / ((size_t)gnx * gny));
+    float g0 = grad[m], g1 = grad[m + G], g2 = grad[m + 2 * G];
+    for (int c = 1; c > -2; --c)
+        for (int b = 1; b > -2; --b)
+            for (int a = 1; a > -2; --a) {
+                const int X = x - a, Y = y - b, Z = z - c;  // the source node
+                if (X < 1 || Y < 1 || Z < 1 || X > gnx - 2 || Y > gny - 2 || Z > gnz - 2) continue;
+                const size_t s = ((size_t)Z * gny + Y) * gnx + X;
+                const int i = 26 - ((c + 1) * 9 + (b + 1) * 3 + (a + 1));
+                const float v0 = -2.f * diag[s] * B.b[0][i];
+                const float v1 = -2.f * diag[s + G] * B.b[1][i];
+                const float v2 = -2.f * diag[s + 2 * G] * B.b[2][i];
+                g0 += ratio * (inv.m[0][0] * v0 + inv.m[0][1] * v1 + inv.m[0][2] * v2);
+                g1 += ratio * (inv.m[1][0] * v0 + inv.m[1][1] * v1 + inv.m[1][2] * v2);
+                g2 += ratio * (inv.m[2][0] * v0 + inv.m[2][1] * v1 + inv.m[2][2] * v2);
+            }
+    if (valid) {
+        grad[n] = g0;
+        grad[n + G] = g1;
+        grad[n + 2 * G] = g2;
+    }
+    if (maxBits) {
+        uint32_t bits = valid ? __float_as_uint(sqrtf(g0 * g0 + g1 * g1 + g2 * g2)) : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = __shfl_down(bits, off, 64);
+            bits = o > bits ? o : bits;
+        }
+        if ((threadIdx.x & 63) == 0) atomicMax(maxBits, bits);
+    }
 }
 
 // ------------------------------------------------------------------ optimiser helpers
@@ -488,6 +858,15 @@ __global__ void __launch_bounds__(kBlock)
 k_normalise(float *__restrict__ g, size_t n, double inv) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) g[i] = (float)((double)g[i] * inv);
+}
+
+// Normalisation by the max length held as float bits; block 0 also publishes it as a double.
+__global__ void __launch_bounds__(kBlock)
+k_normalise_bits(float *__restrict__ g, size_t n, const uint32_t *__restrict__ bits, double *__restrict__ maxLen) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double m = (double)__uint_as_float(*bits);
+    if (i == 0) *maxLen = m;
+    if (i < n && m != 0.0) g[i] = (float)((double)g[i] * (1.0 / m));
 }
 
 // Normalisation by a device-resident max length (fused gradient path): skips when it is 0.
@@ -702,8 +1081,12 @@ void op_joint_histogram(nrh_f3d *h, const float *warped) {
 }
 
 void op_nmi_finalize(nrh_f3d *h) {
-    hipLaunchKernelGGL(k_nmi_finalize, dim3(1), dim3(1024), 0, h->stream, h->histCount.p, h->params.ref_bins,
-                       h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);
+    if (h->totalBins() <= kFinLds)
+        hipLaunchKernelGGL(k_nmi_finalize_lds, dim3(1), dim3(1024), 0, h->stream, h->histCount.p,
+                           h->params.ref_bins, h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);
+    else
+        hipLaunchKernelGGL(k_nmi_finalize, dim3(1), dim3(1024), 0, h->stream, h->histCount.p, h->params.ref_bins,
+                           h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);
     NRH_HIP(hipGetLastError());
 }
 
@@ -802,20 +1185,102 @@ void op_bending_energy(nrh_f3d *h, double *out_device) {
     const unsigned nb = blocks_for(h->G);
     h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)nb + 64));
     hipLaunchKernelGGL(k_be_value, dim3(nb), dim3(kBlock), 0, h->stream, h->cppPos.p, h->cpp.nx, h->cpp.ny, h->cpp.nz,
-                       B, h->partials.p);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, h->partials.p, (int)nb,
-                       1.0 / (double)(3 * h->G), out_device);
+                       B, h->partials.p, h->maxBits.p + 6, 1.0 / (double)(3 * h->G), out_device);
     NRH_HIP(hipGetLastError());
 }
 
-void op_bending_energy_gradient(nrh_f3d *h, float weight) {
+void op_bending_energy_gradient(nrh_f3d *h, float weight, bool trackMax) {
     static const BeBasis B = make_be_basis();
     h->beDeriv.alloc(18 * h->G);
+    uint32_t *mb = trackMax ? h->maxBits.p : nullptr;
     hipLaunchKernelGGL(k_be_deriv, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->cppPos.p, h->cpp.nx,
-                       h->cpp.ny, h->cpp.nz, to_affine(h->cpp.xyz), B, h->beDeriv.p);
+                       h->cpp.ny, h->cpp.nz, to_affine(h->cpp.xyz), B, h->beDeriv.p, mb);
     const float ratio = weight / (float)h->G;
     hipLaunchKernelGGL(k_be_grad, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->beDeriv.p, h->cpp.nx,
-                       h->cpp.ny, h->cpp.nz, B, ratio, h->nodeGrad.p);
+                       h->cpp.ny, h->cpp.nz, B, ratio, h->nodeGrad.p, mb, h->cppPos.p, to_affine(h->cpp.xyz));
+    NRH_HIP(hipGetLastError());
+    h->cppVersion++;  // the round trip may have changed the last bit of some points
+}
+
+namespace {
+
+float sig6(double v) {
+    if (v == 0) return 0.f;
+    const double a = std::fabs(v);
+    const int e = (int)std::floor(std::log10(a));
+    const double scale = std::pow(10.0, 5 - e);
+    const double r = std::floor(a * scale + 0.5) / scale;
+    return (float)(v < 0 ? -r : r);
+}
+
+// set_first_order_basis_values 3-D (reg-lib/cpu/_reg_splineBasis.cpp:182-290): the reference writes
+// the tensor products of the B-spline first derivative {-1/2, 0, 1/2} and value {1/6, 2/3, 1/6} as
+// 6-significant-digit literals; they are regenerated here.
+LeBasis make_le_basis() {
+    const double v[3] = { 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0 }, d1[3] = { -0.5, 0.0, 0.5 };
+    LeBasis B;
+    int i = 0;
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a, ++i) {
+                B.b[0][i] = sig6(d1[a] * v[b] * v[c]);
+                B.b[1][i] = sig6(v[a] * d1[b] * v[c]);
+                B.b[2][i] = sig6(v[a] * v[b] * d1[c]);
+            }
+    return B;
+}
+
+M33 grid_reorientation(const nrh_f3d *h) {
+    M33 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r.m[i][j] = h->cpp.ijk[i * 4 + j];
+    return r;
+}
+
+// nifti_mat33_inverse on the host, with the long double reciprocal of niftilib.
+M33 host_m33_inverse(const M33& R) {
+    const double r11 = R.m[0][0], r12 = R.m[0][1], r13 = R.m[0][2];
+    const double r21 = R.m[1][0], r22 = R.m[1][1], r23 = R.m[1][2];
+    const double r31 = R.m[2][0], r32 = R.m[2][1], r33 = R.m[2][2];
+    double deti = r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 -
+                  r31 * r22 * r13;
+    if (deti != 0.0L) deti = (double)(1.0L / (long double)deti);
+    M33 Q;
+    Q.m[0][0] = (float)(deti * (r22 * r33 - r32 * r23));
+    Q.m[0][1] = (float)(deti * (-r12 * r33 + r32 * r13));
+    Q.m[0][2] = (float)(deti * (r12 * r23 - r22 * r13));
+    Q.m[1][0] = (float)(deti * (-r21 * r33 + r31 * r23));
+    Q.m[1][1] = (float)(deti * (r11 * r33 - r31 * r13));
+    Q.m[1][2] = (float)(deti * (-r11 * r23 + r21 * r13));
+    Q.m[2][0] = (float)(deti * (r21 * r32 - r31 * r22));
+    Q.m[2][1] = (float)(deti * (-r11 * r32 + r31 * r12));
+    Q.m[2][2] = (float)(deti * (r11 * r22 - r21 * r12));
+    return Q;
+}
+
+}  // namespace
+
+void op_linear_energy(nrh_f3d *h, double *out_device, bool keepDiag) {
+    static const LeBasis B = make_le_basis();
+    const unsigned nb = blocks_for(h->G);
+    h->leDiag.alloc(3 * h->G);
+    h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)nb + 64));
+    hipLaunchKernelGGL(k_le_terms, dim3(nb), dim3(kBlock), 0, h->stream, h->cppPos.p, h->cpp.nx, h->cpp.ny, h->cpp.nz,
+                       B, grid_reorientation(h), h->leDiag.p, h->partials.p + nb + 32, h->maxBits.p + 7,
+                       1.0 / (double)(3 * h->G), out_device);
+    NRH_HIP(hipGetLastError());
+    h->leDiagVersion = keepDiag ? h->cppVersion : 0;
+}
+
+void op_linear_energy_gradient(nrh_f3d *h, float weight, bool trackMax) {
+    static const LeBasis B = make_le_basis();
+    if (h->leDiagVersion != h->cppVersion || h->leDiag.empty())
+        op_linear_energy(h, h->scalars.p + 7, true);  // the per-node diagonal of the current points
+    if (trackMax) op_fill_u32(h->maxBits.p, 1, 0u, h->stream);
+    const float ratio = weight / (float)h->G;
+    hipLaunchKernelGGL(k_le_grad, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->leDiag.p, h->cpp.nx,
+                       h->cpp.ny, h->cpp.nz, B, host_m33_inverse(grid_reorientation(h)), ratio, h->nodeGrad.p,
+                       trackMax ? h->maxBits.p : nullptr);
     NRH_HIP(hipGetLastError());
 }
 
@@ -831,6 +1296,19 @@ void op_normalise(nrh_f3d *h, double max_length) {
     if (max_length == 0) return;
     hipLaunchKernelGGL(k_normalise, dim3(blocks_for(3 * h->G)), dim3(kBlock), 0, h->stream, h->nodeGrad.p, 3 * h->G,
                        1.0 / max_length);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_normalise_from_bits(nrh_f3d *h) {
+    hipLaunchKernelGGL(k_normalise_bits, dim3(blocks_for(3 * h->G)), dim3(kBlock), 0, h->stream, h->nodeGrad.p,
+                       3 * h->G, (const uint32_t *)h->maxBits.p, h->scalars.p + 1);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_max_length_bits(nrh_f3d *h) {
+    op_fill_u32(h->maxBits.p, 1, 0u, h->stream);
+    const unsigned b = std::min(blocks_for(h->G), 1024u);
+    hipLaunchKernelGGL(k_max_length, dim3(b), dim3(kBlock), 0, h->stream, h->nodeGrad.p, h->G, h->maxBits.p);
     NRH_HIP(hipGetLastError());
 }
 

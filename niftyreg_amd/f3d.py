@@ -14,7 +14,7 @@ from .geometry import Geom
 
 
 def default_params(bins=64, padding=float("nan"), similarity_weight=0.999, be_weight=0.001, image_storage=0,
-                   measure=_lib.MEASURE_NMI, lncc_sigma=-5.0):
+                   measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, le_weight=0.0):
     p = _lib.nrh_f3d_params()
     p.measure = measure
     p.lncc_sigma = lncc_sigma
@@ -23,7 +23,7 @@ def default_params(bins=64, padding=float("nan"), similarity_weight=0.999, be_we
     p.padding = padding
     p.similarity_weight = similarity_weight
     p.be_weight = be_weight
-    p.le_weight = 0.0
+    p.le_weight = le_weight
     p.image_storage = image_storage
     return p
 
@@ -133,6 +133,14 @@ class F3dContent:
         v = ctypes.c_double()
         self._op("nrh_f3d_bending_energy", ctypes.byref(v))
         return v.value
+
+    def ApproxLinearEnergy(self):
+        v = ctypes.c_double()
+        self._op("nrh_f3d_linear_energy", ctypes.byref(v))
+        return v.value
+
+    def ApproxLinearEnergyGradient(self, weight):
+        self._op("nrh_f3d_linear_energy_gradient", ctypes.c_float(weight))
 
     def ApproxBendingEnergyGradient(self, weight):
         self._op("nrh_f3d_bending_energy_gradient", ctypes.c_float(weight))

@@ -207,6 +207,16 @@ void HipCompute::ApproxBendingEnergyGradient(float weight) {
     Check(nrh_f3d_bending_energy_gradient(Handle(), weight), "nrh_f3d_bending_energy_gradient");
 }
 
+double HipCompute::ApproxLinearEnergy() {
+    double v = 0;
+    Check(nrh_f3d_linear_energy(Handle(), &v), "nrh_f3d_linear_energy");
+    return v;
+}
+
+void HipCompute::ApproxLinearEnergyGradient(float weight) {
+    Check(nrh_f3d_linear_energy_gradient(Handle(), weight), "nrh_f3d_linear_energy_gradient");
+}
+
 void HipCompute::GetDeformationField(bool composition, bool bspline) {
     if (composition || !bspline) NR_FATAL_ERROR("Only the non-composed cubic B-spline field is supported on HIP");
     Check(nrh_f3d_deformation_field(Handle()), "nrh_f3d_deformation_field");
@@ -249,6 +259,31 @@ void HipCompute::VoxelCentricToNodeCentric(float weight) {
 void HipCompute::GetAffineDeformationField(bool) {
     NR_FATAL_ERROR("Affine fields of an F3D content are not computed on the HIP platform");
 }
+
+// Terms reg_f3d only calls when their weights are non-zero (-jl, -land, -smoothGrad, F3D2): the
+// device holds the current control points, so the base class's CPU code would read stale host
+// data; they fail instead.
+double HipCompute::GetJacobianPenaltyTerm(bool) { NR_FATAL_ERROR("The Jacobian penalty is not implemented on HIP"); return 0; }
+void HipCompute::JacobianPenaltyTermGradient(float, bool) { NR_FATAL_ERROR("The Jacobian penalty is not implemented on HIP"); }
+double HipCompute::CorrectFolding(bool) { NR_FATAL_ERROR("Folding correction is not implemented on HIP"); return 0; }
+double HipCompute::GetLandmarkDistance(size_t, float *, float *) {
+    NR_FATAL_ERROR("Landmark terms are not implemented on HIP");
+    return 0;
+}
+void HipCompute::LandmarkDistanceGradient(size_t, float *, float *, float) {
+    NR_FATAL_ERROR("Landmark terms are not implemented on HIP");
+}
+void HipCompute::SmoothGradient(float) { NR_FATAL_ERROR("Gradient smoothing is not implemented on HIP"); }
+void HipCompute::GetApproximatedGradient(InterfaceOptimiser&) {
+    NR_FATAL_ERROR("The finite-difference gradient is not implemented on HIP");
+}
+void HipCompute::GetDefFieldFromVelocityGrid(const bool) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
+void HipCompute::ExponentiateGradient(Content&) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
+void HipCompute::UpdateVelocityField(float, bool, bool, bool) {
+    NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP");
+}
+void HipCompute::BchUpdate(float, int) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
+void HipCompute::SymmetriseVelocityFields(Content&) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
 
 // ---------------------------------------------------------------- measures
 void reg_nmi_hip::InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, int *refMask, nifti_image *warpedImg,

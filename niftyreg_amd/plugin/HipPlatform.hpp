@@ -120,6 +120,8 @@ public:
     virtual void ResampleImage(int interpolation, float paddingValue) override;
     virtual double ApproxBendingEnergy() override;
     virtual void ApproxBendingEnergyGradient(float weight) override;
+    virtual double ApproxLinearEnergy() override;
+    virtual void ApproxLinearEnergyGradient(float weight) override;
     virtual void GetDeformationField(bool composition, bool bspline) override;
     virtual void UpdateControlPointPosition(float *currentDof, const float *bestDof, const float *gradient,
                                             const float scale, const bool optimiseX, const bool optimiseY,
@@ -129,6 +131,20 @@ public:
     virtual void NormaliseGradient(double maxGradLength, bool optimiseX, bool optimiseY, bool optimiseZ) override;
     virtual void ConvolveVoxelBasedMeasureGradient(float weight) override;
     virtual void GetAffineDeformationField(bool compose) override;
+    virtual double GetJacobianPenaltyTerm(bool approx) override;
+    virtual void JacobianPenaltyTermGradient(float weight, bool approx) override;
+    virtual double CorrectFolding(bool approx) override;
+    virtual double GetLandmarkDistance(size_t landmarkNumber, float *landmarkReference,
+                                       float *landmarkFloating) override;
+    virtual void LandmarkDistanceGradient(size_t landmarkNumber, float *landmarkReference, float *landmarkFloating,
+                                          float weight) override;
+    virtual void SmoothGradient(float sigma) override;
+    virtual void GetApproximatedGradient(InterfaceOptimiser& opt) override;
+    virtual void GetDefFieldFromVelocityGrid(const bool updateStepNumber) override;
+    virtual void ExponentiateGradient(Content& conBw) override;
+    virtual void UpdateVelocityField(float scale, bool optimiseX, bool optimiseY, bool optimiseZ) override;
+    virtual void BchUpdate(float scale, int bchUpdateValue) override;
+    virtual void SymmetriseVelocityFields(Content& conBw) override;
 
 protected:
     virtual void VoxelCentricToNodeCentric(float weight) override;

@@ -23,11 +23,12 @@ extern "C" {
 
 const char* plugin_last_error() { return plugin_error.c_str(); }
 
-// measure = similarity value, be = bending energy; grad = transformation gradient (3 G) after the
-// similarity and bending-energy terms; voxelGrad (3 N, may be NULL) = convolved voxel gradient.
+// measure = similarity value, be = bending energy + 1000 x linear energy (leW > 0); grad =
+// transformation gradient (3 G) after the similarity, bending-energy and linear-energy (weight
+// leW, skipped when 0) terms; voxelGrad (3 N, may be NULL) = convolved voxel gradient.
 int plugin_f3d_eval(int hip, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
                     const nr_geom *cpp, const float *cppData, int bins, int lncc, float simW, float beW,
-                    double *measure, double *be, float *grad, float *voxelGrad) {
+                    float leW, double *measure, double *be, float *grad, float *voxelGrad) {
     try {
         NiftiImage r = make_image(ref, refData), f = make_image(flo, floData), g = make_grid(cpp, cppData);
         std::unique_ptr<F3dContentCreator> cc(hip ? new HipF3dContentCreator() : new F3dContentCreator());
@@ -49,7 +50,7 @@ int plugin_f3d_eval(int hip, const nr_geom *ref, const float *refData, const nr_
         comp->GetDeformationField(false, true);
         comp->ResampleImage(1, pad);
         *measure = m->GetSimilarityMeasureValue();
-        *be = comp->ApproxBendingEnergy();
+        *be = comp->ApproxBendingEnergy() + (leW > 0 ? 1000.0 * comp->ApproxLinearEnergy() : 0.0);
         // objective gradient
         comp->GetDeformationField(false, true);
         comp->ResampleImage(1, pad);
@@ -58,6 +59,7 @@ int plugin_f3d_eval(int hip, const nr_geom *ref, const float *refData, const nr_
         m->GetVoxelBasedSimilarityMeasureGradient(0);
         comp->ConvolveVoxelBasedMeasureGradient(simW);
         comp->ApproxBendingEnergyGradient(beW);
+        if (leW > 0) comp->ApproxLinearEnergyGradient(leW);
         NiftiImage& tg = con->GetTransformationGradient();
         std::memcpy(grad, tg->data, tg.nVoxels() * sizeof(float));
         if (voxelGrad) {
