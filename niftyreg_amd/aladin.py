@@ -116,11 +116,18 @@ def lts(ref_pos, warp_pos, inlier_lts=50, affine=True, matrix=None):
     return m.reshape(4, 4)
 
 
+def symmetric_update(forward: "AladinContent", backward: "AladinContent"):
+    """The matrix averaging of reg_aladin_sym::UpdateTransformationMatrix on the two handles'
+    device matrices (reg-lib/_reg_aladin_sym.cpp:140-153)."""
+    _lib.call("nrh_aladin_symmetric_update", forward.h, backward.h)
+
+
 def register(ref_geom: Geom, ref, flo_geom: Geom, flo, levels=3, levels_to_perform=0, maxit=5, block_percent=50,
-             inlier_lts=50, rigid=True, affine=True, align_centre=True):
-    """reg_aladin<float>::Run, non-symmetric; returns the 4x4 affine."""
+             inlier_lts=50, rigid=True, affine=True, align_centre=True, symmetric=False):
+    """reg_aladin<float>::Run, or reg_aladin_sym<float>::Run with symmetric=True (the reg_aladin
+    application's default); returns the 4x4 affine."""
     cfg = _lib.nrh_aladin_config(levels, levels_to_perform, maxit, block_percent, inlier_lts, int(rigid), int(affine),
-                                 int(align_centre))
+                                 int(align_centre), int(symmetric))
     ref = np.ascontiguousarray(ref, np.float32).reshape(-1)
     flo = np.ascontiguousarray(flo, np.float32).reshape(-1)
     rg, fg = ref_geom.to_c(), flo_geom.to_c()

@@ -18,12 +18,14 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "recip_x87.hpp"
 #include "host.hpp"
 
 struct nrh_aladin {
     nrh_geom ref{}, flo{};
     nrh_aladin_params params{};
     hipStream_t stream = nullptr;
+    bool ownsStream = true;           // false when the stream is another handle's (symmetric pair)
     int device = 0;
     size_t N = 0, Nf = 0;
     nrh::DevBuf<float> refImg, floImg, warped;
@@ -34,7 +36,8 @@ struct nrh_aladin {
     nrh::DevBuf<int> activeList;      // rank -> block index
     nrh::DevBuf<float> refPos, warpPos;
     nrh::DevBuf<float> matrix;        // current affine, 16 floats row-major
-    nrh::DevBuf<int> counters;        // [0] defined blocks of the last matching, [1] LTS status
+    nrh::DevBuf<int> counters;        // [0] defined blocks of the last matching, [1] LTS status,
+                                      // [2] symmetric-update status
     nrh::DevBuf<float> ltsP, ltsQ;    // compacted correspondences
     nrh::DevBuf<uint32_t> ltsRes;     // residual bit patterns
     int *pinned = nullptr;            // 4 ints of host-visible status
@@ -604,6 +607,197 @@ k_lts(const float *__restrict__ refPos, const float *__restrict__ warpPos, int n
     if (tid < 16) matrix[tid] = s.M[tid];
 }
 
+// ------------------------------------------------------------------ symmetric update
+// reg_aladin_sym::UpdateTransformationMatrix (reg-lib/_reg_aladin_sym.cpp:128-157) after the two
+// LTS fits: forward <- avg(forward, inv(backward)), backward <- avg(inv(forward), backward), where
+// avg = NiftyReg::Mat44Avg2 (reg-lib/cpu/Maths.cpp:479-491) = expm((logm A + logm B) / 2).  The
+// reference evaluates logm / expm with Eigen in double; here they are computed in double by
+// inverse scaling and squaring (Denman-Beavers square roots, then the [8/8] Pade approximant of
+// log(I + X) in Gauss-Legendre form) and by Taylor scaling and squaring, with the reference's
+// float roundings between the steps.  One thread: 4x4 matrices, once per iteration.
+struct GaussLegendre8 {
+    double x[8], w[8];  // nodes and weights on [0, 1]
+};
+
+__device__ void d4_mul(const double *A, const double *B, double *C) {
+    double T[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double acc = 0;
+            for (int k = 0; k < 4; ++k) acc = fma(A[i * 4 + k], B[k * 4 + j], acc);
+            T[i * 4 + j] = acc;
+        }
+    for (int i = 0; i < 16; ++i) C[i] = T[i];
+}
+
+__device__ double d4_norm1(const double *A) {
+    double best = 0;
+    for (int j = 0; j < 4; ++j) {
+        const double c = fabs(A[j]) + fabs(A[4 + j]) + fabs(A[8 + j]) + fabs(A[12 + j]);
+        best = c > best ? c : best;
+    }
+    return best;
+}
+
+// Gauss-Jordan with partial pivoting; false when singular.
+__device__ bool d4_inverse(const double *A, double *X) {
+    double M[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            M[i][j] = A[i * 4 + j];
+            M[i][j + 4] = i == j ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (fabs(M[r][c]) > fabs(M[p][c])) p = r;
+        if (M[p][c] == 0.0) return false;
+        if (p != c)
+            for (int j = 0; j < 8; ++j) {
+                const double t = M[c][j];
+                M[c][j] = M[p][j];
+                M[p][j] = t;
+            }
+        const double inv = 1.0 / M[c][c];
+        for (int j = 0; j < 8; ++j) M[c][j] *= inv;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double f = M[r][c];
+            for (int j = 0; j < 8; ++j) M[r][j] = fma(-f, M[c][j], M[r][j]);
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) X[i * 4 + j] = M[i][j + 4];
+    return true;
+}
+
+__device__ bool d4_logm(const double *A, const GaussLegendre8& gl, double *L) {
+    double Y[16], Z[16], Yi[16], Zi[16], X[16];
+    for (int i = 0; i < 16; ++i) Y[i] = A[i];
+    int k = 0;
+    for (;;) {
+        for (int i = 0; i < 16; ++i) X[i] = Y[i] - ((i % 5) == 0 ? 1.0 : 0.0);
+        if (d4_norm1(X) <= 0.25) break;
+        if (k >= 60) return false;
+        for (int i = 0; i < 16; ++i) Z[i] = (i % 5) == 0 ? 1.0 : 0.0;
+        for (int it = 0; it < 100; ++it) {
+            if (!d4_inverse(Y, Yi) || !d4_inverse(Z, Zi)) return false;
+            double diff = 0;
+            for (int i = 0; i < 16; ++i) {
+                const double yn = 0.5 * (Y[i] + Zi[i]);
+                diff += fabs(yn - Y[i]);
+                Z[i] = 0.5 * (Z[i] + Yi[i]);
+                Y[i] = yn;
+            }
+            if (diff <= 1e-15 * d4_norm1(Y)) break;
+        }
+        ++k;
+    }
+    double S[16] = {}, T[16], Ti[16], P[16];
+    for (int j = 0; j < 8; ++j) {
+        for (int i = 0; i < 16; ++i) T[i] = fma(gl.x[j], X[i], (i % 5) == 0 ? 1.0 : 0.0);
+        if (!d4_inverse(T, Ti)) return false;
+        d4_mul(X, Ti, P);
+        for (int i = 0; i < 16; ++i) S[i] = fma(gl.w[j], P[i], S[i]);
+    }
+    const double scale = ldexp(1.0, k);
+    for (int i = 0; i < 16; ++i) L[i] = scale * S[i];
+    return true;
+}
+
+__device__ void d4_expm(const double *A, double *E) {
+    const double n = d4_norm1(A);
+    int s = 0;
+    if (n > 0.5) s = (int)ceil(log2(n / 0.5));
+    const double f = ldexp(1.0, -s);
+    double B[16], term[16], sum[16];
+    for (int i = 0; i < 16; ++i) {
+        B[i] = A[i] * f;
+        term[i] = (i % 5) == 0 ? 1.0 : 0.0;
+        sum[i] = term[i];
+    }
+    for (int k = 1; k <= 20; ++k) {
+        d4_mul(term, B, term);
+        for (int i = 0; i < 16; ++i) {
+            term[i] /= k;
+            sum[i] += term[i];
+        }
+    }
+    for (int i = 0; i < s; ++i) d4_mul(sum, sum, sum);
+    for (int i = 0; i < 16; ++i) E[i] = sum[i];
+}
+
+// nifti_mat44_inverse (reg-io/niftilib/nifti1_io.c:1680-1717): cofactors in double, the
+// determinant's reciprocal rounded as x87 long double then double, float results.
+__device__ void nifti_inverse(const float *R, float *Q) {
+#pragma clang fp contract(off)
+    const double r11 = R[0], r12 = R[1], r13 = R[2], v1 = R[3];
+    const double r21 = R[4], r22 = R[5], r23 = R[6], v2 = R[7];
+    const double r31 = R[8], r32 = R[9], r33 = R[10], v3 = R[11];
+    double deti = r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 -
+                  r31 * r22 * r13;
+    if (deti != 0.0) deti = recip_x87(deti);  // niftilib's 1.0l / deti
+    Q[0] = (float)(deti * (r22 * r33 - r32 * r23));
+    Q[1] = (float)(deti * (-r12 * r33 + r32 * r13));
+    Q[2] = (float)(deti * (r12 * r23 - r22 * r13));
+    Q[3] = (float)(deti * (-r12 * r23 * v3 + r12 * v2 * r33 + r22 * r13 * v3 - r22 * v1 * r33 - r32 * r13 * v2 +
+                           r32 * v1 * r23));
+    Q[4] = (float)(deti * (-r21 * r33 + r31 * r23));
+    Q[5] = (float)(deti * (r11 * r33 - r31 * r13));
+    Q[6] = (float)(deti * (-r11 * r23 + r21 * r13));
+    Q[7] = (float)(deti * (r11 * r23 * v3 - r11 * v2 * r33 - r21 * r13 * v3 + r21 * v1 * r33 + r31 * r13 * v2 -
+                           r31 * v1 * r23));
+    Q[8] = (float)(deti * (r21 * r32 - r31 * r22));
+    Q[9] = (float)(deti * (-r11 * r32 + r31 * r12));
+    Q[10] = (float)(deti * (r11 * r22 - r21 * r12));
+    Q[11] = (float)(deti * (-r11 * r22 * v3 + r11 * r32 * v2 + r21 * r12 * v3 - r21 * r32 * v1 - r31 * r12 * v2 +
+                            r31 * r22 * v1));
+    Q[12] = Q[13] = Q[14] = 0.f;
+    Q[15] = deti == 0.0 ? 0.f : 1.f;
+}
+
+// Mat44Avg2: the logarithms rounded to float with zero bottom rows, summed in double and
+// rounded to float, halved, exponentiated in double, rounded to float.
+__device__ bool mat44_avg2(const float *A, const float *B, const GaussLegendre8& gl, float *out) {
+    double a[16], b[16], la[16], lb[16], h[16], e[16];
+    for (int i = 0; i < 16; ++i) {
+        a[i] = A[i];
+        b[i] = B[i];
+    }
+    if (!d4_logm(a, gl, la) || !d4_logm(b, gl, lb)) return false;
+    for (int i = 0; i < 16; ++i) {
+        float fa = (float)la[i], fb = (float)lb[i];
+        if (i >= 12) fa = fb = 0.f;
+        const float sum = (float)((double)fa + (double)fb);
+        h[i] = (double)(sum * 0.5f);
+    }
+    d4_expm(h, e);
+    for (int i = 0; i < 16; ++i) out[i] = (float)e[i];
+    return true;
+}
+
+__global__ void __launch_bounds__(64)
+k_symmetric_update(float *__restrict__ fw, float *__restrict__ bw, GaussLegendre8 gl, int *__restrict__ status) {
+    if (threadIdx.x != 0) return;
+    float F[16], B[16], fi[16], bi[16], nf[16], nb[16];
+    for (int i = 0; i < 16; ++i) {
+        F[i] = fw[i];
+        B[i] = bw[i];
+    }
+    nifti_inverse(F, fi);
+    nifti_inverse(B, bi);
+    if (!mat44_avg2(F, bi, gl, nf) || !mat44_avg2(fi, B, gl, nb)) {
+        *status = 3;  // matrices left unchanged
+        return;
+    }
+    for (int i = 12; i < 15; ++i) nf[i] = nb[i] = 0.f;
+    nf[15] = nb[15] = 1.f;
+    for (int i = 0; i < 16; ++i) {
+        fw[i] = nf[i];
+        bw[i] = nb[i];
+    }
+}
+
 // ------------------------------------------------------------------ host side
 unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -684,7 +878,7 @@ void launch_lts(nrh_aladin *h, bool affine) {
 }
 
 void read_counters(nrh_aladin *h) {
-    NRH_HIP(hipMemcpyAsync(h->pinned, h->counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    NRH_HIP(hipMemcpyAsync(h->pinned, h->counters.p, 3 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     NRH_HIP(hipStreamSynchronize(h->stream));
 }
 
@@ -694,8 +888,58 @@ void raise_lts_status(int status) {
     NRH_REQUIRE(status != 2, NRH_ERR_NUMERIC, "singular least-squares system in the LTS fit");
 }
 
+void raise_symmetric_status(int status) {
+    NRH_REQUIRE(status == 0, NRH_ERR_NUMERIC,
+                "the matrix logarithm of the symmetric update failed (a matrix with a non-positive determinant)");
+}
+
+// Nodes and weights of the 8-point Gauss-Legendre rule on [0, 1] (Newton on P_8).
+GaussLegendre8 gauss_legendre8() {
+    GaussLegendre8 g;
+    const int m = 8;
+    auto legendre = [&](double t, double& pm, double& dpm) {
+        double p0 = 1.0, p1 = t;
+        for (int k = 2; k <= m; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        pm = p1;
+        dpm = m * (t * p1 - p0) / (t * t - 1.0);
+    };
+    for (int i = 0; i < m; ++i) {
+        double t = std::cos(3.14159265358979323846 * (i + 0.75) / (m + 0.5)), p, dp;
+        for (int it = 0; it < 100; ++it) {
+            legendre(t, p, dp);
+            const double dt = p / dp;
+            t -= dt;
+            if (std::fabs(dt) < 1e-17) break;
+        }
+        legendre(t, p, dp);
+        g.x[i] = 0.5 * (t + 1.0);
+        g.w[i] = 1.0 / ((1.0 - t * t) * dp * dp);
+    }
+    return g;
+}
+
+// Both matrices live on the device; the kernel runs on the forward handle's stream after the
+// backward handle's queued work.
+void launch_symmetric_update(nrh_aladin *fw, nrh_aladin *bw) {
+    static const GaussLegendre8 gl = gauss_legendre8();
+    if (fw->stream != bw->stream) {
+        hipEvent_t ev;
+        NRH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        NRH_HIP(hipEventRecord(ev, bw->stream));
+        NRH_HIP(hipStreamWaitEvent(fw->stream, ev, 0));
+        NRH_HIP(hipEventDestroy(ev));
+    }
+    hipLaunchKernelGGL(k_symmetric_update, dim3(1), dim3(64), 0, fw->stream, fw->matrix.p, bw->matrix.p, gl,
+                       fw->counters.p + 2);
+    NRH_HIP(hipGetLastError());
+}
+
 nrh_aladin *create(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
-                   const int *mask, const nrh_aladin_params *params) {
+                   const int *mask, const nrh_aladin_params *params, hipStream_t sharedStream = nullptr) {
     check_geom(ref, "reference");
     check_geom(flo, "floating");
     check_aladin_params(params);
@@ -708,7 +952,12 @@ nrh_aladin *create(const nrh_geom *ref, const float *ref_data, const nrh_geom *f
     h->N = voxels(*ref);
     h->Nf = voxels(*flo);
     NRH_HIP(hipGetDevice(&h->device));
-    NRH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (sharedStream) {
+        h->stream = sharedStream;
+        h->ownsStream = false;
+    } else {
+        NRH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    }
     NRH_HIP(hipHostMalloc((void **)&h->pinned, 4 * sizeof(int), hipHostMallocDefault));
     h->refImg.alloc(h->N);
     h->floImg.alloc(h->Nf);
@@ -731,6 +980,7 @@ nrh_aladin *create(const nrh_geom *ref, const float *ref_data, const nrh_geom *f
     NRH_HIP(hipMemsetAsync(h->counters.p, 0, h->counters.bytes(), h->stream));
     h->pinned[0] = -1;
     h->pinned[1] = 0;
+    h->pinned[2] = 0;
     select_blocks(h.get());
     return h.release();
 }
@@ -739,7 +989,7 @@ void destroy(nrh_aladin *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->pinned) (void)hipHostFree(h->pinned);
-    hipStream_t s = h->stream;
+    hipStream_t s = h->ownsStream ? h->stream : nullptr;
     delete h;
     if (s) (void)hipStreamDestroy(s);
 }
@@ -759,8 +1009,11 @@ struct AladinHandle {
     ~AladinHandle() { destroy(h); }
 };
 
-// reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364), non-symmetric, with
-// InitialiseRegistration's centre alignment (:160-183).
+// reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364) with InitialiseRegistration's
+// centre alignment (:160-183); with cfg.symmetric, reg_aladin_sym<float> (reg-lib/_reg_aladin_sym.cpp):
+// a backward handle (floating -> reference, every voxel active) shares the forward handle's
+// stream, its matrix starts as the inverse of the forward one, and each iteration warps both
+// ways, fits both ways and averages the two matrices on the device.
 void aladin_register(const nrh_geom& refG, const float *refData, const nrh_geom& floG, const float *floData,
                      const nrh_aladin_config& cfg, float *out) {
     NRH_REQUIRE(cfg.levels >= 1 && cfg.max_iterations >= 0, NRH_ERR_ARG, "invalid Aladin configuration");
@@ -781,20 +1034,34 @@ void aladin_register(const nrh_geom& refG, const float *refData, const nrh_geom&
         M[7] = fc[1] - rc[1];
         M[11] = fc[2] - rc[2];
     }
+    float Mb[16];
+    mat44_inverse(M, Mb);
     std::vector<Level> rp = pyramid(refG, refData, cfg.levels, toPerform);
     std::vector<Level> fp = pyramid(floG, floData, cfg.levels, toPerform);
     const nrh_aladin_params params{ cfg.block_percent, cfg.inlier_lts, std::numeric_limits<float>::quiet_NaN() };
     for (int level = 0; level < toPerform; ++level) {
-        AladinHandle ah;
+        AladinHandle ah, bh;  // bh (sharing ah's stream) is destroyed first
         ah.h = create(&rp[level].geom, rp[level].data.data(), &fp[level].geom, fp[level].data.data(), nullptr, &params);
-        nrh_aladin *h = ah.h;
+        nrh_aladin *h = ah.h, *b = nullptr;
         set_matrix(h, M);
+        if (cfg.symmetric) {
+            bh.h = create(&fp[level].geom, fp[level].data.data(), &rp[level].geom, rp[level].data.data(), nullptr,
+                          &params, h->stream);
+            b = bh.h;
+            set_matrix(b, Mb);
+        }
         const int maxIt = level == 0 ? 2 * cfg.max_iterations : cfg.max_iterations;
         auto resolve = [&](int iterations, bool affine) {
             for (int it = 0; it < iterations; ++it) {
                 launch_warp(h);
+                if (b) launch_warp(b);
                 launch_block_matching(h);
                 launch_lts(h, affine);
+                if (b) {
+                    launch_block_matching(b);
+                    launch_lts(b, affine);
+                    launch_symmetric_update(h, b);
+                }
             }
         };
         if ((cfg.rigid && !cfg.affine) || (cfg.affine && cfg.rigid && level == 0))
@@ -802,6 +1069,12 @@ void aladin_register(const nrh_geom& refG, const float *refData, const nrh_geom&
         if (cfg.affine) resolve(maxIt, true);
         read_counters(h);
         raise_lts_status(h->pinned[1]);
+        raise_symmetric_status(h->pinned[2]);
+        if (b) {
+            read_counters(b);
+            raise_lts_status(b->pinned[1]);
+            get_matrix(b, Mb);
+        }
         get_matrix(h, M);
     }
     std::memcpy(out, M, sizeof(M));
@@ -939,6 +1212,17 @@ int nrh_lts(const float *ref_pos, const float *warp_pos, int n, int inlier_lts, 
         NRH_HIP(hipStreamSynchronize(stream));
         raise_lts_status(st);
         std::memcpy(m, out, sizeof(out));
+    });
+}
+
+int nrh_aladin_symmetric_update(nrh_aladin *forward, nrh_aladin *backward) {
+    return guard([&] {
+        NRH_REQUIRE(forward && backward && forward != backward, NRH_ERR_ARG, "two distinct handles are required");
+        NRH_HIP(hipMemsetAsync(forward->counters.p + 2, 0, sizeof(int), forward->stream));
+        launch_symmetric_update(forward, backward);
+        read_counters(forward);
+        if (backward->stream != forward->stream) NRH_HIP(hipStreamSynchronize(backward->stream));
+        raise_symmetric_status(forward->pinned[2]);
     });
 }
 

@@ -672,6 +672,24 @@ static void node_position(const orc_geom *cpp, int x, int y, int z, float out[3]
                  M(cpp->xyz, r, 3);
 }
 
+/* The in-place reg_getDisplacementFromDeformation / reg_getDeformationFromDisplacement pair that
+ * reg_spline_approxBendingEnergyGradient3D wraps around its work (reg-lib/cpu/_reg_localTrans_regul.cpp:
+ * 312, 450): every point becomes (p - node) + node in float. */
+void orc_displacement_round_trip(const orc_geom *cpp, float *cppData) {
+    const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    for (int z = 0; z < cpp->nz; ++z)
+        for (int y = 0; y < cpp->ny; ++y)
+            for (int x = 0; x < cpp->nx; ++x) {
+                const size_t n = ((size_t)z * cpp->ny + y) * cpp->nx + x;
+                float p[3];
+                node_position(cpp, x, y, z, p);
+                for (int k = 0; k < 3; ++k) {
+                    const float d = cppData[n + k * g] - p[k];
+                    cppData[n + k * g] = d + p[k];
+                }
+            }
+}
+
 void orc_bending_energy_gradient(const orc_geom *cpp, const float *cppData, float weight, float *grad) {
     const int nx = cpp->nx, ny = cpp->ny, nz = cpp->nz;
     const size_t g = (size_t)nx * ny * nz;
@@ -1344,4 +1362,216 @@ void orc_linear_energy_gradient(const orc_geom *cpp, const float *cppData, float
                             grad[idx + 2 * g] += ratio * (inv.m[2][0] * gv0 + inv.m[2][1] * gv1 + inv.m[2][2] * gv2);
                         }
             }
+}
+
+/* ---- symmetric Aladin: matrix inverse and log-Euclidean average ------------------------- */
+
+/* nifti_mat44_inverse (reg-io/niftilib/nifti1_io.c:1680-1717): cofactors in double, the
+ * determinant's reciprocal in long double, float results, bottom row (0, 0, 0, 1). */
+void orc_nifti_mat44_inverse(const float A[16], float out[16]) {
+    const double r11 = A[0], r12 = A[1], r13 = A[2], v1 = A[3];
+    const double r21 = A[4], r22 = A[5], r23 = A[6], v2 = A[7];
+    const double r31 = A[8], r32 = A[9], r33 = A[10], v3 = A[11];
+    double deti = r11 * r22 * r33 - r11 * r32 * r23 - r21 * r12 * r33 + r21 * r32 * r13 + r31 * r12 * r23 -
+                  r31 * r22 * r13;
+    if (deti != 0.0L) deti = (double)(1.0L / (long double)deti);
+    out[0] = (float)(deti * (r22 * r33 - r32 * r23));
+    out[1] = (float)(deti * (-r12 * r33 + r32 * r13));
+    out[2] = (float)(deti * (r12 * r23 - r22 * r13));
+    out[3] = (float)(deti * (-r12 * r23 * v3 + r12 * v2 * r33 + r22 * r13 * v3 - r22 * v1 * r33 - r32 * r13 * v2 +
+                             r32 * v1 * r23));
+    out[4] = (float)(deti * (-r21 * r33 + r31 * r23));
+    out[5] = (float)(deti * (r11 * r33 - r31 * r13));
+    out[6] = (float)(deti * (-r11 * r23 + r21 * r13));
+    out[7] = (float)(deti * (r11 * r23 * v3 - r11 * v2 * r33 - r21 * r13 * v3 + r21 * v1 * r33 + r31 * r13 * v2 -
+                             r31 * v1 * r23));
+    out[8] = (float)(deti * (r21 * r32 - r31 * r22));
+    out[9] = (float)(deti * (-r11 * r32 + r31 * r12));
+    out[10] = (float)(deti * (r11 * r22 - r21 * r12));
+    out[11] = (float)(deti * (-r11 * r22 * v3 + r11 * r32 * v2 + r21 * r12 * v3 - r21 * r32 * v1 - r31 * r12 * v2 +
+                              r31 * r22 * v1));
+    out[12] = out[13] = out[14] = 0.f;
+    out[15] = deti == 0.0 ? 0.f : 1.f;
+}
+
+/* 4x4 double helpers (row-major). */
+static void d4_mul(const double *A, const double *B, double *C) {
+    double T[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double s = 0;
+            for (int k = 0; k < 4; ++k) s += A[i * 4 + k] * B[k * 4 + j];
+            T[i * 4 + j] = s;
+        }
+    memcpy(C, T, sizeof(T));
+}
+
+static double d4_norm1(const double *A) {
+    double best = 0;
+    for (int j = 0; j < 4; ++j) {
+        double s = 0;
+        for (int i = 0; i < 4; ++i) s += fabs(A[i * 4 + j]);
+        if (s > best) best = s;
+    }
+    return best;
+}
+
+/* Gauss-Jordan inverse with partial pivoting; returns 0 when singular. */
+static int d4_inverse(const double *A, double *X) {
+    double M[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            M[i][j] = A[i * 4 + j];
+            M[i][j + 4] = i == j ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (fabs(M[r][c]) > fabs(M[p][c])) p = r;
+        if (M[p][c] == 0.0) return 0;
+        if (p != c)
+            for (int j = 0; j < 8; ++j) {
+                const double t = M[c][j];
+                M[c][j] = M[p][j];
+                M[p][j] = t;
+            }
+        const double inv = 1.0 / M[c][c];
+        for (int j = 0; j < 8; ++j) M[c][j] *= inv;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double f = M[r][c];
+            if (f != 0.0)
+                for (int j = 0; j < 8; ++j) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) X[i * 4 + j] = M[i][j + 4];
+    return 1;
+}
+
+/* Gauss-Legendre nodes and weights on [0, 1] (Newton on the Legendre polynomial). */
+static void gauss_legendre01(int m, double *x, double *w) {
+    for (int i = 0; i < m; ++i) {
+        double t = cos(3.14159265358979323846 * (i + 0.75) / (m + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = t;
+            for (int k = 2; k <= m; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = m * (t * p1 - p0) / (t * t - 1.0);
+            const double dt = p1 / dp;
+            t -= dt;
+            if (fabs(dt) < 1e-17) break;
+        }
+        {
+            double p0 = 1.0, p1 = t;
+            for (int k = 2; k <= m; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = m * (t * p1 - p0) / (t * t - 1.0);
+        }
+        x[i] = 0.5 * (t + 1.0);
+        w[i] = 1.0 / ((1.0 - t * t) * dp * dp);  /* 2/((1-t^2)P'^2) on [-1,1], halved on [0,1] */
+    }
+}
+
+/* Principal matrix logarithm by inverse scaling and squaring: Denman-Beavers square roots until
+ * ||A - I||_1 <= 1/4, then the [8/8] Pade approximant of log(I + X) in its partial-fraction
+ * (8-point Gauss-Legendre) form, scaled back by 2^k.  Returns 0 when a square root fails. */
+static int d4_logm(const double *A, double *L) {
+    double Y[16], Z[16], Yi[16], Zi[16], X[16];
+    memcpy(Y, A, sizeof(Y));
+    int k = 0;
+    for (;;) {
+        for (int i = 0; i < 16; ++i) X[i] = Y[i] - ((i % 5) == 0 ? 1.0 : 0.0);
+        if (d4_norm1(X) <= 0.25 || k >= 60) break;
+        for (int i = 0; i < 16; ++i) Z[i] = (i % 5) == 0 ? 1.0 : 0.0;
+        for (int it = 0; it < 100; ++it) {
+            if (!d4_inverse(Y, Yi) || !d4_inverse(Z, Zi)) return 0;
+            double diff = 0;
+            for (int i = 0; i < 16; ++i) {
+                const double yn = 0.5 * (Y[i] + Zi[i]);
+                diff += fabs(yn - Y[i]);
+                Z[i] = 0.5 * (Z[i] + Yi[i]);
+                Y[i] = yn;
+            }
+            if (diff <= 1e-15 * d4_norm1(Y)) break;
+        }
+        ++k;
+    }
+    double xg[8], wg[8], S[16] = { 0 }, T[16], Ti[16], P[16];
+    gauss_legendre01(8, xg, wg);
+    for (int j = 0; j < 8; ++j) {
+        for (int i = 0; i < 16; ++i) T[i] = ((i % 5) == 0 ? 1.0 : 0.0) + xg[j] * X[i];
+        if (!d4_inverse(T, Ti)) return 0;
+        d4_mul(X, Ti, P);
+        for (int i = 0; i < 16; ++i) S[i] += wg[j] * P[i];
+    }
+    const double scale = ldexp(1.0, k);
+    for (int i = 0; i < 16; ++i) L[i] = scale * S[i];
+    return 1;
+}
+
+/* Matrix exponential by scaling and squaring with a degree-20 Taylor polynomial
+ * (||A / 2^s||_1 <= 1/2). */
+static void d4_expm(const double *A, double *E) {
+    const double n = d4_norm1(A);
+    int s = 0;
+    if (n > 0.5) s = (int)ceil(log2(n / 0.5));
+    const double f = ldexp(1.0, -s);
+    double B[16], term[16], sum[16];
+    for (int i = 0; i < 16; ++i) {
+        B[i] = A[i] * f;
+        term[i] = (i % 5) == 0 ? 1.0 : 0.0;
+        sum[i] = term[i];
+    }
+    for (int k = 1; k <= 20; ++k) {
+        d4_mul(term, B, term);
+        for (int i = 0; i < 16; ++i) {
+            term[i] /= k;
+            sum[i] += term[i];
+        }
+    }
+    for (int i = 0; i < s; ++i) d4_mul(sum, sum, sum);
+    memcpy(E, sum, sizeof(sum));
+}
+
+/* NiftyReg::Mat44Avg2 (reg-lib/cpu/Maths.cpp:479-491) with Mat44Logm / Mat44Expm (:414-477):
+ * the logarithms rounded to float, their bottom rows zeroed, summed in double and rounded to
+ * float, halved, exponentiated in double and rounded to float.  Returns 0 if a logarithm fails. */
+int orc_mat44_avg2(const float A[16], const float B[16], float out[16]) {
+    double a[16], b[16], la[16], lb[16], h[16], e[16];
+    for (int i = 0; i < 16; ++i) {
+        a[i] = A[i];
+        b[i] = B[i];
+    }
+    if (!d4_logm(a, la) || !d4_logm(b, lb)) return 0;
+    for (int i = 0; i < 16; ++i) {
+        float fa = (float)la[i], fb = (float)lb[i];
+        if (i >= 12) fa = fb = 0.f;
+        const float sum = (float)((double)fa + (double)fb);
+        h[i] = (double)(float)(sum * 0.5);
+    }
+    d4_expm(h, e);
+    for (int i = 0; i < 16; ++i) out[i] = (float)e[i];
+    return 1;
+}
+
+/* reg_aladin_sym::UpdateTransformationMatrix (reg-lib/_reg_aladin_sym.cpp:128-157) after both
+ * LTS fits: forward <- avg2(forward, inverse(backward)), backward <- avg2(inverse(forward),
+ * backward), both bottom rows reset to (0, 0, 0, 1). */
+int orc_symmetric_update(float fw[16], float bw[16]) {
+    float fi[16], bi[16], nf[16], nb[16];
+    orc_nifti_mat44_inverse(fw, fi);
+    orc_nifti_mat44_inverse(bw, bi);
+    if (!orc_mat44_avg2(fw, bi, nf) || !orc_mat44_avg2(fi, bw, nb)) return 0;
+    for (int i = 12; i < 15; ++i) nf[i] = nb[i] = 0.f;
+    nf[15] = nb[15] = 1.f;
+    memcpy(fw, nf, sizeof(nf));
+    memcpy(bw, nb, sizeof(nb));
+    return 1;
 }

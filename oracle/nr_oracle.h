@@ -123,6 +123,10 @@ void orc_lncc_gradient(const orc_geom *g, const float *ref, const float *war, co
                        float sigma, double weight, float *grad);
 
 
+/* The displacement round trip reg_spline_approxBendingEnergyGradient3D applies to the control
+ * points in place (reg-lib/cpu/_reg_localTrans_regul.cpp:312, 450). */
+void orc_displacement_round_trip(const orc_geom *cpp, float *cppData);
+
 /* ---- linear energy ---------------------------------------------------------------------- */
 /* set_first_order_basis_values 3-D (reg-lib/cpu/_reg_splineBasis.cpp:182-290) */
 void orc_first_order_basis(float bx[27], float by[27], float bz[27]);
@@ -130,6 +134,14 @@ void orc_first_order_basis(float bx[27], float by[27], float bz[27]);
 double orc_linear_energy(const orc_geom *cpp, const float *cppData);
 /* reg_spline_approxLinearEnergyGradient3D (reg-lib/cpu/_reg_localTrans_regul.cpp:739-824); grad += */
 void orc_linear_energy_gradient(const orc_geom *cpp, const float *cppData, float weight, float *grad);
+
+/* ---- symmetric Aladin ------------------------------------------------------------------- */
+/* nifti_mat44_inverse (reg-io/niftilib/nifti1_io.c:1680-1717), row-major 4x4 */
+void orc_nifti_mat44_inverse(const float A[16], float out[16]);
+/* NiftyReg::Mat44Avg2 (reg-lib/cpu/Maths.cpp:479-491): expm((logm A + logm B) / 2) */
+int orc_mat44_avg2(const float A[16], const float B[16], float out[16]);
+/* the matrix update of reg_aladin_sym::UpdateTransformationMatrix (reg-lib/_reg_aladin_sym.cpp:128-157) */
+int orc_symmetric_update(float fw[16], float bw[16]);
 
 #ifdef __cplusplus
 }

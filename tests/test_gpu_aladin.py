@@ -204,3 +204,56 @@ def test_register_recovers_affine_and_matches_reference(gpu, rigid):
         tol_lin, tol_t = (0.03, 1.0) if rigid else (5e-3, 0.1)
         assert np.abs(M[:3, :3] - R[:3, :3]).max() < tol_lin
         assert np.abs(M[:3, 3] - R[:3, 3]).max() < tol_t
+
+
+# ---------------------------------------------------------------- symmetric Aladin
+def _ulp_or_abs_close(got, want, ulps=2, atol=1e-6):
+    got, want = np.asarray(got, np.float32), np.asarray(want, np.float32)
+    sp = np.spacing(np.maximum(np.abs(got), np.abs(want), dtype=np.float32)).astype(np.float64)
+    d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+    assert np.all((d <= ulps * sp) | (d <= atol)), (got, want)
+
+
+def test_symmetric_update_matches_reference(gpu):
+    """nrh_aladin_symmetric_update on two handles' device matrices against the reference's
+    Mat44Avg2 / nifti_mat44_inverse (the oracle's restatement where the build is absent)."""
+    g = Geom.identity((32, 32, 32))
+    ref = phantoms.blob_phantom(g.dims, 1).ravel()
+    flo = phantoms.blob_phantom(g.dims, 2).ravel()
+    fw, bw = AladinContent(g, ref, g, flo), AladinContent(g, flo, g, ref)
+    for seed in range(6):
+        F = phantoms.random_affine(200 + seed).astype(np.float32)
+        B = (np.linalg.inv(F.astype(np.float64)) @ phantoms.random_affine(300 + seed, scale=(0.98, 1.02)))
+        B = B.astype(np.float32)
+        B[3] = [0, 0, 0, 1]
+        fw.set_matrix(F)
+        bw.set_matrix(B)
+        aladin.symmetric_update(fw, bw)
+        if O.ref_available():
+            wf = O.ref_mat44_avg2(F, O.ref_mat44_inverse(B))
+            wb = O.ref_mat44_avg2(O.ref_mat44_inverse(F), B)
+            wf[3], wb[3] = [0, 0, 0, 1], [0, 0, 0, 1]
+        else:
+            wf, wb = O.orc_symmetric_update(F, B)
+        _ulp_or_abs_close(fw.get_matrix(), wf)
+        _ulp_or_abs_close(bw.get_matrix(), wb)
+    fw.close()
+    bw.close()
+
+
+@pytest.mark.parametrize("rigid", [False, True])
+def test_symmetric_register_matches_reference(gpu, rigid):
+    """reg_aladin_sym (the reg_aladin default) on the device against the reference's run."""
+    dims = (64, 64, 64)
+    ref, flo, A = phantoms.config3_pair(2, dims)
+    g = Geom.identity(dims)
+    M = aladin.register(g, ref, g, flo, levels=3, maxit=5, rigid=rigid, affine=True, symmetric=True)
+    E = _expected_matrix(A, dims)
+    assert np.abs(M[:3, :3] - E[:3, :3]).max() < 0.05
+    assert np.abs(M[:3, 3] - E[:3, 3]).max() < 1.5
+    if O.ref_available():
+        R = O.ref_aladin_run(g, ref, g, flo, levels=3, maxit=5, rigid=rigid, affine=True, symmetric=True)
+        print("device\n", M, "\nreference\n", R, "\nexpected\n", E)
+        tol_lin, tol_t = (0.03, 1.0) if rigid else (5e-3, 0.1)
+        assert np.abs(M[:3, :3] - R[:3, :3]).max() < tol_lin
+        assert np.abs(M[:3, 3] - R[:3, 3]).max() < tol_t
