@@ -75,7 +75,7 @@ typedef struct nrh_f3d_params {
     float padding;            /* warped padding value (NaN by default, reg-lib/_reg_base.cpp:46) */
     float similarity_weight;  /* reg_base::similarityWeight after CheckParameters */
     float be_weight;          /* bending-energy weight of the current level */
-    float le_weight;          /* linear-energy weight; must be 0 (not implemented) */
+    float le_weight;          /* linear-energy weight of the current level */
     int image_storage;        /* 0: float32 images in HBM, 1: float16 storage, float32 compute */
     int measure;              /* NRH_MEASURE_NMI or NRH_MEASURE_LNCC (reg-lib/MeasureCreator.hpp:8) */
     float lncc_sigma;         /* LNCC Gaussian standard deviation, < 0: voxels (reg_lncc default -5) */
@@ -121,6 +121,10 @@ int nrh_f3d_lncc_voxel_gradient(nrh_f3d *h, double weight);
 int nrh_f3d_convolve_voxel_gradient(nrh_f3d *h);           /* Compute::ConvolveImage (private) */
 int nrh_f3d_voxel_to_node(nrh_f3d *h, float weight);       /* Compute::VoxelCentricToNodeCentric */
 int nrh_f3d_zero_node_gradient(nrh_f3d *h);                /* F3dContent::ZeroTransformationGradient */
+/* Compute::ApproxLinearEnergy / ApproxLinearEnergyGradient (reg-lib/Compute.cpp:52-62):
+ * reg_spline_approxLinearEnergyValue3D / Gradient3D (reg-lib/cpu/_reg_localTrans_regul.cpp:559-824). */
+int nrh_f3d_linear_energy(nrh_f3d *h, double *value);
+int nrh_f3d_linear_energy_gradient(nrh_f3d *h, float weight);
 int nrh_f3d_bending_energy(nrh_f3d *h, double *value);     /* Compute::ApproxBendingEnergy */
 int nrh_f3d_bending_energy_gradient(nrh_f3d *h, float weight); /* Compute::ApproxBendingEnergyGradient */
 int nrh_f3d_max_length(nrh_f3d *h, double *value);         /* Compute::GetMaximalLength */
@@ -175,7 +179,7 @@ int nrh_f3d_update_control_points(nrh_f3d *h, float scale);
 typedef struct nrh_f3d_config {
     int bins;              /* user bins (64) */
     float be_weight;       /* -be (0.001) */
-    float le_weight;       /* -le: must be 0 */
+    float le_weight;       /* -le (the reference's default is 0.01) */
     float spacing;         /* -sx: negative = voxels (-5) */
     int levels;            /* -ln (3) */
     int levels_to_perform; /* -lp (0 = all) */
@@ -241,8 +245,16 @@ int nrh_aladin_lts(nrh_aladin *h, int affine);
 /* The same LTS on host arrays (n correspondences, NaN warped x = undefined); m is in/out. */
 int nrh_lts(const float *ref_pos, const float *warp_pos, int n, int inlier_lts, int affine, float m[16]);
 
+/* The matrix step of reg_aladin_sym::UpdateTransformationMatrix (reg-lib/_reg_aladin_sym.cpp:
+ * 140-153), run after both handles' LTS fits: forward <- Mat44Avg2(forward, inverse(backward)),
+ * backward <- Mat44Avg2(inverse(forward), backward) (reg-lib/cpu/Maths.cpp:479-491, the
+ * log-Euclidean mean), bottom rows reset; on the device, in FP64.  backward is the handle whose
+ * reference is forward's floating image.  NRH_ERR_NUMERIC if a logarithm does not exist. */
+int nrh_aladin_symmetric_update(nrh_aladin *forward, nrh_aladin *backward);
+
 /* Full affine registration: reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364) with
- * InitialiseRegistration's pyramid and centre alignment (:88-245), non-symmetric. */
+ * InitialiseRegistration's pyramid and centre alignment (:88-245), or reg_aladin_sym<float>
+ * (reg-lib/_reg_aladin_sym.cpp) when symmetric is set. */
 typedef struct nrh_aladin_config {
     int levels;             /* -ln (3) */
     int levels_to_perform;  /* -lp (0 = all) */
@@ -252,9 +264,50 @@ typedef struct nrh_aladin_config {
     int rigid;              /* rigid stage (1 unless -affDirect) */
     int affine;             /* affine stage (1 unless -rigOnly) */
     int align_centre;       /* 1 unless -noAlign */
+    int symmetric;          /* 1 unless -noSym (reg_aladin's default is the symmetric version) */
 } nrh_aladin_config;
 int nrh_aladin_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                         const nrh_aladin_config *cfg, float matrix[16]);
+
+/* ------------------------------------------------------------------ file formats
+ * The edges of a registration: single-file NIfTI-1 images (.nii / .nii.gz) as
+ * reg_io_ReadImageFile / reg_io_WriteImageFile read and write them (reg-io/_reg_ReadWriteImage.cpp:
+ * 51-81, 145-170, over niftilib), the control-point grid header of reg_createControlPointGrid
+ * (reg-lib/cpu/_reg_localTrans.cpp:21-128) and the affine text matrix of
+ * reg_tool_{Read,Write}AffineFile (reg-io/_reg_ReadWriteMatrix.cpp:83-116).  Host code only. */
+typedef struct nrh_nifti_info {
+    int dim[8];             /* dim[0] = number of axes; after reg_checkAndCorrectDimension */
+    float pixdim[8];        /* spacings in mm (micron / metre units converted) */
+    int datatype;           /* of the file; data is always returned / written as float32 */
+    float scl_slope, scl_inter;  /* slope 1 when the file holds 0 */
+    int qform_code, sform_code;
+    float quatern[3];       /* quatern_b, c, d */
+    float qoffset[3];
+    float qfac;             /* -1 or 1 (pixdim[0] of the file) */
+    float srow[12];         /* srow_x, srow_y, srow_z */
+    float qto_xyz[16];      /* voxel -> mm of the qform as niftilib and reg_checkAndCorrectDimension */
+    float sto_xyz[16];      /* set them (row-major; the sform is zero when sform_code <= 0) */
+    int intent_code;
+    float intent_p[3];
+    char intent_name[16];
+    float cal_min, cal_max;
+    int xyzt_units;
+    char descrip[80];
+} nrh_nifti_info;
+int nrh_nifti_read_info(const char *path, nrh_nifti_info *info);
+/* The voxel values converted to float32 (scl_slope / scl_inter not applied); capacity in floats. */
+int nrh_nifti_read_data(const char *path, float *data, size_t capacity);
+/* The geometry the registrations use: sform if sform_code > 0, else qform (reg-lib/Content.h:31-36). */
+int nrh_nifti_geometry(const nrh_nifti_info *info, nrh_geom *geom);
+/* The header of the control-point grid reg_f3d writes for a reference image and a grid spacing
+ * in mm: dims ceil(n d / s + 3), origin moved by one node in the qform and the sform, intent
+ * vector with name "NREG_TRANS" and intent_p1 = CUB_SPLINE_GRID (2). */
+int nrh_control_point_grid_info(const nrh_nifti_info *ref, const float spacing[3], nrh_nifti_info *grid);
+/* Writes a float32 NIfTI-1 single file (.nii, or gzip-compressed .nii.gz). */
+int nrh_nifti_write(const char *path, const nrh_nifti_info *info, const float *data);
+/* The 4x4 affine text file: four lines of four numbers ("%.7g"), row-major. */
+int nrh_affine_read(const char *path, float m[16]);
+int nrh_affine_write(const char *path, const float m[16]);
 
 #ifdef __cplusplus
 }

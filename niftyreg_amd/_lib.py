@@ -55,7 +55,17 @@ class nrh_aladin_params(ctypes.Structure):
 class nrh_aladin_config(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("max_iterations", ctypes.c_int),
                 ("block_percent", ctypes.c_int), ("inlier_lts", ctypes.c_int), ("rigid", ctypes.c_int),
-                ("affine", ctypes.c_int), ("align_centre", ctypes.c_int)]
+                ("affine", ctypes.c_int), ("align_centre", ctypes.c_int), ("symmetric", ctypes.c_int)]
+
+
+class nrh_nifti_info(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int * 8), ("pixdim", ctypes.c_float * 8), ("datatype", ctypes.c_int),
+                ("scl_slope", ctypes.c_float), ("scl_inter", ctypes.c_float), ("qform_code", ctypes.c_int),
+                ("sform_code", ctypes.c_int), ("quatern", ctypes.c_float * 3), ("qoffset", ctypes.c_float * 3),
+                ("qfac", ctypes.c_float), ("srow", ctypes.c_float * 12), ("qto_xyz", ctypes.c_float * 16),
+                ("sto_xyz", ctypes.c_float * 16), ("intent_code", ctypes.c_int), ("intent_p", ctypes.c_float * 3),
+                ("intent_name", ctypes.c_char * 16), ("cal_min", ctypes.c_float), ("cal_max", ctypes.c_float),
+                ("xyzt_units", ctypes.c_int), ("descrip", ctypes.c_char * 80)]
 
 
 _lib = None

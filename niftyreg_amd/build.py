@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnrhip.so")
 
-SOURCES = ["capi.cpp", "capi_eval.cpp", "f3d_ops.hip", "f3d_fused.hip", "driver.cpp", "aladin.hip", "lncc.hip"]
+SOURCES = ["capi.cpp", "capi_eval.cpp", "f3d_ops.hip", "f3d_fused.hip", "driver.cpp", "aladin.hip", "lncc.hip",
+           "nifti_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NRH_OFFLOAD_ARCH", "gfx950")
@@ -44,7 +45,7 @@ def build(verbose=True):
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         objs = list(ex.map(_compile, sources))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

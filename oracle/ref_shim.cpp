@@ -17,7 +17,9 @@
 #include "_reg_blockMatching.h"
 #include "_reg_f3d.h"
 #include "_reg_aladin.h"
+#include "_reg_aladin_sym.h"
 #include "F3dContent.h"
+#include "_reg_ReadWriteMatrix.h"
 #include <cstring>
 #include <vector>
 #include <memory>
@@ -532,13 +534,15 @@ typedef struct {
     int rigid;            // perform the rigid stage
     int affine;           // perform the affine stage
     int align_centre;
+    int symmetric;        // reg_aladin_sym (the reg_aladin default) instead of reg_aladin
 } ref_aladin_params;
 
 // reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364); matrix receives the final affine.
 int ref_aladin_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
                    const ref_aladin_params *p, float *matrix) {
     return guarded([&] {
-        reg_aladin<float> a;
+        std::unique_ptr<reg_aladin<float>> owner(p->symmetric ? new reg_aladin_sym<float>() : new reg_aladin<float>());
+        reg_aladin<float>& a = *owner;
         a.SetPlatformType(PlatformType::Cpu);
         a.SetInputReference(make_image(ref, refData));
         a.SetInputFloating(make_image(flo, floData));
@@ -553,6 +557,108 @@ int ref_aladin_run(const nr_geom *ref, const float *refData, const nr_geom *flo,
         a.SetVerbose(false);
         a.Run();
         from_mat44(*a.GetTransformationMatrix(), matrix);
+    });
+}
+
+// NiftyReg::Mat44Avg2 (reg-lib/cpu/Maths.cpp:479-491): expm((logm(A) + logm(B)) / 2) through Eigen.
+int ref_mat44_avg2(const float *A, const float *B, float *out) {
+    return guarded([&] {
+        mat44 a = to_mat44(A), b = to_mat44(B);
+        from_mat44(NiftyReg::Mat44Avg2(&a, &b), out);
+    });
+}
+
+// nifti_mat44_inverse (reg-io/niftilib/nifti1_io.c:1680-1717).
+int ref_mat44_inverse(const float *A, float *out) {
+    return guarded([&] { from_mat44(nifti_mat44_inverse(to_mat44(A)), out); });
+}
+
+// ---------------------------------------------------------------- file formats
+// What the reference's readers see in a NIfTI file: nifti_image_read + reg_checkAndCorrectDimension
+// (reg_io_ReadImageFile, reg-io/_reg_ReadWriteImage.cpp:51-81, without its PNG / NRRD branches).
+typedef struct {
+    int dim[8];
+    float pixdim[8];
+    int datatype, qform_code, sform_code;
+    float quatern[3], qoffset[3], qfac;
+    float qto_xyz[16], sto_xyz[16];
+    float scl_slope, scl_inter;
+    int intent_code;
+    float intent_p1;
+    char intent_name[16];
+} ref_nifti_fields;
+
+static void fields_of(const nifti_image *im, ref_nifti_fields *f) {
+    std::memset(f, 0, sizeof(*f));
+    for (int i = 0; i < 8; ++i) {
+        f->dim[i] = im->dim[i];
+        f->pixdim[i] = im->pixdim[i];
+    }
+    f->datatype = im->datatype;
+    f->qform_code = im->qform_code;
+    f->sform_code = im->sform_code;
+    f->quatern[0] = im->quatern_b; f->quatern[1] = im->quatern_c; f->quatern[2] = im->quatern_d;
+    f->qoffset[0] = im->qoffset_x; f->qoffset[1] = im->qoffset_y; f->qoffset[2] = im->qoffset_z;
+    f->qfac = im->qfac;
+    from_mat44(im->qto_xyz, f->qto_xyz);
+    from_mat44(im->sto_xyz, f->sto_xyz);
+    f->scl_slope = im->scl_slope;
+    f->scl_inter = im->scl_inter;
+    f->intent_code = im->intent_code;
+    f->intent_p1 = im->intent_p1;
+    std::memcpy(f->intent_name, im->intent_name, 16);
+}
+
+// Reads a file as the reference does; data (optional, capacity floats) receives the voxels
+// converted to float.
+int ref_nifti_read(const char *path, ref_nifti_fields *f, float *data, size_t capacity) {
+    return guarded([&] {
+        nifti_image *im = nifti_image_read(path, data != nullptr);
+        if (!im) throw std::runtime_error(std::string("niftilib cannot read ") + path);
+        NiftiImage img(im, NiftiImage::Copy::Acquire);
+        reg_checkAndCorrectDimension(img);
+        fields_of(img, f);
+        if (data) {
+            reg_tools_changeDatatype<float>(img);
+            if (img.nVoxels() > capacity) throw std::runtime_error("capacity");
+            std::memcpy(data, img->data, img.nVoxels() * sizeof(float));
+        }
+    });
+}
+
+// reg_createControlPointGrid<float> for the reference image in `refPath` and a spacing in mm
+// (as reg_f3d builds its output grid); the grid is written to `gridPath` with niftilib
+// (reg_io_WriteImageFile's NIfTI branch) when gridPath is not null.
+int ref_control_point_grid(const char *refPath, const float spacing[3], const char *gridPath, ref_nifti_fields *f) {
+    return guarded([&] {
+        nifti_image *im = nifti_image_read(refPath, 0);
+        if (!im) throw std::runtime_error(std::string("niftilib cannot read ") + refPath);
+        NiftiImage ref(im, NiftiImage::Copy::Acquire);
+        reg_checkAndCorrectDimension(ref);
+        NiftiImage grid;
+        reg_createControlPointGrid<float>(grid, ref, spacing);
+        fields_of(grid, f);
+        if (gridPath) {
+            nifti_set_filenames(grid, gridPath, 0, 0);
+            nifti_image_write(grid);
+        }
+    });
+}
+
+// reg_tool_WriteAffineFile / reg_tool_ReadAffineFile (reg-io/_reg_ReadWriteMatrix.cpp:83-116).
+int ref_affine_write(const char *path, const float *m) {
+    return guarded([&] {
+        const mat44 a = to_mat44(m);
+        reg_tool_WriteAffineFile(&a, path);
+    });
+}
+
+int ref_affine_read(const char *path, float *m) {
+    return guarded([&] {
+        mat44 a;
+        std::string p(path);
+        reg_tool_ReadAffineFile(&a, &p[0]);
+        from_mat44(a, m);
     });
 }
 

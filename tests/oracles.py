@@ -422,7 +422,7 @@ def nmi_at(refg, flog, cppg, cpp, ref_img, flo_img, bins=64, fp64=False):
 class ref_aladin_params(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("max_iterations", ctypes.c_int),
                 ("block_percent", ctypes.c_int), ("inlier_lts", ctypes.c_int), ("rigid", ctypes.c_int),
-                ("affine", ctypes.c_int), ("align_centre", ctypes.c_int)]
+                ("affine", ctypes.c_int), ("align_centre", ctypes.c_int), ("symmetric", ctypes.c_int)]
 
 
 def block_count(g):
@@ -465,6 +465,19 @@ def ref_block_matching(g, img, warped=None, percent=50, mask=None):
     return tb, n, rp[:3 * n], wp[:3 * n], int(counts[2])
 
 
+def ref_mat44_avg2(A, B):
+    out = np.zeros(16, np.float32)
+    rcheck(ref().ref_mat44_avg2(fp(np.ascontiguousarray(A, np.float32).ravel()),
+                                fp(np.ascontiguousarray(B, np.float32).ravel()), fp(out)))
+    return out.reshape(4, 4)
+
+
+def ref_mat44_inverse(A):
+    out = np.zeros(16, np.float32)
+    rcheck(ref().ref_mat44_inverse(fp(np.ascontiguousarray(A, np.float32).ravel()), fp(out)))
+    return out.reshape(4, 4)
+
+
 def ref_lts(rp, wp, n_defined, percent=50, affine=True, matrix=None):
     m = np.eye(4, dtype=np.float32) if matrix is None else np.array(matrix, np.float32).copy()
     rcheck(ref().ref_lts(fp(rp), fp(wp), len(rp) // 3, n_defined, percent, int(affine), fp(m)))
@@ -472,8 +485,9 @@ def ref_lts(rp, wp, n_defined, percent=50, affine=True, matrix=None):
 
 
 def ref_aladin_run(refg, ref_img, flog, flo_img, levels=3, maxit=5, block_percent=50, inlier=50, rigid=False,
-                   affine=True, align_centre=True):
-    p = ref_aladin_params(levels, levels, maxit, block_percent, inlier, int(rigid), int(affine), int(align_centre))
+                   affine=True, align_centre=True, symmetric=False):
+    p = ref_aladin_params(levels, levels, maxit, block_percent, inlier, int(rigid), int(affine), int(align_centre),
+                          int(symmetric))
     m = np.zeros(16, np.float32)
     rcheck(ref().ref_aladin_run(ctypes.byref(rg(refg)), fp(ref_img.ravel()), ctypes.byref(rg(flog)),
                                 fp(flo_img.ravel()), ctypes.byref(p), fp(m)))
@@ -522,7 +536,8 @@ def plugin_available():
     return os.path.exists(PLUGIN)
 
 
-def plugin_f3d_eval(hip, refg, ref_img, flog, flo_img, cppg, cpp, bins=64, lncc=False, simw=0.999, bew=0.001):
+def plugin_f3d_eval(hip, refg, ref_img, flog, flo_img, cppg, cpp, bins=64, lncc=False, simw=0.999, bew=0.001,
+                    lew=0.01):
     """One objective value + gradient through the reference's own Content / Compute / Measure
     calls, on the reference CPU classes (hip=False) or the HIP platform plugin (hip=True)."""
     global _plugin
@@ -535,7 +550,8 @@ def plugin_f3d_eval(hip, refg, ref_img, flog, flo_img, cppg, cpp, bins=64, lncc=
     rc = _plugin.plugin_f3d_eval(int(hip), ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).ravel()),
                                  ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).ravel()),
                                  ctypes.byref(rg(cppg)), fp(np.ascontiguousarray(cpp, np.float32)), bins, int(lncc),
-                                 ctypes.c_float(simw), ctypes.c_float(bew), ctypes.byref(m), ctypes.byref(be),
+                                 ctypes.c_float(simw), ctypes.c_float(bew), ctypes.c_float(lew), ctypes.byref(m),
+                                 ctypes.byref(be),
                                  fp(grad), fp(vg))
     if rc != 0:
         raise RuntimeError(_plugin.plugin_last_error().decode())
@@ -543,6 +559,13 @@ def plugin_f3d_eval(hip, refg, ref_img, flog, flo_img, cppg, cpp, bins=64, lncc=
 
 
 # ---------------------------------------------------------------- linear energy
+def orc_round_trip(cppg, cpp):
+    """The control points after the reference's bending-energy gradient: (p - node) + node."""
+    c = np.ascontiguousarray(cpp, np.float32).copy()
+    orc().orc_displacement_round_trip(ctypes.byref(og(cppg)), fp(c))
+    return c
+
+
 def orc_le(cppg, cpp):
     orc().orc_linear_energy.restype = ctypes.c_double
     return orc().orc_linear_energy(ctypes.byref(og(cppg)), fp(cpp))
@@ -564,3 +587,66 @@ def ref_le_grad(cppg, cpp, weight, init=None):
     g = np.zeros(3 * cppg.nvox, np.float32) if init is None else init.astype(np.float32).copy()
     rcheck(ref().ref_le_gradient(ctypes.byref(rg(cppg)), fp(cpp), ctypes.c_float(weight), fp(g)))
     return g
+
+
+# ---------------------------------------------------------------- symmetric Aladin matrices
+def orc_nifti_inverse(A):
+    out = np.zeros(16, np.float32)
+    orc().orc_nifti_mat44_inverse(fp(np.ascontiguousarray(A, np.float32).ravel()), fp(out))
+    return out.reshape(4, 4)
+
+
+def orc_avg2(A, B):
+    out = np.zeros(16, np.float32)
+    ok = orc().orc_mat44_avg2(fp(np.ascontiguousarray(A, np.float32).ravel()),
+                              fp(np.ascontiguousarray(B, np.float32).ravel()), fp(out))
+    assert ok, "matrix logarithm failed"
+    return out.reshape(4, 4)
+
+
+def orc_symmetric_update(fw, bw):
+    f = np.ascontiguousarray(fw, np.float32).ravel().copy()
+    b = np.ascontiguousarray(bw, np.float32).ravel().copy()
+    assert orc().orc_symmetric_update(fp(f), fp(b)), "matrix logarithm failed"
+    return f.reshape(4, 4), b.reshape(4, 4)
+
+
+# ---------------------------------------------------------------- file formats (niftilib)
+class ref_nifti_fields(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int * 8), ("pixdim", ctypes.c_float * 8), ("datatype", ctypes.c_int),
+                ("qform_code", ctypes.c_int), ("sform_code", ctypes.c_int), ("quatern", ctypes.c_float * 3),
+                ("qoffset", ctypes.c_float * 3), ("qfac", ctypes.c_float), ("qto_xyz", ctypes.c_float * 16),
+                ("sto_xyz", ctypes.c_float * 16), ("scl_slope", ctypes.c_float), ("scl_inter", ctypes.c_float),
+                ("intent_code", ctypes.c_int), ("intent_p1", ctypes.c_float), ("intent_name", ctypes.c_char * 16)]
+
+
+def ref_nifti_read(path, with_data=True):
+    f = ref_nifti_fields()
+    data = None
+    if with_data:
+        f0 = ref_nifti_fields()
+        rcheck(ref().ref_nifti_read(str(path).encode(), ctypes.byref(f0), None, ctypes.c_size_t(0)))
+        n = int(np.prod([f0.dim[i] for i in range(1, 8)]))
+        data = np.zeros(n, np.float32)
+        rcheck(ref().ref_nifti_read(str(path).encode(), ctypes.byref(f), fp(data), ctypes.c_size_t(n)))
+    else:
+        rcheck(ref().ref_nifti_read(str(path).encode(), ctypes.byref(f), None, ctypes.c_size_t(0)))
+    return f, data
+
+
+def ref_control_point_grid(ref_path, spacing, grid_path=None):
+    f = ref_nifti_fields()
+    sp = (ctypes.c_float * 3)(*[float(v) for v in spacing])
+    rcheck(ref().ref_control_point_grid(str(ref_path).encode(), sp,
+                                        None if grid_path is None else str(grid_path).encode(), ctypes.byref(f)))
+    return f
+
+
+def ref_affine_write(path, m):
+    rcheck(ref().ref_affine_write(str(path).encode(), fp(np.ascontiguousarray(m, np.float32).ravel())))
+
+
+def ref_affine_read(path):
+    m = np.zeros(16, np.float32)
+    rcheck(ref().ref_affine_read(str(path).encode(), fp(m)))
+    return m.reshape(4, 4)
