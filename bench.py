@@ -35,6 +35,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--storage", choices=["f32", "f16"], default="f32")
+    p.add_argument("--no-secondary", action="store_true", help="skip the Aladin and LNCC lines")
     return p.parse_args()
 
 
@@ -61,6 +62,118 @@ def cpu_baseline(ref, flo, geom):
     return {"value": n / dt, "unit": "evals/s", "cores": threads, "kind": "reference",
             "sample": f"{n} obj+grad evals of reg_f3d<float> at 256x256x160 (one level, -be 0.001 -le 0, 64 bins), "
                       f"OMP_NUM_THREADS={threads}, after one warm-up eval"}
+
+
+def _sync_time(torch, fn, n):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / n
+
+
+def bench_aladin(torch, steps, with_cpu):
+    """One Aladin iteration (warp, block matching, LTS) on the 128^3 blob pair with a random
+    affine (50% of the 32^3 blocks, capture range 3, 50% LTS inliers), inputs resident in HBM."""
+    from niftyreg_amd import phantoms
+    from niftyreg_amd.aladin import AladinContent
+    from niftyreg_amd.geometry import Geom
+    dims = (128, 128, 128)
+    ref, flo, _ = phantoms.config3_pair(0, dims)
+    g = Geom.identity(dims)
+    h = AladinContent(g, ref, g, flo)
+    eye = np.eye(4, dtype=np.float32)
+    h.set_matrix(eye)
+    for _ in range(3):
+        h.GetWarpedImage()
+        h.BlockMatching()
+        h.Lts(True)
+    h.set_matrix(eye)
+    warp_s = _sync_time(torch, h.GetWarpedImage, steps)
+    bm_s = _sync_time(torch, h.BlockMatching, steps)
+    lts_s = _sync_time(torch, lambda: (h.set_matrix(eye), h.Lts(True)), steps)
+    it_s = warp_s + bm_s + lts_s
+    N = g.nvox
+    active = h.active_blocks
+    flops = active * 343 * 64 * 11.0  # pass 1: 3 adds; pass 2: 2 sub, 3 mul, 3 add (per offset and voxel)
+    out = {
+        "metric": "Aladin iterations/s at 128^3 (warp + block matching + LTS)",
+        "value": 1.0 / it_s, "unit": "iterations/s",
+        "ms": {"affine_warp": warp_s * 1e3, "block_matching": bm_s * 1e3, "lts": lts_s * 1e3},
+        "config": {"workload": "128^3 float32 blob pair, random affine, 4^3 blocks, 50% kept, capture 3, LTS 50%",
+                   "active_blocks": active},
+        "roofline": {"bound": "valu", "kernel": "k_block_match", "achieved": flops / bm_s / 1e12,
+                     "peak": 157.3, "unit": "TFLOP/s", "frac": flops / bm_s / 1e12 / 157.3,
+                     "hbm_GBs": 8.0 * N / bm_s / 1e9},
+        "cpu_baseline": None,
+    }
+    h.close()
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracles
+        if oracles.ref_available():
+            A = eye.copy()
+            t0 = time.perf_counter()
+            n = 0
+            while n < 3:
+                field = oracles.ref_affine_field(g, A)
+                w = oracles.ref_resample(g, flo.ravel(), g, field)
+                _, _, rp, wp, d = oracles.ref_block_matching(g, ref.ravel(), w, 50)
+                oracles.ref_lts(rp, wp, d, 50, True, A)
+                n += 1
+            dt = time.perf_counter() - t0
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            out["cpu_baseline"] = {"value": n / dt, "unit": "iterations/s", "cores": threads, "kind": "reference",
+                                   "sample": f"{n} iterations of reg_affine_getDeformationField + reg_resampleImage + "
+                                             "initialise_block_matching_method + block_matching_method + optimize "
+                                             f"(reference CPU build) at 128^3, OMP_NUM_THREADS={threads}"}
+    return out
+
+
+def bench_lncc(torch, ref, flo, geom, cppg, cpp, steps, with_cpu):
+    """F3D with LNCC (Gaussian sigma 5 voxels): one objective value + one gradient at 256x256x160
+    through the per-operation chain."""
+    from niftyreg_amd import _lib
+    from niftyreg_amd.f3d import F3dContent, default_params
+    h = F3dContent(geom, ref, geom, flo, cppg, cpp,
+                   params=default_params(similarity_weight=1.0 - BE, be_weight=BE, measure=_lib.MEASURE_LNCC,
+                                         lncc_sigma=-5.0))
+    h.rescale_intensities((0, 1), (0, 1))
+    h.objective()
+    h.gradient()
+    step_s = _sync_time(torch, lambda: (h.objective(), h.gradient()), steps)
+    value, terms = h.objective()
+    h.close()
+    N = geom.nvox
+    # the two statistics sets: 6 + 4 planar arrays through 3 axis passes, read + write, plus the rest
+    conv_bytes = (6 + 6 + 4) * 2 * 4 * 3 * N
+    out = {
+        "metric": "F3D LNCC obj+grad evals/s at 256x256x160", "value": 1.0 / step_s, "unit": "evals/s",
+        "ms_per_step": step_s * 1e3, "lncc": terms[0] / (1.0 - BE),
+        "config": {"workload": "F3D LNCC (Gaussian sigma 5 voxels) objective+gradient, 256x256x160 shell pair, "
+                               "BE 0.001, 5-voxel grid"},
+        "roofline": {"bound": "hbm", "kernel": "k_conv_axis (all passes of one eval)",
+                     "achieved": conv_bytes / step_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": conv_bytes / step_s / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": None,
+    }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracles
+        if oracles.ref_available():
+            r = oracles.RefF3d(geom, ref, geom, flo, be=BE, le=0.0, spacing=-5.0, levels=1, measure=1, lncc_sigma=-5.0)
+            r.set_cpp(cpp)
+            t0 = time.perf_counter()
+            r.value()
+            r.gradient()
+            dt = time.perf_counter() - t0
+            r.close()
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "evals/s", "cores": threads, "kind": "reference",
+                                   "sample": f"1 LNCC obj+grad eval of reg_f3d<float> at 256x256x160, "
+                                             f"OMP_NUM_THREADS={threads}"}
+    return out
 
 
 def measured_traffic():
@@ -155,6 +268,9 @@ def main():
     h.close()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ref, flo, geom)
+    if not a.no_secondary:
+        out["secondary"] = [bench_aladin(torch, 20, not a.no_cpu_baseline),
+                            bench_lncc(torch, ref, flo, geom, cppg, cpp, 10, not a.no_cpu_baseline)]
     print(json.dumps(out))
 
 
