@@ -1,26 +1,24 @@
 // Fused F3D evaluation for gfx950.
 //
-// One objective evaluation is one sweep over the reference grid (k_fused_pass<false>):
-// B-spline field, trilinear warp and joint-histogram fill, with no field or warped image
-// written to HBM.  One gradient evaluation is a second sweep (k_fused_pass<true>): the field
-// and warp are recomputed together with the image gradient and the NMI voxel gradient, and
-// the voxel gradient is projected straight onto the control points.
+// One objective evaluation is one sweep over the reference grid (k_sweep<false>): B-spline
+// field, trilinear warp and joint-histogram fill, with no field or warped image written to HBM.
+// One gradient evaluation is a second sweep (k_sweep<true>): the field and warp are recomputed
+// together with the image gradient and the NMI voxel gradient, which is written once (planar
+// [3][N]); three separable passes then project it onto the control points.
 //
-// Tiles.  A workgroup owns a 64 x 8 x 8 voxel tile (x fastest, one wave per 64-voxel row).
-// The control points that influence the tile (a window of at most 20 x 6 x 6 nodes when the
-// grid spacing is >= 4 voxels) are staged in LDS, and the cubic B-spline sum is evaluated
-// separably: z contraction for the whole tile, y contraction per z slice, x contraction per
-// voxel -- about 17 FMA per voxel and component instead of 64.
+// Tiles.  A workgroup owns a 64 x 8 x 8 voxel tile (x fastest).  The control points that
+// influence the tile (a window of at most 20 x 6 x 6 nodes when the grid spacing is >= 4 voxels)
+// are staged in LDS and contracted along z and y for every row of the tile at once; each voxel
+// then needs only its 4-tap x contraction, and the voxels of a tile run without barriers.
 //
-// Node gradient.  The reference convolves the voxel gradient with the cubic B-spline kernel
-// of radius 2*spacing along x, y, z (each pass normalised by the in-image kernel sum) and
-// samples the result at the nodes (reg-lib/Compute.cpp:230-281).  When the grid spacing is an
-// integer number of voxels and the nodes fall on voxels, the weight of voxel v on node n is
-// exactly the field's basis value B_{n - pre(v)}(t(v)) on each axis, divided by the per-axis
-// normalisation at the node.  The gradient sweep therefore applies the transpose of the
-// separable field evaluation to the voxel gradient (x, then y, then z) and writes one
-// partial node window per tile; k_node_reduce sums the windows in a fixed order
-// (deterministic), normalises, reorients and weights them.
+// Node gradient.  The reference convolves the voxel gradient with the cubic B-spline kernel of
+// radius 2*spacing along x, y, z (each pass normalised by the in-image kernel sum) and samples the
+// result at the nodes (reg-lib/Compute.cpp:230-281).  When the grid spacing is an integer number
+// of voxels and the nodes fall on voxels, the weight of voxel v on node n is exactly the field's
+// basis value B_{n - pre(v)}(t(v)) on each axis, divided by the per-axis normalisation at the
+// node.  The projection is therefore the transpose of the separable field evaluation: a pass
+// along x, one along y, one along z (with normalisation, reorientation and weight), each summing
+// in a fixed order (deterministic).
 
 #include "device.hpp"
 #include "f3d_handle.hpp"
@@ -34,7 +32,6 @@ namespace {
 
 constexpr int TX = 64, TY = 8, TZ = 8;
 constexpr int WXM = 20, WYM = 6, WZM = 6;  // node-window bounds for spacing >= 4 voxels
-constexpr int WIN = WXM * WYM * WZM;      // floats per component of one tile window
 constexpr int kThreads = 256;
 
 struct FusedArgs {
@@ -56,7 +53,8 @@ struct FusedArgs {
     uint32_t *hist;
     const float *logTab;
     const double *ent;
-    float *partials;
+    const float *kTab;  // gradient sweep: K[r + w rb] = log p_r + log p_w - NMI log p_rw (padded to 4)
+    float *gvox;      // gradient sweep: the NMI voxel gradient (planar [3][N])
     float *fieldOut;  // optional: the deformation field of the sweep (planar [3][N]), for checking
 };
 
@@ -73,18 +71,151 @@ __device__ __forceinline__ int xcd_tile(int b, int numTiles) {
     return t;
 }
 
+// One voxel of a sweep, in two steps so that a wave can have the floating-image gathers of
+// several voxels in flight: sweep_fetch evaluates the B-spline field from the tile's
+// y/z-contracted window, maps it into the floating image and loads the 8 trilinear taps
+// (branch-free: clamped addresses, out-of-image taps replaced by the padding value afterwards);
+// sweep_finish interpolates and either counts the joint histogram (value sweep) or forms the NMI
+// voxel gradient (gradient sweep).
+struct VoxelTaps {
+    float v[2][2][2];
+    float rx, ry, rz;
+    float rv;
+    bool active;
+};
+
 template<class TImg, bool GRAD>
-__global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
-    // ---- LDS
-    __shared__ float sCp[3][WZM][WYM][WXM];
-    __shared__ float sAyz[TY][WXM][3];
+__device__ __forceinline__ VoxelTaps sweep_fetch(const FusedArgs a, const float *ayz, const float4 b, const int pxRel,
+                                                 const int px, const float4 by, const int py, const float4 bz,
+                                                 const int pz, const size_t idx) {
+    const TImg *ref = (const TImg *)a.ref;
+    const TImg *flo = (const TImg *)a.flo;
+    VoxelTaps t;
+    t.active = !a.mask || a.mask[idx] > -1;
+    // field = identity + displacement: the B-spline sum of the node positions is the affine map of
+    // (pre + t + 1) on each axis (cubic B-splines reproduce linear functions), evaluated in double;
+    // the window holds displacements.
+    const double ux = (double)px + (double)(b.y + 2.f * b.z + 3.f * b.w);
+    const double uy = (double)py + (double)(by.y + 2.f * by.z + 3.f * by.w);
+    const double uz = (double)pz + (double)(bz.y + 2.f * bz.z + 3.f * bz.w);
+    float w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float *q = ayz + pxRel * 3 + c;
+        const float d = b.x * q[0] + b.y * q[3] + b.z * q[6] + b.w * q[9];
+        const double id = (double)a.cppXyz.m[c * 4] * ux + (double)a.cppXyz.m[c * 4 + 1] * uy +
+                          (double)a.cppXyz.m[c * 4 + 2] * uz + (double)a.cppXyz.m[c * 4 + 3];
+        w[c] = (float)(id + (double)d);
+    }
+    if (!GRAD && a.fieldOut && t.active) {
+        const size_t N = (size_t)a.nx * a.ny * a.nz;
+        a.fieldOut[idx] = w[0];
+        a.fieldOut[idx + N] = w[1];
+        a.fieldOut[idx + 2 * N] = w[2];
+    }
+    const float3 p = affine_apply_f(a.floIjk, w[0], w[1], w[2]);
+    const int fx = floor_int(p.x), fy = floor_int(p.y), fz = floor_int(p.z);
+    t.rx = p.x - (float)fx;
+    t.ry = p.y - (float)fy;
+    t.rz = p.z - (float)fz;
+    const int X[2] = { min(max(fx, 0), a.fnx - 1), min(max(fx + 1, 0), a.fnx - 1) };
+    const int Y[2] = { min(max(fy, 0), a.fny - 1), min(max(fy + 1, 0), a.fny - 1) };
+    const int Z[2] = { min(max(fz, 0), a.fnz - 1), min(max(fz + 1, 0), a.fnz - 1) };
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa) t.v[c][bb][aa] = ld(flo, ((size_t)Z[c] * a.fny + Y[bb]) * a.fnx + X[aa]);
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa) {
+                const bool in = (unsigned)(fx + aa) < (unsigned)a.fnx && (unsigned)(fy + bb) < (unsigned)a.fny &&
+                                (unsigned)(fz + c) < (unsigned)a.fnz;
+                t.v[c][bb][aa] = in ? t.v[c][bb][aa] : a.pad;
+            }
+    t.rv = ld(ref, idx);
+    return t;
+}
+
+template<bool GRAD>
+__device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, float *dyn, float coef, float& g0,
+                                             float& g1, float& g2) {
+    g0 = g1 = g2 = 0.f;
+    if (t.rx < 0) t.rx = 0;
+    if (t.ry < 0) t.ry = 0;
+    if (t.rz < 0) t.rz = 0;
+    const float rx = t.rx, ry = t.ry, rz = t.rz;
+    const float x0b = 1.f - rx, y0b = 1.f - ry, z0b = 1.f - rz;
+    const float e00 = t.v[0][0][0] * x0b + t.v[0][0][1] * rx, e01 = t.v[0][1][0] * x0b + t.v[0][1][1] * rx;
+    const float e10 = t.v[1][0][0] * x0b + t.v[1][0][1] * rx, e11 = t.v[1][1][0] * x0b + t.v[1][1][1] * rx;
+    const float f0 = e00 * y0b + e01 * ry, f1 = e10 * y0b + e11 * ry;
+    const float war = f0 * z0b + f1 * rz;
+    const float rv = t.rv;
+    if (!GRAD) {
+        if (t.active && rv == rv && war == war && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
+            atomicAdd(&((uint32_t *)dyn)[(int)rv + (int)war * a.rb], 1u);
+    } else if (t.active && rv == rv && war == war) {
+        // image gradient in floating voxel units (TrilinearImageGradient)
+        const float d00 = t.v[0][0][1] - t.v[0][0][0], d01 = t.v[0][1][1] - t.v[0][1][0];
+        const float d10 = t.v[1][0][1] - t.v[1][0][0], d11 = t.v[1][1][1] - t.v[1][1][0];
+        float gx = (d00 * y0b + d01 * ry) * z0b + (d10 * y0b + d11 * ry) * rz;
+        float gy = (e01 - e00) * z0b + (e11 - e10) * rz;
+        float gz = f1 - f0;
+        if (gx != gx) gx = 0.f;
+        if (gy != gy) gy = 0.f;
+        if (gz != gz) gz = 0.f;
+        // NMI derivative (reg_getVoxelBasedNmiGradient3d, reg-lib/cpu/_reg_nmi.cpp:501-593):
+        //   (sum_r B_r log p_r)(sum_w B'_w) + (sum_r B_r)(sum_w B'_w log p_w)
+        //   - NMI sum_r sum_w B_r B'_w log p_rw  =  sum_r sum_w B_r B'_w K[r, w]
+        // over the 4 bins around each intensity (bins outside the histogram get weight 0, as the
+        // reference skips them).  B_r: cubic B-spline weights of the reference intensity's
+        // fraction; B'_w: their derivative for the warped one.
+        const float rfl = floorf(rv), wfl = floorf(war);
+        const float f = rv - rfl, gg = war - wfl;
+        const float f2 = f * f, f3 = f2 * f, omf = 1.f - f;
+        float br[4] = { omf * omf * omf * (1.f / 6.f), 0.66666669f - f2 + 0.5f * f3,
+                        (-3.f * f3 + 3.f * f2 + 3.f * f + 1.f) * (1.f / 6.f), f3 * (1.f / 6.f) };
+        const float omg = 1.f - gg;
+        float dw[4] = { -0.5f * omg * omg, (1.5f * gg - 2.f) * gg, (0.5f + 1.5f * gg) * omg, 0.5f * gg * gg };
+        const int rb0 = (int)rfl - 1, wb0 = (int)wfl - 1;
+        int ri[4], wc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool okr = (unsigned)(rb0 + k) < (unsigned)a.rb;
+            const bool okw = (unsigned)(wb0 + k) < (unsigned)a.fb;
+            ri[k] = okr ? rb0 + k : 0;
+            br[k] = okr ? br[k] : 0.f;
+            wc[k] = (okw ? wb0 + k : 0) * a.rb;
+            dw[k] = okw ? dw[k] : 0.f;
+        }
+        float S = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float T = br[0] * dyn[ri[0] + wc[j]] + br[1] * dyn[ri[1] + wc[j]] + br[2] * dyn[ri[2] + wc[j]] +
+                            br[3] * dyn[ri[3] + wc[j]];
+            S += dw[j] * T;
+        }
+        S *= coef;
+        g0 = S * gx;
+        g1 = S * gy;
+        g2 = S * gz;
+    }
+}
+
+// A 64 x 8 x 8 tile per workgroup.  The control-point window is staged in LDS and contracted
+// along z and y for every (z, y) row of the tile in one pass; the voxels then run without
+// barriers (one wave per x row), which keeps many floating-image gathers in flight.  The window
+// shares LDS with the histogram / NMI table, which are filled after the contraction.
+template<class TImg, bool GRAD>
+__global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
+    __shared__ float sAyz[TZ][TY][WXM][3];
     __shared__ float4 sBx[TX], sBy[TY], sBz[TZ];
     __shared__ int sPx[TX], sPy[TY], sPz[TZ];
-    __shared__ int sXFirst[WXM], sXLast[WXM];
-    extern __shared__ float dyn[];  // histogram (value) or log table (gradient)
-    __shared__ float sG[GRAD ? TY : 1][GRAD ? TX : 1][3];
-    __shared__ float sU1[GRAD ? TY : 1][GRAD ? WXM : 1][3];
-    __shared__ float sOut[GRAD ? WZM : 1][GRAD ? WYM : 1][GRAD ? WXM : 1][3];
+    extern __shared__ __align__(16) float dyn[];  // window [3][WZM][WYM][WXM], then histogram or table K
 
     const int tid = threadIdx.x;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -92,10 +223,7 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
     const int tx = t % a.tilesX, ty = (t / a.tilesX) % a.tilesY, tz = t / (a.tilesX * a.tilesY);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
     const int nX = min(TX, a.nx - x0), nY = min(TY, a.ny - y0), nZ = min(TZ, a.nz - z0);
-    const TImg *ref = (const TImg *)a.ref;
-    const TImg *flo = (const TImg *)a.flo;
 
-    // ---- per-axis basis tables of the tile
     if (tid < TX) {
         const int x = x0 + min(tid, nX - 1);
         sPx[tid] = a.preX[x];
@@ -113,302 +241,262 @@ __global__ void __launch_bounds__(kThreads, 2) k_fused_pass(FusedArgs a) {
     const int WX = a.preX[x0 + nX - 1] - px0 + 4;
     const int WY = a.preY[y0 + nY - 1] - py0 + 4;
     const int WZ = a.preZ[z0 + nZ - 1] - pz0 + 4;
-
+    float *win = dyn;  // [c][iz][iy][ix] with strides WZM*WYM*WXM, WYM*WXM, WXM
+    {
+        // every load of the window is issued before the first LDS store; thread (row slot, ix) with
+        // compile-time row strides, so no index needs a division by a run-time size
+        constexpr int kRows = 3 * WZM * WYM, kSlots = kThreads / 32, kIter = (kRows + kSlots - 1) / kSlots;
+        const int ix = tid & 31;
+        float cv[kIter];
+#pragma unroll
+        for (int u = 0; u < kIter; ++u) {
+            const int r = (tid >> 5) + u * kSlots, c = r / (WZM * WYM), iz = (r / WYM) % WZM, iy = r % WYM;
+            if (r < kRows && ix < WX && iz < WZ && iy < WY)
+                cv[u] = a.cpp[c * a.G + ((size_t)(pz0 + iz) * a.gny + (py0 + iy)) * a.gnx + (px0 + ix)];
+        }
+#pragma unroll
+        for (int u = 0; u < kIter; ++u) {
+            const int r = (tid >> 5) + u * kSlots, c = r / (WZM * WYM), iz = (r / WYM) % WZM, iy = r % WYM;
+            if (r < kRows && ix < WX && iz < WZ && iy < WY) {
+                const double node = (double)a.cppXyz.m[c * 4] * (px0 + ix) +
+                                    (double)a.cppXyz.m[c * 4 + 1] * (py0 + iy) +
+                                    (double)a.cppXyz.m[c * 4 + 2] * (pz0 + iz) + (double)a.cppXyz.m[c * 4 + 3];
+                win[r * WXM + ix] = (float)((double)cv[u] - node);
+            }
+        }
+    }
+    __syncthreads();
+    // z then y contraction for every (z, y) row of the tile
+    for (int i = tid; i < nZ * nY * WX; i += kThreads) {
+        const int ix = i % WX, y = (i / WX) % nY, z = i / (WX * nY);
+        const float4 bz = sBz[z], b = sBy[y];
+        const int oz = sPz[z] - pz0, o = sPy[y] - py0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float *w0 = win + ((c * WZM + oz) * WYM + o) * WXM + ix;
+            float az[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                az[j] = bz.x * w0[j * WXM] + bz.y * w0[WYM * WXM + j * WXM] + bz.z * w0[2 * WYM * WXM + j * WXM] +
+                        bz.w * w0[3 * WYM * WXM + j * WXM];
+            sAyz[z][y][ix][c] = b.x * az[0] + b.y * az[1] + b.z * az[2] + b.w * az[3];
+        }
+    }
+    __syncthreads();
+    float coef = 0.f;
     if (GRAD) {
-        // K[r + w rb] = log p_r + log p_w - NMI log p_rw: the voxel NMI derivative is then the
-        // bicubic contraction sum_r sum_w B(ref - r) B'(war - w) K[r, w] (see below)
-        const float nmiF = (float)a.ent[4];
-        const float *lj = a.logTab, *lr = a.logTab + a.rb * a.fb, *lw = lr + a.rb;
-        for (int i = tid; i < a.rb * a.fb; i += kThreads) dyn[i] = (lr[i % a.rb] + lw[i / a.rb]) - nmiF * lj[i];
+        // the table K of k_nmi_table: the voxel NMI derivative is the bicubic contraction
+        // sum_r sum_w B(ref - r) B'(war - w) K[r, w]; four float4 loads in flight per thread
+        const int n4 = (a.rb * a.fb + 3) / 4;
+        const float4 *src = (const float4 *)a.kTab;
+        float4 *dst4 = (float4 *)dyn;
+        for (int base = 0; base < n4; base += 4 * kThreads) {
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = base + u * kThreads + tid;
+                if (i < n4) v[u] = src[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = base + u * kThreads + tid;
+                if (i < n4) dst4[i] = v[u];
+            }
+        }
+        coef = (float)(1.0 / (a.ent[2] * a.ent[3]));
     } else {
         uint32_t *h = (uint32_t *)dyn;
         for (int i = tid; i < a.rb * a.fb; i += kThreads) h[i] = 0u;
     }
-    // ---- control-point window
-    for (int i = tid; i < 3 * WZ * WY * WX; i += kThreads) {
-        const int ix = i % WX, iy = (i / WX) % WY, iz = (i / (WX * WY)) % WZ, c = i / (WX * WY * WZ);
-        const double node = (double)a.cppXyz.m[c * 4] * (px0 + ix) + (double)a.cppXyz.m[c * 4 + 1] * (py0 + iy) +
-                            (double)a.cppXyz.m[c * 4 + 2] * (pz0 + iz) + (double)a.cppXyz.m[c * 4 + 3];
-        sCp[c][iz][iy][ix] =
-            (float)((double)a.cpp[c * a.G + ((size_t)(pz0 + iz) * a.gny + (py0 + iy)) * a.gnx + (px0 + ix)] - node);
-    }
-    __syncthreads();
-    if (GRAD && tid < WX) {
-        // voxels of the tile whose 4-node support contains window node tid
-        int f = nX, l = -1;
-        for (int x = 0; x < nX; ++x) {
-            const int o = tid - (sPx[x] - px0);
-            if (o >= 0 && o < 4) { f = min(f, x); l = max(l, x); }
-        }
-        sXFirst[tid] = f;
-        sXLast[tid] = l;
-    }
-    if (GRAD) {
-        for (int i = tid; i < WZM * WYM * WXM * 3; i += kThreads) (&sOut[0][0][0][0])[i] = 0.f;
-    }
     __syncthreads();
 
-    const int lane = tid & 63, row = tid >> 6;
-    double coefD = 0, nmiD = 0;
-    if (GRAD) {
-        nmiD = a.ent[4];
-        coefD = 1.0 / (a.ent[2] * a.ent[3]);
-    }
-    const float coef = (float)coefD, nmi = (float)nmiD;
-    const int nb2 = a.rb * a.fb;
-
-    for (int z = 0; z < nZ; ++z) {
-        // ---- stage B: z then y contraction of the window for this slice
-        {
-            const float4 bz = sBz[z];
-            const int oz = sPz[z] - pz0;
-            for (int i = tid; i < nY * WX; i += kThreads) {
-                const int ix = i % WX, y = i / WX;
-                const float4 b = sBy[y];
-                const int o = sPy[y] - py0;
-                for (int c = 0; c < 3; ++c) {
-                    float az[4];
-                    for (int j = 0; j < 4; ++j)
-                        az[j] = bz.x * sCp[c][oz][o + j][ix] + bz.y * sCp[c][oz + 1][o + j][ix] +
-                                bz.z * sCp[c][oz + 2][o + j][ix] + bz.w * sCp[c][oz + 3][o + j][ix];
-                    sAyz[y][ix][c] = b.x * az[0] + b.y * az[1] + b.z * az[2] + b.w * az[3];
-                }
-            }
-        }
-        __syncthreads();
-        // ---- per voxel: field, warp (+ gradient)
-        for (int y = row; y < TY; y += kThreads / 64) {
-            const int x = lane;
-            float g0 = 0.f, g1 = 0.f, g2 = 0.f;
-            if (x < nX && y < nY) {
-                const size_t idx = ((size_t)(z0 + z) * a.ny + (y0 + y)) * a.nx + (x0 + x);
-                const bool active = !a.mask || a.mask[idx] > -1;
-                if (active) {
-                    const float4 b = sBx[x];
-                    const int o = sPx[x] - px0;
-                    // field = identity + displacement: the B-spline sum of the node positions is
-                    // the affine map of (pre + t + 1) on each axis (cubic B-splines reproduce
-                    // linear functions), evaluated in double; the window holds displacements.
-                    const float4 by = sBy[y], bz = sBz[z];
-                    const double ux = (double)sPx[x] + (double)(b.y + 2.f * b.z + 3.f * b.w);
-                    const double uy = (double)sPy[y] + (double)(by.y + 2.f * by.z + 3.f * by.w);
-                    const double uz = (double)sPz[z] + (double)(bz.y + 2.f * bz.z + 3.f * bz.w);
-                    float w[3];
-                    for (int c = 0; c < 3; ++c) {
-                        const float d = b.x * sAyz[y][o][c] + b.y * sAyz[y][o + 1][c] + b.z * sAyz[y][o + 2][c] +
-                                        b.w * sAyz[y][o + 3][c];
-                        const double id = (double)a.cppXyz.m[c * 4] * ux + (double)a.cppXyz.m[c * 4 + 1] * uy +
-                                          (double)a.cppXyz.m[c * 4 + 2] * uz + (double)a.cppXyz.m[c * 4 + 3];
-                        w[c] = (float)(id + (double)d);
-                    }
-                    if (a.fieldOut) {
-                        a.fieldOut[idx] = w[0];
-                        a.fieldOut[idx + (size_t)a.nx * a.ny * a.nz] = w[1];
-                        a.fieldOut[idx + 2 * (size_t)a.nx * a.ny * a.nz] = w[2];
-                    }
-                    const float3 p = affine_apply_f(a.floIjk, w[0], w[1], w[2]);
-                    const int fx = floor_int(p.x), fy = floor_int(p.y), fz = floor_int(p.z);
-                    float rx = p.x - (float)fx, ry = p.y - (float)fy, rz = p.z - (float)fz;
-                    float v[2][2][2];
-                    const bool inside = fx >= 0 && fx + 1 < a.fnx && fy >= 0 && fy + 1 < a.fny && fz >= 0 && fz + 1 < a.fnz;
-                    if (inside) {
-                        const size_t base = ((size_t)fz * a.fny + fy) * a.fnx + fx;
-                        const size_t sy = a.fnx, sz = (size_t)a.fnx * a.fny;
-                        v[0][0][0] = ld(flo, base);          v[0][0][1] = ld(flo, base + 1);
-                        v[0][1][0] = ld(flo, base + sy);     v[0][1][1] = ld(flo, base + sy + 1);
-                        v[1][0][0] = ld(flo, base + sz);     v[1][0][1] = ld(flo, base + sz + 1);
-                        v[1][1][0] = ld(flo, base + sz + sy); v[1][1][1] = ld(flo, base + sz + sy + 1);
-                    } else {
-                        for (int c = 0; c < 2; ++c)
-                            for (int bb = 0; bb < 2; ++bb)
-                                for (int aa = 0; aa < 2; ++aa) {
-                                    const int X = fx + aa, Y = fy + bb, Z = fz + c;
-                                    v[c][bb][aa] = (X >= 0 && X < a.fnx && Y >= 0 && Y < a.fny && Z >= 0 && Z < a.fnz)
-                                        ? ld(flo, ((size_t)Z * a.fny + Y) * a.fnx + X) : a.pad;
-                                }
-                    }
-                    if (rx < 0) rx = 0;
-                    if (ry < 0) ry = 0;
-                    if (rz < 0) rz = 0;
-                    const float x0b = 1.f - rx, y0b = 1.f - ry, z0b = 1.f - rz;
-                    const float e00 = v[0][0][0] * x0b + v[0][0][1] * rx, e01 = v[0][1][0] * x0b + v[0][1][1] * rx;
-                    const float e10 = v[1][0][0] * x0b + v[1][0][1] * rx, e11 = v[1][1][0] * x0b + v[1][1][1] * rx;
-                    const float f0 = e00 * y0b + e01 * ry, f1 = e10 * y0b + e11 * ry;
-                    const float war = f0 * z0b + f1 * rz;
-                    const float rv = ld(ref, idx);
-                    if (!GRAD) {
-                        if (rv == rv && war == war && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
-                            atomicAdd(&((uint32_t *)dyn)[(int)rv + (int)war * a.rb], 1u);
-                    } else if (rv == rv && war == war) {
-                        // image gradient in floating voxel units (TrilinearImageGradient)
-                        const float d00 = v[0][0][1] - v[0][0][0], d01 = v[0][1][1] - v[0][1][0];
-                        const float d10 = v[1][0][1] - v[1][0][0], d11 = v[1][1][1] - v[1][1][0];
-                        float gx = (d00 * y0b + d01 * ry) * z0b + (d10 * y0b + d11 * ry) * rz;
-                        float gy = (e01 - e00) * z0b + (e11 - e10) * rz;
-                        float gz = f1 - f0;
-                        if (gx != gx) gx = 0.f;
-                        if (gy != gy) gy = 0.f;
-                        if (gz != gz) gz = 0.f;
-                        // NMI derivative (reg_getVoxelBasedNmiGradient3d, reg-lib/cpu/_reg_nmi.cpp:501-593):
-                        //   (sum_r B_r log p_r)(sum_w B'_w) + (sum_r B_r)(sum_w B'_w log p_w)
-                        //   - NMI sum_r sum_w B_r B'_w log p_rw  =  sum_r sum_w B_r B'_w K[r, w]
-                        // over the 4 bins around each intensity (bins outside the histogram get
-                        // weight 0, as the reference skips them).  B_r: cubic B-spline weights of the
-                        // reference intensity's fraction; B'_w: their derivative for the warped one.
-                        const float rfl = floorf(rv), wfl = floorf(war);
-                        const float f = rv - rfl, gg = war - wfl;
-                        const float f2 = f * f, f3 = f2 * f, omf = 1.f - f;
-                        float br[4] = { omf * omf * omf * (1.f / 6.f), 0.66666669f - f2 + 0.5f * f3,
-                                        (-3.f * f3 + 3.f * f2 + 3.f * f + 1.f) * (1.f / 6.f), f3 * (1.f / 6.f) };
-                        const float omg = 1.f - gg;
-                        float dw[4] = { -0.5f * omg * omg, (1.5f * gg - 2.f) * gg, (0.5f + 1.5f * gg) * omg,
-                                        0.5f * gg * gg };
-                        const int rb0 = (int)rfl - 1, wb0 = (int)wfl - 1;
-                        int ri[4], wc[4];
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane < nX) {
+        const float4 b = sBx[lane];
+        const int px = sPx[lane], pxRel = px - px0;
+        const size_t N = (size_t)a.nx * a.ny * a.nz;
+        constexpr int kRowsPerWave = TY / (kThreads / 64);
+        for (int z = 0; z < nZ; ++z) {
+            // the rows of this wave in slice z: all taps loaded before any is used
+            VoxelTaps t[kRowsPerWave];
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const bool okr = (unsigned)(rb0 + k) < (unsigned)a.rb;
-                            const bool okw = (unsigned)(wb0 + k) < (unsigned)a.fb;
-                            ri[k] = okr ? rb0 + k : 0;
-                            br[k] = okr ? br[k] : 0.f;
-                            wc[k] = (okw ? wb0 + k : 0) * a.rb;
-                            dw[k] = okw ? dw[k] : 0.f;
-                        }
-                        float S = 0.f;
+            for (int k = 0; k < kRowsPerWave; ++k) {
+                const int y = min(wave + k * (kThreads / 64), nY - 1);
+                const size_t idx = ((size_t)(z0 + z) * a.ny + (y0 + y)) * a.nx + (x0 + lane);
+                t[k] = sweep_fetch<TImg, GRAD>(a, &sAyz[z][y][0][0], b, pxRel, px, sBy[y], sPy[y], sBz[z], sPz[z],
+                                               idx);
+            }
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const float T = br[0] * dyn[ri[0] + wc[j]] + br[1] * dyn[ri[1] + wc[j]] +
-                                            br[2] * dyn[ri[2] + wc[j]] + br[3] * dyn[ri[3] + wc[j]];
-                            S += dw[j] * T;
-                        }
-                        S *= coef;
-                        g0 = S * gx;
-                        g1 = S * gy;
-                        g2 = S * gz;
-                    }
-                }
-            }
-            if (GRAD) {
-                sG[y][x][0] = g0;
-                sG[y][x][1] = g1;
-                sG[y][x][2] = g2;
-            }
-        }
-        if (GRAD) {
-            __syncthreads();
-            // ---- stage C^T: x adjoint, per (y, window node)
-            for (int i = tid; i < nY * WX; i += kThreads) {
-                const int ix = i % WX, y = i / WX;
-                float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-                for (int x = sXFirst[ix]; x <= sXLast[ix]; ++x) {
-                    const int o = ix - (sPx[x] - px0);
-                    const float4 b = sBx[x];
-                    const float w = o == 0 ? b.x : o == 1 ? b.y : o == 2 ? b.z : b.w;
-                    s0 += w * sG[y][x][0];
-                    s1 += w * sG[y][x][1];
-                    s2 += w * sG[y][x][2];
-                }
-                sU1[y][ix][0] = s0;
-                sU1[y][ix][1] = s1;
-                sU1[y][ix][2] = s2;
-            }
-            __syncthreads();
-            // ---- stage B^T: y adjoint of the slice, then its z-adjoint share added to the
-            // tile's node window (slices in z order, one owner thread per (node y, node x))
-            const float4 bzz = sBz[z];
-            const int ozz = sPz[z] - pz0;
-            for (int i = tid; i < WY * WX; i += kThreads) {
-                const int ix = i % WX, iy = i / WX;
-                float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-                for (int y = 0; y < nY; ++y) {
-                    const int o = iy - (sPy[y] - py0);
-                    if (o < 0 || o > 3) continue;
-                    const float4 b = sBy[y];
-                    const float w = o == 0 ? b.x : o == 1 ? b.y : o == 2 ? b.z : b.w;
-                    s0 += w * sU1[y][ix][0];
-                    s1 += w * sU1[y][ix][1];
-                    s2 += w * sU1[y][ix][2];
-                }
-                const float bzk[4] = { bzz.x, bzz.y, bzz.z, bzz.w };
-                for (int k = 0; k < 4; ++k) {
-                    sOut[ozz + k][iy][ix][0] += bzk[k] * s0;
-                    sOut[ozz + k][iy][ix][1] += bzk[k] * s1;
-                    sOut[ozz + k][iy][ix][2] += bzk[k] * s2;
+            for (int k = 0; k < kRowsPerWave; ++k) {
+                const int y = wave + k * (kThreads / 64);
+                if (y >= nY) continue;  // uniform over the wave
+                float g0, g1, g2;
+                sweep_finish<GRAD>(a, t[k], dyn, coef, g0, g1, g2);
+                if (GRAD) {
+                    const size_t idx = ((size_t)(z0 + z) * a.ny + (y0 + y)) * a.nx + (x0 + lane);
+                    a.gvox[idx] = g0;
+                    a.gvox[idx + N] = g1;
+                    a.gvox[idx + 2 * N] = g2;
                 }
             }
         }
-        __syncthreads();
     }
-
     if (!GRAD) {
+        __syncthreads();
         const uint32_t *h = (const uint32_t *)dyn;
-        for (int i = tid; i < nb2; i += kThreads)
+        for (int i = tid; i < a.rb * a.fb; i += kThreads)
             if (h[i]) atomicAdd(&a.hist[i], h[i]);
-    } else {
-        // ---- the tile's partial node window
-        float *out = a.partials + (size_t)t * 3 * WIN;
-        for (int i = tid; i < WZ * WY * WX; i += kThreads) {
-            const int ix = i % WX, iy = (i / WX) % WY, iz = i / (WX * WY);
-            const int wi = (iz * WYM + iy) * WXM + ix;
-            out[wi] = sOut[iz][iy][ix][0];
-            out[WIN + wi] = sOut[iz][iy][ix][1];
-            out[2 * WIN + wi] = sOut[iz][iy][ix][2];
-        }
     }
 }
 
-// Sum of the tile windows covering each node (fixed tile order), per-axis normalisation,
-// reorientation by the floating image's mm->voxel 3x3 (transposed), weighting; nodes whose
-// voxel position lies outside the reference image receive 0 (reg_voxelCentricToNodeCentric
-// samples nothing there).
-struct ReduceArgs {
-    const float *partials;
-    int gnx, gny, gnz;
+// Projection of the voxel gradient onto the nodes, one axis at a time.  The reference convolves
+// the voxel gradient with the cubic B-spline kernel along x, y, z (each pass normalised by the
+// in-image kernel sum) and samples the result at the nodes (reg-lib/Compute.cpp:230-281); with
+// nodes on voxels and integer spacing that equals the transposed field evaluation
+// sum_v B_x B_y B_z g(v), divided per axis by the kernel sum at the node.  Three separable passes
+// (x: [3][nz][ny][gnx], y: [3][nz][gny][gnx], z: the nodes) sum in a fixed order.
+// x pass: X[c][row][ix] = sum_x B(x, ix) g[c][row][x] for rows = (z, y).  A workgroup stages R
+// whole rows in LDS with coalesced loads.  Every voxel of a cell (the voxels sharing pre(x))
+// feeds the same four nodes, so one thread per (row, cell) first forms the cell's four weighted
+// sums per component, then one thread per (row, node) adds the four cells that reach the node.
+struct ProjXArgs {
+    const float *g;       // [3][rows][nx]
+    float *out;           // [3][rows][gnx]
+    int nx, gnx;
+    size_t rows;
+    int rowsPerBlock;
+    int pre0, cells;      // pre(0) and the number of cells along x
+    const int2 *cellX;    // voxels [first, last] of each cell
+    const float4 *bX;
+};
+
+__global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
+    extern __shared__ float ldsX[];  // rows [3][R][nx], then cell sums [R][cells][4][3]
+    const int R = p.rowsPerBlock;
+    const size_t row0 = (size_t)blockIdx.x * R;
+    const int nr = (int)min((size_t)R, p.rows - row0);
+    const size_t planeIn = p.rows * (size_t)p.nx, planeOut = p.rows * (size_t)p.gnx;
+    float *rowsL = ldsX, *cellL = ldsX + (size_t)3 * R * p.nx;
+    const int per = nr * p.nx;
+    for (int c = 0; c < 3; ++c) {
+        const float *src = p.g + c * planeIn + row0 * p.nx;
+        float *dst = rowsL + (size_t)c * R * p.nx;
+        for (int base = 0; base < per; base += 8 * 256) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = base + u * 256 + (int)threadIdx.x;
+                if (i < per) v[u] = src[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = base + u * 256 + (int)threadIdx.x;
+                if (i < per) dst[i] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nr * p.cells; i += blockDim.x) {
+        const int cell = i % p.cells, r = i / p.cells;
+        const int2 xr = p.cellX[cell];
+        float acc[4][3] = {};
+        for (int x = xr.x; x <= xr.y; ++x) {
+            const float4 b = p.bX[x];
+            const float v0 = rowsL[r * p.nx + x], v1 = rowsL[(R + r) * p.nx + x], v2 = rowsL[(2 * R + r) * p.nx + x];
+            acc[0][0] += b.x * v0; acc[0][1] += b.x * v1; acc[0][2] += b.x * v2;
+            acc[1][0] += b.y * v0; acc[1][1] += b.y * v1; acc[1][2] += b.y * v2;
+            acc[2][0] += b.z * v0; acc[2][1] += b.z * v1; acc[2][2] += b.z * v2;
+            acc[3][0] += b.w * v0; acc[3][1] += b.w * v1; acc[3][2] += b.w * v2;
+        }
+        float *o = cellL + ((size_t)r * p.cells + cell) * 12;
+        for (int k = 0; k < 4; ++k)
+            for (int c = 0; c < 3; ++c) o[k * 3 + c] = acc[k][c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nr * p.gnx; i += blockDim.x) {
+        const int node = i % p.gnx, r = i / p.gnx;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+        for (int k = 3; k >= 0; --k) {  // cells in x order: node - 3 first
+            const int cell = node - k - p.pre0;
+            if (cell < 0 || cell >= p.cells) continue;
+            const float *o = cellL + ((size_t)r * p.cells + cell) * 12 + k * 3;
+            s0 += o[0];
+            s1 += o[1];
+            s2 += o[2];
+        }
+        const size_t q = (row0 + r) * p.gnx + node;
+        p.out[q] = s0;
+        p.out[q + planeOut] = s1;
+        p.out[q + 2 * planeOut] = s2;
+    }
+}
+
+// y pass: Y[c][z][iy][ix] = sum_y B(y, iy) X[c][z][y][ix] (consecutive threads read consecutive ix).
+struct ProjYArgs {
+    const float *in;   // [3][nz][ny][gnx]
+    float *out;        // [3][nz][gny][gnx]
+    int ny, nz, gnx, gny;
+    const int *preY;
+    const float4 *bY;
+    const int2 *rangeY;
+};
+
+__global__ void __launch_bounds__(256) k_project_y(ProjYArgs p) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)p.nz * p.gny * p.gnx) return;
+    const int ix = (int)(t % p.gnx), iy = (int)((t / p.gnx) % p.gny), z = (int)(t / ((size_t)p.gnx * p.gny));
+    const size_t planeIn = (size_t)p.nz * p.ny * p.gnx, planeOut = (size_t)p.nz * p.gny * p.gnx;
+    const int2 r = p.rangeY[iy];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll 4
+    for (int y = r.x; y <= r.y; ++y) {
+        const int k = iy - p.preY[y];
+        const float4 b = p.bY[y];
+        const float w = k == 0 ? b.x : k == 1 ? b.y : k == 2 ? b.z : b.w;
+        const size_t q = ((size_t)z * p.ny + y) * p.gnx + ix;
+        s0 += w * p.in[q];
+        s1 += w * p.in[q + planeIn];
+        s2 += w * p.in[q + 2 * planeIn];
+    }
+    p.out[t] = s0;
+    p.out[t + planeOut] = s1;
+    p.out[t + 2 * planeOut] = s2;
+}
+
+// z pass fused with the per-node normalisation, the reorientation by the floating image's
+// mm->voxel 3x3 (transposed, as reg_voxelCentricToNodeCentric) and the weight; nodes whose voxel
+// position lies outside the reference image receive 0.
+struct NodeArgs {
+    const float *in;  // [3][nz][gny][gnx]
+    int nz, gnx, gny, gnz;
     size_t G;
-    int tilesX, tilesY;
-    const int2 *tx, *ty, *tz;          // first / last tile covering each node index, per axis
-    const int *wx0, *wy0, *wz0;        // window origin (node index) of each tile, per axis
-    const float *invNx, *invNy, *invNz;  // 1 / kernel sum at the node, 0 outside the image
+    const int *preZ;
+    const float4 *bZ;
+    const int2 *rangeZ;
+    const float *invNx, *invNy, *invNz;
     Affine reorient;
     float weight;
     float *nodeGrad;
 };
 
-__global__ void __launch_bounds__(256) k_node_reduce(ReduceArgs r) {
+__global__ void __launch_bounds__(256) k_project_nodes(NodeArgs r) {
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= r.G) return;
     const int i = (int)(n % r.gnx), j = (int)((n / r.gnx) % r.gny), k = (int)(n / ((size_t)r.gnx * r.gny));
     const float inv = r.invNx[i] * r.invNy[j] * r.invNz[k];
     float u0 = 0.f, u1 = 0.f, u2 = 0.f;
     if (inv != 0.f) {
-        const int2 ax = r.tx[i], ay = r.ty[j], az = r.tz[k];
-        const int na = ax.y - ax.x + 1, nab = na * (ay.y - ay.x + 1);
-        const int cnt = nab * (az.y - az.x + 1);
-        // windows in (z, y, x) tile order, loaded four at a time so the loads overlap; the sums
-        // keep that order
-        for (int q0 = 0; q0 < cnt; q0 += 4) {
-            float v[4][3];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int q = q0 + u;
-                if (q < cnt) {
-                    const int a = ax.x + q % na, b = ay.x + (q % nab) / na, c = az.x + q / nab;
-                    const size_t tile = ((size_t)c * r.tilesY + b) * r.tilesX + a;
-                    const int wi = ((k - r.wz0[c]) * WYM + (j - r.wy0[b])) * WXM + (i - r.wx0[a]);
-                    const float *p = r.partials + tile * 3 * WIN + wi;
-                    v[u][0] = p[0];
-                    v[u][1] = p[WIN];
-                    v[u][2] = p[2 * WIN];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (q0 + u < cnt) {
-                    u0 += v[u][0];
-                    u1 += v[u][1];
-                    u2 += v[u][2];
-                }
-            }
+        const size_t plane = (size_t)r.nz * r.gny * r.gnx, col = (size_t)j * r.gnx + i;
+        const int2 rz = r.rangeZ[k];
+#pragma unroll 4
+        for (int z = rz.x; z <= rz.y; ++z) {
+            const int q = k - r.preZ[z];
+            const float4 b = r.bZ[z];
+            const float w = q == 0 ? b.x : q == 1 ? b.y : q == 2 ? b.z : b.w;
+            const size_t a = (size_t)z * r.gny * r.gnx + col;
+            u0 += w * r.in[a];
+            u1 += w * r.in[a + plane];
+            u2 += w * r.in[a + 2 * plane];
         }
         u0 *= inv;
         u1 *= inv;
@@ -418,6 +506,16 @@ __global__ void __launch_bounds__(256) k_node_reduce(ReduceArgs r) {
     r.nodeGrad[n] = (R[0] * u0 + R[4] * u1 + R[8] * u2) * r.weight;
     r.nodeGrad[n + r.G] = (R[1] * u0 + R[5] * u1 + R[9] * u2) * r.weight;
     r.nodeGrad[n + 2 * r.G] = (R[2] * u0 + R[6] * u1 + R[10] * u2) * r.weight;
+}
+
+// K[r + w rb] = log p_r + log p_w - NMI log p_rw in float, the table the gradient sweep contracts.
+__global__ void __launch_bounds__(256) k_nmi_table(const float *__restrict__ logTab, const double *__restrict__ ent,
+                                                   int rb, int fb, float *__restrict__ K) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rb * fb) return;
+    const float nmiF = (float)ent[4];
+    const float *lj = logTab, *lr = logTab + rb * fb, *lw = lr + rb;
+    K[i] = (lr[i % rb] + lw[i / rb]) - nmiF * lj[i];
 }
 
 __global__ void k_axis_table(int n, float gvs, int lut, int *pre, float4 *b) {
@@ -444,9 +542,12 @@ struct FusedPlan {
     int tilesX = 0, tilesY = 0, tilesZ = 0, numTiles = 0;
     DevBuf<int> preX, preY, preZ;
     DevBuf<float4> bX, bY, bZ;
-    DevBuf<int2> tX, tY, tZ;
-    DevBuf<int> wx0, wy0, wz0;
+    DevBuf<int2> rX, rY, rZ;     // voxels [first, last] in the support of each node, per axis
+    DevBuf<int2> cellX;          // voxels [first, last] of each x cell (voxels sharing pre(x))
+    int pre0X = 0, cellsX = 0;
     DevBuf<float> invNx, invNy, invNz;
+    DevBuf<float> gvox, projX, projY;  // voxel gradient [3][N]; after the x pass; after the y pass
+    DevBuf<float> kTab;                // NMI derivative table, rb * fb rounded up to 4 floats
 };
 
 // The plan is kept beside the handle in a side table keyed by handle address.
@@ -498,8 +599,7 @@ static FusedPlan *plan_of(nrh_f3d *h) {
         const int wmax[3] = { WXM, WYM, WZM };
         DevBuf<int> *pre[3] = { &p->preX, &p->preY, &p->preZ };
         DevBuf<float4> *bb[3] = { &p->bX, &p->bY, &p->bZ };
-        DevBuf<int2> *tt[3] = { &p->tX, &p->tY, &p->tZ };
-        DevBuf<int> *w0[3] = { &p->wx0, &p->wy0, &p->wz0 };
+        DevBuf<int2> *rr[3] = { &p->rX, &p->rY, &p->rZ };
         DevBuf<float> *inv[3] = { &p->invNx, &p->invNy, &p->invNz };
         int tiles[3];
         const float pd[3] = { h->ref.dx, h->ref.dy, h->ref.dz };
@@ -514,22 +614,35 @@ static FusedPlan *plan_of(nrh_f3d *h) {
             NRH_HIP(hipMemcpyAsync(hpre.data(), pre[ax]->p, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
             NRH_HIP(hipStreamSynchronize(h->stream));
             tiles[ax] = (n + tsz[ax] - 1) / tsz[ax];
-            std::vector<int> org(tiles[ax]);
-            std::vector<int2> cover(gdims[ax], make_int2(1, 0));
             for (int t = 0; t < tiles[ax]; ++t) {
                 const int a0 = t * tsz[ax], a1 = std::min(n, a0 + tsz[ax]) - 1;
-                org[t] = hpre[a0];
-                const int wn = hpre[a1] - hpre[a0] + 4;
-                if (wn > wmax[ax]) ok = false;
-                for (int q = hpre[a0]; q < hpre[a0] + wn && q < gdims[ax]; ++q) {
-                    if (cover[q].x > cover[q].y) cover[q] = make_int2(t, t);
-                    else cover[q].y = t;
-                }
+                if (hpre[a1] - hpre[a0] + 4 > wmax[ax]) ok = false;
             }
-            w0[ax]->alloc(tiles[ax]);
-            tt[ax]->alloc(gdims[ax]);
-            NRH_HIP(hipMemcpyAsync(w0[ax]->p, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
-            NRH_HIP(hipMemcpyAsync(tt[ax]->p, cover.data(), cover.size() * sizeof(int2), hipMemcpyHostToDevice, h->stream));
+            // voxels whose 4-node support contains each node (pre is non-decreasing)
+            std::vector<int2> range(gdims[ax], make_int2(1, 0));
+            for (int v = 0; v < n; ++v)
+                for (int k = 0; k < 4; ++k) {
+                    const int q = hpre[v] + k;
+                    if (q < 0 || q >= gdims[ax]) continue;
+                    if (range[q].x > range[q].y) range[q] = make_int2(v, v);
+                    else range[q].y = v;
+                }
+            if (ax == 0) {
+                p->pre0X = hpre[0];
+                p->cellsX = hpre[n - 1] - hpre[0] + 1;
+                std::vector<int2> cells(p->cellsX, make_int2(1, 0));
+                for (int v = 0; v < n; ++v) {
+                    int2& c = cells[hpre[v] - hpre[0]];
+                    if (c.x > c.y) c = make_int2(v, v);
+                    else c.y = v;
+                }
+                p->cellX.alloc(cells.size());
+                NRH_HIP(hipMemcpyAsync(p->cellX.p, cells.data(), cells.size() * sizeof(int2), hipMemcpyHostToDevice,
+                                       h->stream));
+            }
+            rr[ax]->alloc(gdims[ax]);
+            NRH_HIP(hipMemcpyAsync(rr[ax]->p, range.data(), range.size() * sizeof(int2), hipMemcpyHostToDevice,
+                                   h->stream));
             // per-node inverse normalisation: kernel of reg_tools_kernelConvolution (reg-lib/cpu/_reg_tools.cpp:868-904)
             // summed over the taps inside the image, as its density sum (float products, double sum)
             const double temp = (double)(sp[ax] / pd[ax]);
@@ -566,7 +679,10 @@ static FusedPlan *plan_of(nrh_f3d *h) {
     return p;
 }
 
-bool fused_supported(nrh_f3d *h) { return plan_of(h)->ok; }
+// The sweeps keep the joint histogram / NMI table in LDS: at most 12288 entries (bins <= 106).
+bool fused_supported(nrh_f3d *h) {
+    return plan_of(h)->ok && (size_t)h->params.ref_bins * h->params.flo_bins <= 12288;
+}
 
 void collect_sweep_times(nrh_f3d *h) {
     for (int k = 0; k < 2; ++k) {
@@ -616,19 +732,24 @@ static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
 
 static unsigned grid_for(int numTiles) { return (unsigned)((numTiles + 7) / 8 * 8); }
 
+// Dynamic LDS of a sweep: the control-point window, then the histogram / NMI table in its place.
+static size_t sweep_lds(const FusedArgs& a) {
+    return std::max<size_t>(((size_t)a.rb * a.fb + 3) / 4 * 4, (size_t)3 * WZM * WYM * WXM) * sizeof(float);
+}
+
 void fused_histogram(nrh_f3d *h, float *fieldOut) {
     FusedPlan *p = plan_of(h);
-    NRH_REQUIRE(p->ok, NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry");
+    NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
     FusedArgs a = make_args(h, p);
     a.hist = h->histCount.p;
     a.fieldOut = fieldOut;
     op_fill_u32(h->histCount.p, (size_t)a.rb * a.fb, 0u, h->stream);
-    const size_t lds = (size_t)a.rb * a.fb * sizeof(uint32_t);
+    const size_t lds = sweep_lds(a);
     if (h->profile) NRH_HIP(hipEventRecord(h->ev[0], h->stream));
     if (h->params.image_storage == 1)
-        hipLaunchKernelGGL((k_fused_pass<__half, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
+        hipLaunchKernelGGL((k_sweep<__half, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     else
-        hipLaunchKernelGGL((k_fused_pass<float, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
+        hipLaunchKernelGGL((k_sweep<float, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     NRH_HIP(hipGetLastError());
     if (h->profile) {
         NRH_HIP(hipEventRecord(h->ev[1], h->stream));
@@ -640,35 +761,73 @@ void fused_histogram(nrh_f3d *h, float *fieldOut) {
 
 void fused_gradient(nrh_f3d *h, float weight) {
     FusedPlan *p = plan_of(h);
-    NRH_REQUIRE(p->ok, NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry");
+    NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
     FusedArgs a = make_args(h, p);
     a.logTab = h->histLogF.p;
     a.ent = h->ent.p;
-    h->nodePartials.alloc((size_t)p->numTiles * 3 * WIN);
-    a.partials = h->nodePartials.p;
-    const size_t lds = (size_t)a.rb * a.fb * sizeof(float);
+    const size_t N = h->N;
+    const int nx = h->ref.nx, ny = h->ref.ny, nz = h->ref.nz;
+    const int gnx = h->cpp.nx, gny = h->cpp.ny, gnz = h->cpp.nz;
+    p->gvox.alloc(3 * N);
+    p->projX.alloc((size_t)3 * nz * ny * gnx);
+    p->projY.alloc((size_t)3 * nz * gny * gnx);
+    a.gvox = p->gvox.p;
+    const int nbins = a.rb * a.fb;
+    p->kTab.alloc((size_t)(nbins + 3) / 4 * 4);
+    hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, a.logTab, a.ent, a.rb, a.fb,
+                       p->kTab.p);
+    a.kTab = p->kTab.p;
+    const size_t lds = sweep_lds(a);
     if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
     if (h->params.image_storage == 1)
-        hipLaunchKernelGGL((k_fused_pass<__half, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
+        hipLaunchKernelGGL((k_sweep<__half, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     else
-        hipLaunchKernelGGL((k_fused_pass<float, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
+        hipLaunchKernelGGL((k_sweep<float, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     NRH_HIP(hipGetLastError());
     if (h->profile) {
         NRH_HIP(hipEventRecord(h->ev[3], h->stream));
         h->evPending[1] = true;
     }
-    ReduceArgs r{};
-    r.partials = h->nodePartials.p;
-    r.gnx = h->cpp.nx; r.gny = h->cpp.ny; r.gnz = h->cpp.nz;
+    auto blocks = [](size_t n) { return dim3((unsigned)((n + 255) / 256)); };
+    ProjXArgs px{};
+    px.g = p->gvox.p;
+    px.out = p->projX.p;
+    px.nx = nx;
+    px.gnx = gnx;
+    px.rows = (size_t)ny * nz;
+    px.rowsPerBlock = std::max(1, 2048 / nx);  // rows: <= 24 KB of LDS for nx <= 2048
+    px.pre0 = p->pre0X;
+    px.cells = p->cellsX;
+    px.cellX = p->cellX.p;
+    px.bX = p->bX.p;
+    const size_t ldsX = ((size_t)3 * px.rowsPerBlock * nx + (size_t)px.rowsPerBlock * px.cells * 12) * sizeof(float);
+    NRH_REQUIRE(ldsX <= 64 * 1024, NRH_ERR_UNSUPPORTED, "image rows too long for the fused projection");
+    hipLaunchKernelGGL(k_project_x, dim3((unsigned)((px.rows + px.rowsPerBlock - 1) / px.rowsPerBlock)), dim3(256),
+                       ldsX, h->stream, px);
+    ProjYArgs py{};
+    py.in = p->projX.p;
+    py.out = p->projY.p;
+    py.ny = ny;
+    py.nz = nz;
+    py.gnx = gnx;
+    py.gny = gny;
+    py.preY = p->preY.p;
+    py.bY = p->bY.p;
+    py.rangeY = p->rY.p;
+    hipLaunchKernelGGL(k_project_y, blocks((size_t)nz * gny * gnx), dim3(256), 0, h->stream, py);
+    NodeArgs r{};
+    r.in = p->projY.p;
+    r.nz = nz;
+    r.gnx = gnx; r.gny = gny; r.gnz = gnz;
     r.G = h->G;
-    r.tilesX = p->tilesX; r.tilesY = p->tilesY;
-    r.tx = p->tX.p; r.ty = p->tY.p; r.tz = p->tZ.p;
-    r.wx0 = p->wx0.p; r.wy0 = p->wy0.p; r.wz0 = p->wz0.p;
+    r.preZ = p->preZ.p;
+    r.bZ = p->bZ.p;
+    r.rangeZ = p->rZ.p;
     r.invNx = p->invNx.p; r.invNy = p->invNy.p; r.invNz = p->invNz.p;
     r.reorient = to_affine(h->flo.ijk);
     r.weight = node_weight_ratio(h, weight);
     r.nodeGrad = h->nodeGrad.p;
-    hipLaunchKernelGGL(k_node_reduce, dim3((unsigned)((h->G + 255) / 256)), dim3(256), 0, h->stream, r);
+    hipLaunchKernelGGL(k_project_nodes, blocks(h->G), dim3(256), 0, h->stream, r);
     NRH_HIP(hipGetLastError());
 }
 

@@ -39,7 +39,6 @@ struct nrh_f3d {
     nrh::DevBuf<uint32_t> maxBits;
 
     // fused node-gradient partial sums (one window per tile)
-    nrh::DevBuf<float> nodePartials;
 
     // LNCC: two ping-pong sets of planar numerators + density, combined mask, reductions
     nrh::DevBuf<float> lnccA, lnccB, lnccKern;
