@@ -177,6 +177,8 @@ int nrh_f3d_create(const nrh_geom *ref, const float *ref_data, const nrh_geom *f
         NRH_HIP(hipGetDevice(&h->device));
         NRH_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         NRH_HIP(hipHostMalloc((void **)&h->pinned, 16 * sizeof(double), hipHostMallocDefault));
+        NRH_HIP(hipHostMalloc((void **)&h->published, 16 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped));
+        NRH_HIP(hipHostGetDevicePointer((void **)&h->publishedDev, h->published, 0));
         // Images: float32 always kept (per-operation path); float16 copies added for the fused
         // path when requested.
         h->refImg.alloc(h->N);
@@ -214,6 +216,7 @@ int nrh_f3d_destroy(nrh_f3d *h) {
         if (!h) return;
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         if (h->pinned) (void)hipHostFree(h->pinned);
+        if (h->published) (void)hipHostFree(h->published);
         fused_release(h);
         for (int i = 0; i < 4; ++i)
             if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);

@@ -41,15 +41,13 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
         }
         if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
         if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
-        NRH_HIP(hipMemcpyAsync(h->pinned, lncc ? h->scalars.p : h->ent.p, 5 * sizeof(double), hipMemcpyDeviceToHost,
-                               h->stream));
-        NRH_HIP(hipMemcpyAsync(h->pinned + 5, h->scalars.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        NRH_HIP(hipMemcpyAsync(h->pinned + 6, h->scalars.p + 9, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
         NRH_HIP(hipStreamSynchronize(h->stream));
         collect_sweep_times(h);
-        const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * h->pinned[4] : 0.0;
-        const double wBE = p.be_weight > 0 ? (double)p.be_weight * h->pinned[5] : 0.0;
-        const double wLE = p.le_weight > 0 ? (double)p.le_weight * h->pinned[6] : 0.0;
+        const double *r = h->published;
+        const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * r[4] : 0.0;
+        const double wBE = p.be_weight > 0 ? (double)p.be_weight * r[5] : 0.0;
+        const double wLE = p.le_weight > 0 ? (double)p.le_weight * r[6] : 0.0;
         *value = wMeasure - wBE - wLE;
         if (terms) {
             terms[0] = wMeasure;

@@ -64,7 +64,8 @@ template<> __device__ __forceinline__ float ld<__half>(const __half *p, size_t i
 
 // Tile index with the tiles of one XCD contiguous: consecutive workgroups are dealt round-robin
 // to the 8 XCDs, so workgroup b runs on XCD b % 8; giving XCD k a contiguous block of tiles keeps
-// neighbouring tiles (which read overlapping floating-image lines) in the same L2.
+// neighbouring tiles (which read overlapping floating-image lines) in the same L2.  A bijection
+// of [0, numTiles) only when numTiles is a multiple of 8: launches round their grids up.
 __device__ __forceinline__ int xcd_tile(int b, int numTiles) {
     const int per = (numTiles + 7) / 8;
     const int t = (b % 8) * per + b / 8;
@@ -84,61 +85,75 @@ struct VoxelTaps {
     bool active;
 };
 
-template<class TImg, bool GRAD>
-__device__ __forceinline__ VoxelTaps sweep_fetch(const FusedArgs a, const float *ayz, const float4 b, const int pxRel,
-                                                 const int px, const float4 by, const int py, const float4 bz,
-                                                 const int pz, const size_t idx) {
-    const TImg *ref = (const TImg *)a.ref;
-    const TImg *flo = (const TImg *)a.flo;
-    VoxelTaps t;
-    t.active = !a.mask || a.mask[idx] > -1;
-    // field = identity + displacement: the B-spline sum of the node positions is the affine map of
-    // (pre + t + 1) on each axis (cubic B-splines reproduce linear functions), evaluated in double;
-    // the window holds displacements.
-    const double ux = (double)px + (double)(b.y + 2.f * b.z + 3.f * b.w);
-    const double uy = (double)py + (double)(by.y + 2.f * by.z + 3.f * by.w);
-    const double uz = (double)pz + (double)(bz.y + 2.f * bz.z + 3.f * bz.w);
+// Position of a voxel in the floating image.  The identity part of the field is the affine map
+// of (pre + t + 1) per axis (cubic B-splines reproduce linear functions), split as idX[c] (the
+// lane's x term, double) + idYZ[c] (the row's y, z and offset terms, double); the window holds
+// displacements, contracted here along x.
+struct VoxelPos {
+    int fx, fy, fz;
+    float rx, ry, rz;
+};
+
+__device__ __forceinline__ VoxelPos sweep_position(const FusedArgs& a, const float *ayz, const float4 b,
+                                                   const int pxRel, const double idX[3], const double *idYZ,
+                                                   float *fieldOut, const uint32_t idx, const bool active) {
     float w[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float *q = ayz + pxRel * 3 + c;
         const float d = b.x * q[0] + b.y * q[3] + b.z * q[6] + b.w * q[9];
-        const double id = (double)a.cppXyz.m[c * 4] * ux + (double)a.cppXyz.m[c * 4 + 1] * uy +
-                          (double)a.cppXyz.m[c * 4 + 2] * uz + (double)a.cppXyz.m[c * 4 + 3];
-        w[c] = (float)(id + (double)d);
+        w[c] = (float)((idX[c] + idYZ[c]) + (double)d);
     }
-    if (!GRAD && a.fieldOut && t.active) {
-        const size_t N = (size_t)a.nx * a.ny * a.nz;
-        a.fieldOut[idx] = w[0];
-        a.fieldOut[idx + N] = w[1];
-        a.fieldOut[idx + 2 * N] = w[2];
+    if (fieldOut && active) {
+        const uint32_t N = (uint32_t)a.nx * a.ny * a.nz;
+        fieldOut[idx] = w[0];
+        fieldOut[idx + N] = w[1];
+        fieldOut[idx + 2 * N] = w[2];
     }
     const float3 p = affine_apply_f(a.floIjk, w[0], w[1], w[2]);
-    const int fx = floor_int(p.x), fy = floor_int(p.y), fz = floor_int(p.z);
-    t.rx = p.x - (float)fx;
-    t.ry = p.y - (float)fy;
-    t.rz = p.z - (float)fz;
-    const int X[2] = { min(max(fx, 0), a.fnx - 1), min(max(fx + 1, 0), a.fnx - 1) };
-    const int Y[2] = { min(max(fy, 0), a.fny - 1), min(max(fy + 1, 0), a.fny - 1) };
-    const int Z[2] = { min(max(fz, 0), a.fnz - 1), min(max(fz + 1, 0), a.fnz - 1) };
+    VoxelPos r;
+    r.fx = floor_int(p.x);
+    r.fy = floor_int(p.y);
+    r.fz = floor_int(p.z);
+    r.rx = p.x - (float)r.fx;
+    r.ry = p.y - (float)r.fy;
+    r.rz = p.z - (float)r.fz;
+    return r;
+}
+
+__device__ __forceinline__ bool taps_inside(const FusedArgs& a, const VoxelPos& p) {
+    return p.fx >= 0 && p.fx + 1 < a.fnx && p.fy >= 0 && p.fy + 1 < a.fny && p.fz >= 0 && p.fz + 1 < a.fnz;
+}
+
+// The 8 trilinear taps; `inside`: every tap lies in the image (one base address, fixed strides),
+// otherwise clamped addresses with out-of-image taps replaced by the padding value.  Offsets are
+// 32-bit: the fused path requires images below 2^31 voxels.
+template<class TImg>
+__device__ __forceinline__ void load_taps(const FusedArgs& a, const TImg *__restrict__ flo, const VoxelPos& p,
+                                          bool inside, float v[2][2][2]) {
+    const uint32_t sy = (uint32_t)a.fnx, sz = (uint32_t)a.fnx * a.fny;
+    if (inside) {
+        const TImg *q = flo + ((uint32_t)p.fz * sz + (uint32_t)p.fy * sy + (uint32_t)p.fx);
+        v[0][0][0] = ld(q, 0);       v[0][0][1] = ld(q, 1);
+        v[0][1][0] = ld(q, sy);      v[0][1][1] = ld(q, sy + 1);
+        v[1][0][0] = ld(q, sz);      v[1][0][1] = ld(q, sz + 1);
+        v[1][1][0] = ld(q, sz + sy); v[1][1][1] = ld(q, sz + sy + 1);
+    } else {
+        const uint32_t X[2] = { (uint32_t)min(max(p.fx, 0), a.fnx - 1), (uint32_t)min(max(p.fx + 1, 0), a.fnx - 1) };
+        const uint32_t Y[2] = { (uint32_t)min(max(p.fy, 0), a.fny - 1), (uint32_t)min(max(p.fy + 1, 0), a.fny - 1) };
+        const uint32_t Z[2] = { (uint32_t)min(max(p.fz, 0), a.fnz - 1), (uint32_t)min(max(p.fz + 1, 0), a.fnz - 1) };
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int bb = 0; bb < 2; ++bb)
+            for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-            for (int aa = 0; aa < 2; ++aa) t.v[c][bb][aa] = ld(flo, ((size_t)Z[c] * a.fny + Y[bb]) * a.fnx + X[aa]);
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-            for (int aa = 0; aa < 2; ++aa) {
-                const bool in = (unsigned)(fx + aa) < (unsigned)a.fnx && (unsigned)(fy + bb) < (unsigned)a.fny &&
-                                (unsigned)(fz + c) < (unsigned)a.fnz;
-                t.v[c][bb][aa] = in ? t.v[c][bb][aa] : a.pad;
-            }
-    t.rv = ld(ref, idx);
-    return t;
+                for (int aa = 0; aa < 2; ++aa) {
+                    const bool in = (unsigned)(p.fx + aa) < (unsigned)a.fnx &&
+                                    (unsigned)(p.fy + bb) < (unsigned)a.fny && (unsigned)(p.fz + c) < (unsigned)a.fnz;
+                    const float t = ld(flo, Z[c] * sz + Y[bb] * sy + X[aa]);
+                    v[c][bb][aa] = in ? t : a.pad;
+                }
+    }
 }
 
 template<bool GRAD>
@@ -213,6 +228,7 @@ __device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, fl
 template<class TImg, bool GRAD>
 __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
     __shared__ float sAyz[TZ][TY][WXM][3];
+    __shared__ double sIdYZ[TZ][TY][3];  // identity terms of each row: m[c][1] uy + m[c][2] uz + m[c][3]
     __shared__ float4 sBx[TX], sBy[TY], sBz[TZ];
     __shared__ int sPx[TX], sPy[TY], sPz[TZ];
     extern __shared__ __align__(16) float dyn[];  // window [3][WZM][WYM][WXM], then histogram or table K
@@ -266,21 +282,39 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
         }
     }
     __syncthreads();
-    // z then y contraction for every (z, y) row of the tile
-    for (int i = tid; i < nZ * nY * WX; i += kThreads) {
-        const int ix = i % WX, y = (i / WX) % nY, z = i / (WX * nY);
-        const float4 bz = sBz[z], b = sBy[y];
-        const int oz = sPz[z] - pz0, o = sPy[y] - py0;
+    // z contraction of the window for every slice of the tile (into the LDS after the window),
+    // then y contraction for every (z, y) row
+    float *az = dyn + 3 * WZM * WYM * WXM;  // [c][z][iy][ix] with strides TZ*WYM*WXM, WYM*WXM, WXM
+    for (int i = tid; i < nZ * WY * WX; i += kThreads) {
+        const int ix = i % WX, iy = (i / WX) % WY, z = i / (WX * WY);
+        const float4 bz = sBz[z];
+        const int oz = sPz[z] - pz0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float *w0 = win + ((c * WZM + oz) * WYM + o) * WXM + ix;
-            float az[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                az[j] = bz.x * w0[j * WXM] + bz.y * w0[WYM * WXM + j * WXM] + bz.z * w0[2 * WYM * WXM + j * WXM] +
-                        bz.w * w0[3 * WYM * WXM + j * WXM];
-            sAyz[z][y][ix][c] = b.x * az[0] + b.y * az[1] + b.z * az[2] + b.w * az[3];
+            const float *w0 = win + ((c * WZM + oz) * WYM + iy) * WXM + ix;
+            az[((c * TZ + z) * WYM + iy) * WXM + ix] = bz.x * w0[0] + bz.y * w0[WYM * WXM] +
+                                                      bz.z * w0[2 * WYM * WXM] + bz.w * w0[3 * WYM * WXM];
         }
+    }
+    __syncthreads();
+    for (int i = tid; i < nZ * nY * WX; i += kThreads) {
+        const int ix = i % WX, y = (i / WX) % nY, z = i / (WX * nY);
+        const float4 b = sBy[y];
+        const int o = sPy[y] - py0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float *a0 = az + ((c * TZ + z) * WYM + o) * WXM + ix;
+            sAyz[z][y][ix][c] = b.x * a0[0] + b.y * a0[WXM] + b.z * a0[2 * WXM] + b.w * a0[3 * WXM];
+        }
+    }
+    if (tid < nZ * TY * 3) {
+        const int c = tid % 3, y = (tid / 3) % TY, z = tid / (3 * TY);
+        const int yy = min(y, nY - 1);
+        const float4 by = sBy[yy], bz = sBz[z];
+        const double uy = (double)sPy[yy] + (double)(by.y + 2.f * by.z + 3.f * by.w);
+        const double uz = (double)sPz[z] + (double)(bz.y + 2.f * bz.z + 3.f * bz.w);
+        sIdYZ[z][y][c] = (double)a.cppXyz.m[c * 4 + 1] * uy + (double)a.cppXyz.m[c * 4 + 2] * uz +
+                         (double)a.cppXyz.m[c * 4 + 3];
     }
     __syncthreads();
     float coef = 0.f;
@@ -291,12 +325,10 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
         const float4 *src = (const float4 *)a.kTab;
         float4 *dst4 = (float4 *)dyn;
         for (int base = 0; base < n4; base += 4 * kThreads) {
+            // unconditional loads (clamped index) keep v in registers with all four in flight
             float4 v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int i = base + u * kThreads + tid;
-                if (i < n4) v[u] = src[i];
-            }
+            for (int u = 0; u < 4; ++u) v[u] = src[min(base + u * kThreads + tid, n4 - 1)];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int i = base + u * kThreads + tid;
@@ -311,20 +343,41 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
     __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
+    const TImg *__restrict__ ref = (const TImg *)a.ref;
+    const TImg *__restrict__ flo = (const TImg *)a.flo;
     if (lane < nX) {
         const float4 b = sBx[lane];
         const int px = sPx[lane], pxRel = px - px0;
-        const size_t N = (size_t)a.nx * a.ny * a.nz;
+        const double ux = (double)px + (double)(b.y + 2.f * b.z + 3.f * b.w);
+        const double idX[3] = { (double)a.cppXyz.m[0] * ux, (double)a.cppXyz.m[4] * ux, (double)a.cppXyz.m[8] * ux };
+        const uint32_t N = (uint32_t)a.nx * a.ny * a.nz;
+        float *fieldOut = GRAD ? nullptr : a.fieldOut;
         constexpr int kRowsPerWave = TY / (kThreads / 64);
         for (int z = 0; z < nZ; ++z) {
-            // the rows of this wave in slice z: all taps loaded before any is used
+            // the rows of this wave in slice z: every tap is loaded before any is used
+            VoxelPos pos[kRowsPerWave];
             VoxelTaps t[kRowsPerWave];
+            uint32_t idx[kRowsPerWave];
+            bool inside = true;
 #pragma unroll
             for (int k = 0; k < kRowsPerWave; ++k) {
                 const int y = min(wave + k * (kThreads / 64), nY - 1);
-                const size_t idx = ((size_t)(z0 + z) * a.ny + (y0 + y)) * a.nx + (x0 + lane);
-                t[k] = sweep_fetch<TImg, GRAD>(a, &sAyz[z][y][0][0], b, pxRel, px, sBy[y], sPy[y], sBz[z], sPz[z],
-                                               idx);
+                idx[k] = ((uint32_t)(z0 + z) * a.ny + (uint32_t)(y0 + y)) * a.nx + (uint32_t)(x0 + lane);
+                t[k].active = !a.mask || a.mask[idx[k]] > -1;
+                pos[k] = sweep_position(a, &sAyz[z][y][0][0], b, pxRel, idX, &sIdYZ[z][y][0], fieldOut, idx[k],
+                                        t[k].active);
+                inside = inside && taps_inside(a, pos[k]);
+            }
+            // wave-uniform choice between the fixed-stride and the clamped gathers (lanes past
+            // the tile's last column sit out this vote, so they never force the clamped path)
+            const bool allInside = __all(inside);
+#pragma unroll
+            for (int k = 0; k < kRowsPerWave; ++k) {
+                load_taps<TImg>(a, flo, pos[k], allInside, t[k].v);
+                t[k].rx = pos[k].rx;
+                t[k].ry = pos[k].ry;
+                t[k].rz = pos[k].rz;
+                t[k].rv = ld(ref, idx[k]);
             }
 #pragma unroll
             for (int k = 0; k < kRowsPerWave; ++k) {
@@ -333,10 +386,9 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
                 float g0, g1, g2;
                 sweep_finish<GRAD>(a, t[k], dyn, coef, g0, g1, g2);
                 if (GRAD) {
-                    const size_t idx = ((size_t)(z0 + z) * a.ny + (y0 + y)) * a.nx + (x0 + lane);
-                    a.gvox[idx] = g0;
-                    a.gvox[idx + N] = g1;
-                    a.gvox[idx + 2 * N] = g2;
+                    a.gvox[idx[k]] = g0;
+                    a.gvox[idx[k] + N] = g1;
+                    a.gvox[idx[k] + 2 * N] = g2;
                 }
             }
         }
@@ -370,6 +422,8 @@ struct ProjXArgs {
     const float4 *bX;
 };
 
+constexpr int kCellStride = 13;  // 12 sums per cell, padded to an odd stride (no LDS bank conflicts)
+
 __global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
     extern __shared__ float ldsX[];  // rows [3][R][nx], then cell sums [R][cells][4][3]
     const int R = p.rowsPerBlock;
@@ -378,22 +432,21 @@ __global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
     const size_t planeIn = p.rows * (size_t)p.nx, planeOut = p.rows * (size_t)p.gnx;
     float *rowsL = ldsX, *cellL = ldsX + (size_t)3 * R * p.nx;
     const int per = nr * p.nx;
-    for (int c = 0; c < 3; ++c) {
-        const float *src = p.g + c * planeIn + row0 * p.nx;
-        float *dst = rowsL + (size_t)c * R * p.nx;
-        for (int base = 0; base < per; base += 8 * 256) {
-            float v[8];
+    for (int base = 0; base < per; base += 8 * 256) {
+        // the three components' loads in flight together
+        float v[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                v[c][u] = p.g[c * planeIn + row0 * p.nx + min(base + u * 256 + (int)threadIdx.x, per - 1)];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int i = base + u * 256 + (int)threadIdx.x;
-                if (i < per) v[u] = src[i];
+                if (i < per) rowsL[(size_t)c * R * p.nx + i] = v[c][u];
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int i = base + u * 256 + (int)threadIdx.x;
-                if (i < per) dst[i] = v[u];
-            }
-        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nr * p.cells; i += blockDim.x) {
@@ -408,7 +461,7 @@ __global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
             acc[2][0] += b.z * v0; acc[2][1] += b.z * v1; acc[2][2] += b.z * v2;
             acc[3][0] += b.w * v0; acc[3][1] += b.w * v1; acc[3][2] += b.w * v2;
         }
-        float *o = cellL + ((size_t)r * p.cells + cell) * 12;
+        float *o = cellL + ((size_t)r * p.cells + cell) * kCellStride;
         for (int k = 0; k < 4; ++k)
             for (int c = 0; c < 3; ++c) o[k * 3 + c] = acc[k][c];
     }
@@ -419,7 +472,7 @@ __global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
         for (int k = 3; k >= 0; --k) {  // cells in x order: node - 3 first
             const int cell = node - k - p.pre0;
             if (cell < 0 || cell >= p.cells) continue;
-            const float *o = cellL + ((size_t)r * p.cells + cell) * 12 + k * 3;
+            const float *o = cellL + ((size_t)r * p.cells + cell) * kCellStride + k * 3;
             s0 += o[0];
             s1 += o[1];
             s2 += o[2];
@@ -442,7 +495,8 @@ struct ProjYArgs {
 };
 
 __global__ void __launch_bounds__(256) k_project_y(ProjYArgs p) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // blocks of one XCD take a contiguous range of z slabs, so each slab's rows go through one L2
+    const size_t t = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= (size_t)p.nz * p.gny * p.gnx) return;
     const int ix = (int)(t % p.gnx), iy = (int)((t / p.gnx) % p.gny), z = (int)(t / ((size_t)p.gnx * p.gny));
     const size_t planeIn = (size_t)p.nz * p.ny * p.gnx, planeOut = (size_t)p.nz * p.gny * p.gnx;
@@ -480,7 +534,7 @@ struct NodeArgs {
 };
 
 __global__ void __launch_bounds__(256) k_project_nodes(NodeArgs r) {
-    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (n >= r.G) return;
     const int i = (int)(n % r.gnx), j = (int)((n / r.gnx) % r.gny), k = (int)(n / ((size_t)r.gnx * r.gny));
     const float inv = r.invNx[i] * r.invNy[j] * r.invNz[k];
@@ -680,8 +734,10 @@ static FusedPlan *plan_of(nrh_f3d *h) {
 }
 
 // The sweeps keep the joint histogram / NMI table in LDS: at most 12288 entries (bins <= 106).
+// The voxel offsets of the sweeps are 32-bit: 3 N and the floating image below 2^32 elements.
 bool fused_supported(nrh_f3d *h) {
-    return plan_of(h)->ok && (size_t)h->params.ref_bins * h->params.flo_bins <= 12288;
+    return plan_of(h)->ok && (size_t)h->params.ref_bins * h->params.flo_bins <= 12288 &&
+           3 * (size_t)h->N <= 0xffffffffull && (size_t)h->Nf <= 0xffffffffull;
 }
 
 void collect_sweep_times(nrh_f3d *h) {
@@ -732,9 +788,11 @@ static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
 
 static unsigned grid_for(int numTiles) { return (unsigned)((numTiles + 7) / 8 * 8); }
 
-// Dynamic LDS of a sweep: the control-point window, then the histogram / NMI table in its place.
+// Dynamic LDS of a sweep: the control-point window and its z contraction, then the histogram / NMI
+// table in their place.
 static size_t sweep_lds(const FusedArgs& a) {
-    return std::max<size_t>(((size_t)a.rb * a.fb + 3) / 4 * 4, (size_t)3 * WZM * WYM * WXM) * sizeof(float);
+    // window [3][WZM][WYM][WXM] + its z contraction [3][TZ][WYM][WXM]; then histogram / table K
+    return std::max<size_t>(((size_t)a.rb * a.fb + 3) / 4 * 4, (size_t)3 * (WZM + TZ) * WYM * WXM) * sizeof(float);
 }
 
 void fused_histogram(nrh_f3d *h, float *fieldOut) {
@@ -788,7 +846,6 @@ void fused_gradient(nrh_f3d *h, float weight) {
         NRH_HIP(hipEventRecord(h->ev[3], h->stream));
         h->evPending[1] = true;
     }
-    auto blocks = [](size_t n) { return dim3((unsigned)((n + 255) / 256)); };
     ProjXArgs px{};
     px.g = p->gvox.p;
     px.out = p->projX.p;
@@ -800,7 +857,8 @@ void fused_gradient(nrh_f3d *h, float weight) {
     px.cells = p->cellsX;
     px.cellX = p->cellX.p;
     px.bX = p->bX.p;
-    const size_t ldsX = ((size_t)3 * px.rowsPerBlock * nx + (size_t)px.rowsPerBlock * px.cells * 12) * sizeof(float);
+    const size_t ldsX =
+        ((size_t)3 * px.rowsPerBlock * nx + (size_t)px.rowsPerBlock * px.cells * kCellStride) * sizeof(float);
     NRH_REQUIRE(ldsX <= 64 * 1024, NRH_ERR_UNSUPPORTED, "image rows too long for the fused projection");
     hipLaunchKernelGGL(k_project_x, dim3((unsigned)((px.rows + px.rowsPerBlock - 1) / px.rowsPerBlock)), dim3(256),
                        ldsX, h->stream, px);
@@ -814,7 +872,9 @@ void fused_gradient(nrh_f3d *h, float weight) {
     py.preY = p->preY.p;
     py.bY = p->bY.p;
     py.rangeY = p->rY.p;
-    hipLaunchKernelGGL(k_project_y, blocks((size_t)nz * gny * gnx), dim3(256), 0, h->stream, py);
+    // xcd_tile needs a grid that is a multiple of 8 (extra blocks find no work)
+    auto blocks8 = [](size_t n) { return dim3((unsigned)(((n + 255) / 256 + 7) / 8 * 8)); };
+    hipLaunchKernelGGL(k_project_y, blocks8((size_t)nz * gny * gnx), dim3(256), 0, h->stream, py);
     NodeArgs r{};
     r.in = p->projY.p;
     r.nz = nz;
@@ -827,7 +887,7 @@ void fused_gradient(nrh_f3d *h, float weight) {
     r.reorient = to_affine(h->flo.ijk);
     r.weight = node_weight_ratio(h, weight);
     r.nodeGrad = h->nodeGrad.p;
-    hipLaunchKernelGGL(k_project_nodes, blocks(h->G), dim3(256), 0, h->stream, r);
+    hipLaunchKernelGGL(k_project_nodes, blocks8(h->G), dim3(256), 0, h->stream, r);
     NRH_HIP(hipGetLastError());
 }
 

@@ -23,7 +23,6 @@ struct nrh_f3d {
     nrh::DevBuf<float> field, warped, warpedGrad, voxelGrad;
     nrh::DevBuf<float> convTmp, convDens, convDensTmp, convKernel;
     nrh::DevBuf<uint8_t> convNan;
-    nrh::DevBuf<float> beDeriv;
     nrh::DevBuf<float> leDiag;      // per-node rotation-free displacement Jacobian diagonal [3][G]
     uint64_t leDiagVersion = 0;     // cppVersion leDiag was computed at (0: none)
 
@@ -48,6 +47,7 @@ struct nrh_f3d {
     int lnccRadius[3] = { 0, 0, 0 }, lnccKernOff[3] = { 0, 0, 0 };
 
     double *pinned = nullptr;  // host-pinned scalar read-back
+    double *published = nullptr, *publishedDev = nullptr;  // coherent host memory the device writes results to
 
     uint64_t cppVersion = 1;   // bumped on every control-point change
     uint64_t histVersion = 0;  // cppVersion the histogram was computed at
@@ -95,6 +95,10 @@ void op_fill_f32(float *p, size_t n, float v, hipStream_t s);
 void op_cg(nrh_f3d *h, int first);
 void op_update_dof(nrh_f3d *h, float scale);
 double read_scalar(nrh_f3d *h, const double *dev);
+// Copies n0, n1, n2 doubles from s0, s1, s2 (device) to h->published[0..] with one small kernel
+// (no copy engine); the caller synchronises the stream before reading.
+void op_publish(nrh_f3d *h, const double *s0, int n0, const double *s1 = nullptr, int n1 = 0,
+                const double *s2 = nullptr, int n2 = 0);
 
 // LNCC measure (lncc.hip); value = mean |local correlation| into out_device[0] (count in [1])
 void op_lncc_value(nrh_f3d *h, double *out_device);

@@ -461,39 +461,91 @@ struct BeBasis {
     float b[6][27];  // xx, yy, zz, xy, yz, xz
 };
 
-// reg_spline_approxBendingEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:73-153): per-block
-// FP64 partial sums over interior nodes.
-__global__ void __launch_bounds__(kBlock)
-k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BeBasis B, double *__restrict__ partials,
-           uint32_t *__restrict__ ticket, double scale, double *__restrict__ out) {
+// Node position in float, reg_getDisplacementFromDeformation_3D (reg-lib/cpu/_reg_tools.cpp:2004-2019)
+__device__ __forceinline__ float node_pos(const Affine& m, int r, int x, int y, int z) {
 #pragma clang fp contract(off)
+    return m.m[r * 4 + 0] * (float)x + m.m[r * 4 + 1] * (float)y + m.m[r * 4 + 2] * (float)z + m.m[r * 4 + 3];
+}
+
+// Bending energy on bricks of 8 x 8 x 4 nodes (one workgroup, one node per thread): the control
+// points of the brick and its halo are staged in LDS once, so every 27-neighbour stencil reads
+// LDS instead of global memory.
+constexpr int BBX = 8, BBY = 8, BBZ = 4;
+
+struct BrickGrid {
+    int bx, by, bz;  // bricks per axis
+};
+
+__device__ __forceinline__ void brick_origin(const BrickGrid& g, int b, int& x0, int& y0, int& z0) {
+    x0 = (b % g.bx) * BBX;
+    y0 = ((b / g.bx) % g.by) * BBY;
+    z0 = (b / (g.bx * g.by)) * BBZ;
+}
+
+// reg_spline_approxBendingEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:73-153) over the
+// interior nodes of the brick; FP64 block partials, summed in block order by the last block.
+__global__ void __launch_bounds__(256)
+k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, BeBasis B,
+           double *__restrict__ partials, uint32_t *__restrict__ ticket, double scale, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    constexpr int HX = BBX + 2, HY = BBY + 2, HZ = BBZ + 2, HN = HX * HY * HZ;
+    __shared__ float sC[3][HN];  // deformation values of the brick and a one-node halo
+    __shared__ float sB[27][8];  // basis values per neighbour (xx, yy, zz, xy, yz, xz), read as broadcasts
     __shared__ double red[16];
     __shared__ bool last;
     const size_t G = (size_t)gnx * gny * gnz;
-    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int x0, y0, z0;
+    brick_origin(bg, blockIdx.x, x0, y0, z0);
+    if (threadIdx.x < 27 * 6) sB[threadIdx.x / 6][threadIdx.x % 6] = B.b[threadIdx.x % 6][threadIdx.x / 6];
+    {
+        constexpr int kIter = (3 * HN + 255) / 256;
+        float v[kIter];
+#pragma unroll
+        for (int u = 0; u < kIter; ++u) {
+            // clamped, unconditional loads keep v in registers
+            const int i = min((int)threadIdx.x + u * 256, 3 * HN - 1);
+            const int c = i / HN, r = i % HN, hx = r % HX, hy = (r / HX) % HY, hz = r / (HX * HY);
+            const int X = min(max(x0 - 1 + hx, 0), gnx - 1), Y = min(max(y0 - 1 + hy, 0), gny - 1),
+                      Z = min(max(z0 - 1 + hz, 0), gnz - 1);
+            v[u] = cpp[c * G + ((size_t)Z * gny + Y) * gnx + X];
+        }
+#pragma unroll
+        for (int u = 0; u < kIter; ++u) {
+            const int i = threadIdx.x + u * 256;
+            if (i < 3 * HN) sC[i / HN][i % HN] = v[u];
+        }
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % BBX, ty = (threadIdx.x / BBX) % BBY, tz = threadIdx.x / (BBX * BBY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
     double s = 0;
-    if (n < G) {
-        const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
-        if (x > 0 && y > 0 && z > 0 && x < gnx - 1 && y < gny - 1 && z < gnz - 1) {
-            float d[3][6] = {};
-            int i = 0;
-            for (int c = -1; c < 2; ++c)
-                for (int b = -1; b < 2; ++b)
-                    for (int a = -1; a < 2; ++a, ++i) {
-                        const size_t idx = ((size_t)(z + c) * gny + y + b) * gnx + x + a;
-                        for (int k = 0; k < 3; ++k) {
-                            const float cv = cpp[idx + k * G];
-                            for (int t = 0; t < 6; ++t) d[k][t] += B.b[t][i] * cv;
-                        }
+    if (x > 0 && y > 0 && z > 0 && x < gnx - 1 && y < gny - 1 && z < gnz - 1) {
+        float d[3][6] = {};
+        int i = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a, ++i) {
+                    const int h = ((tz + c) * HY + ty + b) * HX + tx + a;
+                    float bt[6];
+#pragma unroll
+                    for (int t = 0; t < 6; ++t) bt[t] = sB[i][t];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float cv = sC[k][h];
+#pragma unroll
+                        for (int t = 0; t < 6; ++t) d[k][t] += bt[t] * cv;
                     }
-            s = (double)(d[0][0] * d[0][0] + d[0][1] * d[0][1] + d[0][2] * d[0][2]) +
-                2.0 * (double)(d[0][3] * d[0][3] + d[0][4] * d[0][4] + d[0][5] * d[0][5]);
-            for (int k = 1; k < 3; ++k) {
-                s += (double)(d[k][0] * d[k][0]);
-                s += (double)(d[k][1] * d[k][1]);
-                s += (double)(d[k][2] * d[k][2]);
-                s += 2.0 * (double)(d[k][3] * d[k][3] + d[k][4] * d[k][4] + d[k][5] * d[k][5]);
-            }
+                }
+        s = (double)(d[0][0] * d[0][0] + d[0][1] * d[0][1] + d[0][2] * d[0][2]) +
+            2.0 * (double)(d[0][3] * d[0][3] + d[0][4] * d[0][4] + d[0][5] * d[0][5]);
+        for (int k = 1; k < 3; ++k) {
+            s += (double)(d[k][0] * d[k][0]);
+            s += (double)(d[k][1] * d[k][1]);
+            s += (double)(d[k][2] * d[k][2]);
+            s += 2.0 * (double)(d[k][3] * d[k][3] + d[k][4] * d[k][4] + d[k][5] * d[k][5]);
         }
     }
     s = block_sum_double(s, red);
@@ -515,75 +567,100 @@ k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BeBasis B, 
     }
 }
 
-__global__ void __launch_bounds__(1024)
-k_reduce_partials(const double *__restrict__ partials, int n, double scale, double *__restrict__ out) {
-    __shared__ double red[16];
-    double s = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
-    s = block_sum_double(s, red);
-    if (threadIdx.x == 0) *out = s * scale;
-}
-
-// Node position in float, reg_getDisplacementFromDeformation_3D (reg-lib/cpu/_reg_tools.cpp:2004-2019)
-__device__ __forceinline__ float node_pos(const Affine& m, int r, int x, int y, int z) {
+// reg_spline_approxBendingEnergyGradient3D (regul.cpp:287-451) on a brick: the displacement
+// (cpp - node) of the brick with a two-node halo in LDS, the 18 second derivatives of every node
+// of the brick with a one-node halo (regul.cpp:317-379, the cross terms doubled in double), then
+// the adjoint 27-point gather grad += weight / G * value (regul.cpp:381-448) for the brick's
+// nodes.  Optionally tracks the max length of the final gradient.  The reference's in-place
+// (p - node) + node round trip of the points is k_round_trip, run after this kernel.
+__global__ void __launch_bounds__(256)
+k_be_gradient(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, Affine xyz, BeBasis B,
+              float ratio, float *__restrict__ grad, uint32_t *__restrict__ maxBits) {
 #pragma clang fp contract(off)
-    return m.m[r * 4 + 0] * (float)x + m.m[r * 4 + 1] * (float)y + m.m[r * 4 + 2] * (float)z + m.m[r * 4 + 3];
-}
-
-// First half of reg_spline_approxBendingEnergyGradient3D (regul.cpp:317-379): 18 second
-// derivatives of the displacement at every node.
-__global__ void __launch_bounds__(kBlock)
-k_be_deriv(const float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz, BeBasis B, float *__restrict__ der,
-           uint32_t *__restrict__ maxBits) {
-#pragma clang fp contract(off)
+    constexpr int CX = BBX + 4, CY = BBY + 4, CZ = BBZ + 4, CN = CX * CY * CZ;  // points: two-node halo
+    constexpr int DX = BBX + 2, DY = BBY + 2, DZ = BBZ + 2, DN = DX * DY * DZ;  // derivatives: one-node halo
+    __shared__ float sC[3][CN];
+    __shared__ float sD[18][DN];
+    __shared__ float sB[27][8];  // basis values per neighbour (xx, yy, zz, xy, yz, xz), read as broadcasts
     const size_t G = (size_t)gnx * gny * gnz;
-    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n == 0 && maxBits) *maxBits = 0u;  // reset for k_be_grad's max-length tracking
-    if (n >= G) return;
-    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
-    float d[3][6] = {};
-    int i = 0;
-    for (int c = -1; c < 2; ++c)
-        for (int b = -1; b < 2; ++b)
-            for (int a = -1; a < 2; ++a, ++i) {
-                const int X = x + a, Y = y + b, Z = z + c;
-                if (X < 0 || Y < 0 || Z < 0 || X >= gnx || Y >= gny || Z >= gnz) continue;
-                const size_t idx = ((size_t)Z * gny + Y) * gnx + X;
-                for (int k = 0; k < 3; ++k) {
-                    const float cv = cpp[idx + k * G] - node_pos(xyz, k, X, Y, Z);
-                    for (int t = 0; t < 6; ++t) d[k][t] += B.b[t][i] * cv;
-                }
-            }
-    float *o = der + 18 * n;
-    for (int t = 0; t < 3; ++t)
-        for (int k = 0; k < 3; ++k) o[t * 3 + k] = d[k][t];
-    for (int t = 3; t < 6; ++t)
-        for (int k = 0; k < 3; ++k) o[t * 3 + k] = (float)(2.0 * (double)d[k][t]);
-}
-
-// Second half (regul.cpp:381-448): adjoint 27-point gather, grad += weight / G * value.
-// The reference converts the control points to displacements in place and back around the
-// computation (regul.cpp:312, 450), so every point becomes (p - node) + node in float; the
-// kernel applies the same round trip to its node (k_be_deriv, the only reader, has finished).
-__global__ void __launch_bounds__(kBlock)
-k_be_grad(const float *__restrict__ der, int gnx, int gny, int gnz, BeBasis B, float ratio, float *__restrict__ grad,
-          uint32_t *__restrict__ maxBits, float *__restrict__ cpp, Affine xyz) {
-#pragma clang fp contract(off)
-    const size_t G = (size_t)gnx * gny * gnz;
-    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = n < G;
+    int x0, y0, z0;
+    brick_origin(bg, blockIdx.x, x0, y0, z0);
+    if (threadIdx.x < 27 * 6) sB[threadIdx.x / 6][threadIdx.x % 6] = B.b[threadIdx.x % 6][threadIdx.x / 6];
+    {
+        constexpr int kIter = (3 * CN + 255) / 256;
+        float v[kIter];
+#pragma unroll
+        for (int u = 0; u < kIter; ++u) {
+            // clamped, unconditional loads keep v in registers (out-of-grid entries are never read)
+            const int i = min((int)threadIdx.x + u * 256, 3 * CN - 1);
+            const int c = i / CN, r = i % CN, hx = r % CX, hy = (r / CX) % CY, hz = r / (CX * CY);
+            const int X = min(max(x0 - 2 + hx, 0), gnx - 1), Y = min(max(y0 - 2 + hy, 0), gny - 1),
+                      Z = min(max(z0 - 2 + hz, 0), gnz - 1);
+            v[u] = cpp[c * G + ((size_t)Z * gny + Y) * gnx + X] - node_pos(xyz, c, X, Y, Z);
+        }
+#pragma unroll
+        for (int u = 0; u < kIter; ++u) {
+            const int i = threadIdx.x + u * 256;
+            if (i < 3 * CN) sC[i / CN][i % CN] = v[u];
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < DN; r += 256) {
+        const int hx = r % DX, hy = (r / DX) % DY, hz = r / (DX * DY);
+        const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+        float d[3][6] = {};
+        if (x >= 0 && y >= 0 && z >= 0 && x < gnx && y < gny && z < gnz) {
+            int i = 0;
+#pragma unroll
+            for (int c = -1; c < 2; ++c)
+#pragma unroll
+                for (int b = -1; b < 2; ++b)
+#pragma unroll
+                    for (int a = -1; a < 2; ++a, ++i) {
+                        const int X = x + a, Y = y + b, Z = z + c;
+                        if (X < 0 || Y < 0 || Z < 0 || X >= gnx || Y >= gny || Z >= gnz) continue;
+                        const int h = ((hz + 1 + c) * CY + hy + 1 + b) * CX + hx + 1 + a;
+                        float bt[6];
+#pragma unroll
+                        for (int t = 0; t < 6; ++t) bt[t] = sB[i][t];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const float cv = sC[k][h];
+#pragma unroll
+                            for (int t = 0; t < 6; ++t) d[k][t] += bt[t] * cv;
+                        }
+                    }
+        }
+        for (int t = 0; t < 3; ++t)
+            for (int k = 0; k < 3; ++k) sD[t * 3 + k][r] = d[k][t];
+        for (int t = 3; t < 6; ++t)
+            for (int k = 0; k < 3; ++k) sD[t * 3 + k][r] = (float)(2.0 * (double)d[k][t]);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % BBX, ty = (threadIdx.x / BBX) % BBY, tz = threadIdx.x / (BBX * BBY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    const bool valid = x < gnx && y < gny && z < gnz;
     float g0 = 0.f, g1 = 0.f, g2 = 0.f;
     if (valid) {
-        const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+        const size_t n = ((size_t)z * gny + y) * gnx + x;
         float gv[3] = { 0.f, 0.f, 0.f };
         int a = 0;
-        for (int Z = z - 1; Z < z + 2; ++Z)
-            for (int Y = y - 1; Y < y + 2; ++Y)
-                for (int X = x - 1; X < x + 2; ++X, ++a) {
+#pragma unroll
+        for (int c = -1; c < 2; ++c)
+#pragma unroll
+            for (int b = -1; b < 2; ++b)
+#pragma unroll
+                for (int q = -1; q < 2; ++q, ++a) {
+                    const int X = x + q, Y = y + b, Z = z + c;
                     if (X < 0 || Y < 0 || Z < 0 || X >= gnx || Y >= gny || Z >= gnz) continue;
-                    const float *dp = der + 18 * (((size_t)Z * gny + Y) * gnx + X);
+                    const int r = ((tz + 1 + c) * DY + ty + 1 + b) * DX + tx + 1 + q;
+                    float bt[6];
+#pragma unroll
+                    for (int t = 0; t < 6; ++t) bt[t] = sB[a][t];
+#pragma unroll
                     for (int t = 0; t < 6; ++t)
-                        for (int k = 0; k < 3; ++k) gv[k] += dp[t * 3 + k] * B.b[t][a];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) gv[k] += sD[t * 3 + k][r] * bt[t];
                 }
         g0 = grad[n] + ratio * gv[0];
         g1 = grad[n + G] + ratio * gv[1];
@@ -591,19 +668,30 @@ k_be_grad(const float *__restrict__ der, int gnx, int gny, int gnz, BeBasis B, f
         grad[n] = g0;
         grad[n + G] = g1;
         grad[n + 2 * G] = g2;
-        for (int k = 0; k < 3; ++k) {
-            const float node = node_pos(xyz, k, x, y, z);
-            cpp[n + k * G] = (cpp[n + k * G] - node) + node;
-        }
     }
     if (maxBits) {
         // reg_getMaximalLength of the final gradient (lengths >= 0 order like their bit patterns)
-        uint32_t b = valid ? __float_as_uint(sqrtf(g0 * g0 + g1 * g1 + g2 * g2)) : 0u;
+        uint32_t bm = valid ? __float_as_uint(sqrtf(g0 * g0 + g1 * g1 + g2 * g2)) : 0u;
         for (int off = 32; off > 0; off >>= 1) {
-            const uint32_t o = __shfl_down(b, off, 64);
-            b = o > b ? o : b;
+            const uint32_t o = __shfl_down(bm, off, 64);
+            bm = o > bm ? o : bm;
         }
-        if ((threadIdx.x & 63) == 0) atomicMax(maxBits, b);
+        if ((threadIdx.x & 63) == 0) atomicMax(maxBits, bm);
+    }
+}
+
+// The in-place reg_getDisplacementFromDeformation / reg_getDeformationFromDisplacement pair the
+// reference wraps around the bending-energy gradient (regul.cpp:312, 450): p <- (p - node) + node.
+__global__ void __launch_bounds__(256)
+k_round_trip(float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz) {
+#pragma clang fp contract(off)
+    const size_t G = (size_t)gnx * gny * gnz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= G) return;
+    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+    for (int k = 0; k < 3; ++k) {
+        const float node = node_pos(xyz, k, x, y, z);
+        cpp[n + k * G] = (cpp[n + k * G] - node) + node;
     }
 }
 
@@ -1180,24 +1268,34 @@ void op_voxel_to_node(nrh_f3d *h, float weight) {
     NRH_HIP(hipGetLastError());
 }
 
+static BrickGrid brick_grid(const nrh_f3d *h) {
+    BrickGrid g;
+    g.bx = (h->cpp.nx + BBX - 1) / BBX;
+    g.by = (h->cpp.ny + BBY - 1) / BBY;
+    g.bz = (h->cpp.nz + BBZ - 1) / BBZ;
+    return g;
+}
+
 void op_bending_energy(nrh_f3d *h, double *out_device) {
     static const BeBasis B = make_be_basis();
-    const unsigned nb = blocks_for(h->G);
+    const BrickGrid bg = brick_grid(h);
+    const unsigned nb = (unsigned)(bg.bx * bg.by * bg.bz);
     h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)nb + 64));
-    hipLaunchKernelGGL(k_be_value, dim3(nb), dim3(kBlock), 0, h->stream, h->cppPos.p, h->cpp.nx, h->cpp.ny, h->cpp.nz,
-                       B, h->partials.p, h->maxBits.p + 6, 1.0 / (double)(3 * h->G), out_device);
+    hipLaunchKernelGGL(k_be_value, dim3(nb), dim3(256), 0, h->stream, h->cppPos.p, h->cpp.nx, h->cpp.ny, h->cpp.nz,
+                       bg, B, h->partials.p, h->maxBits.p + 6, 1.0 / (double)(3 * h->G), out_device);
     NRH_HIP(hipGetLastError());
 }
 
 void op_bending_energy_gradient(nrh_f3d *h, float weight, bool trackMax) {
     static const BeBasis B = make_be_basis();
-    h->beDeriv.alloc(18 * h->G);
+    const BrickGrid bg = brick_grid(h);
     uint32_t *mb = trackMax ? h->maxBits.p : nullptr;
-    hipLaunchKernelGGL(k_be_deriv, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->cppPos.p, h->cpp.nx,
-                       h->cpp.ny, h->cpp.nz, to_affine(h->cpp.xyz), B, h->beDeriv.p, mb);
+    if (trackMax) op_fill_u32(h->maxBits.p, 1, 0u, h->stream);
     const float ratio = weight / (float)h->G;
-    hipLaunchKernelGGL(k_be_grad, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->beDeriv.p, h->cpp.nx,
-                       h->cpp.ny, h->cpp.nz, B, ratio, h->nodeGrad.p, mb, h->cppPos.p, to_affine(h->cpp.xyz));
+    hipLaunchKernelGGL(k_be_gradient, dim3((unsigned)(bg.bx * bg.by * bg.bz)), dim3(256), 0, h->stream, h->cppPos.p,
+                       h->cpp.nx, h->cpp.ny, h->cpp.nz, bg, to_affine(h->cpp.xyz), B, ratio, h->nodeGrad.p, mb);
+    hipLaunchKernelGGL(k_round_trip, dim3(blocks_for(h->G)), dim3(256), 0, h->stream, h->cppPos.p, h->cpp.nx,
+                       h->cpp.ny, h->cpp.nz, to_affine(h->cpp.xyz));
     NRH_HIP(hipGetLastError());
     h->cppVersion++;  // the round trip may have changed the last bit of some points
 }
@@ -1350,10 +1448,26 @@ void op_update_dof(nrh_f3d *h, float scale) {
     h->cppVersion++;
 }
 
+namespace {
+__global__ void k_publish(double *__restrict__ dst, const double *__restrict__ s0, int n0,
+                          const double *__restrict__ s1, int n1, const double *__restrict__ s2, int n2) {
+    const int i = threadIdx.x;
+    if (i < n0) dst[i] = s0[i];
+    else if (i < n0 + n1) dst[i] = s1[i - n0];
+    else if (i < n0 + n1 + n2) dst[i] = s2[i - n0 - n1];
+}
+}  // namespace
+
+void op_publish(nrh_f3d *h, const double *s0, int n0, const double *s1, int n1, const double *s2, int n2) {
+    NRH_REQUIRE(n0 + n1 + n2 <= 16, NRH_ERR_ARG, "too many scalars to publish");
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, h->stream, h->publishedDev, s0, n0, s1, n1, s2, n2);
+    NRH_HIP(hipGetLastError());
+}
+
 double read_scalar(nrh_f3d *h, const double *dev) {
-    NRH_HIP(hipMemcpyAsync(h->pinned, dev, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    op_publish(h, dev, 1);
     NRH_HIP(hipStreamSynchronize(h->stream));
-    return h->pinned[0];
+    return h->published[0];
 }
 
 }  // namespace nrh
