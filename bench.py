@@ -7,9 +7,11 @@ with 64 bins (68 internal), bending energy 0.001, control-point spacing 5 voxels
 plus one objective gradient at the same control points (the unit "obj+grad eval"), run
 through libnrhip's fused sweeps with the images and control points resident in HBM.
 
-Prints one JSON line.  The roofline entry covers the dominant kernel (the gradient sweep),
-timed with HIP events on the stream it runs on; the CPU baseline times the reference CPU
-path (oracle/_ref, OpenMP) on a bounded sample of the same workload.
+Prints one JSON line.  The roofline entry covers the dominant kernel (the gradient sweep,
+k_sweep<float, true>), timed with HIP events on the stream it runs on; its algorithmic bytes are
+the reference and floating images read once (8 B per voxel), its measured traffic also counts
+the voxel-gradient image it writes for the projection passes.  The CPU baseline times the
+reference CPU path (oracle/_ref, OpenMP) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -260,7 +262,7 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic() if not storage else None,
             "traffic_source": "profiles/f3d_eval_256x256x160_hbm_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
-            "kernel": "k_fused_pass<float,true> (gradient sweep)",
+            "kernel": "k_sweep<float,true> (gradient sweep)",
             "kernel_ms": grad_ms, "histogram_sweep_ms": hist_ms, "algorithmic_bytes_per_launch": algo_bytes,
         },
         "cpu_baseline": None,

@@ -803,20 +803,27 @@ struct LeBasis {
 };
 
 // Rotation-free displacement Jacobian at an interior node (reg-lib/cpu/_reg_localTrans_regul.cpp:596-634).
+// B: the first-order basis in LDS (broadcast reads; as a kernel argument its 81 values would not
+// fit the scalar registers).
 __device__ M33 le_matrix(const float *__restrict__ cpp, size_t G, int gnx, int gny, int x, int y, int z,
-                         const LeBasis& B, const M33& reo) {
+                         const float (*B)[27], const M33& reo) {
 #pragma clang fp contract(off)
     M33 M = {};
     int i = 0;
+#pragma unroll
     for (int c = -1; c < 2; ++c)
+#pragma unroll
         for (int b = -1; b < 2; ++b)
+#pragma unroll
             for (int a = -1; a < 2; ++a, ++i) {
                 const size_t idx = ((size_t)(z + c) * gny + y + b) * gnx + x + a;
                 const float cv[3] = { cpp[idx], cpp[idx + G], cpp[idx + 2 * G] };
+                const float b0 = B[0][i], b1 = B[1][i], b2 = B[2][i];
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    M.m[0][k] += B.b[0][i] * cv[k];
-                    M.m[1][k] += B.b[1][i] * cv[k];
-                    M.m[2][k] += B.b[2][i] * cv[k];
+                    M.m[0][k] += b0 * cv[k];
+                    M.m[1][k] += b1 * cv[k];
+                    M.m[2][k] += b2 * cv[k];
                 }
             }
     M = m33_mul(reo, M);
@@ -836,6 +843,9 @@ k_le_terms(const float *__restrict__ cpp, int gnx, int gny, int gnz, LeBasis B, 
 #pragma clang fp contract(off)
     __shared__ double red[16];
     __shared__ bool last;
+    __shared__ float sB[3][27];
+    if (threadIdx.x < 81) sB[threadIdx.x / 27][threadIdx.x % 27] = B.b[threadIdx.x / 27][threadIdx.x % 27];
+    __syncthreads();
     const size_t G = (size_t)gnx * gny * gnz;
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0;
@@ -843,7 +853,7 @@ k_le_terms(const float *__restrict__ cpp, int gnx, int gny, int gnz, LeBasis B, 
         const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
         float d0 = 0.f, d1 = 0.f, d2 = 0.f;
         if (x > 0 && y > 0 && z > 0 && x < gnx - 1 && y < gny - 1 && z < gnz - 1) {
-            const M33 M = le_matrix(cpp, G, gnx, gny, x, y, z, B, reo);
+            const M33 M = le_matrix(cpp, G, gnx, gny, x, y, z, sB, reo);
             for (int b = 0; b < 3; ++b)
                 for (int a = 0; a < 3; ++a) {
                     const double v = 0.5 * (double)(M.m[a][b] + M.m[b][a]);
@@ -882,6 +892,9 @@ __global__ void __launch_bounds__(kBlock)
 k_le_grad(const float *__restrict__ diag, int gnx, int gny, int gnz, LeBasis B, M33 inv, float ratio,
           float *__restrict__ grad, uint32_t *__restrict__ maxBits) {
 #pragma clang fp contract(off)
+    __shared__ float sB[3][27];
+    if (threadIdx.x < 81) sB[threadIdx.x / 27][threadIdx.x % 27] = B.b[threadIdx.x / 27][threadIdx.x % 27];
+    __syncthreads();
     const size_t G = (size_t)gnx * gny * gnz;
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = n < G;
@@ -895,9 +908,9 @@ k_le_grad(const float *__restrict__ diag, int gnx, int gny, int gnz, LeBasis B, 
                 if (X < 1 || Y < 1 || Z < 1 || X > gnx - 2 || Y > gny - 2 || Z > gnz - 2) continue;
                 const size_t s = ((size_t)Z * gny + Y) * gnx + X;
                 const int i = 26 - ((c + 1) * 9 + (b + 1) * 3 + (a + 1));
-                const float v0 = -2.f * diag[s] * B.b[0][i];
-                const float v1 = -2.f * diag[s + G] * B.b[1][i];
-                const float v2 = -2.f * diag[s + 2 * G] * B.b[2][i];
+                const float v0 = -2.f * diag[s] * sB[0][i];
+                const float v1 = -2.f * diag[s + G] * sB[1][i];
+                const float v2 = -2.f * diag[s + 2 * G] * sB[2][i];
                 g0 += ratio * (inv.m[0][0] * v0 + inv.m[0][1] * v1 + inv.m[0][2] * v2);
                 g1 += ratio * (inv.m[1][0] * v0 + inv.m[1][1] * v1 + inv.m[1][2] * v2);
                 g2 += ratio * (inv.m[2][0] * v0 + inv.m[2][1] * v1 + inv.m[2][2] * v2);
