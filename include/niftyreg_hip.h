@@ -65,6 +65,13 @@ int nrh_refine_control_point_grid(const nrh_geom *ref, const nrh_geom *grid, con
  * smoothing (0.7355 voxel) and nearest-neighbour decimation by 2 along the axes flagged in
  * axes[0..2].  Call with out_data NULL to obtain the output geometry first. */
 int nrh_downsample_image(const nrh_geom *in, const float *data, const int axes[3], nrh_geom *out, float *out_data);
+/* reg_createSymmetricControlPointGrids<float> (reg-lib/cpu/_reg_localTrans.cpp:132-358): the
+ * forward and backward velocity grids of reg_f3d2, axis-aligned (sform only) over the union of
+ * both images' corners.  affine must be NULL (a half-affine initialisation is not supported:
+ * NRH_ERR_UNSUPPORTED).  fw / bw receive the identity positions (3*G floats each; NULL to query
+ * the geometry). */
+int nrh_create_symmetric_control_point_grids(const nrh_geom *ref, const nrh_geom *flo, const float spacing_mm[3],
+                                             const float *affine, nrh_geom *grid, float *fw, float *bw);
 
 /* ------------------------------------------------------------------ F3D */
 typedef struct nrh_f3d nrh_f3d;
@@ -138,6 +145,42 @@ int nrh_f3d_download_node_gradient(nrh_f3d *h, float *planar);  /* 3*G */
 int nrh_f3d_upload_node_gradient(nrh_f3d *h, const float *planar);
 /* reg_nmi's jointHistogramPro / jointHistogramLog (rb*fb + rb + fb doubles each) */
 int nrh_f3d_download_histogram(nrh_f3d *h, double *pro, double *logh);
+
+/* ---- velocity grids (reg_f3d2) ----
+ * A handle whose control points form a cubic B-spline velocity grid (reg_f3d2's forward grid on
+ * the reference -> floating handle, its backward grid on a second handle created with the two
+ * images swapped).  Grids with affine extensions are not supported. */
+/* The grid set-up of reg_f3d2::Initialise (reg-lib/_reg_f3d2.cpp:585-590): intent_p1 =
+ * SPLINE_VEL_GRID, intent_p2 = steps (squarings of the exponentiation; 0 lets the next
+ * nrh_f3d_def_field_from_velocity choose). */
+int nrh_f3d_set_velocity_grid(nrh_f3d *h, int steps);
+/* the grid's intent_p2 (read after an exponentiation that updated it; set to copy the forward
+ * step count onto the backward grid, reg-lib/_reg_f3d2.cpp:146-152) */
+int nrh_f3d_get_velocity_steps(nrh_f3d *h, int *steps);
+int nrh_f3d_set_velocity_steps(nrh_f3d *h, int steps);
+/* Compute::GetDefFieldFromVelocityGrid (reg-lib/Compute.cpp:223-228): scaling and squaring of
+ * the grid into the handle's deformation field (then nrh_f3d_resample / _image_gradient use it). */
+int nrh_f3d_def_field_from_velocity(nrh_f3d *h, int update_step_number);
+/* Compute::ExponentiateGradient (reg-lib/Compute.cpp:283-326): h's voxel-based gradient carried
+ * along the intermediate fields of bw's velocity grid and averaged. */
+int nrh_f3d_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw);
+/* Compute::UpdateVelocityField (reg-lib/Compute.cpp:336-348): grid += node gradient * scale. */
+int nrh_f3d_update_velocity_field(nrh_f3d *h, float scale, int optimise_x, int optimise_y, int optimise_z);
+/* Compute::SymmetriseVelocityFields (reg-lib/Compute.cpp:357-391) of the two grids. */
+int nrh_f3d_symmetrise_velocity_fields(nrh_f3d *fw, nrh_f3d *bw);
+/* Compute::BchUpdate (reg-lib/Compute.cpp:350-355): the reference's compute_BCH_update stops
+ * with "needs updating" (reg-lib/cpu/_reg_localTrans.cpp:3828); this returns NRH_ERR_UNSUPPORTED. */
+int nrh_f3d_bch_update(nrh_f3d *h, float scale, int bch_update_value);
+/* Compute::DefFieldCompose (reg-lib/Compute.cpp:393-396): the handle's deformation field is
+ * replaced by `field` (host, planar 3*voxels(geom), positions) evaluated at its positions. */
+int nrh_f3d_compose_field(nrh_f3d *h, const nrh_geom *geom, const float *field);
+/* Compute::ResampleGradient (reg-lib/Compute.cpp:399-405), linear only: the voxel-based gradient
+ * resampled through the deformation field, into out (host, 3*N). */
+int nrh_f3d_resample_gradient(nrh_f3d *h, int interpolation, float padding, float *out);
+/* Deformation / voxel-gradient upload for the per-operation checks (DefContent::SetDeformationField,
+ * F3dContent::GetVoxelBasedMeasureGradient written by the caller). */
+int nrh_f3d_upload_field(nrh_f3d *h, const float *planar);
+int nrh_f3d_upload_voxel_gradient(nrh_f3d *h, const float *planar);
 
 /* Fused objective: reg_f3d::GetObjectiveFunctionValue (reg-lib/_reg_f3d.cpp:443-463) in one
  * device sweep (field, warp and joint histogram fused; no field or warped image is written).

@@ -14,8 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnrhip.so")
 
-SOURCES = ["capi.cpp", "capi_eval.cpp", "f3d_ops.hip", "f3d_fused.hip", "driver.cpp", "aladin.hip", "lncc.hip",
-           "nifti_io.cpp"]
+SOURCES = ["capi.cpp", "capi_eval.cpp", "capi_velocity.cpp", "f3d_ops.hip", "f3d_fused.hip", "f3d_velocity.hip",
+           "driver.cpp", "aladin.hip", "lncc.hip", "nifti_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NRH_OFFLOAD_ARCH", "gfx950")

@@ -27,6 +27,7 @@ extern "C" {
 int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
     return guard([&] {
         NRH_REQUIRE(h && value, NRH_ERR_ARG, "null argument");
+        NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_objective");
         const nrh_f3d_params& p = h->params;
         const bool lncc = p.measure == NRH_MEASURE_LNCC;
         if (p.similarity_weight > 0) {
@@ -60,6 +61,7 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
 int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
     return guard([&] {
         NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
+        NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_gradient");
         const nrh_f3d_params& p = h->params;
         if (p.similarity_weight > 0 && p.measure == NRH_MEASURE_LNCC) {
             // reg_f3d::GetSimilarityMeasureGradient with reg_lncc (reg-lib/_reg_f3d.cpp:353-362,

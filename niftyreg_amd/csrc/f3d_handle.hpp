@@ -46,6 +46,13 @@ struct nrh_f3d {
     float lnccSigma = 0.f;
     int lnccRadius[3] = { 0, 0, 0 }, lnccKernOff[3] = { 0, 0, 0 };
 
+    // velocity grid (reg_f3d2): intent_p1 SPLINE_VEL_GRID and intent_p2 (squaring steps; 0: choose)
+    bool velocity = false;
+    int velSteps = 0;
+    nrh::DevBuf<float> velTmp;      // flow / ping-pong field [3][N]
+    nrh::DevBuf<float> velFields;   // intermediate fields of the gradient exponentiation
+    nrh::DevBuf<uint32_t> velKeys;  // min / max keys of the flow displacement
+
     double *pinned = nullptr;  // host-pinned scalar read-back
     double *published = nullptr, *publishedDev = nullptr;  // coherent host memory the device writes results to
 
@@ -108,6 +115,14 @@ float node_weight_ratio(const nrh_f3d *h, float weight);
 void op_normalise_device(nrh_f3d *h);
 void op_max_length_bits(nrh_f3d *h);     // max length of nodeGrad into maxBits[0]
 void op_normalise_from_bits(nrh_f3d *h);  // normalise by maxBits[0]; scalars[1] = the length
+
+// velocity grids (f3d_velocity.hip)
+void op_vel_def_field(nrh_f3d *h, bool update);
+void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw);
+void op_resample_gradient(nrh_f3d *h, float pad, float *out);
+void op_compose_field(nrh_f3d *h, const nrh_geom& g, const float *def);
+void op_update_velocity(nrh_f3d *h, float scale, bool ox, bool oy, bool oz);
+void op_symmetrise_velocity(nrh_f3d *fw, nrh_f3d *bw);
 
 // fused evaluation (f3d_fused.hip)
 bool fused_supported(nrh_f3d *h);

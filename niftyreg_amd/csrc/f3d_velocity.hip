@@ -1,0 +1,640 @@
+// Velocity-grid operations of symmetric diffeomorphic F3D (reg_f3d2) for gfx950.
+//
+// A velocity grid holds cubic B-spline control points (positions in mm) whose displacement is a
+// stationary velocity field; its deformation is the exponential map computed by scaling and
+// squaring: the flow (the grid evaluated at every voxel's position), its displacement divided by
+// 2^s, then s self-compositions with trilinear interpolation (reg-lib/cpu/_reg_localTrans.cpp:
+// 3465-3683).  ExponentiateGradient carries the voxel-based gradient along the intermediate
+// fields of the opposite direction's grid (reg-lib/Compute.cpp:283-326).
+//
+// Every kernel is one thread per voxel (or node) and follows the reference's scalar arithmetic
+// (float where it computes in float, double where reg_tools promotes, no fused multiply-adds),
+// so the results equal the reference CPU build bit for bit.  The flow sums its 64 B-spline terms
+// in the reference's order; the compositions and the gradient resampling gather 8 taps from the
+// previous field, which stays in L2 for the small displacements the squaring produces.
+// Grids with affine extensions (a reg_f3d2 run given an input affine) are not handled here.
+
+#include "device.hpp"
+#include "f3d_handle.hpp"
+#include "m33.hpp"
+#include <cmath>
+
+namespace nrh {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct FieldGeom {
+    int nx, ny, nz;
+    size_t n;
+    Affine xyz, ijk;
+};
+
+FieldGeom field_geom(const nrh_geom& g) {
+    FieldGeom f;
+    f.nx = g.nx; f.ny = g.ny; f.nz = g.nz;
+    f.n = voxels(g);
+    f.xyz = to_affine(g.xyz);
+    f.ijk = to_affine(g.ijk);
+    return f;
+}
+
+struct VelGrid {
+    const float *cp;
+    int gnx, gny, gnz;
+    size_t G;
+    Affine xyz, ijk;
+};
+
+// voxel -> mm in float, as reg_getDeformationFromDisplacement_3D (reg-lib/cpu/_reg_tools.cpp:
+// 2107-2150) adds it
+__device__ __forceinline__ float3 id_pos(const Affine& m, int x, int y, int z) {
+#pragma clang fp contract(off)
+    float3 r;
+    r.x = m.m[0] * (float)x + m.m[1] * (float)y + m.m[2] * (float)z + m.m[3];
+    r.y = m.m[4] * (float)x + m.m[5] * (float)y + m.m[6] * (float)z + m.m[7];
+    r.z = m.m[8] * (float)x + m.m[9] * (float)y + m.m[10] * (float)z + m.m[11];
+    return r;
+}
+
+// mm -> voxel in float (reg-lib/cpu/_reg_localTrans.cpp:940-952, 2384-2396)
+__device__ __forceinline__ float3 to_vox(const Affine& m, float3 p) {
+#pragma clang fp contract(off)
+    float3 r;
+    r.x = m.m[0] * p.x + m.m[1] * p.y + m.m[2] * p.z + m.m[3];
+    r.y = m.m[4] * p.x + m.m[5] * p.y + m.m[6] * p.z + m.m[7];
+    r.z = m.m[8] * p.x + m.m[9] * p.y + m.m[10] * p.z + m.m[11];
+    return r;
+}
+
+__device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+// get_SlidedValues 3-D for positions (reg-lib/cpu/_reg_splineBasis.cpp:496-551): the value at the
+// clamped index plus the mm offset of the clamped steps.
+__device__ __forceinline__ float3 slided(const float *__restrict__ d, size_t n, int nx, int ny, int nz,
+                                         const Affine& xyz, int x, int y, int z) {
+#pragma clang fp contract(off)
+    const int cx = clampi(x, nx), cy = clampi(y, ny), cz = clampi(z, nz);
+    const float sx = (float)(x - cx), sy = (float)(y - cy), sz = (float)(z - cz);
+    const size_t idx = ((size_t)cz * ny + cy) * nx + cx;
+    float3 r;
+    r.x = d[idx] + (sx * xyz.m[0] + sy * xyz.m[1] + sz * xyz.m[2]);
+    r.y = d[idx + n] + (sx * xyz.m[4] + sy * xyz.m[5] + sz * xyz.m[6]);
+    r.z = d[idx + 2 * n] + (sx * xyz.m[8] + sy * xyz.m[9] + sz * xyz.m[10]);
+    return r;
+}
+
+// Order-preserving integer image of a float, for atomic min / max.
+__device__ __forceinline__ uint32_t float_key(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// The velocity grid evaluated at every voxel's position of the field geometry: the composition
+// branch of reg_cubic_spline_getDeformationField3D (reg-lib/cpu/_reg_localTrans.cpp:903-1079,
+// scalar order) applied to the identity field, as reg_spline_getFlowFieldFromVelocityGrid
+// (:3465-3491) does.  SCALED: writes (flow - x) / scaling + x, the first field of the squaring
+// (:3546-3561); otherwise writes the displacement flow - x and folds its extremes into keys[0]
+// (min) and keys[1] (max).
+template<bool SCALED>
+__global__ void __launch_bounds__(kBlock) k_vel_flow(VelGrid g, FieldGeom f, double scaling, float *__restrict__ out,
+                                                     uint32_t *__restrict__ keys) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    float lo = INFINITY, hi = -INFINITY;
+    if (i < f.n) {
+        const int x = (int)(i % f.nx), y = (int)((i / f.nx) % f.ny), z = (int)(i / ((size_t)f.nx * f.ny));
+        const float3 p = id_pos(f.xyz, x, y, z);
+        const float3 v = to_vox(g.ijk, p);
+        int pre[3];
+        float b[3][4];
+        const float vc[3] = { v.x, v.y, v.z };
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            pre[r] = floor_int(vc[r]);
+            float t = vc[r] - (float)pre[r];
+            pre[r]--;
+            if (t < 0) t = 0;
+            bspline_basis(t, b[r]);
+        }
+        const bool inside = pre[0] >= 0 && pre[0] + 3 < g.gnx && pre[1] >= 0 && pre[1] + 3 < g.gny && pre[2] >= 0 &&
+                            pre[2] + 3 < g.gnz;
+        float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+        for (int c = 0; c < 4; ++c) {
+            const int Z = pre[2] + c;
+            // get_GridValues (reg-lib/cpu/_reg_splineBasis.cpp:602-665): once a row of a slice
+            // falls outside the grid, the rest of that slice goes through the sliding path
+            bool out = !(Z > -1 && Z < g.gnz);
+            for (int bb = 0; bb < 4; ++bb) {
+                const int Y = pre[1] + bb;
+                if (!(Y > -1 && Y < g.gny && !out)) out = true;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int X = pre[0] + a;
+                    float3 cv;
+                    if (inside || (X > -1 && X < g.gnx && !out)) {
+                        const size_t idx = ((size_t)Z * g.gny + Y) * g.gnx + X;
+                        cv.x = g.cp[idx];
+                        cv.y = g.cp[idx + g.G];
+                        cv.z = g.cp[idx + 2 * g.G];
+                    } else {
+                        cv = slided(g.cp, g.G, g.gnx, g.gny, g.gnz, g.xyz, X, Y, Z);
+                    }
+                    const float w = b[0][a] * b[1][bb] * b[2][c];
+                    r0 += cv.x * w;
+                    r1 += cv.y * w;
+                    r2 += cv.z * w;
+                }
+            }
+        }
+        const float d0 = r0 - p.x, d1 = r1 - p.y, d2 = r2 - p.z;
+        if (SCALED) {
+            out[i] = (float)((double)d0 / scaling) + p.x;
+            out[i + f.n] = (float)((double)d1 / scaling) + p.y;
+            out[i + 2 * f.n] = (float)((double)d2 / scaling) + p.z;
+        } else {
+            out[i] = d0;
+            out[i + f.n] = d1;
+            out[i + 2 * f.n] = d2;
+            lo = fminf(d0, fminf(d1, d2));
+            hi = fmaxf(d0, fmaxf(d1, d2));
+        }
+    }
+    if (!SCALED) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, off, 64));
+            hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&keys[0], float_key(lo));
+            atomicMax(&keys[1], float_key(hi));
+        }
+    }
+}
+
+// Displacement d -> d / scaling + x (reg_tools_divideValueToImage in double, then
+// reg_getDeformationFromDisplacement in float).
+__global__ void __launch_bounds__(kBlock) k_vel_scale(const float *disp, FieldGeom f, double scaling, float *out) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= f.n) return;
+    const int x = (int)(i % f.nx), y = (int)((i / f.nx) % f.ny), z = (int)(i / ((size_t)f.nx * f.ny));
+    const float3 p = id_pos(f.xyz, x, y, z);
+    out[i] = (float)((double)disp[i] / scaling) + p.x;
+    out[i + f.n] = (float)((double)disp[i + f.n] / scaling) + p.y;
+    out[i + 2 * f.n] = (float)((double)disp[i + 2 * f.n] / scaling) + p.z;
+}
+
+// reg_defField_compose3D without a mask (reg-lib/cpu/_reg_localTrans.cpp:2333-2457): out[i] is
+// the trilinear interpolation of field D (geometry d) at position pos[i], with sliding outside
+// D.  out may alias pos (each thread reads only its own position) but not D.
+__global__ void __launch_bounds__(kBlock) k_compose(const float *__restrict__ D, FieldGeom d, const float *pos,
+                                                    size_t m, float *out) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const float3 real = make_float3(pos[i], pos[i + m], pos[i + 2 * m]);
+    const float3 v = to_vox(d.ijk, real);
+    const int px = floor_int(v.x), py = floor_int(v.y), pz = floor_int(v.z);
+    const float rx1 = v.x - (float)px, ry1 = v.y - (float)py, rz1 = v.z - (float)pz;
+    const float rx[2] = { 1.f - rx1, rx1 }, ry[2] = { 1.f - ry1, ry1 }, rz[2] = { 1.f - rz1, rz1 };
+    const bool inside = px > -1 && px + 1 < d.nx && py > -1 && py + 1 < d.ny && pz > -1 && pz + 1 < d.nz;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int Z = pz + c;
+        const bool inZ = Z > -1 && Z < d.nz;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int Y = py + b;
+            const float tb = ry[b] * rz[c];
+            const bool inY = Y > -1 && Y < d.ny;
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int X = px + a;
+                float3 dv;
+                if (inside || (X > -1 && X < d.nx && inY && inZ)) {
+                    const size_t idx = ((size_t)Z * d.ny + Y) * d.nx + X;
+                    dv = make_float3(D[idx], D[idx + d.n], D[idx + 2 * d.n]);
+                } else {
+                    dv = slided(D, d.n, d.nx, d.ny, d.nz, d.xyz, X, Y, Z);
+                }
+                const float w = rx[a] * tb;
+                a0 += dv.x * w;
+                a1 += dv.y * w;
+                a2 += dv.z * w;
+            }
+        }
+    }
+    out[i] = a0;
+    out[i + m] = a1;
+    out[i + 2 * m] = a2;
+}
+
+struct ResampleGradArgs {
+    const float *grad;      // vector image to resample (planar [3][gn])
+    int gnx, gny, gnz;
+    size_t gn;
+    Affine gradIjk;         // its mm -> voxel
+    const float *def;       // positions (planar [3][n]) on the output geometry
+    int nx, ny, nz;
+    size_t n;
+    float reo[9];           // inverse of the polar rotation of the field's sform
+    float sp[3];            // real spacing of the output
+    float pad;
+    const float *accum;     // optional: result + accum[i] (double sum)
+    double divisor;         // optional (> 0): then / divisor
+    float *out;             // must not alias grad or def
+};
+
+// reg_trilinearResampleGradient (reg-lib/cpu/_reg_resampling.cpp:1547-1766): the vector image
+// sampled at each position (Floor / Ceil taps, padding outside) times the field's forward
+// difference Jacobian, rotation removed and divided by the real spacing; at the last index of
+// an axis the difference steps back one voxel, and so do the other two axes' differences there.
+__global__ void __launch_bounds__(kBlock) k_resample_gradient(ResampleGradArgs r) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= r.n) return;
+    const int x = (int)(i % r.nx), y = (int)((i / r.nx) % r.ny), z = (int)(i / ((size_t)r.nx * r.ny));
+    const float3 fc = to_vox(r.gradIjk, make_float3(r.def[i], r.def[i + r.n], r.def[i + 2 * r.n]));
+    const float fcv[3] = { fc.x, fc.y, fc.z };
+    int lo[3], hi[3];
+    float bw[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = floor_int(fcv[k]);
+        const int t = (int)fcv[k];
+        hi[k] = t + (fcv[k] > (float)t);
+        bw[k][1] = fabsf(fcv[k] - (float)lo[k]);
+        bw[k][0] = 1.f - bw[k][1];
+    }
+    const int gdim[3] = { r.gnx, r.gny, r.gnz };
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int Z = c ? hi[2] : lo[2];
+        if (Z > -1 && Z < gdim[2]) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int Y = b ? hi[1] : lo[1];
+                if (Y > -1 && Y < gdim[1]) {
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        const int X = a ? hi[0] : lo[0];
+                        const float w = bw[0][a] * bw[1][b] * bw[2][c];
+                        if (X > -1 && X < gdim[0]) {
+                            const size_t idx = ((size_t)Z * r.gny + Y) * r.gnx + X;
+                            vx += r.grad[idx] * w;
+                            vy += r.grad[idx + r.gn] * w;
+                            vz += r.grad[idx + 2 * r.gn] * w;
+                        } else {
+                            vx += r.pad * w;
+                            vy += r.pad * w;
+                            vz += r.pad * w;
+                        }
+                    }
+                } else {
+                    const float w = r.pad * bw[1][b] * bw[2][c];
+                    vx += w;
+                    vy += w;
+                    vz += w;
+                }
+            }
+        } else {
+            const float w = r.pad * bw[2][c];
+            vx += w;
+            vy += w;
+            vz += w;
+        }
+    }
+    float J[3][3] = {};
+    const float basis[2] = { 1.f, 0.f }, deriv[2] = { -1.f, 1.f };
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        int Z = z + c;
+        float bz0 = basis[c], bz1 = deriv[c];
+        if (z == r.nz - 1) {
+            if (c == 1) Z -= 2;
+            bz0 = fabsf(bz0 - 1.f);
+            bz1 = -bz1;
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            int Y = y + b;
+            float by0 = basis[b], by1 = deriv[b];
+            if (y == r.ny - 1) {
+                if (b == 1) Y -= 2;
+                by0 = fabsf(by0 - 1.f);
+                by1 = -by1;
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                int X = x + a;
+                float bx0 = basis[a], bx1 = deriv[a];
+                if (x == r.nx - 1) {
+                    if (a == 1) X -= 2;
+                    bx0 = fabsf(bx0 - 1.f);
+                    bx1 = -bx1;
+                }
+                const float w0 = bx1 * by0 * bz0, w1 = bx0 * by1 * bz0, w2 = bx0 * by0 * bz1;
+                // fields with one voxel along an axis would step to index -1: clamp the read
+                // (its weight is then zero on that axis in the reference as well)
+                const size_t idx = ((size_t)clampi(Z, r.nz) * r.ny + clampi(Y, r.ny)) * r.nx + clampi(X, r.nx);
+                const float dx = r.def[idx], dy = r.def[idx + r.n], dz = r.def[idx + 2 * r.n];
+                J[0][0] += w0 * dx;
+                J[0][1] += w1 * dx;
+                J[0][2] += w2 * dx;
+                J[1][0] += w0 * dy;
+                J[1][1] += w1 * dy;
+                J[1][2] += w2 * dy;
+                J[2][0] += w0 * dz;
+                J[2][1] += w1 * dz;
+                J[2][2] += w2 * dz;
+            }
+        }
+    }
+    float R[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            R[a][b] = (r.reo[a * 3] * J[0][b] + r.reo[a * 3 + 1] * J[1][b] + r.reo[a * 3 + 2] * J[2][b]) / r.sp[b];
+    float o[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) o[a] = R[a][0] * vx + R[a][1] * vy + R[a][2] * vz;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float v = o[a];
+        if (r.accum) v = (float)((double)v + (double)r.accum[i + a * r.n]);
+        if (r.divisor > 0.0) v = (float)((double)v / r.divisor);
+        r.out[i + a * r.n] = v;
+    }
+}
+
+// Compute::UpdateVelocityField (reg-lib/Compute.cpp:336-348): grid + gradient * scale, the
+// gradient zeroed on the axes not optimised; reg_tools arithmetic in double.
+__global__ void __launch_bounds__(kBlock) k_update_velocity(float *__restrict__ cp, const float *__restrict__ grad,
+                                                            float scale, size_t G, int ox, int oy, int oz) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 3 * G) return;
+    const int c = (int)(i / G);
+    const bool on = c == 0 ? ox : (c == 1 ? oy : oz);
+    const float s = on ? (float)((double)grad[i] * (double)scale) : 0.f;
+    cp[i] = (float)((double)cp[i] + (double)s);
+}
+
+// Compute::SymmetriseVelocityFields (reg-lib/Compute.cpp:357-391): displacements f, b of the two
+// grids (same geometry) become (f - b) / 2 and (b - f) / 2.
+__global__ void __launch_bounds__(kBlock) k_symmetrise(float *__restrict__ fw, float *__restrict__ bw, FieldGeom g) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.n) return;
+    const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((size_t)g.nx * g.ny));
+    const float3 p = id_pos(g.xyz, x, y, z);
+    const float pv[3] = { p.x, p.y, p.z };
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t k = i + c * g.n;
+        const float f = fw[k] - pv[c], b = bw[k] - pv[c];
+        const float nf = (float)((double)f - (double)b), nb = (float)((double)b - (double)f);
+        fw[k] = (float)((double)nf * 0.5) + pv[c];
+        bw[k] = (float)((double)nb * 0.5) + pv[c];
+    }
+}
+
+inline dim3 blocks_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+VelGrid vel_grid(const nrh_f3d *h) {
+    VelGrid g;
+    g.cp = h->cppPos.p;
+    g.gnx = h->cpp.nx; g.gny = h->cpp.ny; g.gnz = h->cpp.nz;
+    g.G = h->G;
+    g.xyz = to_affine(h->cpp.xyz);
+    g.ijk = to_affine(h->cpp.ijk);
+    return g;
+}
+
+// reg-lib/cpu/_reg_localTrans.cpp:3520-3544: the number of squarings that brings the largest
+// displacement component below 0.28 mm (3-D) or 0.35 mm (2-D) per step, at least 6.
+int squaring_number(float mn, float mx, bool is3d) {
+    float extrema = std::fabs(mn) < mx ? mx : std::fabs(mn);
+    const float maxLength = is3d ? 0.28f : 0.35f;
+    int sq = 1;
+    while (extrema / (float)std::pow(2.0, sq) >= maxLength && sq < 1000) sq++;
+    return sq < 6 ? 6 : sq;
+}
+
+float key_to_float(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+double scaling_of(int steps) {
+    const int sq = steps < 0 ? -steps : steps;
+    const double s = (double)(float)std::pow(2.0, sq);
+    return steps < 0 ? -s : s;
+}
+
+void launch_compose(const float *D, const FieldGeom& d, const float *pos, size_t m, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_compose, blocks_for(m), dim3(kBlock), 0, s, D, d, pos, m, out);
+}
+
+// The rotation-free, spacing-normalised Jacobian factors of reg_trilinearResampleGradient for a
+// field on geometry g: reorientation from the sform (reg-lib/cpu/_reg_resampling.cpp:1580) and
+// the real spacing (reg_getRealImageSpacing, reg-lib/cpu/_reg_tools.cpp:253-274) when the sform
+// is set, else the voxel size.  Host arithmetic as the reference: niftilib's float mat33 helpers
+// and Mat44Mul<float>.
+void resample_gradient_factors(const nrh_geom& g, float reo[9], float sp[3]) {
+    M33 s;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) s.m[a][b] = g.xyz[a * 4 + b];
+    const M33 R = m33_inverse(m33_polar(s));
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) reo[a * 3 + b] = R.m[a][b];
+    if (g.sform) {
+        auto mul = [&](const float in[3], float out[3]) {
+            for (int r = 0; r < 3; ++r)
+                out[r] = (float)((double)g.xyz[r * 4] * (double)in[0] + (double)g.xyz[r * 4 + 1] * (double)in[1] +
+                                 (double)g.xyz[r * 4 + 2] * (double)in[2] + (double)g.xyz[r * 4 + 3]);
+        };
+        const float o[3] = { 0.f, 0.f, 0.f };
+        float r1[3];
+        mul(o, r1);
+        for (int k = 0; k < 3; ++k) {
+            float e[3] = { 0.f, 0.f, 0.f }, r2[3];
+            e[k] = 1.f;
+            mul(e, r2);
+            volatile float a = r1[0] - r2[0], b = r1[1] - r2[1], c = r1[2] - r2[2];
+            volatile float a2 = a * a, b2 = b * b, c2 = c * c;
+            volatile float sum = a2 + b2;
+            sum = sum + c2;
+            sp[k] = std::sqrt((float)sum);
+        }
+        if (g.nz <= 1) sp[2] = g.dz;
+    } else {
+        sp[0] = g.dx; sp[1] = g.dy; sp[2] = g.dz;
+    }
+}
+
+}  // namespace
+
+// Compute::GetDefFieldFromVelocityGrid (reg-lib/Compute.cpp:223-228) for a grid without affine
+// extensions: reg_spline_getDefFieldFromVelocityGrid into the handle's deformation field.
+void op_vel_def_field(nrh_f3d *h, bool update) {
+    NRH_REQUIRE(h->velocity, NRH_ERR_STATE, "the control points are not a velocity grid");
+    ensure_field(h);
+    h->velTmp.alloc(3 * h->N);
+    const FieldGeom f = field_geom(h->ref);
+    const VelGrid g = vel_grid(h);
+    int steps = h->velSteps;
+    int sq;
+    if (update || steps == 0) {
+        const uint32_t init[2] = { 0xffffffffu, 0u };
+        h->velKeys.alloc(2);
+        NRH_HIP(hipMemcpyAsync(h->velKeys.p, init, sizeof init, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_vel_flow<false>, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, 1.0, h->velTmp.p,
+                           h->velKeys.p);
+        NRH_HIP(hipGetLastError());
+        uint32_t keys[2];
+        NRH_HIP(hipMemcpyAsync(keys, h->velKeys.p, sizeof keys, hipMemcpyDeviceToHost, h->stream));
+        NRH_HIP(hipStreamSynchronize(h->stream));
+        sq = squaring_number(key_to_float(keys[0]), key_to_float(keys[1]), h->ref.nz > 1);
+        steps = steps >= 0 ? sq : -sq;
+        // the scaled flow goes where the last squaring must not land: the field after sq swaps
+        float *first = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
+        if (first == h->velTmp.p) {
+            // in place is safe: each thread reads and writes only its own voxel
+            hipLaunchKernelGGL(k_vel_scale, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->velTmp.p, f,
+                               scaling_of(steps), h->velTmp.p);
+        } else {
+            hipLaunchKernelGGL(k_vel_scale, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->velTmp.p, f,
+                               scaling_of(steps), h->field.p);
+        }
+        NRH_HIP(hipGetLastError());
+    } else {
+        sq = steps < 0 ? -steps : steps;
+        float *first = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
+        hipLaunchKernelGGL(k_vel_flow<true>, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, scaling_of(steps),
+                           first, nullptr);
+        NRH_HIP(hipGetLastError());
+    }
+    float *src = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
+    float *dst = (src == h->field.p) ? h->velTmp.p : h->field.p;
+    for (int k = 0; k < sq; ++k) {
+        launch_compose(src, f, src, f.n, dst, h->stream);
+        std::swap(src, dst);
+    }
+    NRH_HIP(hipGetLastError());
+    h->velSteps = steps;
+}
+
+// reg_spline_getIntermediateDefFieldFromVelGrid (reg-lib/cpu/_reg_localTrans.cpp:3616-3683) of
+// grid `g` (steps squarings) on geometry f: fields[0..count-1] (count <= |steps| + 1).
+static void intermediate_fields(const VelGrid& g, const FieldGeom& f, int steps, float *fields, int count,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_vel_flow<true>, blocks_for(f.n), dim3(kBlock), 0, s, g, f, scaling_of(steps), fields,
+                       nullptr);
+    for (int k = 0; k + 1 < count; ++k)
+        launch_compose(fields + (size_t)k * 3 * f.n, f, fields + (size_t)k * 3 * f.n, f.n,
+                       fields + (size_t)(k + 1) * 3 * f.n, s);
+    NRH_HIP(hipGetLastError());
+}
+
+// Compute::ExponentiateGradient (reg-lib/Compute.cpp:283-326) without affine components: the
+// voxel-based gradient of h, resampled along the intermediate fields of bw's velocity grid
+// (evaluated on h's reference geometry) and accumulated, then divided by 2^|steps of bw|.
+void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw) {
+    NRH_REQUIRE(bw->velocity, NRH_ERR_STATE, "the backward control points are not a velocity grid");
+    NRH_REQUIRE(!h->voxelGrad.empty(), NRH_ERR_STATE, "voxel-based gradient not computed");
+    const int steps = bw->velSteps;
+    const int sq = steps < 0 ? -steps : steps;
+    if (sq == 0) return;
+    // the backward grid's last update ran on its own stream
+    NRH_HIP(hipStreamSynchronize(bw->stream));
+    const FieldGeom f = field_geom(h->ref);
+    h->velFields.alloc((size_t)sq * 3 * f.n);
+    h->velTmp.alloc(3 * f.n);
+    intermediate_fields(vel_grid(bw), f, steps, h->velFields.p, sq, h->stream);
+    ResampleGradArgs r{};
+    r.gnx = h->ref.nx; r.gny = h->ref.ny; r.gnz = h->ref.nz;
+    r.gn = f.n;
+    r.gradIjk = f.ijk;
+    r.nx = f.nx; r.ny = f.ny; r.nz = f.nz;
+    r.n = f.n;
+    resample_gradient_factors(h->ref, r.reo, r.sp);
+    r.pad = 0.f;
+    float *src = h->voxelGrad.p, *dst = h->velTmp.p;
+    const double div = std::pow(2.0, sq);
+    for (int k = 0; k < sq; ++k) {
+        r.grad = src;
+        r.accum = src;
+        r.def = h->velFields.p + (size_t)k * 3 * f.n;
+        r.divisor = (k == sq - 1) ? div : 0.0;
+        r.out = dst;
+        hipLaunchKernelGGL(k_resample_gradient, blocks_for(f.n), dim3(kBlock), 0, h->stream, r);
+        std::swap(src, dst);
+    }
+    NRH_HIP(hipGetLastError());
+    if (src != h->voxelGrad.p)
+        NRH_HIP(hipMemcpyAsync(h->voxelGrad.p, src, 3 * f.n * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+}
+
+// Compute::ResampleGradient(1, padding) (reg-lib/Compute.cpp:399-405): the voxel-based gradient
+// resampled through the handle's deformation field into out (device, planar [3][N]).
+void op_resample_gradient(nrh_f3d *h, float pad, float *out) {
+    NRH_REQUIRE(!h->voxelGrad.empty(), NRH_ERR_STATE, "voxel-based gradient not computed");
+    NRH_REQUIRE(!h->field.empty(), NRH_ERR_STATE, "deformation field not computed");
+    const FieldGeom f = field_geom(h->ref);
+    ResampleGradArgs r{};
+    r.grad = h->voxelGrad.p;
+    r.gnx = h->ref.nx; r.gny = h->ref.ny; r.gnz = h->ref.nz;
+    r.gn = f.n;
+    r.gradIjk = f.ijk;
+    r.def = h->field.p;
+    r.nx = f.nx; r.ny = f.ny; r.nz = f.nz;
+    r.n = f.n;
+    resample_gradient_factors(h->ref, r.reo, r.sp);
+    r.pad = pad;
+    r.accum = nullptr;
+    r.divisor = 0.0;
+    r.out = out;
+    hipLaunchKernelGGL(k_resample_gradient, blocks_for(f.n), dim3(kBlock), 0, h->stream, r);
+    NRH_HIP(hipGetLastError());
+}
+
+// Compute::DefFieldCompose (reg-lib/Compute.cpp:393-396): the handle's deformation field becomes
+// `def` (device, geometry g) evaluated at its positions.
+void op_compose_field(nrh_f3d *h, const nrh_geom& g, const float *def) {
+    NRH_REQUIRE(!h->field.empty(), NRH_ERR_STATE, "deformation field not computed");
+    launch_compose(def, field_geom(g), h->field.p, h->N, h->field.p, h->stream);
+    NRH_HIP(hipGetLastError());
+}
+
+void op_update_velocity(nrh_f3d *h, float scale, bool ox, bool oy, bool oz) {
+    NRH_REQUIRE(h->velocity, NRH_ERR_STATE, "the control points are not a velocity grid");
+    hipLaunchKernelGGL(k_update_velocity, blocks_for(3 * h->G), dim3(kBlock), 0, h->stream, h->cppPos.p,
+                       h->nodeGrad.p, scale, h->G, (int)ox, (int)oy, (int)oz);
+    NRH_HIP(hipGetLastError());
+    h->cppVersion++;
+}
+
+void op_symmetrise_velocity(nrh_f3d *fw, nrh_f3d *bw) {
+    NRH_REQUIRE(fw->velocity && bw->velocity, NRH_ERR_STATE, "the control points are not velocity grids");
+    NRH_REQUIRE(fw->cpp.nx == bw->cpp.nx && fw->cpp.ny == bw->cpp.ny && fw->cpp.nz == bw->cpp.nz &&
+                    std::memcmp(fw->cpp.xyz, bw->cpp.xyz, sizeof fw->cpp.xyz) == 0,
+                NRH_ERR_ARG, "forward and backward velocity grids differ in geometry");
+    // the backward grid's last update ran on its own stream
+    NRH_HIP(hipStreamSynchronize(bw->stream));
+    hipLaunchKernelGGL(k_symmetrise, blocks_for(fw->G), dim3(kBlock), 0, fw->stream, fw->cppPos.p, bw->cppPos.p,
+                       field_geom(fw->cpp));
+    NRH_HIP(hipGetLastError());
+    NRH_HIP(hipStreamSynchronize(fw->stream));
+    fw->cppVersion++;
+    bw->cppVersion++;
+}
+
+}  // namespace nrh

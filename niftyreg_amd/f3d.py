@@ -154,6 +154,58 @@ class F3dContent:
         self._op("nrh_f3d_normalise_gradient", ctypes.c_double(max_length))
 
     # --- downloads
+    # --- velocity grids (reg_f3d2; Compute's velocity virtuals)
+    def SetVelocityGrid(self, steps=6):
+        """Mark the control points as a velocity grid (reg_f3d2::Initialise, intent_p2 = steps)."""
+        self._op("nrh_f3d_set_velocity_grid", int(steps))
+
+    @property
+    def velocity_steps(self):
+        s = ctypes.c_int()
+        self._op("nrh_f3d_get_velocity_steps", ctypes.byref(s))
+        return s.value
+
+    @velocity_steps.setter
+    def velocity_steps(self, steps):
+        self._op("nrh_f3d_set_velocity_steps", int(steps))
+
+    def GetDefFieldFromVelocityGrid(self, update_step_number=True):
+        self._op("nrh_f3d_def_field_from_velocity", int(bool(update_step_number)))
+
+    def ExponentiateGradient(self, backward):
+        self._op("nrh_f3d_exponentiate_gradient", backward.h)
+
+    def UpdateVelocityField(self, scale, optimise_x=True, optimise_y=True, optimise_z=True):
+        self._op("nrh_f3d_update_velocity_field", ctypes.c_float(scale), int(optimise_x), int(optimise_y),
+                 int(optimise_z))
+
+    def SymmetriseVelocityFields(self, backward):
+        self._op("nrh_f3d_symmetrise_velocity_fields", backward.h)
+
+    def BchUpdate(self, scale, bch_update_value):
+        self._op("nrh_f3d_bch_update", ctypes.c_float(scale), int(bch_update_value))
+
+    def DefFieldCompose(self, geom: Geom, field):
+        f = np.ascontiguousarray(field, np.float32).reshape(-1)
+        assert f.size == 3 * geom.nvox
+        g = geom.to_c()
+        self._op("nrh_f3d_compose_field", ctypes.byref(g), _lib.fptr(f))
+
+    def ResampleGradient(self, interpolation=1, padding=0.0):
+        out = np.empty(3 * self.N, np.float32)
+        self._op("nrh_f3d_resample_gradient", int(interpolation), ctypes.c_float(padding), _lib.fptr(out))
+        return out
+
+    def set_deformation_field(self, field):
+        f = np.ascontiguousarray(field, np.float32).reshape(-1)
+        assert f.size == 3 * self.N
+        self._op("nrh_f3d_upload_field", _lib.fptr(f))
+
+    def set_voxel_gradient(self, grad):
+        g = np.ascontiguousarray(grad, np.float32).reshape(-1)
+        assert g.size == 3 * self.N
+        self._op("nrh_f3d_upload_voxel_gradient", _lib.fptr(g))
+
     def _dl(self, name, n, dtype=np.float32):
         out = np.empty(n, dtype)
         self._op(name, _lib.fptr(out))

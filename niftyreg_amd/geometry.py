@@ -106,3 +106,20 @@ def create_control_point_grid(ref: Geom, spacing_mm):
     data = np.zeros(3 * geom.nvox, np.float32)
     _lib.call("nrh_create_control_point_grid", ctypes.byref(rc), _lib.fptr(sp), ctypes.byref(g), _lib.fptr(data))
     return geom, data
+
+
+def create_symmetric_control_point_grids(ref: Geom, flo: Geom, spacing_mm):
+    """reg_createSymmetricControlPointGrids<float> without an affine (reg-lib/cpu/_reg_localTrans.cpp:132-358).
+
+    Returns (grid Geom, forward identity positions, backward identity positions)."""
+    sp = np.asarray(spacing_mm, np.float32)
+    g = _lib.nrh_geom()
+    rc, fc = ref.to_c(), flo.to_c()
+    _lib.call("nrh_create_symmetric_control_point_grids", ctypes.byref(rc), ctypes.byref(fc), _lib.fptr(sp), None,
+              ctypes.byref(g), None, None)
+    geom = Geom.from_c(g)
+    fw = np.zeros(3 * geom.nvox, np.float32)
+    bw = np.zeros(3 * geom.nvox, np.float32)
+    _lib.call("nrh_create_symmetric_control_point_grids", ctypes.byref(rc), ctypes.byref(fc), _lib.fptr(sp), None,
+              ctypes.byref(g), _lib.fptr(fw), _lib.fptr(bw))
+    return geom, fw, bw

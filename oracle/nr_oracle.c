@@ -1575,3 +1575,468 @@ int orc_symmetric_update(float fw[16], float bw[16]) {
     memcpy(bw, nb, sizeof(nb));
     return 1;
 }
+
+/* ------------------------------------------------------------------ velocity fields (F3D2) */
+/* The scaling-and-squaring exponentiation of a cubic B-spline velocity grid and the operations
+ * reg_f3d2 builds on it.  Grids and fields hold positions (deformations) in mm. */
+
+/* Voxel -> mm position in float, as reg_getDeformationFromDisplacement_3D adds it
+ * (reg-lib/cpu/_reg_tools.cpp:2107-2150): float products and sums, left to right. */
+static void identity_pos(const float *m, int x, int y, int z, float out[3]) {
+    for (int r = 0; r < 3; ++r)
+        out[r] = M(m, r, 0) * (float)x + M(m, r, 1) * (float)y + M(m, r, 2) * (float)z + M(m, r, 3);
+}
+
+/* mm -> voxel in float (the real-to-voxel products of reg-lib/cpu/_reg_localTrans.cpp:2384-2396
+ * and 940-952) */
+static void to_voxel(const float *m, const float p[3], float out[3]) {
+    for (int r = 0; r < 3; ++r) out[r] = M(m, r, 0) * p[0] + M(m, r, 1) * p[1] + M(m, r, 2) * p[2] + M(m, r, 3);
+}
+
+static int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+/* get_SlidedValues 3-D (reg-lib/cpu/_reg_splineBasis.cpp:496-551) for a deformation (not a
+ * displacement): the value at the clamped index plus the mm offset of the clamped steps. */
+static void slided(const float *d, size_t n, int nx, int ny, int nz, const float *xyz, int x, int y, int z,
+                   float out[3]) {
+    const int cx = clampi(x, nx), cy = clampi(y, ny), cz = clampi(z, nz);
+    const int sx = x - cx, sy = y - cy, sz = z - cz;
+    const size_t idx = ((size_t)cz * ny + cy) * nx + cx;
+    for (int r = 0; r < 3; ++r) {
+        const float shift = (float)sx * M(xyz, r, 0) + (float)sy * M(xyz, r, 1) + (float)sz * M(xyz, r, 2);
+        out[r] = d[idx + r * n] + shift;
+    }
+}
+
+/* get_GridValues 3-D (reg-lib/cpu/_reg_splineBasis.cpp:602-665) with sliding outside the grid,
+ * including its flag that, once a row falls outside, reads the rest of that slice through the
+ * sliding path (which returns the same values inside). */
+static void grid_values_slided(const orc_geom *cpp, const float *d, int x0, int y0, int z0, float *gx, float *gy,
+                               float *gz) {
+    const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    int coord = 0;
+    for (int Z = z0; Z < z0 + 4; ++Z) {
+        int out = !(Z > -1 && Z < cpp->nz);
+        for (int Y = y0; Y < y0 + 4; ++Y) {
+            if (!(Y > -1 && Y < cpp->ny && !out)) out = 1;
+            for (int X = x0; X < x0 + 4; ++X, ++coord) {
+                if (X > -1 && X < cpp->nx && !out) {
+                    const size_t idx = ((size_t)Z * cpp->ny + Y) * cpp->nx + X;
+                    gx[coord] = d[idx];
+                    gy[coord] = d[idx + g];
+                    gz[coord] = d[idx + 2 * g];
+                } else {
+                    float v[3];
+                    slided(d, g, cpp->nx, cpp->ny, cpp->nz, cpp->xyz, X, Y, Z, v);
+                    gx[coord] = v[0];
+                    gy[coord] = v[1];
+                    gz[coord] = v[2];
+                }
+            }
+        }
+    }
+}
+
+/* reg_spline_getFlowFieldFromVelocityGrid (reg-lib/cpu/_reg_localTrans.cpp:3465-3491) for a grid
+ * without affine extensions: the identity field of the reference geometry, then the composition
+ * branch of reg_cubic_spline_getDeformationField3D (:903-1079, scalar order) evaluates the grid at
+ * those positions.  flow: planar [3][N] positions. */
+void orc_vel_flow(const orc_geom *ref, const orc_geom *cpp, const float *vel, float *flow) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    size_t i = 0;
+    for (int z = 0; z < ref->nz; ++z)
+        for (int y = 0; y < ref->ny; ++y)
+            for (int x = 0; x < ref->nx; ++x, ++i) {
+                float real[3], vox[3], xb[4], yb[4], zb[4], gx[64], gy[64], gz[64];
+                identity_pos(ref->xyz, x, y, z, real);
+                to_voxel(cpp->ijk, real, vox);
+                int pre[3];
+                float *bases[3] = { xb, yb, zb };
+                for (int r = 0; r < 3; ++r) {
+                    pre[r] = floor_int(vox[r]);
+                    float t = vox[r] - (float)pre[r];
+                    pre[r]--;
+                    if (t < 0) t = 0;
+                    orc_bspline_basis(t, bases[r]);
+                }
+                grid_values_slided(cpp, vel, pre[0], pre[1], pre[2], gx, gy, gz);
+                float r0 = 0, r1 = 0, r2 = 0;
+                int coord = 0;
+                for (int c = 0; c < 4; ++c)
+                    for (int b = 0; b < 4; ++b)
+                        for (int a = 0; a < 4; ++a, ++coord) {
+                            const float w = xb[a] * yb[b] * zb[c];
+                            r0 += gx[coord] * w;
+                            r1 += gy[coord] * w;
+                            r2 += gz[coord] * w;
+                        }
+                flow[i] = r0;
+                flow[i + n] = r1;
+                flow[i + 2 * n] = r2;
+            }
+}
+
+/* reg_defField_compose3D without a mask (reg-lib/cpu/_reg_localTrans.cpp:2333-2457): every
+ * position of `upd` is replaced by the trilinear interpolation of `def` (same geometry g) at that
+ * position, with sliding outside the field.  upd may alias nothing else. */
+void orc_def_field_compose(const orc_geom *g, const float *def, float *upd) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    for (size_t i = 0; i < n; ++i) {
+        const float real[3] = { upd[i], upd[i + n], upd[i + 2 * n] };
+        float vox[3];
+        to_voxel(g->ijk, real, vox);
+        int pre[3];
+        float rel[3][2];
+        for (int r = 0; r < 3; ++r) {
+            pre[r] = floor_int(vox[r]);
+            rel[r][1] = vox[r] - (float)pre[r];
+            rel[r][0] = 1.f - rel[r][1];
+        }
+        float acc[3] = { 0, 0, 0 };
+        for (int c = 0; c < 2; ++c) {
+            const int Z = pre[2] + c;
+            const int inZ = Z > -1 && Z < g->nz;
+            for (int b = 0; b < 2; ++b) {
+                const int Y = pre[1] + b;
+                const float tb = rel[1][b] * rel[2][c];
+                const int inY = Y > -1 && Y < g->ny;
+                for (int a = 0; a < 2; ++a) {
+                    const int X = pre[0] + a;
+                    float v[3];
+                    if (X > -1 && X < g->nx && inY && inZ) {
+                        const size_t idx = ((size_t)Z * g->ny + Y) * g->nx + X;
+                        v[0] = def[idx];
+                        v[1] = def[idx + n];
+                        v[2] = def[idx + 2 * n];
+                    } else {
+                        slided(def, n, g->nx, g->ny, g->nz, g->xyz, X, Y, Z, v);
+                    }
+                    const float w = rel[0][a] * tb;
+                    acc[0] += v[0] * w;
+                    acc[1] += v[1] * w;
+                    acc[2] += v[2] * w;
+                }
+            }
+        }
+        upd[i] = acc[0];
+        upd[i + n] = acc[1];
+        upd[i + 2 * n] = acc[2];
+    }
+}
+
+/* The flow as a displacement scaled by 2^-steps (sign of *steps: direction) back in deformation
+ * form: reg_getDisplacementFromDeformation, reg_tools_divideValueToImage and
+ * reg_getDeformationFromDisplacement (reg-lib/cpu/_reg_localTrans.cpp:3513-3561). */
+static void scaled_flow(const orc_geom *ref, float *flow, int steps, int update, int *stepsOut) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    size_t i = 0;
+    for (int z = 0; z < ref->nz; ++z)
+        for (int y = 0; y < ref->ny; ++y)
+            for (int x = 0; x < ref->nx; ++x, ++i) {
+                float p[3];
+                identity_pos(ref->xyz, x, y, z, p);
+                for (int r = 0; r < 3; ++r) flow[i + r * n] -= p[r];
+            }
+    int sq = 1;
+    if (update || steps == 0) {
+        /* reg-lib/cpu/_reg_localTrans.cpp:3520-3544: the largest |displacement| sets the number of
+         * squarings (each step below 0.28 mm in 3-D, 0.35 mm in 2-D), at least 6 */
+        float mn = flow[0], mx = flow[0];
+        for (size_t k = 1; k < 3 * n; ++k) {
+            if (flow[k] < mn) mn = flow[k];
+            if (mx < flow[k]) mx = flow[k];
+        }
+        float extrema = fabsf(mn) < mx ? mx : fabsf(mn);
+        const float maxLength = ref->nz > 1 ? 0.28f : 0.35f;
+        while (extrema / (float)pow(2, sq) >= maxLength) sq++;
+        if (sq < 6) sq = 6;
+        steps = steps >= 0 ? sq : -sq;
+    } else {
+        sq = steps < 0 ? -steps : steps;
+    }
+    const double scaling = steps < 0 ? -(double)(float)pow(2, sq) : (double)(float)pow(2, sq);
+    i = 0;
+    for (int z = 0; z < ref->nz; ++z)
+        for (int y = 0; y < ref->ny; ++y)
+            for (int x = 0; x < ref->nx; ++x, ++i) {
+                float p[3];
+                identity_pos(ref->xyz, x, y, z, p);
+                for (int r = 0; r < 3; ++r) flow[i + r * n] = (float)((double)flow[i + r * n] / scaling) + p[r];
+            }
+    *stepsOut = steps;
+}
+
+/* reg_spline_getDefFieldFromVelocityGrid for a velocity grid without affine extensions
+ * (reg-lib/cpu/_reg_localTrans.cpp:3583-3613, 3493-3581): flow field, displacement scaled by
+ * 2^-steps, then `steps` self-compositions.  *steps is the grid's intent_p2: read, and updated
+ * when `update` is set or it is 0. */
+void orc_vel_def_field(const orc_geom *ref, const orc_geom *cpp, const float *vel, int *steps, int update,
+                       float *field) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    float *flow = (float *)malloc(3 * n * sizeof(float));
+    orc_vel_flow(ref, cpp, vel, flow);
+    int s = *steps;
+    scaled_flow(ref, flow, s, update, &s);
+    *steps = s;
+    const int sq = s < 0 ? -s : s;
+    memcpy(field, flow, 3 * n * sizeof(float));
+    for (int k = 0; k < sq; ++k) {
+        orc_def_field_compose(ref, field, flow);
+        memcpy(field, flow, 3 * n * sizeof(float));
+    }
+    free(flow);
+}
+
+/* reg_spline_getIntermediateDefFieldFromVelGrid without affine extensions
+ * (reg-lib/cpu/_reg_localTrans.cpp:3616-3683): fields[0] is the scaled flow, fields[k+1] the
+ * composition of fields[k] with itself; (|steps| + 1) planar [3][N] fields. */
+void orc_vel_intermediate(const orc_geom *ref, const orc_geom *cpp, const float *vel, int steps, float *fields) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    orc_vel_flow(ref, cpp, vel, fields);
+    int s;
+    scaled_flow(ref, fields, steps, 0, &s);
+    const int sq = steps < 0 ? -steps : steps;
+    for (int k = 0; k < sq; ++k) {
+        memcpy(fields + (size_t)(k + 1) * 3 * n, fields + (size_t)k * 3 * n, 3 * n * sizeof(float));
+        orc_def_field_compose(ref, fields + (size_t)k * 3 * n, fields + (size_t)(k + 1) * 3 * n);
+    }
+}
+
+/* reg_getRealImageSpacing (reg-lib/cpu/_reg_tools.cpp:253-274): column lengths of the sform from
+ * Mat44Mul<float> images of voxels 0 and e_k. */
+static void real_spacing(const float *xyz, float sp[3]) {
+    const float o[3] = { 0, 0, 0 };
+    float r1[3];
+    mat44_mul_f(xyz, o, r1);
+    for (int k = 0; k < 3; ++k) {
+        float e[3] = { 0, 0, 0 }, r2[3];
+        e[k] = 1;
+        mat44_mul_f(xyz, e, r2);
+        const float a = r1[0] - r2[0], b = r1[1] - r2[1], c = r1[2] - r2[2];
+        sp[k] = sqrtf(a * a + b * b + c * c);
+    }
+}
+
+/* reg_trilinearResampleGradient (reg-lib/cpu/_reg_resampling.cpp:1547-1766): the vector image
+ * `grad` (geometry g) sampled trilinearly at the positions `def` (same geometry), each sample
+ * multiplied by the field's forward-difference Jacobian (rotation of the sform removed, divided
+ * by the real spacing).  Padding for out-of-image taps. */
+void orc_resample_gradient(const orc_geom *g, const float *grad, const float *def, float pad, float *out) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    orc_m33 s;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s.m[i][j] = g->xyz[i * 4 + j];
+    const orc_m33 reo = m33_inverse(m33_polar(s));
+    float sp[3];
+    real_spacing(g->xyz, sp);
+    const float basis[2] = { 1, 0 }, deriv[2] = { -1, 1 };
+    size_t i = 0;
+    for (int z = 0; z < g->nz; ++z)
+        for (int y = 0; y < g->ny; ++y)
+            for (int x = 0; x < g->nx; ++x, ++i) {
+                const float d[3] = { def[i], def[i + n], def[i + 2 * n] };
+                float fc[3];
+                to_voxel(g->ijk, d, fc);
+                int lo[3], hi[3];
+                float bw[3][2];
+                for (int r = 0; r < 3; ++r) {
+                    lo[r] = floor_int(fc[r]);
+                    const int t = (int)fc[r];
+                    hi[r] = t + (fc[r] > (float)t);
+                    bw[r][1] = fabsf(fc[r] - (float)lo[r]);
+                    bw[r][0] = 1 - bw[r][1];
+                }
+                float vx = 0, vy = 0, vz = 0;
+                for (int c = 0; c < 2; ++c) {
+                    const int Z = c ? hi[2] : lo[2];
+                    if (Z > -1 && Z < g->nz) {
+                        for (int b = 0; b < 2; ++b) {
+                            const int Y = b ? hi[1] : lo[1];
+                            if (Y > -1 && Y < g->ny) {
+                                for (int a = 0; a < 2; ++a) {
+                                    const int X = a ? hi[0] : lo[0];
+                                    const float w = bw[0][a] * bw[1][b] * bw[2][c];
+                                    if (X > -1 && X < g->nx) {
+                                        const size_t idx = ((size_t)Z * g->ny + Y) * g->nx + X;
+                                        vx += grad[idx] * w;
+                                        vy += grad[idx + n] * w;
+                                        vz += grad[idx + 2 * n] * w;
+                                    } else {
+                                        vx += pad * w;
+                                        vy += pad * w;
+                                        vz += pad * w;
+                                    }
+                                }
+                            } else {
+                                vx += pad * bw[1][b] * bw[2][c];
+                                vy += pad * bw[1][b] * bw[2][c];
+                                vz += pad * bw[1][b] * bw[2][c];
+                            }
+                        }
+                    } else {
+                        vx += pad * bw[2][c];
+                        vy += pad * bw[2][c];
+                        vz += pad * bw[2][c];
+                    }
+                }
+                /* forward differences of the field; at the last index of an axis the pair steps
+                 * back by one (and the other two derivatives are then read one voxel back too) */
+                orc_m33 J;
+                memset(&J, 0, sizeof J);
+                for (int c = 0; c < 2; ++c) {
+                    int Z = z + c;
+                    float bz0 = basis[c], bz1 = deriv[c];
+                    if (z == g->nz - 1) {
+                        if (c == 1) Z -= 2;
+                        bz0 = fabsf(bz0 - 1);
+                        bz1 *= -1;
+                    }
+                    for (int b = 0; b < 2; ++b) {
+                        int Y = y + b;
+                        float by0 = basis[b], by1 = deriv[b];
+                        if (y == g->ny - 1) {
+                            if (b == 1) Y -= 2;
+                            by0 = fabsf(by0 - 1);
+                            by1 *= -1;
+                        }
+                        for (int a = 0; a < 2; ++a) {
+                            int X = x + a;
+                            float bx0 = basis[a], bx1 = deriv[a];
+                            if (x == g->nx - 1) {
+                                if (a == 1) X -= 2;
+                                bx0 = fabsf(bx0 - 1);
+                                bx1 *= -1;
+                            }
+                            const float w0 = bx1 * by0 * bz0, w1 = bx0 * by1 * bz0, w2 = bx0 * by0 * bz1;
+                            const size_t idx = ((size_t)Z * g->ny + Y) * g->nx + X;
+                            const float dx = def[idx], dy = def[idx + n], dz = def[idx + 2 * n];
+                            J.m[0][0] += w0 * dx;
+                            J.m[0][1] += w1 * dx;
+                            J.m[0][2] += w2 * dx;
+                            J.m[1][0] += w0 * dy;
+                            J.m[1][1] += w1 * dy;
+                            J.m[1][2] += w2 * dy;
+                            J.m[2][0] += w0 * dz;
+                            J.m[2][1] += w1 * dz;
+                            J.m[2][2] += w2 * dz;
+                        }
+                    }
+                }
+                J = m33_mul(reo, J);
+                for (int r = 0; r < 3; ++r)
+                    for (int k = 0; k < 3; ++k) J.m[r][k] /= sp[k];
+                out[i] = J.m[0][0] * vx + J.m[0][1] * vy + J.m[0][2] * vz;
+                out[i + n] = J.m[1][0] * vx + J.m[1][1] * vy + J.m[1][2] * vz;
+                out[i + 2 * n] = J.m[2][0] * vx + J.m[2][1] * vy + J.m[2][2] * vz;
+            }
+}
+
+/* Compute::ExponentiateGradient without affine components (reg-lib/Compute.cpp:283-326): the
+ * backward grid's intermediate fields at the forward reference geometry; for k < |steps|,
+ * grad <- resample(grad, fields[k]) + grad; finally grad / 2^|steps|. */
+void orc_exponentiate_gradient(const orc_geom *ref, const orc_geom *cppBw, const float *velBw, int steps,
+                               float *grad) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    const int sq = steps < 0 ? -steps : steps;
+    float *fields = (float *)malloc((size_t)(sq + 1) * 3 * n * sizeof(float));
+    float *tmp = (float *)malloc(3 * n * sizeof(float));
+    orc_vel_intermediate(ref, cppBw, velBw, steps, fields);
+    for (int k = 0; k < sq; ++k) {
+        orc_resample_gradient(ref, grad, fields + (size_t)k * 3 * n, 0.f, tmp);
+        for (size_t i = 0; i < 3 * n; ++i) grad[i] = (float)((double)tmp[i] + (double)grad[i]);
+    }
+    const double div = pow(2, sq);
+    for (size_t i = 0; i < 3 * n; ++i) grad[i] = (float)((double)grad[i] / div);
+    free(fields);
+    free(tmp);
+}
+
+/* Compute::UpdateVelocityField (reg-lib/Compute.cpp:336-348): scaled = gradient * scale (zeroed
+ * on the axes not optimised), grid = grid + scaled; reg_tools arithmetic in double. */
+void orc_update_velocity(float *grid, const float *grad, float scale, size_t nodes, int ox, int oy, int oz) {
+    const int on[3] = { ox, oy, oz };
+    for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < nodes; ++i) {
+            const float s = on[c] ? (float)((double)grad[i + c * nodes] * (double)scale) : 0.f;
+            grid[i + c * nodes] = (float)((double)grid[i + c * nodes] + (double)s);
+        }
+}
+
+/* Compute::SymmetriseVelocityFields (reg-lib/Compute.cpp:357-391): both grids (same geometry)
+ * to displacements, fw <- (fw - bw) / 2 and bw <- (bw - fw) / 2 from the values before the
+ * update, back to positions. */
+void orc_symmetrise_velocity(const orc_geom *cpp, float *fw, float *bw) {
+    const size_t g = (size_t)cpp->nx * cpp->ny * cpp->nz;
+    size_t i = 0;
+    for (int z = 0; z < cpp->nz; ++z)
+        for (int y = 0; y < cpp->ny; ++y)
+            for (int x = 0; x < cpp->nx; ++x, ++i) {
+                float p[3];
+                identity_pos(cpp->xyz, x, y, z, p);
+                for (int r = 0; r < 3; ++r) {
+                    const float f = fw[i + r * g] - p[r], b = bw[i + r * g] - p[r];
+                    const float nf = (float)((double)f - (double)b), nb = (float)((double)b - (double)f);
+                    fw[i + r * g] = (float)((double)nf * 0.5) + p[r];
+                    bw[i + r * g] = (float)((double)nb * 0.5) + p[r];
+                }
+            }
+}
+
+/* reg_createSymmetricControlPointGrids without an affine input (reg-lib/cpu/_reg_localTrans.cpp:
+ * 132-358): the grid covers both images' corners in mm, axis-aligned, sform only.  Writes the
+ * grid geometry; fw and bw (may be NULL) receive the identity positions. */
+void orc_symmetric_grids(const orc_geom *ref, const orc_geom *flo, const float spacing[3], orc_geom *grid,
+                         float *fw, float *bw) {
+    float mn[3], mx[3];
+    for (int c = 0; c < 8; ++c) {
+        const float rc[3] = { (c & 1) ? (float)ref->nx : 0.f, (c & 2) ? (float)ref->ny : 0.f,
+                              (c & 4) ? (float)ref->nz : 0.f };
+        const float fc[3] = { (c & 1) ? (float)flo->nx : 0.f, (c & 2) ? (float)flo->ny : 0.f,
+                              (c & 4) ? (float)flo->nz : 0.f };
+        float ro[3], fo[3];
+        mat44_mul_f(ref->xyz, rc, ro);
+        mat44_mul_f(flo->xyz, fc, fo);
+        for (int r = 0; r < 3; ++r) {
+            if (c == 0) {
+                mn[r] = ro[r] < fo[r] ? ro[r] : fo[r];
+                mx[r] = ro[r] < fo[r] ? fo[r] : ro[r];
+            } else {
+                if (ro[r] < mn[r]) mn[r] = ro[r];
+                if (fo[r] < mn[r]) mn[r] = fo[r];
+                if (mx[r] < ro[r]) mx[r] = ro[r];
+                if (mx[r] < fo[r]) mx[r] = fo[r];
+            }
+        }
+    }
+    int dims[3];
+    for (int r = 0; r < 3; ++r) {
+        const float v = (mx[r] - mn[r]) / spacing[r] + 3.f;
+        const int t = (int)v;
+        dims[r] = t + (v > (float)t);
+    }
+    memset(grid, 0, sizeof *grid);
+    grid->nx = dims[0];
+    grid->ny = dims[1];
+    grid->nz = dims[2];
+    grid->dx = spacing[0];
+    grid->dy = spacing[1];
+    grid->dz = spacing[2];
+    for (int r = 0; r < 3; ++r) {
+        M(grid->xyz, r, r) = spacing[r];
+        M(grid->xyz, r, 3) = mn[r] - spacing[r];
+    }
+    M(grid->xyz, 3, 3) = 1.f;
+    orc_nifti_mat44_inverse(grid->xyz, grid->ijk);
+    const size_t g = (size_t)dims[0] * dims[1] * dims[2];
+    size_t i = 0;
+    for (int z = 0; z < dims[2]; ++z)
+        for (int y = 0; y < dims[1]; ++y)
+            for (int x = 0; x < dims[0]; ++x, ++i) {
+                float p[3];
+                identity_pos(grid->xyz, x, y, z, p);
+                for (int r = 0; r < 3; ++r) {
+                    if (fw) fw[i + r * g] = p[r];
+                    if (bw) bw[i + r * g] = p[r];
+                }
+            }
+}

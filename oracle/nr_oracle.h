@@ -143,6 +143,32 @@ int orc_mat44_avg2(const float A[16], const float B[16], float out[16]);
 /* the matrix update of reg_aladin_sym::UpdateTransformationMatrix (reg-lib/_reg_aladin_sym.cpp:128-157) */
 int orc_symmetric_update(float fw[16], float bw[16]);
 
+/* ---- velocity fields (F3D2) --------------------------------------------------------------- */
+/* Grids without affine extensions; fields and grids hold mm positions, planar [3][n]. */
+/* reg_spline_getFlowFieldFromVelocityGrid (reg-lib/cpu/_reg_localTrans.cpp:3465-3491) */
+void orc_vel_flow(const orc_geom *ref, const orc_geom *cpp, const float *vel, float *flow);
+/* reg_defField_compose3D, no mask (reg-lib/cpu/_reg_localTrans.cpp:2333-2457); upd in place */
+void orc_def_field_compose(const orc_geom *g, const float *def, float *upd);
+/* reg_spline_getDefFieldFromVelocityGrid (reg-lib/cpu/_reg_localTrans.cpp:3583-3613); *steps is
+ * the grid's intent_p2 (updated when `update` is set or it is 0) */
+void orc_vel_def_field(const orc_geom *ref, const orc_geom *cpp, const float *vel, int *steps, int update,
+                       float *field);
+/* reg_spline_getIntermediateDefFieldFromVelGrid (reg-lib/cpu/_reg_localTrans.cpp:3616-3683);
+ * (|steps| + 1) fields */
+void orc_vel_intermediate(const orc_geom *ref, const orc_geom *cpp, const float *vel, int steps, float *fields);
+/* reg_trilinearResampleGradient (reg-lib/cpu/_reg_resampling.cpp:1547-1766) */
+void orc_resample_gradient(const orc_geom *g, const float *grad, const float *def, float pad, float *out);
+/* Compute::ExponentiateGradient (reg-lib/Compute.cpp:283-326); grad in place */
+void orc_exponentiate_gradient(const orc_geom *ref, const orc_geom *cppBw, const float *velBw, int steps,
+                               float *grad);
+/* Compute::UpdateVelocityField (reg-lib/Compute.cpp:336-348) */
+void orc_update_velocity(float *grid, const float *grad, float scale, size_t nodes, int ox, int oy, int oz);
+/* Compute::SymmetriseVelocityFields (reg-lib/Compute.cpp:357-391) */
+void orc_symmetrise_velocity(const orc_geom *cpp, float *fw, float *bw);
+/* reg_createSymmetricControlPointGrids, no affine (reg-lib/cpu/_reg_localTrans.cpp:132-358) */
+void orc_symmetric_grids(const orc_geom *ref, const orc_geom *flo, const float spacing[3], orc_geom *grid,
+                         float *fw, float *bw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,8 @@
 #include "_reg_aladin.h"
 #include "_reg_aladin_sym.h"
 #include "F3dContent.h"
+#include "Compute.h"
+#include "_reg_f3d2.h"
 #include "_reg_ReadWriteMatrix.h"
 #include <cstring>
 #include <vector>
@@ -297,6 +299,114 @@ int ref_downsample(const nr_geom *g, const float *data, nr_geom *out, float *out
         reg_downsampleImage<float>(img, 1, axes);
         geom_of(img, out);
         if (outData) std::memcpy(outData, img->data, img.nVoxels() * sizeof(float));
+    });
+}
+
+
+// ---------------------------------------------------------------- velocity fields (F3D2)
+static NiftiImage make_velocity_grid(const nr_geom *cpp, const float *data, int steps) {
+    NiftiImage grid = make_grid(cpp, data);
+    grid->intent_p1 = SPLINE_VEL_GRID;
+    grid->intent_p2 = (float)steps;
+    return grid;
+}
+
+// reg_spline_getDefFieldFromVelocityGrid (cpu/_reg_localTrans.cpp:3583-3613); *steps: intent_p2
+int ref_vel_def_field(const nr_geom *ref, const nr_geom *cpp, const float *vel, int *steps, int update,
+                      float *field) {
+    return guarded([&] {
+        NiftiImage grid = make_velocity_grid(cpp, vel, *steps);
+        NiftiImage def = make_vector_image(ref, nullptr);
+        reg_spline_getDefFieldFromVelocityGrid(grid, def, update != 0);
+        *steps = (int)grid->intent_p2;
+        std::memcpy(field, def->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// reg_spline_getIntermediateDefFieldFromVelGrid (cpu/_reg_localTrans.cpp:3616-3683)
+int ref_vel_intermediate(const nr_geom *ref, const nr_geom *cpp, const float *vel, int steps, float *fields) {
+    return guarded([&] {
+        NiftiImage grid = make_velocity_grid(cpp, vel, steps);
+        const int sq = steps < 0 ? -steps : steps;
+        std::vector<NiftiImage> defs(sq + 1);
+        for (auto& d : defs) d = make_vector_image(ref, nullptr);
+        reg_spline_getIntermediateDefFieldFromVelGrid(grid, defs.data());
+        for (int k = 0; k <= sq; ++k)
+            std::memcpy(fields + (size_t)k * 3 * nvox(ref), defs[k]->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// reg_defField_compose (cpu/_reg_localTrans.cpp:2460-2476), no mask; upd in place
+int ref_def_field_compose(const nr_geom *g, const float *def, float *upd) {
+    return guarded([&] {
+        NiftiImage d = make_vector_image(g, def);
+        NiftiImage u = make_vector_image(g, upd);
+        reg_defField_compose(d, u, nullptr);
+        std::memcpy(upd, u->data, 3 * nvox(g) * sizeof(float));
+    });
+}
+
+// reg_resampleGradient (cpu/_reg_resampling.cpp:1769-1797), linear
+int ref_resample_gradient(const nr_geom *g, const float *grad, const float *def, float pad, float *out) {
+    return guarded([&] {
+        NiftiImage gi = make_vector_image(g, grad);
+        NiftiImage di = make_vector_image(g, def);
+        NiftiImage oi = make_vector_image(g, nullptr);
+        reg_resampleGradient(gi, oi, di, 1, pad);
+        std::memcpy(out, oi->data, 3 * nvox(g) * sizeof(float));
+    });
+}
+
+// Compute::ExponentiateGradient (Compute.cpp:283-326) on two F3dContent objects: the forward
+// one (reference -> floating) holding `grad` as its voxel-based gradient, the backward one
+// (floating -> reference) holding the backward velocity grid.
+int ref_exponentiate_gradient(const nr_geom *ref, const nr_geom *flo, const nr_geom *cpp, const float *velFw,
+                              const float *velBw, int steps, float *grad) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, nullptr), floImg = make_image(flo, nullptr);
+        NiftiImage gridFw = make_velocity_grid(cpp, velFw, steps), gridBw = make_velocity_grid(cpp, velBw, steps);
+        F3dContent con(refImg, floImg, gridFw), conBw(floImg, refImg, gridBw);
+        NiftiImage& vg = con.GetVoxelBasedMeasureGradient();
+        std::memcpy(vg->data, grad, 3 * nvox(ref) * sizeof(float));
+        Compute(con).ExponentiateGradient(conBw);
+        std::memcpy(grad, vg->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// Compute::UpdateVelocityField (Compute.cpp:336-348) and Compute::SymmetriseVelocityFields
+// (Compute.cpp:357-391) on a forward / backward content pair.  mode 0: update the forward grid
+// with `grad` (scale, axes); mode 1: symmetrise both grids.
+int ref_velocity_update(const nr_geom *ref, const nr_geom *flo, const nr_geom *cpp, float *velFw, float *velBw,
+                        const float *grad, float scale, const int axes[3], int mode) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, nullptr), floImg = make_image(flo, nullptr);
+        NiftiImage gridFw = make_velocity_grid(cpp, velFw, 6), gridBw = make_velocity_grid(cpp, velBw, 6);
+        F3dContent con(refImg, floImg, gridFw), conBw(floImg, refImg, gridBw);
+        Compute compute(con);
+        if (mode == 0) {
+            NiftiImage& tg = con.GetTransformationGradient();
+            std::memcpy(tg->data, grad, 3 * nvox(cpp) * sizeof(float));
+            compute.UpdateVelocityField(scale, axes[0] != 0, axes[1] != 0, axes[2] != 0);
+        } else {
+            compute.SymmetriseVelocityFields(conBw);
+        }
+        std::memcpy(velFw, con.GetControlPointGrid()->data, 3 * nvox(cpp) * sizeof(float));
+        std::memcpy(velBw, conBw.GetControlPointGrid()->data, 3 * nvox(cpp) * sizeof(float));
+    });
+}
+
+// reg_createSymmetricControlPointGrids<float> without an affine (cpu/_reg_localTrans.cpp:132-358)
+int ref_symmetric_grids(const nr_geom *ref, const nr_geom *flo, const float spacing[3], nr_geom *out, float *fw,
+                        float *bw, int capacity) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, nullptr), floImg = make_image(flo, nullptr);
+        NiftiImage gFw, gBw;
+        reg_createSymmetricControlPointGrids<float>(gFw, gBw, refImg, floImg, nullptr, spacing);
+        geom_of(gFw, out);
+        if ((int)gFw.nVoxels() <= capacity) {
+            if (fw) std::memcpy(fw, gFw->data, gFw.nVoxels() * sizeof(float));
+            if (bw) std::memcpy(bw, gBw->data, gBw.nVoxels() * sizeof(float));
+        }
     });
 }
 

@@ -650,3 +650,151 @@ def ref_affine_read(path):
     m = np.zeros(16, np.float32)
     rcheck(ref().ref_affine_read(str(path).encode(), fp(m)))
     return m.reshape(4, 4)
+
+
+# ---------------------------------------------------------------- velocity fields (F3D2)
+REF_SCALAR = os.path.join(ROOT, "oracle", "_ref", "libnrref_nosse.so")
+_ref_scalar = None
+
+
+def ref_scalar():
+    """The reference built without USE_SSE: its summation order is the one the oracle follows."""
+    global _ref_scalar
+    if _ref_scalar is None:
+        _ref_scalar = ctypes.CDLL(REF_SCALAR)
+        _ref_scalar.ref_last_error.restype = ctypes.c_char_p
+    return _ref_scalar
+
+
+def _scheck(lib, rc):
+    if rc != 0:
+        raise RuntimeError(lib.ref_last_error().decode())
+
+
+def orc_vel_flow(refg, cppg, vel):
+    flow = np.zeros(3 * refg.nvox, np.float32)
+    orc().orc_vel_flow(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vel), fp(flow))
+    return flow
+
+
+def orc_vel_def_field(refg, cppg, vel, steps, update=False):
+    field = np.zeros(3 * refg.nvox, np.float32)
+    s = ctypes.c_int(int(steps))
+    orc().orc_vel_def_field(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vel), ctypes.byref(s),
+                            int(bool(update)), fp(field))
+    return field, s.value
+
+
+def ref_vel_def_field(refg, cppg, vel, steps, update=False, scalar=True):
+    lib = ref_scalar() if scalar else ref()
+    field = np.zeros(3 * refg.nvox, np.float32)
+    s = ctypes.c_int(int(steps))
+    _scheck(lib, lib.ref_vel_def_field(ctypes.byref(rg(refg)), ctypes.byref(rg(cppg)), fp(vel), ctypes.byref(s),
+                                       int(bool(update)), fp(field)))
+    return field, s.value
+
+
+def orc_vel_intermediate(refg, cppg, vel, steps):
+    out = np.zeros((abs(steps) + 1) * 3 * refg.nvox, np.float32)
+    orc().orc_vel_intermediate(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vel), int(steps), fp(out))
+    return out.reshape(abs(steps) + 1, 3 * refg.nvox)
+
+
+def ref_vel_intermediate(refg, cppg, vel, steps):
+    lib = ref_scalar()
+    out = np.zeros((abs(steps) + 1) * 3 * refg.nvox, np.float32)
+    _scheck(lib, lib.ref_vel_intermediate(ctypes.byref(rg(refg)), ctypes.byref(rg(cppg)), fp(vel), int(steps),
+                                          fp(out)))
+    return out.reshape(abs(steps) + 1, 3 * refg.nvox)
+
+
+def orc_def_compose(g, d, upd):
+    u = np.ascontiguousarray(upd, np.float32).copy()
+    orc().orc_def_field_compose(ctypes.byref(og(g)), fp(np.ascontiguousarray(d, np.float32)), fp(u))
+    return u
+
+
+def ref_def_compose(g, d, upd):
+    lib = ref_scalar()
+    u = np.ascontiguousarray(upd, np.float32).copy()
+    _scheck(lib, lib.ref_def_field_compose(ctypes.byref(rg(g)), fp(np.ascontiguousarray(d, np.float32)), fp(u)))
+    return u
+
+
+def orc_resample_gradient(g, grad, d, pad=0.0):
+    out = np.zeros(3 * g.nvox, np.float32)
+    orc().orc_resample_gradient(ctypes.byref(og(g)), fp(grad), fp(d), ctypes.c_float(pad), fp(out))
+    return out
+
+
+def ref_resample_gradient(g, grad, d, pad=0.0):
+    lib = ref_scalar()
+    out = np.zeros(3 * g.nvox, np.float32)
+    _scheck(lib, lib.ref_resample_gradient(ctypes.byref(rg(g)), fp(grad), fp(d), ctypes.c_float(pad), fp(out)))
+    return out
+
+
+def orc_exponentiate_gradient(refg, cppg, vel_bw, steps, grad):
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    orc().orc_exponentiate_gradient(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vel_bw), int(steps), fp(g))
+    return g
+
+
+def ref_exponentiate_gradient(refg, flog, cppg, vel_fw, vel_bw, steps, grad):
+    lib = ref_scalar()
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    _scheck(lib, lib.ref_exponentiate_gradient(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)), ctypes.byref(rg(cppg)),
+                                               fp(vel_fw), fp(vel_bw), int(steps), fp(g)))
+    return g
+
+
+def orc_update_velocity(grid, grad, scale, axes=(1, 1, 1)):
+    g = np.ascontiguousarray(grid, np.float32).copy()
+    orc().orc_update_velocity(fp(g), fp(grad), ctypes.c_float(scale), ctypes.c_size_t(g.size // 3), *map(int, axes))
+    return g
+
+
+def orc_symmetrise_velocity(cppg, fw, bw):
+    f = np.ascontiguousarray(fw, np.float32).copy()
+    b = np.ascontiguousarray(bw, np.float32).copy()
+    orc().orc_symmetrise_velocity(ctypes.byref(og(cppg)), fp(f), fp(b))
+    return f, b
+
+
+def ref_velocity_update(refg, flog, cppg, fw, bw, grad=None, scale=0.0, axes=(1, 1, 1), mode=0):
+    lib = ref_scalar()
+    f = np.ascontiguousarray(fw, np.float32).copy()
+    b = np.ascontiguousarray(bw, np.float32).copy()
+    gr = np.zeros_like(f) if grad is None else np.ascontiguousarray(grad, np.float32)
+    ax = (ctypes.c_int * 3)(*map(int, axes))
+    _scheck(lib, lib.ref_velocity_update(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)), ctypes.byref(rg(cppg)),
+                                         fp(f), fp(b), fp(gr), ctypes.c_float(scale), ax, int(mode)))
+    return f, b
+
+
+def orc_symmetric_grids(refg, flog, spacing):
+    from niftyreg_amd.geometry import Geom
+    sp = np.asarray(spacing, np.float32)
+    o = orc_geom()
+    orc().orc_symmetric_grids(ctypes.byref(og(refg)), ctypes.byref(og(flog)), fp(sp), ctypes.byref(o), None, None)
+    G = o.nx * o.ny * o.nz
+    fw = np.zeros(3 * G, np.float32)
+    bw = np.zeros(3 * G, np.float32)
+    orc().orc_symmetric_grids(ctypes.byref(og(refg)), ctypes.byref(og(flog)), fp(sp), ctypes.byref(o), fp(fw), fp(bw))
+    g = Geom((o.nx, o.ny, o.nz), np.array(o.xyz[:], np.float32).reshape(4, 4),
+             np.array(o.ijk[:], np.float32).reshape(4, 4), (o.dx, o.dy, o.dz))
+    return g, fw, bw
+
+
+def ref_symmetric_grids(refg, flog, spacing):
+    lib = ref_scalar()
+    sp = np.asarray(spacing, np.float32)
+    o = nr_geom()
+    _scheck(lib, lib.ref_symmetric_grids(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)), fp(sp), ctypes.byref(o),
+                                         None, None, 0))
+    G = o.nx * o.ny * o.nz
+    fw = np.zeros(3 * G, np.float32)
+    bw = np.zeros(3 * G, np.float32)
+    _scheck(lib, lib.ref_symmetric_grids(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)), fp(sp), ctypes.byref(o),
+                                         fp(fw), fp(bw), 3 * G))
+    return o, fw, bw
