@@ -181,6 +181,17 @@ int nrh_f3d_resample_gradient(nrh_f3d *h, int interpolation, float padding, floa
  * F3dContent::GetVoxelBasedMeasureGradient written by the caller). */
 int nrh_f3d_upload_field(nrh_f3d *h, const float *planar);
 int nrh_f3d_upload_voxel_gradient(nrh_f3d *h, const float *planar);
+/* reg_f3d2::GetObjectiveFunctionValue (reg-lib/_reg_f3d2.cpp:540-560) over the forward handle fw
+ * (reference -> floating) and the backward handle bw (floating -> reference, bin counts
+ * swapped), NMI: both exponentiations (the forward step number updated, the backward one
+ * copied), both warps, symmetric NMI, bending / linear energy of both grids.  terms: wMeasure,
+ * wBE, wLE (may be NULL).  The two directions run on the two handles' streams concurrently. */
+int nrh_f3d2_objective(nrh_f3d *fw, nrh_f3d *bw, double *value, double terms[3]);
+/* reg_f3d2::GetObjectiveFunctionGradient (reg-lib/_reg_f3d2.cpp:417-440): voxel-based NMI
+ * gradients of both directions, each exponentiated along the other's grid, convolved and
+ * projected; penalty gradients of both grids.  With max_length non-NULL, reg_f3d2::
+ * NormaliseGradient (:373-395): both divided by the larger maximal length, which is returned. */
+int nrh_f3d2_gradient(nrh_f3d *fw, nrh_f3d *bw, double *max_length);
 
 /* Fused objective: reg_f3d::GetObjectiveFunctionValue (reg-lib/_reg_f3d.cpp:443-463) in one
  * device sweep (field, warp and joint histogram fused; no field or warped image is written).

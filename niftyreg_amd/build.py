@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnrhip.so")
 
-SOURCES = ["capi.cpp", "capi_eval.cpp", "capi_velocity.cpp", "f3d_ops.hip", "f3d_fused.hip", "f3d_velocity.hip",
+SOURCES = ["capi.cpp", "capi_eval.cpp", "capi_velocity.cpp", "capi_f3d2.cpp", "f3d_ops.hip", "f3d_fused.hip", "f3d_velocity.hip",
            "driver.cpp", "aladin.hip", "lncc.hip", "nifti_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

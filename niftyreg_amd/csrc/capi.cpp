@@ -220,6 +220,7 @@ int nrh_f3d_destroy(nrh_f3d *h) {
         fused_release(h);
         for (int i = 0; i < 4; ++i)
             if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+        if (h->evPair) (void)hipEventDestroy(h->evPair);
         hipStream_t s = h->stream;
         delete h;
         if (s) (void)hipStreamDestroy(s);

@@ -1,0 +1,154 @@
+// Symmetric diffeomorphic F3D (reg_f3d2) objective and gradient over two handles: the forward
+// one (reference -> floating, forward velocity grid) and the backward one (floating ->
+// reference, backward velocity grid, bin counts swapped).  Each handle's chain runs on its own
+// stream, so the two directions overlap on the device; they meet where the reference couples
+// them: the backward grid takes the forward step count, each gradient is exponentiated along
+// the other direction's grid (both before either bending-energy gradient rounds its grid
+// through displacements), and the two gradients share one normalisation length.
+
+#include "f3d_handle.hpp"
+#include <algorithm>
+
+namespace nrh {
+
+namespace {
+
+void check_pair(nrh_f3d *fw, nrh_f3d *bw) {
+    NRH_REQUIRE(fw && bw && fw != bw, NRH_ERR_ARG, "two distinct handles are required");
+    NRH_REQUIRE(fw->velocity && bw->velocity, NRH_ERR_STATE, "both control point grids must be velocity grids");
+    const nrh_f3d_params &p = fw->params, &q = bw->params;
+    NRH_REQUIRE(p.measure == NRH_MEASURE_NMI && q.measure == NRH_MEASURE_NMI, NRH_ERR_UNSUPPORTED,
+                "the symmetric evaluation supports NMI");
+    NRH_REQUIRE(p.similarity_weight == q.similarity_weight && p.be_weight == q.be_weight && p.le_weight == q.le_weight,
+                NRH_ERR_ARG, "forward and backward handles must share the objective weights");
+    NRH_REQUIRE(p.ref_bins == q.flo_bins && p.flo_bins == q.ref_bins, NRH_ERR_ARG,
+                "the backward handle's bin counts must be the forward ones swapped");
+}
+
+hipEvent_t pair_event(nrh_f3d *h) {
+    if (!h->evPair) NRH_HIP(hipEventCreateWithFlags(&h->evPair, hipEventDisableTiming));
+    return h->evPair;
+}
+
+// Each stream waits for the other's work issued so far.
+void join(nrh_f3d *a, nrh_f3d *b) {
+    NRH_HIP(hipEventRecord(pair_event(a), a->stream));
+    NRH_HIP(hipEventRecord(pair_event(b), b->stream));
+    NRH_HIP(hipStreamWaitEvent(a->stream, b->evPair, 0));
+    NRH_HIP(hipStreamWaitEvent(b->stream, a->evPair, 0));
+}
+
+// reg_f3d2::GetDeformationField + WarpFloatingImage (reg-lib/_reg_f3d2.cpp:139-172): forward
+// exponentiation (step number updated: the optimiser exists), the backward one with the forward
+// step count, both resamplings.
+void warp_both(nrh_f3d *fw, nrh_f3d *bw) {
+    op_vel_def_field(fw, true);
+    bw->velSteps = fw->velSteps;
+    op_vel_def_field(bw, false);
+    op_resample(fw);
+    op_resample(bw);
+}
+
+// reg_nmi::GetSimilarityMeasureValue with isSymmetric (reg-lib/cpu/_reg_measure.h:63-80):
+// the forward and the backward joint histograms and entropies.
+void histograms_both(nrh_f3d *fw, nrh_f3d *bw) {
+    for (nrh_f3d *h : { fw, bw }) {
+        op_joint_histogram(h, h->warped.p);
+        op_nmi_finalize(h);
+        h->histVersion = h->cppVersion;
+    }
+}
+
+}  // namespace
+
+}  // namespace nrh
+
+using namespace nrh;
+
+extern "C" {
+
+// reg_f3d2::GetObjectiveFunctionValue (reg-lib/_reg_f3d2.cpp:540-560) with NMI, bending and
+// linear energy: wMeasure = w_sim (NMI_fw + NMI_bw), wBE = w_be BE_fw + w_be BE_bw, likewise wLE.
+int nrh_f3d2_objective(nrh_f3d *fw, nrh_f3d *bw, double *value, double terms[3]) {
+    return guard([&] {
+        check_pair(fw, bw);
+        NRH_REQUIRE(value, NRH_ERR_ARG, "null argument");
+        const nrh_f3d_params& p = fw->params;
+        if (p.similarity_weight > 0) {
+            warp_both(fw, bw);
+            histograms_both(fw, bw);
+        }
+        for (nrh_f3d *h : { fw, bw }) {
+            if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
+            if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
+            op_publish(h, h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
+        }
+        NRH_HIP(hipStreamSynchronize(fw->stream));
+        NRH_HIP(hipStreamSynchronize(bw->stream));
+        const double *f = fw->published, *b = bw->published;
+        const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * (f[4] + b[4]) : 0.0;
+        const double wBE = p.be_weight > 0 ? (double)p.be_weight * f[5] + (double)p.be_weight * b[5] : 0.0;
+        const double wLE = p.le_weight > 0 ? (double)p.le_weight * f[6] + (double)p.le_weight * b[6] : 0.0;
+        *value = wMeasure - wBE - wLE;
+        if (terms) {
+            terms[0] = wMeasure;
+            terms[1] = wBE;
+            terms[2] = wLE;
+        }
+    });
+}
+
+// reg_f3d2::GetObjectiveFunctionGradient (reg-lib/_reg_f3d2.cpp:417-440) and, when max_length
+// is non-NULL, reg_f3d2::NormaliseGradient (:373-395): both node gradients divided by the larger
+// of the two maximal lengths.
+int nrh_f3d2_gradient(nrh_f3d *fw, nrh_f3d *bw, double *max_length) {
+    return guard([&] {
+        check_pair(fw, bw);
+        const nrh_f3d_params& p = fw->params;
+        if (p.similarity_weight > 0) {
+            // reg_f3d2::GetVoxelBasedGradient (reg-lib/_reg_f3d2.cpp:246-301)
+            warp_both(fw, bw);
+            for (nrh_f3d *h : { fw, bw }) {
+                ensure_voxel_grad(h);
+                op_fill_f32(h->voxelGrad.p, 3 * h->N, 0.f, h->stream);
+                op_image_gradient(h);
+            }
+            histograms_both(fw, bw);
+            op_nmi_voxel_gradient(fw, 1.0);
+            op_nmi_voxel_gradient(bw, 1.0);
+            // ExponentiateGradient (reg-lib/_reg_f3d2.cpp:597-610): each direction reads the
+            // other's grid, which no stream changes before the join below
+            op_exponentiate_gradient(fw, bw, false);
+            op_exponentiate_gradient(bw, fw, false);
+            // reg_f3d2::GetSimilarityMeasureGradient (:333-342)
+            for (nrh_f3d *h : { fw, bw }) {
+                op_convolve_voxel_gradient(h);
+                op_voxel_to_node(h, p.similarity_weight);
+            }
+        } else {
+            op_fill_f32(fw->nodeGrad.p, 3 * fw->G, 0.f, fw->stream);
+            op_fill_f32(bw->nodeGrad.p, 3 * bw->G, 0.f, bw->stream);
+        }
+        // the bending-energy gradient rounds each grid through displacements in place: only after
+        // both exponentiations have read the grids
+        join(fw, bw);
+        for (nrh_f3d *h : { fw, bw }) {
+            if (p.be_weight > 0) op_bending_energy_gradient(h, p.be_weight);
+            if (p.le_weight > 0) op_linear_energy_gradient(h, p.le_weight);
+        }
+        if (max_length) {
+            op_max_length(fw);
+            op_max_length(bw);
+            const float lf = (float)read_scalar(fw, fw->scalars.p + 1);
+            const float lb = (float)read_scalar(bw, bw->scalars.p + 1);
+            const float m = std::max(lb, lf);
+            op_normalise(fw, m);
+            op_normalise(bw, m);
+            *max_length = m;
+        }
+        NRH_HIP(hipStreamSynchronize(fw->stream));
+        NRH_HIP(hipStreamSynchronize(bw->stream));
+    });
+}
+
+}  // extern "C"

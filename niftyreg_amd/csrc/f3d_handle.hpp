@@ -52,6 +52,7 @@ struct nrh_f3d {
     nrh::DevBuf<float> velTmp;      // flow / ping-pong field [3][N]
     nrh::DevBuf<float> velFields;   // intermediate fields of the gradient exponentiation
     nrh::DevBuf<uint32_t> velKeys;  // min / max keys of the flow displacement
+    hipEvent_t evPair = nullptr;    // cross-stream ordering of a forward / backward pair
 
     double *pinned = nullptr;  // host-pinned scalar read-back
     double *published = nullptr, *publishedDev = nullptr;  // coherent host memory the device writes results to
@@ -118,7 +119,8 @@ void op_normalise_from_bits(nrh_f3d *h);  // normalise by maxBits[0]; scalars[1]
 
 // velocity grids (f3d_velocity.hip)
 void op_vel_def_field(nrh_f3d *h, bool update);
-void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw);
+// syncOther: wait for bw's stream first (its grid may have been updated there)
+void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther = true);
 void op_resample_gradient(nrh_f3d *h, float pad, float *out);
 void op_compose_field(nrh_f3d *h, const nrh_geom& g, const float *def);
 void op_update_velocity(nrh_f3d *h, float scale, bool ox, bool oy, bool oz);

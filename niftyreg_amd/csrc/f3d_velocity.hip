@@ -547,14 +547,14 @@ static void intermediate_fields(const VelGrid& g, const FieldGeom& f, int steps,
 // Compute::ExponentiateGradient (reg-lib/Compute.cpp:283-326) without affine components: the
 // voxel-based gradient of h, resampled along the intermediate fields of bw's velocity grid
 // (evaluated on h's reference geometry) and accumulated, then divided by 2^|steps of bw|.
-void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw) {
+void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther) {
     NRH_REQUIRE(bw->velocity, NRH_ERR_STATE, "the backward control points are not a velocity grid");
     NRH_REQUIRE(!h->voxelGrad.empty(), NRH_ERR_STATE, "voxel-based gradient not computed");
     const int steps = bw->velSteps;
     const int sq = steps < 0 ? -steps : steps;
     if (sq == 0) return;
     // the backward grid's last update ran on its own stream
-    NRH_HIP(hipStreamSynchronize(bw->stream));
+    if (syncOther) NRH_HIP(hipStreamSynchronize(bw->stream));
     const FieldGeom f = field_geom(h->ref);
     h->velFields.alloc((size_t)sq * 3 * f.n);
     h->velTmp.alloc(3 * f.n);

@@ -310,3 +310,30 @@ def register(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.0
               ctypes.byref(out), _lib.fptr(cpp), ctypes.c_size_t(cap), ctypes.byref(nmi), ctypes.byref(its))
     g = Geom.from_c(out)
     return g, cpp[:3 * g.nvox].copy(), nmi.value, its.value
+
+
+class F3d2Pair:
+    """reg_f3d2's forward / backward pair (reg-lib/_reg_f3d2.h): two F3dContent handles whose
+    control points are velocity grids of one symmetric grid geometry, the backward handle with
+    the images (and bin counts) swapped.  objective() / gradient() run the symmetric evaluation
+    with the two directions on the two handles' streams."""
+
+    def __init__(self, forward: F3dContent, backward: F3dContent, steps=6):
+        self.fw, self.bw = forward, backward
+        self.fw.SetVelocityGrid(steps)
+        self.bw.SetVelocityGrid(steps)
+
+    def objective(self):
+        v = ctypes.c_double()
+        t = (ctypes.c_double * 3)()
+        _lib.call("nrh_f3d2_objective", self.fw.h, self.bw.h, ctypes.byref(v), t)
+        return v.value, np.array(t[:])
+
+    def gradient(self, normalise=False):
+        m = ctypes.c_double()
+        _lib.call("nrh_f3d2_gradient", self.fw.h, self.bw.h, ctypes.byref(m) if normalise else None)
+        return m.value if normalise else None
+
+    def close(self):
+        self.fw.close()
+        self.bw.close()

@@ -461,7 +461,7 @@ typedef struct {
     float lncc_sigma;    // LNCC kernel standard deviation (< 0: voxels)
 } ref_f3d_params;
 
-static void configure(RefF3d& r, const ref_f3d_params *p) {
+static void configure(reg_f3d<float>& r, const ref_f3d_params *p) {
     r.SetPlatformType(PlatformType::Cpu);
     if (p->measure == 1) {
         r.UseLNCC(0, p->lncc_sigma);
@@ -586,6 +586,134 @@ int ref_f3d_run(const nr_geom *ref, const float *refData, const nr_geom *flo, co
             std::memcpy(cppData, g->data, g.nVoxels() * sizeof(float));
     });
 }
+
+} // extern "C"
+
+// ---------------------------------------------------------------- symmetric F3D through reg_f3d2
+namespace {
+class RefF3d2: public reg_f3d2<float> {
+public:
+    RefF3d2(): reg_f3d2<float>(1, 1) {}
+    void Prepare(int level) {
+        this->Initialise();
+        this->InitCurrentLevel(level);
+        this->InitialiseSimilarity();
+        this->SetOptimiser();
+    }
+    double Value() { return this->GetObjectiveFunctionValue(); }
+    void Gradient() { this->GetObjectiveFunctionGradient(); }
+    double NormGrad() { return this->NormaliseGradient(); }
+    F3dContent& Con() { return dynamic_cast<F3dContent&>(*this->con); }
+    F3dContent& ConBw() { return *this->conBw; }
+    double WMeasure() { return this->currentWMeasure; }
+    double WBE() { return this->currentWBE; }
+    double WLE() { return this->currentWLE; }
+    float SimWeight() { return this->similarityWeight; }
+    float BEWeight() { return this->bendingEnergyWeight; }
+    float LEWeight() { return this->linearEnergyWeight; }
+};
+
+struct F3d2Handle {
+    std::unique_ptr<RefF3d2> reg;
+};
+} // namespace
+
+extern "C" {
+
+// reg_f3d2<float> prepared like reg_f3d (Initialise, InitCurrentLevel, InitialiseSimilarity,
+// SetOptimiser) so that single symmetric objective / gradient evaluations can be timed and
+// compared.
+void* ref_f3d2_create(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                      const ref_f3d_params *p, int level) {
+    F3d2Handle *h = new F3d2Handle;
+    int rc = guarded([&] {
+        h->reg.reset(new RefF3d2);
+        configure(*h->reg, p);
+        h->reg->SetReferenceImage(make_image(ref, refData));
+        h->reg->SetFloatingImage(make_image(flo, floData));
+        h->reg->Prepare(level);
+    });
+    if (rc) { delete h; return nullptr; }
+    return h;
+}
+
+void ref_f3d2_destroy(void *hp) { delete static_cast<F3d2Handle*>(hp); }
+
+int ref_f3d2_level_geom(void *hp, nr_geom *cpp, nr_geom *ref, nr_geom *flo) {
+    F3d2Handle *h = static_cast<F3d2Handle*>(hp);
+    return guarded([&] {
+        F3dContent& c = h->reg->Con();
+        geom_of(c.F3dContent::GetControlPointGrid(), cpp);
+        geom_of(c.Content::GetReference(), ref);
+        geom_of(c.Content::GetFloating(), flo);
+    });
+}
+
+int ref_f3d2_level_images(void *hp, float *refData, float *floData) {
+    F3d2Handle *h = static_cast<F3d2Handle*>(hp);
+    return guarded([&] {
+        F3dContent& c = h->reg->Con();
+        NiftiImage& r = c.Content::GetReference();
+        NiftiImage& f = c.Content::GetFloating();
+        if (refData) std::memcpy(refData, r->data, r.nVoxelsPerVolume() * sizeof(float));
+        if (floData) std::memcpy(floData, f->data, f.nVoxelsPerVolume() * sizeof(float));
+    });
+}
+
+// Forward / backward grids (positions) and their intent_p2 step numbers.
+int ref_f3d2_set_cpp(void *hp, const float *fw, const float *bw) {
+    F3d2Handle *h = static_cast<F3d2Handle*>(hp);
+    return guarded([&] {
+        NiftiImage& g = h->reg->Con().F3dContent::GetControlPointGrid();
+        NiftiImage& gb = h->reg->ConBw().F3dContent::GetControlPointGrid();
+        if (fw) std::memcpy(g->data, fw, g.nVoxels() * sizeof(float));
+        if (bw) std::memcpy(gb->data, bw, gb.nVoxels() * sizeof(float));
+    });
+}
+
+int ref_f3d2_get_cpp(void *hp, float *fw, float *bw, int *steps) {
+    F3d2Handle *h = static_cast<F3d2Handle*>(hp);
+    return guarded([&] {
+        NiftiImage& g = h->reg->Con().F3dContent::GetControlPointGrid();
+        NiftiImage& gb = h->reg->ConBw().F3dContent::GetControlPointGrid();
+        if (fw) std::memcpy(fw, g->data, g.nVoxels() * sizeof(float));
+        if (bw) std::memcpy(bw, gb->data, gb.nVoxels() * sizeof(float));
+        if (steps) { steps[0] = (int)g->intent_p2; steps[1] = (int)gb->intent_p2; }
+    });
+}
+
+// reg_f3d2::GetObjectiveFunctionValue (_reg_f3d2.cpp:540-560).  terms: wMeasure, wBE, wLE.
+int ref_f3d2_value(void *hp, double *value, double *terms) {
+    F3d2Handle *h = static_cast<F3d2Handle*>(hp);
+    return guarded([&] {
+        *value = h->reg->Value();
+        if (terms) { terms[0] = h->reg->WMeasure(); terms[1] = h->reg->WBE(); terms[2] = h->reg->WLE(); }
+    });
+}
+
+// reg_f3d2::GetObjectiveFunctionGradient (_reg_f3d2.cpp:417-440), optionally followed by
+// reg_f3d2::NormaliseGradient (:373-395).  Transformation gradients of both directions.
+int ref_f3d2_gradient(void *hp, int normalise, double *maxLength, float *gradFw, float *gradBw) {
+    F3d2Handle *h = static_cast<F3d2Handle*>(hp);
+    return guarded([&] {
+        h->reg->Gradient();
+        if (normalise) { double m = h->reg->NormGrad(); if (maxLength) *maxLength = m; }
+        NiftiImage& tg = h->reg->Con().GetTransformationGradient();
+        NiftiImage& tb = h->reg->ConBw().GetTransformationGradient();
+        if (gradFw) std::memcpy(gradFw, tg->data, tg.nVoxels() * sizeof(float));
+        if (gradBw) std::memcpy(gradBw, tb->data, tb.nVoxels() * sizeof(float));
+    });
+}
+
+int ref_f3d2_weights(void *hp, float *w) {
+    F3d2Handle *h = static_cast<F3d2Handle*>(hp);
+    w[0] = h->reg->SimWeight(); w[1] = h->reg->BEWeight(); w[2] = h->reg->LEWeight();
+    return 0;
+}
+
+} // extern "C"
+
+extern "C" {
 
 // ---------------------------------------------------------------- Aladin (block matching + LTS)
 

@@ -798,3 +798,78 @@ def ref_symmetric_grids(refg, flog, spacing):
     _scheck(lib, lib.ref_symmetric_grids(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)), fp(sp), ctypes.byref(o),
                                          fp(fw), fp(bw), 3 * G))
     return o, fw, bw
+
+
+class RefF3d2:
+    """reg_f3d2<float> (symmetric, velocity grids) prepared at one level for single evaluations.
+
+    scalar=True uses the reference built without USE_SSE (the oracle's summation order)."""
+
+    def __init__(self, refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=1,
+                 level=0, maxit=150, padding=float("nan"), scalar=True):
+        p = ref_f3d_params()
+        p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
+        p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
+        p.measure, p.lncc_sigma = 0, -5.0
+        self.p = p
+        self.lib = r = ref_scalar() if scalar else ref()
+        r.ref_f3d2_create.restype = ctypes.c_void_p
+        r.ref_f3d2_create.argtypes = [ctypes.c_void_p] * 6
+        h = r.ref_f3d2_create(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
+                              ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
+                              ctypes.byref(p), ctypes.c_void_p(level))
+        if not h:
+            raise RuntimeError(r.ref_last_error().decode())
+        self.h = ctypes.c_void_p(h)
+        for n in ("ref_f3d2_destroy", "ref_f3d2_level_geom", "ref_f3d2_level_images", "ref_f3d2_set_cpp",
+                  "ref_f3d2_get_cpp", "ref_f3d2_value", "ref_f3d2_gradient", "ref_f3d2_weights"):
+            getattr(r, n).argtypes = None
+        cg, rgm, fgm = nr_geom(), nr_geom(), nr_geom()
+        _scheck(r, r.ref_f3d2_level_geom(self.h, ctypes.byref(cg), ctypes.byref(rgm), ctypes.byref(fgm)))
+        self.cpp_geom, self.ref_geom, self.flo_geom = cg, rgm, fgm
+        self.G = cg.nx * cg.ny * cg.nz
+
+    def images(self):
+        nr_ = self.ref_geom.nx * self.ref_geom.ny * self.ref_geom.nz
+        nf = self.flo_geom.nx * self.flo_geom.ny * self.flo_geom.nz
+        a, b = np.zeros(nr_, np.float32), np.zeros(nf, np.float32)
+        _scheck(self.lib, self.lib.ref_f3d2_level_images(self.h, fp(a), fp(b)))
+        return a, b
+
+    def weights(self):
+        w = np.zeros(3, np.float32)
+        _scheck(self.lib, self.lib.ref_f3d2_weights(self.h, fp(w)))
+        return float(w[0]), float(w[1]), float(w[2])
+
+    def get_cpp(self):
+        f, b = np.zeros(3 * self.G, np.float32), np.zeros(3 * self.G, np.float32)
+        s = np.zeros(2, np.int32)
+        _scheck(self.lib, self.lib.ref_f3d2_get_cpp(self.h, fp(f), fp(b), ip(s)))
+        return f, b, (int(s[0]), int(s[1]))
+
+    def set_cpp(self, fw, bw):
+        _scheck(self.lib, self.lib.ref_f3d2_set_cpp(self.h, fp(np.ascontiguousarray(fw, np.float32)),
+                                                    fp(np.ascontiguousarray(bw, np.float32))))
+
+    def value(self):
+        v = ctypes.c_double()
+        t = np.zeros(3, np.float64)
+        _scheck(self.lib, self.lib.ref_f3d2_value(self.h, ctypes.byref(v), dp(t)))
+        return v.value, t
+
+    def gradient(self, normalise=False):
+        gf, gb = np.zeros(3 * self.G, np.float32), np.zeros(3 * self.G, np.float32)
+        m = ctypes.c_double()
+        _scheck(self.lib, self.lib.ref_f3d2_gradient(self.h, int(normalise), ctypes.byref(m), fp(gf), fp(gb)))
+        return gf, gb, m.value
+
+    def close(self):
+        if self.h:
+            self.lib.ref_f3d2_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

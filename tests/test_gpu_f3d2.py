@@ -1,0 +1,92 @@
+"""Symmetric diffeomorphic F3D (reg_f3d2) evaluations on the device against the reference.
+
+The reference session is reg_f3d2<float> from the reference's own sources built without USE_SSE
+(oracle/_ref/libnrref_nosse.so), prepared at one level like reg_base::Run: symmetric velocity
+grids, symmetric NMI, bending and linear energy of both grids.  The device runs the same chain
+(scaling-and-squaring exponentiation of both grids, both warps, both histograms, voxel
+gradients exponentiated along the opposite grid, convolution and node projection, penalty
+gradients, the shared normalisation) with each direction on its own stream.  Every kernel on
+that chain follows the scalar reference's arithmetic, so the normalised node gradients of both
+directions are compared bit for bit; the objective's terms are FP64 sums whose order differs
+from the reference's OpenMP loops (1e-12 relative).
+"""
+import numpy as np
+import pytest
+
+import oracles as O
+from niftyreg_amd import phantoms
+from niftyreg_amd.f3d import F3d2Pair, F3dContent, default_params
+from niftyreg_amd.geometry import Geom
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not O.ref_available(), reason="reference build absent")]
+
+
+def _oblique(dims):
+    th = 0.25
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    m = np.eye(4, dtype=np.float32)
+    m[:3, :3] = (R @ np.diag([1.1, 0.95, 1.2])).astype(np.float32)
+    m[:3, 3] = [4.0, -3.0, 2.0]
+    return Geom(dims, m)
+
+
+def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5):
+    if which == "identity":
+        rg = fg = Geom.identity((40, 36, 32))
+    else:
+        rg, fg = _oblique((36, 34, 30)), Geom.identity((34, 38, 28), (1.1, 1.0, 1.2), (-2.0, 1.0, 3.0))
+    ref, flo = phantoms.config0_pair(3, rg.dims)
+    if fg.dims != rg.dims:
+        flo = phantoms.blob_phantom(fg.dims, 4)
+    r = O.RefF3d2(rg, ref, fg, flo, be=be, le=le, spacing=-5.0, levels=1)
+    fw0, bw0, _ = r.get_cpp()
+    rng = np.random.default_rng(seed)
+    fw = (fw0 + rng.uniform(-amplitude, amplitude, fw0.size)).astype(np.float32)
+    bw = (bw0 + rng.uniform(-amplitude, amplitude, bw0.size)).astype(np.float32)
+    r.set_cpp(fw, bw)
+    simw, bew, lew = r.weights()
+    cg = O.geom_from_nr(r.cpp_geom)
+    pf = default_params(similarity_weight=simw, be_weight=bew, le_weight=lew)
+    pb = default_params(similarity_weight=simw, be_weight=bew, le_weight=lew)
+    hf = F3dContent(rg, ref.ravel(), fg, flo.ravel(), cg, fw, params=pf)
+    hb = F3dContent(fg, flo.ravel(), rg, ref.ravel(), cg, bw, params=pb)
+    hf.rescale_intensities((2, 65), (2, 65))
+    hb.rescale_intensities((2, 65), (2, 65))
+    return r, F3d2Pair(hf, hb, steps=6)
+
+
+@pytest.mark.parametrize("which", ["identity", "oblique"])
+def test_symmetric_objective_and_gradient_match_reference(gpu, which):
+    r, pair = _session(which)
+    rv, rterms = r.value()
+    v, terms = pair.objective()
+    for k in range(3):
+        assert abs(terms[k] - rterms[k]) <= 1e-12 * abs(rterms[k]), (k, terms, rterms)
+    assert abs(v - rv) <= 1e-12 * abs(rv)
+    rgf, rgb, rmax = r.gradient(normalise=True)
+    m = pair.gradient(normalise=True)
+    assert m == rmax
+    gf, gb = pair.fw.node_gradient(), pair.bw.node_gradient()
+    assert np.array_equal(gf, rgf), np.count_nonzero(gf != rgf)
+    assert np.array_equal(gb, rgb), np.count_nonzero(gb != rgb)
+    # the bending-energy gradient's displacement round trip reaches both device grids too
+    f, b, steps = r.get_cpp()
+    assert np.array_equal(pair.fw.get_control_points(), f) and np.array_equal(pair.bw.get_control_points(), b)
+    assert pair.fw.velocity_steps == steps[0] and pair.bw.velocity_steps == steps[1]
+    r.close()
+    pair.close()
+
+
+def test_large_velocities_update_the_step_number(gpu):
+    r, pair = _session("identity", amplitude=30.0, seed=5)
+    rv, _ = r.value()
+    v, _ = pair.objective()
+    f, b, steps = r.get_cpp()
+    assert steps[0] > 6 and pair.fw.velocity_steps == steps[0] and pair.bw.velocity_steps == steps[1]
+    assert abs(v - rv) <= 1e-12 * abs(rv)
+    rgf, rgb, rmax = r.gradient(normalise=True)
+    m = pair.gradient(normalise=True)
+    assert m == rmax
+    assert np.array_equal(pair.fw.node_gradient(), rgf) and np.array_equal(pair.bw.node_gradient(), rgb)
+    r.close()
+    pair.close()
