@@ -37,7 +37,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--storage", choices=["f32", "f16"], default="f32")
-    p.add_argument("--no-secondary", action="store_true", help="skip the Aladin and LNCC lines")
+    p.add_argument("--no-secondary", action="store_true", help="skip the Aladin, LNCC and F3D2 lines")
+    p.add_argument("--only-f3d2", action="store_true",
+                   help="run only the symmetric (reg_f3d2) line and print it (for profiling)")
     return p.parse_args()
 
 
@@ -178,6 +180,73 @@ def bench_lncc(torch, ref, flo, geom, cppg, cpp, steps, with_cpu):
     return out
 
 
+def bench_f3d2(torch, ref, flo, geom, steps, with_cpu):
+    """Symmetric diffeomorphic F3D (reg_f3d2): one objective value + one gradient of the forward /
+    backward pair at 256x256x160 (velocity grids exponentiated by scaling and squaring, six
+    squarings, both directions warped, symmetric NMI, gradients exponentiated along the opposite
+    grid), inputs resident in HBM."""
+    from niftyreg_amd.f3d import F3d2Pair, F3dContent, default_params
+    from niftyreg_amd.geometry import create_symmetric_control_point_grids
+    cg, fw0, bw0 = create_symmetric_control_point_grids(geom, geom, [5.0, 5.0, 5.0])
+    rng = np.random.default_rng(11)
+    vfw = (fw0 + rng.uniform(-2.0, 2.0, fw0.size)).astype(np.float32)
+    vbw = (bw0 + rng.uniform(-2.0, 2.0, bw0.size)).astype(np.float32)
+    params = lambda: default_params(bins=BINS, similarity_weight=1.0 - BE, be_weight=BE)
+    hf = F3dContent(geom, ref, geom, flo, cg, vfw, params=params())
+    hb = F3dContent(geom, flo, geom, ref, cg, vbw, params=params())
+    hf.rescale_intensities((2, BINS + 1), (2, BINS + 1))
+    hb.rescale_intensities((2, BINS + 1), (2, BINS + 1))
+    pair = F3d2Pair(hf, hb, steps=6)
+    for _ in range(3):
+        pair.objective()
+        pair.gradient()
+    step_s = _sync_time(torch, lambda: (pair.objective(), pair.gradient()), steps)
+    value, terms = pair.objective()
+    squarings = hf.velocity_steps
+    pair.close()
+    N = geom.nvox
+    # HBM bytes of one eval at compulsory traffic, per direction (both run): two exponentiations
+    # (objective and gradient), each the flow (12 B per voxel written) and `squarings`
+    # compositions (12 B read, 12 B written); the gradient exponentiation's intermediate fields
+    # (flow + squarings - 1 compositions) and `squarings` resamplings (field and gradient read,
+    # gradient written: 36 B); the per-operation chain: two warps (field 12 + floating 4 read,
+    # 4 written), image gradient (field 12 + floating 4 read, 12 written), two histograms (8),
+    # NMI voxel gradient (4 + 4 + 12 read, 12 written), convolution (3 axis passes of 12 + 12)
+    # and node projection (12 read).
+    s = squarings
+    exp_bytes = 12 + 24 * s
+    grad_exp_bytes = 12 + 24 * (s - 1) + 36 * s
+    chain_bytes = 2 * 20 + 28 + 2 * 8 + 32 + 72 + 12
+    per_dir = 2 * exp_bytes + grad_exp_bytes + chain_bytes
+    eval_bytes = 2 * per_dir * N
+    out = {
+        "metric": "F3D2 (symmetric, velocity) obj+grad evals/s at 256x256x160", "value": 1.0 / step_s,
+        "unit": "evals/s", "ms_per_step": step_s * 1e3, "squarings": squarings, "nmi_sum": terms[0] / (1.0 - BE),
+        "config": {"workload": "reg_f3d2 NMI objective+gradient eval of the forward/backward pair, 256x256x160 "
+                               "shell pair, 64 bins, BE 0.001, 5-voxel symmetric velocity grids 55x55x35"},
+        "roofline": {"bound": "hbm", "kernel": "whole eval (compulsory bytes of every pass)",
+                     "achieved": eval_bytes / step_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": eval_bytes / step_s / 1e9 / HBM_PEAK_GBS, "bytes_per_eval": eval_bytes},
+        "cpu_baseline": None,
+    }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracles
+        if oracles.ref_available():
+            r = oracles.RefF3d2(geom, ref, geom, flo, bins=BINS, be=BE, le=0.0, spacing=-5.0, levels=1, scalar=False)
+            r.set_cpp(vfw, vbw)
+            t0 = time.perf_counter()
+            r.value()
+            r.gradient()
+            dt = time.perf_counter() - t0
+            r.close()
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "evals/s", "cores": threads, "kind": "reference",
+                                   "sample": f"1 obj+grad eval of reg_f3d2<float> at 256x256x160 (same grids), "
+                                             f"OMP_NUM_THREADS={threads}"}
+    return out
+
+
 def measured_traffic():
     """HBM bytes per gradient-sweep launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (scaled by the copy-kernel calibration recorded there), or None."""
@@ -210,6 +279,9 @@ def main():
 
     geom = Geom.identity(DIMS)
     ref, flo = phantoms.config1_pair(seed=0, dims=DIMS)
+    if a.only_f3d2:
+        print(json.dumps(bench_f3d2(torch, ref, flo, geom, a.steps, not a.no_cpu_baseline)))
+        return
     cppg, cpp = create_control_point_grid(geom, [5.0, 5.0, 5.0])
     rng = np.random.default_rng(7)
     cpp = (cpp + rng.uniform(-2.0, 2.0, cpp.size)).astype(np.float32)
@@ -272,7 +344,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(ref, flo, geom)
     if not a.no_secondary:
         out["secondary"] = [bench_aladin(torch, 20, not a.no_cpu_baseline),
-                            bench_lncc(torch, ref, flo, geom, cppg, cpp, 10, not a.no_cpu_baseline)]
+                            bench_lncc(torch, ref, flo, geom, cppg, cpp, 10, not a.no_cpu_baseline),
+                            bench_f3d2(torch, ref, flo, geom, 10, not a.no_cpu_baseline)]
     print(json.dumps(out))
 
 

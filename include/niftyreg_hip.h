@@ -192,6 +192,9 @@ int nrh_f3d2_objective(nrh_f3d *fw, nrh_f3d *bw, double *value, double terms[3])
  * projected; penalty gradients of both grids.  With max_length non-NULL, reg_f3d2::
  * NormaliseGradient (:373-395): both divided by the larger maximal length, which is returned. */
 int nrh_f3d2_gradient(nrh_f3d *fw, nrh_f3d *bw, double *max_length);
+/* ConjugateGradient<T>::UpdateGradientValues with the backward arrays of reg_f3d2
+ * (reg-lib/Optimiser.cpp:231-310): both directions' gradients in one conjugate-gradient step. */
+int nrh_f3d2_conjugate_gradient(nrh_f3d *fw, nrh_f3d *bw, int first);
 
 /* Fused objective: reg_f3d::GetObjectiveFunctionValue (reg-lib/_reg_f3d.cpp:443-463) in one
  * device sweep (field, warp and joint histogram fused; no field or warped image is written).

@@ -137,6 +137,7 @@ int nrh_f3d_fused_field(nrh_f3d *h) {
         NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
         NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry");
         ensure_field(h);
+        h->velFieldVersion = 0;
         fused_histogram(h, h->field.p);
         NRH_HIP(hipStreamSynchronize(h->stream));
         collect_sweep_times(h);

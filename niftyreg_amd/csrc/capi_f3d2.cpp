@@ -120,10 +120,15 @@ int nrh_f3d2_gradient(nrh_f3d *fw, nrh_f3d *bw, double *max_length) {
             // other's grid, which no stream changes before the join below
             op_exponentiate_gradient(fw, bw, false);
             op_exponentiate_gradient(bw, fw, false);
-            // reg_f3d2::GetSimilarityMeasureGradient (:333-342)
+            // reg_f3d2::GetSimilarityMeasureGradient (:333-342): convolution and node extraction,
+            // as the separable projection when the grid's nodes lie on voxels at an integer spacing
             for (nrh_f3d *h : { fw, bw }) {
-                op_convolve_voxel_gradient(h);
-                op_voxel_to_node(h, p.similarity_weight);
+                if (fused_projection_supported(h)) {
+                    fused_project(h, h->voxelGrad.p, p.similarity_weight);
+                } else {
+                    op_convolve_voxel_gradient(h);
+                    op_voxel_to_node(h, p.similarity_weight);
+                }
             }
         } else {
             op_fill_f32(fw->nodeGrad.p, 3 * fw->G, 0.f, fw->stream);
@@ -148,6 +153,15 @@ int nrh_f3d2_gradient(nrh_f3d *fw, nrh_f3d *bw, double *max_length) {
         }
         NRH_HIP(hipStreamSynchronize(fw->stream));
         NRH_HIP(hipStreamSynchronize(bw->stream));
+    });
+}
+
+// ConjugateGradient<T>::UpdateGradientValues with a backward direction (reg-lib/Optimiser.cpp:
+// 231-310): one gamma from both directions' dot products.
+int nrh_f3d2_conjugate_gradient(nrh_f3d *fw, nrh_f3d *bw, int first) {
+    return guard([&] {
+        NRH_REQUIRE(fw && bw && fw != bw, NRH_ERR_ARG, "two distinct handles are required");
+        op_cg_pair(fw, bw, first);
     });
 }
 

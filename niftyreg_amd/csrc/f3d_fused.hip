@@ -817,37 +817,15 @@ void fused_histogram(nrh_f3d *h, float *fieldOut) {
     h->histVersion = h->cppVersion;
 }
 
-void fused_gradient(nrh_f3d *h, float weight) {
-    FusedPlan *p = plan_of(h);
-    NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
-    FusedArgs a = make_args(h, p);
-    a.logTab = h->histLogF.p;
-    a.ent = h->ent.p;
-    const size_t N = h->N;
+// The voxel gradient g (planar [3][N]) projected onto the nodes: x pass (rows staged in LDS,
+// cell sums), y pass, z pass with normalisation, reorientation and weight, into h->nodeGrad.
+static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float weight) {
     const int nx = h->ref.nx, ny = h->ref.ny, nz = h->ref.nz;
     const int gnx = h->cpp.nx, gny = h->cpp.ny, gnz = h->cpp.nz;
-    p->gvox.alloc(3 * N);
     p->projX.alloc((size_t)3 * nz * ny * gnx);
     p->projY.alloc((size_t)3 * nz * gny * gnx);
-    a.gvox = p->gvox.p;
-    const int nbins = a.rb * a.fb;
-    p->kTab.alloc((size_t)(nbins + 3) / 4 * 4);
-    hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, a.logTab, a.ent, a.rb, a.fb,
-                       p->kTab.p);
-    a.kTab = p->kTab.p;
-    const size_t lds = sweep_lds(a);
-    if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
-    if (h->params.image_storage == 1)
-        hipLaunchKernelGGL((k_sweep<__half, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
-    else
-        hipLaunchKernelGGL((k_sweep<float, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
-    NRH_HIP(hipGetLastError());
-    if (h->profile) {
-        NRH_HIP(hipEventRecord(h->ev[3], h->stream));
-        h->evPending[1] = true;
-    }
     ProjXArgs px{};
-    px.g = p->gvox.p;
+    px.g = g;
     px.out = p->projX.p;
     px.nx = nx;
     px.gnx = gnx;
@@ -889,6 +867,41 @@ void fused_gradient(nrh_f3d *h, float weight) {
     r.nodeGrad = h->nodeGrad.p;
     hipLaunchKernelGGL(k_project_nodes, blocks8(h->G), dim3(256), 0, h->stream, r);
     NRH_HIP(hipGetLastError());
+}
+
+void fused_gradient(nrh_f3d *h, float weight) {
+    FusedPlan *p = plan_of(h);
+    NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
+    FusedArgs a = make_args(h, p);
+    a.logTab = h->histLogF.p;
+    a.ent = h->ent.p;
+    p->gvox.alloc(3 * h->N);
+    a.gvox = p->gvox.p;
+    const int nbins = a.rb * a.fb;
+    p->kTab.alloc((size_t)(nbins + 3) / 4 * 4);
+    hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, a.logTab, a.ent, a.rb, a.fb,
+                       p->kTab.p);
+    a.kTab = p->kTab.p;
+    const size_t lds = sweep_lds(a);
+    if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
+    if (h->params.image_storage == 1)
+        hipLaunchKernelGGL((k_sweep<__half, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
+    else
+        hipLaunchKernelGGL((k_sweep<float, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
+    NRH_HIP(hipGetLastError());
+    if (h->profile) {
+        NRH_HIP(hipEventRecord(h->ev[3], h->stream));
+        h->evPending[1] = true;
+    }
+    project_to_nodes(h, p, p->gvox.p, weight);
+}
+
+bool fused_projection_supported(nrh_f3d *h) { return plan_of(h)->ok; }
+
+void fused_project(nrh_f3d *h, const float *g, float weight) {
+    FusedPlan *p = plan_of(h);
+    NRH_REQUIRE(p->ok, NRH_ERR_UNSUPPORTED, "separable node projection unavailable for this geometry");
+    project_to_nodes(h, p, g, weight);
 }
 
 }  // namespace nrh

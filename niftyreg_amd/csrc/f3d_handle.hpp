@@ -53,6 +53,11 @@ struct nrh_f3d {
     nrh::DevBuf<float> velFields;   // intermediate fields of the gradient exponentiation
     nrh::DevBuf<uint32_t> velKeys;  // min / max keys of the flow displacement
     hipEvent_t evPair = nullptr;    // cross-stream ordering of a forward / backward pair
+    // the deformation field holds the exponentiation of the grid at velFieldVersion (0: it does
+    // not) with velFieldSteps squarings; velFieldUpdated: the step number was chosen from the flow
+    uint64_t velFieldVersion = 0;
+    int velFieldSteps = 0;
+    bool velFieldUpdated = false;
 
     double *pinned = nullptr;  // host-pinned scalar read-back
     double *published = nullptr, *publishedDev = nullptr;  // coherent host memory the device writes results to
@@ -101,6 +106,7 @@ void op_rescale(nrh_f3d *h, float *data, size_t n, float lo, float hi);
 void op_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void op_fill_f32(float *p, size_t n, float v, hipStream_t s);
 void op_cg(nrh_f3d *h, int first);
+void op_cg_pair(nrh_f3d *fw, nrh_f3d *bw, int first);  // synchronous
 void op_update_dof(nrh_f3d *h, float scale);
 double read_scalar(nrh_f3d *h, const double *dev);
 // Copies n0, n1, n2 doubles from s0, s1, s2 (device) to h->published[0..] with one small kernel
@@ -132,5 +138,9 @@ void fused_release(const nrh_f3d *h);
 void fused_histogram(nrh_f3d *h, float *fieldOut = nullptr);
 void fused_gradient(nrh_f3d *h, float weight);
 void collect_sweep_times(nrh_f3d *h);  // after a stream synchronisation
+// The separable node projection of a voxel gradient (integer spacing, nodes on voxels): the
+// convolution + VoxelCentricToNodeCentric of the reference as three passes, into nodeGrad.
+bool fused_projection_supported(nrh_f3d *h);
+void fused_project(nrh_f3d *h, const float *voxelGrad, float weight);
 
 }  // namespace nrh

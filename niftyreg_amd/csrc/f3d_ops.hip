@@ -344,16 +344,30 @@ k_nmi_voxel_gradient(const float *__restrict__ ref, const float *__restrict__ wa
     const double nmi = (ent[0] + ent[1]) / ent[2];
     const size_t refOff = (size_t)rb * fb, floOff = refOff + rb;
     double jd[3] = { 0, 0, 0 }, rd[3] = { 0, 0, 0 }, wd[3] = { 0, 0, 0 };
-    for (int r = (int)(rv - 1.f); r < (int)(rv + 3.f); ++r) {
+    // the reference's bin loops, r in [(int)(rv - 1), (int)(rv + 3)) and w likewise (at most four
+    // each); the Parzen values of each axis are evaluated once instead of once per pair
+    const int r0 = (int)(rv - 1.f), r1 = (int)(rv + 3.f), w0 = (int)(wv - 1.f), w1 = (int)(wv + 3.f);
+    double bR[4], dW[4], rl[4], wl[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int r = r0 + k, w = w0 + k;
+        bR[k] = nmi_basis((double)(rv - (float)r));
+        dW[k] = nmi_basis_deriv((double)(wv - (float)w));
+        rl[k] = (-1 < r && r < rb) ? logh[r + refOff] : 0.0;
+        wl[k] = (-1 < w && w < fb) ? logh[w + floOff] : 0.0;
+    }
+    for (int kr = 0; kr < 4 && r0 + kr < r1; ++kr) {
+        const int r = r0 + kr;
         if (!(-1 < r && r < rb)) continue;
-        for (int w = (int)(wv - 1.f); w < (int)(wv + 3.f); ++w) {
+        for (int kw = 0; kw < 4 && w0 + kw < w1; ++kw) {
+            const int w = w0 + kw;
             if (!(-1 < w && w < fb)) continue;
-            const double common = nmi_basis((double)(rv - (float)r)) * nmi_basis_deriv((double)(wv - (float)w));
-            const double jl = logh[r + w * rb], rl = logh[r + refOff], wl = logh[w + floOff];
+            const double common = bR[kr] * dW[kw];
+            const double jl = logh[r + w * rb];
             for (int d = 0; d < 3; ++d) {
                 if (g[d] == g[d]) {
-                    rd[d] += common * (double)g[d] * rl;
-                    wd[d] += common * (double)g[d] * wl;
+                    rd[d] += common * (double)g[d] * rl[kr];
+                    wd[d] += common * (double)g[d] * wl[kw];
                     jd[d] += common * (double)g[d] * jl;
                 }
             }
@@ -1044,6 +1058,7 @@ void op_spline_field(nrh_f3d *h) {
         const int pre = lut ? c / 5 : (int)((float)c / g[i]);
         NRH_REQUIRE(pre + 3 < gd[i], NRH_ERR_UNSUPPORTED, "control point grid does not cover the reference image");
     }
+    h->velFieldVersion = 0;  // the field no longer holds a velocity exponentiation
     hipLaunchKernelGGL(k_spline_field, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, h->ref.nx, h->ref.ny,
                        h->ref.nz, h->cpp.nx, h->cpp.ny, h->G, h->cppPos.p, h->mask.p, g[0], g[1], g[2], lut,
                        h->field.p, h->N);
@@ -1349,6 +1364,58 @@ void op_cg(nrh_f3d *h, int first) {
                            n, (const double *)(h->scalars.p + 2));
     }
     NRH_HIP(hipGetLastError());
+}
+
+// The symmetric conjugate-gradient update (reg-lib/Optimiser.cpp:231-310): both directions'
+// dot products summed before one gamma updates both; launched on fw's stream after bw's work.
+__global__ void __launch_bounds__(1024)
+k_cg_gamma_pair(const double *__restrict__ pf, int nf, const double *__restrict__ pb, int nb, double *__restrict__ gf,
+                double *__restrict__ gb) {
+    __shared__ double red[16];
+    double g[2][2];
+    const double *parts[2] = { pf, pb };
+    const int counts[2] = { nf, nb };
+    for (int d = 0; d < 2; ++d) {
+        double gg = 0, dgg = 0;
+        for (int i = threadIdx.x; i < counts[d]; i += blockDim.x) {
+            gg += parts[d][2 * i];
+            dgg += parts[d][2 * i + 1];
+        }
+        gg = block_sum_double(gg, red);
+        __syncthreads();
+        dgg = block_sum_double(dgg, red);
+        __syncthreads();
+        g[d][0] = gg;
+        g[d][1] = dgg;
+    }
+    if (threadIdx.x == 0) *gf = *gb = (g[0][1] + g[1][1]) / (g[0][0] + g[1][0]);
+}
+
+void op_cg_pair(nrh_f3d *fw, nrh_f3d *bw, int first) {
+    // bw's arrays are updated on fw's stream: wait for bw's queued work first
+    NRH_HIP(hipStreamSynchronize(bw->stream));
+    hipStream_t s = fw->stream;
+    const size_t nf = 3 * fw->G, nb = 3 * bw->G;
+    if (first) {
+        hipLaunchKernelGGL(k_cg_init, dim3(blocks_for(nf)), dim3(kBlock), 0, s, fw->nodeGrad.p, fw->cgG.p, fw->cgH.p, nf);
+        hipLaunchKernelGGL(k_cg_init, dim3(blocks_for(nb)), dim3(kBlock), 0, s, bw->nodeGrad.p, bw->cgG.p, bw->cgH.p, nb);
+    } else {
+        const unsigned bf = std::min(blocks_for(nf), 512u), bb = std::min(blocks_for(nb), 512u);
+        fw->partials.alloc(std::max<size_t>(fw->partials.n, 2 * (size_t)bf + 64));
+        bw->partials.alloc(std::max<size_t>(bw->partials.n, 2 * (size_t)bb + 64));
+        hipLaunchKernelGGL(k_cg_dots, dim3(bf), dim3(kBlock), 0, s, fw->nodeGrad.p, fw->cgG.p, fw->cgH.p, nf,
+                           fw->partials.p);
+        hipLaunchKernelGGL(k_cg_dots, dim3(bb), dim3(kBlock), 0, s, bw->nodeGrad.p, bw->cgG.p, bw->cgH.p, nb,
+                           bw->partials.p);
+        hipLaunchKernelGGL(k_cg_gamma_pair, dim3(1), dim3(1024), 0, s, fw->partials.p, (int)bf, bw->partials.p, (int)bb,
+                           fw->scalars.p + 2, bw->scalars.p + 2);
+        hipLaunchKernelGGL(k_cg_apply, dim3(blocks_for(nf)), dim3(kBlock), 0, s, fw->nodeGrad.p, fw->cgG.p, fw->cgH.p,
+                           nf, (const double *)(fw->scalars.p + 2));
+        hipLaunchKernelGGL(k_cg_apply, dim3(blocks_for(nb)), dim3(kBlock), 0, s, bw->nodeGrad.p, bw->cgG.p, bw->cgH.p,
+                           nb, (const double *)(bw->scalars.p + 2));
+    }
+    NRH_HIP(hipGetLastError());
+    NRH_HIP(hipStreamSynchronize(s));
 }
 
 void op_update_dof(nrh_f3d *h, float scale) {

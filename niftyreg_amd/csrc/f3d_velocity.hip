@@ -70,6 +70,14 @@ __device__ __forceinline__ float3 to_vox(const Affine& m, float3 p) {
 
 __device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
 
+// Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD b % 8).  This maps
+// them so that each XCD takes one contiguous run of voxel blocks (a slab of rows and slices):
+// the rows a block gathers from its neighbours then sit in the same XCD's L2.  A bijection of
+// [0, nb) when nb is a multiple of 8 (launches round their grids up).
+__device__ __forceinline__ size_t xcd_block(unsigned b, unsigned nb) {
+    return (size_t)(b & 7u) * (nb >> 3) + (b >> 3);
+}
+
 // get_SlidedValues 3-D for positions (reg-lib/cpu/_reg_splineBasis.cpp:496-551): the value at the
 // clamped index plus the mm offset of the clamped steps.
 __device__ __forceinline__ float3 slided(const float *__restrict__ d, size_t n, int nx, int ny, int nz,
@@ -91,36 +99,55 @@ __device__ __forceinline__ uint32_t float_key(float v) {
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// The velocity grid evaluated at every voxel's position of the field geometry: the composition
-// branch of reg_cubic_spline_getDeformationField3D (reg-lib/cpu/_reg_localTrans.cpp:903-1079,
-// scalar order) applied to the identity field, as reg_spline_getFlowFieldFromVelocityGrid
-// (:3465-3491) does.  SCALED: writes (flow - x) / scaling + x, the first field of the squaring
-// (:3546-3561); otherwise writes the displacement flow - x and folds its extremes into keys[0]
-// (min) and keys[1] (max).
-template<bool SCALED>
-__global__ void __launch_bounds__(kBlock) k_vel_flow(VelGrid g, FieldGeom f, double scaling, float *__restrict__ out,
-                                                     uint32_t *__restrict__ keys) {
+// The velocity grid evaluated at voxel i's position: the composition branch of
+// reg_cubic_spline_getDeformationField3D (reg-lib/cpu/_reg_localTrans.cpp:903-1079, scalar
+// order) applied to the identity field, as reg_spline_getFlowFieldFromVelocityGrid (:3465-3491)
+// does.  Returns the displacement flow - x and the position x.  When every lane of the wave has
+// its 4x4x4 support inside the grid, each slab of 16 nodes is loaded at once (48 loads in flight)
+// before its terms are summed in the reference's order.
+__device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, size_t i, bool active, float3& disp,
+                                        float3& pos) {
 #pragma clang fp contract(off)
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    float lo = INFINITY, hi = -INFINITY;
-    if (i < f.n) {
-        const int x = (int)(i % f.nx), y = (int)((i / f.nx) % f.ny), z = (int)(i / ((size_t)f.nx * f.ny));
-        const float3 p = id_pos(f.xyz, x, y, z);
-        const float3 v = to_vox(g.ijk, p);
-        int pre[3];
-        float b[3][4];
-        const float vc[3] = { v.x, v.y, v.z };
+    const size_t ii = active ? i : 0;
+    const int x = (int)(ii % f.nx), y = (int)((ii / f.nx) % f.ny), z = (int)(ii / ((size_t)f.nx * f.ny));
+    const float3 p = id_pos(f.xyz, x, y, z);
+    const float3 v = to_vox(g.ijk, p);
+    int pre[3];
+    float b[3][4];
+    const float vc[3] = { v.x, v.y, v.z };
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            pre[r] = floor_int(vc[r]);
-            float t = vc[r] - (float)pre[r];
-            pre[r]--;
-            if (t < 0) t = 0;
-            bspline_basis(t, b[r]);
+    for (int r = 0; r < 3; ++r) {
+        pre[r] = floor_int(vc[r]);
+        float t = vc[r] - (float)pre[r];
+        pre[r]--;
+        if (t < 0) t = 0;
+        bspline_basis(t, b[r]);
+    }
+    const bool inside = pre[0] >= 0 && pre[0] + 3 < g.gnx && pre[1] >= 0 && pre[1] + 3 < g.gny && pre[2] >= 0 &&
+                        pre[2] + 3 < g.gnz;
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    if (__all(inside || !active)) {
+        const size_t sy = g.gnx, sz = (size_t)g.gnx * g.gny;
+        const size_t base = active ? ((size_t)pre[2] * g.gny + pre[1]) * g.gnx + pre[0] : 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float cx[16], cy[16], cz[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const size_t idx = base + c * sz + (k >> 2) * sy + (k & 3);
+                cx[k] = g.cp[idx];
+                cy[k] = g.cp[idx + g.G];
+                cz[k] = g.cp[idx + 2 * g.G];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const float w = b[0][k & 3] * b[1][k >> 2] * b[2][c];
+                r0 += cx[k] * w;
+                r1 += cy[k] * w;
+                r2 += cz[k] * w;
+            }
         }
-        const bool inside = pre[0] >= 0 && pre[0] + 3 < g.gnx && pre[1] >= 0 && pre[1] + 3 < g.gny && pre[2] >= 0 &&
-                            pre[2] + 3 < g.gnz;
-        float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    } else if (active) {
         for (int c = 0; c < 4; ++c) {
             const int Z = pre[2] + c;
             // get_GridValues (reg-lib/cpu/_reg_splineBasis.cpp:602-665): once a row of a slice
@@ -133,7 +160,7 @@ __global__ void __launch_bounds__(kBlock) k_vel_flow(VelGrid g, FieldGeom f, dou
                 for (int a = 0; a < 4; ++a) {
                     const int X = pre[0] + a;
                     float3 cv;
-                    if (inside || (X > -1 && X < g.gnx && !out)) {
+                    if (X > -1 && X < g.gnx && !out) {
                         const size_t idx = ((size_t)Z * g.gny + Y) * g.gnx + X;
                         cv.x = g.cp[idx];
                         cv.y = g.cp[idx + g.G];
@@ -148,28 +175,64 @@ __global__ void __launch_bounds__(kBlock) k_vel_flow(VelGrid g, FieldGeom f, dou
                 }
             }
         }
-        const float d0 = r0 - p.x, d1 = r1 - p.y, d2 = r2 - p.z;
-        if (SCALED) {
-            out[i] = (float)((double)d0 / scaling) + p.x;
-            out[i + f.n] = (float)((double)d1 / scaling) + p.y;
-            out[i + 2 * f.n] = (float)((double)d2 / scaling) + p.z;
-        } else {
-            out[i] = d0;
-            out[i + f.n] = d1;
-            out[i + 2 * f.n] = d2;
-            lo = fminf(d0, fminf(d1, d2));
-            hi = fmaxf(d0, fmaxf(d1, d2));
+    }
+    disp = make_float3(r0 - p.x, r1 - p.y, r2 - p.z);
+    pos = p;
+}
+
+// The scaled flow, the first field of the squaring (reg-lib/cpu/_reg_localTrans.cpp:3546-3561):
+// (flow - x) / scaling + x.
+__global__ void __launch_bounds__(kBlock) k_vel_flow_scaled(VelGrid g, FieldGeom f, double scaling,
+                                                            float *__restrict__ out) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = i < f.n;
+    float3 d, p;
+    flow_at(g, f, i, active, d, p);
+    if (!active) return;
+    out[i] = (float)((double)d.x / scaling) + p.x;
+    out[i + f.n] = (float)((double)d.y / scaling) + p.y;
+    out[i + 2 * f.n] = (float)((double)d.z / scaling) + p.z;
+}
+
+// The flow's displacement flow - x, with its extremes folded into keys[0] (min) and keys[1]
+// (max): grid-stride loop, one block-wide reduction and one atomic pair per block.
+__global__ void __launch_bounds__(kBlock) k_vel_flow_disp(VelGrid g, FieldGeom f, float *__restrict__ out,
+                                                          uint32_t *__restrict__ keys) {
+#pragma clang fp contract(off)
+    __shared__ float sLo[kBlock / 64], sHi[kBlock / 64];
+    float lo = INFINITY, hi = -INFINITY;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const size_t rounds = (f.n + stride - 1) / stride;
+    for (size_t r = 0; r < rounds; ++r) {
+        const size_t i = r * stride + (size_t)blockIdx.x * kBlock + threadIdx.x;
+        const bool active = i < f.n;
+        float3 d, p;
+        flow_at(g, f, i, active, d, p);
+        if (active) {
+            out[i] = d.x;
+            out[i + f.n] = d.y;
+            out[i + 2 * f.n] = d.z;
+            lo = fminf(lo, fminf(d.x, fminf(d.y, d.z)));
+            hi = fmaxf(hi, fmaxf(d.x, fmaxf(d.y, d.z)));
         }
     }
-    if (!SCALED) {
-        for (int off = 32; off > 0; off >>= 1) {
-            lo = fminf(lo, __shfl_xor(lo, off, 64));
-            hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sLo[threadIdx.x >> 6] = lo;
+        sHi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+            lo = fminf(lo, sLo[w]);
+            hi = fmaxf(hi, sHi[w]);
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMin(&keys[0], float_key(lo));
-            atomicMax(&keys[1], float_key(hi));
-        }
+        atomicMin(&keys[0], float_key(lo));
+        atomicMax(&keys[1], float_key(hi));
     }
 }
 
@@ -192,41 +255,64 @@ __global__ void __launch_bounds__(kBlock) k_vel_scale(const float *disp, FieldGe
 __global__ void __launch_bounds__(kBlock) k_compose(const float *__restrict__ D, FieldGeom d, const float *pos,
                                                     size_t m, float *out) {
 #pragma clang fp contract(off)
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= m) return;
-    const float3 real = make_float3(pos[i], pos[i + m], pos[i + 2 * m]);
+    const size_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool active = i < m;
+    const size_t ii = active ? i : 0;
+    const float3 real = make_float3(pos[ii], pos[ii + m], pos[ii + 2 * m]);
     const float3 v = to_vox(d.ijk, real);
     const int px = floor_int(v.x), py = floor_int(v.y), pz = floor_int(v.z);
     const float rx1 = v.x - (float)px, ry1 = v.y - (float)py, rz1 = v.z - (float)pz;
     const float rx[2] = { 1.f - rx1, rx1 }, ry[2] = { 1.f - ry1, ry1 }, rz[2] = { 1.f - rz1, rz1 };
     const bool inside = px > -1 && px + 1 < d.nx && py > -1 && py + 1 < d.ny && pz > -1 && pz + 1 < d.nz;
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    if (__all(inside || !active) && d.nx > 1 && d.ny > 1 && d.nz > 1) {
+        // every tap inside: all 24 loads issued before the sums (reference order kept)
+        const size_t base = active ? ((size_t)pz * d.ny + py) * d.nx + px : 0;
+        const size_t sy = d.nx, sz = (size_t)d.nx * d.ny;
+        float t0[8], t1[8], t2[8];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int Z = pz + c;
-        const bool inZ = Z > -1 && Z < d.nz;
+        for (int k = 0; k < 8; ++k) {
+            const size_t idx = base + (k >> 2) * sz + ((k >> 1) & 1) * sy + (k & 1);
+            t0[k] = D[idx];
+            t1[k] = D[idx + d.n];
+            t2[k] = D[idx + 2 * d.n];
+        }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int Y = py + b;
-            const float tb = ry[b] * rz[c];
-            const bool inY = Y > -1 && Y < d.ny;
+        for (int k = 0; k < 8; ++k) {
+            const float w = rx[k & 1] * (ry[(k >> 1) & 1] * rz[k >> 2]);
+            a0 += t0[k] * w;
+            a1 += t1[k] * w;
+            a2 += t2[k] * w;
+        }
+    } else {
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const int X = px + a;
-                float3 dv;
-                if (inside || (X > -1 && X < d.nx && inY && inZ)) {
-                    const size_t idx = ((size_t)Z * d.ny + Y) * d.nx + X;
-                    dv = make_float3(D[idx], D[idx + d.n], D[idx + 2 * d.n]);
-                } else {
-                    dv = slided(D, d.n, d.nx, d.ny, d.nz, d.xyz, X, Y, Z);
+        for (int c = 0; c < 2; ++c) {
+            const int Z = pz + c;
+            const bool inZ = Z > -1 && Z < d.nz;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int Y = py + b;
+                const float tb = ry[b] * rz[c];
+                const bool inY = Y > -1 && Y < d.ny;
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const int X = px + a;
+                    float3 dv;
+                    if (X > -1 && X < d.nx && inY && inZ) {
+                        const size_t idx = ((size_t)Z * d.ny + Y) * d.nx + X;
+                        dv = make_float3(D[idx], D[idx + d.n], D[idx + 2 * d.n]);
+                    } else {
+                        dv = slided(D, d.n, d.nx, d.ny, d.nz, d.xyz, X, Y, Z);
+                    }
+                    const float w = rx[a] * tb;
+                    a0 += dv.x * w;
+                    a1 += dv.y * w;
+                    a2 += dv.z * w;
                 }
-                const float w = rx[a] * tb;
-                a0 += dv.x * w;
-                a1 += dv.y * w;
-                a2 += dv.z * w;
             }
         }
     }
+    if (!active) return;
     out[i] = a0;
     out[i + m] = a1;
     out[i + 2 * m] = a2;
@@ -254,7 +340,7 @@ struct ResampleGradArgs {
 // an axis the difference steps back one voxel, and so do the other two axes' differences there.
 __global__ void __launch_bounds__(kBlock) k_resample_gradient(ResampleGradArgs r) {
 #pragma clang fp contract(off)
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= r.n) return;
     const int x = (int)(i % r.nx), y = (int)((i / r.nx) % r.ny), z = (int)(i / ((size_t)r.nx * r.ny));
     const float3 fc = to_vox(r.gradIjk, make_float3(r.def[i], r.def[i + r.n], r.def[i + 2 * r.n]));
@@ -271,6 +357,26 @@ __global__ void __launch_bounds__(kBlock) k_resample_gradient(ResampleGradArgs r
     }
     const int gdim[3] = { r.gnx, r.gny, r.gnz };
     float vx = 0.f, vy = 0.f, vz = 0.f;
+    const bool inside = lo[0] > -1 && hi[0] < gdim[0] && lo[1] > -1 && hi[1] < gdim[1] && lo[2] > -1 && hi[2] < gdim[2];
+    if (__all(inside)) {
+        // every tap inside: the 24 loads issued together, summed in the reference's order
+        float t0[8], t1[8], t2[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int X = (k & 1) ? hi[0] : lo[0], Y = ((k >> 1) & 1) ? hi[1] : lo[1], Z = (k >> 2) ? hi[2] : lo[2];
+            const size_t idx = ((size_t)Z * r.gny + Y) * r.gnx + X;
+            t0[k] = r.grad[idx];
+            t1[k] = r.grad[idx + r.gn];
+            t2[k] = r.grad[idx + 2 * r.gn];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float w = bw[0][k & 1] * bw[1][(k >> 1) & 1] * bw[2][k >> 2];
+            vx += t0[k] * w;
+            vy += t1[k] * w;
+            vz += t2[k] * w;
+        }
+    } else {
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const int Z = c ? hi[2] : lo[2];
@@ -307,6 +413,7 @@ __global__ void __launch_bounds__(kBlock) k_resample_gradient(ResampleGradArgs r
             vy += w;
             vz += w;
         }
+    }
     }
     float J[3][3] = {};
     const float basis[2] = { 1.f, 0.f }, deriv[2] = { -1.f, 1.f };
@@ -406,7 +513,19 @@ __global__ void __launch_bounds__(kBlock) k_symmetrise(float *__restrict__ fw, f
 
 inline dim3 blocks_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
+// grid of a kernel indexed through xcd_block: a multiple of 8 workgroups
+inline dim3 blocks8_for(size_t n) { return dim3((unsigned)(((n + kBlock - 1) / kBlock + 7) / 8 * 8)); }
+
+// k_vel_flow_disp: enough blocks to fill the device (8 per CU), each one atomic pair at the end.
+inline unsigned flow_disp_blocks(size_t n) {
+    const size_t b = (n + kBlock - 1) / kBlock;
+    return (unsigned)std::min<size_t>(b, 2048);
+}
+
 VelGrid vel_grid(const nrh_f3d *h) {
+    // the flow's fast path reads a 4x4x4 window from node 0 on lanes without a voxel
+    NRH_REQUIRE(h->cpp.nx >= 4 && h->cpp.ny >= 4 && h->cpp.nz >= 4, NRH_ERR_UNSUPPORTED,
+                "velocity grids need at least 4 nodes along each axis");
     VelGrid g;
     g.cp = h->cppPos.p;
     g.gnx = h->cpp.nx; g.gny = h->cpp.ny; g.gnz = h->cpp.nz;
@@ -440,7 +559,7 @@ double scaling_of(int steps) {
 }
 
 void launch_compose(const float *D, const FieldGeom& d, const float *pos, size_t m, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_compose, blocks_for(m), dim3(kBlock), 0, s, D, d, pos, m, out);
+    hipLaunchKernelGGL(k_compose, blocks8_for(m), dim3(kBlock), 0, s, D, d, pos, m, out);
 }
 
 // The rotation-free, spacing-normalised Jacobian factors of reg_trilinearResampleGradient for a
@@ -486,18 +605,28 @@ void resample_gradient_factors(const nrh_geom& g, float reo[9], float sp[3]) {
 // extensions: reg_spline_getDefFieldFromVelocityGrid into the handle's deformation field.
 void op_vel_def_field(nrh_f3d *h, bool update) {
     NRH_REQUIRE(h->velocity, NRH_ERR_STATE, "the control points are not a velocity grid");
+    // the field already holds this exponentiation: same control points and the same step rule
+    if (h->velFieldVersion == h->cppVersion && !h->field.empty()) {
+        if (!update && h->velSteps == h->velFieldSteps) return;
+        if (update && h->velFieldUpdated && (h->velSteps >= 0) == (h->velFieldSteps > 0)) {
+            h->velSteps = h->velFieldSteps;
+            return;
+        }
+    }
+    h->velFieldVersion = 0;
     ensure_field(h);
     h->velTmp.alloc(3 * h->N);
     const FieldGeom f = field_geom(h->ref);
     const VelGrid g = vel_grid(h);
     int steps = h->velSteps;
+    const bool chosen = update || steps == 0;
     int sq;
     if (update || steps == 0) {
         const uint32_t init[2] = { 0xffffffffu, 0u };
         h->velKeys.alloc(2);
         NRH_HIP(hipMemcpyAsync(h->velKeys.p, init, sizeof init, hipMemcpyHostToDevice, h->stream));
-        hipLaunchKernelGGL(k_vel_flow<false>, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, 1.0, h->velTmp.p,
-                           h->velKeys.p);
+        hipLaunchKernelGGL(k_vel_flow_disp, dim3(flow_disp_blocks(f.n)), dim3(kBlock), 0, h->stream, g, f,
+                           h->velTmp.p, h->velKeys.p);
         NRH_HIP(hipGetLastError());
         uint32_t keys[2];
         NRH_HIP(hipMemcpyAsync(keys, h->velKeys.p, sizeof keys, hipMemcpyDeviceToHost, h->stream));
@@ -518,8 +647,8 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
     } else {
         sq = steps < 0 ? -steps : steps;
         float *first = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
-        hipLaunchKernelGGL(k_vel_flow<true>, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, scaling_of(steps),
-                           first, nullptr);
+        hipLaunchKernelGGL(k_vel_flow_scaled, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, scaling_of(steps),
+                           first);
         NRH_HIP(hipGetLastError());
     }
     float *src = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
@@ -530,14 +659,16 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
     }
     NRH_HIP(hipGetLastError());
     h->velSteps = steps;
+    h->velFieldVersion = h->cppVersion;
+    h->velFieldSteps = steps;
+    h->velFieldUpdated = chosen;
 }
 
 // reg_spline_getIntermediateDefFieldFromVelGrid (reg-lib/cpu/_reg_localTrans.cpp:3616-3683) of
 // grid `g` (steps squarings) on geometry f: fields[0..count-1] (count <= |steps| + 1).
 static void intermediate_fields(const VelGrid& g, const FieldGeom& f, int steps, float *fields, int count,
                                 hipStream_t s) {
-    hipLaunchKernelGGL(k_vel_flow<true>, blocks_for(f.n), dim3(kBlock), 0, s, g, f, scaling_of(steps), fields,
-                       nullptr);
+    hipLaunchKernelGGL(k_vel_flow_scaled, blocks_for(f.n), dim3(kBlock), 0, s, g, f, scaling_of(steps), fields);
     for (int k = 0; k + 1 < count; ++k)
         launch_compose(fields + (size_t)k * 3 * f.n, f, fields + (size_t)k * 3 * f.n, f.n,
                        fields + (size_t)(k + 1) * 3 * f.n, s);
@@ -575,7 +706,7 @@ void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther) {
         r.def = h->velFields.p + (size_t)k * 3 * f.n;
         r.divisor = (k == sq - 1) ? div : 0.0;
         r.out = dst;
-        hipLaunchKernelGGL(k_resample_gradient, blocks_for(f.n), dim3(kBlock), 0, h->stream, r);
+        hipLaunchKernelGGL(k_resample_gradient, blocks8_for(f.n), dim3(kBlock), 0, h->stream, r);
         std::swap(src, dst);
     }
     NRH_HIP(hipGetLastError());
@@ -602,7 +733,7 @@ void op_resample_gradient(nrh_f3d *h, float pad, float *out) {
     r.accum = nullptr;
     r.divisor = 0.0;
     r.out = out;
-    hipLaunchKernelGGL(k_resample_gradient, blocks_for(f.n), dim3(kBlock), 0, h->stream, r);
+    hipLaunchKernelGGL(k_resample_gradient, blocks8_for(f.n), dim3(kBlock), 0, h->stream, r);
     NRH_HIP(hipGetLastError());
 }
 
@@ -610,6 +741,7 @@ void op_resample_gradient(nrh_f3d *h, float pad, float *out) {
 // `def` (device, geometry g) evaluated at its positions.
 void op_compose_field(nrh_f3d *h, const nrh_geom& g, const float *def) {
     NRH_REQUIRE(!h->field.empty(), NRH_ERR_STATE, "deformation field not computed");
+    h->velFieldVersion = 0;
     launch_compose(def, field_geom(g), h->field.p, h->N, h->field.p, h->stream);
     NRH_HIP(hipGetLastError());
 }

@@ -136,6 +136,43 @@ void HipF3dContent::ZeroVoxelBasedMeasureGradient() {
     onDevice |= VoxelGradient;
 }
 
+void HipF3dContent::SetDeformationField(NiftiImage&& deformationFieldIn) {
+    F3dContent::SetDeformationField(std::move(deformationFieldIn));
+    onDevice &= ~Field;
+    EnsureFieldOnDevice();
+}
+
+void HipF3dContent::UpdateDeformationField() {
+    onDevice &= ~Field;
+    EnsureFieldOnDevice();
+}
+
+void HipF3dContent::UpdateVoxelBasedMeasureGradient() {
+    onDevice &= ~VoxelGradient;
+    EnsureVoxelGradientOnDevice();
+}
+
+void HipF3dContent::EnsureFieldOnDevice() {
+    if (onDevice & Field) return;
+    Check(nrh_f3d_upload_field(handle, static_cast<const float*>(deformationField->data)), "nrh_f3d_upload_field");
+    onDevice |= Field;
+}
+
+void HipF3dContent::EnsureVoxelGradientOnDevice() {
+    if (onDevice & VoxelGradient) return;
+    Check(nrh_f3d_upload_voxel_gradient(handle, static_cast<const float*>(voxelBasedMeasureGradient->data)),
+          "nrh_f3d_upload_voxel_gradient");
+    onDevice |= VoxelGradient;
+}
+
+void HipF3dContent::SyncVelocityGrid() {
+    if (controlPointGrid->intent_p1 != SPLINE_VEL_GRID) NR_FATAL_ERROR("The control point grid is not a velocity grid");
+    if (controlPointGrid->num_ext > 0)
+        NR_FATAL_ERROR("Velocity grids with affine extensions are not supported on the HIP platform");
+    if (transformationMatrix) NR_FATAL_ERROR("Velocity grids with an affine transformation are not supported on HIP");
+    Check(nrh_f3d_set_velocity_grid(handle, (int)controlPointGrid->intent_p2), "nrh_f3d_set_velocity_grid");
+}
+
 // ---------------------------------------------------------------- HipAladinContent
 HipAladinContent::HipAladinContent(NiftiImage& referenceIn, NiftiImage& floatingIn, int *referenceMaskIn,
                                    mat44 *transformationMatrixIn, size_t bytesIn, const unsigned percentageOfBlocks,
@@ -177,6 +214,8 @@ ContentCreator* HipContentCreatorFactory::Produce(const ContentType conType) {
     switch (conType) {
     case ContentType::F3d:
         return new HipF3dContentCreator();
+    case ContentType::F3d2:
+        return new HipF3d2ContentCreator();
     case ContentType::Aladin:
         return new HipAladinContentCreator();
     default:
@@ -277,13 +316,65 @@ void HipCompute::SmoothGradient(float) { NR_FATAL_ERROR("Gradient smoothing is n
 void HipCompute::GetApproximatedGradient(InterfaceOptimiser&) {
     NR_FATAL_ERROR("The finite-difference gradient is not implemented on HIP");
 }
-void HipCompute::GetDefFieldFromVelocityGrid(const bool) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
-void HipCompute::ExponentiateGradient(Content&) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
-void HipCompute::UpdateVelocityField(float, bool, bool, bool) {
-    NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP");
+// ---- velocity grids (reg_f3d2)
+void HipCompute::GetDefFieldFromVelocityGrid(const bool updateStepNumber) {
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    if (c.GridHeader()->intent_p1 == CUB_SPLINE_GRID) {
+        // reg_spline_getDefFieldFromVelocityGrid on a displacement grid: the plain spline field
+        GetDeformationField(false, true);
+        return;
+    }
+    c.SyncVelocityGrid();
+    Check(nrh_f3d_def_field_from_velocity(Handle(), updateStepNumber ? 1 : 0), "nrh_f3d_def_field_from_velocity");
+    int steps = 0;
+    Check(nrh_f3d_get_velocity_steps(Handle(), &steps), "nrh_f3d_get_velocity_steps");
+    c.GridHeader()->intent_p2 = (float)steps;
+    c.MarkOnDevice(HipF3dContent::Field);
 }
-void HipCompute::BchUpdate(float, int) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
-void HipCompute::SymmetriseVelocityFields(Content&) { NR_FATAL_ERROR("Velocity grids (F3D2) are not implemented on HIP"); }
+
+void HipCompute::ExponentiateGradient(Content& conBwIn) {
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    HipF3dContent& b = dynamic_cast<HipF3dContent&>(conBwIn);
+    b.SyncVelocityGrid();
+    c.EnsureVoxelGradientOnDevice();
+    Check(nrh_f3d_exponentiate_gradient(Handle(), b.GetHandle()), "nrh_f3d_exponentiate_gradient");
+    c.MarkOnDevice(HipF3dContent::VoxelGradient);
+}
+
+void HipCompute::UpdateVelocityField(float scale, bool optimiseX, bool optimiseY, bool optimiseZ) {
+    dynamic_cast<HipF3dContent&>(con).SyncVelocityGrid();
+    Check(nrh_f3d_update_velocity_field(Handle(), scale, optimiseX, optimiseY, optimiseZ),
+          "nrh_f3d_update_velocity_field");
+}
+
+void HipCompute::BchUpdate(float scale, int bchUpdateValue) {
+    Check(nrh_f3d_bch_update(Handle(), scale, bchUpdateValue), "nrh_f3d_bch_update");
+}
+
+void HipCompute::SymmetriseVelocityFields(Content& conBwIn) {
+    HipF3dContent& b = dynamic_cast<HipF3dContent&>(conBwIn);
+    dynamic_cast<HipF3dContent&>(con).SyncVelocityGrid();
+    b.SyncVelocityGrid();
+    Check(nrh_f3d_symmetrise_velocity_fields(Handle(), b.GetHandle()), "nrh_f3d_symmetrise_velocity_fields");
+}
+
+void HipCompute::DefFieldCompose(const NiftiImage& defField) {
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    c.EnsureFieldOnDevice();
+    const nrh_geom g = GeomOf(defField);
+    Check(nrh_f3d_compose_field(Handle(), &g, static_cast<const float*>(defField->data)), "nrh_f3d_compose_field");
+    c.MarkOnDevice(HipF3dContent::Field);
+}
+
+NiftiImage HipCompute::ResampleGradient(int interpolation, float padding) {
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    c.EnsureFieldOnDevice();
+    c.EnsureVoxelGradientOnDevice();
+    NiftiImage warped(c.DefContent::GetVoxelBasedMeasureGradient(), NiftiImage::Copy::ImageInfoAndAllocData);
+    Check(nrh_f3d_resample_gradient(Handle(), interpolation, padding, static_cast<float*>(warped->data)),
+          "nrh_f3d_resample_gradient");
+    return warped;
+}
 
 // ---------------------------------------------------------------- measures
 void reg_nmi_hip::InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, int *refMask, nifti_image *warpedImg,
@@ -293,12 +384,19 @@ void reg_nmi_hip::InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, in
     reg_nmi::InitialiseMeasure(refImg, floImg, refMask, warpedImg, warpedGrad, voxelBasedGrad, localWeightSim, floMask,
                                warpedImgBw, warpedGradBw, voxelBasedGradBw);
     if (!content) NR_FATAL_ERROR("reg_nmi_hip needs a HIP F3D content");
-    if (isSymmetric || referenceTimePoints != 1) NR_FATAL_ERROR("HIP NMI: one time point, non-symmetric");
+    if (referenceTimePoints != 1) NR_FATAL_ERROR("HIP NMI: one time point");
+    if (isSymmetric && !contentBw) NR_FATAL_ERROR("HIP NMI: the symmetric measure needs the backward HIP content");
     const int rb = referenceBinNumber[0], fb = floatingBinNumber[0];
     content->SetMeasure(NRH_MEASURE_NMI, rb, fb, -5.f);
     // the host images were rescaled to [2, bins - 3] by reg_nmi; the device copies follow
     Check(nrh_f3d_rescale_intensities(content->GetHandle(), 2.f, (float)rb - 3.f, 2.f, (float)fb - 3.f),
           "nrh_f3d_rescale_intensities");
+    if (isSymmetric) {
+        // the backward content's reference is the floating image (reg-lib/cpu/_reg_nmi.cpp:400-413)
+        contentBw->SetMeasure(NRH_MEASURE_NMI, fb, rb, -5.f);
+        Check(nrh_f3d_rescale_intensities(contentBw->GetHandle(), 2.f, (float)fb - 3.f, 2.f, (float)rb - 3.f),
+              "nrh_f3d_rescale_intensities");
+    }
 }
 
 double reg_nmi_hip::GetSimilarityMeasureValueFw() {
@@ -307,10 +405,27 @@ double reg_nmi_hip::GetSimilarityMeasureValueFw() {
     return timePointWeights[0] * nmi;
 }
 
+double reg_nmi_hip::GetSimilarityMeasureValueBw() {
+    double nmi = 0, ent[4];
+    Check(nrh_f3d_nmi_value(contentBw->GetHandle(), &nmi, ent), "nrh_f3d_nmi_value");
+    return timePointWeights[0] * nmi;
+}
+
 void reg_nmi_hip::GetVoxelBasedSimilarityMeasureGradientFw(int currentTimePoint) {
+    // reg_nmi::GetVoxelBasedSimilarityMeasureGradientFw (reg-lib/cpu/_reg_nmi.cpp:622-637): the
+    // joint histograms (both directions when symmetric) of the current warped images first
+    this->GetSimilarityMeasureValue();
+    content->EnsureVoxelGradientOnDevice();
     Check(nrh_f3d_nmi_voxel_gradient(content->GetHandle(), timePointWeights[currentTimePoint]),
           "nrh_f3d_nmi_voxel_gradient");
     content->MarkOnDevice(HipF3dContent::VoxelGradient);
+}
+
+void reg_nmi_hip::GetVoxelBasedSimilarityMeasureGradientBw(int currentTimePoint) {
+    contentBw->EnsureVoxelGradientOnDevice();
+    Check(nrh_f3d_nmi_voxel_gradient(contentBw->GetHandle(), timePointWeights[currentTimePoint]),
+          "nrh_f3d_nmi_voxel_gradient");
+    contentBw->MarkOnDevice(HipF3dContent::VoxelGradient);
 }
 
 void reg_lncc_hip::InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, int *refMask, nifti_image *warpedImg,
@@ -352,10 +467,14 @@ reg_measure* HipMeasureCreator::Create(const MeasureType measureType) {
 }
 
 void HipMeasureCreator::Initialise(reg_measure& measure, DefContent& con, DefContent *conBw) {
-    if (conBw) NR_FATAL_ERROR("Symmetric measures are not supported on the HIP platform");
     HipF3dContent *c = dynamic_cast<HipF3dContent*>(&con);
-    if (auto *m = dynamic_cast<reg_nmi_hip*>(&measure)) m->SetContent(c);
-    if (auto *m = dynamic_cast<reg_lncc_hip*>(&measure)) m->SetContent(c);
+    HipF3dContent *cBw = conBw ? dynamic_cast<HipF3dContent*>(conBw) : nullptr;
+    if (conBw && !cBw) NR_FATAL_ERROR("The backward content is not a HIP F3D content");
+    if (auto *m = dynamic_cast<reg_nmi_hip*>(&measure)) m->SetContent(c, cBw);
+    if (auto *m = dynamic_cast<reg_lncc_hip*>(&measure)) {
+        if (cBw) NR_FATAL_ERROR("The symmetric LNCC measure is not supported on the HIP platform");
+        m->SetContent(c);
+    }
     MeasureCreator::Initialise(measure, con, conBw);
 }
 
@@ -417,20 +536,28 @@ Kernel* HipKernelFactory::Produce(std::string name, Content *con) const {
 // ---------------------------------------------------------------- optimiser
 void HipConjugateGradient::StoreCurrentDof() {
     Check(nrh_f3d_store_best(content.GetHandle()), "nrh_f3d_store_best");
+    if (contentBw) Check(nrh_f3d_store_best(contentBw->GetHandle()), "nrh_f3d_store_best");
 }
 
 void HipConjugateGradient::RestoreBestDof() {
     Check(nrh_f3d_restore_best(content.GetHandle()), "nrh_f3d_restore_best");
+    if (contentBw) Check(nrh_f3d_restore_best(contentBw->GetHandle()), "nrh_f3d_restore_best");
 }
 
 void HipConjugateGradient::UpdateGradientValues() {
-    Check(nrh_f3d_conjugate_gradient(content.GetHandle(), this->firstCall ? 1 : 0), "nrh_f3d_conjugate_gradient");
+    if (contentBw)
+        Check(nrh_f3d2_conjugate_gradient(content.GetHandle(), contentBw->GetHandle(), this->firstCall ? 1 : 0),
+              "nrh_f3d2_conjugate_gradient");
+    else
+        Check(nrh_f3d_conjugate_gradient(content.GetHandle(), this->firstCall ? 1 : 0), "nrh_f3d_conjugate_gradient");
     this->firstCall = false;
 }
 
 void HipConjugateGradient::Perturbation(float length) {
     content.GetControlPointGrid();  // the host arrays the optimiser was initialised with
+    if (contentBw) contentBw->GetControlPointGrid();
     NiftyReg::ConjugateGradient<float>::Perturbation(length);
     content.UpdateControlPointGrid();
+    if (contentBw) contentBw->UpdateControlPointGrid();
     StoreCurrentDof();
 }

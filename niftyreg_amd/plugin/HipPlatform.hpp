@@ -16,6 +16,7 @@
 #include "ComputeFactory.h"
 #include "ContentCreatorFactory.h"
 #include "ConvolutionKernel.h"
+#include "F3d2ContentCreator.h"
 #include "F3dContent.h"
 #include "F3dContentCreator.h"
 #include "KernelFactory.h"
@@ -55,8 +56,20 @@ public:
     virtual void UpdateTransformationGradient() override;
     virtual void ZeroTransformationGradient() override;
     virtual void ZeroVoxelBasedMeasureGradient() override;
+    virtual void SetDeformationField(NiftiImage&& deformationFieldIn) override;
+    virtual void UpdateDeformationField() override;
+    virtual void UpdateVoxelBasedMeasureGradient() override;
 
     nrh_f3d* GetHandle() { return handle; }
+    // The host grid's header (intent_p1 / intent_p2 of a velocity grid) without downloading the
+    // control points.
+    nifti_image* GridHeader() { return controlPointGrid; }
+    // Marks the device control points as a velocity grid with the host header's step number
+    // (reg_f3d2 keeps intent_p2 in the header; fails for grids with affine extensions).
+    void SyncVelocityGrid();
+    // Upload the host field / voxel gradient unless the device already holds the current one.
+    void EnsureFieldOnDevice();
+    void EnsureVoxelGradientOnDevice();
     // Called by the measures once their bin numbers / rescaling are known.
     void SetMeasure(int measure, int refBins, int floBins, float lnccSigma);
     // Device buffers that hold data (set by HipCompute and the measures); the accessors only
@@ -95,6 +108,23 @@ public:
 
 private:
     nrh_aladin *handle = nullptr;
+};
+
+// reg_f3d2's forward / backward pair (reg-lib/F3d2ContentCreator.h:6-21) as HIP contents.
+class HipF3d2ContentCreator: public F3d2ContentCreator {
+public:
+    virtual std::pair<F3dContent*, F3dContent*> Create(NiftiImage& reference, NiftiImage& floating,
+                                                       NiftiImage& controlPointGrid, NiftiImage& controlPointGridBw,
+                                                       NiftiImage *localWeightSim = nullptr, int *referenceMask = nullptr,
+                                                       int *floatingMask = nullptr, mat44 *transformationMatrix = nullptr,
+                                                       mat44 *transformationMatrixBw = nullptr,
+                                                       size_t bytes = sizeof(float)) override {
+        auto con = new HipF3dContent(reference, floating, controlPointGrid, localWeightSim, referenceMask,
+                                     transformationMatrix, bytes);
+        auto conBw = new HipF3dContent(floating, reference, controlPointGridBw, nullptr, floatingMask,
+                                       transformationMatrixBw, bytes);
+        return { con, conBw };
+    }
 };
 
 class HipAladinContentCreator: public AladinContentCreator {
@@ -145,6 +175,8 @@ public:
     virtual void UpdateVelocityField(float scale, bool optimiseX, bool optimiseY, bool optimiseZ) override;
     virtual void BchUpdate(float scale, int bchUpdateValue) override;
     virtual void SymmetriseVelocityFields(Content& conBw) override;
+    virtual void DefFieldCompose(const NiftiImage& defField) override;
+    virtual NiftiImage ResampleGradient(int interpolation, float padding) override;
 
 protected:
     virtual void VoxelCentricToNodeCentric(float weight) override;
@@ -161,17 +193,20 @@ public:
 // ---------------------------------------------------------------- Measures
 class reg_nmi_hip: public reg_nmi {
 public:
-    void SetContent(HipF3dContent *c) { content = c; }
+    // cBw: the backward content of a symmetric (reg_f3d2) registration
+    void SetContent(HipF3dContent *c, HipF3dContent *cBw = nullptr) { content = c; contentBw = cBw; }
     virtual void InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, int *refMask, nifti_image *warpedImg,
                                    nifti_image *warpedGrad, nifti_image *voxelBasedGrad,
                                    nifti_image *localWeightSim = nullptr, int *floMask = nullptr,
                                    nifti_image *warpedImgBw = nullptr, nifti_image *warpedGradBw = nullptr,
                                    nifti_image *voxelBasedGradBw = nullptr) override;
     virtual double GetSimilarityMeasureValueFw() override;
+    virtual double GetSimilarityMeasureValueBw() override;
     virtual void GetVoxelBasedSimilarityMeasureGradientFw(int currentTimePoint) override;
+    virtual void GetVoxelBasedSimilarityMeasureGradientBw(int currentTimePoint) override;
 
 private:
-    HipF3dContent *content = nullptr;
+    HipF3dContent *content = nullptr, *contentBw = nullptr;
 };
 
 class reg_lncc_hip: public reg_lncc {
@@ -252,9 +287,11 @@ public:
 // ---------------------------------------------------------------- Optimiser
 // Conjugate gradient with the current / best / gradient arrays and the two CG arrays resident
 // in the F3D handle (the HIP counterpart of CudaConjugateGradient, reg-lib/cuda/CudaOptimiser.hpp).
+// With a backward content (reg_f3d2) both directions' arrays take part, as the reference's
+// symmetric ConjugateGradient (reg-lib/Optimiser.cpp:231-310).
 class HipConjugateGradient: public NiftyReg::ConjugateGradient<float> {
 public:
-    explicit HipConjugateGradient(HipF3dContent& con): content(con) {}
+    explicit HipConjugateGradient(HipF3dContent& con, HipF3dContent *conBw = nullptr): content(con), contentBw(conBw) {}
     virtual void StoreCurrentDof() override;
     virtual void RestoreBestDof() override;
     virtual void Perturbation(float length) override;
@@ -265,4 +302,5 @@ protected:
 
 private:
     HipF3dContent& content;
+    HipF3dContent *contentBw;
 };

@@ -73,4 +73,78 @@ int plugin_f3d_eval(int hip, const nr_geom *ref, const float *refData, const nr_
     }
 }
 
+// The calls reg_f3d2 makes for one objective value and one gradient (reg-lib/_reg_f3d2.cpp:139-172,
+// 216-232, 246-301, 333-342, 417-440, 540-560, 597-610) with NMI, through either platform: both
+// velocity grids exponentiated (the forward step number updated, the backward one copied),
+// both warps, symmetric NMI, both voxel gradients exponentiated along the opposite grid, both
+// convolved and projected, penalty gradients of both grids.  cppFw / cppBw are updated in
+// place (the bending-energy gradient rounds them through displacements); steps[0..1] receive
+// the grids' intent_p2.  measure = NMI_fw + NMI_bw; be = BE_fw + BE_bw (+ 1000 (LE_fw + LE_bw)).
+int plugin_f3d2_eval(int hip, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                     const nr_geom *cpp, float *cppFw, float *cppBw, int bins, float simW, float beW, float leW,
+                     double *measure, double *be, float *gradFw, float *gradBw, int *steps) {
+    try {
+        NiftiImage r = make_image(ref, refData), f = make_image(flo, floData);
+        NiftiImage g = make_grid(cpp, cppFw), gb = make_grid(cpp, cppBw);
+        g->intent_p1 = gb->intent_p1 = SPLINE_VEL_GRID;
+        g->intent_p2 = gb->intent_p2 = 6;
+        std::unique_ptr<F3d2ContentCreator> cc(hip ? new HipF3d2ContentCreator() : new F3d2ContentCreator());
+        auto pair = cc->Create(r, f, g, gb);
+        std::unique_ptr<F3dContent> con(pair.first), conBw(pair.second);
+        std::unique_ptr<ComputeFactory> cf(hip ? new HipComputeFactory() : new ComputeFactory());
+        std::unique_ptr<Compute> comp(cf->Produce(*con)), compBw(cf->Produce(*conBw));
+        std::unique_ptr<MeasureCreator> mc(hip ? new HipMeasureCreator() : new MeasureCreator());
+        std::unique_ptr<reg_measure> m(mc->Create(MeasureType::Nmi));
+        dynamic_cast<reg_nmi&>(*m).SetReferenceBinNumber(bins + 4, 0);
+        dynamic_cast<reg_nmi&>(*m).SetFloatingBinNumber(bins + 4, 0);
+        m->SetTimePointWeight(0, 1.0);
+        mc->Initialise(*m, *con, conBw.get());
+        const float pad = std::numeric_limits<float>::quiet_NaN();
+        auto warp = [&]() {
+            comp->GetDefFieldFromVelocityGrid(true);
+            gb->intent_p2 = g->intent_p2;
+            compBw->GetDefFieldFromVelocityGrid(false);
+            comp->ResampleImage(1, pad);
+            compBw->ResampleImage(1, pad);
+        };
+        // objective value
+        double b = comp->ApproxBendingEnergy() + compBw->ApproxBendingEnergy();
+        if (leW > 0) b += 1000.0 * (comp->ApproxLinearEnergy() + compBw->ApproxLinearEnergy());
+        *be = b;
+        warp();
+        *measure = m->GetSimilarityMeasureValue();
+        // objective gradient
+        warp();
+        con->ZeroVoxelBasedMeasureGradient();
+        conBw->ZeroVoxelBasedMeasureGradient();
+        comp->GetImageGradient(1, pad, 0);
+        compBw->GetImageGradient(1, pad, 0);
+        m->GetVoxelBasedSimilarityMeasureGradient(0);
+        comp->ExponentiateGradient(*conBw);
+        compBw->ExponentiateGradient(*con);
+        comp->ConvolveVoxelBasedMeasureGradient(simW);
+        compBw->ConvolveVoxelBasedMeasureGradient(simW);
+        comp->ApproxBendingEnergyGradient(beW);
+        compBw->ApproxBendingEnergyGradient(beW);
+        if (leW > 0) {
+            comp->ApproxLinearEnergyGradient(leW);
+            compBw->ApproxLinearEnergyGradient(leW);
+        }
+        NiftiImage& tg = con->GetTransformationGradient();
+        NiftiImage& tb = conBw->GetTransformationGradient();
+        std::memcpy(gradFw, tg->data, tg.nVoxels() * sizeof(float));
+        std::memcpy(gradBw, tb->data, tb.nVoxels() * sizeof(float));
+        NiftiImage& cf2 = con->GetControlPointGrid();
+        NiftiImage& cb2 = conBw->GetControlPointGrid();
+        std::memcpy(cppFw, cf2->data, cf2.nVoxels() * sizeof(float));
+        std::memcpy(cppBw, cb2->data, cb2.nVoxels() * sizeof(float));
+        steps[0] = (int)g->intent_p2;
+        steps[1] = (int)gb->intent_p2;
+        return 0;
+    } catch (const std::exception& e) {
+        plugin_error = e.what();
+        return 1;
+    }
+}
+
 }  // extern "C"

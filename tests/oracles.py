@@ -873,3 +873,26 @@ class RefF3d2:
             self.close()
         except Exception:
             pass
+
+
+def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, simw=0.989, bew=0.001, lew=0.01):
+    """One reg_f3d2 objective value + gradient through the reference's own Content / Compute /
+    Measure calls (forward and backward contents), on the reference CPU classes (hip=False) or
+    the HIP platform plugin (hip=True).  Returns (measure, be, grad_fw, grad_bw, cpp_fw, cpp_bw, steps)."""
+    global _plugin
+    if _plugin is None:
+        _plugin = ctypes.CDLL(PLUGIN)
+        _plugin.plugin_last_error.restype = ctypes.c_char_p
+    m, be = ctypes.c_double(), ctypes.c_double()
+    f = np.ascontiguousarray(fw, np.float32).copy()
+    b = np.ascontiguousarray(bw, np.float32).copy()
+    gf = np.zeros(3 * cppg.nvox, np.float32)
+    gb = np.zeros(3 * cppg.nvox, np.float32)
+    steps = np.zeros(2, np.int32)
+    rc = _plugin.plugin_f3d2_eval(int(hip), ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).ravel()),
+                                  ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).ravel()),
+                                  ctypes.byref(rg(cppg)), fp(f), fp(b), bins, ctypes.c_float(simw), ctypes.c_float(bew),
+                                  ctypes.c_float(lew), ctypes.byref(m), ctypes.byref(be), fp(gf), fp(gb), ip(steps))
+    if rc != 0:
+        raise RuntimeError(_plugin.plugin_last_error().decode())
+    return m.value, be.value, gf, gb, f, b, (int(steps[0]), int(steps[1]))

@@ -6,9 +6,13 @@ grids, symmetric NMI, bending and linear energy of both grids.  The device runs 
 (scaling-and-squaring exponentiation of both grids, both warps, both histograms, voxel
 gradients exponentiated along the opposite grid, convolution and node projection, penalty
 gradients, the shared normalisation) with each direction on its own stream.  Every kernel on
-that chain follows the scalar reference's arithmetic, so the normalised node gradients of both
-directions are compared bit for bit; the objective's terms are FP64 sums whose order differs
-from the reference's OpenMP loops (1e-12 relative).
+that chain follows the scalar reference's arithmetic; the objective's terms are FP64 sums whose
+order differs from the reference's OpenMP loops (1e-12 relative).  Where the grid's nodes lie on
+voxels at an integer spacing, the convolution + node extraction runs as the separable node
+projection (the same weights summed in another order): the normalised gradients then agree to
+1e-5 relative (L2) and the maximal length to 1e-6; otherwise (an oblique reference, a grid
+offset by a fraction of a voxel) the reference-order convolution runs and both directions'
+gradients are compared bit for bit.
 """
 import numpy as np
 import pytest
@@ -55,6 +59,20 @@ def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5):
     return r, F3d2Pair(hf, hb, steps=6)
 
 
+def _check_gradients(pair, rgf, rgb, rmax, m, exact):
+    gf, gb = pair.fw.node_gradient(), pair.bw.node_gradient()
+    assert pair.fw.fused_available() == (not exact) and pair.bw.fused_available() == (not exact)
+    if exact:
+        assert m == rmax
+        assert np.array_equal(gf, rgf), np.count_nonzero(gf != rgf)
+        assert np.array_equal(gb, rgb), np.count_nonzero(gb != rgb)
+    else:
+        assert abs(m - rmax) <= 1e-6 * rmax
+        for g, rg in ((gf, rgf), (gb, rgb)):
+            rel = np.linalg.norm(g.astype(np.float64) - rg) / np.linalg.norm(rg)
+            assert rel <= 1e-5, rel
+
+
 @pytest.mark.parametrize("which", ["identity", "oblique"])
 def test_symmetric_objective_and_gradient_match_reference(gpu, which):
     r, pair = _session(which)
@@ -65,10 +83,7 @@ def test_symmetric_objective_and_gradient_match_reference(gpu, which):
     assert abs(v - rv) <= 1e-12 * abs(rv)
     rgf, rgb, rmax = r.gradient(normalise=True)
     m = pair.gradient(normalise=True)
-    assert m == rmax
-    gf, gb = pair.fw.node_gradient(), pair.bw.node_gradient()
-    assert np.array_equal(gf, rgf), np.count_nonzero(gf != rgf)
-    assert np.array_equal(gb, rgb), np.count_nonzero(gb != rgb)
+    _check_gradients(pair, rgf, rgb, rmax, m, exact=(which == "oblique"))
     # the bending-energy gradient's displacement round trip reaches both device grids too
     f, b, steps = r.get_cpp()
     assert np.array_equal(pair.fw.get_control_points(), f) and np.array_equal(pair.bw.get_control_points(), b)
@@ -86,7 +101,6 @@ def test_large_velocities_update_the_step_number(gpu):
     assert abs(v - rv) <= 1e-12 * abs(rv)
     rgf, rgb, rmax = r.gradient(normalise=True)
     m = pair.gradient(normalise=True)
-    assert m == rmax
-    assert np.array_equal(pair.fw.node_gradient(), rgf) and np.array_equal(pair.bw.node_gradient(), rgb)
+    _check_gradients(pair, rgf, rgb, rmax, m, exact=False)
     r.close()
     pair.close()

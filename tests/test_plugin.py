@@ -62,3 +62,56 @@ def test_hip_platform_matches_reference_platform(gpu, lncc, which):
     assert abs(hbe - cbe) <= 1e-12 * abs(cbe)
     assert np.array_equal(hvg, cvg, equal_nan=True)
     assert np.array_equal(hgrad, cgrad)
+
+
+def _f3d2_case(which, seed=2, amplitude=2.0):
+    from niftyreg_amd.geometry import create_symmetric_control_point_grids
+    if which == "cube":
+        rg_ = fg = Geom.identity((36, 34, 30))
+    else:
+        rg_, fg = _oblique((34, 30, 28)), Geom.identity((32, 34, 26), (1.1, 1.0, 1.2))
+    ref = phantoms.blob_phantom(rg_.dims, seed).ravel()
+    flo = phantoms.blob_phantom(fg.dims, seed + 3).ravel()
+    cg, fw, bw = create_symmetric_control_point_grids(rg_, fg, [5.0, 5.0, 5.0])
+    rng = np.random.default_rng(seed)
+    fw = (fw + rng.uniform(-amplitude, amplitude, fw.size)).astype(np.float32)
+    bw = (bw + rng.uniform(-amplitude, amplitude, bw.size)).astype(np.float32)
+    return rg_, fg, ref, flo, cg, fw, bw
+
+
+@needs_plugin
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["cube", "oblique"])
+def test_hip_platform_symmetric_matches_reference_platform(gpu, which):
+    """reg_f3d2's calls (velocity exponentiation, symmetric NMI, gradient exponentiation along the
+    opposite grid, both directions' convolution, penalties) through both platforms."""
+    rg_, fg, ref, flo, cg, fw, bw = _f3d2_case(which)
+    c = O.plugin_f3d2_eval(False, rg_, ref, fg, flo, cg, fw, bw)
+    h = O.plugin_f3d2_eval(True, rg_, ref, fg, flo, cg, fw, bw)
+    assert abs(h[0] - c[0]) <= 1e-12 * abs(c[0])
+    assert abs(h[1] - c[1]) <= 1e-12 * abs(c[1])
+    assert h[6] == c[6]
+    for k in (2, 3, 4, 5):
+        assert np.array_equal(h[k], c[k]), (k, np.count_nonzero(h[k] != c[k]))
+
+
+@needs_plugin
+@pytest.mark.parametrize("which", ["cube", "oblique"])
+def test_symmetric_check_sequence_matches_reg_f3d2(which):
+    """The plugin check's call sequence on the reference CPU classes equals reg_f3d2 itself (one
+    level, -be 0.001 -le 0.01): the same transformation gradients of both directions, the same
+    rounded grids and step numbers, the similarity term to FP64 order."""
+    rg_, fg, ref, flo, cg, fw, bw = _f3d2_case(which)
+    r = O.RefF3d2(rg_, ref, fg, flo, be=0.001, le=0.01, spacing=5.0, levels=1)
+    assert O.geom_from_nr(r.cpp_geom).dims == cg.dims
+    simw, bew, lew = r.weights()
+    r.set_cpp(fw, bw)
+    rv, rterms = r.value()
+    rgf, rgb, _ = r.gradient(normalise=False)
+    rf, rb, rsteps = r.get_cpp()
+    r.close()
+    c = O.plugin_f3d2_eval(False, rg_, ref, fg, flo, cg, fw, bw, simw=simw, bew=bew, lew=lew)
+    assert abs(simw * c[0] - rterms[0]) <= 1e-12 * abs(rterms[0])
+    assert c[6] == rsteps
+    assert np.array_equal(c[2], rgf) and np.array_equal(c[3], rgb)
+    assert np.array_equal(c[4], rf) and np.array_equal(c[5], rb)
