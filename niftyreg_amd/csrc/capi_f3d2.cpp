@@ -38,24 +38,22 @@ void join(nrh_f3d *a, nrh_f3d *b) {
     NRH_HIP(hipStreamWaitEvent(b->stream, a->evPair, 0));
 }
 
-// reg_f3d2::GetDeformationField + WarpFloatingImage (reg-lib/_reg_f3d2.cpp:139-172): forward
-// exponentiation (step number updated: the optimiser exists), the backward one with the forward
-// step count, both resamplings.
-void warp_both(nrh_f3d *fw, nrh_f3d *bw) {
+// reg_f3d2::GetDeformationField + WarpFloatingImage (reg-lib/_reg_f3d2.cpp:139-172) and the
+// symmetric NMI's joint histograms (reg-lib/cpu/_reg_measure.h:63-80): forward exponentiation
+// (step number updated: the optimiser exists), the backward one with the forward step count,
+// both resamplings and histograms.  A gradient at the control points of the last objective
+// finds all of it in place: the reference recomputes the same values.
+void warp_and_histograms(nrh_f3d *fw, nrh_f3d *bw) {
     op_vel_def_field(fw, true);
     bw->velSteps = fw->velSteps;
     op_vel_def_field(bw, false);
-    op_resample(fw);
-    op_resample(bw);
-}
-
-// reg_nmi::GetSimilarityMeasureValue with isSymmetric (reg-lib/cpu/_reg_measure.h:63-80):
-// the forward and the backward joint histograms and entropies.
-void histograms_both(nrh_f3d *fw, nrh_f3d *bw) {
     for (nrh_f3d *h : { fw, bw }) {
+        if (h->pairStateVersion == h->cppVersion) continue;
+        op_resample(h);
         op_joint_histogram(h, h->warped.p);
         op_nmi_finalize(h);
         h->histVersion = h->cppVersion;
+        h->pairStateVersion = h->cppVersion;
     }
 }
 
@@ -74,10 +72,7 @@ int nrh_f3d2_objective(nrh_f3d *fw, nrh_f3d *bw, double *value, double terms[3])
         check_pair(fw, bw);
         NRH_REQUIRE(value, NRH_ERR_ARG, "null argument");
         const nrh_f3d_params& p = fw->params;
-        if (p.similarity_weight > 0) {
-            warp_both(fw, bw);
-            histograms_both(fw, bw);
-        }
+        if (p.similarity_weight > 0) warp_and_histograms(fw, bw);
         for (nrh_f3d *h : { fw, bw }) {
             if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
             if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
@@ -107,17 +102,18 @@ int nrh_f3d2_gradient(nrh_f3d *fw, nrh_f3d *bw, double *max_length) {
         const nrh_f3d_params& p = fw->params;
         if (p.similarity_weight > 0) {
             // reg_f3d2::GetVoxelBasedGradient (reg-lib/_reg_f3d2.cpp:246-301)
-            warp_both(fw, bw);
+            warp_and_histograms(fw, bw);
             for (nrh_f3d *h : { fw, bw }) {
                 ensure_voxel_grad(h);
                 op_fill_f32(h->voxelGrad.p, 3 * h->N, 0.f, h->stream);
                 op_image_gradient(h);
             }
-            histograms_both(fw, bw);
             op_nmi_voxel_gradient(fw, 1.0);
             op_nmi_voxel_gradient(bw, 1.0);
             // ExponentiateGradient (reg-lib/_reg_f3d2.cpp:597-610): each direction reads the
-            // other's grid, which no stream changes before the join below
+            // other's grid (unchanged until the join before the penalty gradients) and, on a
+            // shared geometry, the other's squaring chain, written on the other stream
+            join(fw, bw);
             op_exponentiate_gradient(fw, bw, false);
             op_exponentiate_gradient(bw, fw, false);
             // reg_f3d2::GetSimilarityMeasureGradient (:333-342): convolution and node extraction,

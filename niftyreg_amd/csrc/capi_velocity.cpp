@@ -182,6 +182,7 @@ int nrh_f3d_upload_field(nrh_f3d *h, const float *planar) {
         NRH_REQUIRE(h && planar, NRH_ERR_ARG, "null argument");
         ensure_field(h);
         h->velFieldVersion = 0;
+        h->pairStateVersion = 0;
         NRH_HIP(hipMemcpyAsync(h->field.p, planar, 3 * h->N * sizeof(float), hipMemcpyHostToDevice, h->stream));
         NRH_HIP(hipStreamSynchronize(h->stream));
     });

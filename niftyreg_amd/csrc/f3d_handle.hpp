@@ -58,6 +58,11 @@ struct nrh_f3d {
     uint64_t velFieldVersion = 0;
     int velFieldSteps = 0;
     bool velFieldUpdated = false;
+    nrh::DevBuf<float> velChain;      // the squaring's fields before the last, [k][3][N]
+    uint64_t velChainVersion = 0;     // cppVersion of velChain (0: stale)
+    // cppVersion at which the symmetric evaluation left the warped image and the joint histogram
+    // of the current exponentiation (0: some other operation changed them since)
+    uint64_t pairStateVersion = 0;
 
     double *pinned = nullptr;  // host-pinned scalar read-back
     double *published = nullptr, *publishedDev = nullptr;  // coherent host memory the device writes results to

@@ -1059,6 +1059,7 @@ void op_spline_field(nrh_f3d *h) {
         NRH_REQUIRE(pre + 3 < gd[i], NRH_ERR_UNSUPPORTED, "control point grid does not cover the reference image");
     }
     h->velFieldVersion = 0;  // the field no longer holds a velocity exponentiation
+    h->pairStateVersion = 0;
     hipLaunchKernelGGL(k_spline_field, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, h->ref.nx, h->ref.ny,
                        h->ref.nz, h->cpp.nx, h->cpp.ny, h->G, h->cppPos.p, h->mask.p, g[0], g[1], g[2], lut,
                        h->field.p, h->N);
@@ -1066,6 +1067,7 @@ void op_spline_field(nrh_f3d *h) {
 }
 
 void op_resample(nrh_f3d *h) {
+    h->pairStateVersion = 0;
     ensure_warped(h);
     NRH_REQUIRE(!h->field.empty(), NRH_ERR_STATE, "resample before the deformation field");
     NRH_REQUIRE(!h->floImg.empty(), NRH_ERR_UNSUPPORTED, "per-operation resampling needs float32 image storage");
@@ -1086,6 +1088,7 @@ void op_image_gradient(nrh_f3d *h) {
 }
 
 void op_joint_histogram(nrh_f3d *h, const float *warped) {
+    h->pairStateVersion = 0;
     const int rb = h->params.ref_bins, fb = h->params.flo_bins;
     op_fill_u32(h->histCount.p, (size_t)rb * fb, 0u, h->stream);
     const unsigned b = std::min(blocks_for(h->N), 2048u);

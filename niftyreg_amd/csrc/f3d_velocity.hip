@@ -445,6 +445,9 @@ __global__ void __launch_bounds__(kBlock) k_resample_gradient(ResampleGradArgs r
                     bx1 = -bx1;
                 }
                 const float w0 = bx1 * by0 * bz0, w1 = bx0 * by1 * bz0, w2 = bx0 * by0 * bz1;
+                // half of the eight pairs carry three zero weights; their terms add zeros, which
+                // leave the sums unchanged, so their loads are skipped
+                if (w0 == 0.f && w1 == 0.f && w2 == 0.f) continue;
                 // fields with one voxel along an axis would step to index -1: clamp the read
                 // (its weight is then zero on that axis in the reference as well)
                 const size_t idx = ((size_t)clampi(Z, r.nz) * r.ny + clampi(Y, r.ny)) * r.nx + clampi(X, r.nx);
@@ -614,14 +617,16 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
         }
     }
     h->velFieldVersion = 0;
+    h->velChainVersion = 0;
+    h->pairStateVersion = 0;
     ensure_field(h);
-    h->velTmp.alloc(3 * h->N);
     const FieldGeom f = field_geom(h->ref);
     const VelGrid g = vel_grid(h);
     int steps = h->velSteps;
     const bool chosen = update || steps == 0;
     int sq;
-    if (update || steps == 0) {
+    if (chosen) {
+        h->velTmp.alloc(3 * h->N);
         const uint32_t init[2] = { 0xffffffffu, 0u };
         h->velKeys.alloc(2);
         NRH_HIP(hipMemcpyAsync(h->velKeys.p, init, sizeof init, hipMemcpyHostToDevice, h->stream));
@@ -633,35 +638,29 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
         NRH_HIP(hipStreamSynchronize(h->stream));
         sq = squaring_number(key_to_float(keys[0]), key_to_float(keys[1]), h->ref.nz > 1);
         steps = steps >= 0 ? sq : -sq;
-        // the scaled flow goes where the last squaring must not land: the field after sq swaps
-        float *first = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
-        if (first == h->velTmp.p) {
-            // in place is safe: each thread reads and writes only its own voxel
-            hipLaunchKernelGGL(k_vel_scale, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->velTmp.p, f,
-                               scaling_of(steps), h->velTmp.p);
-        } else {
-            hipLaunchKernelGGL(k_vel_scale, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->velTmp.p, f,
-                               scaling_of(steps), h->field.p);
-        }
-        NRH_HIP(hipGetLastError());
     } else {
         sq = steps < 0 ? -steps : steps;
-        float *first = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
+    }
+    // the squaring chain: fields[k] for k < sq in velChain (the intermediate fields of
+    // reg_spline_getIntermediateDefFieldFromVelGrid, which the opposite direction's gradient
+    // exponentiation reuses on a shared geometry), the last one in the deformation field
+    h->velChain.alloc((size_t)std::max(sq, 1) * 3 * f.n);
+    auto field_k = [&](int k) { return k == sq ? h->field.p : h->velChain.p + (size_t)k * 3 * f.n; };
+    if (chosen) {
+        hipLaunchKernelGGL(k_vel_scale, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->velTmp.p, f,
+                           scaling_of(steps), field_k(0));
+    } else {
         hipLaunchKernelGGL(k_vel_flow_scaled, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, scaling_of(steps),
-                           first);
-        NRH_HIP(hipGetLastError());
+                           field_k(0));
     }
-    float *src = (sq % 2 == 0) ? h->field.p : h->velTmp.p;
-    float *dst = (src == h->field.p) ? h->velTmp.p : h->field.p;
-    for (int k = 0; k < sq; ++k) {
-        launch_compose(src, f, src, f.n, dst, h->stream);
-        std::swap(src, dst);
-    }
+    NRH_HIP(hipGetLastError());
+    for (int k = 0; k < sq; ++k) launch_compose(field_k(k), f, field_k(k), f.n, field_k(k + 1), h->stream);
     NRH_HIP(hipGetLastError());
     h->velSteps = steps;
     h->velFieldVersion = h->cppVersion;
     h->velFieldSteps = steps;
     h->velFieldUpdated = chosen;
+    h->velChainVersion = sq > 0 ? h->cppVersion : 0;
 }
 
 // reg_spline_getIntermediateDefFieldFromVelGrid (reg-lib/cpu/_reg_localTrans.cpp:3616-3683) of
@@ -687,9 +686,20 @@ void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther) {
     // the backward grid's last update ran on its own stream
     if (syncOther) NRH_HIP(hipStreamSynchronize(bw->stream));
     const FieldGeom f = field_geom(h->ref);
-    h->velFields.alloc((size_t)sq * 3 * f.n);
+    // bw's own squaring chain holds these fields when both handles share the reference geometry
+    // and bw's last exponentiation used its current control points and this step count
+    const bool sameGeom = bw->ref.nx == h->ref.nx && bw->ref.ny == h->ref.ny && bw->ref.nz == h->ref.nz &&
+                          std::memcmp(bw->ref.xyz, h->ref.xyz, sizeof h->ref.xyz) == 0 &&
+                          std::memcmp(bw->ref.ijk, h->ref.ijk, sizeof h->ref.ijk) == 0;
+    const float *fields;
+    if (sameGeom && bw->velChainVersion == bw->cppVersion && bw->velFieldSteps == steps) {
+        fields = bw->velChain.p;
+    } else {
+        h->velFields.alloc((size_t)sq * 3 * f.n);
+        intermediate_fields(vel_grid(bw), f, steps, h->velFields.p, sq, h->stream);
+        fields = h->velFields.p;
+    }
     h->velTmp.alloc(3 * f.n);
-    intermediate_fields(vel_grid(bw), f, steps, h->velFields.p, sq, h->stream);
     ResampleGradArgs r{};
     r.gnx = h->ref.nx; r.gny = h->ref.ny; r.gnz = h->ref.nz;
     r.gn = f.n;
@@ -703,7 +713,7 @@ void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther) {
     for (int k = 0; k < sq; ++k) {
         r.grad = src;
         r.accum = src;
-        r.def = h->velFields.p + (size_t)k * 3 * f.n;
+        r.def = fields + (size_t)k * 3 * f.n;
         r.divisor = (k == sq - 1) ? div : 0.0;
         r.out = dst;
         hipLaunchKernelGGL(k_resample_gradient, blocks8_for(f.n), dim3(kBlock), 0, h->stream, r);
@@ -742,6 +752,7 @@ void op_resample_gradient(nrh_f3d *h, float pad, float *out) {
 void op_compose_field(nrh_f3d *h, const nrh_geom& g, const float *def) {
     NRH_REQUIRE(!h->field.empty(), NRH_ERR_STATE, "deformation field not computed");
     h->velFieldVersion = 0;
+    h->pairStateVersion = 0;
     launch_compose(def, field_geom(g), h->field.p, h->N, h->field.p, h->stream);
     NRH_HIP(hipGetLastError());
 }
