@@ -205,20 +205,18 @@ def bench_f3d2(torch, ref, flo, geom, steps, with_cpu):
     squarings = hf.velocity_steps
     pair.close()
     N = geom.nvox
-    # HBM bytes of one eval at compulsory traffic, per direction (both run): two exponentiations
-    # (objective and gradient), each the flow (12 B per voxel written) and `squarings`
-    # compositions (12 B read, 12 B written); the gradient exponentiation's intermediate fields
-    # (flow + squarings - 1 compositions) and `squarings` resamplings (field and gradient read,
-    # gradient written: 36 B); the per-operation chain: two warps (field 12 + floating 4 read,
-    # 4 written), image gradient (field 12 + floating 4 read, 12 written), two histograms (8),
-    # NMI voxel gradient (4 + 4 + 12 read, 12 written), convolution (3 axis passes of 12 + 12)
-    # and node projection (12 read).
+    # HBM bytes of one eval at compulsory traffic, per direction, for the passes that run (the
+    # gradient finds the objective's exponentiation, warped image and histogram in place, and on
+    # this shared geometry each gradient exponentiation reuses the opposite direction's squaring
+    # chain): the flow (12 B per voxel written; the forward one, whose step number is chosen from
+    # its extremes, also rescaled: +24) and `squarings` compositions (12 B read, 12 B written);
+    # the warp (field 12 + floating 4 read, 4 written) and histogram (8); the image gradient
+    # (field 12 + floating 4 read, 12 written); the voxel-gradient fill (12) and NMI voxel
+    # gradient (4 + 4 + 12 + 12 read, 12 written); `squarings` gradient resamplings (field and
+    # gradient read, gradient written: 36); the separable node projection's x pass (12 read).
     s = squarings
-    exp_bytes = 12 + 24 * s
-    grad_exp_bytes = 12 + 24 * (s - 1) + 36 * s
-    chain_bytes = 2 * 20 + 28 + 2 * 8 + 32 + 72 + 12
-    per_dir = 2 * exp_bytes + grad_exp_bytes + chain_bytes
-    eval_bytes = 2 * per_dir * N
+    per_dir = 12 + 24 * s + 20 + 8 + 28 + 12 + 44 + 36 * s + 12
+    eval_bytes = (2 * per_dir + 24) * N
     out = {
         "metric": "F3D2 (symmetric, velocity) obj+grad evals/s at 256x256x160", "value": 1.0 / step_s,
         "unit": "evals/s", "ms_per_step": step_s * 1e3, "squarings": squarings, "nmi_sum": terms[0] / (1.0 - BE),
