@@ -250,6 +250,15 @@ typedef struct nrh_f3d_config {
 int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                      const nrh_f3d_config *cfg, nrh_geom *grid_out, float *cpp_out, size_t cpp_capacity,
                      double *final_nmi, int *iterations);
+/* A whole symmetric registration: reg_f3d2's run (reg_base<T>::Run with the reg_f3d2 overrides,
+ * reg-lib/_reg_f3d2.cpp, reg-lib/_reg_base.cpp:727-819) with NMI, no affine initialisation:
+ * symmetric velocity grids at the coarsest level (spacing < 0: voxels of the mean voxel size),
+ * refined by subdivision per level; every evaluation on the device.  fw_out / bw_out (capacity
+ * floats each, may be NULL) receive the final forward / backward velocity grids, final_nmi the
+ * sum of both directions' NMI. */
+int nrh_f3d2_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
+                      const nrh_f3d_config *cfg, nrh_geom *grid_out, float *fw_out, float *bw_out, size_t capacity,
+                      double *final_nmi, int *iterations);
 
 /* ------------------------------------------------------------------ Aladin (affine) */
 /* One pyramid level of reg_aladin on the device: the HIP counterpart of AladinContent

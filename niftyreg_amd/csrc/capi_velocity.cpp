@@ -9,8 +9,8 @@ namespace nrh {
 // reg_createSymmetricControlPointGrids<float> without an affine (reg-lib/cpu/_reg_localTrans.cpp:
 // 132-358): corners of both images in mm (Mat44Mul<float>), the grid spans their bounding box
 // with one node of margin, axis-aligned, sform only.
-static void symmetric_grids(const nrh_geom& ref, const nrh_geom& flo, const float spacing[3], nrh_geom& grid,
-                            float *fw, float *bw) {
+void symmetric_grids(const nrh_geom& ref, const nrh_geom& flo, const float spacing[3], nrh_geom& grid, float *fw,
+                     float *bw) {
     auto mul = [](const float *m, const float in[3], float out[3]) {
         for (int r = 0; r < 3; ++r)
             out[r] = (float)((double)m[r * 4] * (double)in[0] + (double)m[r * 4 + 1] * (double)in[1] +

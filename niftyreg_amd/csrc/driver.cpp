@@ -128,10 +128,44 @@ void downsample(const nrh_geom& g, const std::vector<float>& in, const bool axes
 // subdivision.  Node 2x-1 of the new grid takes (1 6 1)/8 of old nodes x-1, x, x+1 and node 2x
 // takes (1 1)/2 of x, x+1 on each axis; old nodes outside the grid count as 0.  The new header is
 // the control-point grid of the level's reference image at half the spacing.
+static void subdivide(const nrh_geom& old, const std::vector<float>& oldData, const nrh_geom& grid,
+                      std::vector<float>& data);
+
 void refine_grid(const nrh_geom& ref, const nrh_geom& old, const std::vector<float>& oldData, nrh_geom& grid,
                  std::vector<float>& data) {
     const float sp[3] = { old.dx / 2.0f, old.dy / 2.0f, old.dz / 2.0f };
     create_grid(ref, sp, grid, nullptr);
+    subdivide(old, oldData, grid, data);
+}
+
+// reg_spline_refineControlPointGrid3D without a reference image (reg-lib/cpu/_reg_localTrans.cpp:
+// 1825-1829, 2218-2236), as reg_f3d2 refines its velocity grids: (n - 3) * 2 + 3 nodes per axis,
+// the sform's columns halved and its origin moved to the new node (1, 1, 1) of the halved map.
+void refine_grid_self(const nrh_geom& old, const std::vector<float>& oldData, nrh_geom& grid,
+                      std::vector<float>& data) {
+    grid = old;
+    grid.nx = (old.nx - 3) * 2 + 3;
+    grid.ny = (old.ny - 3) * 2 + 3;
+    grid.nz = (old.nz - 3) * 2 + 3;
+    grid.dx = old.dx / 2.0f;
+    grid.dy = old.dy / 2.0f;
+    grid.dz = old.dz / 2.0f;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            if (r < 2 || old.nz > 1) grid.xyz[r * 4 + c] = old.xyz[r * 4 + c] / 2.f;
+    for (int r = 0; r < 3; ++r) {
+        const double o = (double)grid.xyz[r * 4] + (double)grid.xyz[r * 4 + 1] + (double)grid.xyz[r * 4 + 2] +
+                         (double)grid.xyz[r * 4 + 3];
+        if (r < 2 || old.nz > 1) grid.xyz[r * 4 + 3] = (float)o;
+    }
+    mat44_inverse(grid.xyz, grid.ijk);
+    subdivide(old, oldData, grid, data);
+}
+
+// The cubic B-spline subdivision of reg_spline_refineControlPointGrid3D's data loops onto the
+// new grid `grid` (reg-lib/cpu/_reg_localTrans.cpp:1838-2110).
+static void subdivide(const nrh_geom& old, const std::vector<float>& oldData, const nrh_geom& grid,
+                      std::vector<float>& data) {
     const int on[3] = { old.nx, old.ny, old.nz };
     const size_t OG = voxels(old), NG = voxels(grid);
     data.assign(3 * NG, 0.f);
@@ -353,6 +387,172 @@ void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& f
     cppOut = cpp;
 }
 
+
+// reg_f3d2's run (reg_base<T>::Run with the reg_f3d2 overrides, reg-lib/_reg_f3d2.cpp): symmetric
+// velocity grids over both images at the coarsest level, refined by subdivision at each level,
+// the conjugate-gradient / line-search loop over both grids with the velocity update and the
+// symmetrisation after every step; every evaluation runs on the device through the pair.
+void register_pair_sym(const nrh_geom& refG, const float *refData, const nrh_geom& floG, const float *floData,
+                       const nrh_f3d_config& cfg, nrh_geom& gridOut, std::vector<float>& fwOut,
+                       std::vector<float>& bwOut, double& finalNmi, int& totalIterations) {
+    NRH_REQUIRE(cfg.levels >= 1 && cfg.bins >= 1 && cfg.max_iterations >= 0, NRH_ERR_ARG, "invalid configuration");
+    NRH_REQUIRE(cfg.measure == NRH_MEASURE_NMI, NRH_ERR_UNSUPPORTED, "the symmetric registration supports NMI");
+    NRH_REQUIRE(refG.nz > 1 && floG.nz > 1, NRH_ERR_UNSUPPORTED, "the symmetric registration is 3-D only");
+    const int levels = cfg.levels;
+    int toPerform = cfg.levels_to_perform > 0 ? std::min(cfg.levels_to_perform, levels) : levels;
+    if (!cfg.use_pyramid) toPerform = levels;
+    // reg_f3d2::CheckParameters (reg-lib/_reg_f3d2.cpp:118-136); no inverse-consistency term
+    float be = cfg.be_weight, le = cfg.le_weight;
+    const float penalty = be + le;
+    float simW;
+    if (penalty >= 1) {
+        simW = 0;
+        be /= penalty;
+        le /= penalty;
+    } else {
+        simW = 1 - penalty;
+    }
+    std::vector<Level> rp, fp;
+    if (cfg.use_pyramid) {
+        rp = pyramid(refG, refData, levels, toPerform);
+        fp = pyramid(floG, floData, levels, toPerform);
+    } else {
+        rp.assign(1, Level{ refG, std::vector<float>(refData, refData + voxels(refG)) });
+        fp.assign(1, Level{ floG, std::vector<float>(floData, floData + voxels(floG)) });
+    }
+    // reg_f3d2::Initialise (reg-lib/_reg_f3d2.cpp:536-590): spacing in voxels of the mean voxel size
+    const float pd[3][2] = { { refG.dx, floG.dx }, { refG.dy, floG.dy }, { refG.dz, floG.dz } };
+    float gsp[3];
+    for (int a = 0; a < 3; ++a) {
+        gsp[a] = cfg.spacing;
+        if (gsp[a] < 0) gsp[a] *= -(pd[a][0] + pd[a][1]) / 2.f;
+        gsp[a] *= std::pow(2.f, (float)(levels - 1));
+    }
+    nrh_geom grid;
+    std::vector<float> fw, bw;
+    symmetric_grids(rp[0].geom, fp[0].geom, gsp, grid, nullptr, nullptr);
+    fw.resize(3 * voxels(grid));
+    bw.resize(3 * voxels(grid));
+    symmetric_grids(rp[0].geom, fp[0].geom, gsp, grid, fw.data(), bw.data());
+    int steps = 6;  // intent_p2 of both grids
+    size_t maxIter = (size_t)cfg.max_iterations * (size_t)std::pow(2.0, toPerform - 1);
+    totalIterations = 0;
+    finalNmi = 0;
+    for (int level = 0; level < toPerform; ++level) {
+        const int idx = cfg.use_pyramid ? level : 0;
+        const nrh_geom& lr = rp[idx].geom;
+        const nrh_geom& lf = fp[idx].geom;
+        // reg_f3d2::InitCurrentLevel (reg-lib/_reg_f3d2.cpp:55-100): the largest voxel size of both
+        float maxStep = std::max(lr.dx, lr.dy);
+        maxStep = std::max(maxStep, lf.dx);
+        maxStep = std::max(maxStep, lf.dy);
+        maxStep = std::max(maxStep, lr.dz);
+        maxStep = std::max(maxStep, lf.dz);
+        if (level == 0) {
+            be = be / std::pow(16.f, (float)(levels - 1));
+            le = le / std::pow(3.f, (float)(levels - 1));
+        } else {
+            be = be * 16;
+            le = le * 3;
+            nrh_geom ng;
+            std::vector<float> nf, nb;
+            refine_grid_self(grid, fw, ng, nf);
+            refine_grid_self(grid, bw, ng, nb);
+            grid = ng;
+            fw.swap(nf);
+            bw.swap(nb);
+        }
+        nrh_f3d_params p{};
+        p.ref_bins = cfg.bins + 4;
+        p.flo_bins = cfg.bins + 4;
+        p.padding = cfg.padding;
+        p.similarity_weight = simW;
+        p.be_weight = be;
+        p.le_weight = le;
+        p.image_storage = 0;
+        p.measure = NRH_MEASURE_NMI;
+        p.lncc_sigma = cfg.lncc_sigma;
+        Handle F, B;
+        check(nrh_f3d_create(&lr, rp[idx].data.data(), &lf, fp[idx].data.data(), nullptr, &grid, &p, &F.h));
+        check(nrh_f3d_create(&lf, fp[idx].data.data(), &lr, rp[idx].data.data(), nullptr, &grid, &p, &B.h));
+        nrh_f3d *hf = F.h, *hb = B.h;
+        check(nrh_f3d_set_control_points(hf, fw.data()));
+        check(nrh_f3d_set_control_points(hb, bw.data()));
+        check(nrh_f3d_set_velocity_grid(hf, steps));
+        check(nrh_f3d_set_velocity_grid(hb, steps));
+        check(nrh_f3d_rescale_intensities(hf, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
+        check(nrh_f3d_rescale_intensities(hb, 2.f, p.flo_bins - 3.f, 2.f, p.ref_bins - 3.f));
+        check(nrh_f3d_store_best(hf));
+        check(nrh_f3d_store_best(hb));
+        double best = 0, terms[3];
+        check(nrh_f3d2_objective(hf, hb, &best, terms));
+        size_t iter = 0;
+        const float smallest = maxStep / 100.f;
+        float currentSize = maxStep;
+        bool firstCall = true, justRestarted = false;
+        // reg_f3d2::UpdateParameters (reg-lib/_reg_f3d2.cpp:612-648): restore the best grids, add
+        // the scaled gradients, symmetrise
+        auto update = [&](float scale) {
+            check(nrh_f3d_restore_best(hf));
+            check(nrh_f3d_restore_best(hb));
+            check(nrh_f3d_update_velocity_field(hf, scale, 1, 1, 1));
+            check(nrh_f3d_update_velocity_field(hb, scale, 1, 1, 1));
+            check(nrh_f3d_symmetrise_velocity_fields(hf, hb));
+        };
+        while (currentSize) {
+            if (iter >= maxIter) break;
+            double maxLen = 0;
+            check(nrh_f3d2_gradient(hf, hb, &maxLen));
+            ++iter;
+            currentSize = std::min(currentSize, maxStep);
+            check(nrh_f3d2_conjugate_gradient(hf, hb, firstCall ? 1 : 0));
+            firstCall = false;
+            size_t lineIter = 0;
+            float added = 0;
+            float curLen = currentSize;
+            while (curLen > smallest && lineIter < 12 && iter < maxIter) {
+                update(-curLen);
+                double value = 0;
+                check(nrh_f3d2_objective(hf, hb, &value, terms));
+                if (value > best) {
+                    best = value;
+                    added += curLen;
+                    curLen *= 1.1f;
+                    curLen = std::min(curLen, maxStep);
+                    check(nrh_f3d_store_best(hf));
+                    check(nrh_f3d_store_best(hb));
+                } else {
+                    curLen *= 0.5;
+                }
+                ++iter;
+                ++lineIter;
+            }
+            currentSize = added;
+            check(nrh_f3d_restore_best(hf));
+            check(nrh_f3d_restore_best(hb));
+            if (currentSize == 0) {
+                if (!justRestarted && iter < maxIter) {
+                    firstCall = true;
+                    currentSize = maxStep;
+                    justRestarted = true;
+                }
+            } else {
+                justRestarted = false;
+            }
+        }
+        totalIterations += (int)iter;
+        check(nrh_f3d_get_control_points(hf, fw.data()));
+        check(nrh_f3d_get_control_points(hb, bw.data()));
+        double value = 0;
+        check(nrh_f3d2_objective(hf, hb, &value, terms));
+        check(nrh_f3d_get_velocity_steps(hf, &steps));
+        finalNmi = simW > 0 ? terms[0] / simW : 0.0;
+        maxIter /= 2;
+    }
+    gridOut = grid;
+    fwOut = fw;
+    bwOut = bw;
+}
 }  // namespace nrh
 
 using namespace nrh;
@@ -376,6 +576,27 @@ int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom 
             NRH_REQUIRE(cpp.size() <= cpp_capacity, NRH_ERR_ARG, "cpp_out too small for the final grid");
             std::memcpy(cpp_out, cpp.data(), cpp.size() * sizeof(float));
         }
+        if (final_nmi) *final_nmi = nmi;
+        if (iterations) *iterations = it;
+    });
+}
+
+int nrh_f3d2_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
+                      const nrh_f3d_config *cfg, nrh_geom *grid_out, float *fw_out, float *bw_out, size_t capacity,
+                      double *final_nmi, int *iterations) {
+    return guard([&] {
+        check_geom(ref, "reference");
+        check_geom(flo, "floating");
+        NRH_REQUIRE(ref_data && flo_data && cfg && grid_out, NRH_ERR_ARG, "null argument");
+        nrh_geom g;
+        std::vector<float> fw, bw;
+        double nmi = 0;
+        int it = 0;
+        register_pair_sym(*ref, ref_data, *flo, flo_data, *cfg, g, fw, bw, nmi, it);
+        *grid_out = g;
+        if (fw_out || bw_out) NRH_REQUIRE(fw.size() <= capacity, NRH_ERR_ARG, "output arrays too small for the final grid");
+        if (fw_out) std::memcpy(fw_out, fw.data(), fw.size() * sizeof(float));
+        if (bw_out) std::memcpy(bw_out, bw.data(), bw.size() * sizeof(float));
         if (final_nmi) *final_nmi = nmi;
         if (iterations) *iterations = it;
     });

@@ -11,6 +11,10 @@ void mat44_inverse(const float *R, float *Q);
 // reg_createControlPointGrid<float> (reg-lib/cpu/_reg_localTrans.cpp:21-128)
 void create_grid(const nrh_geom& ref, const float spacing[3], nrh_geom& grid, float *identity);
 
+// reg_createSymmetricControlPointGrids<float> without an affine (reg-lib/cpu/_reg_localTrans.cpp:132-358)
+void symmetric_grids(const nrh_geom& ref, const nrh_geom& flo, const float spacing[3], nrh_geom& grid, float *fw,
+                     float *bw);
+
 struct Level {
     nrh_geom geom;
     std::vector<float> data;

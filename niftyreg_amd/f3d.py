@@ -337,3 +337,34 @@ class F3d2Pair:
     def close(self):
         self.fw.close()
         self.bw.close()
+
+
+def register_symmetric(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.01, spacing=-5.0,
+                       levels=3, levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True):
+    """Full reg_f3d2 registration of one pair on the device (symmetric velocity grids, NMI).
+
+    Returns (grid Geom, forward grid, backward grid, final NMI sum of both directions, iterations)."""
+    cfg = _lib.nrh_f3d_config()
+    cfg.bins, cfg.be_weight, cfg.le_weight, cfg.spacing = bins, be, le, spacing
+    cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
+    cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), 0
+    cfg.measure, cfg.lncc_sigma = _lib.MEASURE_NMI, -5.0
+    ref = np.ascontiguousarray(ref, np.float32).reshape(-1)
+    flo = np.ascontiguousarray(flo, np.float32).reshape(-1)
+    rg, fg = ref_geom.to_c(), flo_geom.to_c()
+    out = _lib.nrh_geom()
+    # the final grid spans both images at the final spacing (mm or voxels of the mean voxel size)
+    ext = [max(n * d, m * e) + 2 * abs(o) for n, d, m, e, o in
+           zip(ref_geom.dims, ref_geom.pixdim, flo_geom.dims, flo_geom.pixdim,
+               np.abs(ref_geom.xyz[:3, 3] - flo_geom.xyz[:3, 3]))]
+    sp = [abs(spacing) * (d + e) / 2 if spacing < 0 else spacing for d, e in zip(ref_geom.pixdim, flo_geom.pixdim)]
+    cap = 3 * int(np.prod([int(np.ceil(x / s + 8)) for x, s in zip(ext, sp)]))
+    fw = np.zeros(cap, np.float32)
+    bw = np.zeros(cap, np.float32)
+    nmi = ctypes.c_double()
+    its = ctypes.c_int()
+    _lib.call("nrh_f3d2_register", ctypes.byref(rg), _lib.fptr(ref), ctypes.byref(fg), _lib.fptr(flo),
+              ctypes.byref(cfg), ctypes.byref(out), _lib.fptr(fw), _lib.fptr(bw), ctypes.c_size_t(cap),
+              ctypes.byref(nmi), ctypes.byref(its))
+    g = Geom.from_c(out)
+    return g, fw[:3 * g.nvox].copy(), bw[:3 * g.nvox].copy(), nmi.value, its.value

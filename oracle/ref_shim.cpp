@@ -705,6 +705,27 @@ int ref_f3d2_gradient(void *hp, int normalise, double *maxLength, float *gradFw,
     });
 }
 
+// Full symmetric registration through reg_f3d2's run (reg_base::Run with the reg_f3d2 overrides).
+// Output: the final-level grid geometry, both velocity grids and their intent_p2.
+int ref_f3d2_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                 const ref_f3d_params *p, nr_geom *cppOut, float *fw, float *bw, int capacity, int *steps) {
+    return guarded([&] {
+        RefF3d2 r;
+        configure(r, p);
+        r.SetReferenceImage(make_image(ref, refData));
+        r.SetFloatingImage(make_image(flo, floData));
+        r.Run();
+        NiftiImage g = r.GetControlPointPositionImage();
+        NiftiImage gb = r.GetBackwardControlPointPositionImage();
+        geom_of(g, cppOut);
+        if ((int)g.nVoxels() <= capacity) {
+            if (fw) std::memcpy(fw, g->data, g.nVoxels() * sizeof(float));
+            if (bw) std::memcpy(bw, gb->data, gb.nVoxels() * sizeof(float));
+        }
+        if (steps) { steps[0] = (int)g->intent_p2; steps[1] = (int)gb->intent_p2; }
+    });
+}
+
 int ref_f3d2_weights(void *hp, float *w) {
     F3d2Handle *h = static_cast<F3d2Handle*>(hp);
     w[0] = h->reg->SimWeight(); w[1] = h->reg->BEWeight(); w[2] = h->reg->LEWeight();

@@ -896,3 +896,33 @@ def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, s
     if rc != 0:
         raise RuntimeError(_plugin.plugin_last_error().decode())
     return m.value, be.value, gf, gb, f, b, (int(steps[0]), int(steps[1]))
+
+
+def ref_f3d2_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.01, spacing=-5.0, levels=3, maxit=150,
+                 padding=float("nan")):
+    """reg_f3d2<float>'s run on the reference CPU build; returns (grid geom, fw, bw, steps)."""
+    p = ref_f3d_params()
+    p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
+    p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
+    p.measure, p.lncc_sigma = 0, -5.0
+    out = nr_geom()
+    cap = 3 * 200 ** 3 // 8
+    fw = np.zeros(cap, np.float32)
+    bw = np.zeros(cap, np.float32)
+    steps = np.zeros(2, np.int32)
+    rcheck(ref().ref_f3d2_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
+                              ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
+                              ctypes.byref(p), ctypes.byref(out), fp(fw), fp(bw), cap, ip(steps)))
+    n = 3 * out.nx * out.ny * out.nz
+    return geom_from_nr(out), fw[:n].copy(), bw[:n].copy(), (int(steps[0]), int(steps[1]))
+
+
+def nmi_sym_at(refg, flog, cppg, fw, bw, ref_img, flo_img, bins=64):
+    """NMI(reference, floating warped by exp(fw)) + NMI(floating, reference warped by exp(bw)) on
+    the oracle, each exponentiation's step number chosen from its flow (images rescaled here)."""
+    rb = bins + 4
+    r = orc_rescale(np.asarray(ref_img, np.float32).ravel(), 2, rb - 3)
+    f = orc_rescale(np.asarray(flo_img, np.float32).ravel(), 2, rb - 3)
+    ff, _ = orc_vel_def_field(refg, cppg, np.ascontiguousarray(fw, np.float32), 6, True)
+    fb, _ = orc_vel_def_field(flog, cppg, np.ascontiguousarray(bw, np.float32), 6, True)
+    return orc_nmi(r, orc_resample(flog, f, refg, ff), rb, rb)[0] + orc_nmi(f, orc_resample(refg, r, flog, fb), rb, rb)[0]

@@ -49,3 +49,33 @@ def test_config0_registration_matches_reference(gpu, levels, maxit):
         # the identity on this pair (the reference stops at its start NMI too)
         assert nmi_gpu > nmi0 + 0.01
     assert nmi_gpu >= nmi_ref - FINAL_NMI_TOL
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build absent")
+@pytest.mark.parametrize("levels,maxit", [(2, 30)])
+def test_symmetric_registration_matches_reference(gpu, levels, maxit):
+    """reg_f3d2 (-vel, NMI, -be 0.001 -le 0.01) on both platforms.  Both final velocity grid pairs
+    are scored with the oracle (each direction exponentiated with its own step choice, the NMI of
+    both directions summed); the device may not end lower than the reference by more than twice
+    the single-direction tolerance."""
+    from niftyreg_amd.f3d import register_symmetric
+    from niftyreg_amd.geometry import create_symmetric_control_point_grids
+    ref, flo = phantoms.config0_pair(0)
+    g = Geom.identity((64, 64, 64))
+    t0 = time.perf_counter()
+    cg, fw, bw, nmi, its = register_symmetric(g, ref, g, flo, levels=levels, maxit=maxit)
+    t_gpu = time.perf_counter() - t0
+    score_gpu = O.nmi_sym_at(g, g, cg, fw, bw, ref, flo)
+    t0 = time.perf_counter()
+    rcg, rfw, rbw, rsteps = O.ref_f3d2_run(g, ref, g, flo, levels=levels, maxit=maxit)
+    t_cpu = time.perf_counter() - t0
+    score_ref = O.nmi_sym_at(g, g, rcg, rfw, rbw, ref, flo)
+    g0, f0, b0 = create_symmetric_control_point_grids(g, g, [5.0 * 2 ** (levels - 1)] * 3)
+    score0 = O.nmi_sym_at(g, g, g0, f0, b0, ref, flo)
+    print(f"symmetric, {levels} levels: start {score0:.6f}, device {score_gpu:.6f} ({its} it, {t_gpu:.2f} s), "
+          f"reference {score_ref:.6f} ({t_cpu:.2f} s)")
+    assert cg.dims == rcg.dims
+    assert np.array_equal(cg.xyz, rcg.xyz)
+    assert abs(nmi - score_gpu) <= 1e-4 * score_gpu
+    assert score_gpu > score0 + 0.02
+    assert score_gpu >= score_ref - 2 * FINAL_NMI_TOL
