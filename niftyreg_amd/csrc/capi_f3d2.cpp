@@ -17,8 +17,9 @@ void check_pair(nrh_f3d *fw, nrh_f3d *bw) {
     NRH_REQUIRE(fw && bw && fw != bw, NRH_ERR_ARG, "two distinct handles are required");
     NRH_REQUIRE(fw->velocity && bw->velocity, NRH_ERR_STATE, "both control point grids must be velocity grids");
     const nrh_f3d_params &p = fw->params, &q = bw->params;
-    NRH_REQUIRE(p.measure == NRH_MEASURE_NMI && q.measure == NRH_MEASURE_NMI, NRH_ERR_UNSUPPORTED,
-                "the symmetric evaluation supports NMI");
+    NRH_REQUIRE(p.measure == q.measure, NRH_ERR_ARG, "forward and backward handles must use the same measure");
+    NRH_REQUIRE(p.measure == NRH_MEASURE_NMI || p.lncc_sigma == q.lncc_sigma, NRH_ERR_ARG,
+                "forward and backward LNCC kernels differ");
     NRH_REQUIRE(p.similarity_weight == q.similarity_weight && p.be_weight == q.be_weight && p.le_weight == q.le_weight,
                 NRH_ERR_ARG, "forward and backward handles must share the objective weights");
     NRH_REQUIRE(p.ref_bins == q.flo_bins && p.flo_bins == q.ref_bins, NRH_ERR_ARG,
@@ -50,9 +51,11 @@ void warp_and_histograms(nrh_f3d *fw, nrh_f3d *bw) {
     for (nrh_f3d *h : { fw, bw }) {
         if (h->pairStateVersion == h->cppVersion) continue;
         op_resample(h);
-        op_joint_histogram(h, h->warped.p);
-        op_nmi_finalize(h);
-        h->histVersion = h->cppVersion;
+        if (h->params.measure == NRH_MEASURE_NMI) {
+            op_joint_histogram(h, h->warped.p);
+            op_nmi_finalize(h);
+            h->histVersion = h->cppVersion;
+        }
         h->pairStateVersion = h->cppVersion;
     }
 }
@@ -72,11 +75,17 @@ int nrh_f3d2_objective(nrh_f3d *fw, nrh_f3d *bw, double *value, double terms[3])
         check_pair(fw, bw);
         NRH_REQUIRE(value, NRH_ERR_ARG, "null argument");
         const nrh_f3d_params& p = fw->params;
-        if (p.similarity_weight > 0) warp_and_histograms(fw, bw);
+        const bool lncc = p.measure == NRH_MEASURE_LNCC;
+        if (p.similarity_weight > 0) {
+            warp_and_histograms(fw, bw);
+            // reg_lncc::GetSimilarityMeasureValueFw / Bw (reg-lib/cpu/_reg_lncc.cpp:357-390)
+            if (lncc)
+                for (nrh_f3d *h : { fw, bw }) op_lncc_value(h, h->scalars.p + 4);
+        }
         for (nrh_f3d *h : { fw, bw }) {
             if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
             if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
-            op_publish(h, h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
+            op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
         }
         NRH_HIP(hipStreamSynchronize(fw->stream));
         NRH_HIP(hipStreamSynchronize(bw->stream));
@@ -108,8 +117,10 @@ int nrh_f3d2_gradient(nrh_f3d *fw, nrh_f3d *bw, double *max_length) {
                 op_fill_f32(h->voxelGrad.p, 3 * h->N, 0.f, h->stream);
                 op_image_gradient(h);
             }
-            op_nmi_voxel_gradient(fw, 1.0);
-            op_nmi_voxel_gradient(bw, 1.0);
+            for (nrh_f3d *h : { fw, bw }) {
+                if (p.measure == NRH_MEASURE_LNCC) op_lncc_voxel_gradient(h, 1.0);
+                else op_nmi_voxel_gradient(h, 1.0);
+            }
             // ExponentiateGradient (reg-lib/_reg_f3d2.cpp:597-610): each direction reads the
             // other's grid (unchanged until the join before the penalty gradients) and, on a
             // shared geometry, the other's squaring chain, written on the other stream

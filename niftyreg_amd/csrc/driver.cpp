@@ -396,7 +396,6 @@ void register_pair_sym(const nrh_geom& refG, const float *refData, const nrh_geo
                        const nrh_f3d_config& cfg, nrh_geom& gridOut, std::vector<float>& fwOut,
                        std::vector<float>& bwOut, double& finalNmi, int& totalIterations) {
     NRH_REQUIRE(cfg.levels >= 1 && cfg.bins >= 1 && cfg.max_iterations >= 0, NRH_ERR_ARG, "invalid configuration");
-    NRH_REQUIRE(cfg.measure == NRH_MEASURE_NMI, NRH_ERR_UNSUPPORTED, "the symmetric registration supports NMI");
     NRH_REQUIRE(refG.nz > 1 && floG.nz > 1, NRH_ERR_UNSUPPORTED, "the symmetric registration is 3-D only");
     const int levels = cfg.levels;
     int toPerform = cfg.levels_to_perform > 0 ? std::min(cfg.levels_to_perform, levels) : levels;
@@ -470,7 +469,7 @@ void register_pair_sym(const nrh_geom& refG, const float *refData, const nrh_geo
         p.be_weight = be;
         p.le_weight = le;
         p.image_storage = 0;
-        p.measure = NRH_MEASURE_NMI;
+        p.measure = cfg.measure;
         p.lncc_sigma = cfg.lncc_sigma;
         Handle F, B;
         check(nrh_f3d_create(&lr, rp[idx].data.data(), &lf, fp[idx].data.data(), nullptr, &grid, &p, &F.h));
@@ -480,8 +479,13 @@ void register_pair_sym(const nrh_geom& refG, const float *refData, const nrh_geo
         check(nrh_f3d_set_control_points(hb, bw.data()));
         check(nrh_f3d_set_velocity_grid(hf, steps));
         check(nrh_f3d_set_velocity_grid(hb, steps));
-        check(nrh_f3d_rescale_intensities(hf, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
-        check(nrh_f3d_rescale_intensities(hb, 2.f, p.flo_bins - 3.f, 2.f, p.ref_bins - 3.f));
+        if (p.measure == NRH_MEASURE_LNCC) {
+            check(nrh_f3d_rescale_intensities(hf, 0.f, 1.f, 0.f, 1.f));
+            check(nrh_f3d_rescale_intensities(hb, 0.f, 1.f, 0.f, 1.f));
+        } else {
+            check(nrh_f3d_rescale_intensities(hf, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
+            check(nrh_f3d_rescale_intensities(hb, 2.f, p.flo_bins - 3.f, 2.f, p.ref_bins - 3.f));
+        }
         check(nrh_f3d_store_best(hf));
         check(nrh_f3d_store_best(hb));
         double best = 0, terms[3];

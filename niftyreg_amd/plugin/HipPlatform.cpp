@@ -435,11 +435,17 @@ void reg_lncc_hip::InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, i
     reg_lncc::InitialiseMeasure(refImg, floImg, refMask, warpedImg, warpedGrad, voxelBasedGrad, localWeightSim,
                                 floMask, warpedImgBw, warpedGradBw, voxelBasedGradBw);
     if (!content) NR_FATAL_ERROR("reg_lncc_hip needs a HIP F3D content");
-    if (isSymmetric || referenceTimePoints != 1) NR_FATAL_ERROR("HIP LNCC: one time point, non-symmetric");
+    if (referenceTimePoints != 1) NR_FATAL_ERROR("HIP LNCC: one time point");
+    if (isSymmetric && !contentBw) NR_FATAL_ERROR("HIP LNCC: the symmetric measure needs the backward HIP content");
     if (kernelType != ConvKernelType::Gaussian) NR_FATAL_ERROR("HIP LNCC uses the Gaussian kernel");
     content->SetMeasure(NRH_MEASURE_LNCC, 68, 68, kernelStandardDeviation[0]);
     // reg_lncc rescaled the host images to [0, 1]; the device copies follow
     Check(nrh_f3d_rescale_intensities(content->GetHandle(), 0.f, 1.f, 0.f, 1.f), "nrh_f3d_rescale_intensities");
+    if (isSymmetric) {
+        // the backward content's reference is the floating image (reg-lib/cpu/_reg_lncc.cpp:376-390)
+        contentBw->SetMeasure(NRH_MEASURE_LNCC, 68, 68, kernelStandardDeviation[0]);
+        Check(nrh_f3d_rescale_intensities(contentBw->GetHandle(), 0.f, 1.f, 0.f, 1.f), "nrh_f3d_rescale_intensities");
+    }
 }
 
 double reg_lncc_hip::GetSimilarityMeasureValueFw() {
@@ -448,10 +454,24 @@ double reg_lncc_hip::GetSimilarityMeasureValueFw() {
     return timePointWeights[0] * v;
 }
 
+double reg_lncc_hip::GetSimilarityMeasureValueBw() {
+    double v = 0;
+    Check(nrh_f3d_lncc_value(contentBw->GetHandle(), &v), "nrh_f3d_lncc_value");
+    return timePointWeights[0] * v;
+}
+
 void reg_lncc_hip::GetVoxelBasedSimilarityMeasureGradientFw(int currentTimePoint) {
+    content->EnsureVoxelGradientOnDevice();
     Check(nrh_f3d_lncc_voxel_gradient(content->GetHandle(), timePointWeights[currentTimePoint]),
           "nrh_f3d_lncc_voxel_gradient");
     content->MarkOnDevice(HipF3dContent::VoxelGradient);
+}
+
+void reg_lncc_hip::GetVoxelBasedSimilarityMeasureGradientBw(int currentTimePoint) {
+    contentBw->EnsureVoxelGradientOnDevice();
+    Check(nrh_f3d_lncc_voxel_gradient(contentBw->GetHandle(), timePointWeights[currentTimePoint]),
+          "nrh_f3d_lncc_voxel_gradient");
+    contentBw->MarkOnDevice(HipF3dContent::VoxelGradient);
 }
 
 reg_measure* HipMeasureCreator::Create(const MeasureType measureType) {
@@ -471,10 +491,7 @@ void HipMeasureCreator::Initialise(reg_measure& measure, DefContent& con, DefCon
     HipF3dContent *cBw = conBw ? dynamic_cast<HipF3dContent*>(conBw) : nullptr;
     if (conBw && !cBw) NR_FATAL_ERROR("The backward content is not a HIP F3D content");
     if (auto *m = dynamic_cast<reg_nmi_hip*>(&measure)) m->SetContent(c, cBw);
-    if (auto *m = dynamic_cast<reg_lncc_hip*>(&measure)) {
-        if (cBw) NR_FATAL_ERROR("The symmetric LNCC measure is not supported on the HIP platform");
-        m->SetContent(c);
-    }
+    if (auto *m = dynamic_cast<reg_lncc_hip*>(&measure)) m->SetContent(c, cBw);
     MeasureCreator::Initialise(measure, con, conBw);
 }
 

@@ -211,17 +211,20 @@ private:
 
 class reg_lncc_hip: public reg_lncc {
 public:
-    void SetContent(HipF3dContent *c) { content = c; }
+    // cBw: the backward content of a symmetric (reg_f3d2) registration
+    void SetContent(HipF3dContent *c, HipF3dContent *cBw = nullptr) { content = c; contentBw = cBw; }
     virtual void InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, int *refMask, nifti_image *warpedImg,
                                    nifti_image *warpedGrad, nifti_image *voxelBasedGrad,
                                    nifti_image *localWeightSim = nullptr, int *floMask = nullptr,
                                    nifti_image *warpedImgBw = nullptr, nifti_image *warpedGradBw = nullptr,
                                    nifti_image *voxelBasedGradBw = nullptr) override;
     virtual double GetSimilarityMeasureValueFw() override;
+    virtual double GetSimilarityMeasureValueBw() override;
     virtual void GetVoxelBasedSimilarityMeasureGradientFw(int currentTimePoint) override;
+    virtual void GetVoxelBasedSimilarityMeasureGradientBw(int currentTimePoint) override;
 
 private:
-    HipF3dContent *content = nullptr;
+    HipF3dContent *content = nullptr, *contentBw = nullptr;
 };
 
 class HipMeasureCreator: public MeasureCreator {

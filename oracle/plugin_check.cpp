@@ -77,12 +77,12 @@ int plugin_f3d_eval(int hip, const nr_geom *ref, const float *refData, const nr_
 // 216-232, 246-301, 333-342, 417-440, 540-560, 597-610) with NMI, through either platform: both
 // velocity grids exponentiated (the forward step number updated, the backward one copied),
 // both warps, symmetric NMI, both voxel gradients exponentiated along the opposite grid, both
-// convolved and projected, penalty gradients of both grids.  cppFw / cppBw are updated in
+// convolved and projected, penalty gradients of both grids (NMI, or LNCC with lncc != 0).  cppFw / cppBw are updated in
 // place (the bending-energy gradient rounds them through displacements); steps[0..1] receive
 // the grids' intent_p2.  measure = NMI_fw + NMI_bw; be = BE_fw + BE_bw (+ 1000 (LE_fw + LE_bw)).
 int plugin_f3d2_eval(int hip, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
-                     const nr_geom *cpp, float *cppFw, float *cppBw, int bins, float simW, float beW, float leW,
-                     double *measure, double *be, float *gradFw, float *gradBw, int *steps) {
+                     const nr_geom *cpp, float *cppFw, float *cppBw, int bins, int lncc, float simW, float beW,
+                     float leW, double *measure, double *be, float *gradFw, float *gradBw, int *steps) {
     try {
         NiftiImage r = make_image(ref, refData), f = make_image(flo, floData);
         NiftiImage g = make_grid(cpp, cppFw), gb = make_grid(cpp, cppBw);
@@ -94,9 +94,13 @@ int plugin_f3d2_eval(int hip, const nr_geom *ref, const float *refData, const nr
         std::unique_ptr<ComputeFactory> cf(hip ? new HipComputeFactory() : new ComputeFactory());
         std::unique_ptr<Compute> comp(cf->Produce(*con)), compBw(cf->Produce(*conBw));
         std::unique_ptr<MeasureCreator> mc(hip ? new HipMeasureCreator() : new MeasureCreator());
-        std::unique_ptr<reg_measure> m(mc->Create(MeasureType::Nmi));
-        dynamic_cast<reg_nmi&>(*m).SetReferenceBinNumber(bins + 4, 0);
-        dynamic_cast<reg_nmi&>(*m).SetFloatingBinNumber(bins + 4, 0);
+        std::unique_ptr<reg_measure> m(mc->Create(lncc ? MeasureType::Lncc : MeasureType::Nmi));
+        if (lncc) {
+            dynamic_cast<reg_lncc&>(*m).SetKernelStandardDeviation(0, -5.f);
+        } else {
+            dynamic_cast<reg_nmi&>(*m).SetReferenceBinNumber(bins + 4, 0);
+            dynamic_cast<reg_nmi&>(*m).SetFloatingBinNumber(bins + 4, 0);
+        }
         m->SetTimePointWeight(0, 1.0);
         mc->Initialise(*m, *con, conBw.get());
         const float pad = std::numeric_limits<float>::quiet_NaN();

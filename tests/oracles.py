@@ -806,11 +806,11 @@ class RefF3d2:
     scalar=True uses the reference built without USE_SSE (the oracle's summation order)."""
 
     def __init__(self, refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=1,
-                 level=0, maxit=150, padding=float("nan"), scalar=True):
+                 level=0, maxit=150, padding=float("nan"), scalar=True, measure=0, lncc_sigma=-5.0):
         p = ref_f3d_params()
         p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
         p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
-        p.measure, p.lncc_sigma = 0, -5.0
+        p.measure, p.lncc_sigma = measure, lncc_sigma
         self.p = p
         self.lib = r = ref_scalar() if scalar else ref()
         r.ref_f3d2_create.restype = ctypes.c_void_p
@@ -875,7 +875,8 @@ class RefF3d2:
             pass
 
 
-def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, simw=0.989, bew=0.001, lew=0.01):
+def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, simw=0.989, bew=0.001, lew=0.01,
+                     lncc=False):
     """One reg_f3d2 objective value + gradient through the reference's own Content / Compute /
     Measure calls (forward and backward contents), on the reference CPU classes (hip=False) or
     the HIP platform plugin (hip=True).  Returns (measure, be, grad_fw, grad_bw, cpp_fw, cpp_bw, steps)."""
@@ -891,7 +892,8 @@ def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, s
     steps = np.zeros(2, np.int32)
     rc = _plugin.plugin_f3d2_eval(int(hip), ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).ravel()),
                                   ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).ravel()),
-                                  ctypes.byref(rg(cppg)), fp(f), fp(b), bins, ctypes.c_float(simw), ctypes.c_float(bew),
+                                  ctypes.byref(rg(cppg)), fp(f), fp(b), bins, int(lncc), ctypes.c_float(simw),
+                                  ctypes.c_float(bew),
                                   ctypes.c_float(lew), ctypes.byref(m), ctypes.byref(be), fp(gf), fp(gb), ip(steps))
     if rc != 0:
         raise RuntimeError(_plugin.plugin_last_error().decode())

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracles as O
-from niftyreg_amd import phantoms
+from niftyreg_amd import _lib, phantoms
 from niftyreg_amd.f3d import F3d2Pair, F3dContent, default_params
 from niftyreg_amd.geometry import Geom
 
@@ -34,7 +34,7 @@ def _oblique(dims):
     return Geom(dims, m)
 
 
-def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5):
+def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5, lncc=False):
     if which == "identity":
         rg = fg = Geom.identity((40, 36, 32))
     else:
@@ -42,7 +42,7 @@ def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5):
     ref, flo = phantoms.config0_pair(3, rg.dims)
     if fg.dims != rg.dims:
         flo = phantoms.blob_phantom(fg.dims, 4)
-    r = O.RefF3d2(rg, ref, fg, flo, be=be, le=le, spacing=-5.0, levels=1)
+    r = O.RefF3d2(rg, ref, fg, flo, be=be, le=le, spacing=-5.0, levels=1, measure=1 if lncc else 0)
     fw0, bw0, _ = r.get_cpp()
     rng = np.random.default_rng(seed)
     fw = (fw0 + rng.uniform(-amplitude, amplitude, fw0.size)).astype(np.float32)
@@ -50,18 +50,19 @@ def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5):
     r.set_cpp(fw, bw)
     simw, bew, lew = r.weights()
     cg = O.geom_from_nr(r.cpp_geom)
-    pf = default_params(similarity_weight=simw, be_weight=bew, le_weight=lew)
-    pb = default_params(similarity_weight=simw, be_weight=bew, le_weight=lew)
+    m = _lib.MEASURE_LNCC if lncc else _lib.MEASURE_NMI
+    pf = default_params(similarity_weight=simw, be_weight=bew, le_weight=lew, measure=m, lncc_sigma=-5.0)
+    pb = default_params(similarity_weight=simw, be_weight=bew, le_weight=lew, measure=m, lncc_sigma=-5.0)
     hf = F3dContent(rg, ref.ravel(), fg, flo.ravel(), cg, fw, params=pf)
     hb = F3dContent(fg, flo.ravel(), rg, ref.ravel(), cg, bw, params=pb)
-    hf.rescale_intensities((2, 65), (2, 65))
-    hb.rescale_intensities((2, 65), (2, 65))
+    rng_ = (0, 1) if lncc else (2, 65)
+    hf.rescale_intensities(rng_, rng_)
+    hb.rescale_intensities(rng_, rng_)
     return r, F3d2Pair(hf, hb, steps=6)
 
 
 def _check_gradients(pair, rgf, rgb, rmax, m, exact):
     gf, gb = pair.fw.node_gradient(), pair.bw.node_gradient()
-    assert pair.fw.fused_available() == (not exact) and pair.bw.fused_available() == (not exact)
     if exact:
         assert m == rmax
         assert np.array_equal(gf, rgf), np.count_nonzero(gf != rgf)
@@ -102,5 +103,21 @@ def test_large_velocities_update_the_step_number(gpu):
     rgf, rgb, rmax = r.gradient(normalise=True)
     m = pair.gradient(normalise=True)
     _check_gradients(pair, rgf, rgb, rmax, m, exact=False)
+    r.close()
+    pair.close()
+
+
+@pytest.mark.parametrize("which", ["identity", "oblique"])
+def test_symmetric_lncc_matches_reference(gpu, which):
+    """reg_f3d2 with LNCC (Gaussian sigma 5 voxels): both directions' LNCC values and voxel
+    gradients on the device (the backward one on the floating image's geometry)."""
+    r, pair = _session(which, lncc=True)
+    rv, rterms = r.value()
+    v, terms = pair.objective()
+    for k in range(3):
+        assert abs(terms[k] - rterms[k]) <= 1e-12 * abs(rterms[k]), (k, terms, rterms)
+    rgf, rgb, rmax = r.gradient(normalise=True)
+    m = pair.gradient(normalise=True)
+    _check_gradients(pair, rgf, rgb, rmax, m, exact=(which == "oblique"))
     r.close()
     pair.close()

@@ -81,13 +81,14 @@ def _f3d2_case(which, seed=2, amplitude=2.0):
 
 @needs_plugin
 @pytest.mark.gpu
+@pytest.mark.parametrize("lncc", [False, True])
 @pytest.mark.parametrize("which", ["cube", "oblique"])
-def test_hip_platform_symmetric_matches_reference_platform(gpu, which):
-    """reg_f3d2's calls (velocity exponentiation, symmetric NMI, gradient exponentiation along the
-    opposite grid, both directions' convolution, penalties) through both platforms."""
+def test_hip_platform_symmetric_matches_reference_platform(gpu, which, lncc):
+    """reg_f3d2's calls (velocity exponentiation, symmetric NMI or LNCC, gradient exponentiation
+    along the opposite grid, both directions' convolution, penalties) through both platforms."""
     rg_, fg, ref, flo, cg, fw, bw = _f3d2_case(which)
-    c = O.plugin_f3d2_eval(False, rg_, ref, fg, flo, cg, fw, bw)
-    h = O.plugin_f3d2_eval(True, rg_, ref, fg, flo, cg, fw, bw)
+    c = O.plugin_f3d2_eval(False, rg_, ref, fg, flo, cg, fw, bw, lncc=lncc)
+    h = O.plugin_f3d2_eval(True, rg_, ref, fg, flo, cg, fw, bw, lncc=lncc)
     assert abs(h[0] - c[0]) <= 1e-12 * abs(c[0])
     assert abs(h[1] - c[1]) <= 1e-12 * abs(c[1])
     assert h[6] == c[6]
