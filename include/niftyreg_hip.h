@@ -67,11 +67,18 @@ int nrh_refine_control_point_grid(const nrh_geom *ref, const nrh_geom *grid, con
 int nrh_downsample_image(const nrh_geom *in, const float *data, const int axes[3], nrh_geom *out, float *out_data);
 /* reg_createSymmetricControlPointGrids<float> (reg-lib/cpu/_reg_localTrans.cpp:132-358): the
  * forward and backward velocity grids of reg_f3d2, axis-aligned (sform only) over the union of
- * both images' corners.  affine must be NULL (a half-affine initialisation is not supported:
- * NRH_ERR_UNSUPPORTED).  fw / bw receive the identity positions (3*G floats each; NULL to query
- * the geometry). */
+ * both images' corners, each image first carried half-way by its half affine when `affine`
+ * (reference -> floating, row-major 4x4; NULL: none) is given.  fw / bw receive the identity
+ * positions (3*G floats each; NULL to query the geometry).  The half affines, which the
+ * reference stores in the grids' extensions, come from nrh_symmetric_half_affines. */
 int nrh_create_symmetric_control_point_grids(const nrh_geom *ref, const nrh_geom *flo, const float spacing_mm[3],
                                              const float *affine, nrh_geom *grid, float *fw, float *bw);
+/* The half affines of reg_createSymmetricControlPointGrids (reg-lib/cpu/_reg_localTrans.cpp:
+ * 150-166, NiftyReg::Mat44Logm / Mat44Expm of reg-lib/cpu/Maths.cpp:414-478): half_fw =
+ * expm(logm(affine) / 2), half_bw = expm(logm(affine^-1) / 2), rounded through float as the
+ * reference.  *extensions = 1 when the reference stores them as the grids' extensions 0 and 1
+ * (both halves differ from the identity, :320-353), else 0. */
+int nrh_symmetric_half_affines(const float affine[16], float half_fw[16], float half_bw[16], int *extensions);
 
 /* ------------------------------------------------------------------ F3D */
 typedef struct nrh_f3d nrh_f3d;
@@ -149,7 +156,7 @@ int nrh_f3d_download_histogram(nrh_f3d *h, double *pro, double *logh);
 /* ---- velocity grids (reg_f3d2) ----
  * A handle whose control points form a cubic B-spline velocity grid (reg_f3d2's forward grid on
  * the reference -> floating handle, its backward grid on a second handle created with the two
- * images swapped).  Grids with affine extensions are not supported. */
+ * images swapped), with its affine parts set by nrh_f3d_set_velocity_affines. */
 /* The grid set-up of reg_f3d2::Initialise (reg-lib/_reg_f3d2.cpp:585-590): intent_p1 =
  * SPLINE_VEL_GRID, intent_p2 = steps (squarings of the exponentiation; 0 lets the next
  * nrh_f3d_def_field_from_velocity choose). */
@@ -158,6 +165,16 @@ int nrh_f3d_set_velocity_grid(nrh_f3d *h, int steps);
  * step count onto the backward grid, reg-lib/_reg_f3d2.cpp:146-152) */
 int nrh_f3d_get_velocity_steps(nrh_f3d *h, int *steps);
 int nrh_f3d_set_velocity_steps(nrh_f3d *h, int steps);
+/* The affine parts of a velocity grid (row-major 4x4 each, NULL when absent): ext0 / ext1 are the
+ * grid's NIfTI extensions 0 and 1 (reg_f3d2 stores the half affine in both).  ext0 is applied
+ * before the spline (reg-lib/cpu/_reg_localTrans.cpp:1478-1487), removed before the scaling and
+ * squaring and restored after it (:3508-3515, 3569-3573), and maps the grid into the reference
+ * space of the node gradient (:1568-1592); ext1 is composed last (:3576-3577).  transformation
+ * is the content's transformation matrix (Content::GetTransformationMatrix: reg_f3d2 gives the
+ * forward content the input affine and the backward one its inverse), whose displacement the
+ * opposite direction's gradient exponentiation removes from each intermediate field
+ * (reg-lib/Compute.cpp:303-316). */
+int nrh_f3d_set_velocity_affines(nrh_f3d *h, const float *ext0, const float *ext1, const float *transformation);
 /* Compute::GetDefFieldFromVelocityGrid (reg-lib/Compute.cpp:223-228): scaling and squaring of
  * the grid into the handle's deformation field (then nrh_f3d_resample / _image_gradient use it). */
 int nrh_f3d_def_field_from_velocity(nrh_f3d *h, int update_step_number);
@@ -246,12 +263,15 @@ typedef struct nrh_f3d_config {
     int image_storage;     /* 0 float32, 1 float16 */
     int measure;           /* NRH_MEASURE_NMI (--nmi) or NRH_MEASURE_LNCC (--lncc) */
     float lncc_sigma;      /* --lncc value: Gaussian standard deviation, < 0 voxels (-5) */
+    const float *affine;   /* -aff: reference -> floating 4x4, row-major (NULL: none).  reg_f3d
+                              starts its grid at the affine's positions (reg-lib/_reg_f3d.cpp:
+                              190-191); reg_f3d2 splits it into half affines on both grids */
 } nrh_f3d_config;
 int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                      const nrh_f3d_config *cfg, nrh_geom *grid_out, float *cpp_out, size_t cpp_capacity,
                      double *final_nmi, int *iterations);
 /* A whole symmetric registration: reg_f3d2's run (reg_base<T>::Run with the reg_f3d2 overrides,
- * reg-lib/_reg_f3d2.cpp, reg-lib/_reg_base.cpp:727-819) with NMI, no affine initialisation:
+ * reg-lib/_reg_f3d2.cpp, reg-lib/_reg_base.cpp:727-819) with NMI or LNCC and an optional affine:
  * symmetric velocity grids at the coarsest level (spacing < 0: voxels of the mean voxel size),
  * refined by subdivision per level; every evaluation on the device.  fw_out / bw_out (capacity
  * floats each, may be NULL) receive the final forward / backward velocity grids, final_nmi the

@@ -42,7 +42,8 @@ class nrh_f3d_config(ctypes.Structure):
     _fields_ = [("bins", ctypes.c_int), ("be_weight", ctypes.c_float), ("le_weight", ctypes.c_float),
                 ("spacing", ctypes.c_float), ("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int),
                 ("max_iterations", ctypes.c_int), ("padding", ctypes.c_float), ("use_pyramid", ctypes.c_int),
-                ("image_storage", ctypes.c_int), ("measure", ctypes.c_int), ("lncc_sigma", ctypes.c_float)]
+                ("image_storage", ctypes.c_int), ("measure", ctypes.c_int), ("lncc_sigma", ctypes.c_float),
+                ("affine", ctypes.POINTER(ctypes.c_float))]
 
 MEASURE_NMI = 0
 MEASURE_LNCC = 1

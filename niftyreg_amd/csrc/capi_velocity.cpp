@@ -2,15 +2,63 @@
 
 #include "f3d_handle.hpp"
 #include "host.hpp"
+#include "m44.hpp"
 #include <cstring>
 
 namespace nrh {
 
-// reg_createSymmetricControlPointGrids<float> without an affine (reg-lib/cpu/_reg_localTrans.cpp:
-// 132-358): corners of both images in mm (Mat44Mul<float>), the grid spans their bounding box
-// with one node of margin, axis-aligned, sform only.
+// operator*(mat44, mat44) of reg-lib/cpu/Maths.hpp:237-246: double sums, float results.
+void mat44_product(const float *A, const float *B, float *R) {
+    float T[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            volatile double acc = (double)A[i * 4] * (double)B[j];
+            acc = acc + (double)A[i * 4 + 1] * (double)B[4 + j];
+            acc = acc + (double)A[i * 4 + 2] * (double)B[8 + j];
+            acc = acc + (double)A[i * 4 + 3] * (double)B[12 + j];
+            T[i * 4 + j] = (float)acc;
+        }
+    std::memcpy(R, T, sizeof T);
+}
+
+// The half affines of reg_createSymmetricControlPointGrids (reg-lib/cpu/_reg_localTrans.cpp:
+// 150-166): NiftyReg::Mat44Logm (float -> double logarithm -> float), the float matrix times 0.5,
+// NiftyReg::Mat44Expm (float -> double exponential -> float); the backward half from
+// nifti_mat44_inverse of the affine.  Returns 1 when the reference stores them in the grids'
+// extensions (both halves differ from the identity, :320-353), 0 when it stores none.
+int half_affines(const float A[16], float hf[16], float hb[16]) {
+    static const GaussLegendre8 gl = gauss_legendre8();
+    auto half = [&](const float *M, float *out) {
+        double m[16], l[16], h[16], e[16];
+        for (int i = 0; i < 16; ++i) m[i] = M[i];
+        NRH_REQUIRE(d4_logm(m, gl, l), NRH_ERR_NUMERIC,
+                    "the matrix logarithm of the affine failed (a matrix with a non-positive determinant)");
+        for (int i = 0; i < 16; ++i) h[i] = (double)(float)((double)(float)l[i] * 0.5);
+        d4_expm(h, e);
+        for (int i = 0; i < 16; ++i) out[i] = (float)e[i];
+    };
+    float inv[16];
+    nifti_inverse(A, inv);
+    half(A, hf);
+    half(inv, hb);
+    auto identity = [](const float *M) {
+        for (int i = 0; i < 16; ++i)
+            if (M[i] != ((i % 5) == 0 ? 1.f : 0.f)) return false;
+        return true;
+    };
+    return !identity(hf) && !identity(hb);
+}
+
+// reg_createSymmetricControlPointGrids<float> (reg-lib/cpu/_reg_localTrans.cpp:132-358): both
+// images' spaces carried half-way by the half affines (identity when NULL), their corners in mm
+// (Mat44Mul<float>), the grid spans the bounding box with one node of margin, axis-aligned, sform
+// only.
 void symmetric_grids(const nrh_geom& ref, const nrh_geom& flo, const float spacing[3], nrh_geom& grid, float *fw,
-                     float *bw) {
+                     float *bw, const float *halfFw, const float *halfBw) {
+    static const float eye[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
+    float refSpace[16], floSpace[16];
+    mat44_product(halfFw ? halfFw : eye, ref.xyz, refSpace);
+    mat44_product(halfBw ? halfBw : eye, flo.xyz, floSpace);
     auto mul = [](const float *m, const float in[3], float out[3]) {
         for (int r = 0; r < 3; ++r)
             out[r] = (float)((double)m[r * 4] * (double)in[0] + (double)m[r * 4 + 1] * (double)in[1] +
@@ -21,8 +69,8 @@ void symmetric_grids(const nrh_geom& ref, const nrh_geom& flo, const float spaci
         const float rc[3] = { (c & 1) ? (float)ref.nx : 0.f, (c & 2) ? (float)ref.ny : 0.f, (c & 4) ? (float)ref.nz : 0.f };
         const float fc[3] = { (c & 1) ? (float)flo.nx : 0.f, (c & 2) ? (float)flo.ny : 0.f, (c & 4) ? (float)flo.nz : 0.f };
         float ro[3], fo[3];
-        mul(ref.xyz, rc, ro);
-        mul(flo.xyz, fc, fo);
+        mul(refSpace, rc, ro);
+        mul(floSpace, fc, fo);
         for (int r = 0; r < 3; ++r) {
             if (c == 0) {
                 mn[r] = fo[r] < ro[r] ? fo[r] : ro[r];
@@ -85,9 +133,32 @@ int nrh_create_symmetric_control_point_grids(const nrh_geom *ref, const nrh_geom
         NRH_REQUIRE(spacing_mm[0] > 0 && spacing_mm[1] > 0 && spacing_mm[2] > 0, NRH_ERR_ARG,
                     "grid spacing must be positive (mm)");
         NRH_REQUIRE(ref->nz > 1 && flo->nz > 1, NRH_ERR_UNSUPPORTED, "symmetric grids are 3-D only");
-        NRH_REQUIRE(affine == nullptr, NRH_ERR_UNSUPPORTED,
-                    "symmetric grids with a half-affine initialisation are not supported");
-        symmetric_grids(*ref, *flo, spacing_mm, *grid, fw, bw);
+        float hf[16], hb[16];
+        if (affine) half_affines(affine, hf, hb);
+        symmetric_grids(*ref, *flo, spacing_mm, *grid, fw, bw, affine ? hf : nullptr, affine ? hb : nullptr);
+    });
+}
+
+int nrh_symmetric_half_affines(const float affine[16], float half_fw[16], float half_bw[16], int *extensions) {
+    return guard([&] {
+        NRH_REQUIRE(affine && half_fw && half_bw, NRH_ERR_ARG, "null argument");
+        const int ext = half_affines(affine, half_fw, half_bw);
+        if (extensions) *extensions = ext;
+    });
+}
+
+int nrh_f3d_set_velocity_affines(nrh_f3d *h, const float *ext0, const float *ext1, const float *transformation) {
+    return guard([&] {
+        NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
+        h->velExt0 = ext0 != nullptr;
+        h->velExt1 = ext1 != nullptr;
+        h->velTrans = transformation != nullptr;
+        if (ext0) std::memcpy(h->velExt0M, ext0, sizeof h->velExt0M);
+        if (ext1) std::memcpy(h->velExt1M, ext1, sizeof h->velExt1M);
+        if (transformation) std::memcpy(h->velTransM, transformation, sizeof h->velTransM);
+        h->velFieldVersion = 0;
+        h->velChainVersion = 0;
+        h->pairStateVersion = 0;
     });
 }
 

@@ -286,6 +286,24 @@ void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& f
         create_grid(g0, gsp, grid, nullptr);
         cpp.resize(3 * voxels(grid));
         create_grid(g0, gsp, grid, cpp.data());
+        if (cfg.affine) {
+            // reg_affine_getDeformationField(affine, grid) (reg-lib/_reg_f3d.cpp:190-191,
+            // reg-lib/cpu/_reg_globalTrans.cpp:74-133): node positions A S i in double
+            float T[16];
+            mat44_product(cfg.affine, grid.xyz, T);
+            const size_t G = voxels(grid);
+            size_t i = 0;
+            for (int z = 0; z < grid.nz; ++z)
+                for (int y = 0; y < grid.ny; ++y)
+                    for (int x = 0; x < grid.nx; ++x, ++i)
+                        for (int r = 0; r < 3; ++r) {
+                            volatile double v = (double)T[r * 4] * (double)x;
+                            v = v + (double)T[r * 4 + 1] * (double)y;
+                            v = v + (double)T[r * 4 + 2] * (double)z;
+                            v = v + (double)T[r * 4 + 3];
+                            cpp[i + r * G] = (float)v;
+                        }
+        }
     }
     size_t maxIter = (size_t)cfg.max_iterations * (size_t)std::pow(2.0, toPerform - 1);
     totalIterations = 0;
@@ -427,12 +445,23 @@ void register_pair_sym(const nrh_geom& refG, const float *refData, const nrh_geo
         if (gsp[a] < 0) gsp[a] *= -(pd[a][0] + pd[a][1]) / 2.f;
         gsp[a] *= std::pow(2.f, (float)(levels - 1));
     }
+    // reg_createSymmetricControlPointGrids with the half affines (reg-lib/_reg_f3d2.cpp:560-565);
+    // the backward content's transformation is the affine's inverse (:602-603)
+    float hf[16], hb[16], affInv[16];
+    int ext = 0;
+    const float *halfF = nullptr, *halfB = nullptr;
+    if (cfg.affine) {
+        ext = half_affines(cfg.affine, hf, hb);
+        halfF = hf;
+        halfB = hb;
+        mat44_inverse(cfg.affine, affInv);
+    }
     nrh_geom grid;
     std::vector<float> fw, bw;
-    symmetric_grids(rp[0].geom, fp[0].geom, gsp, grid, nullptr, nullptr);
+    symmetric_grids(rp[0].geom, fp[0].geom, gsp, grid, nullptr, nullptr, halfF, halfB);
     fw.resize(3 * voxels(grid));
     bw.resize(3 * voxels(grid));
-    symmetric_grids(rp[0].geom, fp[0].geom, gsp, grid, fw.data(), bw.data());
+    symmetric_grids(rp[0].geom, fp[0].geom, gsp, grid, fw.data(), bw.data(), halfF, halfB);
     int steps = 6;  // intent_p2 of both grids
     size_t maxIter = (size_t)cfg.max_iterations * (size_t)std::pow(2.0, toPerform - 1);
     totalIterations = 0;
@@ -479,6 +508,10 @@ void register_pair_sym(const nrh_geom& refG, const float *refData, const nrh_geo
         check(nrh_f3d_set_control_points(hb, bw.data()));
         check(nrh_f3d_set_velocity_grid(hf, steps));
         check(nrh_f3d_set_velocity_grid(hb, steps));
+        if (cfg.affine) {
+            check(nrh_f3d_set_velocity_affines(hf, ext ? halfF : nullptr, ext ? halfF : nullptr, cfg.affine));
+            check(nrh_f3d_set_velocity_affines(hb, ext ? halfB : nullptr, ext ? halfB : nullptr, affInv));
+        }
         if (p.measure == NRH_MEASURE_LNCC) {
             check(nrh_f3d_rescale_intensities(hf, 0.f, 1.f, 0.f, 1.f));
             check(nrh_f3d_rescale_intensities(hb, 0.f, 1.f, 0.f, 1.f));

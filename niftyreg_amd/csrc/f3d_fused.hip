@@ -896,7 +896,8 @@ void fused_gradient(nrh_f3d *h, float weight) {
     project_to_nodes(h, p, p->gvox.p, weight);
 }
 
-bool fused_projection_supported(nrh_f3d *h) { return plan_of(h)->ok; }
+// (a grid extension 0 puts the nodes off the voxel lattice: the convolution path handles it)
+bool fused_projection_supported(nrh_f3d *h) { return !(h->velocity && h->velExt0) && plan_of(h)->ok; }
 
 void fused_project(nrh_f3d *h, const float *g, float weight) {
     FusedPlan *p = plan_of(h);

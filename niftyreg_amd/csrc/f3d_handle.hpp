@@ -63,6 +63,11 @@ struct nrh_f3d {
     // cppVersion at which the symmetric evaluation left the warped image and the joint histogram
     // of the current exponentiation (0: some other operation changed them since)
     uint64_t pairStateVersion = 0;
+    // affine components of a velocity grid: its NIfTI extensions 0 (applied before the spline,
+    // removed before the squaring and restored after it) and 1 (composed after it), and the
+    // content's transformation matrix (removed from the opposite direction's intermediate fields)
+    bool velExt0 = false, velExt1 = false, velTrans = false;
+    float velExt0M[16], velExt1M[16], velTransM[16];
 
     double *pinned = nullptr;  // host-pinned scalar read-back
     double *published = nullptr, *publishedDev = nullptr;  // coherent host memory the device writes results to

@@ -6,6 +6,8 @@
 
 #include "device.hpp"
 #include "m33.hpp"
+#include "m44.hpp"
+#include "host.hpp"
 #include "f3d_handle.hpp"
 #include <cmath>
 
@@ -1190,10 +1192,29 @@ float node_weight_ratio(const nrh_f3d *h, float weight) {
 
 void op_voxel_to_node(nrh_f3d *h, float weight) {
     NRH_REQUIRE(!h->voxelGrad.empty(), NRH_ERR_STATE, "no voxel-based gradient");
-    const Affine T = mat_mul_affine(h->ref.ijk, h->cpp.xyz);
+    Affine T = mat_mul_affine(h->ref.ijk, h->cpp.xyz);
+    Affine R = to_affine(h->flo.ijk);
+    if (h->velocity && h->velExt0) {
+        // the grid's extension 0 maps the grid into the reference space (reg-lib/cpu/
+        // _reg_localTrans.cpp:1568-1592): inverse(A) between the grid and the image, and
+        // inverse(A)'s 3x3 times the floating orientation in the reorientation
+        float inv[16], gridToRef[16];
+        nifti_inverse(h->velExt0M, inv);
+        mat44_product(inv, h->cpp.xyz, gridToRef);
+        T = mat_mul_affine(h->ref.ijk, gridToRef);
+        M33 a, o;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                a.m[i][j] = h->velExt0M[i * 4 + j];
+                o.m[i][j] = h->flo.ijk[i * 4 + j];
+            }
+        const M33 r = m33_mul(m33_inverse(a), o);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) R.m[i * 4 + j] = r.m[i][j];
+    }
     const float w = node_weight_ratio(h, weight);
     hipLaunchKernelGGL(k_voxel_to_node, dim3(blocks_for(h->G)), dim3(kBlock), 0, h->stream, h->voxelGrad.p, h->ref.nx,
-                       h->ref.ny, h->ref.nz, h->cpp.nx, h->cpp.ny, h->cpp.nz, T, to_affine(h->flo.ijk), w, h->nodeGrad.p);
+                       h->ref.ny, h->ref.nz, h->cpp.nx, h->cpp.ny, h->cpp.nz, T, R, w, h->nodeGrad.p);
     NRH_HIP(hipGetLastError());
 }
 

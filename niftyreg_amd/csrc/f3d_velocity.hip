@@ -16,6 +16,7 @@
 
 #include "device.hpp"
 #include "f3d_handle.hpp"
+#include "host.hpp"
 #include "m33.hpp"
 #include <cmath>
 
@@ -46,6 +47,31 @@ struct VelGrid {
     size_t G;
     Affine xyz, ijk;
 };
+
+// Affine parts of a velocity grid on one field geometry.  pre: the grid's extension 0, applied to
+// the identity before the spline (reg-lib/cpu/_reg_localTrans.cpp:1478-1487), removed from the
+// flow as the affine-only field A S x (:3508-3515) and added back after the squaring (:3569-3573);
+// post: extension 1, composed last (:3576-3577); sub: the opposite content's transformation,
+// whose displacement the gradient exponentiation removes from each intermediate field
+// (reg-lib/Compute.cpp:303-316).  Rows 0-2 of each matrix.
+struct VelAff {
+    int pre, post, sub;
+    float A[12];   // extension 0
+    float AS[12];  // extension 0 times the field's sform
+    float B[12];   // extension 1
+    float TS[12];  // the transformation times the field's sform
+};
+
+// Mat44Mul<double> (reg-lib/cpu/Maths.hpp:272-286) of a position, rounded to float as
+// reg_affine_deformationField3D stores it (reg-lib/cpu/_reg_globalTrans.cpp:74-133).
+__device__ __forceinline__ float3 mul_d(const float *m, double x, double y, double z) {
+#pragma clang fp contract(off)
+    float3 r;
+    r.x = (float)((double)m[0] * x + (double)m[1] * y + (double)m[2] * z + (double)m[3]);
+    r.y = (float)((double)m[4] * x + (double)m[5] * y + (double)m[6] * z + (double)m[7]);
+    r.z = (float)((double)m[8] * x + (double)m[9] * y + (double)m[10] * z + (double)m[11]);
+    return r;
+}
 
 // voxel -> mm in float, as reg_getDeformationFromDisplacement_3D (reg-lib/cpu/_reg_tools.cpp:
 // 2107-2150) adds it
@@ -104,13 +130,15 @@ __device__ __forceinline__ uint32_t float_key(float v) {
 // order) applied to the identity field, as reg_spline_getFlowFieldFromVelocityGrid (:3465-3491)
 // does.  Returns the displacement flow - x and the position x.  When every lane of the wave has
 // its 4x4x4 support inside the grid, each slab of 16 nodes is loaded at once (48 loads in flight)
-// before its terms are summed in the reference's order.
-__device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, size_t i, bool active, float3& disp,
-                                        float3& pos) {
+// before its terms are summed in the reference's order.  With an extension 0 the spline is
+// evaluated at A x and the displacement is taken from the affine-only field A S i in double.
+__device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, const VelAff& a, size_t i, bool active,
+                                        float3& disp, float3& pos) {
 #pragma clang fp contract(off)
     const size_t ii = active ? i : 0;
     const int x = (int)(ii % f.nx), y = (int)((ii / f.nx) % f.ny), z = (int)(ii / ((size_t)f.nx * f.ny));
-    const float3 p = id_pos(f.xyz, x, y, z);
+    const float3 p0 = id_pos(f.xyz, x, y, z);
+    const float3 p = a.pre ? mul_d(a.A, p0.x, p0.y, p0.z) : p0;
     const float3 v = to_vox(g.ijk, p);
     int pre[3];
     float b[3][4];
@@ -176,19 +204,25 @@ __device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, si
             }
         }
     }
-    disp = make_float3(r0 - p.x, r1 - p.y, r2 - p.z);
-    pos = p;
+    if (a.pre) {
+        const float3 ao = mul_d(a.AS, (double)x, (double)y, (double)z);
+        disp = make_float3((float)((double)r0 - (double)ao.x), (float)((double)r1 - (double)ao.y),
+                           (float)((double)r2 - (double)ao.z));
+    } else {
+        disp = make_float3(r0 - p.x, r1 - p.y, r2 - p.z);
+    }
+    pos = p0;
 }
 
 // The scaled flow, the first field of the squaring (reg-lib/cpu/_reg_localTrans.cpp:3546-3561):
 // (flow - x) / scaling + x.
-__global__ void __launch_bounds__(kBlock) k_vel_flow_scaled(VelGrid g, FieldGeom f, double scaling,
+__global__ void __launch_bounds__(kBlock) k_vel_flow_scaled(VelGrid g, FieldGeom f, VelAff a, double scaling,
                                                             float *__restrict__ out) {
 #pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = i < f.n;
     float3 d, p;
-    flow_at(g, f, i, active, d, p);
+    flow_at(g, f, a, i, active, d, p);
     if (!active) return;
     out[i] = (float)((double)d.x / scaling) + p.x;
     out[i + f.n] = (float)((double)d.y / scaling) + p.y;
@@ -197,8 +231,8 @@ __global__ void __launch_bounds__(kBlock) k_vel_flow_scaled(VelGrid g, FieldGeom
 
 // The flow's displacement flow - x, with its extremes folded into keys[0] (min) and keys[1]
 // (max): grid-stride loop, one block-wide reduction and one atomic pair per block.
-__global__ void __launch_bounds__(kBlock) k_vel_flow_disp(VelGrid g, FieldGeom f, float *__restrict__ out,
-                                                          uint32_t *__restrict__ keys) {
+__global__ void __launch_bounds__(kBlock) k_vel_flow_disp(VelGrid g, FieldGeom f, VelAff a,
+                                                          float *__restrict__ out, uint32_t *__restrict__ keys) {
 #pragma clang fp contract(off)
     __shared__ float sLo[kBlock / 64], sHi[kBlock / 64];
     float lo = INFINITY, hi = -INFINITY;
@@ -208,7 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_vel_flow_disp(VelGrid g, FieldGeom f
         const size_t i = r * stride + (size_t)blockIdx.x * kBlock + threadIdx.x;
         const bool active = i < f.n;
         float3 d, p;
-        flow_at(g, f, i, active, d, p);
+        flow_at(g, f, a, i, active, d, p);
         if (active) {
             out[i] = d.x;
             out[i + f.n] = d.y;
@@ -247,6 +281,39 @@ __global__ void __launch_bounds__(kBlock) k_vel_scale(const float *disp, FieldGe
     out[i] = (float)((double)disp[i] / scaling) + p.x;
     out[i + f.n] = (float)((double)disp[i + f.n] / scaling) + p.y;
     out[i + 2 * f.n] = (float)((double)disp[i + 2 * f.n] / scaling) + p.z;
+}
+
+// The affine parts put back on `count` fields after the squaring: with an extension 0 the
+// displacement (float, reg_getDisplacementFromDeformation) plus the affine-only field (double
+// sum, reg_tools_addImageToImage); then extension 1 composed (Mat44Mul<double>); then, for the
+// gradient exponentiation, the displacement of the opposite content's transformation
+// (T S i in double, minus x in float) subtracted in double.
+__global__ void __launch_bounds__(kBlock) k_vel_affine_parts(float *fields, int count, FieldGeom f, VelAff a) {
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= f.n) return;
+    const int x = (int)(i % f.nx), y = (int)((i / f.nx) % f.ny), z = (int)(i / ((size_t)f.nx * f.ny));
+    const float3 p0 = id_pos(f.xyz, x, y, z);
+    const float3 ao = a.pre ? mul_d(a.AS, (double)x, (double)y, (double)z) : make_float3(0.f, 0.f, 0.f);
+    float3 td = make_float3(0.f, 0.f, 0.f);
+    if (a.sub) {
+        const float3 t = mul_d(a.TS, (double)x, (double)y, (double)z);
+        td = make_float3(t.x - p0.x, t.y - p0.y, t.z - p0.z);
+    }
+    for (int k = 0; k < count; ++k) {
+        float *F = fields + (size_t)k * 3 * f.n;
+        float3 v = make_float3(F[i], F[i + f.n], F[i + 2 * f.n]);
+        if (a.pre)
+            v = make_float3((float)((double)(v.x - p0.x) + (double)ao.x), (float)((double)(v.y - p0.y) + (double)ao.y),
+                            (float)((double)(v.z - p0.z) + (double)ao.z));
+        if (a.post) v = mul_d(a.B, v.x, v.y, v.z);
+        if (a.sub)
+            v = make_float3((float)((double)v.x - (double)td.x), (float)((double)v.y - (double)td.y),
+                            (float)((double)v.z - (double)td.z));
+        F[i] = v.x;
+        F[i + f.n] = v.y;
+        F[i + 2 * f.n] = v.z;
+    }
 }
 
 // reg_defField_compose3D without a mask (reg-lib/cpu/_reg_localTrans.cpp:2333-2457): out[i] is
@@ -555,6 +622,27 @@ float key_to_float(uint32_t k) {
     return v;
 }
 
+// The affine parts of grid handle `grid` on field geometry `field`; `sub` (may be null) supplies
+// the transformation removed by the gradient exponentiation.
+VelAff vel_aff(const nrh_f3d *grid, const nrh_geom& field, const nrh_f3d *sub) {
+    VelAff a{};
+    a.pre = grid->velExt0;
+    a.post = grid->velExt1;
+    a.sub = sub && sub->velTrans;
+    float m[16];
+    if (a.pre) {
+        std::memcpy(a.A, grid->velExt0M, sizeof a.A);
+        mat44_product(grid->velExt0M, field.xyz, m);
+        std::memcpy(a.AS, m, sizeof a.AS);
+    }
+    if (a.post) std::memcpy(a.B, grid->velExt1M, sizeof a.B);
+    if (a.sub) {
+        mat44_product(sub->velTransM, field.xyz, m);
+        std::memcpy(a.TS, m, sizeof a.TS);
+    }
+    return a;
+}
+
 double scaling_of(int steps) {
     const int sq = steps < 0 ? -steps : steps;
     const double s = (double)(float)std::pow(2.0, sq);
@@ -604,8 +692,8 @@ void resample_gradient_factors(const nrh_geom& g, float reo[9], float sp[3]) {
 
 }  // namespace
 
-// Compute::GetDefFieldFromVelocityGrid (reg-lib/Compute.cpp:223-228) for a grid without affine
-// extensions: reg_spline_getDefFieldFromVelocityGrid into the handle's deformation field.
+// Compute::GetDefFieldFromVelocityGrid (reg-lib/Compute.cpp:223-228):
+// reg_spline_getDefFieldFromVelocityGrid into the handle's deformation field.
 void op_vel_def_field(nrh_f3d *h, bool update) {
     NRH_REQUIRE(h->velocity, NRH_ERR_STATE, "the control points are not a velocity grid");
     // the field already holds this exponentiation: same control points and the same step rule
@@ -622,6 +710,7 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
     ensure_field(h);
     const FieldGeom f = field_geom(h->ref);
     const VelGrid g = vel_grid(h);
+    const VelAff a = vel_aff(h, h->ref, nullptr);
     int steps = h->velSteps;
     const bool chosen = update || steps == 0;
     int sq;
@@ -630,7 +719,7 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
         const uint32_t init[2] = { 0xffffffffu, 0u };
         h->velKeys.alloc(2);
         NRH_HIP(hipMemcpyAsync(h->velKeys.p, init, sizeof init, hipMemcpyHostToDevice, h->stream));
-        hipLaunchKernelGGL(k_vel_flow_disp, dim3(flow_disp_blocks(f.n)), dim3(kBlock), 0, h->stream, g, f,
+        hipLaunchKernelGGL(k_vel_flow_disp, dim3(flow_disp_blocks(f.n)), dim3(kBlock), 0, h->stream, g, f, a,
                            h->velTmp.p, h->velKeys.p);
         NRH_HIP(hipGetLastError());
         uint32_t keys[2];
@@ -650,11 +739,13 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
         hipLaunchKernelGGL(k_vel_scale, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->velTmp.p, f,
                            scaling_of(steps), field_k(0));
     } else {
-        hipLaunchKernelGGL(k_vel_flow_scaled, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, scaling_of(steps),
+        hipLaunchKernelGGL(k_vel_flow_scaled, blocks_for(f.n), dim3(kBlock), 0, h->stream, g, f, a, scaling_of(steps),
                            field_k(0));
     }
     NRH_HIP(hipGetLastError());
     for (int k = 0; k < sq; ++k) launch_compose(field_k(k), f, field_k(k), f.n, field_k(k + 1), h->stream);
+    if (a.pre || a.post)
+        hipLaunchKernelGGL(k_vel_affine_parts, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->field.p, 1, f, a);
     NRH_HIP(hipGetLastError());
     h->velSteps = steps;
     h->velFieldVersion = h->cppVersion;
@@ -665,18 +756,22 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
 
 // reg_spline_getIntermediateDefFieldFromVelGrid (reg-lib/cpu/_reg_localTrans.cpp:3616-3683) of
 // grid `g` (steps squarings) on geometry f: fields[0..count-1] (count <= |steps| + 1).
-static void intermediate_fields(const VelGrid& g, const FieldGeom& f, int steps, float *fields, int count,
-                                hipStream_t s) {
-    hipLaunchKernelGGL(k_vel_flow_scaled, blocks_for(f.n), dim3(kBlock), 0, s, g, f, scaling_of(steps), fields);
+// The affine parts (extensions of the grid, the transformation removed) follow the squaring.
+static void intermediate_fields(const VelGrid& g, const FieldGeom& f, const VelAff& a, int steps, float *fields,
+                                int count, hipStream_t s) {
+    hipLaunchKernelGGL(k_vel_flow_scaled, blocks_for(f.n), dim3(kBlock), 0, s, g, f, a, scaling_of(steps), fields);
     for (int k = 0; k + 1 < count; ++k)
         launch_compose(fields + (size_t)k * 3 * f.n, f, fields + (size_t)k * 3 * f.n, f.n,
                        fields + (size_t)(k + 1) * 3 * f.n, s);
+    if (a.pre || a.post || a.sub)
+        hipLaunchKernelGGL(k_vel_affine_parts, blocks_for(f.n), dim3(kBlock), 0, s, fields, count, f, a);
     NRH_HIP(hipGetLastError());
 }
 
-// Compute::ExponentiateGradient (reg-lib/Compute.cpp:283-326) without affine components: the
-// voxel-based gradient of h, resampled along the intermediate fields of bw's velocity grid
-// (evaluated on h's reference geometry) and accumulated, then divided by 2^|steps of bw|.
+// Compute::ExponentiateGradient (reg-lib/Compute.cpp:283-326): the voxel-based gradient of h,
+// resampled along the intermediate fields of bw's velocity grid (evaluated on h's reference
+// geometry, less the displacement of bw's transformation matrix) and accumulated, then divided by
+// 2^|steps of bw|.
 void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther) {
     NRH_REQUIRE(bw->velocity, NRH_ERR_STATE, "the backward control points are not a velocity grid");
     NRH_REQUIRE(!h->voxelGrad.empty(), NRH_ERR_STATE, "voxel-based gradient not computed");
@@ -691,12 +786,15 @@ void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther) {
     const bool sameGeom = bw->ref.nx == h->ref.nx && bw->ref.ny == h->ref.ny && bw->ref.nz == h->ref.nz &&
                           std::memcmp(bw->ref.xyz, h->ref.xyz, sizeof h->ref.xyz) == 0 &&
                           std::memcmp(bw->ref.ijk, h->ref.ijk, sizeof h->ref.ijk) == 0;
+    // (the chain holds the fields before their affine parts)
+    const VelAff a = vel_aff(bw, h->ref, bw);
+    const bool affine = a.pre || a.post || a.sub;
     const float *fields;
-    if (sameGeom && bw->velChainVersion == bw->cppVersion && bw->velFieldSteps == steps) {
+    if (sameGeom && !affine && bw->velChainVersion == bw->cppVersion && bw->velFieldSteps == steps) {
         fields = bw->velChain.p;
     } else {
         h->velFields.alloc((size_t)sq * 3 * f.n);
-        intermediate_fields(vel_grid(bw), f, steps, h->velFields.p, sq, h->stream);
+        intermediate_fields(vel_grid(bw), f, a, steps, h->velFields.p, sq, h->stream);
         fields = h->velFields.p;
     }
     h->velTmp.alloc(3 * f.n);

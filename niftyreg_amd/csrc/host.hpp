@@ -13,7 +13,11 @@ void create_grid(const nrh_geom& ref, const float spacing[3], nrh_geom& grid, fl
 
 // reg_createSymmetricControlPointGrids<float> without an affine (reg-lib/cpu/_reg_localTrans.cpp:132-358)
 void symmetric_grids(const nrh_geom& ref, const nrh_geom& flo, const float spacing[3], nrh_geom& grid, float *fw,
-                     float *bw);
+                     float *bw, const float *halfFw = nullptr, const float *halfBw = nullptr);
+// operator*(mat44, mat44) of reg-lib/cpu/Maths.hpp:237-246 (double sums, float results)
+void mat44_product(const float *A, const float *B, float *R);
+// reg_createSymmetricControlPointGrids's half affines; 1 when the grids carry them as extensions
+int half_affines(const float A[16], float hf[16], float hb[16]);
 
 struct Level {
     nrh_geom geom;

@@ -159,6 +159,12 @@ class F3dContent:
         """Mark the control points as a velocity grid (reg_f3d2::Initialise, intent_p2 = steps)."""
         self._op("nrh_f3d_set_velocity_grid", int(steps))
 
+    def set_velocity_affines(self, ext0=None, ext1=None, transformation=None):
+        """The grid's extensions 0 / 1 and the content's transformation matrix (4x4 each or None)."""
+        mats = [None if m is None else np.ascontiguousarray(m, np.float32).reshape(16) for m in (ext0, ext1, transformation)]
+        self._keep_affines = mats
+        self._op("nrh_f3d_set_velocity_affines", *[None if m is None else _lib.fptr(m) for m in mats])
+
     @property
     def velocity_steps(self):
         s = ctypes.c_int()
@@ -287,11 +293,14 @@ class F3dContent:
 
 def register(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3,
              levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True, image_storage=0,
-             measure=_lib.MEASURE_NMI, lncc_sigma=-5.0):
+             measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, affine=None):
     """Full reg_f3d registration of one pair on the device (reg_base<float>::Run).
 
+    affine: optional reference -> floating 4x4 initialisation (-aff).
     Returns (grid Geom, control points planar float32, final similarity value, iterations)."""
     cfg = _lib.nrh_f3d_config()
+    aff = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
+    cfg.affine = None if aff is None else _lib.fptr(aff)
     cfg.bins, cfg.be_weight, cfg.le_weight, cfg.spacing = bins, be, le, spacing
     cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
     cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), image_storage
@@ -340,15 +349,20 @@ class F3d2Pair:
 
 
 def register_symmetric(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.01, spacing=-5.0,
-                       levels=3, levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True):
-    """Full reg_f3d2 registration of one pair on the device (symmetric velocity grids, NMI).
+                       levels=3, levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True,
+                       measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, affine=None):
+    """Full reg_f3d2 registration of one pair on the device (symmetric velocity grids, NMI or LNCC).
 
-    Returns (grid Geom, forward grid, backward grid, final NMI sum of both directions, iterations)."""
+    affine: optional reference -> floating 4x4 (-aff), split into half affines on both grids.
+    Returns (grid Geom, forward grid, backward grid, final similarity sum of both directions,
+    iterations)."""
     cfg = _lib.nrh_f3d_config()
     cfg.bins, cfg.be_weight, cfg.le_weight, cfg.spacing = bins, be, le, spacing
     cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
     cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), 0
-    cfg.measure, cfg.lncc_sigma = _lib.MEASURE_NMI, -5.0
+    cfg.measure, cfg.lncc_sigma = measure, lncc_sigma
+    aff = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
+    cfg.affine = None if aff is None else _lib.fptr(aff)
     ref = np.ascontiguousarray(ref, np.float32).reshape(-1)
     flo = np.ascontiguousarray(flo, np.float32).reshape(-1)
     rg, fg = ref_geom.to_c(), flo_geom.to_c()
@@ -358,7 +372,8 @@ def register_symmetric(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.0
            zip(ref_geom.dims, ref_geom.pixdim, flo_geom.dims, flo_geom.pixdim,
                np.abs(ref_geom.xyz[:3, 3] - flo_geom.xyz[:3, 3]))]
     sp = [abs(spacing) * (d + e) / 2 if spacing < 0 else spacing for d, e in zip(ref_geom.pixdim, flo_geom.pixdim)]
-    cap = 3 * int(np.prod([int(np.ceil(x / s + 8)) for x, s in zip(ext, sp)]))
+    grow = 1.0 if affine is None else 2.0  # the half affines may rotate the bounding box
+    cap = 3 * int(np.prod([int(np.ceil(grow * x / s + 8)) for x, s in zip(ext, sp)]))
     fw = np.zeros(cap, np.float32)
     bw = np.zeros(cap, np.float32)
     nmi = ctypes.c_double()

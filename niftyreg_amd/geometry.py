@@ -108,18 +108,33 @@ def create_control_point_grid(ref: Geom, spacing_mm):
     return geom, data
 
 
-def create_symmetric_control_point_grids(ref: Geom, flo: Geom, spacing_mm):
-    """reg_createSymmetricControlPointGrids<float> without an affine (reg-lib/cpu/_reg_localTrans.cpp:132-358).
+def create_symmetric_control_point_grids(ref: Geom, flo: Geom, spacing_mm, affine=None):
+    """reg_createSymmetricControlPointGrids<float> (reg-lib/cpu/_reg_localTrans.cpp:132-358).
 
-    Returns (grid Geom, forward identity positions, backward identity positions)."""
+    affine: optional reference -> floating 4x4.  Returns (grid Geom, forward identity positions,
+    backward identity positions); the half affines come from symmetric_half_affines."""
     sp = np.asarray(spacing_mm, np.float32)
+    a = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
+    ap = None if a is None else _lib.fptr(a)
     g = _lib.nrh_geom()
     rc, fc = ref.to_c(), flo.to_c()
-    _lib.call("nrh_create_symmetric_control_point_grids", ctypes.byref(rc), ctypes.byref(fc), _lib.fptr(sp), None,
+    _lib.call("nrh_create_symmetric_control_point_grids", ctypes.byref(rc), ctypes.byref(fc), _lib.fptr(sp), ap,
               ctypes.byref(g), None, None)
     geom = Geom.from_c(g)
     fw = np.zeros(3 * geom.nvox, np.float32)
     bw = np.zeros(3 * geom.nvox, np.float32)
-    _lib.call("nrh_create_symmetric_control_point_grids", ctypes.byref(rc), ctypes.byref(fc), _lib.fptr(sp), None,
+    _lib.call("nrh_create_symmetric_control_point_grids", ctypes.byref(rc), ctypes.byref(fc), _lib.fptr(sp), ap,
               ctypes.byref(g), _lib.fptr(fw), _lib.fptr(bw))
     return geom, fw, bw
+
+
+def symmetric_half_affines(affine):
+    """The half affines of reg_createSymmetricControlPointGrids (reg-lib/cpu/_reg_localTrans.cpp:150-166).
+
+    Returns (half forward 4x4, half backward 4x4, whether the grids carry them as extensions)."""
+    a = np.ascontiguousarray(affine, np.float32).reshape(16)
+    hf = np.zeros(16, np.float32)
+    hb = np.zeros(16, np.float32)
+    ext = ctypes.c_int()
+    _lib.call("nrh_symmetric_half_affines", _lib.fptr(a), _lib.fptr(hf), _lib.fptr(hb), ctypes.byref(ext))
+    return hf.reshape(4, 4), hb.reshape(4, 4), bool(ext.value)

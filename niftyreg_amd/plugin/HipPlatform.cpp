@@ -167,10 +167,22 @@ void HipF3dContent::EnsureVoxelGradientOnDevice() {
 
 void HipF3dContent::SyncVelocityGrid() {
     if (controlPointGrid->intent_p1 != SPLINE_VEL_GRID) NR_FATAL_ERROR("The control point grid is not a velocity grid");
-    if (controlPointGrid->num_ext > 0)
-        NR_FATAL_ERROR("Velocity grids with affine extensions are not supported on the HIP platform");
-    if (transformationMatrix) NR_FATAL_ERROR("Velocity grids with an affine transformation are not supported on HIP");
     Check(nrh_f3d_set_velocity_grid(handle, (int)controlPointGrid->intent_p2), "nrh_f3d_set_velocity_grid");
+    // the grid's extensions 0 / 1 (half affines) and the content's transformation matrix
+    float ext[2][16], trans[16];
+    const float *e[2] = { nullptr, nullptr };
+    for (int k = 0; k < 2 && k < controlPointGrid->num_ext; ++k) {
+        if (!controlPointGrid->ext_list[k].edata) continue;
+        const mat44 *m = reinterpret_cast<const mat44*>(controlPointGrid->ext_list[k].edata);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) ext[k][i * 4 + j] = m->m[i][j];
+        e[k] = ext[k];
+    }
+    if (transformationMatrix)
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) trans[i * 4 + j] = transformationMatrix->m[i][j];
+    Check(nrh_f3d_set_velocity_affines(handle, e[0], e[1], transformationMatrix ? trans : nullptr),
+          "nrh_f3d_set_velocity_affines");
 }
 
 // ---------------------------------------------------------------- HipAladinContent

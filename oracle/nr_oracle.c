@@ -22,6 +22,9 @@ static int floor_int(float x) {
 }
 
 /* Mat44Mul<float> (reg-lib/cpu/Maths.hpp:272-285): double accumulation, float result. */
+/* 3x3 float matrix (niftilib mat33) */
+typedef struct { float m[3][3]; } orc_m33;
+
 static void mat44_mul_f(const float *m, const float in[3], float out[3]) {
     for (int r = 0; r < 3; ++r)
         out[r] = (float)((double)M(m, r, 0) * (double)in[0] + (double)M(m, r, 1) * (double)in[1] +
@@ -537,12 +540,44 @@ void orc_convolve_voxel_gradient(const orc_geom *ref, const orc_geom *cpp, float
             orc_kernel_convolution_axis(ref, voxelGrad + comp * n, sp[axis], 3, NULL, axis);
 }
 
+static void voxel_to_node(const orc_geom *cpp, const orc_geom *ref, const float *voxelGrad,
+                          const float reorient_in[16], const float *ext0, float weight, float *nodeGrad);
 void orc_voxel_to_node(const orc_geom *cpp, const orc_geom *ref, const float *voxelGrad,
                        const float reorient_ijk[16], float weight, float *nodeGrad) {
+    voxel_to_node(cpp, ref, voxelGrad, reorient_ijk, NULL, weight, nodeGrad);
+}
+void orc_voxel_to_node_ext(const orc_geom *cpp, const orc_geom *ref, const float *voxelGrad,
+                           const float reorient_ijk[16], const float ext0[16], float weight, float *nodeGrad) {
+    voxel_to_node(cpp, ref, voxelGrad, reorient_ijk, ext0, weight, nodeGrad);
+}
+/* reg-lib/cpu/_reg_localTrans.cpp:1568-1592: a grid extension 0 (A) puts inverse(A) between the
+ * grid and the image, and inverse(A)'s 3x3 (nifti_mat33_inverse) before the reorientation
+ * (nifti_mat33_mul). */
+static orc_m33 m33_inverse(orc_m33 R);
+static orc_m33 m33_mul(orc_m33 A, orc_m33 B);
+static void voxel_to_node(const orc_geom *cpp, const orc_geom *ref, const float *voxelGrad,
+                          const float reorient_in[16], const float *ext0, float weight, float *nodeGrad) {
     const size_t nv = (size_t)ref->nx * ref->ny * ref->nz;
     const size_t ng = (size_t)cpp->nx * cpp->ny * cpp->nz;
-    float T[16];
-    mat44_mm(ref->ijk, cpp->xyz, T);
+    float T[16], reorient_ijk[16];
+    memcpy(reorient_ijk, reorient_in, sizeof reorient_ijk);
+    if (ext0) {
+        float inv[16], g2r[16];
+        orc_nifti_mat44_inverse(ext0, inv);
+        mat44_mm(inv, cpp->xyz, g2r);
+        mat44_mm(ref->ijk, g2r, T);
+        orc_m33 a, o;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                a.m[i][j] = M(ext0, i, j);
+                o.m[i][j] = M(reorient_in, i, j);
+            }
+        const orc_m33 r = m33_mul(m33_inverse(a), o);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) M(reorient_ijk, i, j) = r.m[i][j];
+    } else {
+        mat44_mm(ref->ijk, cpp->xyz, T);
+    }
     /* ratio = node spacing (sform column norm) / voxel pixdim, folded into the weight */
     for (int i = 0; i < 3; ++i) {
         float r = sqrtf(M(cpp->xyz, i, 0) * M(cpp->xyz, i, 0) + M(cpp->xyz, i, 1) * M(cpp->xyz, i, 1) +
@@ -1161,7 +1196,6 @@ void orc_lncc_gradient(const orc_geom *g, const float *ref, const float *war, co
 /* 3x3 float matrices with the arithmetic of niftilib's nifti_mat33_* helpers
  * (reg-io/niftilib/nifti1_io.c:1807-1960): double intermediates where those use double
  * (including the long double reciprocal of the determinant), float products and sums elsewhere. */
-typedef struct { float m[3][3]; } orc_m33;
 
 static orc_m33 m33_inverse(orc_m33 R) {
     double r11 = R.m[0][0], r12 = R.m[0][1], r13 = R.m[0][2];
@@ -1641,7 +1675,18 @@ static void grid_values_slided(const orc_geom *cpp, const float *d, int x0, int 
  * without affine extensions: the identity field of the reference geometry, then the composition
  * branch of reg_cubic_spline_getDeformationField3D (:903-1079, scalar order) evaluates the grid at
  * those positions.  flow: planar [3][N] positions. */
+/* Mat44Mul<double> (reg-lib/cpu/Maths.hpp:272-286) of a position, rounded to float. */
+static void mul_d(const float *m, double x, double y, double z, float out[3]) {
+    for (int r = 0; r < 3; ++r)
+        out[r] = (float)((double)M(m, r, 0) * x + (double)M(m, r, 1) * y + (double)M(m, r, 2) * z + (double)M(m, r, 3));
+}
+static void vel_flow(const orc_geom *ref, const orc_geom *cpp, const float *vel, const float *ext0, float *flow);
 void orc_vel_flow(const orc_geom *ref, const orc_geom *cpp, const float *vel, float *flow) {
+    vel_flow(ref, cpp, vel, NULL, flow);
+}
+/* With a grid extension 0 the identity is first composed with it (reg_affine_deformationField3D
+ * in composition mode, reg-lib/cpu/_reg_localTrans.cpp:1478-1487). */
+static void vel_flow(const orc_geom *ref, const orc_geom *cpp, const float *vel, const float *ext0, float *flow) {
     const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
     size_t i = 0;
     for (int z = 0; z < ref->nz; ++z)
@@ -1649,6 +1694,10 @@ void orc_vel_flow(const orc_geom *ref, const orc_geom *cpp, const float *vel, fl
             for (int x = 0; x < ref->nx; ++x, ++i) {
                 float real[3], vox[3], xb[4], yb[4], zb[4], gx[64], gy[64], gz[64];
                 identity_pos(ref->xyz, x, y, z, real);
+                if (ext0) {
+                    const float id[3] = { real[0], real[1], real[2] };
+                    mul_d(ext0, id[0], id[1], id[2], real);
+                }
                 to_voxel(cpp->ijk, real, vox);
                 int pre[3];
                 float *bases[3] = { xb, yb, zb };
@@ -1727,7 +1776,8 @@ void orc_def_field_compose(const orc_geom *g, const float *def, float *upd) {
 /* The flow as a displacement scaled by 2^-steps (sign of *steps: direction) back in deformation
  * form: reg_getDisplacementFromDeformation, reg_tools_divideValueToImage and
  * reg_getDeformationFromDisplacement (reg-lib/cpu/_reg_localTrans.cpp:3513-3561). */
-static void scaled_flow(const orc_geom *ref, float *flow, int steps, int update, int *stepsOut) {
+static void scaled_flow(const orc_geom *ref, float *flow, int steps, int update, int *stepsOut,
+                        const float *affOnly) {
     const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
     size_t i = 0;
     for (int z = 0; z < ref->nz; ++z)
@@ -1735,7 +1785,12 @@ static void scaled_flow(const orc_geom *ref, float *flow, int steps, int update,
             for (int x = 0; x < ref->nx; ++x, ++i) {
                 float p[3];
                 identity_pos(ref->xyz, x, y, z, p);
-                for (int r = 0; r < 3; ++r) flow[i + r * n] -= p[r];
+                for (int r = 0; r < 3; ++r) {
+                    /* reg_tools_subtractImageFromImage (double) of the affine-only field, or
+                     * reg_getDisplacementFromDeformation (float) */
+                    if (affOnly) flow[i + r * n] = (float)((double)flow[i + r * n] - (double)affOnly[i + r * n]);
+                    else flow[i + r * n] -= p[r];
+                }
             }
     int sq = 1;
     if (update || steps == 0) {
@@ -1772,11 +1827,66 @@ static void scaled_flow(const orc_geom *ref, float *flow, int steps, int update,
  * when `update` is set or it is 0. */
 void orc_vel_def_field(const orc_geom *ref, const orc_geom *cpp, const float *vel, int *steps, int update,
                        float *field) {
+    orc_vel_def_field_aff(ref, cpp, vel, NULL, NULL, steps, update, field);
+}
+
+/* The affine-only field of reg_affine_deformationField3D without composition
+ * (reg-lib/cpu/_reg_globalTrans.cpp:74-133): (A * sform) i in double, rounded to float. */
+static float *affine_only(const orc_geom *g, const float *A) {
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    float *out = (float *)malloc(3 * n * sizeof(float));
+    float T[16];
+    mat44_mm(A, g->xyz, T);
+    size_t i = 0;
+    for (int z = 0; z < g->nz; ++z)
+        for (int y = 0; y < g->ny; ++y)
+            for (int x = 0; x < g->nx; ++x, ++i) {
+                float p[3];
+                mul_d(T, x, y, z, p);
+                for (int r = 0; r < 3; ++r) out[i + r * n] = p[r];
+            }
+    return out;
+}
+
+/* After the squaring (reg-lib/cpu/_reg_localTrans.cpp:3569-3577, 3662-3683): with an extension 0
+ * the displacement (float) plus the affine-only field (double); then extension 1 composed
+ * (Mat44Mul<double>). */
+static void affine_parts(const orc_geom *ref, float *fields, int count, const float *affOnly, const float *ext1) {
+    const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
+    for (int k = 0; k < count; ++k) {
+        float *F = fields + (size_t)k * 3 * n;
+        size_t i = 0;
+        for (int z = 0; z < ref->nz; ++z)
+            for (int y = 0; y < ref->ny; ++y)
+                for (int x = 0; x < ref->nx; ++x, ++i) {
+                    float v[3] = { F[i], F[i + n], F[i + 2 * n] };
+                    if (affOnly) {
+                        float p[3];
+                        identity_pos(ref->xyz, x, y, z, p);
+                        for (int r = 0; r < 3; ++r) {
+                            const float d = v[r] - p[r];
+                            v[r] = (float)((double)d + (double)affOnly[i + r * n]);
+                        }
+                    }
+                    if (ext1) {
+                        const float w[3] = { v[0], v[1], v[2] };
+                        mul_d(ext1, w[0], w[1], w[2], v);
+                    }
+                    F[i] = v[0];
+                    F[i + n] = v[1];
+                    F[i + 2 * n] = v[2];
+                }
+    }
+}
+
+void orc_vel_def_field_aff(const orc_geom *ref, const orc_geom *cpp, const float *vel, const float *ext0,
+                           const float *ext1, int *steps, int update, float *field) {
     const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
     float *flow = (float *)malloc(3 * n * sizeof(float));
-    orc_vel_flow(ref, cpp, vel, flow);
+    float *affOnly = ext0 ? affine_only(ref, ext0) : NULL;
+    vel_flow(ref, cpp, vel, ext0, flow);
     int s = *steps;
-    scaled_flow(ref, flow, s, update, &s);
+    scaled_flow(ref, flow, s, update, &s, affOnly);
     *steps = s;
     const int sq = s < 0 ? -s : s;
     memcpy(field, flow, 3 * n * sizeof(float));
@@ -1784,6 +1894,8 @@ void orc_vel_def_field(const orc_geom *ref, const orc_geom *cpp, const float *ve
         orc_def_field_compose(ref, field, flow);
         memcpy(field, flow, 3 * n * sizeof(float));
     }
+    affine_parts(ref, field, 1, affOnly, ext1);
+    free(affOnly);
     free(flow);
 }
 
@@ -1791,15 +1903,23 @@ void orc_vel_def_field(const orc_geom *ref, const orc_geom *cpp, const float *ve
  * (reg-lib/cpu/_reg_localTrans.cpp:3616-3683): fields[0] is the scaled flow, fields[k+1] the
  * composition of fields[k] with itself; (|steps| + 1) planar [3][N] fields. */
 void orc_vel_intermediate(const orc_geom *ref, const orc_geom *cpp, const float *vel, int steps, float *fields) {
+    orc_vel_intermediate_aff(ref, cpp, vel, NULL, NULL, steps, fields);
+}
+
+void orc_vel_intermediate_aff(const orc_geom *ref, const orc_geom *cpp, const float *vel, const float *ext0,
+                              const float *ext1, int steps, float *fields) {
     const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
-    orc_vel_flow(ref, cpp, vel, fields);
+    float *affOnly = ext0 ? affine_only(ref, ext0) : NULL;
+    vel_flow(ref, cpp, vel, ext0, fields);
     int s;
-    scaled_flow(ref, fields, steps, 0, &s);
+    scaled_flow(ref, fields, steps, 0, &s, affOnly);
     const int sq = steps < 0 ? -steps : steps;
     for (int k = 0; k < sq; ++k) {
         memcpy(fields + (size_t)(k + 1) * 3 * n, fields + (size_t)k * 3 * n, 3 * n * sizeof(float));
         orc_def_field_compose(ref, fields + (size_t)k * 3 * n, fields + (size_t)(k + 1) * 3 * n);
     }
+    affine_parts(ref, fields, sq + 1, affOnly, ext1);
+    free(affOnly);
 }
 
 /* reg_getRealImageSpacing (reg-lib/cpu/_reg_tools.cpp:253-274): column lengths of the sform from
@@ -1936,11 +2056,37 @@ void orc_resample_gradient(const orc_geom *g, const float *grad, const float *de
  * grad <- resample(grad, fields[k]) + grad; finally grad / 2^|steps|. */
 void orc_exponentiate_gradient(const orc_geom *ref, const orc_geom *cppBw, const float *velBw, int steps,
                                float *grad) {
+    orc_exponentiate_gradient_aff(ref, cppBw, velBw, NULL, NULL, NULL, steps, grad);
+}
+
+/* With the backward content's transformation T, its displacement ((T * sform) i in double,
+ * minus x in float) is subtracted (double) from each intermediate field first
+ * (reg-lib/Compute.cpp:303-316). */
+void orc_exponentiate_gradient_aff(const orc_geom *ref, const orc_geom *cppBw, const float *velBw,
+                                   const float *ext0, const float *ext1, const float *transBw, int steps,
+                                   float *grad) {
     const size_t n = (size_t)ref->nx * ref->ny * ref->nz;
     const int sq = steps < 0 ? -steps : steps;
     float *fields = (float *)malloc((size_t)(sq + 1) * 3 * n * sizeof(float));
     float *tmp = (float *)malloc(3 * n * sizeof(float));
-    orc_vel_intermediate(ref, cppBw, velBw, steps, fields);
+    orc_vel_intermediate_aff(ref, cppBw, velBw, ext0, ext1, steps, fields);
+    if (transBw) {
+        float *disp = affine_only(ref, transBw);
+        size_t i = 0;
+        for (int z = 0; z < ref->nz; ++z)
+            for (int y = 0; y < ref->ny; ++y)
+                for (int x = 0; x < ref->nx; ++x, ++i) {
+                    float p[3];
+                    identity_pos(ref->xyz, x, y, z, p);
+                    for (int r = 0; r < 3; ++r) disp[i + r * n] -= p[r];
+                }
+        for (int k = 0; k < sq; ++k)
+            for (size_t j = 0; j < 3 * n; ++j) {
+                float *F = fields + (size_t)k * 3 * n;
+                F[j] = (float)((double)F[j] - (double)disp[j]);
+            }
+        free(disp);
+    }
     for (int k = 0; k < sq; ++k) {
         orc_resample_gradient(ref, grad, fields + (size_t)k * 3 * n, 0.f, tmp);
         for (size_t i = 0; i < 3 * n; ++i) grad[i] = (float)((double)tmp[i] + (double)grad[i]);
@@ -1987,15 +2133,56 @@ void orc_symmetrise_velocity(const orc_geom *cpp, float *fw, float *bw) {
  * grid geometry; fw and bw (may be NULL) receive the identity positions. */
 void orc_symmetric_grids(const orc_geom *ref, const orc_geom *flo, const float spacing[3], orc_geom *grid,
                          float *fw, float *bw) {
-    float mn[3], mx[3];
+    orc_symmetric_grids_aff(ref, flo, spacing, NULL, grid, fw, bw);
+}
+
+/* The half affines of reg_createSymmetricControlPointGrids (reg-lib/cpu/_reg_localTrans.cpp:
+ * 150-166): Mat44Logm (float -> double -> float), times 0.5, Mat44Expm; the backward half from
+ * the inverse.  Returns 1 when both differ from the identity (the grids then carry them as
+ * extensions 0 and 1, :320-353), 0 otherwise, -1 when a logarithm fails. */
+int orc_half_affines(const float A[16], float hf[16], float hb[16]) {
+    float inv[16];
+    orc_nifti_mat44_inverse(A, inv);
+    const float *src[2] = { A, inv };
+    float *dst[2] = { hf, hb };
+    for (int t = 0; t < 2; ++t) {
+        double m[16], l[16], h[16], e[16];
+        for (int i = 0; i < 16; ++i) m[i] = src[t][i];
+        if (!d4_logm(m, l)) return -1;
+        for (int i = 0; i < 16; ++i) h[i] = (double)(float)((double)(float)l[i] * 0.5);
+        d4_expm(h, e);
+        for (int i = 0; i < 16; ++i) dst[t][i] = (float)e[i];
+    }
+    int idf = 1, idb = 1;
+    for (int i = 0; i < 16; ++i) {
+        const float one = (i % 5) == 0 ? 1.f : 0.f;
+        if (hf[i] != one) idf = 0;
+        if (hb[i] != one) idb = 0;
+    }
+    return !idf && !idb;
+}
+
+/* With an affine the image spaces are first multiplied by the half affines (operator*(mat44,
+ * mat44), reg-lib/cpu/Maths.hpp:237-246). */
+void orc_symmetric_grids_aff(const orc_geom *ref, const orc_geom *flo, const float spacing[3], const float *affine,
+                             orc_geom *grid, float *fw, float *bw) {
+    float mn[3], mx[3], refSpace[16], floSpace[16];
+    memcpy(refSpace, ref->xyz, sizeof refSpace);
+    memcpy(floSpace, flo->xyz, sizeof floSpace);
+    if (affine) {
+        float hf[16], hb[16];
+        orc_half_affines(affine, hf, hb);
+        mat44_mm(hf, ref->xyz, refSpace);
+        mat44_mm(hb, flo->xyz, floSpace);
+    }
     for (int c = 0; c < 8; ++c) {
         const float rc[3] = { (c & 1) ? (float)ref->nx : 0.f, (c & 2) ? (float)ref->ny : 0.f,
                               (c & 4) ? (float)ref->nz : 0.f };
         const float fc[3] = { (c & 1) ? (float)flo->nx : 0.f, (c & 2) ? (float)flo->ny : 0.f,
                               (c & 4) ? (float)flo->nz : 0.f };
         float ro[3], fo[3];
-        mat44_mul_f(ref->xyz, rc, ro);
-        mat44_mul_f(flo->xyz, fc, fo);
+        mat44_mul_f(refSpace, rc, ro);
+        mat44_mul_f(floSpace, fc, fo);
         for (int r = 0; r < 3; ++r) {
             if (c == 0) {
                 mn[r] = ro[r] < fo[r] ? ro[r] : fo[r];

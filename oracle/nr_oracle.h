@@ -168,6 +168,21 @@ void orc_symmetrise_velocity(const orc_geom *cpp, float *fw, float *bw);
 /* reg_createSymmetricControlPointGrids, no affine (reg-lib/cpu/_reg_localTrans.cpp:132-358) */
 void orc_symmetric_grids(const orc_geom *ref, const orc_geom *flo, const float spacing[3], orc_geom *grid,
                          float *fw, float *bw);
+/* ---- velocity grids with affine parts: ext0 / ext1 = the grid's extensions 0 and 1, transBw =
+ * the backward content's transformation matrix (NULL when absent) */
+int orc_half_affines(const float A[16], float hf[16], float hb[16]);
+void orc_symmetric_grids_aff(const orc_geom *ref, const orc_geom *flo, const float spacing[3], const float *affine,
+                             orc_geom *grid, float *fw, float *bw);
+void orc_vel_def_field_aff(const orc_geom *ref, const orc_geom *cpp, const float *vel, const float *ext0,
+                           const float *ext1, int *steps, int update, float *field);
+void orc_vel_intermediate_aff(const orc_geom *ref, const orc_geom *cpp, const float *vel, const float *ext0,
+                              const float *ext1, int steps, float *fields);
+void orc_exponentiate_gradient_aff(const orc_geom *ref, const orc_geom *cppBw, const float *velBw,
+                                   const float *ext0, const float *ext1, const float *transBw, int steps,
+                                   float *grad);
+/* reg_voxelCentricToNodeCentric with a grid extension 0 (reg-lib/cpu/_reg_localTrans.cpp:1568-1592) */
+void orc_voxel_to_node_ext(const orc_geom *cpp, const orc_geom *ref, const float *voxelGrad,
+                           const float reorient_ijk[16], const float ext0[16], float weight, float *nodeGrad);
 
 #ifdef __cplusplus
 }

@@ -410,6 +410,97 @@ int ref_symmetric_grids(const nr_geom *ref, const nr_geom *flo, const float spac
     });
 }
 
+
+// ---------------------------------------------------------------- velocity grids with affine parts
+// A grid's NIfTI extensions 0 and 1 hold mat44 affines (reg-lib/cpu/_reg_localTrans.cpp:320-353).
+static void attach_affines(NiftiImage& grid, const float *ext0, const float *ext1) {
+    for (const float *e : { ext0, ext1 }) {
+        if (!e) break;
+        mat44 m = to_mat44(e);
+        nifti_add_extension(grid, reinterpret_cast<const char *>(&m), (int)sizeof(mat44), NIFTI_ECODE_IGNORE);
+    }
+}
+
+// reg_createSymmetricControlPointGrids<float> with an affine: geometry, positions and the forward
+// / backward grids' extension 0 (*count: the number of extensions, 0 or 2).
+int ref_symmetric_grids_aff(const nr_geom *ref, const nr_geom *flo, const float spacing[3], const float *affine,
+                            nr_geom *out, float *fw, float *bw, int capacity, float *extFw, float *extBw, int *count) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, nullptr), floImg = make_image(flo, nullptr);
+        NiftiImage gFw, gBw;
+        mat44 a = to_mat44(affine);
+        reg_createSymmetricControlPointGrids<float>(gFw, gBw, refImg, floImg, &a, spacing);
+        geom_of(gFw, out);
+        if ((int)gFw.nVoxels() <= capacity) {
+            if (fw) std::memcpy(fw, gFw->data, gFw.nVoxels() * sizeof(float));
+            if (bw) std::memcpy(bw, gBw->data, gBw.nVoxels() * sizeof(float));
+        }
+        *count = gFw->num_ext;
+        if (gFw->num_ext > 0) {
+            from_mat44(*reinterpret_cast<mat44 *>(gFw->ext_list[0].edata), extFw);
+            from_mat44(*reinterpret_cast<mat44 *>(gBw->ext_list[0].edata), extBw);
+        }
+    });
+}
+
+int ref_vel_def_field_aff(const nr_geom *ref, const nr_geom *cpp, const float *vel, const float *ext0,
+                          const float *ext1, int *steps, int update, float *field) {
+    return guarded([&] {
+        NiftiImage grid = make_velocity_grid(cpp, vel, *steps);
+        attach_affines(grid, ext0, ext1);
+        NiftiImage def = make_vector_image(ref, nullptr);
+        reg_spline_getDefFieldFromVelocityGrid(grid, def, update != 0);
+        *steps = (int)grid->intent_p2;
+        std::memcpy(field, def->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+int ref_vel_intermediate_aff(const nr_geom *ref, const nr_geom *cpp, const float *vel, const float *ext0,
+                             const float *ext1, int steps, float *fields) {
+    return guarded([&] {
+        NiftiImage grid = make_velocity_grid(cpp, vel, steps);
+        attach_affines(grid, ext0, ext1);
+        const int sq = steps < 0 ? -steps : steps;
+        std::vector<NiftiImage> defs(sq + 1);
+        for (auto& d : defs) d = make_vector_image(ref, nullptr);
+        reg_spline_getIntermediateDefFieldFromVelGrid(grid, defs.data());
+        for (int k = 0; k <= sq; ++k)
+            std::memcpy(fields + (size_t)k * 3 * nvox(ref), defs[k]->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// Compute::ExponentiateGradient with the backward grid's extensions and the backward content's
+// transformation matrix.
+int ref_exponentiate_gradient_aff(const nr_geom *ref, const nr_geom *flo, const nr_geom *cpp, const float *velFw,
+                                  const float *velBw, const float *ext0, const float *ext1, const float *transBw,
+                                  int steps, float *grad) {
+    return guarded([&] {
+        NiftiImage refImg = make_image(ref, nullptr), floImg = make_image(flo, nullptr);
+        NiftiImage gridFw = make_velocity_grid(cpp, velFw, steps), gridBw = make_velocity_grid(cpp, velBw, steps);
+        attach_affines(gridBw, ext0, ext1);
+        mat44 t = transBw ? to_mat44(transBw) : mat44{};
+        F3dContent con(refImg, floImg, gridFw), conBw(floImg, refImg, gridBw, nullptr, nullptr, transBw ? &t : nullptr);
+        NiftiImage& vg = con.GetVoxelBasedMeasureGradient();
+        std::memcpy(vg->data, grad, 3 * nvox(ref) * sizeof(float));
+        Compute(con).ExponentiateGradient(conBw);
+        std::memcpy(grad, vg->data, 3 * nvox(ref) * sizeof(float));
+    });
+}
+
+// reg_voxelCentricToNodeCentric (no convolution) on a grid with extension 0; reorientation from
+// the floating image's mm -> voxel matrix as Compute::VoxelCentricToNodeCentric passes it.
+int ref_voxel_to_node_ext(const nr_geom *cpp, const nr_geom *ref, const nr_geom *flo, const float *voxelGrad,
+                          const float *ext0, float weight, float *nodeGrad) {
+    return guarded([&] {
+        NiftiImage vg = make_vector_image(ref, voxelGrad);
+        NiftiImage floImg = make_image(flo, nullptr);
+        NiftiImage ng = make_grid(cpp, nullptr);
+        attach_affines(ng, ext0, nullptr);
+        reg_voxelCentricToNodeCentric(ng, vg, weight, false, Content::GetIJKMatrix(*floImg));
+        std::memcpy(nodeGrad, ng->data, 3 * nvox(cpp) * sizeof(float));
+    });
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------- full F3D through reg_f3d
@@ -459,6 +550,7 @@ typedef struct {
     float padding;       // warped padding value
     int measure;         // 0 NMI, 1 LNCC
     float lncc_sigma;    // LNCC kernel standard deviation (< 0: voxels)
+    const float *affine; // input affine (reference -> floating), NULL for none
 } ref_f3d_params;
 
 static void configure(reg_f3d<float>& r, const ref_f3d_params *p) {
@@ -478,6 +570,7 @@ static void configure(reg_f3d<float>& r, const ref_f3d_params *p) {
     r.SetMaximalIterationNumber(p->maxit);
     if (!p->use_pyramid) r.DoNotUsePyramidalApproach();
     r.SetWarpedPaddingValue(p->padding);
+    if (p->affine) r.SetAffineTransformation(to_mat44(p->affine));
     r.DoNotPrintOutInformation();
 }
 

@@ -49,7 +49,7 @@ class ref_f3d_params(ctypes.Structure):
     _fields_ = [("bins", ctypes.c_int), ("be", ctypes.c_float), ("le", ctypes.c_float), ("spacing", ctypes.c_float),
                 ("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("maxit", ctypes.c_int),
                 ("use_pyramid", ctypes.c_int), ("padding", ctypes.c_float), ("measure", ctypes.c_int),
-                ("lncc_sigma", ctypes.c_float)]
+                ("lncc_sigma", ctypes.c_float), ("affine", ctypes.POINTER(ctypes.c_float))]
 
 
 def og(g):
@@ -806,11 +806,13 @@ class RefF3d2:
     scalar=True uses the reference built without USE_SSE (the oracle's summation order)."""
 
     def __init__(self, refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=1,
-                 level=0, maxit=150, padding=float("nan"), scalar=True, measure=0, lncc_sigma=-5.0):
+                 level=0, maxit=150, padding=float("nan"), scalar=True, measure=0, lncc_sigma=-5.0, affine=None):
         p = ref_f3d_params()
         p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
         p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
         p.measure, p.lncc_sigma = measure, lncc_sigma
+        self._affine = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
+        p.affine = None if affine is None else fp(self._affine)
         self.p = p
         self.lib = r = ref_scalar() if scalar else ref()
         r.ref_f3d2_create.restype = ctypes.c_void_p
@@ -901,12 +903,14 @@ def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, s
 
 
 def ref_f3d2_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.01, spacing=-5.0, levels=3, maxit=150,
-                 padding=float("nan")):
+                 padding=float("nan"), affine=None):
     """reg_f3d2<float>'s run on the reference CPU build; returns (grid geom, fw, bw, steps)."""
     p = ref_f3d_params()
     p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
     p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
     p.measure, p.lncc_sigma = 0, -5.0
+    aff = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
+    p.affine = None if aff is None else fp(aff)
     out = nr_geom()
     cap = 3 * 200 ** 3 // 8
     fw = np.zeros(cap, np.float32)
@@ -928,3 +932,122 @@ def nmi_sym_at(refg, flog, cppg, fw, bw, ref_img, flo_img, bins=64):
     ff, _ = orc_vel_def_field(refg, cppg, np.ascontiguousarray(fw, np.float32), 6, True)
     fb, _ = orc_vel_def_field(flog, cppg, np.ascontiguousarray(bw, np.float32), 6, True)
     return orc_nmi(r, orc_resample(flog, f, refg, ff), rb, rb)[0] + orc_nmi(f, orc_resample(refg, r, flog, fb), rb, rb)[0]
+
+
+# ---------------------------------------------------------------- velocity grids with affine parts
+def _m16(m):
+    return None if m is None else np.ascontiguousarray(m, np.float32).reshape(16)
+
+
+def _fpn(a):
+    return None if a is None else fp(a)
+
+
+def orc_half_affines(A):
+    hf, hb = np.zeros(16, np.float32), np.zeros(16, np.float32)
+    ext = orc().orc_half_affines(fp(_m16(A)), fp(hf), fp(hb))
+    assert ext >= 0
+    return hf.reshape(4, 4), hb.reshape(4, 4), bool(ext)
+
+
+def orc_symmetric_grids_aff(refg, flog, spacing, A):
+    from niftyreg_amd.geometry import Geom
+    sp = np.asarray(spacing, np.float32)
+    a = _m16(A)
+    o = orc_geom()
+    orc().orc_symmetric_grids_aff(ctypes.byref(og(refg)), ctypes.byref(og(flog)), fp(sp), fp(a), ctypes.byref(o),
+                                  None, None)
+    G = o.nx * o.ny * o.nz
+    fw, bw = np.zeros(3 * G, np.float32), np.zeros(3 * G, np.float32)
+    orc().orc_symmetric_grids_aff(ctypes.byref(og(refg)), ctypes.byref(og(flog)), fp(sp), fp(a), ctypes.byref(o),
+                                  fp(fw), fp(bw))
+    g = Geom((o.nx, o.ny, o.nz), np.array(o.xyz[:], np.float32).reshape(4, 4),
+             np.array(o.ijk[:], np.float32).reshape(4, 4), (o.dx, o.dy, o.dz))
+    return g, fw, bw
+
+
+def ref_symmetric_grids_aff(refg, flog, spacing, A):
+    """(geometry, fw, bw, extension 0 of the forward grid, of the backward grid, extension count)"""
+    lib = ref_scalar()
+    sp = np.asarray(spacing, np.float32)
+    a = _m16(A)
+    o = nr_geom()
+    ef, eb, cnt = np.zeros(16, np.float32), np.zeros(16, np.float32), ctypes.c_int()
+    _scheck(lib, lib.ref_symmetric_grids_aff(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)), fp(sp), fp(a),
+                                             ctypes.byref(o), None, None, 0, fp(ef), fp(eb), ctypes.byref(cnt)))
+    G = o.nx * o.ny * o.nz
+    fw, bw = np.zeros(3 * G, np.float32), np.zeros(3 * G, np.float32)
+    _scheck(lib, lib.ref_symmetric_grids_aff(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)), fp(sp), fp(a),
+                                             ctypes.byref(o), fp(fw), fp(bw), 3 * G, fp(ef), fp(eb), ctypes.byref(cnt)))
+    return o, fw, bw, ef.reshape(4, 4), eb.reshape(4, 4), cnt.value
+
+
+def orc_vel_def_field_aff(refg, cppg, vel, ext0, ext1, steps, update=False):
+    field = np.zeros(3 * refg.nvox, np.float32)
+    s = ctypes.c_int(int(steps))
+    e0, e1 = _m16(ext0), _m16(ext1)
+    orc().orc_vel_def_field_aff(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vel), _fpn(e0), _fpn(e1),
+                                ctypes.byref(s), int(bool(update)), fp(field))
+    return field, s.value
+
+
+def ref_vel_def_field_aff(refg, cppg, vel, ext0, ext1, steps, update=False):
+    lib = ref_scalar()
+    field = np.zeros(3 * refg.nvox, np.float32)
+    s = ctypes.c_int(int(steps))
+    e0, e1 = _m16(ext0), _m16(ext1)
+    _scheck(lib, lib.ref_vel_def_field_aff(ctypes.byref(rg(refg)), ctypes.byref(rg(cppg)), fp(vel), _fpn(e0),
+                                           _fpn(e1), ctypes.byref(s), int(bool(update)), fp(field)))
+    return field, s.value
+
+
+def orc_vel_intermediate_aff(refg, cppg, vel, ext0, ext1, steps):
+    out = np.zeros((abs(steps) + 1) * 3 * refg.nvox, np.float32)
+    e0, e1 = _m16(ext0), _m16(ext1)
+    orc().orc_vel_intermediate_aff(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vel), _fpn(e0), _fpn(e1),
+                                   int(steps), fp(out))
+    return out.reshape(abs(steps) + 1, -1)
+
+
+def ref_vel_intermediate_aff(refg, cppg, vel, ext0, ext1, steps):
+    lib = ref_scalar()
+    out = np.zeros((abs(steps) + 1) * 3 * refg.nvox, np.float32)
+    e0, e1 = _m16(ext0), _m16(ext1)
+    _scheck(lib, lib.ref_vel_intermediate_aff(ctypes.byref(rg(refg)), ctypes.byref(rg(cppg)), fp(vel), _fpn(e0),
+                                              _fpn(e1), int(steps), fp(out)))
+    return out.reshape(abs(steps) + 1, -1)
+
+
+def orc_exponentiate_gradient_aff(refg, cppg, vel_bw, ext0, ext1, trans_bw, steps, grad):
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    e0, e1, t = _m16(ext0), _m16(ext1), _m16(trans_bw)
+    orc().orc_exponentiate_gradient_aff(ctypes.byref(og(refg)), ctypes.byref(og(cppg)), fp(vel_bw), _fpn(e0),
+                                        _fpn(e1), _fpn(t), int(steps), fp(g))
+    return g
+
+
+def ref_exponentiate_gradient_aff(refg, flog, cppg, vel_fw, vel_bw, ext0, ext1, trans_bw, steps, grad):
+    lib = ref_scalar()
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    e0, e1, t = _m16(ext0), _m16(ext1), _m16(trans_bw)
+    _scheck(lib, lib.ref_exponentiate_gradient_aff(ctypes.byref(rg(refg)), ctypes.byref(rg(flog)),
+                                                   ctypes.byref(rg(cppg)), fp(vel_fw), fp(vel_bw), _fpn(e0), _fpn(e1),
+                                                   _fpn(t), int(steps), fp(g)))
+    return g
+
+
+def orc_voxel_to_node_ext(cppg, refg, vg, flog, ext0, weight):
+    ng = np.zeros(3 * cppg.nvox, np.float32)
+    ijk = np.ascontiguousarray(flog.ijk, np.float32).reshape(16)
+    orc().orc_voxel_to_node_ext(ctypes.byref(og(cppg)), ctypes.byref(og(refg)), fp(vg), fp(ijk), fp(_m16(ext0)),
+                                ctypes.c_float(weight), fp(ng))
+    return ng
+
+
+def ref_voxel_to_node_ext(cppg, refg, vg, flog, ext0, weight):
+    lib = ref_scalar()
+    ng = np.zeros(3 * cppg.nvox, np.float32)
+    _scheck(lib, lib.ref_voxel_to_node_ext(ctypes.byref(rg(cppg)), ctypes.byref(rg(refg)), ctypes.byref(rg(flog)),
+                                           fp(np.ascontiguousarray(vg, np.float32)), fp(_m16(ext0)),
+                                           ctypes.c_float(weight), fp(ng)))
+    return ng

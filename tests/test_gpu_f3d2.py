@@ -34,7 +34,15 @@ def _oblique(dims):
     return Geom(dims, m)
 
 
-def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5, lncc=False):
+def _affine():
+    c, s = np.cos(0.06), np.sin(0.06)
+    m = np.eye(4)
+    m[:3, :3] = np.array([[c, -s, 0.02], [s, c, -0.03], [0.01, 0.04, 1.0]]) @ np.diag([1.02, 0.98, 1.01])
+    m[:3, 3] = [2.0, -1.5, 1.0]
+    return m.astype(np.float32)
+
+
+def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5, lncc=False, affine=None):
     if which == "identity":
         rg = fg = Geom.identity((40, 36, 32))
     else:
@@ -42,7 +50,8 @@ def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5, lncc=False):
     ref, flo = phantoms.config0_pair(3, rg.dims)
     if fg.dims != rg.dims:
         flo = phantoms.blob_phantom(fg.dims, 4)
-    r = O.RefF3d2(rg, ref, fg, flo, be=be, le=le, spacing=-5.0, levels=1, measure=1 if lncc else 0)
+    r = O.RefF3d2(rg, ref, fg, flo, be=be, le=le, spacing=-5.0, levels=1, measure=1 if lncc else 0,
+                  affine=affine)
     fw0, bw0, _ = r.get_cpp()
     rng = np.random.default_rng(seed)
     fw = (fw0 + rng.uniform(-amplitude, amplitude, fw0.size)).astype(np.float32)
@@ -55,6 +64,12 @@ def _session(which, be=0.001, le=0.01, seed=1, amplitude=1.5, lncc=False):
     pb = default_params(similarity_weight=simw, be_weight=bew, le_weight=lew, measure=m, lncc_sigma=-5.0)
     hf = F3dContent(rg, ref.ravel(), fg, flo.ravel(), cg, fw, params=pf)
     hb = F3dContent(fg, flo.ravel(), rg, ref.ravel(), cg, bw, params=pb)
+    if affine is not None:
+        # the grids' extensions hold the half affines; the contents hold A and its inverse
+        from niftyreg_amd.geometry import symmetric_half_affines
+        halfF, halfB, ext = symmetric_half_affines(affine)
+        hf.set_velocity_affines(halfF if ext else None, halfF if ext else None, affine)
+        hb.set_velocity_affines(halfB if ext else None, halfB if ext else None, O.orc_nifti_inverse(affine))
     rng_ = (0, 1) if lncc else (2, 65)
     hf.rescale_intensities(rng_, rng_)
     hb.rescale_intensities(rng_, rng_)
@@ -119,5 +134,22 @@ def test_symmetric_lncc_matches_reference(gpu, which):
     rgf, rgb, rmax = r.gradient(normalise=True)
     m = pair.gradient(normalise=True)
     _check_gradients(pair, rgf, rgb, rmax, m, exact=(which == "oblique"))
+    r.close()
+    pair.close()
+
+
+@pytest.mark.parametrize("which", ["identity", "oblique"])
+def test_symmetric_with_affine_matches_reference(gpu, which):
+    """reg_f3d2 with an input affine: half affines in both grids' extensions, A / A^-1 in the
+    contents.  The node gradient maps through the extension (the reference-order convolution
+    path): objective terms to 1e-12, gradients bit for bit."""
+    r, pair = _session(which, affine=_affine())
+    rv, rterms = r.value()
+    v, terms = pair.objective()
+    for k in range(3):
+        assert abs(terms[k] - rterms[k]) <= 1e-12 * abs(rterms[k]), (k, terms, rterms)
+    rgf, rgb, rmax = r.gradient(normalise=True)
+    m = pair.gradient(normalise=True)
+    _check_gradients(pair, rgf, rgb, rmax, m, exact=True)
     r.close()
     pair.close()

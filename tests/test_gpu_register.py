@@ -79,3 +79,44 @@ def test_symmetric_registration_matches_reference(gpu, levels, maxit):
     assert abs(nmi - score_gpu) <= 1e-4 * score_gpu
     assert score_gpu > score0 + 0.02
     assert score_gpu >= score_ref - 2 * FINAL_NMI_TOL
+
+
+def test_symmetric_registration_with_affine_matches_reference(gpu):
+    """reg_f3d2 with an input affine (-aff) on both platforms: the same symmetric grid geometry
+    (both images carried half-way by the half affines), and final grid pairs that score as well as
+    the reference's under the device objective (whose affine terms match the reference's)."""
+    from niftyreg_amd.f3d import F3d2Pair, F3dContent, default_params, register_symmetric
+    from niftyreg_amd.geometry import symmetric_half_affines
+    ref, flo = phantoms.config0_pair(0)
+    g = Geom.identity((64, 64, 64))
+    c, s = np.cos(0.05), np.sin(0.05)
+    A = np.eye(4)
+    A[:3, :3] = [[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]]
+    A[:3, 3] = [1.5, -1.0, 0.5]
+    A = A.astype(np.float32)
+    cg, fw, bw, nmi, its = register_symmetric(g, ref, g, flo, levels=2, maxit=30, affine=A)
+    rcg, rfw, rbw, rsteps = O.ref_f3d2_run(g, ref, g, flo, levels=2, maxit=30, affine=A)
+    assert cg.dims == rcg.dims and np.array_equal(cg.xyz, rcg.xyz)
+    hf, hb, ext = symmetric_half_affines(A)
+
+    def score(f, b):
+        p = default_params(similarity_weight=1.0, be_weight=0.0, le_weight=0.0)
+        cf = F3dContent(g, ref.ravel(), g, flo.ravel(), cg, f, params=p)
+        cb = F3dContent(g, flo.ravel(), g, ref.ravel(), cg, b, params=p)
+        cf.set_velocity_affines(hf, hf, A)
+        cb.set_velocity_affines(hb, hb, O.orc_nifti_inverse(A))
+        for h in (cf, cb):
+            h.rescale_intensities((2, 65), (2, 65))
+        pair = F3d2Pair(cf, cb, steps=6)
+        v, _ = pair.objective()
+        pair.close()
+        return v
+
+    # the identity velocity on the final grid (no displacement: only the half affines act)
+    k, j, i = np.meshgrid(*[np.arange(n) for n in cg.dims[::-1]], indexing="ij")
+    lattice = np.stack([i.ravel(), j.ravel(), k.ravel(), np.ones(i.size)])
+    ident = (cg.xyz.astype(np.float64) @ lattice)[:3].astype(np.float32).ravel()
+    s_dev, s_ref, s0 = score(fw, bw), score(rfw, rbw), score(ident, ident)
+    print(f"symmetric + affine: start {s0:.6f}, device {s_dev:.6f} ({its} it), reference {s_ref:.6f}")
+    assert s_dev > s0 + 0.02
+    assert s_dev >= s_ref - 2 * FINAL_NMI_TOL

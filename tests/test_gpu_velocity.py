@@ -132,3 +132,43 @@ def test_velocity_handles_refuse_the_displacement_path(gpu):
         fw.objective()
     fw.close()
     bw.close()
+
+
+def _affine(angle=0.08, shift=(2.5, -1.5, 1.0), scale=(1.03, 0.97, 1.01)):
+    c, s = np.cos(angle), np.sin(angle)
+    R = np.array([[c, -s, 0.02], [s, c, -0.03], [0.01, 0.04, 1.0]])
+    m = np.eye(4)
+    m[:3, :3] = R @ np.diag(scale)
+    m[:3, 3] = shift
+    return m.astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["identity", "oblique", "large"])
+def test_velocity_grids_with_affine_parts_match_oracle(gpu, which):
+    """Grid extensions (the half affines) on both handles, the transformation matrices A / A^-1:
+    exponentiation (with and without the step-number update) and the gradient exponentiation."""
+    from niftyreg_amd.geometry import symmetric_half_affines
+    ref_g, flo_g, cg, fw, bw, vfw, vbw, rng = _pair(which, seed=6)
+    A = _affine()
+    hf, hb, ext = symmetric_half_affines(A)
+    assert ext
+    Ainv = O.orc_nifti_inverse(A)
+    fw.set_velocity_affines(hf, hf, A)
+    bw.set_velocity_affines(hb, hb, Ainv)
+    for update, steps in ((True, 6), (False, 7)):
+        fw.velocity_steps = steps
+        fw.GetDefFieldFromVelocityGrid(update)
+        want, s = O.orc_vel_def_field_aff(ref_g, cg, vfw, hf, hf, steps, update)
+        assert fw.velocity_steps == s
+        got = fw.deformation_field()
+        assert np.array_equal(got, want), (update, steps, np.count_nonzero(got != want))
+    grad = rng.normal(size=3 * ref_g.nvox).astype(np.float32)
+    fw.set_voxel_gradient(grad)
+    bw.velocity_steps = 6
+    bw.GetDefFieldFromVelocityGrid(False)  # a chain on bw that the affine case must not reuse
+    fw.ExponentiateGradient(bw)
+    want = O.orc_exponentiate_gradient_aff(ref_g, cg, vbw, hb, hb, Ainv, 6, grad)
+    got = fw.voxel_gradient()
+    assert np.array_equal(got, want), np.count_nonzero(got != want)
+    fw.close()
+    bw.close()
