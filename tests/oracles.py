@@ -388,12 +388,14 @@ def fp64_field(refg, cppg, cpp):
 
 
 def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3, maxit=150,
-                padding=float("nan"), measure=0, lncc_sigma=-5.0):
+                padding=float("nan"), measure=0, lncc_sigma=-5.0, affine=None):
     """reg_f3d<float>::Run on the reference CPU build; returns (grid geom, control points)."""
     p = ref_f3d_params()
     p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
     p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
     p.measure, p.lncc_sigma = measure, lncc_sigma
+    aff = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
+    p.affine = None if aff is None else fp(aff)
     out = nr_geom()
     sp = [abs(spacing) * d if spacing < 0 else spacing for d in refg.pixdim]
     cap = 3 * int(np.prod([int(np.ceil(n * d / s + 4)) for n, d, s in zip(refg.dims, refg.pixdim, sp)]))
