@@ -82,14 +82,27 @@ int plugin_f3d_eval(int hip, const nr_geom *ref, const float *refData, const nr_
 // the grids' intent_p2.  measure = NMI_fw + NMI_bw; be = BE_fw + BE_bw (+ 1000 (LE_fw + LE_bw)).
 int plugin_f3d2_eval(int hip, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
                      const nr_geom *cpp, float *cppFw, float *cppBw, int bins, int lncc, float simW, float beW,
-                     float leW, double *measure, double *be, float *gradFw, float *gradBw, int *steps) {
+                     float leW, double *measure, double *be, float *gradFw, float *gradBw, int *steps,
+                     const float *affine, const float *halfFw, const float *halfBw) {
     try {
         NiftiImage r = make_image(ref, refData), f = make_image(flo, floData);
         NiftiImage g = make_grid(cpp, cppFw), gb = make_grid(cpp, cppBw);
         g->intent_p1 = gb->intent_p1 = SPLINE_VEL_GRID;
         g->intent_p2 = gb->intent_p2 = 6;
+        // with an input affine: the half affines in both extensions of each grid, the affine and
+        // its inverse in the contents (reg_createSymmetricControlPointGrids, reg_f3d2::InitContent)
+        mat44 a, ainv;
+        if (affine) {
+            a = to_mat44(affine);
+            ainv = nifti_mat44_inverse(a);
+            const mat44 hf = to_mat44(halfFw), hb = to_mat44(halfBw);
+            for (int k = 0; k < 2; ++k) {
+                nifti_add_extension(g, reinterpret_cast<const char *>(&hf), (int)sizeof(mat44), NIFTI_ECODE_IGNORE);
+                nifti_add_extension(gb, reinterpret_cast<const char *>(&hb), (int)sizeof(mat44), NIFTI_ECODE_IGNORE);
+            }
+        }
         std::unique_ptr<F3d2ContentCreator> cc(hip ? new HipF3d2ContentCreator() : new F3d2ContentCreator());
-        auto pair = cc->Create(r, f, g, gb);
+        auto pair = cc->Create(r, f, g, gb, nullptr, nullptr, nullptr, affine ? &a : nullptr, affine ? &ainv : nullptr);
         std::unique_ptr<F3dContent> con(pair.first), conBw(pair.second);
         std::unique_ptr<ComputeFactory> cf(hip ? new HipComputeFactory() : new ComputeFactory());
         std::unique_ptr<Compute> comp(cf->Produce(*con)), compBw(cf->Produce(*conBw));

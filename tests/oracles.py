@@ -880,7 +880,7 @@ class RefF3d2:
 
 
 def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, simw=0.989, bew=0.001, lew=0.01,
-                     lncc=False):
+                     lncc=False, affine=None, halves=None):
     """One reg_f3d2 objective value + gradient through the reference's own Content / Compute /
     Measure calls (forward and backward contents), on the reference CPU classes (hip=False) or
     the HIP platform plugin (hip=True).  Returns (measure, be, grad_fw, grad_bw, cpp_fw, cpp_bw, steps)."""
@@ -898,7 +898,8 @@ def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, s
                                   ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).ravel()),
                                   ctypes.byref(rg(cppg)), fp(f), fp(b), bins, int(lncc), ctypes.c_float(simw),
                                   ctypes.c_float(bew),
-                                  ctypes.c_float(lew), ctypes.byref(m), ctypes.byref(be), fp(gf), fp(gb), ip(steps))
+                                  ctypes.c_float(lew), ctypes.byref(m), ctypes.byref(be), fp(gf), fp(gb), ip(steps),
+                                  _fpn(_m16(affine)), *[_fpn(_m16(h)) for h in (halves or (None, None))])
     if rc != 0:
         raise RuntimeError(_plugin.plugin_last_error().decode())
     return m.value, be.value, gf, gb, f, b, (int(steps[0]), int(steps[1]))

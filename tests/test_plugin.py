@@ -96,6 +96,27 @@ def test_hip_platform_symmetric_matches_reference_platform(gpu, which, lncc):
         assert np.array_equal(h[k], c[k]), (k, np.count_nonzero(h[k] != c[k]))
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["cube", "oblique"])
+def test_hip_platform_symmetric_with_affine_matches_reference_platform(gpu, which):
+    """The same calls with an input affine: the half affines in the grids' extensions (read by
+    HipF3dContent), the affine and its inverse in the contents."""
+    rg_, fg, ref, flo, cg, fw, bw = _f3d2_case(which)
+    c_, s_ = np.cos(0.05), np.sin(0.05)
+    A = np.eye(4)
+    A[:3, :3] = [[c_, -s_, 0.01], [s_, c_, 0.0], [0.0, 0.02, 1.0]]
+    A[:3, 3] = [1.2, -0.8, 0.5]
+    A = A.astype(np.float32)
+    hf, hb, _ = O.orc_half_affines(A)
+    c = O.plugin_f3d2_eval(False, rg_, ref, fg, flo, cg, fw, bw, affine=A, halves=(hf, hb))
+    h = O.plugin_f3d2_eval(True, rg_, ref, fg, flo, cg, fw, bw, affine=A, halves=(hf, hb))
+    assert abs(h[0] - c[0]) <= 1e-12 * abs(c[0])
+    assert abs(h[1] - c[1]) <= 1e-12 * abs(c[1])
+    assert h[6] == c[6]
+    for k in (2, 3, 4, 5):
+        assert np.array_equal(h[k], c[k]), (k, np.count_nonzero(h[k] != c[k]))
+
+
 @needs_plugin
 @pytest.mark.parametrize("which", ["cube", "oblique"])
 def test_symmetric_check_sequence_matches_reg_f3d2(which):
