@@ -224,9 +224,11 @@ __global__ void __launch_bounds__(kBlock) k_vel_flow_scaled(VelGrid g, FieldGeom
     float3 d, p;
     flow_at(g, f, a, i, active, d, p);
     if (!active) return;
-    out[i] = (float)((double)d.x / scaling) + p.x;
-    out[i + f.n] = (float)((double)d.y / scaling) + p.y;
-    out[i + 2 * f.n] = (float)((double)d.z / scaling) + p.z;
+    // scaling is +-2^s: the double quotient equals the product with its exact reciprocal
+    const double inv = 1.0 / scaling;
+    out[i] = (float)((double)d.x * inv) + p.x;
+    out[i + f.n] = (float)((double)d.y * inv) + p.y;
+    out[i + 2 * f.n] = (float)((double)d.z * inv) + p.z;
 }
 
 // The flow's displacement flow - x, with its extremes folded into keys[0] (min) and keys[1]
@@ -278,9 +280,10 @@ __global__ void __launch_bounds__(kBlock) k_vel_scale(const float *disp, FieldGe
     if (i >= f.n) return;
     const int x = (int)(i % f.nx), y = (int)((i / f.nx) % f.ny), z = (int)(i / ((size_t)f.nx * f.ny));
     const float3 p = id_pos(f.xyz, x, y, z);
-    out[i] = (float)((double)disp[i] / scaling) + p.x;
-    out[i + f.n] = (float)((double)disp[i + f.n] / scaling) + p.y;
-    out[i + 2 * f.n] = (float)((double)disp[i + 2 * f.n] / scaling) + p.z;
+    const double inv = 1.0 / scaling;  // exact: scaling is +-2^s
+    out[i] = (float)((double)disp[i] * inv) + p.x;
+    out[i + f.n] = (float)((double)disp[i + f.n] * inv) + p.y;
+    out[i + 2 * f.n] = (float)((double)disp[i + 2 * f.n] * inv) + p.z;
 }
 
 // The affine parts put back on `count` fields after the squaring: with an extension 0 the
@@ -482,55 +485,56 @@ __global__ void __launch_bounds__(kBlock) k_resample_gradient(ResampleGradArgs r
         }
     }
     }
-    float J[3][3] = {};
-    const float basis[2] = { 1.f, 0.f }, deriv[2] = { -1.f, 1.f };
+    // Forward-difference Jacobian over the eight (c, b, a) pairs of the reference's loop.  Only
+    // four pairs carry a non-zero weight: the base pair (a*, b*, c*), where a* = 1 on the last
+    // index of the x axis (the difference steps back) and 0 elsewhere, and the base with one of a,
+    // b, c flipped.  The other pairs add zeros, which leave the sums unchanged, so the four pairs
+    // are visited in the reference's order (ascending c*4 + b*2 + a) with their 12 loads issued
+    // together.
+    const int ax = x == r.nx - 1, ay = y == r.ny - 1, az = z == r.nz - 1;
+    const int base = az * 4 + ay * 2 + ax;
+    int pr[4] = { base, base ^ 1, base ^ 2, base ^ 4 };
+    // sorting network on four keys
+    auto cswap = [](int& u, int& v) {
+        const int lo_ = u < v ? u : v, hi_ = u < v ? v : u;
+        u = lo_;
+        v = hi_;
+    };
+    cswap(pr[0], pr[1]);
+    cswap(pr[2], pr[3]);
+    cswap(pr[0], pr[2]);
+    cswap(pr[1], pr[3]);
+    cswap(pr[1], pr[2]);
+    float jw[4][3], jd[4][3];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        int Z = z + c;
-        float bz0 = basis[c], bz1 = deriv[c];
-        if (z == r.nz - 1) {
-            if (c == 1) Z -= 2;
-            bz0 = fabsf(bz0 - 1.f);
-            bz1 = -bz1;
-        }
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            int Y = y + b;
-            float by0 = basis[b], by1 = deriv[b];
-            if (y == r.ny - 1) {
-                if (b == 1) Y -= 2;
-                by0 = fabsf(by0 - 1.f);
-                by1 = -by1;
-            }
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                int X = x + a;
-                float bx0 = basis[a], bx1 = deriv[a];
-                if (x == r.nx - 1) {
-                    if (a == 1) X -= 2;
-                    bx0 = fabsf(bx0 - 1.f);
-                    bx1 = -bx1;
-                }
-                const float w0 = bx1 * by0 * bz0, w1 = bx0 * by1 * bz0, w2 = bx0 * by0 * bz1;
-                // half of the eight pairs carry three zero weights; their terms add zeros, which
-                // leave the sums unchanged, so their loads are skipped
-                if (w0 == 0.f && w1 == 0.f && w2 == 0.f) continue;
-                // fields with one voxel along an axis would step to index -1: clamp the read
-                // (its weight is then zero on that axis in the reference as well)
-                const size_t idx = ((size_t)clampi(Z, r.nz) * r.ny + clampi(Y, r.ny)) * r.nx + clampi(X, r.nx);
-                const float dx = r.def[idx], dy = r.def[idx + r.n], dz = r.def[idx + 2 * r.n];
-                J[0][0] += w0 * dx;
-                J[0][1] += w1 * dx;
-                J[0][2] += w2 * dx;
-                J[1][0] += w0 * dy;
-                J[1][1] += w1 * dy;
-                J[1][2] += w2 * dy;
-                J[2][0] += w0 * dz;
-                J[2][1] += w1 * dz;
-                J[2][2] += w2 * dz;
-            }
-        }
+    for (int q = 0; q < 4; ++q) {
+        const int a = pr[q] & 1, b = (pr[q] >> 1) & 1, c = pr[q] >> 2;
+        // reference weights: basis {1, 0}, derivative {-1, 1}; on the last index the step goes back
+        // (index - 1 for the second tap), the basis becomes |basis - 1| and the derivative flips
+        const float bx0 = ax ? (float)a : (float)(1 - a), by0 = ay ? (float)b : (float)(1 - b),
+                    bz0 = az ? (float)c : (float)(1 - c);
+        const float bx1 = ax ? (a ? -1.f : 1.f) : (a ? 1.f : -1.f), by1 = ay ? (b ? -1.f : 1.f) : (b ? 1.f : -1.f),
+                    bz1 = az ? (c ? -1.f : 1.f) : (c ? 1.f : -1.f);
+        jw[q][0] = bx1 * by0 * bz0;
+        jw[q][1] = bx0 * by1 * bz0;
+        jw[q][2] = bx0 * by0 * bz1;
+        const int X = x + a - (ax && a ? 2 : 0), Y = y + b - (ay && b ? 2 : 0), Z = z + c - (az && c ? 2 : 0);
+        // fields with one voxel along an axis would step to index -1: clamp the read (its weight
+        // is then zero on that axis in the reference as well)
+        const size_t idx = ((size_t)clampi(Z, r.nz) * r.ny + clampi(Y, r.ny)) * r.nx + clampi(X, r.nx);
+        jd[q][0] = r.def[idx];
+        jd[q][1] = r.def[idx + r.n];
+        jd[q][2] = r.def[idx + 2 * r.n];
     }
+    float J[3][3] = {};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            J[m][0] += jw[q][0] * jd[q][m];
+            J[m][1] += jw[q][1] * jd[q][m];
+            J[m][2] += jw[q][2] * jd[q][m];
+        }
     float R[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
