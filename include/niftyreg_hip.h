@@ -93,10 +93,21 @@ typedef struct nrh_f3d_params {
     int image_storage;        /* 0: float32 images in HBM, 1: float16 storage, float32 compute */
     int measure;              /* NRH_MEASURE_NMI or NRH_MEASURE_LNCC (reg-lib/MeasureCreator.hpp:8) */
     float lncc_sigma;         /* LNCC Gaussian standard deviation, < 0: voxels (reg_lncc default -5) */
+    int field_order;          /* summation order of the float deformation field wherever it decides a
+                                 floor (NRH_FIELD_ORDER_*), i.e. which reference build's rounding the
+                                 device reproduces (reg-lib/cpu/_reg_localTrans.cpp:1110-1451) */
 } nrh_f3d_params;
 
 #define NRH_MEASURE_NMI 0
 #define NRH_MEASURE_LNCC 1
+
+/* The reference sums the 64 weighted control points of a voxel in float, in an order that
+ * depends on its build: sequentially (USE_SSE off), or in four x lanes whose partial sums are
+ * added pairwise (the 5-voxel look-up-table path with SSE3) or left to right (the general path).
+ * Positions that land within rounding of a voxel plane of the floating image decide the
+ * trilinear cell, and at the image border whether the voxel is inside, by that last bit. */
+#define NRH_FIELD_ORDER_SCALAR 0
+#define NRH_FIELD_ORDER_SSE 1
 
 /* F3dContent + DefContent + Content construction (reg-lib/F3dContent.cpp:4-19,
  * reg-lib/DefContent.cpp:5-15, reg-lib/Content.cpp:5-22) and the device upload of
@@ -266,6 +277,7 @@ typedef struct nrh_f3d_config {
     const float *affine;   /* -aff: reference -> floating 4x4, row-major (NULL: none).  reg_f3d
                               starts its grid at the affine's positions (reg-lib/_reg_f3d.cpp:
                               190-191); reg_f3d2 splits it into half affines on both grids */
+    int field_order;       /* NRH_FIELD_ORDER_*: the reference build whose field rounding is reproduced */
 } nrh_f3d_config;
 int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                      const nrh_f3d_config *cfg, nrh_geom *grid_out, float *cpp_out, size_t cpp_capacity,
@@ -279,6 +291,53 @@ int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom 
 int nrh_f3d2_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                       const nrh_f3d_config *cfg, nrh_geom *grid_out, float *fw_out, float *bw_out, size_t capacity,
                       double *final_nmi, int *iterations);
+
+/* ------------------------------------------------------------------ batches of F3D pairs */
+/* No counterpart in the reference, which runs one registration per process on a process-wide
+ * device context (reg-lib/_reg_base.cpp:727-819, reg-lib/cuda/CudaContext.hpp:9-40).  A batch
+ * evaluates many reg_f3d handles of different image and grid sizes with the same launches:
+ * persistent objective / gradient sweeps over a (pair, tile) work list and one launch per small
+ * kernel for all pairs, on the batch's own stream.  Every pair's arithmetic and reduction order
+ * are those of the pair alone, so each pair's objective, node gradient and max length are
+ * bit-identical to nrh_f3d_objective / nrh_f3d_gradient on that handle.  Pairs must take the
+ * fused path (integer grid spacing >= 4 voxels, nodes on voxels), use NMI with a bending-energy
+ * term, no linear energy and no mask, and share one image storage type. */
+typedef struct nrh_batch nrh_batch;
+/* The handles stay owned by the caller and must outlive the batch; while it exists their work
+ * runs on the batch's stream, and calls on them must not overlap calls on the batch. */
+int nrh_batch_create(nrh_f3d *const *pairs, int n, nrh_batch **out);
+int nrh_batch_destroy(nrh_batch *b);
+/* reg_f3d::GetObjectiveFunctionValue of every pair (reg-lib/_reg_f3d.cpp:443-463): values[n] and
+ * the NMI nmi[n] (either may be NULL). */
+int nrh_batch_objective(nrh_batch *b, double *values, double *nmi);
+/* reg_f3d::GetObjectiveFunctionGradient of every pair (reg-lib/_reg_f3d.cpp:553-577), normalised
+ * by each pair's max length (NormaliseGradient, reg-lib/_reg_f3d.cpp:395-411) when max_lengths is
+ * given, which then receives them. */
+int nrh_batch_gradient(nrh_batch *b, double *max_lengths);
+/* The persistent sweeps' grid: workgroups per XCD (8 XCDs). */
+int nrh_batch_set_sweep_blocks(nrh_batch *b, int blocks_per_xcd);
+/* Times the histogram sweep for 32, 64, 128 and 256 workgroups per XCD (ms[4], per sweep) and
+ * keeps the fastest, returned in *blocks_per_xcd. */
+int nrh_batch_autotune(nrh_batch *b, int *blocks_per_xcd, double ms[4]);
+
+typedef struct nrh_batch_stats {
+    double seconds;          /* wall time of the whole batched registration */
+    double device_seconds;   /* of which: replays of the captured iteration graphs */
+    int graph_replays;       /* iteration graph launches over all levels */
+    int blocks_per_xcd[8];   /* autotuned persistent grid of each level (workgroups per XCD) */
+    int tile[3];             /* voxel tile of the sweeps (x, y, z) */
+} nrh_batch_stats;
+
+/* reg_f3d's whole registration (as nrh_f3d_register, same configuration for every pair) for n
+ * pairs at once: each pair takes exactly the decisions it takes alone, so its final grid, NMI and
+ * iteration count equal nrh_f3d_register's on the same pair; all pairs' evaluations share the
+ * batch launches, and each level's iterations replay one captured HIP graph.  grids_out[n];
+ * cpp_out[q] (capacity[q] floats, may be NULL), final_nmi[n] and iterations[n] may be NULL;
+ * stats may be NULL.  NMI with a bending-energy term, no linear energy. */
+int nrh_f3d_register_batch(int n, const nrh_geom *refs, const float *const *ref_data, const nrh_geom *flos,
+                           const float *const *flo_data, const nrh_f3d_config *cfg, nrh_geom *grids_out,
+                           float *const *cpp_out, const size_t *capacity, double *final_nmi, int *iterations,
+                           nrh_batch_stats *stats);
 
 /* ------------------------------------------------------------------ Aladin (affine) */
 /* One pyramid level of reg_aladin on the device: the HIP counterpart of AladinContent

@@ -35,7 +35,7 @@ class nrh_f3d_params(ctypes.Structure):
     _fields_ = [("ref_bins", ctypes.c_int), ("flo_bins", ctypes.c_int), ("padding", ctypes.c_float),
                 ("similarity_weight", ctypes.c_float), ("be_weight", ctypes.c_float),
                 ("le_weight", ctypes.c_float), ("image_storage", ctypes.c_int), ("measure", ctypes.c_int),
-                ("lncc_sigma", ctypes.c_float)]
+                ("lncc_sigma", ctypes.c_float), ("field_order", ctypes.c_int)]
 
 
 class nrh_f3d_config(ctypes.Structure):
@@ -43,10 +43,17 @@ class nrh_f3d_config(ctypes.Structure):
                 ("spacing", ctypes.c_float), ("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int),
                 ("max_iterations", ctypes.c_int), ("padding", ctypes.c_float), ("use_pyramid", ctypes.c_int),
                 ("image_storage", ctypes.c_int), ("measure", ctypes.c_int), ("lncc_sigma", ctypes.c_float),
-                ("affine", ctypes.POINTER(ctypes.c_float))]
+                ("affine", ctypes.POINTER(ctypes.c_float)), ("field_order", ctypes.c_int)]
 
 MEASURE_NMI = 0
 MEASURE_LNCC = 1
+FIELD_ORDER_SCALAR = 0  # the reference built without USE_SSE
+FIELD_ORDER_SSE = 1     # the reference's default x86 build (USE_SSE, SSE3)
+
+
+class nrh_batch_stats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("device_seconds", ctypes.c_double), ("graph_replays", ctypes.c_int),
+                ("blocks_per_xcd", ctypes.c_int * 8), ("tile", ctypes.c_int * 3)]
 
 
 class nrh_aladin_params(ctypes.Structure):

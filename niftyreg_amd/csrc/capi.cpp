@@ -116,6 +116,8 @@ void check_params(const nrh_f3d_params *p) {
     NRH_REQUIRE(p->image_storage == 0 || p->image_storage == 1, NRH_ERR_ARG, "image_storage must be 0 or 1");
     NRH_REQUIRE(p->measure == NRH_MEASURE_NMI || p->measure == NRH_MEASURE_LNCC, NRH_ERR_ARG, "unknown measure");
     NRH_REQUIRE(p->measure != NRH_MEASURE_LNCC || p->lncc_sigma != 0.f, NRH_ERR_ARG, "LNCC needs a non-zero sigma");
+    NRH_REQUIRE(p->field_order == NRH_FIELD_ORDER_SCALAR || p->field_order == NRH_FIELD_ORDER_SSE, NRH_ERR_ARG,
+                "field_order must be NRH_FIELD_ORDER_SCALAR or NRH_FIELD_ORDER_SSE");
 }
 
 }  // namespace nrh

@@ -98,6 +98,101 @@ __device__ __forceinline__ float sample_linear(const float *__restrict__ img, in
     return intensity;
 }
 
+// One voxel of reg_cubic_spline_getDeformationField3D (non-composition, reg-lib/cpu/
+// _reg_localTrans.cpp:1110-1451): the 64 control points of the cell starting at node (px, py, pz)
+// (z outer, y, x inner as get_GridValues reads them, reg-lib/cpu/_reg_splineBasis.cpp:603-665),
+// weighted by x_a * (y_b * z_c) in float and summed in the order of the selected build
+// (NRH_FIELD_ORDER_*): sequentially (USE_SSE off), or in four x lanes whose partial sums are then
+// added pairwise (the 5-voxel table path, SSE3 horizontal adds) or left to right (general path).
+__device__ __forceinline__ float3 ref_spline_point(const float *__restrict__ cpp, size_t G, int gnx, int gny, int px,
+                                                   int py, int pz, const float xb[4], const float yb[4],
+                                                   const float zb[4], int order, bool lut) {
+#pragma clang fp contract(off)
+    float3 r;
+    if (order == NRH_FIELD_ORDER_SCALAR) {
+        float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+        for (int c = 0; c < 4; ++c)
+            for (int b = 0; b < 4; ++b) {
+                const float yzw = yb[b] * zb[c];
+                const size_t row = ((size_t)(pz + c) * gny + (py + b)) * gnx + px;
+                for (int a = 0; a < 4; ++a) {
+                    const float w = xb[a] * yzw;
+                    r0 += cpp[row + a] * w;
+                    r1 += cpp[row + a + G] * w;
+                    r2 += cpp[row + a + 2 * G] * w;
+                }
+            }
+        r.x = r0; r.y = r1; r.z = r2;
+    } else {
+        float l0[4] = { 0.f, 0.f, 0.f, 0.f }, l1[4] = { 0.f, 0.f, 0.f, 0.f }, l2[4] = { 0.f, 0.f, 0.f, 0.f };
+        for (int c = 0; c < 4; ++c)
+            for (int b = 0; b < 4; ++b) {
+                const float yzw = yb[b] * zb[c];
+                const size_t row = ((size_t)(pz + c) * gny + (py + b)) * gnx + px;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const float w = xb[a] * yzw;
+                    l0[a] = w * cpp[row + a] + l0[a];
+                    l1[a] = w * cpp[row + a + G] + l1[a];
+                    l2[a] = w * cpp[row + a + 2 * G] + l2[a];
+                }
+            }
+        if (lut) {
+            r.x = (l0[0] + l0[1]) + (l0[2] + l0[3]);
+            r.y = (l1[0] + l1[1]) + (l1[2] + l1[3]);
+            r.z = (l2[0] + l2[1]) + (l2[2] + l2[3]);
+        } else {
+            r.x = ((l0[0] + l0[1]) + l0[2]) + l0[3];
+            r.y = ((l1[0] + l1[1]) + l1[2]) + l1[3];
+            r.z = ((l2[0] + l2[1]) + l2[2]) + l2[3];
+        }
+    }
+    return r;
+}
+
+// TrilinearImageGradient at floating voxel position p (reg-lib/cpu/_reg_resampling.cpp:1797-1915):
+// derivative weights (-1, 1) in floating voxel units, padding for taps outside the image, NaN
+// components replaced by 0.
+__device__ __forceinline__ float3 image_gradient_at(const float *__restrict__ flo, int fnx, int fny, int fnz, float3 p,
+                                                    float pad) {
+#pragma clang fp contract(off)
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
+    float r = p.x - (float)px;
+    const float xb[2] = { (float)(1.0 - (double)r), r };
+    r = p.y - (float)py;
+    const float yb[2] = { (float)(1.0 - (double)r), r };
+    r = p.z - (float)pz;
+    const float zb[2] = { (float)(1.0 - (double)r), r };
+    const float deriv[2] = { -1.f, 1.f };
+    for (int c = 0; c < 2; ++c) {
+        const int Z = pz + c;
+        const bool zIn = Z > -1 && Z < fnz;
+        float xx = 0.f, yy = 0.f, zz = 0.f;
+        for (int b = 0; b < 2; ++b) {
+            const int Y = py + b;
+            const bool yzIn = zIn && Y > -1 && Y < fny;
+            float xT = 0.f, yT = 0.f;
+            for (int a = 0; a < 2; ++a) {
+                const int X = px + a;
+                const float coeff = (yzIn && X > -1 && X < fnx) ? flo[((size_t)Z * fny + Y) * fnx + X] : pad;
+                xT += coeff * deriv[a];
+                yT += coeff * xb[a];
+            }
+            xx += xT * yb[b];
+            yy += yT * deriv[b];
+            zz += yT * yb[b];
+        }
+        g0 += xx * zb[c];
+        g1 += yy * zb[c];
+        g2 += zz * deriv[c];
+    }
+    if (g0 != g0) g0 = 0.f;
+    if (g1 != g1) g1 = 0.f;
+    if (g2 != g2) g2 = 0.f;
+    return make_float3(g0, g1, g2);
+}
+
 // GetBasisSplineValue<double> (reg-lib/cpu/_reg_nmi.h:116-129); 2.f/3.f is a float constant there.
 __device__ __forceinline__ double nmi_basis(double x) {
     x = x < 0 ? -x : x;
@@ -143,6 +238,50 @@ __device__ __forceinline__ float nmi_basis_deriv_f(float ox) {
     x -= 2.f;
     const float v = -0.5f * x * x;
     return ox < 0 ? -v : v;
+}
+
+// The voxel term of reg_getVoxelBasedNmiGradient3d (reg-lib/cpu/_reg_nmi.cpp:501-593): the FP64
+// sums over the reference's bin loops r in [(int)(rv - 1), (int)(rv + 3)), w likewise, of
+// B(rv - r) B'(wv - w) g log p, returned as weight (rd + wd - NMI jd) / (H_rf total) per axis.
+// logh: the reference's jointHistogramLog ([rb fb] joint, then rb + fb marginals); ent: H_r, H_f,
+// H_rf, total.  The caller has checked that rv and wv are numbers.
+__device__ __forceinline__ void nmi_voxel_term(float rv, float wv, const float g[3], const double *__restrict__ logh,
+                                               const double *__restrict__ ent, int rb, int fb, double weight,
+                                               float out[3]) {
+#pragma clang fp contract(off)
+    const double nmi = (ent[0] + ent[1]) / ent[2];
+    const size_t refOff = (size_t)rb * fb, floOff = refOff + rb;
+    double jd[3] = { 0, 0, 0 }, rd[3] = { 0, 0, 0 }, wd[3] = { 0, 0, 0 };
+    // the Parzen values of each axis are evaluated once instead of once per pair
+    const int r0 = (int)(rv - 1.f), r1 = (int)(rv + 3.f), w0 = (int)(wv - 1.f), w1 = (int)(wv + 3.f);
+    double bR[4], dW[4], rl[4], wl[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int r = r0 + k, w = w0 + k;
+        bR[k] = nmi_basis((double)(rv - (float)r));
+        dW[k] = nmi_basis_deriv((double)(wv - (float)w));
+        rl[k] = (-1 < r && r < rb) ? logh[r + refOff] : 0.0;
+        wl[k] = (-1 < w && w < fb) ? logh[w + floOff] : 0.0;
+    }
+    for (int kr = 0; kr < 4 && r0 + kr < r1; ++kr) {
+        const int r = r0 + kr;
+        if (!(-1 < r && r < rb)) continue;
+        for (int kw = 0; kw < 4 && w0 + kw < w1; ++kw) {
+            const int w = w0 + kw;
+            if (!(-1 < w && w < fb)) continue;
+            const double common = bR[kr] * dW[kw];
+            const double jl = logh[r + w * rb];
+            for (int d = 0; d < 3; ++d) {
+                if (g[d] == g[d]) {
+                    rd[d] += common * (double)g[d] * rl[kr];
+                    wd[d] += common * (double)g[d] * wl[kw];
+                    jd[d] += common * (double)g[d] * jl;
+                }
+            }
+        }
+    }
+    const double den = ent[2] * ent[3];
+    for (int d = 0; d < 3; ++d) out[d] = (float)(weight * (rd[d] + wd[d] - nmi * jd[d]) / den);
 }
 
 // Block-wide sum of doubles (blockDim.x a multiple of 64, at most 1024).

@@ -4,11 +4,14 @@
 // Every objective and gradient evaluation runs on the device through the F3D handle.
 
 #include "f3d_handle.hpp"
+#include "f3d_batch.hpp"
 #include "host.hpp"
 #include <algorithm>
 #include <cmath>
 #include <limits>
 #include <memory>
+#include <chrono>
+#include <thread>
 
 namespace nrh {
 
@@ -246,32 +249,48 @@ void check(int rc) {
 
 }  // namespace
 
-void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& floG, const float *floData,
-                   const nrh_f3d_config& cfg, nrh_geom& gridOut, std::vector<float>& cppOut, double& finalNmi,
-                   int& totalIterations) {
+// The host side of one reg_f3d registration that does not depend on the device: weights,
+// pyramids, the control-point grid of every level, and the iteration budget.
+struct PairRun {
+    nrh_f3d_config cfg;
+    int levels = 1, toPerform = 1;
+    float be = 0, le = 0, simW = 0;
+    std::vector<Level> rp, fp;
+    nrh_geom grid;
+    std::vector<float> cpp;
+    size_t maxIter = 0;
+    double finalNmi = 0;
+    int totalIterations = 0;
+};
+
+void pair_begin(PairRun& r, const nrh_geom& refG, const float *refData, const nrh_geom& floG, const float *floData,
+                const nrh_f3d_config& cfg) {
     NRH_REQUIRE(cfg.levels >= 1 && cfg.bins >= 1 && cfg.max_iterations >= 0, NRH_ERR_ARG, "invalid configuration");
+    r.cfg = cfg;
     const int levels = cfg.levels;
     int toPerform = cfg.levels_to_perform > 0 ? std::min(cfg.levels_to_perform, levels) : levels;
     if (!cfg.use_pyramid) toPerform = levels;
+    r.levels = levels;
+    r.toPerform = toPerform;
     // reg_f3d::CheckParameters: weights of the objective (reg-lib/_reg_f3d.cpp:142-159)
     float be = cfg.be_weight, le = cfg.le_weight;
     const float penalty = be + le;
-    float simW;
     if (penalty >= 1) {
-        simW = 0;
+        r.simW = 0;
         be /= penalty;
         le /= penalty;
     } else {
-        simW = 1 - penalty;
+        r.simW = 1 - penalty;
     }
+    r.be = be;
+    r.le = le;
     // pyramids (or the full-resolution images at every level with -nopy)
-    std::vector<Level> rp, fp;
     if (cfg.use_pyramid) {
-        rp = pyramid(refG, refData, levels, toPerform);
-        fp = pyramid(floG, floData, levels, toPerform);
+        r.rp = pyramid(refG, refData, levels, toPerform);
+        r.fp = pyramid(floG, floData, levels, toPerform);
     } else {
-        rp.assign(1, Level{ refG, std::vector<float>(refData, refData + voxels(refG)) });
-        fp.assign(1, Level{ floG, std::vector<float>(floData, floData + voxels(floG)) });
+        r.rp.assign(1, Level{ refG, std::vector<float>(refData, refData + voxels(refG)) });
+        r.fp.assign(1, Level{ floG, std::vector<float>(floData, floData + voxels(floG)) });
     }
     // control-point grid at the coarsest level (reg-lib/_reg_f3d.cpp:161-195)
     float spmm = cfg.spacing < 0 ? -cfg.spacing * refG.dx : cfg.spacing;
@@ -279,77 +298,107 @@ void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& f
     float spz = cfg.spacing < 0 ? -cfg.spacing * refG.dz : cfg.spacing;
     const float scale = std::pow(2.f, (float)(levels - 1));
     const float gsp[3] = { spmm * scale, spy * scale, spz * scale };
-    nrh_geom grid;
-    std::vector<float> cpp;
-    {
-        const nrh_geom& g0 = rp[0].geom;
-        create_grid(g0, gsp, grid, nullptr);
-        cpp.resize(3 * voxels(grid));
-        create_grid(g0, gsp, grid, cpp.data());
-        if (cfg.affine) {
-            // reg_affine_getDeformationField(affine, grid) (reg-lib/_reg_f3d.cpp:190-191,
-            // reg-lib/cpu/_reg_globalTrans.cpp:74-133): node positions A S i in double
-            float T[16];
-            mat44_product(cfg.affine, grid.xyz, T);
-            const size_t G = voxels(grid);
-            size_t i = 0;
-            for (int z = 0; z < grid.nz; ++z)
-                for (int y = 0; y < grid.ny; ++y)
-                    for (int x = 0; x < grid.nx; ++x, ++i)
-                        for (int r = 0; r < 3; ++r) {
-                            volatile double v = (double)T[r * 4] * (double)x;
-                            v = v + (double)T[r * 4 + 1] * (double)y;
-                            v = v + (double)T[r * 4 + 2] * (double)z;
-                            v = v + (double)T[r * 4 + 3];
-                            cpp[i + r * G] = (float)v;
-                        }
-        }
+    const nrh_geom& g0 = r.rp[0].geom;
+    create_grid(g0, gsp, r.grid, nullptr);
+    r.cpp.resize(3 * voxels(r.grid));
+    create_grid(g0, gsp, r.grid, r.cpp.data());
+    if (cfg.affine) {
+        // reg_affine_getDeformationField(affine, grid) (reg-lib/_reg_f3d.cpp:190-191,
+        // reg-lib/cpu/_reg_globalTrans.cpp:74-133): node positions A S i in double
+        float T[16];
+        mat44_product(cfg.affine, r.grid.xyz, T);
+        const size_t G = voxels(r.grid);
+        size_t i = 0;
+        for (int z = 0; z < r.grid.nz; ++z)
+            for (int y = 0; y < r.grid.ny; ++y)
+                for (int x = 0; x < r.grid.nx; ++x, ++i)
+                    for (int c = 0; c < 3; ++c) {
+                        volatile double v = (double)T[c * 4] * (double)x;
+                        v = v + (double)T[c * 4 + 1] * (double)y;
+                        v = v + (double)T[c * 4 + 2] * (double)z;
+                        v = v + (double)T[c * 4 + 3];
+                        r.cpp[i + c * G] = (float)v;
+                    }
     }
-    size_t maxIter = (size_t)cfg.max_iterations * (size_t)std::pow(2.0, toPerform - 1);
-    totalIterations = 0;
-    finalNmi = 0;
-    for (int level = 0; level < toPerform; ++level) {
-        const int idx = cfg.use_pyramid ? level : 0;
-        const nrh_geom& lr = rp[idx].geom;
-        const nrh_geom& lf = fp[idx].geom;
-        float maxStep = std::max(lr.dx, lr.dy);
-        maxStep = std::max(maxStep, lr.dz);
-        // reg_f3d::InitCurrentLevel (reg-lib/_reg_f3d.cpp:92-133)
-        if (level == 0) {
-            be = be / std::pow(16.f, (float)(levels - 1));
-            le = le / std::pow(3.f, (float)(levels - 1));
-        } else {
-            be = be * 16;
-            le = le * 3;
-            nrh_geom ng;
-            std::vector<float> nd;
-            refine_grid(lr, grid, cpp, ng, nd);
-            grid = ng;
-            cpp.swap(nd);
-        }
-        nrh_f3d_params p{};
-        p.ref_bins = cfg.bins + 4;
-        p.flo_bins = cfg.bins + 4;
-        p.padding = cfg.padding;
-        p.similarity_weight = simW;
-        p.be_weight = be;
-        p.le_weight = le;
-        p.image_storage = cfg.image_storage;
-        p.measure = cfg.measure;
-        p.lncc_sigma = cfg.lncc_sigma;
+    r.maxIter = (size_t)cfg.max_iterations * (size_t)std::pow(2.0, toPerform - 1);
+    r.totalIterations = 0;
+    r.finalNmi = 0;
+}
+
+// reg_f3d::InitCurrentLevel (reg-lib/_reg_f3d.cpp:92-133): the level's penalty weights and grid
+// (refined after the first level), and the handle parameters and largest step of the level.
+void pair_level(PairRun& r, int level, nrh_f3d_params& p, float& maxStep) {
+    const int idx = r.cfg.use_pyramid ? level : 0;
+    const nrh_geom& lr = r.rp[idx].geom;
+    maxStep = std::max(lr.dx, lr.dy);
+    maxStep = std::max(maxStep, lr.dz);
+    if (level == 0) {
+        r.be = r.be / std::pow(16.f, (float)(r.levels - 1));
+        r.le = r.le / std::pow(3.f, (float)(r.levels - 1));
+    } else {
+        r.be = r.be * 16;
+        r.le = r.le * 3;
+        nrh_geom ng;
+        std::vector<float> nd;
+        refine_grid(lr, r.grid, r.cpp, ng, nd);
+        r.grid = ng;
+        r.cpp.swap(nd);
+    }
+    p = nrh_f3d_params{};
+    p.ref_bins = r.cfg.bins + 4;
+    p.flo_bins = r.cfg.bins + 4;
+    p.padding = r.cfg.padding;
+    p.similarity_weight = r.simW;
+    p.be_weight = r.be;
+    p.le_weight = r.le;
+    p.image_storage = r.cfg.image_storage;
+    p.measure = r.cfg.measure;
+    p.lncc_sigma = r.cfg.lncc_sigma;
+    p.field_order = r.cfg.field_order;
+}
+
+// A level's handle: the level's images, the grid's points, and the measure's intensity rescaling
+// (reg_nmi::InitialiseMeasure to [2, bins - 3], reg-lib/cpu/_reg_nmi.cpp:115-129; reg_lncc to [0, 1],
+// reg-lib/cpu/_reg_lncc.cpp:99-104).
+nrh_f3d *pair_handle(PairRun& r, int level, const nrh_f3d_params& p) {
+    const int idx = r.cfg.use_pyramid ? level : 0;
+    nrh_f3d *h = nullptr;
+    check(nrh_f3d_create(&r.rp[idx].geom, r.rp[idx].data.data(), &r.fp[idx].geom, r.fp[idx].data.data(), nullptr,
+                         &r.grid, &p, &h));
+    Handle H;
+    H.h = h;
+    check(nrh_f3d_set_control_points(h, r.cpp.data()));
+    if (p.measure == NRH_MEASURE_LNCC) check(nrh_f3d_rescale_intensities(h, 0.f, 1.f, 0.f, 1.f));
+    else check(nrh_f3d_rescale_intensities(h, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
+    H.h = nullptr;
+    return h;
+}
+
+// End of a level: the final points and the similarity of the level's last objective.
+void pair_level_end(PairRun& r, double terms0, int iterations) {
+    r.totalIterations += iterations;
+    r.finalNmi = r.simW > 0 ? terms0 / r.simW : 0.0;
+    r.maxIter /= 2;
+}
+
+void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& floG, const float *floData,
+                   const nrh_f3d_config& cfg, nrh_geom& gridOut, std::vector<float>& cppOut, double& finalNmi,
+                   int& totalIterations) {
+    PairRun r;
+    pair_begin(r, refG, refData, floG, floData, cfg);
+    for (int level = 0; level < r.toPerform; ++level) {
+        nrh_f3d_params p;
+        float maxStep;
+        pair_level(r, level, p, maxStep);
         Handle H;
-        check(nrh_f3d_create(&lr, rp[idx].data.data(), &lf, fp[idx].data.data(), nullptr, &grid, &p, &H.h));
+        H.h = pair_handle(r, level, p);
         nrh_f3d *h = H.h;
-        check(nrh_f3d_set_control_points(h, cpp.data()));
-        // reg_nmi::InitialiseMeasure rescales to [2, bins - 3] (reg-lib/cpu/_reg_nmi.cpp:115-129),
-        // reg_lncc::InitialiseMeasure to [0, 1] (reg-lib/cpu/_reg_lncc.cpp:99-104)
-        if (p.measure == NRH_MEASURE_LNCC) check(nrh_f3d_rescale_intensities(h, 0.f, 1.f, 0.f, 1.f));
-        else check(nrh_f3d_rescale_intensities(h, 2.f, p.ref_bins - 3.f, 2.f, p.flo_bins - 3.f));
         // Optimiser::Initialise (reg-lib/Optimiser.cpp:48-90)
         check(nrh_f3d_store_best(h));
         double best = 0, terms[3];
         check(nrh_f3d_objective(h, &best, terms));
         size_t iter = 0;
+        const size_t maxIter = r.maxIter;
         const float smallest = maxStep / 100.f;
         float currentSize = maxStep;
         bool firstCall = true, justRestarted = false;
@@ -394,15 +443,124 @@ void register_pair(const nrh_geom& refG, const float *refData, const nrh_geom& f
                 justRestarted = false;
             }
         }
-        totalIterations += (int)iter;
-        check(nrh_f3d_get_control_points(h, cpp.data()));
+        check(nrh_f3d_get_control_points(h, r.cpp.data()));
         double value = 0;
         check(nrh_f3d_objective(h, &value, terms));
-        finalNmi = simW > 0 ? terms[0] / simW : 0.0;
-        maxIter /= 2;
+        pair_level_end(r, terms[0], (int)iter);
     }
-    gridOut = grid;
-    cppOut = cpp;
+    gridOut = r.grid;
+    cppOut = r.cpp;
+    finalNmi = r.finalNmi;
+    totalIterations = r.totalIterations;
+}
+
+
+// Many reg_f3d registrations at once: every pair runs register_pair's levels and decisions, but
+// all pairs' evaluations share the batch launches of f3d_batch.hip, and each level's iterations
+// replay one captured HIP graph (gradient, conjugate gradient, twelve lockstep line-search steps,
+// restore and stopping rule) until every pair has stopped.  The pyramids are built on host threads.
+void register_batch(int n, const nrh_geom *refs, const float *const *refData, const nrh_geom *flos,
+                    const float *const *floData, const nrh_f3d_config& cfg, std::vector<PairRun>& runs,
+                    nrh_batch_stats *stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    runs.assign(n, PairRun{});
+    {
+        std::vector<std::thread> pool;
+        std::vector<std::string> errs(n);
+        const int T = std::max(1, std::min(n, (int)std::thread::hardware_concurrency()));
+        for (int t = 0; t < T; ++t)
+            pool.emplace_back([&, t] {
+                for (int q = t; q < n; q += T) {
+                    try {
+                        pair_begin(runs[q], refs[q], refData[q], flos[q], floData[q], cfg);
+                    } catch (const std::exception& e) {
+                        errs[q] = e.what();
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+        for (int q = 0; q < n; ++q) NRH_REQUIRE(errs[q].empty(), NRH_ERR_ARG, "pair " + std::to_string(q) + ": " + errs[q]);
+    }
+    const int toPerform = runs[0].toPerform;
+    double deviceSeconds = 0.0;
+    int replays = 0;
+    for (int level = 0; level < toPerform; ++level) {
+        std::vector<nrh_f3d *> handles(n, nullptr);
+        std::vector<float> maxStep(n);
+        std::vector<int> maxIter(n);
+        struct Owner {
+            std::vector<nrh_f3d *> *hs;
+            nrh_batch *b = nullptr;
+            ~Owner() {
+                if (b) batch_free(b);
+                else
+                    for (nrh_f3d *h : *hs)
+                        if (h) nrh_f3d_destroy(h);
+            }
+        } own{ &handles };
+        for (int q = 0; q < n; ++q) {
+            nrh_f3d_params p;
+            pair_level(runs[q], level, p, maxStep[q]);
+            handles[q] = pair_handle(runs[q], level, p);
+            maxIter[q] = (int)std::min<size_t>(runs[q].maxIter, 1u << 30);
+        }
+        nrh_batch *b = batch_new(handles.data(), n);
+        b->ownsPairs = true;
+        own.b = b;
+        batch_autotune(b);
+        if (stats && level < 8) stats->blocks_per_xcd[level] = b->tunedBlocksPerXcd;
+        batch_ls_alloc(b);
+        batch_ls_init(b, maxStep, maxIter);
+        NRH_HIP(hipStreamSynchronize(b->stream));
+        // one iteration of every pair, captured once and replayed
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        NRH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            batch_ls_iteration(b);
+        } catch (...) {
+            (void)hipStreamEndCapture(b->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        NRH_HIP(hipStreamEndCapture(b->stream, &graph));
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        NRH_HIP(ie);
+        NRH_HIP(hipStreamSynchronize(b->stream));
+        const auto d0 = std::chrono::steady_clock::now();
+        const int maxReplays = *std::max_element(maxIter.begin(), maxIter.end()) + 1;
+        hipError_t ge = hipSuccess;
+        for (int r = 0; r < maxReplays; ++r) {
+            ge = hipGraphLaunch(exec, b->stream);
+            if (ge != hipSuccess) break;
+            ge = hipStreamSynchronize(b->stream);
+            ++replays;
+            if (ge != hipSuccess || *b->remaining == 0) break;
+        }
+        (void)hipGraphExecDestroy(exec);
+        NRH_HIP(ge);
+        deviceSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - d0).count();
+        std::vector<LsState> st(n);
+        NRH_HIP(hipMemcpy(st.data(), b->lsState.p, n * sizeof(LsState), hipMemcpyDeviceToHost));
+        batch_touch(b, true);
+        // the final points and their objective (reg_f3d's last objective of the level)
+        batch_objective(b, nullptr);
+        NRH_HIP(hipStreamSynchronize(b->stream));
+        for (int q = 0; q < n; ++q) {
+            check(nrh_f3d_get_control_points(handles[q], runs[q].cpp.data()));
+            const double terms0 = runs[q].simW > 0 ? (double)runs[q].simW * b->published[q * kBatchResults] : 0.0;
+            pair_level_end(runs[q], terms0, st[q].iter);
+        }
+    }
+    if (stats) {
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        stats->device_seconds = deviceSeconds;
+        stats->graph_replays = replays;
+        stats->tile[0] = 64;
+        stats->tile[1] = 8;
+        stats->tile[2] = 8;
+    }
 }
 
 
@@ -500,6 +658,7 @@ void register_pair_sym(const nrh_geom& refG, const float *refData, const nrh_geo
         p.image_storage = 0;
         p.measure = cfg.measure;
         p.lncc_sigma = cfg.lncc_sigma;
+        p.field_order = cfg.field_order;
         Handle F, B;
         check(nrh_f3d_create(&lr, rp[idx].data.data(), &lf, fp[idx].data.data(), nullptr, &grid, &p, &F.h));
         check(nrh_f3d_create(&lf, fp[idx].data.data(), &lr, rp[idx].data.data(), nullptr, &grid, &p, &B.h));
@@ -615,6 +774,35 @@ int nrh_f3d_register(const nrh_geom *ref, const float *ref_data, const nrh_geom 
         }
         if (final_nmi) *final_nmi = nmi;
         if (iterations) *iterations = it;
+    });
+}
+
+int nrh_f3d_register_batch(int n, const nrh_geom *refs, const float *const *ref_data, const nrh_geom *flos,
+                           const float *const *flo_data, const nrh_f3d_config *cfg, nrh_geom *grids_out,
+                           float *const *cpp_out, const size_t *capacity, double *final_nmi, int *iterations,
+                           nrh_batch_stats *stats) {
+    return guard([&] {
+        NRH_REQUIRE(n >= 1 && refs && ref_data && flos && flo_data && cfg && grids_out, NRH_ERR_ARG, "null argument");
+        NRH_REQUIRE(cfg->measure == NRH_MEASURE_NMI && cfg->le_weight == 0.f && cfg->be_weight > 0.f, NRH_ERR_UNSUPPORTED,
+                    "batched registration: NMI with a bending-energy term and no linear energy");
+        for (int q = 0; q < n; ++q) {
+            check_geom(&refs[q], "reference");
+            check_geom(&flos[q], "floating");
+            NRH_REQUIRE(ref_data[q] && flo_data[q], NRH_ERR_ARG, "null image");
+        }
+        if (stats) *stats = nrh_batch_stats{};
+        std::vector<PairRun> runs;
+        register_batch(n, refs, ref_data, flos, flo_data, *cfg, runs, stats);
+        for (int q = 0; q < n; ++q) {
+            grids_out[q] = runs[q].grid;
+            if (cpp_out && cpp_out[q]) {
+                NRH_REQUIRE(capacity && runs[q].cpp.size() <= capacity[q], NRH_ERR_ARG,
+                            "cpp_out too small for the final grid of pair " + std::to_string(q));
+                std::memcpy(cpp_out[q], runs[q].cpp.data(), runs[q].cpp.size() * sizeof(float));
+            }
+            if (final_nmi) final_nmi[q] = runs[q].finalNmi;
+            if (iterations) iterations[q] = runs[q].totalIterations;
+        }
     });
 }
 

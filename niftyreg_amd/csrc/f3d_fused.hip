@@ -19,9 +19,19 @@
 // node.  The projection is therefore the transpose of the separable field evaluation: a pass
 // along x, one along y, one along z (with normalisation, reorientation and weight), each summing
 // in a fixed order (deterministic).
+//
+// Voxels on voxel planes.  The reference evaluates the field in float; where a floating-image
+// position lands within rounding of a voxel plane, its last bit decides the trilinear cell (and
+// so the image gradient, which jumps there) and, on the image border, whether the voxel is inside
+// at all.  On the identity grid every registration starts from, every voxel is such a voxel.  The
+// sweeps therefore list each voxel whose position lies within a tolerance of a plane (above the
+// reference's own float error) instead of using it, and a second pass (k_sweep_fixup) evaluates
+// the listed voxels exactly as the reference does: its float field in the summation order of the
+// selected build, its resampling, image gradient and FP64 NMI voxel term.
 
 #include "device.hpp"
 #include "f3d_handle.hpp"
+#include "f3d_batch.hpp"
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -51,16 +61,35 @@ struct FusedArgs {
     int tilesX, tilesY, tilesZ, numTiles;
     Affine cppXyz;  // grid voxel -> mm
     uint32_t *hist;
-    const float *logTab;
     const double *ent;
     const float *kTab;  // gradient sweep: K[r + w rb] = log p_r + log p_w - NMI log p_rw (padded to 4)
     float *gvox;      // gradient sweep: the NMI voxel gradient (planar [3][N])
     float *fieldOut;  // optional: the deformation field of the sweep (planar [3][N]), for checking
+    // voxels left to k_sweep_fixup: positions within ambTol (floating voxels) of a voxel plane
+    uint32_t *ambList;
+    uint32_t *ambCount;
+    float ambTol;
+    int fieldOrder;  // NRH_FIELD_ORDER_*
+    int lut;         // 5-voxel spacing on every axis (the reference's table path)
+    const double *logh;  // the reference's jointHistogramLog (fixup of the gradient sweep)
 };
 
 template<class T> __device__ __forceinline__ float ld(const T *p, size_t i);
 template<> __device__ __forceinline__ float ld<float>(const float *p, size_t i) { return p[i]; }
 template<> __device__ __forceinline__ float ld<__half>(const __half *p, size_t i) { return __half2float(p[i]); }
+
+// Append the voxels of a wave that ask for it to a list (one atomic per wave).  The list holds
+// one slot per voxel of the image, so it cannot overflow.
+__device__ __forceinline__ void append_voxels(bool want, uint32_t idx, uint32_t *list, uint32_t *count) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (want) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = idx;
+}
 
 // Tile index with the tiles of one XCD contiguous: consecutive workgroups are dealt round-robin
 // to the 8 XCDs, so workgroup b runs on XCD b % 8; giving XCD k a contiguous block of tiles keeps
@@ -92,6 +121,7 @@ struct VoxelTaps {
 struct VoxelPos {
     int fx, fy, fz;
     float rx, ry, rz;
+    bool amb;  // within ambTol of a voxel plane on some axis
 };
 
 __device__ __forceinline__ VoxelPos sweep_position(const FusedArgs& a, const float *ayz, const float4 b,
@@ -112,6 +142,7 @@ __device__ __forceinline__ VoxelPos sweep_position(const FusedArgs& a, const flo
     }
     const float3 p = affine_apply_f(a.floIjk, w[0], w[1], w[2]);
     VoxelPos r;
+    r.amb = fminf(fminf(fabsf(p.x - rintf(p.x)), fabsf(p.y - rintf(p.y))), fabsf(p.z - rintf(p.z))) < a.ambTol;
     r.fx = floor_int(p.x);
     r.fy = floor_int(p.y);
     r.fz = floor_int(p.z);
@@ -157,7 +188,7 @@ __device__ __forceinline__ void load_taps(const FusedArgs& a, const TImg *__rest
 }
 
 template<bool GRAD>
-__device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, float *dyn, float coef, float& g0,
+__device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, float *dyn, double coef, float& g0,
                                              float& g1, float& g2) {
     g0 = g1 = g2 = 0.f;
     if (t.rx < 0) t.rx = 0;
@@ -207,17 +238,18 @@ __device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, fl
             wc[k] = (okw ? wb0 + k : 0) * a.rb;
             dw[k] = okw ? dw[k] : 0.f;
         }
-        float S = 0.f;
+        // the 16 terms summed in FP64, as the reference sums them
+        double S = 0.0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float T = br[0] * dyn[ri[0] + wc[j]] + br[1] * dyn[ri[1] + wc[j]] + br[2] * dyn[ri[2] + wc[j]] +
-                            br[3] * dyn[ri[3] + wc[j]];
-            S += dw[j] * T;
+            const double T = (double)br[0] * (double)dyn[ri[0] + wc[j]] + (double)br[1] * (double)dyn[ri[1] + wc[j]] +
+                             (double)br[2] * (double)dyn[ri[2] + wc[j]] + (double)br[3] * (double)dyn[ri[3] + wc[j]];
+            S += (double)dw[j] * T;
         }
         S *= coef;
-        g0 = S * gx;
-        g1 = S * gy;
-        g2 = S * gz;
+        g0 = (float)(S * (double)gx);
+        g1 = (float)(S * (double)gy);
+        g2 = (float)(S * (double)gz);
     }
 }
 
@@ -225,17 +257,26 @@ __device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, fl
 // along z and y for every (z, y) row of the tile in one pass; the voxels then run without
 // barriers (one wave per x row), which keeps many floating-image gathers in flight.  The window
 // shares LDS with the histogram / NMI table, which are filled after the contraction.
-template<class TImg, bool GRAD>
-__global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
-    __shared__ float sAyz[TZ][TY][WXM][3];
-    __shared__ double sIdYZ[TZ][TY][3];  // identity terms of each row: m[c][1] uy + m[c][2] uz + m[c][3]
-    __shared__ float4 sBx[TX], sBy[TY], sBz[TZ];
-    __shared__ int sPx[TX], sPy[TY], sPz[TZ];
-    extern __shared__ __align__(16) float dyn[];  // window [3][WZM][WYM][WXM], then histogram or table K
+struct SweepShared {
+    float sAyz[TZ][TY][WXM][3];
+    double sIdYZ[TZ][TY][3];  // identity terms of each row: m[c][1] uy + m[c][2] uz + m[c][3]
+    float4 sBx[TX], sBy[TY], sBz[TZ];
+    int sPx[TX], sPy[TY], sPz[TZ];
+};
 
+// One tile t of the pair described by a; dyn: the dynamic LDS (window [3][WZM][WYM][WXM], then
+// the histogram or the table K).  Called by every thread of the workgroup.
+template<class TImg, bool GRAD>
+__device__ __forceinline__ void sweep_tile(const FusedArgs& a, const int t, SweepShared& S, float *dyn) {
+    auto& sAyz = S.sAyz;
+    auto& sIdYZ = S.sIdYZ;
+    auto& sBx = S.sBx;
+    auto& sBy = S.sBy;
+    auto& sBz = S.sBz;
+    auto& sPx = S.sPx;
+    auto& sPy = S.sPy;
+    auto& sPz = S.sPz;
     const int tid = threadIdx.x;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    if (t >= a.numTiles) return;
     const int tx = t % a.tilesX, ty = (t / a.tilesX) % a.tilesY, tz = t / (a.tilesX * a.tilesY);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
     const int nX = min(TX, a.nx - x0), nY = min(TY, a.ny - y0), nZ = min(TZ, a.nz - z0);
@@ -317,7 +358,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
                          (double)a.cppXyz.m[c * 4 + 3];
     }
     __syncthreads();
-    float coef = 0.f;
+    double coef = 0.0;
     if (GRAD) {
         // the table K of k_nmi_table: the voxel NMI derivative is the bicubic contraction
         // sum_r sum_w B(ref - r) B'(war - w) K[r, w]; four float4 loads in flight per thread
@@ -335,7 +376,7 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
                 if (i < n4) dst4[i] = v[u];
             }
         }
-        coef = (float)(1.0 / (a.ent[2] * a.ent[3]));
+        coef = 1.0 / (a.ent[2] * a.ent[3]);
     } else {
         uint32_t *h = (uint32_t *)dyn;
         for (int i = tid; i < a.rb * a.fb; i += kThreads) h[i] = 0u;
@@ -366,6 +407,11 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
                 t[k].active = !a.mask || a.mask[idx[k]] > -1;
                 pos[k] = sweep_position(a, &sAyz[z][y][0][0], b, pxRel, idX, &sIdYZ[z][y][0], fieldOut, idx[k],
                                         t[k].active);
+                // a voxel near a plane goes to the exact pass (rows past the tile's last are
+                // clamped duplicates and list nothing)
+                const bool amb = pos[k].amb && t[k].active && wave + k * (kThreads / 64) < nY;
+                append_voxels(amb, idx[k], a.ambList, a.ambCount);
+                t[k].active = t[k].active && !pos[k].amb;
                 inside = inside && taps_inside(a, pos[k]);
             }
             // wave-uniform choice between the fixed-stride and the clamped gathers (lanes past
@@ -401,6 +447,38 @@ __global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
     }
 }
 
+template<class TImg, bool GRAD>
+__global__ void __launch_bounds__(kThreads) k_sweep(FusedArgs a) {
+    __shared__ SweepShared S;
+    extern __shared__ __align__(16) float dyn[];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= a.numTiles) return;
+    sweep_tile<TImg, GRAD>(a, t, S, dyn);
+}
+
+// Persistent sweep over the (pair, tile) work list of a batch of pairs: item = pair << 20 | tile,
+// sorted by pair.  The list is cut into 8 contiguous ranges, one per XCD (workgroup b runs on XCD
+// b % 8), so a pair's tiles, which read overlapping floating-image lines, share one L2; the
+// workgroups of an XCD take the items of its range in turn.  Items of pairs whose flag in
+// `active` is 0 are skipped (a line search that has ended for that pair).
+template<class TImg, bool GRAD>
+__global__ void __launch_bounds__(kThreads)
+k_sweep_batch(const FusedArgs *__restrict__ args, const uint32_t *__restrict__ items, int nItems,
+              const int *__restrict__ active) {
+    __shared__ SweepShared S;
+    extern __shared__ __align__(16) float dyn[];
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8, perXcd = gridDim.x / 8;
+    const int per = (nItems + 7) / 8;
+    const int lo = xcd * per, hi = min(nItems, lo + per);
+    for (int i = lo + slot; i < hi; i += perXcd) {
+        const uint32_t it = items[i];
+        const int pair = (int)(it >> 20);
+        if (active && !active[pair]) continue;  // uniform over the workgroup
+        sweep_tile<TImg, GRAD>(args[pair], (int)(it & 0xfffffu), S, dyn);
+        __syncthreads();  // the next tile reuses the LDS
+    }
+}
+
 // Projection of the voxel gradient onto the nodes, one axis at a time.  The reference convolves
 // the voxel gradient with the cubic B-spline kernel along x, y, z (each pass normalised by the
 // in-image kernel sum) and samples the result at the nodes (reg-lib/Compute.cpp:230-281); with
@@ -424,10 +502,10 @@ struct ProjXArgs {
 
 constexpr int kCellStride = 13;  // 12 sums per cell, padded to an odd stride (no LDS bank conflicts)
 
-__global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
-    extern __shared__ float ldsX[];  // rows [3][R][nx], then cell sums [R][cells][4][3]
+__device__ __forceinline__ void project_x_block(const ProjXArgs& p, const int block, float *ldsX) {
+    // ldsX: rows [3][R][nx], then cell sums [R][cells][4][3]
     const int R = p.rowsPerBlock;
-    const size_t row0 = (size_t)blockIdx.x * R;
+    const size_t row0 = (size_t)block * R;
     const int nr = (int)min((size_t)R, p.rows - row0);
     const size_t planeIn = p.rows * (size_t)p.nx, planeOut = p.rows * (size_t)p.gnx;
     float *rowsL = ldsX, *cellL = ldsX + (size_t)3 * R * p.nx;
@@ -494,9 +572,7 @@ struct ProjYArgs {
     const int2 *rangeY;
 };
 
-__global__ void __launch_bounds__(256) k_project_y(ProjYArgs p) {
-    // blocks of one XCD take a contiguous range of z slabs, so each slab's rows go through one L2
-    const size_t t = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void project_y_at(const ProjYArgs& p, const size_t t) {
     if (t >= (size_t)p.nz * p.gny * p.gnx) return;
     const int ix = (int)(t % p.gnx), iy = (int)((t / p.gnx) % p.gny), z = (int)(t / ((size_t)p.gnx * p.gny));
     const size_t planeIn = (size_t)p.nz * p.ny * p.gnx, planeOut = (size_t)p.nz * p.gny * p.gnx;
@@ -533,8 +609,7 @@ struct NodeArgs {
     float *nodeGrad;
 };
 
-__global__ void __launch_bounds__(256) k_project_nodes(NodeArgs r) {
-    const size_t n = (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void project_nodes_at(const NodeArgs& r, const size_t n) {
     if (n >= r.G) return;
     const int i = (int)(n % r.gnx), j = (int)((n / r.gnx) % r.gny), k = (int)(n / ((size_t)r.gnx * r.gny));
     const float inv = r.invNx[i] * r.invNy[j] * r.invNz[k];
@@ -562,14 +637,222 @@ __global__ void __launch_bounds__(256) k_project_nodes(NodeArgs r) {
     r.nodeGrad[n + 2 * r.G] = (R[2] * u0 + R[6] * u1 + R[10] * u2) * r.weight;
 }
 
-// K[r + w rb] = log p_r + log p_w - NMI log p_rw in float, the table the gradient sweep contracts.
-__global__ void __launch_bounds__(256) k_nmi_table(const float *__restrict__ logTab, const double *__restrict__ ent,
-                                                   int rb, int fb, float *__restrict__ K) {
+__global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
+    extern __shared__ float ldsX[];
+    project_x_block(p, blockIdx.x, ldsX);
+}
+
+// blocks of one XCD take a contiguous range of z slabs, so each slab's rows go through one L2
+__global__ void __launch_bounds__(256) k_project_y(ProjYArgs p) {
+    project_y_at(p, (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_project_nodes(NodeArgs r) {
+    project_nodes_at(r, (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+}
+
+// Batched projections: item (pair, block of that pair); each output element is computed by the
+// same body, in the same order, as in the pair's own launch.
+__global__ void __launch_bounds__(256)
+k_project_x_batch(const ProjXArgs *__restrict__ A, const int2 *__restrict__ items, const int *active) {
+    extern __shared__ float ldsX[];
+    const int2 it = items[blockIdx.x];
+    if (active && !active[it.x]) return;
+    project_x_block(A[it.x], it.y, ldsX);
+}
+
+__global__ void __launch_bounds__(256)
+k_project_y_batch(const ProjYArgs *__restrict__ A, const int2 *__restrict__ items, const int *active) {
+    const int2 it = items[blockIdx.x];
+    if (active && !active[it.x]) return;
+    project_y_at(A[it.x], (size_t)it.y * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256)
+k_project_nodes_batch(const NodeArgs *__restrict__ A, const int2 *__restrict__ items, const int *active) {
+    const int2 it = items[blockIdx.x];
+    if (active && !active[it.x]) return;
+    project_nodes_at(A[it.x], (size_t)it.y * blockDim.x + threadIdx.x);
+}
+
+// K[r + w rb] = log p_r + log p_w - NMI log p_rw, evaluated in FP64 from the reference's log
+// histogram and rounded once to float: the table the gradient sweep contracts.  Also empties the
+// exact pass's list for the gradient sweep that follows.
+__global__ void __launch_bounds__(256) k_nmi_table(const double *__restrict__ logh, const double *__restrict__ ent,
+                                                   int rb, int fb, float *__restrict__ K, uint32_t *__restrict__ ambCount) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *ambCount = 0u;  // the gradient sweep's exact-pass list starts empty
     if (i >= rb * fb) return;
-    const float nmiF = (float)ent[4];
-    const float *lj = logTab, *lr = logTab + rb * fb, *lw = lr + rb;
-    K[i] = (lr[i % rb] + lw[i / rb]) - nmiF * lj[i];
+    const double nmi = (ent[0] + ent[1]) / ent[2];
+    const double *lr = logh + rb * fb, *lw = lr + rb;
+    K[i] = (float)((lr[i % rb] + lw[i / rb]) - nmi * logh[i]);
+}
+
+// The voxels the sweeps listed, evaluated as the reference evaluates them (float field in the
+// selected build's order, Mat44Mul, ResampleImage3D, and for the gradient TrilinearImageGradient
+// and the FP64 voxel term of reg_getVoxelBasedNmiGradient3d); the histogram pass counts them in
+// LDS and merges the counts, the gradient pass writes their voxel gradient.  Blocks past the
+// listed count exit at once.
+template<class TImg>
+__device__ __forceinline__ float sample_at(const TImg *__restrict__ img, int nx, int ny, int nz, float3 p, float pad) {
+#pragma clang fp contract(off)
+    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
+    float rx = p.x - (float)px, ry = p.y - (float)py, rz = p.z - (float)pz;
+    if (rx < 0) rx = 0;
+    if (ry < 0) ry = 0;
+    if (rz < 0) rz = 0;
+    const float xb[2] = { 1.f - rx, rx }, yb[2] = { 1.f - ry, ry }, zb[2] = { 1.f - rz, rz };
+    float intensity = 0.f;
+    for (int c = 0; c < 2; ++c) {
+        const int Z = pz + c;
+        float yT = 0.f;
+        for (int bb = 0; bb < 2; ++bb) {
+            const int Y = py + bb;
+            float xT = 0.f;
+            for (int aa = 0; aa < 2; ++aa) {
+                const int X = px + aa;
+                const float v = (X > -1 && X < nx && Y > -1 && Y < ny && Z > -1 && Z < nz)
+                                    ? ld(img, ((size_t)Z * ny + Y) * nx + X) : pad;
+                xT += v * xb[aa];
+            }
+            yT += xT * yb[bb];
+        }
+        intensity += yT * zb[c];
+    }
+    return intensity;
+}
+
+template<class TImg>
+__device__ __forceinline__ float3 gradient_at(const TImg *__restrict__ flo, int fnx, int fny, int fnz, float3 p,
+                                              float pad) {
+#pragma clang fp contract(off)
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
+    float r = p.x - (float)px;
+    const float xb[2] = { (float)(1.0 - (double)r), r };
+    r = p.y - (float)py;
+    const float yb[2] = { (float)(1.0 - (double)r), r };
+    r = p.z - (float)pz;
+    const float zb[2] = { (float)(1.0 - (double)r), r };
+    const float deriv[2] = { -1.f, 1.f };
+    for (int c = 0; c < 2; ++c) {
+        const int Z = pz + c;
+        float xx = 0.f, yy = 0.f, zz = 0.f;
+        for (int bb = 0; bb < 2; ++bb) {
+            const int Y = py + bb;
+            float xT = 0.f, yT = 0.f;
+            for (int aa = 0; aa < 2; ++aa) {
+                const int X = px + aa;
+                const float coeff = (X > -1 && X < fnx && Y > -1 && Y < fny && Z > -1 && Z < fnz)
+                                        ? ld(flo, ((size_t)Z * fny + Y) * fnx + X) : pad;
+                xT += coeff * deriv[aa];
+                yT += coeff * xb[aa];
+            }
+            xx += xT * yb[bb];
+            yy += yT * deriv[bb];
+            zz += yT * yb[bb];
+        }
+        g0 += xx * zb[c];
+        g1 += yy * zb[c];
+        g2 += zz * deriv[c];
+    }
+    if (g0 != g0) g0 = 0.f;
+    if (g1 != g1) g1 = 0.f;
+    if (g2 != g2) g2 = 0.f;
+    return make_float3(g0, g1, g2);
+}
+
+constexpr int kFixupBlocks = 1024;
+
+// One listed voxel idx of pair a: the histogram pass counts it into h (LDS), the gradient pass
+// writes its voxel gradient.
+template<class TImg, bool GRAD>
+__device__ __forceinline__ void fixup_voxel(const FusedArgs& a, const uint32_t idx, uint32_t *h) {
+    const TImg *__restrict__ ref = (const TImg *)a.ref;
+    const TImg *__restrict__ flo = (const TImg *)a.flo;
+    const int x = (int)(idx % (uint32_t)a.nx), y = (int)((idx / (uint32_t)a.nx) % (uint32_t)a.ny);
+    const int z = (int)(idx / ((uint32_t)a.nx * a.ny));
+    const float4 bx = a.bX[x], by = a.bY[y], bz = a.bZ[z];
+    const float xb[4] = { bx.x, bx.y, bx.z, bx.w }, yb[4] = { by.x, by.y, by.z, by.w };
+    const float zb[4] = { bz.x, bz.y, bz.z, bz.w };
+    const float3 w = ref_spline_point(a.cpp, a.G, a.gnx, a.gny, a.preX[x], a.preY[y], a.preZ[z], xb, yb, zb,
+                                      a.fieldOrder, a.lut != 0);
+    const float3 p = affine_apply_d(a.floIjk, w.x, w.y, w.z);
+    const float war = sample_at(flo, a.fnx, a.fny, a.fnz, p, a.pad);
+    const float rv = ld(ref, idx);
+    if (!GRAD) {
+        if (rv == rv && war == war && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
+            atomicAdd(&h[(int)rv + (int)war * a.rb], 1u);
+    } else {
+        float d[3] = { 0.f, 0.f, 0.f };
+        if (rv == rv && war == war) {
+            const float3 g = gradient_at(flo, a.fnx, a.fny, a.fnz, p, a.pad);
+            const float gg[3] = { g.x, g.y, g.z };
+            nmi_voxel_term(rv, war, gg, a.logh, a.ent, a.rb, a.fb, 1.0, d);
+        }
+        const uint32_t N = (uint32_t)a.nx * a.ny * a.nz;
+        a.gvox[idx] = d[0];
+        a.gvox[idx + N] = d[1];
+        a.gvox[idx + 2 * N] = d[2];
+    }
+}
+
+// The listed voxels of one pair, grid-strided over the launch; blocks past the listed count
+// return at once.
+template<class TImg, bool GRAD>
+__device__ __forceinline__ void fixup_pair(const FusedArgs& a, uint32_t *h) {
+    const uint32_t n = *a.ambCount;
+    if ((size_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
+    const int tid = threadIdx.x;
+    if (!GRAD) {
+        for (int i = tid; i < a.rb * a.fb; i += blockDim.x) h[i] = 0u;
+        __syncthreads();
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + tid; i < n; i += gridDim.x * blockDim.x)
+        fixup_voxel<TImg, GRAD>(a, a.ambList[i], h);
+    if (!GRAD) {
+        __syncthreads();
+        for (int i = tid; i < a.rb * a.fb; i += blockDim.x)
+            if (h[i]) atomicAdd(&a.hist[i], h[i]);
+    }
+}
+
+template<class TImg, bool GRAD>
+__global__ void __launch_bounds__(256) k_sweep_fixup(FusedArgs a) {
+    extern __shared__ __align__(16) float fixLds[];  // histogram counts (objective pass)
+    fixup_pair<TImg, GRAD>(a, (uint32_t *)fixLds);
+}
+
+template<class TImg, bool GRAD>
+__global__ void __launch_bounds__(256) k_sweep_fixup_batch(const FusedArgs *__restrict__ A, int P, const int *active) {
+    extern __shared__ __align__(16) float fixLds[];
+    for (int q = 0; q < P; ++q) {
+        if (active && !active[q]) continue;
+        fixup_pair<TImg, GRAD>(A[q], (uint32_t *)fixLds);
+        __syncthreads();  // the LDS histogram of the next pair
+    }
+}
+
+// Per pair: zero the joint-histogram counts and the exact pass's list count (block = pair).
+__global__ void k_clear_counts_batch(const FusedArgs *__restrict__ A, const int *active) {
+    if (active && !active[blockIdx.x]) return;
+    const FusedArgs& a = A[blockIdx.x];
+    for (int i = threadIdx.x; i < a.rb * a.fb; i += blockDim.x) a.hist[i] = 0u;
+    if (threadIdx.x == 0) *a.ambCount = 0u;
+}
+
+// Per pair (blockIdx.y): the NMI table K of the gradient sweep; empties the pair's list.
+__global__ void __launch_bounds__(256) k_nmi_table_batch(const FusedArgs *__restrict__ A, const int *active) {
+#pragma clang fp contract(off)
+    if (active && !active[blockIdx.y]) return;
+    const FusedArgs& a = A[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *a.ambCount = 0u;
+    if (i >= a.rb * a.fb) return;
+    const double nmi = (a.ent[0] + a.ent[1]) / a.ent[2];
+    const double *lr = a.logh + a.rb * a.fb, *lw = lr + a.rb;
+    ((float *)a.kTab)[i] = (float)((lr[i % a.rb] + lw[i / a.rb]) - nmi * a.logh[i]);
 }
 
 __global__ void k_axis_table(int n, float gvs, int lut, int *pre, float4 *b) {
@@ -602,6 +885,10 @@ struct FusedPlan {
     DevBuf<float> invNx, invNy, invNz;
     DevBuf<float> gvox, projX, projY;  // voxel gradient [3][N]; after the x pass; after the y pass
     DevBuf<float> kTab;                // NMI derivative table, rb * fb rounded up to 4 floats
+    DevBuf<uint32_t> ambList;          // voxels left to the exact pass (one slot per voxel)
+    DevBuf<uint32_t> ambCount;
+    float ambTol = 0.f;                // floating voxels
+    bool lut = false;
 };
 
 // The plan is kept beside the handle in a side table keyed by handle address.
@@ -726,6 +1013,28 @@ static FusedPlan *plan_of(nrh_f3d *h) {
         p->tilesZ = tiles[2];
         p->numTiles = tiles[0] * tiles[1] * tiles[2];
         p->ok = ok;
+        p->lut = lut;
+        // Tolerance of the exact pass: the reference's float field is within a few float ulps of
+        // its largest coordinates of the exact sum (measured: 1.4e-4 mm at 256 mm, about 9 ulps);
+        // 48 ulps of the grid's largest corner coordinate (with a margin for displacements),
+        // mapped into floating voxels by the largest row of the floating mm -> voxel matrix.
+        double M = 0.0;
+        for (int corner = 0; corner < 8; ++corner) {
+            const double v[3] = { (corner & 1) ? h->cpp.nx - 1.0 : 0.0, (corner & 2) ? h->cpp.ny - 1.0 : 0.0,
+                                  (corner & 4) ? h->cpp.nz - 1.0 : 0.0 };
+            for (int r = 0; r < 3; ++r) {
+                const double w = h->cpp.xyz[r * 4] * v[0] + h->cpp.xyz[r * 4 + 1] * v[1] +
+                                 h->cpp.xyz[r * 4 + 2] * v[2] + h->cpp.xyz[r * 4 + 3];
+                M = std::max(M, std::fabs(w));
+            }
+        }
+        double rowMax = 0.0;
+        for (int r = 0; r < 3; ++r)
+            rowMax = std::max(rowMax, (double)std::fabs(h->flo.ijk[r * 4]) + std::fabs(h->flo.ijk[r * 4 + 1]) +
+                                          std::fabs(h->flo.ijk[r * 4 + 2]));
+        p->ambTol = (float)(48.0 * std::ldexp(1.0, -24) * (1.25 * M + 8.0) * rowMax);
+        p->ambList.alloc(h->N);
+        p->ambCount.alloc(1);
         NRH_HIP(hipStreamSynchronize(h->stream));
     }
     std::lock_guard<std::mutex> lk(g_planMutex);
@@ -783,7 +1092,20 @@ static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
     a.bX = p->bX.p; a.bY = p->bY.p; a.bZ = p->bZ.p;
     a.tilesX = p->tilesX; a.tilesY = p->tilesY; a.tilesZ = p->tilesZ;
     a.numTiles = p->numTiles;
+    a.ambList = p->ambList.p;
+    a.ambCount = p->ambCount.p;
+    a.ambTol = p->ambTol;
+    a.fieldOrder = h->params.field_order;
+    a.lut = p->lut ? 1 : 0;
+    a.logh = h->histLog.p;
+    a.ent = h->ent.p;
     return a;
+}
+
+// Zeroes the joint-histogram counts and the exact pass's list count in one launch.
+__global__ void k_clear_counts(uint32_t *__restrict__ hist, int n, uint32_t *__restrict__ count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) hist[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0u;
 }
 
 static unsigned grid_for(int numTiles) { return (unsigned)((numTiles + 7) / 8 * 8); }
@@ -801,8 +1123,9 @@ void fused_histogram(nrh_f3d *h, float *fieldOut) {
     FusedArgs a = make_args(h, p);
     a.hist = h->histCount.p;
     a.fieldOut = fieldOut;
-    op_fill_u32(h->histCount.p, (size_t)a.rb * a.fb, 0u, h->stream);
+    hipLaunchKernelGGL(k_clear_counts, dim3(8), dim3(256), 0, h->stream, h->histCount.p, a.rb * a.fb, a.ambCount);
     const size_t lds = sweep_lds(a);
+    const size_t ldsFix = (size_t)a.rb * a.fb * sizeof(uint32_t);
     if (h->profile) NRH_HIP(hipEventRecord(h->ev[0], h->stream));
     if (h->params.image_storage == 1)
         hipLaunchKernelGGL((k_sweep<__half, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
@@ -813,18 +1136,24 @@ void fused_histogram(nrh_f3d *h, float *fieldOut) {
         NRH_HIP(hipEventRecord(h->ev[1], h->stream));
         h->evPending[0] = true;
     }
+    if (h->params.image_storage == 1)
+        hipLaunchKernelGGL((k_sweep_fixup<__half, false>), dim3(kFixupBlocks), dim3(256), ldsFix, h->stream, a);
+    else
+        hipLaunchKernelGGL((k_sweep_fixup<float, false>), dim3(kFixupBlocks), dim3(256), ldsFix, h->stream, a);
+    NRH_HIP(hipGetLastError());
     op_nmi_finalize(h);
     h->histVersion = h->cppVersion;
 }
 
-// The voxel gradient g (planar [3][N]) projected onto the nodes: x pass (rows staged in LDS,
-// cell sums), y pass, z pass with normalisation, reorientation and weight, into h->nodeGrad.
-static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float weight) {
+// Arguments of the three projection passes of the voxel gradient g (planar [3][N]) onto the
+// nodes of h (into h->nodeGrad); the x pass's LDS size in *ldsX.
+static void projection_args(nrh_f3d *h, FusedPlan *p, const float *g, float weight, ProjXArgs& px, ProjYArgs& py,
+                            NodeArgs& r, size_t *ldsX) {
     const int nx = h->ref.nx, ny = h->ref.ny, nz = h->ref.nz;
     const int gnx = h->cpp.nx, gny = h->cpp.ny, gnz = h->cpp.nz;
     p->projX.alloc((size_t)3 * nz * ny * gnx);
     p->projY.alloc((size_t)3 * nz * gny * gnx);
-    ProjXArgs px{};
+    px = ProjXArgs{};
     px.g = g;
     px.out = p->projX.p;
     px.nx = nx;
@@ -835,12 +1164,9 @@ static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float wei
     px.cells = p->cellsX;
     px.cellX = p->cellX.p;
     px.bX = p->bX.p;
-    const size_t ldsX =
-        ((size_t)3 * px.rowsPerBlock * nx + (size_t)px.rowsPerBlock * px.cells * kCellStride) * sizeof(float);
-    NRH_REQUIRE(ldsX <= 64 * 1024, NRH_ERR_UNSUPPORTED, "image rows too long for the fused projection");
-    hipLaunchKernelGGL(k_project_x, dim3((unsigned)((px.rows + px.rowsPerBlock - 1) / px.rowsPerBlock)), dim3(256),
-                       ldsX, h->stream, px);
-    ProjYArgs py{};
+    *ldsX = ((size_t)3 * px.rowsPerBlock * nx + (size_t)px.rowsPerBlock * px.cells * kCellStride) * sizeof(float);
+    NRH_REQUIRE(*ldsX <= 64 * 1024, NRH_ERR_UNSUPPORTED, "image rows too long for the fused projection");
+    py = ProjYArgs{};
     py.in = p->projX.p;
     py.out = p->projY.p;
     py.ny = ny;
@@ -850,10 +1176,7 @@ static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float wei
     py.preY = p->preY.p;
     py.bY = p->bY.p;
     py.rangeY = p->rY.p;
-    // xcd_tile needs a grid that is a multiple of 8 (extra blocks find no work)
-    auto blocks8 = [](size_t n) { return dim3((unsigned)(((n + 255) / 256 + 7) / 8 * 8)); };
-    hipLaunchKernelGGL(k_project_y, blocks8((size_t)nz * gny * gnx), dim3(256), 0, h->stream, py);
-    NodeArgs r{};
+    r = NodeArgs{};
     r.in = p->projY.p;
     r.nz = nz;
     r.gnx = gnx; r.gny = gny; r.gnz = gnz;
@@ -865,6 +1188,21 @@ static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float wei
     r.reorient = to_affine(h->flo.ijk);
     r.weight = node_weight_ratio(h, weight);
     r.nodeGrad = h->nodeGrad.p;
+}
+
+// The voxel gradient g (planar [3][N]) projected onto the nodes: x pass (rows staged in LDS,
+// cell sums), y pass, z pass with normalisation, reorientation and weight, into h->nodeGrad.
+static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float weight) {
+    ProjXArgs px;
+    ProjYArgs py;
+    NodeArgs r;
+    size_t ldsX = 0;
+    projection_args(h, p, g, weight, px, py, r, &ldsX);
+    hipLaunchKernelGGL(k_project_x, dim3((unsigned)((px.rows + px.rowsPerBlock - 1) / px.rowsPerBlock)), dim3(256),
+                       ldsX, h->stream, px);
+    // xcd_tile needs a grid that is a multiple of 8 (extra blocks find no work)
+    auto blocks8 = [](size_t n) { return dim3((unsigned)(((n + 255) / 256 + 7) / 8 * 8)); };
+    hipLaunchKernelGGL(k_project_y, blocks8((size_t)py.nz * py.gny * py.gnx), dim3(256), 0, h->stream, py);
     hipLaunchKernelGGL(k_project_nodes, blocks8(h->G), dim3(256), 0, h->stream, r);
     NRH_HIP(hipGetLastError());
 }
@@ -873,14 +1211,13 @@ void fused_gradient(nrh_f3d *h, float weight) {
     FusedPlan *p = plan_of(h);
     NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
     FusedArgs a = make_args(h, p);
-    a.logTab = h->histLogF.p;
     a.ent = h->ent.p;
     p->gvox.alloc(3 * h->N);
     a.gvox = p->gvox.p;
     const int nbins = a.rb * a.fb;
     p->kTab.alloc((size_t)(nbins + 3) / 4 * 4);
-    hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, a.logTab, a.ent, a.rb, a.fb,
-                       p->kTab.p);
+    hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, h->histLog.p, a.ent, a.rb,
+                       a.fb, p->kTab.p, a.ambCount);
     a.kTab = p->kTab.p;
     const size_t lds = sweep_lds(a);
     if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
@@ -893,6 +1230,11 @@ void fused_gradient(nrh_f3d *h, float weight) {
         NRH_HIP(hipEventRecord(h->ev[3], h->stream));
         h->evPending[1] = true;
     }
+    if (h->params.image_storage == 1)
+        hipLaunchKernelGGL((k_sweep_fixup<__half, true>), dim3(kFixupBlocks), dim3(256), 0, h->stream, a);
+    else
+        hipLaunchKernelGGL((k_sweep_fixup<float, true>), dim3(kFixupBlocks), dim3(256), 0, h->stream, a);
+    NRH_HIP(hipGetLastError());
     project_to_nodes(h, p, p->gvox.p, weight);
 }
 
@@ -903,6 +1245,120 @@ void fused_project(nrh_f3d *h, const float *g, float weight) {
     FusedPlan *p = plan_of(h);
     NRH_REQUIRE(p->ok, NRH_ERR_UNSUPPORTED, "separable node projection unavailable for this geometry");
     project_to_nodes(h, p, g, weight);
+}
+
+// ------------------------------------------------------------------ batches of pairs
+// Every pair must take the fused path (integer spacing of at least 4 voxels, nodes on voxels).
+void batch_fused_setup(nrh_batch *b) {
+    std::vector<FusedArgs> args(b->P);
+    std::vector<ProjXArgs> pxs(b->P);
+    std::vector<ProjYArgs> pys(b->P);
+    std::vector<NodeArgs> nds(b->P);
+    std::vector<uint32_t> items;
+    std::vector<int2> itX, itY, itN;
+    b->sweepLds = b->fixLds = b->projXLds = 0;
+    const int storage = b->pairs[0]->params.image_storage;
+    for (int q = 0; q < b->P; ++q) {
+        nrh_f3d *h = b->pairs[q];
+        NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED,
+                    "batched pairs need the fused path (integer grid spacing >= 4 voxels, nodes on voxels)");
+        NRH_REQUIRE(h->params.image_storage == storage, NRH_ERR_ARG, "batched pairs share one image storage type");
+        FusedPlan *p = plan_of(h);
+        NRH_REQUIRE(p->numTiles < (1 << 20) && b->P < 4096, NRH_ERR_UNSUPPORTED, "batch too large for the work list");
+        FusedArgs a = make_args(h, p);
+        a.hist = h->histCount.p;
+        p->gvox.alloc(3 * h->N);
+        a.gvox = p->gvox.p;
+        const int nbins = a.rb * a.fb;
+        p->kTab.alloc((size_t)(nbins + 3) / 4 * 4);
+        a.kTab = p->kTab.p;
+        args[q] = a;
+        b->sweepLds = std::max(b->sweepLds, sweep_lds(a));
+        b->fixLds = std::max(b->fixLds, (size_t)nbins * sizeof(uint32_t));
+        for (int t = 0; t < p->numTiles; ++t) items.push_back(((uint32_t)q << 20) | (uint32_t)t);
+        size_t ldsX = 0;
+        projection_args(h, p, p->gvox.p, h->params.similarity_weight, pxs[q], pys[q], nds[q], &ldsX);
+        b->projXLds = std::max(b->projXLds, ldsX);
+        const int nbx = (int)((pxs[q].rows + pxs[q].rowsPerBlock - 1) / pxs[q].rowsPerBlock);
+        for (int k = 0; k < nbx; ++k) itX.push_back(make_int2(q, k));
+        const int nby = (int)(((size_t)pys[q].nz * pys[q].gny * pys[q].gnx + 255) / 256);
+        for (int k = 0; k < nby; ++k) itY.push_back(make_int2(q, k));
+        const int nbn = (int)((h->G + 255) / 256);
+        for (int k = 0; k < nbn; ++k) itN.push_back(make_int2(q, k));
+    }
+    b->rb = args[0].rb;
+    b->fb = args[0].fb;
+    auto upload = [&](nrh::DevBuf<uint8_t>& dst, const void *src, size_t bytes) {
+        dst.alloc(bytes);
+        NRH_HIP(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, b->stream));
+    };
+    upload(b->fusedArgs, args.data(), args.size() * sizeof(FusedArgs));
+    upload(b->projXArgs, pxs.data(), pxs.size() * sizeof(ProjXArgs));
+    upload(b->projYArgs, pys.data(), pys.size() * sizeof(ProjYArgs));
+    upload(b->nodeArgs, nds.data(), nds.size() * sizeof(NodeArgs));
+    auto uploadItems = [&](nrh::DevBuf<int2>& dst, const std::vector<int2>& v, int& n) {
+        dst.alloc(v.size());
+        NRH_HIP(hipMemcpyAsync(dst.p, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+        n = (int)v.size();
+    };
+    uploadItems(b->itProjX, itX, b->nProjX);
+    uploadItems(b->itProjY, itY, b->nProjY);
+    uploadItems(b->itNodes, itN, b->nNodes);
+    b->sweepItems.alloc(items.size());
+    NRH_HIP(hipMemcpyAsync(b->sweepItems.p, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           b->stream));
+    b->nSweepItems = (int)items.size();
+    if (b->sweepBlocks == 0) batch_set_sweep_blocks(b, b->tunedBlocksPerXcd > 0 ? b->tunedBlocksPerXcd : 128);
+    NRH_HIP(hipStreamSynchronize(b->stream));
+}
+
+// The persistent grid: blocksPerXcd workgroups on each of the 8 XCDs, never more than the items.
+void batch_set_sweep_blocks(nrh_batch *b, int blocksPerXcd) {
+    const int perXcd = std::max(1, std::min(blocksPerXcd, (b->nSweepItems + 7) / 8));
+    b->sweepBlocks = 8 * perXcd;
+}
+
+void batch_histogram(nrh_batch *b, const int *active) {
+    const FusedArgs *A = (const FusedArgs *)b->fusedArgs.p;
+    hipLaunchKernelGGL(k_clear_counts_batch, dim3(b->P), dim3(256), 0, b->stream, A, active);
+    const bool half = b->pairs[0]->params.image_storage == 1;
+    if (half) {
+        hipLaunchKernelGGL((k_sweep_batch<__half, false>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
+                           A, b->sweepItems.p, b->nSweepItems, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, false>), dim3(kFixupBlocks), dim3(256), b->fixLds, b->stream, A,
+                           b->P, active);
+    } else {
+        hipLaunchKernelGGL((k_sweep_batch<float, false>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
+                           A, b->sweepItems.p, b->nSweepItems, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<float, false>), dim3(kFixupBlocks), dim3(256), b->fixLds, b->stream, A,
+                           b->P, active);
+    }
+    NRH_HIP(hipGetLastError());
+    batch_finalize(b, active);
+}
+
+void batch_similarity_gradient(nrh_batch *b, const int *active) {
+    const FusedArgs *A = (const FusedArgs *)b->fusedArgs.p;
+    hipLaunchKernelGGL(k_nmi_table_batch, dim3((b->rb * b->fb + 255) / 256, b->P), dim3(256), 0, b->stream, A, active);
+    const bool half = b->pairs[0]->params.image_storage == 1;
+    if (half) {
+        hipLaunchKernelGGL((k_sweep_batch<__half, true>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
+                           A, b->sweepItems.p, b->nSweepItems, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, true>), dim3(kFixupBlocks), dim3(256), 0, b->stream, A, b->P,
+                           active);
+    } else {
+        hipLaunchKernelGGL((k_sweep_batch<float, true>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
+                           A, b->sweepItems.p, b->nSweepItems, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<float, true>), dim3(kFixupBlocks), dim3(256), 0, b->stream, A, b->P,
+                           active);
+    }
+    hipLaunchKernelGGL(k_project_x_batch, dim3(b->nProjX), dim3(256), b->projXLds, b->stream,
+                       (const ProjXArgs *)b->projXArgs.p, b->itProjX.p, active);
+    hipLaunchKernelGGL(k_project_y_batch, dim3(b->nProjY), dim3(256), 0, b->stream, (const ProjYArgs *)b->projYArgs.p,
+                       b->itProjY.p, active);
+    hipLaunchKernelGGL(k_project_nodes_batch, dim3(b->nNodes), dim3(256), 0, b->stream,
+                       (const NodeArgs *)b->nodeArgs.p, b->itNodes.p, active);
+    NRH_HIP(hipGetLastError());
 }
 
 }  // namespace nrh

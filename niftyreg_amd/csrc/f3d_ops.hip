@@ -9,6 +9,7 @@
 #include "m44.hpp"
 #include "host.hpp"
 #include "f3d_handle.hpp"
+#include "f3d_batch.hpp"
 #include <cmath>
 
 namespace nrh {
@@ -26,7 +27,7 @@ inline unsigned blocks_for(size_t n, int block = kBlock) {
 // reg_cubic_spline_getDeformationField3D, non-composition (reg-lib/cpu/_reg_localTrans.cpp:1080-1453)
 __global__ void __launch_bounds__(kBlock)
 k_spline_field(int nx, int ny, int nz, int gnx, int gny, size_t G, const float *__restrict__ cpp,
-               const int *__restrict__ mask, float gx, float gy, float gz, int lut,
+               const int *__restrict__ mask, float gx, float gy, float gz, int lut, int order,
                float *__restrict__ field, size_t N) {
 #pragma clang fp contract(off)
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -46,21 +47,10 @@ k_spline_field(int nx, int ny, int nz, int gnx, int gny, size_t G, const float *
     bspline_basis(tx, xb);
     bspline_basis(ty, yb);
     bspline_basis(tz, zb);
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-    for (int c = 0; c < 4; ++c)
-        for (int b = 0; b < 4; ++b) {
-            const float yzw = yb[b] * zb[c];
-            const size_t row = ((size_t)(pz + c) * gny + (py + b)) * gnx + px;
-            for (int a = 0; a < 4; ++a) {
-                const float w = xb[a] * yzw;
-                r0 += cpp[row + a] * w;
-                r1 += cpp[row + a + G] * w;
-                r2 += cpp[row + a + 2 * G] * w;
-            }
-        }
-    field[idx] = r0;
-    field[idx + N] = r1;
-    field[idx + 2 * N] = r2;
+    const float3 r = ref_spline_point(cpp, G, gnx, gny, px, py, pz, xb, yb, zb, order, lut != 0);
+    field[idx] = r.x;
+    field[idx + N] = r.y;
+    field[idx + 2 * N] = r.z;
 }
 
 // ------------------------------------------------------------------ resampling
@@ -85,43 +75,11 @@ k_image_gradient_linear(const float *__restrict__ field, size_t N, const int *__
 #pragma clang fp contract(off)
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= N) return;
-    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
-    if (!mask || mask[idx] > -1) {
-        const float3 p = affine_apply_d(ijk, field[idx], field[idx + N], field[idx + 2 * N]);
-        const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
-        float r = p.x - (float)px;
-        const float xb[2] = { (float)(1.0 - (double)r), r };
-        r = p.y - (float)py;
-        const float yb[2] = { (float)(1.0 - (double)r), r };
-        r = p.z - (float)pz;
-        const float zb[2] = { (float)(1.0 - (double)r), r };
-        const float deriv[2] = { -1.f, 1.f };
-        for (int c = 0; c < 2; ++c) {
-            const int Z = pz + c;
-            const bool zIn = Z > -1 && Z < fnz;
-            float xx = 0.f, yy = 0.f, zz = 0.f;
-            for (int b = 0; b < 2; ++b) {
-                const int Y = py + b;
-                const bool yzIn = zIn && Y > -1 && Y < fny;
-                float xT = 0.f, yT = 0.f;
-                for (int a = 0; a < 2; ++a) {
-                    const int X = px + a;
-                    const float coeff = (yzIn && X > -1 && X < fnx) ? flo[((size_t)Z * fny + Y) * fnx + X] : pad;
-                    xT += coeff * deriv[a];
-                    yT += coeff * xb[a];
-                }
-                xx += xT * yb[b];
-                yy += yT * deriv[b];
-                zz += yT * yb[b];
-            }
-            g0 += xx * zb[c];
-            g1 += yy * zb[c];
-            g2 += zz * deriv[c];
-        }
-        if (g0 != g0) g0 = 0.f;
-        if (g1 != g1) g1 = 0.f;
-        if (g2 != g2) g2 = 0.f;
-    }
+    float3 g = make_float3(0.f, 0.f, 0.f);
+    if (!mask || mask[idx] > -1)
+        g = image_gradient_at(flo, fnx, fny, fnz, affine_apply_d(ijk, field[idx], field[idx + N], field[idx + 2 * N]),
+                              pad);
+    const float g0 = g.x, g1 = g.y, g2 = g.z;
     grad[idx] = g0;
     grad[idx + N] = g1;
     grad[idx + 2 * N] = g2;
@@ -231,8 +189,8 @@ k_nmi_finalize(const uint32_t *__restrict__ counts, int rb, int fb, double *__re
 constexpr int kFinLds = 7936;
 constexpr int kFinPer = (kFinLds + 1023) / 1024;
 
-__global__ void __launch_bounds__(1024)
-k_nmi_finalize_lds(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+__device__ __forceinline__ void
+nmi_finalize_lds_block(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
                    double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent) {
     __shared__ double P[kFinLds];
     __shared__ double red[16];
@@ -343,40 +301,9 @@ k_nmi_voxel_gradient(const float *__restrict__ ref, const float *__restrict__ wa
     const float rv = ref[i], wv = war[i];
     if (!(rv == rv && wv == wv)) return;
     const float g[3] = { wg[i], wg[i + N], wg[i + 2 * N] };
-    const double nmi = (ent[0] + ent[1]) / ent[2];
-    const size_t refOff = (size_t)rb * fb, floOff = refOff + rb;
-    double jd[3] = { 0, 0, 0 }, rd[3] = { 0, 0, 0 }, wd[3] = { 0, 0, 0 };
-    // the reference's bin loops, r in [(int)(rv - 1), (int)(rv + 3)) and w likewise (at most four
-    // each); the Parzen values of each axis are evaluated once instead of once per pair
-    const int r0 = (int)(rv - 1.f), r1 = (int)(rv + 3.f), w0 = (int)(wv - 1.f), w1 = (int)(wv + 3.f);
-    double bR[4], dW[4], rl[4], wl[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int r = r0 + k, w = w0 + k;
-        bR[k] = nmi_basis((double)(rv - (float)r));
-        dW[k] = nmi_basis_deriv((double)(wv - (float)w));
-        rl[k] = (-1 < r && r < rb) ? logh[r + refOff] : 0.0;
-        wl[k] = (-1 < w && w < fb) ? logh[w + floOff] : 0.0;
-    }
-    for (int kr = 0; kr < 4 && r0 + kr < r1; ++kr) {
-        const int r = r0 + kr;
-        if (!(-1 < r && r < rb)) continue;
-        for (int kw = 0; kw < 4 && w0 + kw < w1; ++kw) {
-            const int w = w0 + kw;
-            if (!(-1 < w && w < fb)) continue;
-            const double common = bR[kr] * dW[kw];
-            const double jl = logh[r + w * rb];
-            for (int d = 0; d < 3; ++d) {
-                if (g[d] == g[d]) {
-                    rd[d] += common * (double)g[d] * rl[kr];
-                    wd[d] += common * (double)g[d] * wl[kw];
-                    jd[d] += common * (double)g[d] * jl;
-                }
-            }
-        }
-    }
-    const double den = ent[2] * ent[3];
-    for (int d = 0; d < 3; ++d) grad[i + d * N] += (float)(weight * (rd[d] + wd[d] - nmi * jd[d]) / den);
+    float d[3];
+    nmi_voxel_term(rv, wv, g, logh, ent, rb, fb, weight, d);
+    for (int k = 0; k < 3; ++k) grad[i + k * N] += d[k];
 }
 
 // ------------------------------------------------------------------ convolution
@@ -500,9 +427,9 @@ __device__ __forceinline__ void brick_origin(const BrickGrid& g, int b, int& x0,
 
 // reg_spline_approxBendingEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:73-153) over the
 // interior nodes of the brick; FP64 block partials, summed in block order by the last block.
-__global__ void __launch_bounds__(256)
-k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, BeBasis B,
-           double *__restrict__ partials, uint32_t *__restrict__ ticket, double scale, double *__restrict__ out) {
+__device__ __forceinline__ void
+be_value_block(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, BeBasis B,
+           double *__restrict__ partials, uint32_t *__restrict__ ticket, double scale, double *__restrict__ out, int bid, int nbk) {
 #pragma clang fp contract(off)
     constexpr int HX = BBX + 2, HY = BBY + 2, HZ = BBZ + 2, HN = HX * HY * HZ;
     __shared__ float sC[3][HN];  // deformation values of the brick and a one-node halo
@@ -511,7 +438,7 @@ k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid b
     __shared__ bool last;
     const size_t G = (size_t)gnx * gny * gnz;
     int x0, y0, z0;
-    brick_origin(bg, blockIdx.x, x0, y0, z0);
+    brick_origin(bg, bid, x0, y0, z0);
     if (threadIdx.x < 27 * 6) sB[threadIdx.x / 6][threadIdx.x % 6] = B.b[threadIdx.x % 6][threadIdx.x / 6];
     {
         constexpr int kIter = (3 * HN + 255) / 256;
@@ -567,15 +494,15 @@ k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid b
     s = block_sum_double(s, red);
     // the last block to finish sums the per-block partials in block order
     if (threadIdx.x == 0) {
-        partials[blockIdx.x] = s;
+        partials[bid] = s;
         __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+        last = atomicAdd(ticket, 1u) == nbk - 1;
     }
     __syncthreads();
     if (!last) return;
     __threadfence();
     double t = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) t += partials[i];
+    for (int i = threadIdx.x; i < (int)nbk; i += blockDim.x) t += partials[i];
     t = block_sum_double(t, red);
     if (threadIdx.x == 0) {
         *out = t * scale;
@@ -589,9 +516,9 @@ k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid b
 // the adjoint 27-point gather grad += weight / G * value (regul.cpp:381-448) for the brick's
 // nodes.  Optionally tracks the max length of the final gradient.  The reference's in-place
 // (p - node) + node round trip of the points is k_round_trip, run after this kernel.
-__global__ void __launch_bounds__(256)
-k_be_gradient(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, Affine xyz, BeBasis B,
-              float ratio, float *__restrict__ grad, uint32_t *__restrict__ maxBits) {
+__device__ __forceinline__ void
+be_gradient_block(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, Affine xyz, BeBasis B,
+              float ratio, float *__restrict__ grad, uint32_t *__restrict__ maxBits, int bid) {
 #pragma clang fp contract(off)
     constexpr int CX = BBX + 4, CY = BBY + 4, CZ = BBZ + 4, CN = CX * CY * CZ;  // points: two-node halo
     constexpr int DX = BBX + 2, DY = BBY + 2, DZ = BBZ + 2, DN = DX * DY * DZ;  // derivatives: one-node halo
@@ -600,7 +527,7 @@ k_be_gradient(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGri
     __shared__ float sB[27][8];  // basis values per neighbour (xx, yy, zz, xy, yz, xz), read as broadcasts
     const size_t G = (size_t)gnx * gny * gnz;
     int x0, y0, z0;
-    brick_origin(bg, blockIdx.x, x0, y0, z0);
+    brick_origin(bg, bid, x0, y0, z0);
     if (threadIdx.x < 27 * 6) sB[threadIdx.x / 6][threadIdx.x % 6] = B.b[threadIdx.x % 6][threadIdx.x / 6];
     {
         constexpr int kIter = (3 * CN + 255) / 256;
@@ -698,11 +625,11 @@ k_be_gradient(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGri
 
 // The in-place reg_getDisplacementFromDeformation / reg_getDeformationFromDisplacement pair the
 // reference wraps around the bending-energy gradient (regul.cpp:312, 450): p <- (p - node) + node.
-__global__ void __launch_bounds__(256)
-k_round_trip(float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz) {
+__device__ __forceinline__ void
+round_trip_block(float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz, int bid) {
 #pragma clang fp contract(off)
     const size_t G = (size_t)gnx * gny * gnz;
-    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)bid * blockDim.x + threadIdx.x;
     if (n >= G) return;
     const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
     for (int k = 0; k < 3; ++k) {
@@ -876,9 +803,9 @@ k_normalise(float *__restrict__ g, size_t n, double inv) {
 }
 
 // Normalisation by the max length held as float bits; block 0 also publishes it as a double.
-__global__ void __launch_bounds__(kBlock)
-k_normalise_bits(float *__restrict__ g, size_t n, const uint32_t *__restrict__ bits, double *__restrict__ maxLen) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void
+normalise_bits_block(float *__restrict__ g, size_t n, const uint32_t *__restrict__ bits, double *__restrict__ maxLen, int bid) {
+    const size_t i = (size_t)bid * blockDim.x + threadIdx.x;
     const double m = (double)__uint_as_float(*bits);
     if (i == 0) *maxLen = m;
     if (i < n && m != 0.0) g[i] = (float)((double)g[i] * (1.0 / m));
@@ -893,19 +820,19 @@ k_normalise_dev(float *__restrict__ g, size_t n, const double *__restrict__ maxL
 }
 
 // ConjugateGradient<float>::UpdateGradientValues (reg-lib/Optimiser.cpp:231-317)
-__global__ void __launch_bounds__(kBlock)
-k_cg_init(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void
+cg_init_block(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H, size_t n, int bid) {
+    const size_t i = (size_t)bid * blockDim.x + threadIdx.x;
     if (i < n) H[i] = G_[i] = -grad[i];
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_cg_dots(const float *__restrict__ grad, const float *__restrict__ G_, const float *__restrict__ H, size_t n,
-          double *__restrict__ partials) {
+__device__ __forceinline__ void
+cg_dots_block(const float *__restrict__ grad, const float *__restrict__ G_, const float *__restrict__ H, size_t n,
+          double *__restrict__ partials, int bid, int nbk) {
 #pragma clang fp contract(off)
     __shared__ double red[16];
     double gg = 0, dgg = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)bid * blockDim.x + threadIdx.x; i < n; i += (size_t)nbk * blockDim.x) {
         gg += (double)(H[i] * G_[i]);
         dgg += (double)((grad[i] + G_[i]) * grad[i]);
     }
@@ -913,13 +840,13 @@ k_cg_dots(const float *__restrict__ grad, const float *__restrict__ G_, const fl
     __syncthreads();
     dgg = block_sum_double(dgg, red);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = gg;
-        partials[2 * blockIdx.x + 1] = dgg;
+        partials[2 * bid] = gg;
+        partials[2 * bid + 1] = dgg;
     }
 }
 
-__global__ void __launch_bounds__(1024)
-k_cg_gamma(const double *__restrict__ partials, int nb, double *__restrict__ gam) {
+__device__ __forceinline__ void
+cg_gamma_block(const double *__restrict__ partials, int nb, double *__restrict__ gam) {
     __shared__ double red[16];
     double gg = 0, dgg = 0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
@@ -932,10 +859,10 @@ k_cg_gamma(const double *__restrict__ partials, int nb, double *__restrict__ gam
     if (threadIdx.x == 0) *gam = dgg / gg;
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_cg_apply(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H, size_t n, const double *__restrict__ gam) {
+__device__ __forceinline__ void
+cg_apply_block(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H, size_t n, const double *__restrict__ gam, int bid) {
 #pragma clang fp contract(off)
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = (size_t)bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double gm = *gam;
     G_[i] = -grad[i];
@@ -944,10 +871,10 @@ k_cg_apply(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__
 }
 
 // Compute::UpdateControlPointPosition (reg-lib/Compute.cpp:91-127)
-__global__ void __launch_bounds__(kBlock)
-k_update_dof(float *__restrict__ cur, const float *__restrict__ best, const float *__restrict__ grad, float scale, size_t n) {
+__device__ __forceinline__ void
+update_dof_block(float *__restrict__ cur, const float *__restrict__ best, const float *__restrict__ grad, float scale, size_t n, int bid) {
 #pragma clang fp contract(off)
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = (size_t)bid * blockDim.x + threadIdx.x;
     if (i < n) cur[i] = best[i] + scale * grad[i];
 }
 
@@ -1001,6 +928,190 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
 
 __global__ void k_fill_f32(float *p, size_t n, float v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------------------ single-pair kernels
+// Each wraps the block body above with the launch's own block index and count.
+__global__ void __launch_bounds__(1024)
+k_nmi_finalize_lds(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent) {
+    nmi_finalize_lds_block(counts, rb, fb, pro, logh, logf, ent);
+}
+
+__global__ void __launch_bounds__(256)
+k_be_value(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, BeBasis B,
+           double *__restrict__ partials, uint32_t *__restrict__ ticket, double scale, double *__restrict__ out) {
+    be_value_block(cpp, gnx, gny, gnz, bg, B, partials, ticket, scale, out, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(256)
+k_be_gradient(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, Affine xyz, BeBasis B,
+              float ratio, float *__restrict__ grad, uint32_t *__restrict__ maxBits) {
+    be_gradient_block(cpp, gnx, gny, gnz, bg, xyz, B, ratio, grad, maxBits, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_round_trip(float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz) {
+    round_trip_block(cpp, gnx, gny, gnz, xyz, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_normalise_bits(float *__restrict__ g, size_t n, const uint32_t *__restrict__ bits, double *__restrict__ maxLen) {
+    normalise_bits_block(g, n, bits, maxLen, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_init(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H,
+                                                  size_t n) {
+    cg_init_block(grad, G_, H, n, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_cg_dots(const float *__restrict__ grad, const float *__restrict__ G_, const float *__restrict__ H, size_t n,
+          double *__restrict__ partials) {
+    cg_dots_block(grad, G_, H, n, partials, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(1024) k_cg_gamma(const double *__restrict__ partials, int nb, double *__restrict__ gam) {
+    cg_gamma_block(partials, nb, gam);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_cg_apply(float *__restrict__ grad, float *__restrict__ G_, float *__restrict__ H, size_t n, const double *__restrict__ gam) {
+    cg_apply_block(grad, G_, H, n, gam, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_update_dof(float *__restrict__ cur, const float *__restrict__ best, const float *__restrict__ grad, float scale, size_t n) {
+    update_dof_block(cur, best, grad, scale, n, blockIdx.x);
+}
+
+// ------------------------------------------------------------------ batched kernels
+// A batch of pairs runs every small kernel of an evaluation in one launch: the blocks of all
+// pairs are listed in an item table (pair, block of that pair), and each block runs the same body
+// as the single-pair kernel with that pair's arguments, its own block index and count, so every
+// pair's arithmetic and reduction order are those of the pair alone.  Pairs whose flag in
+// `active` is 0 are skipped.
+struct PairOps {
+    float *cpp, *best, *grad, *cgG, *cgH;
+    int gnx, gny, gnz;
+    size_t G;
+    Affine xyz;
+    BrickGrid bg;
+    int beBlocks, cgBlocks;
+    float beRatio;         // bending-energy weight / G
+    double beScale;        // 1 / (3 G)
+    double *partials;      // bending-energy value and CG dot partials
+    uint32_t *beTicket;
+    double *beOut;         // bending energy (scalars[0])
+    uint32_t *maxBits;     // max gradient length as float bits
+    double *maxLen;        // scalars[1]
+    double *gam;           // scalars[2]
+    const uint32_t *counts;
+    double *pro, *logh;
+    float *logf;
+    double *ent;
+    int rb, fb;
+};
+
+__device__ __forceinline__ bool skip_pair(const int *active, int pair) { return active && !active[pair]; }
+
+__global__ void __launch_bounds__(1024) k_nmi_finalize_batch(const PairOps *__restrict__ P, const int *active) {
+    if (skip_pair(active, blockIdx.x)) return;
+    const PairOps& q = P[blockIdx.x];
+    nmi_finalize_lds_block(q.counts, q.rb, q.fb, q.pro, q.logh, q.logf, q.ent);
+}
+
+__global__ void __launch_bounds__(256)
+k_be_value_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, BeBasis B, const int *active) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x)) return;
+    const PairOps& q = P[it.x];
+    be_value_block(q.cpp, q.gnx, q.gny, q.gnz, q.bg, B, q.partials, q.beTicket, q.beScale, q.beOut, it.y, q.beBlocks);
+}
+
+__global__ void __launch_bounds__(256)
+k_be_gradient_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, BeBasis B, const int *active) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x)) return;
+    const PairOps& q = P[it.x];
+    be_gradient_block(q.cpp, q.gnx, q.gny, q.gnz, q.bg, q.xyz, B, q.beRatio, q.grad, q.maxBits, it.y);
+}
+
+__global__ void __launch_bounds__(256)
+k_round_trip_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, const int *active) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x)) return;
+    const PairOps& q = P[it.x];
+    round_trip_block(q.cpp, q.gnx, q.gny, q.gnz, q.xyz, it.y);
+}
+
+// items over 3 G: element-wise kernels of the control-point arrays
+__global__ void __launch_bounds__(kBlock)
+k_normalise_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, const int *active) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x)) return;
+    const PairOps& q = P[it.x];
+    normalise_bits_block(q.grad, 3 * q.G, q.maxBits, q.maxLen, it.y);
+}
+
+// first[pair]: 1 = steepest descent (reset G, H), 0 = conjugate update
+__global__ void __launch_bounds__(kBlock)
+k_cg_init_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, const int *active,
+                const int *__restrict__ first) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x) || !first[it.x]) return;
+    const PairOps& q = P[it.x];
+    cg_init_block(q.grad, q.cgG, q.cgH, 3 * q.G, it.y);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_cg_dots_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, const int *active,
+                const int *__restrict__ first) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x) || first[it.x]) return;
+    const PairOps& q = P[it.x];
+    cg_dots_block(q.grad, q.cgG, q.cgH, 3 * q.G, q.partials, it.y, q.cgBlocks);
+}
+
+__global__ void __launch_bounds__(1024)
+k_cg_gamma_batch(const PairOps *__restrict__ P, const int *active, const int *__restrict__ first) {
+    if (skip_pair(active, blockIdx.x) || first[blockIdx.x]) return;
+    const PairOps& q = P[blockIdx.x];
+    cg_gamma_block(q.partials, q.cgBlocks, q.gam);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_cg_apply_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, const int *active,
+                 const int *__restrict__ first) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x) || first[it.x]) return;
+    const PairOps& q = P[it.x];
+    cg_apply_block(q.grad, q.cgG, q.cgH, 3 * q.G, q.gam, it.y);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_update_dof_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, const int *active,
+                   const float *__restrict__ scales) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x)) return;
+    const PairOps& q = P[it.x];
+    update_dof_block(q.cpp, q.best, q.grad, scales[it.x], 3 * q.G, it.y);
+}
+
+// toBest: best <- cpp (Optimiser::StoreCurrentDof), else cpp <- best (RestoreBestDof)
+__global__ void __launch_bounds__(kBlock)
+k_copy_dof_batch(const PairOps *__restrict__ P, const int2 *__restrict__ items, const int *active, int toBest) {
+    const int2 it = items[blockIdx.x];
+    if (skip_pair(active, it.x)) return;
+    const PairOps& q = P[it.x];
+    const size_t i = (size_t)it.y * blockDim.x + threadIdx.x;
+    if (i >= 3 * q.G) return;
+    if (toBest) q.best[i] = q.cpp[i];
+    else q.cpp[i] = q.best[i];
+}
+
+__global__ void k_zero_max_bits_batch(const PairOps *__restrict__ P, int n, const int *active) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !skip_pair(active, i)) *P[i].maxBits = 0u;
 }
 
 // Round to the 6-decimal literals the reference tabulates (reg-lib/cpu/_reg_splineBasis.cpp:292-457).
@@ -1064,7 +1175,7 @@ void op_spline_field(nrh_f3d *h) {
     h->pairStateVersion = 0;
     hipLaunchKernelGGL(k_spline_field, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, h->ref.nx, h->ref.ny,
                        h->ref.nz, h->cpp.nx, h->cpp.ny, h->G, h->cppPos.p, h->mask.p, g[0], g[1], g[2], lut,
-                       h->field.p, h->N);
+                       h->params.field_order, h->field.p, h->N);
     NRH_HIP(hipGetLastError());
 }
 
@@ -1470,6 +1581,159 @@ double read_scalar(nrh_f3d *h, const double *dev) {
     op_publish(h, dev, 1);
     NRH_HIP(hipStreamSynchronize(h->stream));
     return h->published[0];
+}
+
+// ------------------------------------------------------------------ batches of pairs
+namespace {
+
+// The objective terms of each active pair into results[8 q ..]: NMI, H_r, H_f, H_rf, total, the
+// bending energy, the weighted value (reg_f3d::GetObjectiveFunctionValue with the linear energy
+// off), and the current max gradient length; mirrored into coherent host memory.
+__global__ void k_batch_values(const PairOps *__restrict__ P, int n, const int *active, const float *__restrict__ w,
+                               double *__restrict__ res, double *__restrict__ host) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n || skip_pair(active, q)) return;
+    const PairOps& o = P[q];
+    const double simW = (double)w[2 * q], beW = (double)w[2 * q + 1];
+    double r[kBatchResults];
+    r[0] = o.ent[4];
+    for (int k = 0; k < 4; ++k) r[1 + k] = o.ent[k];
+    r[5] = beW > 0 ? *o.beOut : 0.0;
+    const double wMeasure = simW > 0 ? simW * r[0] : 0.0;
+    const double wBE = beW > 0 ? beW * r[5] : 0.0;
+    r[6] = wMeasure - wBE;
+    r[7] = *o.maxLen;
+    for (int k = 0; k < kBatchResults; ++k) {
+        res[q * kBatchResults + k] = r[k];
+        host[q * kBatchResults + k] = r[k];
+    }
+}
+
+}  // namespace
+
+void batch_ops_setup(nrh_batch *b) {
+    std::vector<PairOps> ops(b->P);
+    std::vector<int2> itBE, itG, itG3, itCg;
+    std::vector<float> w(2 * b->P);
+    for (int q = 0; q < b->P; ++q) {
+        nrh_f3d *h = b->pairs[q];
+        NRH_REQUIRE(h->params.measure == NRH_MEASURE_NMI && h->params.le_weight == 0.f && h->mask.empty(),
+                    NRH_ERR_UNSUPPORTED, "batched pairs use NMI, no linear energy and no mask");
+        NRH_REQUIRE(h->totalBins() <= kFinLds, NRH_ERR_UNSUPPORTED, "too many histogram bins for a batch");
+        const BrickGrid bg = brick_grid(h);
+        const int nb = bg.bx * bg.by * bg.bz;
+        const size_t n3 = 3 * h->G;
+        const int cgBlocks = (int)std::min(blocks_for(n3), 512u);
+        h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)std::max(nb, cgBlocks) + 64));
+        PairOps o{};
+        o.cpp = h->cppPos.p;
+        o.best = h->best.p;
+        o.grad = h->nodeGrad.p;
+        o.cgG = h->cgG.p;
+        o.cgH = h->cgH.p;
+        o.gnx = h->cpp.nx;
+        o.gny = h->cpp.ny;
+        o.gnz = h->cpp.nz;
+        o.G = h->G;
+        o.xyz = to_affine(h->cpp.xyz);
+        o.bg = bg;
+        o.beBlocks = nb;
+        o.cgBlocks = cgBlocks;
+        o.beRatio = h->params.be_weight / (float)h->G;
+        o.beScale = 1.0 / (double)(3 * h->G);
+        o.partials = h->partials.p;
+        o.beTicket = h->maxBits.p + 6;
+        o.beOut = h->scalars.p + 0;
+        o.maxBits = h->maxBits.p;
+        o.maxLen = h->scalars.p + 1;
+        o.gam = h->scalars.p + 2;
+        o.counts = h->histCount.p;
+        o.pro = h->histPro.p;
+        o.logh = h->histLog.p;
+        o.logf = h->histLogF.p;
+        o.ent = h->ent.p;
+        o.rb = h->params.ref_bins;
+        o.fb = h->params.flo_bins;
+        ops[q] = o;
+        w[2 * q] = h->params.similarity_weight;
+        w[2 * q + 1] = h->params.be_weight;
+        for (int k = 0; k < nb; ++k) itBE.push_back(make_int2(q, k));
+        for (unsigned k = 0; k < blocks_for(h->G); ++k) itG.push_back(make_int2(q, (int)k));
+        for (unsigned k = 0; k < blocks_for(n3); ++k) itG3.push_back(make_int2(q, (int)k));
+        for (int k = 0; k < cgBlocks; ++k) itCg.push_back(make_int2(q, k));
+    }
+    b->pairOps.alloc(ops.size() * sizeof(PairOps));
+    NRH_HIP(hipMemcpyAsync(b->pairOps.p, ops.data(), ops.size() * sizeof(PairOps), hipMemcpyHostToDevice, b->stream));
+    b->weights.alloc(w.size());
+    NRH_HIP(hipMemcpyAsync(b->weights.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    auto up = [&](DevBuf<int2>& dst, const std::vector<int2>& v, int& n) {
+        dst.alloc(v.size());
+        NRH_HIP(hipMemcpyAsync(dst.p, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+        n = (int)v.size();
+    };
+    up(b->itBE, itBE, b->nBE);
+    up(b->itG, itG, b->nG);
+    up(b->itG3, itG3, b->nG3);
+    up(b->itCg, itCg, b->nCg);
+    b->results.alloc((size_t)kBatchResults * b->P);
+    NRH_HIP(hipMemsetAsync(b->results.p, 0, b->results.bytes(), b->stream));
+    NRH_HIP(hipStreamSynchronize(b->stream));
+}
+
+void batch_finalize(nrh_batch *b, const int *active) {
+    hipLaunchKernelGGL(k_nmi_finalize_batch, dim3(b->P), dim3(1024), 0, b->stream, (const PairOps *)b->pairOps.p,
+                       active);
+    NRH_HIP(hipGetLastError());
+}
+
+void batch_be_value(nrh_batch *b, const int *active) {
+    static const BeBasis B = make_be_basis();
+    hipLaunchKernelGGL(k_be_value_batch, dim3(b->nBE), dim3(256), 0, b->stream, (const PairOps *)b->pairOps.p,
+                       b->itBE.p, B, active);
+    NRH_HIP(hipGetLastError());
+}
+
+void batch_be_gradient(nrh_batch *b, const int *active) {
+    static const BeBasis B = make_be_basis();
+    const PairOps *P = (const PairOps *)b->pairOps.p;
+    hipLaunchKernelGGL(k_zero_max_bits_batch, dim3((b->P + 255) / 256), dim3(256), 0, b->stream, P, b->P, active);
+    hipLaunchKernelGGL(k_be_gradient_batch, dim3(b->nBE), dim3(256), 0, b->stream, P, b->itBE.p, B, active);
+    hipLaunchKernelGGL(k_round_trip_batch, dim3(b->nG), dim3(256), 0, b->stream, P, b->itG.p, active);
+    NRH_HIP(hipGetLastError());
+}
+
+void batch_normalise(nrh_batch *b, const int *active) {
+    hipLaunchKernelGGL(k_normalise_batch, dim3(b->nG3), dim3(kBlock), 0, b->stream, (const PairOps *)b->pairOps.p,
+                       b->itG3.p, active);
+    NRH_HIP(hipGetLastError());
+}
+
+void batch_cg(nrh_batch *b, const int *active, const int *first) {
+    const PairOps *P = (const PairOps *)b->pairOps.p;
+    hipLaunchKernelGGL(k_cg_init_batch, dim3(b->nG3), dim3(kBlock), 0, b->stream, P, b->itG3.p, active, first);
+    hipLaunchKernelGGL(k_cg_dots_batch, dim3(b->nCg), dim3(kBlock), 0, b->stream, P, b->itCg.p, active, first);
+    hipLaunchKernelGGL(k_cg_gamma_batch, dim3(b->P), dim3(1024), 0, b->stream, P, active, first);
+    hipLaunchKernelGGL(k_cg_apply_batch, dim3(b->nG3), dim3(kBlock), 0, b->stream, P, b->itG3.p, active, first);
+    NRH_HIP(hipGetLastError());
+}
+
+void batch_update(nrh_batch *b, const int *active, const float *scales) {
+    hipLaunchKernelGGL(k_update_dof_batch, dim3(b->nG3), dim3(kBlock), 0, b->stream, (const PairOps *)b->pairOps.p,
+                       b->itG3.p, active, scales);
+    NRH_HIP(hipGetLastError());
+}
+
+void batch_copy_dof(nrh_batch *b, const int *active, bool toBest) {
+    hipLaunchKernelGGL(k_copy_dof_batch, dim3(b->nG3), dim3(kBlock), 0, b->stream, (const PairOps *)b->pairOps.p,
+                       b->itG3.p, active, toBest ? 1 : 0);
+    NRH_HIP(hipGetLastError());
+}
+
+void batch_values(nrh_batch *b, const int *active) {
+    hipLaunchKernelGGL(k_batch_values, dim3((b->P + 63) / 64), dim3(64), 0, b->stream, (const PairOps *)b->pairOps.p,
+                       b->P, active, b->weights.p, b->results.p, b->publishedDev);
+    NRH_HIP(hipGetLastError());
 }
 
 }  // namespace nrh

@@ -348,14 +348,24 @@ int nrh_nifti_read_data(const char *path, float *data, size_t capacity) {
         const Nifti1Header h = read_raw_header(path, gz, swapped);
         nrh_nifti_info info;
         header_to_info(h, &info);
+        // the voxel count and byte size come from the file: multiply with overflow checks and
+        // reject headers beyond 2^40 bytes of voxel data
         size_t n = 1;
-        for (int i = 1; i <= 7; ++i) n *= (size_t)info.dim[i];
+        for (int i = 1; i <= 7; ++i) {
+            const int d = (i > info.dim[0] && info.dim[i] == 0) ? 1 : info.dim[i];  // unused trailing axes
+            NRH_REQUIRE(d >= 1, NRH_ERR_ARG, "bad image dimension in the header");
+            NRH_REQUIRE(!__builtin_mul_overflow(n, (size_t)d, &n), NRH_ERR_ARG,
+                        "image dimensions overflow the voxel count");
+        }
         NRH_REQUIRE(capacity >= n, NRH_ERR_ARG, "output buffer smaller than the image");
         const int nb = bytes_of(h.datatype);
+        size_t rawBytes = 0;
+        NRH_REQUIRE(nb > 0 && !__builtin_mul_overflow(n, (size_t)nb, &rawBytes) && rawBytes <= ((size_t)1 << 40),
+                    NRH_ERR_ARG, "image data size out of range");
         const long skip = (long)h.vox_offset - (long)sizeof(h);
         NRH_REQUIRE(skip >= 0, NRH_ERR_ARG, "bad vox_offset");
         if (skip > 0) NRH_REQUIRE(gzseek(gz.f, skip, SEEK_CUR) >= 0, NRH_ERR_ARG, "cannot seek to the voxel data");
-        std::vector<unsigned char> raw(n * nb);
+        std::vector<unsigned char> raw(rawBytes);
         size_t got = 0;
         while (got < raw.size()) {
             const unsigned chunk = (unsigned)std::min<size_t>(raw.size() - got, 1u << 30);

@@ -14,8 +14,12 @@ from .geometry import Geom
 
 
 def default_params(bins=64, padding=float("nan"), similarity_weight=0.999, be_weight=0.001, image_storage=0,
-                   measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, le_weight=0.0):
+                   measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, le_weight=0.0, field_order=_lib.FIELD_ORDER_SCALAR):
+    """Handle parameters.  field_order: the reference build whose float-field rounding decides
+    floors near voxel planes (FIELD_ORDER_SCALAR: USE_SSE off, the order of the oracle's
+    restatement; FIELD_ORDER_SSE: the default x86 build)."""
     p = _lib.nrh_f3d_params()
+    p.field_order = field_order
     p.measure = measure
     p.lncc_sigma = lncc_sigma
     p.ref_bins = bins + 4          # the reference adds 4 bins for the spline support
@@ -293,10 +297,12 @@ class F3dContent:
 
 def register(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3,
              levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True, image_storage=0,
-             measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, affine=None):
+             measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, affine=None, field_order=_lib.FIELD_ORDER_SSE):
     """Full reg_f3d registration of one pair on the device (reg_base<float>::Run).
 
     affine: optional reference -> floating 4x4 initialisation (-aff).
+    field_order: the reference build whose field rounding is reproduced (default: the SSE build,
+    the reference's default on x86).
     Returns (grid Geom, control points planar float32, final similarity value, iterations)."""
     cfg = _lib.nrh_f3d_config()
     aff = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
@@ -305,6 +311,7 @@ def register(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.0
     cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
     cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), image_storage
     cfg.measure, cfg.lncc_sigma = measure, lncc_sigma
+    cfg.field_order = field_order
     ref = np.ascontiguousarray(ref, np.float32).reshape(-1)
     flo = np.ascontiguousarray(flo, np.float32).reshape(-1)
     rg, fg = ref_geom.to_c(), flo_geom.to_c()
@@ -319,6 +326,98 @@ def register(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.0
               ctypes.byref(out), _lib.fptr(cpp), ctypes.c_size_t(cap), ctypes.byref(nmi), ctypes.byref(its))
     g = Geom.from_c(out)
     return g, cpp[:3 * g.nvox].copy(), nmi.value, its.value
+
+
+class F3dBatch:
+    """Many F3dContent handles evaluated by the same launches (nrh_batch_*).  The handles stay
+    owned by the caller; while the batch exists their work runs on the batch's stream."""
+
+    def __init__(self, contents):
+        self.contents = list(contents)
+        arr = (ctypes.c_void_p * len(self.contents))(*[c.h.value if isinstance(c.h, ctypes.c_void_p) else c.h
+                                                       for c in self.contents])
+        b = ctypes.c_void_p()
+        _lib.call("nrh_batch_create", arr, len(self.contents), ctypes.byref(b))
+        self.b = b
+
+    def objective(self):
+        """(values, nmi) of every pair."""
+        n = len(self.contents)
+        v, m = (ctypes.c_double * n)(), (ctypes.c_double * n)()
+        _lib.call("nrh_batch_objective", self.b, v, m)
+        return np.array(v[:]), np.array(m[:])
+
+    def gradient(self, normalise=False):
+        """Node gradients land in each handle (node_gradient()); returns the max lengths when
+        normalising."""
+        n = len(self.contents)
+        m = (ctypes.c_double * n)()
+        _lib.call("nrh_batch_gradient", self.b, m if normalise else None)
+        return np.array(m[:]) if normalise else None
+
+    def set_sweep_blocks(self, blocks_per_xcd):
+        _lib.call("nrh_batch_set_sweep_blocks", self.b, int(blocks_per_xcd))
+
+    def autotune(self):
+        k = ctypes.c_int()
+        ms = (ctypes.c_double * 4)()
+        _lib.call("nrh_batch_autotune", self.b, ctypes.byref(k), ms)
+        return k.value, list(ms)
+
+    def close(self):
+        if self.b:
+            _lib.call("nrh_batch_destroy", self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def register_batch(pairs, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3, levels_to_perform=0, maxit=150,
+                   padding=float("nan"), use_pyramid=True, image_storage=0, field_order=_lib.FIELD_ORDER_SSE):
+    """reg_f3d registrations of many pairs at once (nrh_f3d_register_batch).
+
+    pairs: list of (ref_geom, ref, flo_geom, flo).  Returns ([(grid Geom, control points, final
+    NMI, iterations)] per pair, stats dict)."""
+    n = len(pairs)
+    cfg = _lib.nrh_f3d_config()
+    cfg.affine = None
+    cfg.bins, cfg.be_weight, cfg.le_weight, cfg.spacing = bins, be, le, spacing
+    cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
+    cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), image_storage
+    cfg.measure, cfg.lncc_sigma = _lib.MEASURE_NMI, -5.0
+    cfg.field_order = field_order
+    refs = (_lib.nrh_geom * n)(*[p[0].to_c() for p in pairs])
+    flos = (_lib.nrh_geom * n)(*[p[2].to_c() for p in pairs])
+    keep = [(np.ascontiguousarray(p[1], np.float32).reshape(-1), np.ascontiguousarray(p[3], np.float32).reshape(-1))
+            for p in pairs]
+    rdata = (ctypes.POINTER(ctypes.c_float) * n)(*[_lib.fptr(k[0]) for k in keep])
+    fdata = (ctypes.POINTER(ctypes.c_float) * n)(*[_lib.fptr(k[1]) for k in keep])
+    caps, outs = [], []
+    for p in pairs:
+        g = p[0]
+        cap = 3 * int(np.prod([int(np.ceil(nv * d / (abs(spacing) * d if spacing < 0 else spacing) + 4))
+                               for nv, d in zip(g.dims, g.pixdim)]))
+        caps.append(cap)
+        outs.append(np.zeros(cap, np.float32))
+    cpp_out = (ctypes.POINTER(ctypes.c_float) * n)(*[_lib.fptr(o) for o in outs])
+    capacity = (ctypes.c_size_t * n)(*caps)
+    grids = (_lib.nrh_geom * n)()
+    nmi = (ctypes.c_double * n)()
+    its = (ctypes.c_int * n)()
+    stats = _lib.nrh_batch_stats()
+    _lib.call("nrh_f3d_register_batch", n, refs, rdata, flos, fdata, ctypes.byref(cfg), grids, cpp_out, capacity, nmi,
+              its, ctypes.byref(stats))
+    res = []
+    for q in range(n):
+        g = Geom.from_c(grids[q])
+        res.append((g, outs[q][:3 * g.nvox].copy(), nmi[q], its[q]))
+    st = {"seconds": stats.seconds, "device_seconds": stats.device_seconds, "graph_replays": stats.graph_replays,
+          "blocks_per_xcd": list(stats.blocks_per_xcd[:levels]), "tile": list(stats.tile)}
+    return res, st
 
 
 class F3d2Pair:
@@ -350,7 +449,7 @@ class F3d2Pair:
 
 def register_symmetric(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.001, le=0.01, spacing=-5.0,
                        levels=3, levels_to_perform=0, maxit=150, padding=float("nan"), use_pyramid=True,
-                       measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, affine=None):
+                       measure=_lib.MEASURE_NMI, lncc_sigma=-5.0, affine=None, field_order=_lib.FIELD_ORDER_SSE):
     """Full reg_f3d2 registration of one pair on the device (symmetric velocity grids, NMI or LNCC).
 
     affine: optional reference -> floating 4x4 (-aff), split into half affines on both grids.
@@ -361,6 +460,7 @@ def register_symmetric(ref_geom: Geom, ref, flo_geom: Geom, flo, bins=64, be=0.0
     cfg.levels, cfg.levels_to_perform, cfg.max_iterations = levels, levels_to_perform, maxit
     cfg.padding, cfg.use_pyramid, cfg.image_storage = padding, int(use_pyramid), 0
     cfg.measure, cfg.lncc_sigma = measure, lncc_sigma
+    cfg.field_order = field_order
     aff = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
     cfg.affine = None if aff is None else _lib.fptr(aff)
     ref = np.ascontiguousarray(ref, np.float32).reshape(-1)

@@ -59,6 +59,12 @@ HipF3dContent::HipF3dContent(NiftiImage& referenceIn, NiftiImage& floatingIn, Ni
     params.image_storage = 0;
     params.measure = NRH_MEASURE_NMI;
     params.lncc_sigma = -5.f;
+    // the field rounding of the build this plugin is compiled into
+#ifdef USE_SSE
+    params.field_order = NRH_FIELD_ORDER_SSE;
+#else
+    params.field_order = NRH_FIELD_ORDER_SCALAR;
+#endif
     const nrh_geom r = GeomOf(reference), f = GeomOf(floating), c = GeomOf(controlPointGrid);
     Check(nrh_f3d_create(&r, static_cast<const float*>(reference->data), &f, static_cast<const float*>(floating->data),
                          referenceMask, &c, &params, &handle), "nrh_f3d_create");

@@ -512,6 +512,11 @@ public:
     void Prepare(int level) {
         this->Initialise();
         this->maxIterationNumber *= 1;
+        // the grid of `level` is the coarsest grid refined once per level before it, as in Run()
+        for (int l = 0; l < level; ++l) {
+            this->InitCurrentLevel(l);
+            this->DeinitCurrentLevel(l);
+        }
         this->InitCurrentLevel(level);
         this->InitialiseSimilarity();
         this->SetOptimiser();
@@ -689,6 +694,10 @@ public:
     RefF3d2(): reg_f3d2<float>(1, 1) {}
     void Prepare(int level) {
         this->Initialise();
+        for (int l = 0; l < level; ++l) {
+            this->InitCurrentLevel(l);
+            this->DeinitCurrentLevel(l);
+        }
         this->InitCurrentLevel(level);
         this->InitialiseSimilarity();
         this->SetOptimiser();

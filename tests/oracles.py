@@ -287,16 +287,18 @@ def ref_affine_field(refg, affine, mask=None):
 
 
 class RefF3d:
-    """reg_f3d<float> prepared at one level for single objective / gradient evaluations."""
+    """reg_f3d<float> prepared at one level for single objective / gradient evaluations.
+    scalar=True uses the reference built without USE_SSE, else the default SSE build."""
 
     def __init__(self, refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=1,
-                 level=0, maxit=150, padding=float("nan"), measure=0, lncc_sigma=-5.0):
+                 level=0, maxit=150, padding=float("nan"), measure=0, lncc_sigma=-5.0, scalar=False):
         p = ref_f3d_params()
         p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
         p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
         p.measure, p.lncc_sigma = measure, lncc_sigma
         self.p = p
-        r = ref()
+        self.lib = r = ref_scalar() if scalar else ref()
+        r.ref_f3d_create.restype = ctypes.c_void_p
         r.ref_f3d_create.argtypes = [ctypes.c_void_p] * 6
         h = r.ref_f3d_create(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
                              ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
@@ -305,46 +307,50 @@ class RefF3d:
             raise RuntimeError(r.ref_last_error().decode())
         self.h = ctypes.c_void_p(h)
         cg, rgm, fgm = nr_geom(), nr_geom(), nr_geom()
-        rcheck(r.ref_f3d_level_geom(self.h, ctypes.byref(cg), ctypes.byref(rgm), ctypes.byref(fgm)))
+        self._check(r.ref_f3d_level_geom(self.h, ctypes.byref(cg), ctypes.byref(rgm), ctypes.byref(fgm)))
         self.cpp_geom, self.ref_geom, self.flo_geom = cg, rgm, fgm
         self.G = cg.nx * cg.ny * cg.nz
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.ref_last_error().decode())
 
     def images(self):
         nr_ = self.ref_geom.nx * self.ref_geom.ny * self.ref_geom.nz
         nf = self.flo_geom.nx * self.flo_geom.ny * self.flo_geom.nz
         a, b = np.zeros(nr_, np.float32), np.zeros(nf, np.float32)
-        rcheck(ref().ref_f3d_level_images(self.h, fp(a), fp(b)))
+        self._check(self.lib.ref_f3d_level_images(self.h, fp(a), fp(b)))
         return a, b
 
     def weights(self, with_le=False):
         """(similarity, bending-energy[, linear-energy]) weights of the prepared level."""
         w = np.zeros(3, np.float32)
-        rcheck(ref().ref_f3d_weights(self.h, fp(w)))
+        self._check(self.lib.ref_f3d_weights(self.h, fp(w)))
         return (float(w[0]), float(w[1]), float(w[2])) if with_le else (float(w[0]), float(w[1]))
 
     def get_cpp(self):
         c = np.zeros(3 * self.G, np.float32)
-        rcheck(ref().ref_f3d_get_cpp(self.h, fp(c)))
+        self._check(self.lib.ref_f3d_get_cpp(self.h, fp(c)))
         return c
 
     def set_cpp(self, c):
-        rcheck(ref().ref_f3d_set_cpp(self.h, fp(np.ascontiguousarray(c, np.float32))))
+        self._check(self.lib.ref_f3d_set_cpp(self.h, fp(np.ascontiguousarray(c, np.float32))))
 
     def value(self):
         v = ctypes.c_double()
         t = np.zeros(3, np.float64)
-        rcheck(ref().ref_f3d_value(self.h, ctypes.byref(v), dp(t)))
+        self._check(self.lib.ref_f3d_value(self.h, ctypes.byref(v), dp(t)))
         return v.value, t
 
     def gradient(self, normalise=False):
         g = np.zeros(3 * self.G, np.float32)
         m = ctypes.c_double()
-        rcheck(ref().ref_f3d_gradient(self.h, int(normalise), ctypes.byref(m), fp(g), None))
+        self._check(self.lib.ref_f3d_gradient(self.h, int(normalise), ctypes.byref(m), fp(g), None))
         return g, m.value
 
     def close(self):
         if self.h:
-            ref().ref_f3d_destroy(self.h)
+            self.lib.ref_f3d_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -388,11 +394,13 @@ def fp64_field(refg, cppg, cpp):
 
 
 def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3, maxit=150,
-                padding=float("nan"), measure=0, lncc_sigma=-5.0, affine=None):
-    """reg_f3d<float>::Run on the reference CPU build; returns (grid geom, control points)."""
+                padding=float("nan"), measure=0, lncc_sigma=-5.0, affine=None, scalar=False, levels_to_perform=0):
+    """reg_f3d<float>::Run on the reference CPU build (scalar=True: the build without USE_SSE);
+    returns (grid geom, control points)."""
     p = ref_f3d_params()
     p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
-    p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
+    p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = (levels, levels_to_perform or levels, maxit, 1,
+                                                                        padding)
     p.measure, p.lncc_sigma = measure, lncc_sigma
     aff = None if affine is None else np.ascontiguousarray(affine, np.float32).reshape(16)
     p.affine = None if aff is None else fp(aff)
@@ -400,7 +408,8 @@ def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing
     sp = [abs(spacing) * d if spacing < 0 else spacing for d in refg.pixdim]
     cap = 3 * int(np.prod([int(np.ceil(n * d / s + 4)) for n, d, s in zip(refg.dims, refg.pixdim, sp)]))
     buf = np.zeros(cap, np.float32)
-    rcheck(ref().ref_f3d_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
+    lib = ref_scalar() if scalar else ref()
+    _scheck(lib, lib.ref_f3d_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
                              ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
                              ctypes.byref(p), ctypes.byref(out), fp(buf), cap))
     n = 3 * out.nx * out.ny * out.nz
@@ -418,6 +427,37 @@ def nmi_at(refg, flog, cppg, cpp, ref_img, flo_img, bins=64, fp64=False):
     field = fp64_field(refg, cppg, cpp).astype(np.float32) if fp64 else orc_field(refg, cppg, cpp)
     war = orc_resample(flog, f, refg, field)
     return orc_nmi(r, war, rb, rb)[0]
+
+
+def ref_nmi_at(refg, ref_img, flog, flo_img, cpp, bins=64, spacing=-5.0, levels=1, scalar=False):
+    """NMI of reg_f3d<float> itself (the reference build, no penalty terms) at the control points
+    cpp of the finest level of a `levels`-level run: the reference's objective as a registration
+    score."""
+    r = RefF3d(refg, ref_img, flog, flo_img, bins=bins, be=0.0, le=0.0, spacing=spacing, levels=levels,
+               level=levels - 1, scalar=scalar)
+    try:
+        assert 3 * r.G == np.asarray(cpp).size, "grid of another geometry"
+        r.set_cpp(cpp)
+        _, terms = r.value()
+        return terms[0] / r.weights()[0]
+    finally:
+        r.close()
+
+
+def ref_nmi_sym_at(refg, ref_img, flog, flo_img, fw, bw, bins=64, spacing=-5.0, levels=1, affine=None,
+                   scalar=False):
+    """Summed forward / backward NMI of reg_f3d2<float> itself (no penalty terms) at the velocity
+    grids fw, bw of the finest level of a `levels`-level run, with the input affine's half affines
+    on the grids when given."""
+    r = RefF3d2(refg, ref_img, flog, flo_img, bins=bins, be=0.0, le=0.0, spacing=spacing, levels=levels,
+                level=levels - 1, scalar=scalar, affine=affine)
+    try:
+        assert 3 * r.G == np.asarray(fw).size, "grid of another geometry"
+        r.set_cpp(fw, bw)
+        _, terms = r.value()
+        return terms[0] / r.weights()[0]
+    finally:
+        r.close()
 
 
 # ---------------------------------------------------------------- Aladin
@@ -906,8 +946,9 @@ def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, s
 
 
 def ref_f3d2_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.01, spacing=-5.0, levels=3, maxit=150,
-                 padding=float("nan"), affine=None):
-    """reg_f3d2<float>'s run on the reference CPU build; returns (grid geom, fw, bw, steps)."""
+                 padding=float("nan"), affine=None, scalar=False):
+    """reg_f3d2<float>'s run on the reference CPU build (scalar=True: the build without USE_SSE);
+    returns (grid geom, fw, bw, steps)."""
     p = ref_f3d_params()
     p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
     p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
@@ -919,9 +960,10 @@ def ref_f3d2_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.01, spaci
     fw = np.zeros(cap, np.float32)
     bw = np.zeros(cap, np.float32)
     steps = np.zeros(2, np.int32)
-    rcheck(ref().ref_f3d2_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
-                              ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
-                              ctypes.byref(p), ctypes.byref(out), fp(fw), fp(bw), cap, ip(steps)))
+    lib = ref_scalar() if scalar else ref()
+    _scheck(lib, lib.ref_f3d2_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
+                                  ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
+                                  ctypes.byref(p), ctypes.byref(out), fp(fw), fp(bw), cap, ip(steps)))
     n = 3 * out.nx * out.ny * out.nz
     return geom_from_nr(out), fw[:n].copy(), bw[:n].copy(), (int(steps[0]), int(steps[1]))
 

@@ -1,12 +1,11 @@
 """Fused objective / gradient sweeps against the oracle's reference-order chain.
 
 Tolerances (stated for this path): deformation 1e-4 voxel, NMI relative 1e-5, node gradient
-relative 1e-3 in the L2 norm.  The reference's gradient is discontinuous wherever a warped
-position crosses a floating-voxel boundary (the trilinear image gradient jumps there), so two
-implementations that round the field differently disagree at those isolated voxels: the
-reference's own SSE and scalar builds differ by 3.0e-2 in max-abs-relative and 2.0e-3 in
-L2-relative on the 256x256x160 node gradient.  The element-wise check is therefore bounded at
-that measured spread (GRAD_MAX_RTOL), and the full-size L2 bound at 1.5x the spread.
+relative 1e-3 in the L2 norm and 2e-3 of the largest component element-wise.  The reference's
+gradient jumps wherever a warped position crosses a floating-voxel plane (the trilinear image
+gradient changes cell there); the sweeps evaluate every voxel within rounding of such a plane
+exactly as the reference build does (the exact pass), so the remaining differences come from
+float rounding elsewhere (measured: 1e-5 to 3e-4 L2, at most 7e-4 element-wise).
 At the full size the fused sweeps are compared with the per-operation HIP chain (bit-exact to
 the oracle on the smaller cases) and checked for run-to-run determinism.
 """
@@ -23,8 +22,8 @@ pytestmark = pytest.mark.gpu
 RB = FB = 68
 NMI_RTOL = 1e-5
 GRAD_RTOL = 1e-3          # L2-relative
-GRAD_MAX_RTOL = 3e-2      # max-abs, relative to the largest component
-FULL_GRAD_RTOL = 3e-3     # L2-relative at 256x256x160
+GRAD_MAX_RTOL = 2e-3      # max-abs, relative to the largest component
+FULL_GRAD_RTOL = GRAD_RTOL
 FIELD_TOL = 1e-4          # voxel
 
 
@@ -90,6 +89,7 @@ def test_fused_gradient_matches_oracle(fcase):
     scale = np.abs(want).max()
     assert scale > 0
     err = np.abs(got - want).max()
+    print(f"fused vs oracle: L2 {l2rel(got, want):.2e}, max-abs {err / scale:.2e}")
     assert l2rel(got, want) <= GRAD_RTOL
     assert err <= GRAD_MAX_RTOL * scale, f"max error {err} vs scale {scale}"
 
@@ -195,8 +195,10 @@ def test_full_size_fused_vs_per_operation(full_size):
     h.ApproxBendingEnergyGradient(np.float32(0.001))
     g_ops = h.node_gradient()
     scale = np.abs(g_ops).max()
+    print(f"full size fused vs per-operation: L2 {l2rel(g_fused, g_ops):.2e}, "
+          f"max-abs {np.abs(g_fused - g_ops).max() / scale:.2e}")
     assert l2rel(g_fused, g_ops) <= FULL_GRAD_RTOL
-    assert np.abs(g_fused - g_ops).max() <= 2 * GRAD_MAX_RTOL * scale
+    assert np.abs(g_fused - g_ops).max() <= GRAD_MAX_RTOL * scale
 
 
 def test_full_size_fused_field(full_size):

@@ -1,15 +1,16 @@
-"""Whole registrations on the device against reg_f3d<float>::Run of the reference CPU build.
+"""Whole registrations on the device against the reference CPU build's own runs.
 
-Line-search accept/reject decisions depend on the last bits of each objective value, so the
-two trajectories diverge after a few iterations; parity of a whole registration is the final
-NMI within 1e-4 (the device may not end lower than the reference by more than that).
+Scores.  Every final grid is scored with the reference's own objective: the NMI of reg_f3d<float>
+(or the summed NMI of reg_f3d2<float>) of the default SSE build, prepared at the registration's
+last level, at that grid.  The device registrations reproduce that build's field rounding
+(field_order SSE), and the device's own final NMI must equal the reference's objective at the
+device's grid.
 
-Both final grids are scored with the same checker: the oracle NMI with the deformation field
-evaluated in double.  The reference's float field puts voxels that map exactly onto the last
-voxel plane (as on the identity grid every registration starts from) inside or outside the
-floating image depending on the last bit: on the 64^3 identity grid its SSE build marks 89% of
-the upper x face as outside, its scalar build 41%, moving the NMI by 2.7e-3.  The fused
-sweeps evaluate the position in double, so their NMI is checked against the double-field score.
+Bound.  Line-search accept/reject decisions depend on the last bits of each objective value, so
+two trajectories part after a few iterations even when every single evaluation agrees to 1e-7:
+the reference's own SSE and scalar builds end the same registrations up to 1.5e-3 apart (both
+runs are made here and printed).  Parity of a whole registration is therefore: the device may not
+end lower than the weaker of the two reference builds by more than 1e-4.
 """
 import time
 
@@ -21,9 +22,17 @@ from niftyreg_amd import phantoms
 from niftyreg_amd.f3d import register
 from niftyreg_amd.geometry import Geom, create_control_point_grid
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not O.ref_available(), reason="reference build absent")]
 
 FINAL_NMI_TOL = 1e-4
+
+
+def rotation(angle, t):
+    c, s = np.cos(angle), np.sin(angle)
+    A = np.eye(4)
+    A[:3, :3] = [[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]]
+    A[:3, 3] = t
+    return A.astype(np.float32)
 
 
 @pytest.mark.parametrize("levels,maxit", [(1, 50), (3, 50)])
@@ -33,31 +42,34 @@ def test_config0_registration_matches_reference(gpu, levels, maxit):
     t0 = time.perf_counter()
     cg, cpp, nmi, its = register(g, ref, g, flo, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=levels, maxit=maxit)
     t_gpu = time.perf_counter() - t0
-    nmi_gpu = O.nmi_at(g, g, cg, cpp, ref, flo, fp64=True)
+    nmi_gpu = O.ref_nmi_at(g, ref, g, flo, cpp, levels=levels)
     t0 = time.perf_counter()
     rcg, rcpp = O.ref_f3d_run(g, ref, g, flo, levels=levels, maxit=maxit)
     t_cpu = time.perf_counter() - t0
-    nmi_ref = O.nmi_at(g, g, rcg, rcpp, ref, flo, fp64=True)
+    _, scpp = O.ref_f3d_run(g, ref, g, flo, levels=levels, maxit=maxit, scalar=True)
+    nmi_ref = O.ref_nmi_at(g, ref, g, flo, rcpp, levels=levels)
+    nmi_scalar = O.ref_nmi_at(g, ref, g, flo, scpp, levels=levels)
     g0, ident = create_control_point_grid(g, [5.0] * 3)
-    nmi0 = O.nmi_at(g, g, g0, ident, ref, flo, fp64=True)
+    nmi0 = O.ref_nmi_at(g, ref, g, flo, ident)
     print(f"levels {levels}: start NMI {nmi0:.6f}, device {nmi_gpu:.6f} ({its} it, {t_gpu:.2f} s), "
-          f"reference {nmi_ref:.6f} ({t_cpu:.2f} s)")
+          f"reference SSE {nmi_ref:.6f} ({t_cpu:.2f} s), scalar {nmi_scalar:.6f}")
     assert cg.dims == rcg.dims
-    assert abs(nmi - nmi_gpu) <= 1e-5 * nmi_gpu
+    # the device's final objective is the reference's objective at the device's grid
+    assert abs(nmi - nmi_gpu) <= 1e-6 * nmi_gpu
     if levels > 1:
         # with one level at 5-voxel spacing neither implementation finds an improving step from
         # the identity on this pair (the reference stops at its start NMI too)
         assert nmi_gpu > nmi0 + 0.01
-    assert nmi_gpu >= nmi_ref - FINAL_NMI_TOL
+    else:
+        assert abs(nmi_gpu - nmi_ref) <= 1e-6 * nmi_ref
+    assert nmi_gpu >= min(nmi_ref, nmi_scalar) - FINAL_NMI_TOL
 
 
-@pytest.mark.skipif(not O.ref_available(), reason="reference build absent")
 @pytest.mark.parametrize("levels,maxit", [(2, 30)])
 def test_symmetric_registration_matches_reference(gpu, levels, maxit):
     """reg_f3d2 (-vel, NMI, -be 0.001 -le 0.01) on both platforms.  Both final velocity grid pairs
-    are scored with the oracle (each direction exponentiated with its own step choice, the NMI of
-    both directions summed); the device may not end lower than the reference by more than twice
-    the single-direction tolerance."""
+    are scored with reg_f3d2's own summed NMI; the device may not end lower than the weaker
+    reference build by more than twice the single-direction tolerance."""
     from niftyreg_amd.f3d import register_symmetric
     from niftyreg_amd.geometry import create_symmetric_control_point_grids
     ref, flo = phantoms.config0_pair(0)
@@ -65,61 +77,48 @@ def test_symmetric_registration_matches_reference(gpu, levels, maxit):
     t0 = time.perf_counter()
     cg, fw, bw, nmi, its = register_symmetric(g, ref, g, flo, levels=levels, maxit=maxit)
     t_gpu = time.perf_counter() - t0
-    score_gpu = O.nmi_sym_at(g, g, cg, fw, bw, ref, flo)
+    score_gpu = O.ref_nmi_sym_at(g, ref, g, flo, fw, bw, levels=levels)
     t0 = time.perf_counter()
     rcg, rfw, rbw, rsteps = O.ref_f3d2_run(g, ref, g, flo, levels=levels, maxit=maxit)
     t_cpu = time.perf_counter() - t0
-    score_ref = O.nmi_sym_at(g, g, rcg, rfw, rbw, ref, flo)
-    g0, f0, b0 = create_symmetric_control_point_grids(g, g, [5.0 * 2 ** (levels - 1)] * 3)
-    score0 = O.nmi_sym_at(g, g, g0, f0, b0, ref, flo)
+    _, sfw, sbw, _ = O.ref_f3d2_run(g, ref, g, flo, levels=levels, maxit=maxit, scalar=True)
+    score_ref = O.ref_nmi_sym_at(g, ref, g, flo, rfw, rbw, levels=levels)
+    score_scalar = O.ref_nmi_sym_at(g, ref, g, flo, sfw, sbw, levels=levels)
+    g0, f0, b0 = create_symmetric_control_point_grids(g, g, [5.0] * 3)
+    score0 = O.ref_nmi_sym_at(g, ref, g, flo, f0, b0, levels=1)
     print(f"symmetric, {levels} levels: start {score0:.6f}, device {score_gpu:.6f} ({its} it, {t_gpu:.2f} s), "
-          f"reference {score_ref:.6f} ({t_cpu:.2f} s)")
+          f"reference SSE {score_ref:.6f} ({t_cpu:.2f} s), scalar {score_scalar:.6f}")
     assert cg.dims == rcg.dims
     assert np.array_equal(cg.xyz, rcg.xyz)
     assert abs(nmi - score_gpu) <= 1e-4 * score_gpu
     assert score_gpu > score0 + 0.02
-    assert score_gpu >= score_ref - 2 * FINAL_NMI_TOL
+    assert score_gpu >= min(score_ref, score_scalar) - 2 * FINAL_NMI_TOL
 
 
 def test_symmetric_registration_with_affine_matches_reference(gpu):
     """reg_f3d2 with an input affine (-aff) on both platforms: the same symmetric grid geometry
     (both images carried half-way by the half affines), and final grid pairs that score as well as
-    the reference's under the device objective (whose affine terms match the reference's)."""
-    from niftyreg_amd.f3d import F3d2Pair, F3dContent, default_params, register_symmetric
-    from niftyreg_amd.geometry import symmetric_half_affines
+    the reference's under reg_f3d2's own summed NMI with the same affine."""
+    from niftyreg_amd.f3d import register_symmetric
     ref, flo = phantoms.config0_pair(0)
     g = Geom.identity((64, 64, 64))
-    c, s = np.cos(0.05), np.sin(0.05)
-    A = np.eye(4)
-    A[:3, :3] = [[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]]
-    A[:3, 3] = [1.5, -1.0, 0.5]
-    A = A.astype(np.float32)
+    A = rotation(0.05, [1.5, -1.0, 0.5])
     cg, fw, bw, nmi, its = register_symmetric(g, ref, g, flo, levels=2, maxit=30, affine=A)
     rcg, rfw, rbw, rsteps = O.ref_f3d2_run(g, ref, g, flo, levels=2, maxit=30, affine=A)
+    _, sfw, sbw, _ = O.ref_f3d2_run(g, ref, g, flo, levels=2, maxit=30, affine=A, scalar=True)
     assert cg.dims == rcg.dims and np.array_equal(cg.xyz, rcg.xyz)
-    hf, hb, ext = symmetric_half_affines(A)
-
-    def score(f, b):
-        p = default_params(similarity_weight=1.0, be_weight=0.0, le_weight=0.0)
-        cf = F3dContent(g, ref.ravel(), g, flo.ravel(), cg, f, params=p)
-        cb = F3dContent(g, flo.ravel(), g, ref.ravel(), cg, b, params=p)
-        cf.set_velocity_affines(hf, hf, A)
-        cb.set_velocity_affines(hb, hb, O.orc_nifti_inverse(A))
-        for h in (cf, cb):
-            h.rescale_intensities((2, 65), (2, 65))
-        pair = F3d2Pair(cf, cb, steps=6)
-        v, _ = pair.objective()
-        pair.close()
-        return v
-
+    s_dev = O.ref_nmi_sym_at(g, ref, g, flo, fw, bw, levels=2, affine=A)
+    s_ref = O.ref_nmi_sym_at(g, ref, g, flo, rfw, rbw, levels=2, affine=A)
+    s_scalar = O.ref_nmi_sym_at(g, ref, g, flo, sfw, sbw, levels=2, affine=A)
     # the identity velocity on the final grid (no displacement: only the half affines act)
     k, j, i = np.meshgrid(*[np.arange(n) for n in cg.dims[::-1]], indexing="ij")
     lattice = np.stack([i.ravel(), j.ravel(), k.ravel(), np.ones(i.size)])
     ident = (cg.xyz.astype(np.float64) @ lattice)[:3].astype(np.float32).ravel()
-    s_dev, s_ref, s0 = score(fw, bw), score(rfw, rbw), score(ident, ident)
-    print(f"symmetric + affine: start {s0:.6f}, device {s_dev:.6f} ({its} it), reference {s_ref:.6f}")
+    s0 = O.ref_nmi_sym_at(g, ref, g, flo, ident, ident, levels=2, affine=A)
+    print(f"symmetric + affine: start {s0:.6f}, device {s_dev:.6f} ({its} it), reference SSE {s_ref:.6f}, "
+          f"scalar {s_scalar:.6f}")
     assert s_dev > s0 + 0.02
-    assert s_dev >= s_ref - 2 * FINAL_NMI_TOL
+    assert s_dev >= min(s_ref, s_scalar) - 2 * FINAL_NMI_TOL
 
 
 def test_affine_initialised_registration_matches_reference(gpu):
@@ -127,23 +126,20 @@ def test_affine_initialised_registration_matches_reference(gpu):
     bit with no iteration), and the registration from there scores as the reference's."""
     ref, flo = phantoms.config0_pair(0)
     g = Geom.identity((64, 64, 64))
-    c, s = np.cos(0.04), np.sin(0.04)
-    A = np.eye(4)
-    A[:3, :3] = [[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]]
-    A[:3, 3] = [1.0, -0.5, 0.8]
-    A = A.astype(np.float32)
+    A = rotation(0.04, [1.0, -0.5, 0.8])
     cg0, cpp0, _, _ = register(g, ref, g, flo, levels=1, maxit=0, affine=A)
     rcg0, rcpp0 = O.ref_f3d_run(g, ref, g, flo, levels=1, maxit=0, affine=A)
     assert cg0.dims == rcg0.dims and np.array_equal(cpp0, rcpp0)
     cg, cpp, nmi, its = register(g, ref, g, flo, levels=2, maxit=30, affine=A)
     rcg, rcpp = O.ref_f3d_run(g, ref, g, flo, levels=2, maxit=30, affine=A)
-    nmi_gpu = O.nmi_at(g, g, cg, cpp, ref, flo, fp64=True)
-    nmi_ref = O.nmi_at(g, g, rcg, rcpp, ref, flo, fp64=True)
-    nmi0 = O.nmi_at(g, g, cg0, cpp0, ref, flo, fp64=True)
-    print(f"affine start NMI {nmi0:.6f}, device {nmi_gpu:.6f} ({its} it), reference {nmi_ref:.6f}")
+    _, scpp = O.ref_f3d_run(g, ref, g, flo, levels=2, maxit=30, affine=A, scalar=True)
+    nmi_gpu = O.ref_nmi_at(g, ref, g, flo, cpp, levels=2)
+    nmi_ref = O.ref_nmi_at(g, ref, g, flo, rcpp, levels=2)
+    nmi_scalar = O.ref_nmi_at(g, ref, g, flo, scpp, levels=2)
+    nmi0 = O.ref_nmi_at(g, ref, g, flo, cpp0, levels=1)
+    print(f"affine start NMI {nmi0:.6f}, device {nmi_gpu:.6f} ({its} it), reference SSE {nmi_ref:.6f}, "
+          f"scalar {nmi_scalar:.6f}")
     assert cg.dims == rcg.dims
+    assert abs(nmi - nmi_gpu) <= 1e-6 * nmi_gpu
     assert nmi_gpu > nmi0 + 0.01
-    # past the bit-exact start both runs take the same kind of steps, but the line searches compare
-    # objectives whose FP64 sums run in another order, so the two trajectories part after a few
-    # iterations; the final NMI agrees to 5e-4 (measured: 1.31639 against 1.31669)
-    assert nmi_gpu >= nmi_ref - 5e-4
+    assert nmi_gpu >= min(nmi_ref, nmi_scalar) - FINAL_NMI_TOL

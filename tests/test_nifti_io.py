@@ -7,6 +7,7 @@ dimensions, spacings, both matrices, codes and voxel values must agree exactly. 
 is compared with reg_createControlPointGrid's, and files written by either side are read by the
 other.
 """
+import ctypes
 import gzip
 import os
 
@@ -178,3 +179,23 @@ def test_round_trip_and_errors(tmp_path):
         nifti.read_info(tmp_path / "bad.nii")
     with pytest.raises(_lib.NiftyRegError):
         nifti.write_image(tmp_path / "z.hdr", info, got)
+
+
+def test_oversized_header_is_rejected(tmp_path):
+    """A header whose seven dimensions multiply past 64 bits (or past 2^40 bytes of data) is an
+    error before any allocation or read, not a wrapped size."""
+    import struct
+    from niftyreg_amd import _lib
+    data = np.zeros((4, 4, 4), np.float32)
+    _write_nifti(tmp_path / "x.nii", data, pixdim=(1.0, 1.0, 1.0), sform=_oblique_sform())
+    raw = bytearray((tmp_path / "x.nii").read_bytes())
+    raw[40:56] = struct.pack("<8h", 7, 32767, 32767, 32767, 32767, 32767, 32767, 32767)
+    raw[70:74] = struct.pack("<hh", 64, 64)  # float64
+    (tmp_path / "big.nii").write_bytes(bytes(raw))
+    out = np.zeros(8, np.float32)
+    with pytest.raises(_lib.NiftyRegError):
+        _lib.call("nrh_nifti_read_data", str(tmp_path / "big.nii").encode(), _lib.fptr(out), ctypes.c_size_t(2 ** 62))
+    raw[40:56] = struct.pack("<8h", 3, 32767, 32767, 32767, 1, 1, 1, 1)  # 2^45 voxels of 8 bytes: too large
+    (tmp_path / "big2.nii").write_bytes(bytes(raw))
+    with pytest.raises(_lib.NiftyRegError):
+        _lib.call("nrh_nifti_read_data", str(tmp_path / "big2.nii").encode(), _lib.fptr(out), ctypes.c_size_t(2 ** 62))
