@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--no-secondary", action="store_true", help="skip the Aladin, LNCC and F3D2 lines")
     p.add_argument("--only-f3d2", action="store_true",
                    help="run only the symmetric (reg_f3d2) line and print it (for profiling)")
+    p.add_argument("--only", choices=["batch-f3d", "batch-aladin", "serve"],
+                   help="run only one of the batched / serving lines and print it")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched F3D, batched Aladin and serving lines")
     return p.parse_args()
 
 
@@ -245,6 +248,201 @@ def bench_f3d2(torch, ref, flo, geom, steps, with_cpu):
     return out
 
 
+def _gen_config2(seed):
+    from niftyreg_amd import phantoms
+    ref, flo = phantoms.config2_pair(seed)
+    return ref, flo
+
+
+def _gen_config3(seed):
+    from niftyreg_amd import phantoms
+    ref, flo, _ = phantoms.config3_pair(seed, (128, 128, 128))
+    return ref, flo
+
+
+def _gen_config4(seed):
+    from niftyreg_amd import phantoms
+    return phantoms.config4_request(seed)
+
+
+def _generate(fn, seeds):
+    """Synthetic inputs on host worker processes (at most 16, the box's CPU share)."""
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=min(16, len(seeds), os.cpu_count() or 1)) as ex:
+        return list(ex.map(fn, seeds))
+
+
+CONFIG2_SEEDS = list(range(1000, 1064))
+CONFIG3_SEEDS = list(range(2000, 2128))
+CONFIG4_SEEDS = list(range(3000, 3012))
+
+
+def bench_batch_f3d(torch, imgs, with_cpu, maxit=200):
+    """Batched F3D: 64 pairs with axes drawn from U{96..224}, shell phantoms under B-spline warps,
+    float16 image storage (float32 arithmetic), NMI 64 bins, BE 0.001, LE 0, 5-voxel grid, three
+    levels, 200 iterations; nrh_f3d_register_batch against the same registrations one at a time
+    (nrh_f3d_register) and the reference CPU build on one pair of median size."""
+    from niftyreg_amd.f3d import register, register_batch
+    from niftyreg_amd.geometry import Geom
+    from niftyreg_amd import phantoms
+    seeds = CONFIG2_SEEDS
+    n_pairs = len(seeds)
+    pairs = []
+    for s, (ref, flo) in zip(seeds, imgs):
+        g = Geom.identity(phantoms.config2_dims(s))
+        pairs.append((g, ref.ravel(), g, flo.ravel()))
+    voxels = [p[0].nvox for p in pairs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, stats = register_batch(pairs, levels=3, maxit=maxit, image_storage=1)
+    t_batch = time.perf_counter() - t0
+    # the same registrations one at a time (a sample of 16 pairs: the rate per pair is the same
+    # registration; every pair's result is the batch's, bit for bit)
+    sample = list(range(0, n_pairs, max(1, n_pairs // 16)))
+    t0 = time.perf_counter()
+    matched = 0
+    for q in sample:
+        g, ref, fg, flo = pairs[q]
+        cg, cpp, nmi, its = register(g, ref, fg, flo, levels=3, maxit=maxit, image_storage=1)
+        matched += int(nmi == res[q][2] and its == res[q][3] and np.array_equal(cpp, res[q][1]))
+    t_single = (time.perf_counter() - t0) / len(sample) * n_pairs
+    out = {
+        "metric": "batched F3D registrations/hour (64 pairs, axes U{96..224})",
+        "value": n_pairs / t_batch * 3600.0, "unit": "registrations/hour",
+        "seconds": t_batch, "device_seconds": stats["device_seconds"], "graph_replays": stats["graph_replays"],
+        "one_at_a_time_gpu": {"value": n_pairs / t_single * 3600.0, "unit": "registrations/hour",
+                              "sample": f"{len(sample)} of the {n_pairs} registrations one after another "
+                                        "(nrh_f3d_register), scaled to all pairs",
+                              "identical_results": f"{matched}/{len(sample)}"},
+        "autotuned_blocks_per_xcd": stats["blocks_per_xcd"], "tile": stats["tile"],
+        "config": {"workload": f"{n_pairs} F3D registrations, shell phantom pairs with axes U{{96..224}}, float16 "
+                               f"storage, NMI 64 bins, BE 0.001, LE 0, 5-voxel grid, 3 levels, {maxit} iterations",
+                   "total_voxels": int(sum(voxels)), "mean_final_nmi": float(np.mean([r[2] for r in res]))},
+        "cpu_baseline": None,
+    }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracles
+        if oracles.ref_available():
+            q = int(np.argsort(voxels)[len(voxels) // 2])
+            g, ref, fg, flo = pairs[q]
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            t0 = time.perf_counter()
+            rcg, rcpp = oracles.ref_f3d_run(g, ref, fg, flo, levels=3, maxit=maxit)
+            dt = time.perf_counter() - t0
+            nmi_ref = oracles.ref_nmi_at(g, ref, fg, flo, rcpp, levels=3)
+            out["cpu_baseline"] = {"value": 3600.0 / dt, "unit": "registrations/hour", "cores": threads,
+                                   "kind": "reference",
+                                   "sample": f"1 reg_f3d<float> registration of the median pair {g.dims} "
+                                             f"(3 levels, {maxit} iterations, float32), OMP_NUM_THREADS={threads}",
+                                   "final_nmi": nmi_ref, "device_final_nmi": res[q][2]}
+    return out
+
+
+def bench_batch_aladin(torch, imgs, with_cpu):
+    """Batched Aladin: 128 pairs of 128^3 blob phantoms under random affines, affine-only and
+    non-symmetric (-noSym -affDirect), 4^3 blocks, 50% blocks, 50% LTS inliers, 3 levels, 5
+    iterations; nrh_aladin_register_batch against one-at-a-time nrh_aladin_register and the
+    reference CPU build on one pair."""
+    from niftyreg_amd import aladin
+    from niftyreg_amd.geometry import Geom
+    n_pairs = len(imgs)
+    g = Geom.identity((128, 128, 128))
+    pairs = [(g, r.ravel(), g, f.ravel()) for r, f in imgs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Ms, status, stats = aladin.register_batch(pairs, levels=3, maxit=5, rigid=False, affine=True)
+    t_batch = time.perf_counter() - t0
+    sample = list(range(0, n_pairs, max(1, n_pairs // 16)))
+    t0 = time.perf_counter()
+    same = 0
+    for q in sample:
+        M1 = aladin.register(*pairs[q], levels=3, maxit=5, rigid=False, affine=True, symmetric=False)
+        same += int(np.array_equal(M1, Ms[q]))
+    t_single = (time.perf_counter() - t0) / len(sample) * n_pairs
+    out = {
+        "metric": "batched Aladin registrations/hour (128 pairs at 128^3)",
+        "value": n_pairs / t_batch * 3600.0, "unit": "registrations/hour", "seconds": t_batch,
+        "device_seconds": stats["device_seconds"], "failed": int(sum(1 for s_ in status if s_ != 0)),
+        "one_at_a_time_gpu": {"value": n_pairs / t_single * 3600.0, "unit": "registrations/hour",
+                              "sample": f"{len(sample)} of the {n_pairs} registrations one after another, scaled",
+                              "identical_results": f"{same}/{len(sample)}"},
+        "config": {"workload": f"{n_pairs} Aladin affine registrations (non-symmetric, affine only), 128^3 blob "
+                               "phantoms under random affines, 4^3 blocks, 50% blocks, 50% LTS inliers, 3 levels, "
+                               "5 iterations"},
+        "cpu_baseline": None,
+    }
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracles
+        if oracles.ref_available():
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            t0 = time.perf_counter()
+            oracles.ref_aladin_run(*pairs[0], levels=3, maxit=5, rigid=False, affine=True)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": 3600.0 / dt, "unit": "registrations/hour", "cores": threads,
+                                   "kind": "reference",
+                                   "sample": f"1 reg_aladin<float> registration of one 128^3 pair (same settings), "
+                                             f"OMP_NUM_THREADS={threads}"}
+    return out
+
+
+def bench_serving(torch, reqs, rate_hz=2.0, maxit=100):
+    """Serving: Poisson arrivals at rate_hz; each request is an Aladin affine (affine-only, 3
+    levels, 5 iterations) followed by F3D (3 levels, `maxit` iterations) from it, on a cubic volume
+    log-uniform between 64^3 and 256^3 with NMI (64 bins) or LNCC (sigma 5 voxels) at random;
+    admission by a 64 GB device-memory budget, at most 8 requests at once.  Latency is from
+    arrival to result; the one-at-a-time form runs the same requests back to back."""
+    from niftyreg_amd import _lib
+    from niftyreg_amd.geometry import Geom
+    from niftyreg_amd.serve import Server, request_config
+    n_requests = len(reqs)
+    arrivals = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate_hz, n_requests))
+    arrivals -= arrivals[0]
+
+    def submit_all(srv, paced):
+        ids = []
+        t0 = time.perf_counter()
+        for k, (dims, ref, flo, measure) in enumerate(reqs):
+            if paced:
+                delay = arrivals[k] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+            g = Geom.identity(dims)
+            m = _lib.MEASURE_LNCC if measure else _lib.MEASURE_NMI
+            ids.append(srv.submit(g, ref, g, flo, request_config(levels=3, maxit=maxit, measure=m)))
+        return ids
+
+    srv = Server(budget_bytes=64 << 30, max_active=8)
+    t0 = time.perf_counter()
+    ids = submit_all(srv, True)
+    res = [srv.wait(i) for i in ids]
+    t_mix = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["finished"] - r["submitted"] for r in res])
+    srv = Server(budget_bytes=64 << 30, max_active=1)
+    t0 = time.perf_counter()
+    ids = submit_all(srv, False)
+    seq = [srv.wait(i) for i in ids]
+    t_seq = time.perf_counter() - t0
+    srv.close()
+    same = sum(int(a["status"] == b["status"] and a.get("similarity") == b.get("similarity")) for a, b in zip(res, seq))
+    return {
+        "metric": "serving registrations/hour and latency (Poisson arrivals)",
+        "value": n_requests / t_mix * 3600.0, "unit": "registrations/hour",
+        "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+        "failed": int(sum(1 for r in res if r["status"] != 0)), "peak_concurrent": st["peak_active"],
+        "peak_budgeted_bytes": st["peak_bytes"],
+        "one_at_a_time_gpu": {"value": n_requests / t_seq * 3600.0, "unit": "registrations/hour",
+                              "p50_latency_s_if_queued": None, "identical_results": f"{same}/{n_requests}"},
+        "config": {"workload": f"{n_requests} requests, Poisson arrivals at {rate_hz}/s, cubic volumes log-uniform "
+                               f"64^3..256^3 (sides {[r[0][0] for r in reqs]}), NMI/LNCC at random "
+                               f"({sum(r[3] for r in reqs)} LNCC), Aladin affine (3 levels, 5 it) then F3D (3 levels, "
+                               f"{maxit} it), 64 GB budget, at most 8 concurrent"},
+    }
+
+
 def measured_traffic():
     """HBM bytes per gradient-sweep launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (scaled by the copy-kernel calibration recorded there), or None."""
@@ -264,6 +462,17 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): every GPU box here has a single MI355X",
               file=sys.stderr)
         sys.exit(2)
+    # synthetic inputs of the batched and serving lines, made on host worker processes before the
+    # GPU is touched
+    data = {}
+    want = [a.only] if a.only else ([] if (a.no_batch or a.no_secondary or a.only_f3d2) else
+                                    ["batch-f3d", "batch-aladin", "serve"])
+    if "batch-f3d" in want:
+        data["batch-f3d"] = _generate(_gen_config2, CONFIG2_SEEDS)
+    if "batch-aladin" in want:
+        data["batch-aladin"] = _generate(_gen_config3, CONFIG3_SEEDS)
+    if "serve" in want:
+        data["serve"] = _generate(_gen_config4, CONFIG4_SEEDS)
     import torch
     from niftyreg_amd import phantoms
     from niftyreg_amd.f3d import F3dContent, default_params
@@ -279,6 +488,12 @@ def main():
     ref, flo = phantoms.config1_pair(seed=0, dims=DIMS)
     if a.only_f3d2:
         print(json.dumps(bench_f3d2(torch, ref, flo, geom, a.steps, not a.no_cpu_baseline)))
+        return
+    if a.only:
+        fn = {"batch-f3d": lambda: bench_batch_f3d(torch, data["batch-f3d"], not a.no_cpu_baseline),
+              "batch-aladin": lambda: bench_batch_aladin(torch, data["batch-aladin"], not a.no_cpu_baseline),
+              "serve": lambda: bench_serving(torch, data["serve"])}[a.only]
+        print(json.dumps(fn()))
         return
     cppg, cpp = create_control_point_grid(geom, [5.0, 5.0, 5.0])
     rng = np.random.default_rng(7)
@@ -344,6 +559,10 @@ def main():
         out["secondary"] = [bench_aladin(torch, 20, not a.no_cpu_baseline),
                             bench_lncc(torch, ref, flo, geom, cppg, cpp, 10, not a.no_cpu_baseline),
                             bench_f3d2(torch, ref, flo, geom, 10, not a.no_cpu_baseline)]
+        if not a.no_batch:
+            out["secondary"] += [bench_batch_f3d(torch, data.pop("batch-f3d"), not a.no_cpu_baseline),
+                                 bench_batch_aladin(torch, data.pop("batch-aladin"), not a.no_cpu_baseline),
+                                 bench_serving(torch, data.pop("serve"))]
     print(json.dumps(out))
 
 

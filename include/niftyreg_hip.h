@@ -413,6 +413,59 @@ typedef struct nrh_aladin_config {
 } nrh_aladin_config;
 int nrh_aladin_register(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                         const nrh_aladin_config *cfg, float matrix[16]);
+/* reg_aladin<float>::Run (non-symmetric) for n pairs at once: no counterpart in the reference,
+ * which registers one pair per process.  Every pair runs the levels, stages and iterations of
+ * nrh_aladin_register with the same configuration (cfg->symmetric must be 0), its warps and
+ * block matchings launched in turn on one stream and all pairs' LTS fits in one launch (a
+ * workgroup per pair); each stage's iteration is captured once as a HIP graph and replayed, so
+ * every pair's matrix equals nrh_aladin_register's.  matrices[16 n] receive the reference ->
+ * floating affines (row-major); status[n] NRH_OK or the pair's own error (NRH_ERR_NUMERIC: fewer
+ * than 2 active blocks, too few correspondences, a singular fit), the other pairs going on.
+ * stats may be NULL. */
+int nrh_aladin_register_batch(int n, const nrh_geom *refs, const float *const *ref_data, const nrh_geom *flos,
+                              const float *const *flo_data, const nrh_aladin_config *cfg, float *matrices,
+                              int *status, nrh_batch_stats *stats);
+
+/* ------------------------------------------------------------------ registration server */
+/* No counterpart in the reference (one registration per process, reg-apps/reg_aladin.cpp and
+ * reg-apps/reg_f3d.cpp).  A request is reg_aladin's affine (nrh_aladin_register with `aladin`)
+ * followed by reg_f3d initialised from it (nrh_f3d_register with `f3d`, its affine field set to
+ * the Aladin result), NMI or LNCC as f3d.measure says.  Requests are admitted in arrival order
+ * when fewer than max_active run and their estimated device memory fits beside the running
+ * requests' within hbm_budget_bytes; each runs on its own host thread and HIP streams, so running
+ * requests overlap on the device, keep their images, grids and optimiser state resident, and a
+ * request's result is the same alone and in any mix.  A failing request (e.g. a flat image:
+ * NRH_ERR_NUMERIC) fails alone. */
+typedef struct nrh_server nrh_server;
+typedef struct nrh_request {
+    nrh_geom ref, flo;
+    const float *ref_data, *flo_data;  /* copied by nrh_server_submit */
+    nrh_aladin_config aladin;
+    nrh_f3d_config f3d;                /* f3d.affine is ignored (the Aladin result is used) */
+} nrh_request;
+typedef struct nrh_result {
+    int status;              /* NRH_OK or the request's own error code */
+    char error[256];         /* its message */
+    float affine[16];        /* the Aladin stage's reference -> floating affine */
+    nrh_geom grid;           /* the final control-point grid */
+    double similarity;       /* final NMI or LNCC of the F3D stage */
+    int iterations;          /* F3D iterations over all levels */
+    double submitted, started, finished;  /* seconds since the server was created */
+    size_t device_bytes;     /* the admission estimate of the request's device memory */
+} nrh_result;
+typedef struct nrh_server_stats {
+    int pending, active, completed, failed, peak_active;
+    size_t peak_bytes, budget_bytes;
+} nrh_server_stats;
+int nrh_server_create(int device, size_t hbm_budget_bytes, int max_active, nrh_server **out);
+/* Returns at once with the request's id (NRH_ERR_NOMEM if the request alone exceeds the budget). */
+int nrh_server_submit(nrh_server *s, const nrh_request *req, uint64_t *id);
+/* Blocks until request `id` has finished; its result can be taken once.  cpp_out (capacity
+ * floats, may be NULL) receives the final control points. */
+int nrh_server_wait(nrh_server *s, uint64_t id, nrh_result *res, float *cpp_out, size_t capacity);
+int nrh_server_get_stats(nrh_server *s, nrh_server_stats *out);
+/* Finishes every submitted request, then frees the server. */
+int nrh_server_destroy(nrh_server *s);
 
 /* ------------------------------------------------------------------ file formats
  * The edges of a registration: single-file NIfTI-1 images (.nii / .nii.gz) as

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libnrhip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "niftyreg_hip.h")
 
 NRH_OK = 0
+NRH_ERR_ARG, NRH_ERR_HIP, NRH_ERR_STATE, NRH_ERR_NOMEM, NRH_ERR_UNSUPPORTED, NRH_ERR_NUMERIC = 1, 2, 3, 4, 5, 6
 ERR_NAMES = {1: "ARG", 2: "HIP", 3: "STATE", 4: "NOMEM", 5: "UNSUPPORTED", 6: "NUMERIC"}
 
 
@@ -56,6 +57,12 @@ class nrh_batch_stats(ctypes.Structure):
                 ("blocks_per_xcd", ctypes.c_int * 8), ("tile", ctypes.c_int * 3)]
 
 
+class nrh_server_stats(ctypes.Structure):
+    _fields_ = [("pending", ctypes.c_int), ("active", ctypes.c_int), ("completed", ctypes.c_int),
+                ("failed", ctypes.c_int), ("peak_active", ctypes.c_int), ("peak_bytes", ctypes.c_size_t),
+                ("budget_bytes", ctypes.c_size_t)]
+
+
 class nrh_aladin_params(ctypes.Structure):
     _fields_ = [("block_percent", ctypes.c_int), ("inlier_lts", ctypes.c_int), ("padding", ctypes.c_float)]
 
@@ -64,6 +71,18 @@ class nrh_aladin_config(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("levels_to_perform", ctypes.c_int), ("max_iterations", ctypes.c_int),
                 ("block_percent", ctypes.c_int), ("inlier_lts", ctypes.c_int), ("rigid", ctypes.c_int),
                 ("affine", ctypes.c_int), ("align_centre", ctypes.c_int), ("symmetric", ctypes.c_int)]
+
+
+class nrh_request(ctypes.Structure):
+    _fields_ = [("ref", nrh_geom), ("flo", nrh_geom), ("ref_data", ctypes.POINTER(ctypes.c_float)),
+                ("flo_data", ctypes.POINTER(ctypes.c_float)), ("aladin", nrh_aladin_config), ("f3d", nrh_f3d_config)]
+
+
+class nrh_result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("error", ctypes.c_char * 256), ("affine", ctypes.c_float * 16),
+                ("grid", nrh_geom), ("similarity", ctypes.c_double), ("iterations", ctypes.c_int),
+                ("submitted", ctypes.c_double), ("started", ctypes.c_double), ("finished", ctypes.c_double),
+                ("device_bytes", ctypes.c_size_t)]
 
 
 class nrh_nifti_info(ctypes.Structure):

@@ -135,3 +135,26 @@ def register(ref_geom: Geom, ref, flo_geom: Geom, flo, levels=3, levels_to_perfo
     _lib.call("nrh_aladin_register", ctypes.byref(rg), _lib.fptr(ref), ctypes.byref(fg), _lib.fptr(flo),
               ctypes.byref(cfg), _lib.fptr(m))
     return m.reshape(4, 4)
+
+
+def register_batch(pairs, levels=3, levels_to_perform=0, maxit=5, block_percent=50, inlier_lts=50, rigid=True,
+                   affine=True, align_centre=True):
+    """reg_aladin<float>::Run (non-symmetric) for many pairs at once (nrh_aladin_register_batch).
+
+    pairs: list of (ref_geom, ref, flo_geom, flo).  Returns (matrices [n, 4, 4], status [n], stats)."""
+    n = len(pairs)
+    cfg = _lib.nrh_aladin_config(levels, levels_to_perform, maxit, block_percent, inlier_lts, int(rigid), int(affine),
+                                 int(align_centre), 0)
+    refs = (_lib.nrh_geom * n)(*[p[0].to_c() for p in pairs])
+    flos = (_lib.nrh_geom * n)(*[p[2].to_c() for p in pairs])
+    keep = [(np.ascontiguousarray(p[1], np.float32).reshape(-1), np.ascontiguousarray(p[3], np.float32).reshape(-1))
+            for p in pairs]
+    rdata = (ctypes.POINTER(ctypes.c_float) * n)(*[_lib.fptr(k[0]) for k in keep])
+    fdata = (ctypes.POINTER(ctypes.c_float) * n)(*[_lib.fptr(k[1]) for k in keep])
+    m = np.zeros(16 * n, np.float32)
+    status = (ctypes.c_int * n)()
+    stats = _lib.nrh_batch_stats()
+    _lib.call("nrh_aladin_register_batch", n, refs, rdata, flos, fdata, ctypes.byref(cfg), _lib.fptr(m), status,
+              ctypes.byref(stats))
+    st = {"seconds": stats.seconds, "device_seconds": stats.device_seconds, "graph_replays": stats.graph_replays}
+    return m.reshape(n, 4, 4), list(status), st

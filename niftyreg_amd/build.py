@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnrhip.so")
 
 SOURCES = ["capi.cpp", "capi_eval.cpp", "capi_velocity.cpp", "capi_f3d2.cpp", "f3d_ops.hip", "f3d_fused.hip", "f3d_velocity.hip",
-           "driver.cpp", "aladin.hip", "lncc.hip", "nifti_io.cpp", "f3d_batch.hip"]
+           "driver.cpp", "aladin.hip", "lncc.hip", "nifti_io.cpp", "f3d_batch.hip", "serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NRH_OFFLOAD_ARCH", "gfx950")

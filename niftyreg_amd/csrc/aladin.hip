@@ -17,6 +17,9 @@
 #include <vector>
 
 #include "common.hpp"
+#include <array>
+#include <chrono>
+#include <thread>
 #include "device.hpp"
 #include "recip_x87.hpp"
 #include "m44.hpp"
@@ -483,12 +486,11 @@ __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q
 // 2 singular fit (the matrix is then left unchanged).  The trimmed subset is the keep smallest
 // residuals (float bit patterns, found by a three-digit radix select); ties at the threshold
 // keep the lowest indices.
-__global__ void __launch_bounds__(kLtsThreads)
-k_lts(const float *__restrict__ refPos, const float *__restrict__ warpPos, int nActive, float *__restrict__ matrix,
-      float *__restrict__ P, float *__restrict__ Q, uint32_t *__restrict__ res, int percent, int affine,
-      int *__restrict__ status) {
+__device__ __forceinline__ void
+lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, int nActive, float *__restrict__ matrix,
+         float *__restrict__ P, float *__restrict__ Q, uint32_t *__restrict__ res, int percent, int affine,
+         int *__restrict__ status, LtsShared& s) {
 #pragma clang fp contract(off)
-    __shared__ LtsShared s;
     const int tid = threadIdx.x;
     if (tid < 16) s.M[tid] = matrix[tid];
     __syncthreads();
@@ -608,6 +610,36 @@ k_lts(const float *__restrict__ refPos, const float *__restrict__ warpPos, int n
     if (tid < 16) matrix[tid] = s.M[tid];
 }
 
+__global__ void __launch_bounds__(kLtsThreads)
+k_lts(const float *__restrict__ refPos, const float *__restrict__ warpPos, int nActive, float *__restrict__ matrix,
+      float *__restrict__ P, float *__restrict__ Q, uint32_t *__restrict__ res, int percent, int affine,
+      int *__restrict__ status) {
+    __shared__ LtsShared s;
+    lts_body(refPos, warpPos, nActive, matrix, P, Q, res, percent, affine, status, s);
+}
+
+// The LTS fits of a batch of pairs in one launch: one workgroup per pair.
+struct LtsArgs {
+    const float *refPos, *warpPos;
+    int nActive;
+    float *matrix, *P, *Q;
+    uint32_t *res;
+    int percent;
+    int *status;
+};
+
+__global__ void __launch_bounds__(kLtsThreads) k_lts_batch(const LtsArgs *__restrict__ A, int affine) {
+    __shared__ LtsShared s;
+    const LtsArgs& a = A[blockIdx.x];
+    lts_body(a.refPos, a.warpPos, a.nActive, a.matrix, a.P, a.Q, a.res, a.percent, affine, a.status, s);
+}
+
+// Empties the defined-block count before a block matching (a kernel, so that an iteration can be
+// captured in a graph).
+__global__ void k_zero_int(int *p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
 // ------------------------------------------------------------------ symmetric update
 // reg_aladin_sym::UpdateTransformationMatrix (reg-lib/_reg_aladin_sym.cpp:128-157) after the two
 // LTS fits: forward <- avg(forward, inv(backward)), backward <- avg(inv(forward), backward), where
@@ -723,7 +755,7 @@ void launch_warp(nrh_aladin *h) {
 }
 
 void launch_block_matching(nrh_aladin *h) {
-    NRH_HIP(hipMemsetAsync(h->counters.p, 0, sizeof(int), h->stream));
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, h->stream, h->counters.p);
     hipLaunchKernelGGL(k_block_match, dim3(h->activeBlocks), dim3(kMatchThreads), 0, h->stream, h->refImg.p,
                        h->warped.p, h->mask.p, h->ref.nx, h->ref.ny, h->ref.nz, h->nb[0], h->nb[1], h->activeList.p,
                        to_affine(h->ref.xyz), h->refPos.p, h->warpPos.p, h->counters.p);
@@ -769,6 +801,8 @@ void launch_symmetric_update(nrh_aladin *fw, nrh_aladin *bw) {
     NRH_HIP(hipGetLastError());
 }
 
+void destroy(nrh_aladin *h);
+
 nrh_aladin *create(const nrh_geom *ref, const float *ref_data, const nrh_geom *flo, const float *flo_data,
                    const int *mask, const nrh_aladin_params *params, hipStream_t sharedStream = nullptr) {
     check_geom(ref, "reference");
@@ -812,7 +846,12 @@ nrh_aladin *create(const nrh_geom *ref, const float *ref_data, const nrh_geom *f
     h->pinned[0] = -1;
     h->pinned[1] = 0;
     h->pinned[2] = 0;
-    select_blocks(h.get());
+    try {
+        select_blocks(h.get());
+    } catch (...) {
+        destroy(h.release());
+        throw;
+    }
     return h.release();
 }
 
@@ -911,6 +950,142 @@ void aladin_register(const nrh_geom& refG, const float *refData, const nrh_geom&
     std::memcpy(out, M, sizeof(M));
 }
 
+// reg_aladin<float>::Run for many pairs at once (non-symmetric): every pair runs aladin_register's
+// levels, stages and iterations, with the pairs' warps and block matchings launched in turn on one
+// stream and all pairs' LTS fits in one launch (a workgroup each); an iteration of all pairs is
+// captured once per stage as a HIP graph and replayed.  A pair whose block selection or fit fails
+// (a flat image: fewer than 2 active blocks, or too few correspondences) stops with its error in
+// status[q]; the other pairs go on.
+void aladin_register_batch(int n, const nrh_geom *refs, const float *const *refData, const nrh_geom *flos,
+                           const float *const *floData, const nrh_aladin_config& cfg, float *out, int *status,
+                           nrh_batch_stats *stats) {
+    NRH_REQUIRE(cfg.levels >= 1 && cfg.max_iterations >= 0, NRH_ERR_ARG, "invalid Aladin configuration");
+    NRH_REQUIRE(cfg.rigid || cfg.affine, NRH_ERR_ARG, "neither a rigid nor an affine stage requested");
+    NRH_REQUIRE(!cfg.symmetric, NRH_ERR_UNSUPPORTED, "batched Aladin runs the non-symmetric registration");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int toPerform = cfg.levels_to_perform > 0 ? std::min(cfg.levels_to_perform, cfg.levels) : cfg.levels;
+    std::vector<std::array<float, 16>> M(n);
+    std::vector<std::vector<Level>> rp(n), fp(n);
+    std::vector<std::string> errs(n);
+    {
+        std::vector<std::thread> pool;
+        const int T = std::max(1, std::min(n, (int)std::thread::hardware_concurrency()));
+        for (int t = 0; t < T; ++t)
+            pool.emplace_back([&, t] {
+                for (int q = t; q < n; q += T) {
+                    try {
+                        rp[q] = pyramid(refs[q], refData[q], cfg.levels, toPerform);
+                        fp[q] = pyramid(flos[q], floData[q], cfg.levels, toPerform);
+                    } catch (const std::exception& e) {
+                        errs[q] = e.what();
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    for (int q = 0; q < n; ++q) {
+        NRH_REQUIRE(errs[q].empty(), NRH_ERR_ARG, "pair " + std::to_string(q) + ": " + errs[q]);
+        float m[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
+        if (cfg.align_centre) {
+            auto centre = [](const nrh_geom& g, float c[3]) {
+                const float v[3] = { (float)g.nx / 2.0f, (float)g.ny / 2.0f, (float)g.nz / 2.0f };
+                for (int r = 0; r < 3; ++r)
+                    c[r] = (float)((double)g.xyz[r * 4] * v[0] + (double)g.xyz[r * 4 + 1] * v[1] +
+                                   (double)g.xyz[r * 4 + 2] * v[2] + (double)g.xyz[r * 4 + 3]);
+            };
+            float fc[3], rc[3];
+            centre(flos[q], fc);
+            centre(refs[q], rc);
+            m[3] = fc[0] - rc[0];
+            m[7] = fc[1] - rc[1];
+            m[11] = fc[2] - rc[2];
+        }
+        std::copy(m, m + 16, M[q].begin());
+        status[q] = NRH_OK;
+    }
+    hipStream_t stream;
+    NRH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    struct StreamOwner {
+        hipStream_t s;
+        ~StreamOwner() { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+    } so{ stream };
+    const nrh_aladin_params params{ cfg.block_percent, cfg.inlier_lts, std::numeric_limits<float>::quiet_NaN() };
+    double deviceSeconds = 0.0;
+    int replays = 0;
+    for (int level = 0; level < toPerform; ++level) {
+        std::vector<AladinHandle> hs(n);
+        std::vector<nrh_aladin *> live;
+        std::vector<int> liveIdx;
+        for (int q = 0; q < n; ++q) {
+            if (status[q] != NRH_OK) continue;
+            try {
+                hs[q].h = create(&rp[q][level].geom, rp[q][level].data.data(), &fp[q][level].geom,
+                                 fp[q][level].data.data(), nullptr, &params, stream);
+                set_matrix(hs[q].h, M[q].data());
+                live.push_back(hs[q].h);
+                liveIdx.push_back(q);
+            } catch (const Error& e) {
+                if (e.code != NRH_ERR_NUMERIC) throw;
+                status[q] = e.code;  // this pair stops; the others go on
+            }
+        }
+        if (live.empty()) break;
+        const int L = (int)live.size();
+        std::vector<LtsArgs> la(L);
+        for (int k = 0; k < L; ++k) {
+            nrh_aladin *h = live[k];
+            la[k] = LtsArgs{ h->refPos.p, h->warpPos.p, h->activeBlocks, h->matrix.p, h->ltsP.p, h->ltsQ.p, h->ltsRes.p,
+                             h->params.inlier_lts, h->counters.p + 1 };
+        }
+        DevBuf<LtsArgs> dla;
+        dla.alloc(L);
+        NRH_HIP(hipMemcpyAsync(dla.p, la.data(), L * sizeof(LtsArgs), hipMemcpyHostToDevice, stream));
+        NRH_HIP(hipStreamSynchronize(stream));
+        const int maxIt = level == 0 ? 2 * cfg.max_iterations : cfg.max_iterations;
+        auto resolve = [&](int iterations, bool affine) {
+            if (iterations <= 0) return;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            NRH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            for (nrh_aladin *h : live) {
+                launch_warp(h);
+                launch_block_matching(h);
+            }
+            hipLaunchKernelGGL(k_lts_batch, dim3(L), dim3(kLtsThreads), 0, stream, dla.p, affine ? 1 : 0);
+            NRH_HIP(hipStreamEndCapture(stream, &graph));
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            NRH_HIP(ie);
+            const auto d0 = std::chrono::steady_clock::now();
+            hipError_t ge = hipSuccess;
+            for (int it = 0; it < iterations && ge == hipSuccess; ++it, ++replays) ge = hipGraphLaunch(exec, stream);
+            if (ge == hipSuccess) ge = hipStreamSynchronize(stream);
+            (void)hipGraphExecDestroy(exec);
+            NRH_HIP(ge);
+            deviceSeconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - d0).count();
+        };
+        if ((cfg.rigid && !cfg.affine) || (cfg.affine && cfg.rigid && level == 0))
+            resolve(maxIt * ((cfg.affine && cfg.rigid && level == 0) ? 4 : 1), false);
+        if (cfg.affine) resolve(maxIt, true);
+        for (int k = 0; k < L; ++k) {
+            nrh_aladin *h = live[k];
+            read_counters(h);
+            const int q = liveIdx[k];
+            if (h->pinned[1] != 0) {
+                status[q] = NRH_ERR_NUMERIC;  // too few correspondences or a singular fit
+                continue;
+            }
+            get_matrix(h, M[q].data());
+        }
+    }
+    for (int q = 0; q < n; ++q) std::copy(M[q].begin(), M[q].end(), out + 16 * q);
+    if (stats) {
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        stats->device_seconds = deviceSeconds;
+        stats->graph_replays = replays;
+    }
+}
+
 }  // namespace
 }  // namespace nrh
 
@@ -924,6 +1099,23 @@ int nrh_aladin_create(const nrh_geom *ref, const float *ref_data, const nrh_geom
         NRH_REQUIRE(out, NRH_ERR_ARG, "null output handle");
         *out = nullptr;
         *out = create(ref, ref_data, flo, flo_data, mask, params);
+    });
+}
+
+int nrh_aladin_register_batch(int n, const nrh_geom *refs, const float *const *ref_data, const nrh_geom *flos,
+                              const float *const *flo_data, const nrh_aladin_config *cfg, float *matrices,
+                              int *status, nrh_batch_stats *stats) {
+    return guard([&] {
+        NRH_REQUIRE(n >= 1 && refs && ref_data && flos && flo_data && cfg && matrices && status, NRH_ERR_ARG,
+                    "null argument");
+        for (int q = 0; q < n; ++q) {
+            check_geom(&refs[q], "reference");
+            check_geom(&flos[q], "floating");
+            NRH_REQUIRE(ref_data[q] && flo_data[q], NRH_ERR_ARG, "null image");
+            NRH_REQUIRE(refs[q].nz > 1, NRH_ERR_UNSUPPORTED, "2-D images are not supported");
+        }
+        if (stats) *stats = nrh_batch_stats{};
+        aladin_register_batch(n, refs, ref_data, flos, flo_data, *cfg, matrices, status, stats);
     });
 }
 

@@ -43,6 +43,7 @@ namespace {
 constexpr int TX = 64, TY = 8, TZ = 8;
 constexpr int WXM = 20, WYM = 6, WZM = 6;  // node-window bounds for spacing >= 4 voxels
 constexpr int kThreads = 256;
+constexpr int kTileVoxels = TX * TY * TZ;
 
 struct FusedArgs {
     const void *ref;
@@ -65,9 +66,10 @@ struct FusedArgs {
     const float *kTab;  // gradient sweep: K[r + w rb] = log p_r + log p_w - NMI log p_rw (padded to 4)
     float *gvox;      // gradient sweep: the NMI voxel gradient (planar [3][N])
     float *fieldOut;  // optional: the deformation field of the sweep (planar [3][N]), for checking
-    // voxels left to k_sweep_fixup: positions within ambTol (floating voxels) of a voxel plane
+    // voxels left to k_sweep_fixup: positions within ambTol (floating voxels) of a voxel plane;
+    // tile t lists its voxels in ambList[t * kTileVoxels ..] and their number in ambCounts[t]
     uint32_t *ambList;
-    uint32_t *ambCount;
+    uint32_t *ambCounts;
     float ambTol;
     int fieldOrder;  // NRH_FIELD_ORDER_*
     int lut;         // 5-voxel spacing on every axis (the reference's table path)
@@ -78,8 +80,8 @@ template<class T> __device__ __forceinline__ float ld(const T *p, size_t i);
 template<> __device__ __forceinline__ float ld<float>(const float *p, size_t i) { return p[i]; }
 template<> __device__ __forceinline__ float ld<__half>(const __half *p, size_t i) { return __half2float(p[i]); }
 
-// Append the voxels of a wave that ask for it to a list (one atomic per wave).  The list holds
-// one slot per voxel of the image, so it cannot overflow.
+// Append the voxels of a wave that ask for it to the tile's list (one LDS atomic per wave on the
+// tile's count).  The tile's list has a slot for each of its voxels, so it cannot overflow.
 __device__ __forceinline__ void append_voxels(bool want, uint32_t idx, uint32_t *list, uint32_t *count) {
     const uint64_t m = __ballot(want);
     if (m == 0) return;
@@ -262,6 +264,7 @@ struct SweepShared {
     double sIdYZ[TZ][TY][3];  // identity terms of each row: m[c][1] uy + m[c][2] uz + m[c][3]
     float4 sBx[TX], sBy[TY], sBz[TZ];
     int sPx[TX], sPy[TY], sPz[TZ];
+    uint32_t ambCount;  // voxels of the tile listed for the exact pass
 };
 
 // One tile t of the pair described by a; dyn: the dynamic LDS (window [3][WZM][WYM][WXM], then
@@ -277,6 +280,8 @@ __device__ __forceinline__ void sweep_tile(const FusedArgs& a, const int t, Swee
     auto& sPy = S.sPy;
     auto& sPz = S.sPz;
     const int tid = threadIdx.x;
+    if (tid == 0) S.ambCount = 0u;  // read after the tile's barriers
+    uint32_t *ambList = a.ambList + (size_t)t * kTileVoxels;
     const int tx = t % a.tilesX, ty = (t / a.tilesX) % a.tilesY, tz = t / (a.tilesX * a.tilesY);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
     const int nX = min(TX, a.nx - x0), nY = min(TY, a.ny - y0), nZ = min(TZ, a.nz - z0);
@@ -410,7 +415,7 @@ __device__ __forceinline__ void sweep_tile(const FusedArgs& a, const int t, Swee
                 // a voxel near a plane goes to the exact pass (rows past the tile's last are
                 // clamped duplicates and list nothing)
                 const bool amb = pos[k].amb && t[k].active && wave + k * (kThreads / 64) < nY;
-                append_voxels(amb, idx[k], a.ambList, a.ambCount);
+                append_voxels(amb, idx[k], ambList, &S.ambCount);
                 t[k].active = t[k].active && !pos[k].amb;
                 inside = inside && taps_inside(a, pos[k]);
             }
@@ -439,8 +444,9 @@ __device__ __forceinline__ void sweep_tile(const FusedArgs& a, const int t, Swee
             }
         }
     }
+    __syncthreads();
+    if (tid == 0) a.ambCounts[t] = S.ambCount;
     if (!GRAD) {
-        __syncthreads();
         const uint32_t *h = (const uint32_t *)dyn;
         for (int i = tid; i < a.rb * a.fb; i += kThreads)
             if (h[i]) atomicAdd(&a.hist[i], h[i]);
@@ -676,13 +682,11 @@ k_project_nodes_batch(const NodeArgs *__restrict__ A, const int2 *__restrict__ i
 }
 
 // K[r + w rb] = log p_r + log p_w - NMI log p_rw, evaluated in FP64 from the reference's log
-// histogram and rounded once to float: the table the gradient sweep contracts.  Also empties the
-// exact pass's list for the gradient sweep that follows.
+// histogram and rounded once to float: the table the gradient sweep contracts.
 __global__ void __launch_bounds__(256) k_nmi_table(const double *__restrict__ logh, const double *__restrict__ ent,
-                                                   int rb, int fb, float *__restrict__ K, uint32_t *__restrict__ ambCount) {
+                                                   int rb, int fb, float *__restrict__ K) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *ambCount = 0u;  // the gradient sweep's exact-pass list starts empty
     if (i >= rb * fb) return;
     const double nmi = (ent[0] + ent[1]) / ent[2];
     const double *lr = logh + rb * fb, *lw = lr + rb;
@@ -763,10 +767,9 @@ __device__ __forceinline__ float3 gradient_at(const TImg *__restrict__ flo, int 
     return make_float3(g0, g1, g2);
 }
 
-constexpr int kFixupBlocks = 1024;
 
-// One listed voxel idx of pair a: the histogram pass counts it into h (LDS), the gradient pass
-// writes its voxel gradient.
+// One listed voxel idx of pair a: the histogram pass counts it (into h in LDS, or straight into
+// the pair's histogram when h is null), the gradient pass writes its voxel gradient.
 template<class TImg, bool GRAD>
 __device__ __forceinline__ void fixup_voxel(const FusedArgs& a, const uint32_t idx, uint32_t *h) {
     const TImg *__restrict__ ref = (const TImg *)a.ref;
@@ -783,7 +786,7 @@ __device__ __forceinline__ void fixup_voxel(const FusedArgs& a, const uint32_t i
     const float rv = ld(ref, idx);
     if (!GRAD) {
         if (rv == rv && war == war && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
-            atomicAdd(&h[(int)rv + (int)war * a.rb], 1u);
+            atomicAdd(&(h ? h : a.hist)[(int)rv + (int)war * a.rb], 1u);
     } else {
         float d[3] = { 0.f, 0.f, 0.f };
         if (rv == rv && war == war) {
@@ -798,20 +801,21 @@ __device__ __forceinline__ void fixup_voxel(const FusedArgs& a, const uint32_t i
     }
 }
 
-// The listed voxels of one pair, grid-strided over the launch; blocks past the listed count
-// return at once.
+// The voxels tile t of pair a listed, one workgroup per tile; a tile with many listed voxels (an
+// identity grid lists them all) counts them in LDS first.
 template<class TImg, bool GRAD>
-__device__ __forceinline__ void fixup_pair(const FusedArgs& a, uint32_t *h) {
-    const uint32_t n = *a.ambCount;
-    if ((size_t)blockIdx.x * blockDim.x >= n) return;  // uniform over the block
+__device__ __forceinline__ void fixup_tile(const FusedArgs& a, const int t, uint32_t *h) {
+    const uint32_t n = a.ambCounts[t];
+    if (n == 0) return;  // uniform over the workgroup
+    const uint32_t *list = a.ambList + (size_t)t * kTileVoxels;
     const int tid = threadIdx.x;
-    if (!GRAD) {
+    const bool useLds = !GRAD && n > 256;
+    if (useLds) {
         for (int i = tid; i < a.rb * a.fb; i += blockDim.x) h[i] = 0u;
         __syncthreads();
     }
-    for (uint32_t i = blockIdx.x * blockDim.x + tid; i < n; i += gridDim.x * blockDim.x)
-        fixup_voxel<TImg, GRAD>(a, a.ambList[i], h);
-    if (!GRAD) {
+    for (uint32_t i = tid; i < n; i += blockDim.x) fixup_voxel<TImg, GRAD>(a, list[i], useLds ? h : nullptr);
+    if (useLds) {
         __syncthreads();
         for (int i = tid; i < a.rb * a.fb; i += blockDim.x)
             if (h[i]) atomicAdd(&a.hist[i], h[i]);
@@ -821,34 +825,33 @@ __device__ __forceinline__ void fixup_pair(const FusedArgs& a, uint32_t *h) {
 template<class TImg, bool GRAD>
 __global__ void __launch_bounds__(256) k_sweep_fixup(FusedArgs a) {
     extern __shared__ __align__(16) float fixLds[];  // histogram counts (objective pass)
-    fixup_pair<TImg, GRAD>(a, (uint32_t *)fixLds);
+    fixup_tile<TImg, GRAD>(a, blockIdx.x, (uint32_t *)fixLds);
 }
 
+// Batched: one workgroup per (pair, tile) item of the sweeps' work list.
 template<class TImg, bool GRAD>
-__global__ void __launch_bounds__(256) k_sweep_fixup_batch(const FusedArgs *__restrict__ A, int P, const int *active) {
+__global__ void __launch_bounds__(256)
+k_sweep_fixup_batch(const FusedArgs *__restrict__ A, const uint32_t *__restrict__ items, const int *active) {
     extern __shared__ __align__(16) float fixLds[];
-    for (int q = 0; q < P; ++q) {
-        if (active && !active[q]) continue;
-        fixup_pair<TImg, GRAD>(A[q], (uint32_t *)fixLds);
-        __syncthreads();  // the LDS histogram of the next pair
-    }
+    const uint32_t it = items[blockIdx.x];
+    const int pair = (int)(it >> 20);
+    if (active && !active[pair]) return;
+    fixup_tile<TImg, GRAD>(A[pair], (int)(it & 0xfffffu), (uint32_t *)fixLds);
 }
 
-// Per pair: zero the joint-histogram counts and the exact pass's list count (block = pair).
+// Per pair: zero the joint-histogram counts (block = pair).
 __global__ void k_clear_counts_batch(const FusedArgs *__restrict__ A, const int *active) {
     if (active && !active[blockIdx.x]) return;
     const FusedArgs& a = A[blockIdx.x];
     for (int i = threadIdx.x; i < a.rb * a.fb; i += blockDim.x) a.hist[i] = 0u;
-    if (threadIdx.x == 0) *a.ambCount = 0u;
 }
 
-// Per pair (blockIdx.y): the NMI table K of the gradient sweep; empties the pair's list.
+// Per pair (blockIdx.y): the NMI table K of the gradient sweep.
 __global__ void __launch_bounds__(256) k_nmi_table_batch(const FusedArgs *__restrict__ A, const int *active) {
 #pragma clang fp contract(off)
     if (active && !active[blockIdx.y]) return;
     const FusedArgs& a = A[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *a.ambCount = 0u;
     if (i >= a.rb * a.fb) return;
     const double nmi = (a.ent[0] + a.ent[1]) / a.ent[2];
     const double *lr = a.logh + a.rb * a.fb, *lw = lr + a.rb;
@@ -885,8 +888,8 @@ struct FusedPlan {
     DevBuf<float> invNx, invNy, invNz;
     DevBuf<float> gvox, projX, projY;  // voxel gradient [3][N]; after the x pass; after the y pass
     DevBuf<float> kTab;                // NMI derivative table, rb * fb rounded up to 4 floats
-    DevBuf<uint32_t> ambList;          // voxels left to the exact pass (one slot per voxel)
-    DevBuf<uint32_t> ambCount;
+    DevBuf<uint32_t> ambList;          // voxels left to the exact pass, kTileVoxels slots per tile
+    DevBuf<uint32_t> ambCounts;        // listed voxels per tile
     float ambTol = 0.f;                // floating voxels
     bool lut = false;
 };
@@ -1033,8 +1036,8 @@ static FusedPlan *plan_of(nrh_f3d *h) {
             rowMax = std::max(rowMax, (double)std::fabs(h->flo.ijk[r * 4]) + std::fabs(h->flo.ijk[r * 4 + 1]) +
                                           std::fabs(h->flo.ijk[r * 4 + 2]));
         p->ambTol = (float)(48.0 * std::ldexp(1.0, -24) * (1.25 * M + 8.0) * rowMax);
-        p->ambList.alloc(h->N);
-        p->ambCount.alloc(1);
+        p->ambList.alloc((size_t)p->numTiles * kTileVoxels);
+        p->ambCounts.alloc(p->numTiles);
         NRH_HIP(hipStreamSynchronize(h->stream));
     }
     std::lock_guard<std::mutex> lk(g_planMutex);
@@ -1093,7 +1096,7 @@ static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
     a.tilesX = p->tilesX; a.tilesY = p->tilesY; a.tilesZ = p->tilesZ;
     a.numTiles = p->numTiles;
     a.ambList = p->ambList.p;
-    a.ambCount = p->ambCount.p;
+    a.ambCounts = p->ambCounts.p;
     a.ambTol = p->ambTol;
     a.fieldOrder = h->params.field_order;
     a.lut = p->lut ? 1 : 0;
@@ -1102,11 +1105,7 @@ static FusedArgs make_args(nrh_f3d *h, FusedPlan *p) {
     return a;
 }
 
-// Zeroes the joint-histogram counts and the exact pass's list count in one launch.
-__global__ void k_clear_counts(uint32_t *__restrict__ hist, int n, uint32_t *__restrict__ count) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) hist[i] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0u;
-}
+
 
 static unsigned grid_for(int numTiles) { return (unsigned)((numTiles + 7) / 8 * 8); }
 
@@ -1123,7 +1122,7 @@ void fused_histogram(nrh_f3d *h, float *fieldOut) {
     FusedArgs a = make_args(h, p);
     a.hist = h->histCount.p;
     a.fieldOut = fieldOut;
-    hipLaunchKernelGGL(k_clear_counts, dim3(8), dim3(256), 0, h->stream, h->histCount.p, a.rb * a.fb, a.ambCount);
+    op_fill_u32(h->histCount.p, (size_t)a.rb * a.fb, 0u, h->stream);
     const size_t lds = sweep_lds(a);
     const size_t ldsFix = (size_t)a.rb * a.fb * sizeof(uint32_t);
     if (h->profile) NRH_HIP(hipEventRecord(h->ev[0], h->stream));
@@ -1137,9 +1136,9 @@ void fused_histogram(nrh_f3d *h, float *fieldOut) {
         h->evPending[0] = true;
     }
     if (h->params.image_storage == 1)
-        hipLaunchKernelGGL((k_sweep_fixup<__half, false>), dim3(kFixupBlocks), dim3(256), ldsFix, h->stream, a);
+        hipLaunchKernelGGL((k_sweep_fixup<__half, false>), dim3(p->numTiles), dim3(256), ldsFix, h->stream, a);
     else
-        hipLaunchKernelGGL((k_sweep_fixup<float, false>), dim3(kFixupBlocks), dim3(256), ldsFix, h->stream, a);
+        hipLaunchKernelGGL((k_sweep_fixup<float, false>), dim3(p->numTiles), dim3(256), ldsFix, h->stream, a);
     NRH_HIP(hipGetLastError());
     op_nmi_finalize(h);
     h->histVersion = h->cppVersion;
@@ -1217,7 +1216,7 @@ void fused_gradient(nrh_f3d *h, float weight) {
     const int nbins = a.rb * a.fb;
     p->kTab.alloc((size_t)(nbins + 3) / 4 * 4);
     hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, h->histLog.p, a.ent, a.rb,
-                       a.fb, p->kTab.p, a.ambCount);
+                       a.fb, p->kTab.p);
     a.kTab = p->kTab.p;
     const size_t lds = sweep_lds(a);
     if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
@@ -1231,9 +1230,9 @@ void fused_gradient(nrh_f3d *h, float weight) {
         h->evPending[1] = true;
     }
     if (h->params.image_storage == 1)
-        hipLaunchKernelGGL((k_sweep_fixup<__half, true>), dim3(kFixupBlocks), dim3(256), 0, h->stream, a);
+        hipLaunchKernelGGL((k_sweep_fixup<__half, true>), dim3(p->numTiles), dim3(256), 0, h->stream, a);
     else
-        hipLaunchKernelGGL((k_sweep_fixup<float, true>), dim3(kFixupBlocks), dim3(256), 0, h->stream, a);
+        hipLaunchKernelGGL((k_sweep_fixup<float, true>), dim3(p->numTiles), dim3(256), 0, h->stream, a);
     NRH_HIP(hipGetLastError());
     project_to_nodes(h, p, p->gvox.p, weight);
 }
@@ -1325,13 +1324,13 @@ void batch_histogram(nrh_batch *b, const int *active) {
     if (half) {
         hipLaunchKernelGGL((k_sweep_batch<__half, false>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, false>), dim3(kFixupBlocks), dim3(256), b->fixLds, b->stream, A,
-                           b->P, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, false>), dim3(b->nSweepItems), dim3(256), b->fixLds, b->stream,
+                           A, b->sweepItems.p, active);
     } else {
         hipLaunchKernelGGL((k_sweep_batch<float, false>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<float, false>), dim3(kFixupBlocks), dim3(256), b->fixLds, b->stream, A,
-                           b->P, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<float, false>), dim3(b->nSweepItems), dim3(256), b->fixLds, b->stream,
+                           A, b->sweepItems.p, active);
     }
     NRH_HIP(hipGetLastError());
     batch_finalize(b, active);
@@ -1344,13 +1343,13 @@ void batch_similarity_gradient(nrh_batch *b, const int *active) {
     if (half) {
         hipLaunchKernelGGL((k_sweep_batch<__half, true>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, true>), dim3(kFixupBlocks), dim3(256), 0, b->stream, A, b->P,
-                           active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, true>), dim3(b->nSweepItems), dim3(256), 0, b->stream, A,
+                           b->sweepItems.p, active);
     } else {
         hipLaunchKernelGGL((k_sweep_batch<float, true>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<float, true>), dim3(kFixupBlocks), dim3(256), 0, b->stream, A, b->P,
-                           active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<float, true>), dim3(b->nSweepItems), dim3(256), 0, b->stream, A,
+                           b->sweepItems.p, active);
     }
     hipLaunchKernelGGL(k_project_x_batch, dim3(b->nProjX), dim3(256), b->projXLds, b->stream,
                        (const ProjXArgs *)b->projXArgs.p, b->itProjX.p, active);

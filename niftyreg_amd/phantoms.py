@@ -146,3 +146,28 @@ def config3_pair(seed=0, dims=(128, 128, 128)):
     ref = blob_phantom(dims, seed)
     A = random_affine(seed + 2000)
     return ref, affine_warp(ref, A), A
+
+
+def config2_dims(seed):
+    """Axis sizes of a batched-F3D pair: each drawn independently from U{96..224}."""
+    r = _rng(seed)
+    return tuple(int(v) for v in r.integers(96, 225, 3))
+
+
+def config2_pair(seed):
+    """A batched-F3D pair: its axis sizes from config2_dims, content as config1_pair."""
+    return config1_pair(seed=seed, dims=config2_dims(seed))
+
+
+def config4_request(seed):
+    """A serving request: a cubic volume whose voxel count is log-uniform between 64^3 and 256^3,
+    a shell phantom, and its copy under a random affine (within 5 degrees, 3 mm, 5% scale) and a
+    B-spline warp (5-voxel spacing, U[-4, 4] voxels); the measure is NMI or LNCC with equal
+    probability.  Returns (dims, ref, flo, measure) with measure 0 (NMI) or 1 (LNCC)."""
+    r = _rng(seed)
+    side = int(round(np.exp(r.uniform(np.log(64), np.log(256)))))
+    dims = (side, side, side)
+    ref = shell_phantom(dims, seed)
+    A = random_affine(seed + 3000, rot_deg=5.0, trans_mm=3.0, scale=(0.95, 1.05), shear=0.02)
+    flo = bspline_warp(affine_warp(ref, A), 5, 4.0, seed + 4000)
+    return dims, ref, flo, int(r.integers(0, 2))

@@ -110,8 +110,8 @@ def test_objective_and_gradient_with_linear_energy_match_reference_driver(gpu):
 def test_registration_with_linear_energy_matches_reference(gpu):
     """reg_f3d -be 0.001 -le 0.01, three levels, on both platforms.  Line-search decisions depend
     on the last bits of each value, so parity is the final objective's similarity part, scored
-    with the reference's own NMI: the device may not end lower than the weaker of the reference's
-    SSE and scalar builds by more than 1e-4, and its linear energy stays of the reference's order."""
+    with the reference's own NMI: the device's may differ from the SSE build's by no more than the
+    scalar build's does plus 1e-4, and its linear energy stays of the reference's order."""
     from niftyreg_amd.f3d import register
     ref, flo = phantoms.config0_pair(0)
     g = Geom.identity((64, 64, 64))
@@ -125,6 +125,6 @@ def test_registration_with_linear_energy_matches_reference(gpu):
     g0, ident = create_control_point_grid(g, [5.0] * 3)
     print(f"LE registration: device {nmi_gpu:.6f}, reference SSE {nmi_ref:.6f}, scalar {nmi_scalar:.6f}")
     assert nmi_gpu > O.ref_nmi_at(g, ref, g, flo, ident) + 0.01
-    assert nmi_gpu >= min(nmi_ref, nmi_scalar) - 1e-4
+    assert abs(nmi_gpu - nmi_ref) <= abs(nmi_scalar - nmi_ref) + 1e-4
     le_gpu, le_ref = O.orc_le(cg, cpp), O.orc_le(rcg, rcpp)
     assert le_gpu <= 2.0 * le_ref + 1e-6, (le_gpu, le_ref)

