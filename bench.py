@@ -28,6 +28,7 @@ HBM_PEAK_GBS = 8000.0
 DIMS = (256, 256, 160)
 BINS = 64
 BE = 0.001
+SWEEP_TIMING_PERIOD = 4  # HIP-event timing of every 4th fused sweep in the timed region
 
 
 def parse():
@@ -456,6 +457,22 @@ def measured_traffic():
     return None
 
 
+VALU_PEAK_INSTS = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second: 256 CUs x 4 SIMDs, 4 cycles each
+
+
+def measured_valu():
+    """VALU instructions per gradient-sweep launch from the committed rocprofv3 SQ_INSTS_VALU
+    summary, or None."""
+    path = os.path.join(ROOT, "profiles", "f3d_sweeps_256x256x160_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("valu_insts_per_launch"):
+                return float(line.split(":")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -507,7 +524,9 @@ def main():
     for _ in range(a.warmup):
         h.objective()
         h.gradient()
-    h.set_profiling(True)
+    # HIP events around every 4th sweep of each kind (each timed sweep costs the stream a few
+    # microseconds of event overhead)
+    h.set_profiling(True, period=SWEEP_TIMING_PERIOD)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -546,6 +565,10 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic() if not storage else None,
+            "valu_frac": (measured_valu() / (grad_ms * 1e-3) / VALU_PEAK_INSTS) if measured_valu() and not storage
+            else None,
+            "valu_source": "profiles/f3d_sweeps_256x256x160_counters.md (rocprofv3 SQ_INSTS_VALU per launch) over "
+                           "the launch time, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction",
             "traffic_source": "profiles/f3d_eval_256x256x160_hbm_traffic.md (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
             "kernel": "k_sweep<float,true> (gradient sweep)",
             "kernel_ms": grad_ms, "histogram_sweep_ms": hist_ms, "algorithmic_bytes_per_launch": algo_bytes,

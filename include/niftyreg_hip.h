@@ -93,9 +93,11 @@ typedef struct nrh_f3d_params {
     int image_storage;        /* 0: float32 images in HBM, 1: float16 storage, float32 compute */
     int measure;              /* NRH_MEASURE_NMI or NRH_MEASURE_LNCC (reg-lib/MeasureCreator.hpp:8) */
     float lncc_sigma;         /* LNCC Gaussian standard deviation, < 0: voxels (reg_lncc default -5) */
-    int field_order;          /* summation order of the float deformation field wherever it decides a
-                                 floor (NRH_FIELD_ORDER_*), i.e. which reference build's rounding the
-                                 device reproduces (reg-lib/cpu/_reg_localTrans.cpp:1110-1451) */
+    int field_order;          /* NRH_FIELD_ORDER_*: which reference build's float summation orders the
+                                 device reproduces -- the B-spline field wherever it decides a floor
+                                 (reg-lib/cpu/_reg_localTrans.cpp:1110-1451), the velocity grid's
+                                 flow (:903-1079) and the density-normalised convolution
+                                 (reg-lib/cpu/_reg_tools.cpp:1018-1060) */
 } nrh_f3d_params;
 
 #define NRH_MEASURE_NMI 0
@@ -232,7 +234,8 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]);
  * similarity gradient, convolution and node projection in one device sweep.  The histogram
  * is recomputed unless the last nrh_f3d_objective ran on the same control points.  When
  * max_length is non-NULL the gradient is then normalised as reg_f3d::NormaliseGradient
- * (reg-lib/_reg_f3d.cpp:395-411) and the maximal length returned. */
+ * (reg-lib/_reg_f3d.cpp:395-411) and the maximal length returned; with max_length NULL the call
+ * only enqueues the work on the handle's stream (the downloads wait for it). */
 int nrh_f3d_gradient(nrh_f3d *h, double *max_length);
 /* 1 when the handle's geometry takes the fused sweeps (integer grid spacing >= 4 voxels with
  * nodes on voxels), 0 when objective / gradient run the per-operation chain. */
@@ -243,8 +246,10 @@ int nrh_f3d_fused_available(nrh_f3d *h, int *available);
 int nrh_f3d_fused_field(nrh_f3d *h);
 /* HIP-event timing of the two fused sweeps on the handle's stream (no reference counterpart;
  * the reference times whole calls with steady_clock, reg-apps/reg_benchmark.cpp:139-163).
- * Enabling resets the totals; nrh_f3d_sweep_times returns the accumulated milliseconds and
- * launch counts of [histogram sweep, gradient sweep]. */
+ * enable = k > 0 times every k-th sweep of each kind (1: every sweep; the events around a timed
+ * sweep cost a few microseconds of stream time), 0 stops timing.  Enabling resets the totals;
+ * nrh_f3d_sweep_times returns the accumulated milliseconds and launch counts of the timed
+ * [histogram sweeps, gradient sweeps]. */
 int nrh_f3d_set_profiling(nrh_f3d *h, int enable);
 int nrh_f3d_sweep_times(nrh_f3d *h, double ms[2], long counts[2]);
 

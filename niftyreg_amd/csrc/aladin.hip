@@ -20,6 +20,8 @@
 #include <array>
 #include <chrono>
 #include <thread>
+#include <map>
+#include <mutex>
 #include "device.hpp"
 #include "recip_x87.hpp"
 #include "m44.hpp"

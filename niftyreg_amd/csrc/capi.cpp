@@ -103,6 +103,9 @@ const float *ref_ptr(nrh_f3d *h) {
 void alloc_histogram(nrh_f3d *h) {
     const int rb = h->params.ref_bins, fb = h->params.flo_bins;
     h->histCount.alloc((size_t)rb * fb);
+    // every histogram accumulates into zeroed counts; each finalisation zeroes them again
+    NRH_HIP(hipMemsetAsync(h->histCount.p, 0, h->histCount.bytes(), h->stream));
+    NRH_HIP(hipStreamSynchronize(h->stream));
     h->histPro.alloc(h->totalBins());
     h->histLog.alloc(h->totalBins());
     h->histLogF.alloc(h->totalBins());
@@ -181,6 +184,7 @@ int nrh_f3d_create(const nrh_geom *ref, const float *ref_data, const nrh_geom *f
         NRH_HIP(hipHostMalloc((void **)&h->pinned, 16 * sizeof(double), hipHostMallocDefault));
         NRH_HIP(hipHostMalloc((void **)&h->published, 16 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped));
         NRH_HIP(hipHostGetDevicePointer((void **)&h->publishedDev, h->published, 0));
+        std::memset(h->published, 0, 16 * sizeof(double));  // published[15]: no sequence number yet
         // Images: float32 always kept (per-operation path); float16 copies added for the fused
         // path when requested.
         h->refImg.alloc(h->N);
@@ -223,6 +227,10 @@ int nrh_f3d_destroy(nrh_f3d *h) {
         for (int i = 0; i < 4; ++i)
             if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
         if (h->evPair) (void)hipEventDestroy(h->evPair);
+        if (h->side) (void)hipStreamSynchronize(h->side);
+        if (h->evFork) (void)hipEventDestroy(h->evFork);
+        if (h->evJoin) (void)hipEventDestroy(h->evJoin);
+        if (h->side) (void)hipStreamDestroy(h->side);
         hipStream_t s = h->stream;
         delete h;
         if (s) (void)hipStreamDestroy(s);

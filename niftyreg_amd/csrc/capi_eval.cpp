@@ -30,20 +30,24 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
         NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_objective");
         const nrh_f3d_params& p = h->params;
         const bool lncc = p.measure == NRH_MEASURE_LNCC;
+        // the bending energy runs in the fused histogram's exact-pass launch
+        const bool beFused = p.be_weight > 0 && p.similarity_weight > 0 && !lncc && fused_supported(h);
         if (p.similarity_weight > 0) {
             if (lncc) {
                 // reg_f3d::WarpFloatingImage then reg_lncc::GetSimilarityMeasureValueFw
                 op_spline_field(h);
                 op_resample(h);
                 op_lncc_value(h, h->scalars.p + 4);
+            } else if (beFused) {
+                fused_histogram(h, nullptr, true);
             } else {
                 histogram_at_current_points(h);
             }
         }
-        if (p.be_weight > 0) op_bending_energy(h, h->scalars.p + 0);
+        if (p.be_weight > 0 && !beFused) op_bending_energy(h, h->scalars.p + 0);
         if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
         op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
-        NRH_HIP(hipStreamSynchronize(h->stream));
+        wait_published(h);
         collect_sweep_times(h);
         const double *r = h->published;
         const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * r[4] : 0.0;
@@ -63,6 +67,8 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
         NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
         NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_gradient");
         const nrh_f3d_params& p = h->params;
+        const bool be = p.be_weight > 0, le = p.le_weight > 0;
+        bool beDone = false;
         if (p.similarity_weight > 0 && p.measure == NRH_MEASURE_LNCC) {
             // reg_f3d::GetSimilarityMeasureGradient with reg_lncc (reg-lib/_reg_f3d.cpp:353-362,
             // reg-lib/_reg_base.cpp:497-549)
@@ -76,8 +82,13 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
             op_voxel_to_node(h, p.similarity_weight);
         } else if (p.similarity_weight > 0) {
             if (fused_supported(h)) {
+                // the bending-energy gradient sums run on the side stream beside the projection
+                // passes; the node projection adds them
+                const bool trackMax = be && max_length != nullptr && !le;
+                if (trackMax) op_fill_u32(h->maxBits.p, 1, 0u, h->stream);
                 if (h->histVersion != h->cppVersion) fused_histogram(h);
-                fused_gradient(h, p.similarity_weight);
+                fused_gradient(h, p.similarity_weight, be ? p.be_weight : 0.f, trackMax ? h->maxBits.p : nullptr);
+                beDone = be;
             } else {
                 // reg_f3d::GetSimilarityMeasureGradient chain (reg-lib/_reg_f3d.cpp:353-362)
                 op_spline_field(h);
@@ -96,17 +107,16 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
             op_fill_f32(h->nodeGrad.p, 3 * h->G, 0.f, h->stream);
         }
         // reg_f3d::GetObjectiveFunctionGradient: bending energy, then linear energy
-        const bool be = p.be_weight > 0, le = p.le_weight > 0;
-        if (be) op_bending_energy_gradient(h, p.be_weight, max_length != nullptr && !le);
+        if (be && !beDone) op_bending_energy_gradient(h, p.be_weight, max_length != nullptr && !le);
         if (le) op_linear_energy_gradient(h, p.le_weight, max_length != nullptr);
         if (max_length) {
             if (!be && !le) op_max_length_bits(h);
             op_normalise_from_bits(h);
             *max_length = read_scalar(h, h->scalars.p + 1);
-        } else {
-            NRH_HIP(hipStreamSynchronize(h->stream));
+            collect_sweep_times(h);
         }
-        collect_sweep_times(h);
+        // without a length to return the call only enqueues: the gradient stays on the device,
+        // and the downloads (or the next call that returns a value) wait for it
     });
 }
 
@@ -115,7 +125,10 @@ int nrh_f3d_set_profiling(nrh_f3d *h, int enable) {
         NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
         if (enable && !h->ev[0])
             for (int i = 0; i < 4; ++i) NRH_HIP(hipEventCreate(&h->ev[i]));
+        NRH_REQUIRE(enable >= 0, NRH_ERR_ARG, "the timing period must be >= 0");
         h->profile = enable != 0;
+        h->profilePeriod = enable > 0 ? enable : 1;
+        h->profileSeen[0] = h->profileSeen[1] = 0;
         h->msSweep[0] = h->msSweep[1] = 0.0;
         h->nSweep[0] = h->nSweep[1] = 0;
         h->evPending[0] = h->evPending[1] = false;
@@ -125,6 +138,8 @@ int nrh_f3d_set_profiling(nrh_f3d *h, int enable) {
 int nrh_f3d_sweep_times(nrh_f3d *h, double ms[2], long counts[2]) {
     return guard([&] {
         NRH_REQUIRE(h && ms && counts, NRH_ERR_ARG, "null argument");
+        NRH_HIP(hipStreamSynchronize(h->stream));
+        collect_sweep_times(h);
         for (int k = 0; k < 2; ++k) {
             ms[k] = h->msSweep[k];
             counts[k] = h->nSweep[k];

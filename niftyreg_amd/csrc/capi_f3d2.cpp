@@ -87,8 +87,8 @@ int nrh_f3d2_objective(nrh_f3d *fw, nrh_f3d *bw, double *value, double terms[3])
             if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
             op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
         }
-        NRH_HIP(hipStreamSynchronize(fw->stream));
-        NRH_HIP(hipStreamSynchronize(bw->stream));
+        wait_published(fw);
+        wait_published(bw);
         const double *f = fw->published, *b = bw->published;
         const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * (f[4] + b[4]) : 0.0;
         const double wBE = p.be_weight > 0 ? (double)p.be_weight * f[5] + (double)p.be_weight * b[5] : 0.0;

@@ -108,44 +108,59 @@ __device__ __forceinline__ float3 ref_spline_point(const float *__restrict__ cpp
                                                    int py, int pz, const float xb[4], const float yb[4],
                                                    const float zb[4], int order, bool lut) {
 #pragma clang fp contract(off)
-    float3 r;
-    if (order == NRH_FIELD_ORDER_SCALAR) {
-        float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-        for (int c = 0; c < 4; ++c)
-            for (int b = 0; b < 4; ++b) {
-                const float yzw = yb[b] * zb[c];
-                const size_t row = ((size_t)(pz + c) * gny + (py + b)) * gnx + px;
-                for (int a = 0; a < 4; ++a) {
-                    const float w = xb[a] * yzw;
-                    r0 += cpp[row + a] * w;
-                    r1 += cpp[row + a + G] * w;
-                    r2 += cpp[row + a + 2 * G] * w;
-                }
+    // the 48 control-point values of each z slab are loaded before any is summed (one round of
+    // memory latency per slab); the sums run in the reference's order
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    float l0[4] = { 0.f, 0.f, 0.f, 0.f }, l1[4] = { 0.f, 0.f, 0.f, 0.f }, l2[4] = { 0.f, 0.f, 0.f, 0.f };
+    for (int c = 0; c < 4; ++c) {
+        float v[4][4][3];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const size_t row = ((size_t)(pz + c) * gny + (py + b)) * gnx + px;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                v[b][a][0] = cpp[row + a];
+                v[b][a][1] = cpp[row + a + G];
+                v[b][a][2] = cpp[row + a + 2 * G];
             }
-        r.x = r0; r.y = r1; r.z = r2;
-    } else {
-        float l0[4] = { 0.f, 0.f, 0.f, 0.f }, l1[4] = { 0.f, 0.f, 0.f, 0.f }, l2[4] = { 0.f, 0.f, 0.f, 0.f };
-        for (int c = 0; c < 4; ++c)
+        }
+        if (order == NRH_FIELD_ORDER_SCALAR) {
+#pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const float yzw = yb[b] * zb[c];
-                const size_t row = ((size_t)(pz + c) * gny + (py + b)) * gnx + px;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     const float w = xb[a] * yzw;
-                    l0[a] = w * cpp[row + a] + l0[a];
-                    l1[a] = w * cpp[row + a + G] + l1[a];
-                    l2[a] = w * cpp[row + a + 2 * G] + l2[a];
+                    r0 += v[b][a][0] * w;
+                    r1 += v[b][a][1] * w;
+                    r2 += v[b][a][2] * w;
                 }
             }
-        if (lut) {
-            r.x = (l0[0] + l0[1]) + (l0[2] + l0[3]);
-            r.y = (l1[0] + l1[1]) + (l1[2] + l1[3]);
-            r.z = (l2[0] + l2[1]) + (l2[2] + l2[3]);
         } else {
-            r.x = ((l0[0] + l0[1]) + l0[2]) + l0[3];
-            r.y = ((l1[0] + l1[1]) + l1[2]) + l1[3];
-            r.z = ((l2[0] + l2[1]) + l2[2]) + l2[3];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float yzw = yb[b] * zb[c];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const float w = xb[a] * yzw;
+                    l0[a] = w * v[b][a][0] + l0[a];
+                    l1[a] = w * v[b][a][1] + l1[a];
+                    l2[a] = w * v[b][a][2] + l2[a];
+                }
+            }
         }
+    }
+    float3 r;
+    if (order == NRH_FIELD_ORDER_SCALAR) {
+        r.x = r0; r.y = r1; r.z = r2;
+    } else if (lut) {
+        r.x = (l0[0] + l0[1]) + (l0[2] + l0[3]);
+        r.y = (l1[0] + l1[1]) + (l1[2] + l1[3]);
+        r.z = (l2[0] + l2[1]) + (l2[2] + l2[3]);
+    } else {
+        r.x = ((l0[0] + l0[1]) + l0[2]) + l0[3];
+        r.y = ((l1[0] + l1[1]) + l1[2]) + l1[3];
+        r.z = ((l2[0] + l2[1]) + l2[2]) + l2[3];
     }
     return r;
 }
@@ -263,14 +278,25 @@ __device__ __forceinline__ void nmi_voxel_term(float rv, float wv, const float g
         rl[k] = (-1 < r && r < rb) ? logh[r + refOff] : 0.0;
         wl[k] = (-1 < w && w < fb) ? logh[w + floOff] : 0.0;
     }
-    for (int kr = 0; kr < 4 && r0 + kr < r1; ++kr) {
+    // the joint log values are loaded together before the sums
+    double jlog[4][4];
+#pragma unroll
+    for (int kr = 0; kr < 4; ++kr)
+#pragma unroll
+        for (int kw = 0; kw < 4; ++kw) {
+            const int r = r0 + kr, w = w0 + kw;
+            jlog[kr][kw] = (-1 < r && r < rb && -1 < w && w < fb) ? logh[r + w * rb] : 0.0;
+        }
+#pragma unroll
+    for (int kr = 0; kr < 4; ++kr) {
         const int r = r0 + kr;
-        if (!(-1 < r && r < rb)) continue;
-        for (int kw = 0; kw < 4 && w0 + kw < w1; ++kw) {
+        if (!(r < r1) || !(-1 < r && r < rb)) continue;
+#pragma unroll
+        for (int kw = 0; kw < 4; ++kw) {
             const int w = w0 + kw;
-            if (!(-1 < w && w < fb)) continue;
+            if (!(w < w1) || !(-1 < w && w < fb)) continue;
             const double common = bR[kr] * dW[kw];
-            const double jl = logh[r + w * rb];
+            const double jl = jlog[kr][kw];
             for (int d = 0; d < 3; ++d) {
                 if (g[d] == g[d]) {
                     rd[d] += common * (double)g[d] * rl[kr];
@@ -302,6 +328,12 @@ __device__ __forceinline__ double block_sum_double(double v, double *smem) {
 __device__ __forceinline__ float wave_sum(float v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
+}
+
+// Node position in float, reg_getDisplacementFromDeformation_3D (reg-lib/cpu/_reg_tools.cpp:2004-2019)
+__device__ __forceinline__ float node_pos(const Affine& m, int r, int x, int y, int z) {
+#pragma clang fp contract(off)
+    return m.m[r * 4 + 0] * (float)x + m.m[r * 4 + 1] * (float)y + m.m[r * 4 + 2] * (float)z + m.m[r * 4 + 3];
 }
 
 }  // namespace nrh

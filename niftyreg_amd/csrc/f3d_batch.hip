@@ -187,6 +187,7 @@ void batch_touch(nrh_batch *b, bool pointsChanged) {
         } else {
             h->histVersion = h->cppVersion;
         }
+        h->kTabCurrent = false;  // the batch's finalisation leaves the pair's own table as it was
         h->pairStateVersion = 0;
         h->velFieldVersion = 0;
     }

@@ -26,6 +26,8 @@ struct nrh_batch {
     int nSweepItems = 0;
     int sweepBlocks = 0;               // persistent grid (a multiple of 8)
     size_t sweepLds = 0, fixLds = 0;
+    nrh::DevBuf<int4> fixChunks;  // exact-pass chunks: (pair, first tile, tiles, -)
+    int nFixChunks = 0;
     int rb = 0, fb = 0;
     nrh::DevBuf<uint8_t> projXArgs, projYArgs, nodeArgs;  // ProjXArgs[P], ProjYArgs[P], NodeArgs[P]
     nrh::DevBuf<int2> itProjX, itProjY, itNodes;

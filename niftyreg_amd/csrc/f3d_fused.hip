@@ -26,10 +26,11 @@
 // at all.  On the identity grid every registration starts from, every voxel is such a voxel.  The
 // sweeps therefore list each voxel whose position lies within a tolerance of a plane (above the
 // reference's own float error) instead of using it, and a second pass (k_sweep_fixup) evaluates
-// the listed voxels exactly as the reference does: its float field in the summation order of the
-// selected build, its resampling, image gradient and FP64 NMI voxel term.
+// the listed voxels as the reference does: its float field in the summation order of the selected
+// build, its resampling, image gradient and FP64 NMI voxel term.
 
 #include "device.hpp"
+#include "be_blocks.hpp"
 #include "f3d_handle.hpp"
 #include "f3d_batch.hpp"
 #include <cmath>
@@ -66,14 +67,14 @@ struct FusedArgs {
     const float *kTab;  // gradient sweep: K[r + w rb] = log p_r + log p_w - NMI log p_rw (padded to 4)
     float *gvox;      // gradient sweep: the NMI voxel gradient (planar [3][N])
     float *fieldOut;  // optional: the deformation field of the sweep (planar [3][N]), for checking
-    // voxels left to k_sweep_fixup: positions within ambTol (floating voxels) of a voxel plane;
+    // voxels left to the exact pass: positions within ambTol (floating voxels) of a voxel plane;
     // tile t lists its voxels in ambList[t * kTileVoxels ..] and their number in ambCounts[t]
     uint32_t *ambList;
     uint32_t *ambCounts;
     float ambTol;
     int fieldOrder;  // NRH_FIELD_ORDER_*
     int lut;         // 5-voxel spacing on every axis (the reference's table path)
-    const double *logh;  // the reference's jointHistogramLog (fixup of the gradient sweep)
+    const double *logh;  // the reference's jointHistogramLog (exact pass of the gradient sweep)
 };
 
 template<class T> __device__ __forceinline__ float ld(const T *p, size_t i);
@@ -253,6 +254,197 @@ __device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, fl
         g1 = (float)(S * (double)gy);
         g2 = (float)(S * (double)gz);
     }
+}
+
+// The exact pass: a voxel a sweep listed, evaluated as the reference evaluates it (float field in
+// the selected build's order, Mat44Mul, ResampleImage3D, and for the gradient
+// TrilinearImageGradient and the FP64 voxel term of reg_getVoxelBasedNmiGradient3d).
+// The 8 trilinear taps around floating position p (loaded together), padding outside the image.
+template<class TImg>
+__device__ __forceinline__ void exact_taps(const TImg *__restrict__ img, int nx, int ny, int nz, int px, int py,
+                                           int pz, float pad, float v[2][2][2]) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa) {
+                const int X = px + aa, Y = py + bb, Z = pz + c;
+                v[c][bb][aa] = (X > -1 && X < nx && Y > -1 && Y < ny && Z > -1 && Z < nz)
+                                   ? ld(img, ((size_t)Z * ny + Y) * nx + X) : pad;
+            }
+}
+
+// ResampleImage3D's trilinear value from the taps (reg-lib/cpu/_reg_resampling.cpp).
+__device__ __forceinline__ float sample_taps(const float v[2][2][2], float3 p, int px, int py, int pz) {
+#pragma clang fp contract(off)
+    float rx = p.x - (float)px, ry = p.y - (float)py, rz = p.z - (float)pz;
+    if (rx < 0) rx = 0;
+    if (ry < 0) ry = 0;
+    if (rz < 0) rz = 0;
+    const float xb[2] = { 1.f - rx, rx }, yb[2] = { 1.f - ry, ry }, zb[2] = { 1.f - rz, rz };
+    float intensity = 0.f;
+    for (int c = 0; c < 2; ++c) {
+        float yT = 0.f;
+        for (int bb = 0; bb < 2; ++bb) {
+            float xT = 0.f;
+            for (int aa = 0; aa < 2; ++aa) xT += v[c][bb][aa] * xb[aa];
+            yT += xT * yb[bb];
+        }
+        intensity += yT * zb[c];
+    }
+    return intensity;
+}
+
+// TrilinearImageGradient from the taps (reg-lib/cpu/_reg_resampling.cpp:1797-1915).
+__device__ __forceinline__ float3 gradient_taps(const float v[2][2][2], float3 p, int px, int py, int pz) {
+#pragma clang fp contract(off)
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    float r = p.x - (float)px;
+    const float xb[2] = { (float)(1.0 - (double)r), r };
+    r = p.y - (float)py;
+    const float yb[2] = { (float)(1.0 - (double)r), r };
+    r = p.z - (float)pz;
+    const float zb[2] = { (float)(1.0 - (double)r), r };
+    const float deriv[2] = { -1.f, 1.f };
+    for (int c = 0; c < 2; ++c) {
+        float xx = 0.f, yy = 0.f, zz = 0.f;
+        for (int bb = 0; bb < 2; ++bb) {
+            float xT = 0.f, yT = 0.f;
+            for (int aa = 0; aa < 2; ++aa) {
+                const float coeff = v[c][bb][aa];
+                xT += coeff * deriv[aa];
+                yT += coeff * xb[aa];
+            }
+            xx += xT * yb[bb];
+            yy += yT * deriv[bb];
+            zz += yT * yb[bb];
+        }
+        g0 += xx * zb[c];
+        g1 += yy * zb[c];
+        g2 += zz * deriv[c];
+    }
+    if (g0 != g0) g0 = 0.f;
+    if (g1 != g1) g1 = 0.f;
+    if (g2 != g2) g2 = 0.f;
+    return make_float3(g0, g1, g2);
+}
+
+// One listed voxel idx of pair a: the histogram pass counts it into the tile's LDS histogram h,
+// the gradient pass writes its voxel gradient.
+template<class TImg, bool GRAD>
+__device__ __forceinline__ void fixup_voxel(const FusedArgs& a, const uint32_t idx, uint32_t *h) {
+    const TImg *__restrict__ ref = (const TImg *)a.ref;
+    const TImg *__restrict__ flo = (const TImg *)a.flo;
+    const int x = (int)(idx % (uint32_t)a.nx), y = (int)((idx / (uint32_t)a.nx) % (uint32_t)a.ny);
+    const int z = (int)(idx / ((uint32_t)a.nx * a.ny));
+    const float4 bx = a.bX[x], by = a.bY[y], bz = a.bZ[z];
+    const float xb[4] = { bx.x, bx.y, bx.z, bx.w }, yb[4] = { by.x, by.y, by.z, by.w };
+    const float zb[4] = { bz.x, bz.y, bz.z, bz.w };
+    const float3 w = ref_spline_point(a.cpp, a.G, a.gnx, a.gny, a.preX[x], a.preY[y], a.preZ[z], xb, yb, zb,
+                                      a.fieldOrder, a.lut != 0);
+    const float3 p = affine_apply_d(a.floIjk, w.x, w.y, w.z);
+    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
+    float v[2][2][2];
+    exact_taps(flo, a.fnx, a.fny, a.fnz, px, py, pz, a.pad, v);
+    const float rv = ld(ref, idx);
+    const float war = sample_taps(v, p, px, py, pz);
+    if (!GRAD) {
+        if (rv == rv && war == war && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
+            atomicAdd(&h[(int)rv + (int)war * a.rb], 1u);
+    } else {
+        float d[3] = { 0.f, 0.f, 0.f };
+        if (rv == rv && war == war) {
+            const float3 g = gradient_taps(v, p, px, py, pz);
+            const float gg[3] = { g.x, g.y, g.z };
+            nmi_voxel_term(rv, war, gg, a.logh, a.ent, a.rb, a.fb, 1.0, d);
+        }
+        const uint32_t N = (uint32_t)a.nx * a.ny * a.nz;
+        a.gvox[idx] = d[0];
+        a.gvox[idx + N] = d[1];
+        a.gvox[idx + 2 * N] = d[2];
+    }
+}
+
+// The exact pass over the voxels that tiles [t0, t0 + nt) of pair a listed, one workgroup for
+// the chunk: the lists are concatenated (offsets in LDS) so that every thread takes listed voxels;
+// the histogram pass counts them in one LDS histogram h per workgroup and merges it once.
+constexpr int kFixChunk = 8;  // tiles per exact-pass workgroup
+
+template<class TImg, bool GRAD>
+__device__ __forceinline__ void fixup_tiles(const FusedArgs& a, const int t0, const int nt, uint32_t *h) {
+    __shared__ uint32_t sOff[kFixChunk + 1];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        uint32_t o = 0;
+        for (int k = 0; k < nt; ++k) {
+            sOff[k] = o;
+            o += a.ambCounts[t0 + k];
+        }
+        for (int k = nt; k <= kFixChunk; ++k) sOff[k] = o;
+    }
+    __syncthreads();
+    const uint32_t total = sOff[kFixChunk];
+    if (total == 0) return;  // uniform over the workgroup
+    if (!GRAD) {
+        for (int i = tid; i < a.rb * a.fb; i += blockDim.x) h[i] = 0u;
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < total; i += blockDim.x) {
+        int k = 0;
+        while (i >= sOff[k + 1]) ++k;
+        const uint32_t idx = a.ambList[(size_t)(t0 + k) * kTileVoxels + (i - sOff[k])];
+        fixup_voxel<TImg, GRAD>(a, idx, h);
+    }
+    if (!GRAD) {
+        __syncthreads();
+        for (int i = tid; i < a.rb * a.fb; i += blockDim.x)
+            if (h[i]) atomicAdd(&a.hist[i], h[i]);
+    }
+}
+
+template<class TImg, bool GRAD>
+__global__ void __launch_bounds__(256) k_sweep_fixup(FusedArgs a) {
+    extern __shared__ __align__(16) uint32_t fixLds[];  // histogram counts (objective pass)
+    const int t0 = blockIdx.x * kFixChunk;
+    fixup_tiles<TImg, GRAD>(a, t0, min(kFixChunk, a.numTiles - t0), fixLds);
+}
+
+// The histogram pass's exact pass with the bending energy in the same launch: workgroups
+// [0, be.nb) evaluate the bending-energy bricks (be_value_block, its result into be.out), the
+// others the exact-pass chunks.  The two are independent (the bricks read only the control
+// points), and both are latency-bound, so one launch overlaps them without a second stream.
+struct BeValueArgs {
+    const float *cpp;
+    int gnx, gny, gnz;
+    BrickGrid bg;
+    double *partials;
+    uint32_t *ticket;
+    double scale;
+    double *out;
+    int nb;  // bricks
+};
+
+template<class TImg>
+__global__ void __launch_bounds__(256) k_sweep_fixup_be(FusedArgs a, BeValueArgs be, BeBasis B) {
+    extern __shared__ __align__(16) uint32_t fixLds[];
+    if ((int)blockIdx.x < be.nb) {  // uniform over the workgroup
+        be_value_block(be.cpp, be.gnx, be.gny, be.gnz, be.bg, B, be.partials, be.ticket, be.scale, be.out,
+                       (int)blockIdx.x, be.nb);
+        return;
+    }
+    const int t0 = ((int)blockIdx.x - be.nb) * kFixChunk;
+    fixup_tiles<TImg, false>(a, t0, min(kFixChunk, a.numTiles - t0), fixLds);
+}
+
+// Batched: one workgroup per chunk (pair, first tile, tiles) of the exact-pass chunk list.
+template<class TImg, bool GRAD>
+__global__ void __launch_bounds__(256)
+k_sweep_fixup_batch(const FusedArgs *__restrict__ A, const int4 *__restrict__ chunks, const int *active) {
+    extern __shared__ __align__(16) uint32_t fixLds[];
+    const int4 c = chunks[blockIdx.x];
+    if (active && !active[c.x]) return;
+    fixup_tiles<TImg, GRAD>(A[c.x], c.y, c.z, fixLds);
 }
 
 // A 64 x 8 x 8 tile per workgroup.  The control-point window is staged in LDS and contracted
@@ -477,7 +669,7 @@ k_sweep_batch(const FusedArgs *__restrict__ args, const uint32_t *__restrict__ i
     const int per = (nItems + 7) / 8;
     const int lo = xcd * per, hi = min(nItems, lo + per);
     for (int i = lo + slot; i < hi; i += perXcd) {
-        const uint32_t it = items[i];
+        const uint32_t it = __builtin_amdgcn_readfirstlane(items[i]);  // uniform: scalar loads of the arguments
         const int pair = (int)(it >> 20);
         if (active && !active[pair]) continue;  // uniform over the workgroup
         sweep_tile<TImg, GRAD>(args[pair], (int)(it & 0xfffffu), S, dyn);
@@ -578,6 +770,10 @@ struct ProjYArgs {
     const int2 *rangeY;
 };
 
+// Taps of the z pass loaded per batch before any is summed: a node's whole support (four cells)
+// at 5-voxel spacing, so each thread has one round of loads in flight.
+constexpr int kTapBatch = 20;
+
 __device__ __forceinline__ void project_y_at(const ProjYArgs& p, const size_t t) {
     if (t >= (size_t)p.nz * p.gny * p.gnx) return;
     const int ix = (int)(t % p.gnx), iy = (int)((t / p.gnx) % p.gny), z = (int)(t / ((size_t)p.gnx * p.gny));
@@ -613,34 +809,80 @@ struct NodeArgs {
     Affine reorient;
     float weight;
     float *nodeGrad;
+    // optional bending-energy term (reg_spline_approxBendingEnergyGradient3D's gathered sums,
+    // computed beside the sweep): grad += beRatio * beGrad, then the points' in-place
+    // (p - node) + node round trip (reg-lib/cpu/_reg_localTrans_regul.cpp:312, 450)
+    const float *beGrad;
+    float beRatio;
+    float *cpp;
+    Affine cppXyz;
+    uint32_t *maxBits;  // optional: max length of the final gradient (float bits, atomicMax)
 };
 
-__device__ __forceinline__ void project_nodes_at(const NodeArgs& r, const size_t n) {
-    if (n >= r.G) return;
+// a + r * b rounded after the product, as the reference's separate multiply and add
+__device__ __forceinline__ float add_product(float a, float r, float b) {
+#pragma clang fp contract(off)
+    return a + r * b;
+}
+
+// sqrt(x^2 + y^2 + z^2) with every product rounded (reg_getMaximalLength)
+__device__ __forceinline__ float length3(float x, float y, float z) {
+#pragma clang fp contract(off)
+    return sqrtf(x * x + y * y + z * z);
+}
+
+// Returns the float bits of the final gradient's length at node n (0 past the grid).
+__device__ __forceinline__ uint32_t project_nodes_at(const NodeArgs& r, const size_t n) {
+    if (n >= r.G) return 0u;
     const int i = (int)(n % r.gnx), j = (int)((n / r.gnx) % r.gny), k = (int)(n / ((size_t)r.gnx * r.gny));
     const float inv = r.invNx[i] * r.invNy[j] * r.invNz[k];
     float u0 = 0.f, u1 = 0.f, u2 = 0.f;
     if (inv != 0.f) {
         const size_t plane = (size_t)r.nz * r.gny * r.gnx, col = (size_t)j * r.gnx + i;
         const int2 rz = r.rangeZ[k];
-#pragma unroll 4
-        for (int z = rz.x; z <= rz.y; ++z) {
-            const int q = k - r.preZ[z];
-            const float4 b = r.bZ[z];
-            const float w = q == 0 ? b.x : q == 1 ? b.y : q == 2 ? b.z : b.w;
-            const size_t a = (size_t)z * r.gny * r.gnx + col;
-            u0 += w * r.in[a];
-            u1 += w * r.in[a + plane];
-            u2 += w * r.in[a + 2 * plane];
+        for (int z0 = rz.x; z0 <= rz.y; z0 += kTapBatch) {
+            float v[kTapBatch][3], w[kTapBatch];
+#pragma unroll
+            for (int u = 0; u < kTapBatch; ++u) {
+                const int z = min(z0 + u, rz.y);  // clamped: taps past the range are loaded, not summed
+                const int q = k - r.preZ[z];
+                const float4 b = r.bZ[z];
+                w[u] = q == 0 ? b.x : q == 1 ? b.y : q == 2 ? b.z : b.w;
+                const size_t a = (size_t)z * r.gny * r.gnx + col;
+                v[u][0] = r.in[a];
+                v[u][1] = r.in[a + plane];
+                v[u][2] = r.in[a + 2 * plane];
+            }
+#pragma unroll
+            for (int u = 0; u < kTapBatch; ++u) {
+                if (z0 + u <= rz.y) {
+                    u0 += w[u] * v[u][0];
+                    u1 += w[u] * v[u][1];
+                    u2 += w[u] * v[u][2];
+                }
+            }
         }
         u0 *= inv;
         u1 *= inv;
         u2 *= inv;
     }
     const float *R = r.reorient.m;
-    r.nodeGrad[n] = (R[0] * u0 + R[4] * u1 + R[8] * u2) * r.weight;
-    r.nodeGrad[n + r.G] = (R[1] * u0 + R[5] * u1 + R[9] * u2) * r.weight;
-    r.nodeGrad[n + 2 * r.G] = (R[2] * u0 + R[6] * u1 + R[10] * u2) * r.weight;
+    float g0 = (R[0] * u0 + R[4] * u1 + R[8] * u2) * r.weight;
+    float g1 = (R[1] * u0 + R[5] * u1 + R[9] * u2) * r.weight;
+    float g2 = (R[2] * u0 + R[6] * u1 + R[10] * u2) * r.weight;
+    if (r.beGrad) {
+        g0 = add_product(g0, r.beRatio, r.beGrad[n]);
+        g1 = add_product(g1, r.beRatio, r.beGrad[n + r.G]);
+        g2 = add_product(g2, r.beRatio, r.beGrad[n + 2 * r.G]);
+        for (int c = 0; c < 3; ++c) {
+            const float node = node_pos(r.cppXyz, c, i, j, k);
+            r.cpp[n + c * r.G] = (r.cpp[n + c * r.G] - node) + node;
+        }
+    }
+    r.nodeGrad[n] = g0;
+    r.nodeGrad[n + r.G] = g1;
+    r.nodeGrad[n + 2 * r.G] = g2;
+    return __float_as_uint(length3(g0, g1, g2));
 }
 
 __global__ void __launch_bounds__(256) k_project_x(ProjXArgs p) {
@@ -654,7 +896,15 @@ __global__ void __launch_bounds__(256) k_project_y(ProjYArgs p) {
 }
 
 __global__ void __launch_bounds__(256) k_project_nodes(NodeArgs r) {
-    project_nodes_at(r, (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    uint32_t bm = project_nodes_at(r, (size_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    if (r.maxBits) {
+        // reg_getMaximalLength of the final gradient (lengths >= 0 order like their bit patterns)
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = __shfl_down(bm, off, 64);
+            bm = o > bm ? o : bm;
+        }
+        if ((threadIdx.x & 63) == 0) atomicMax(r.maxBits, bm);
+    }
 }
 
 // Batched projections: item (pair, block of that pair); each output element is computed by the
@@ -693,158 +943,6 @@ __global__ void __launch_bounds__(256) k_nmi_table(const double *__restrict__ lo
     K[i] = (float)((lr[i % rb] + lw[i / rb]) - nmi * logh[i]);
 }
 
-// The voxels the sweeps listed, evaluated as the reference evaluates them (float field in the
-// selected build's order, Mat44Mul, ResampleImage3D, and for the gradient TrilinearImageGradient
-// and the FP64 voxel term of reg_getVoxelBasedNmiGradient3d); the histogram pass counts them in
-// LDS and merges the counts, the gradient pass writes their voxel gradient.  Blocks past the
-// listed count exit at once.
-template<class TImg>
-__device__ __forceinline__ float sample_at(const TImg *__restrict__ img, int nx, int ny, int nz, float3 p, float pad) {
-#pragma clang fp contract(off)
-    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
-    float rx = p.x - (float)px, ry = p.y - (float)py, rz = p.z - (float)pz;
-    if (rx < 0) rx = 0;
-    if (ry < 0) ry = 0;
-    if (rz < 0) rz = 0;
-    const float xb[2] = { 1.f - rx, rx }, yb[2] = { 1.f - ry, ry }, zb[2] = { 1.f - rz, rz };
-    float intensity = 0.f;
-    for (int c = 0; c < 2; ++c) {
-        const int Z = pz + c;
-        float yT = 0.f;
-        for (int bb = 0; bb < 2; ++bb) {
-            const int Y = py + bb;
-            float xT = 0.f;
-            for (int aa = 0; aa < 2; ++aa) {
-                const int X = px + aa;
-                const float v = (X > -1 && X < nx && Y > -1 && Y < ny && Z > -1 && Z < nz)
-                                    ? ld(img, ((size_t)Z * ny + Y) * nx + X) : pad;
-                xT += v * xb[aa];
-            }
-            yT += xT * yb[bb];
-        }
-        intensity += yT * zb[c];
-    }
-    return intensity;
-}
-
-template<class TImg>
-__device__ __forceinline__ float3 gradient_at(const TImg *__restrict__ flo, int fnx, int fny, int fnz, float3 p,
-                                              float pad) {
-#pragma clang fp contract(off)
-    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
-    const int px = floor_int(p.x), py = floor_int(p.y), pz = floor_int(p.z);
-    float r = p.x - (float)px;
-    const float xb[2] = { (float)(1.0 - (double)r), r };
-    r = p.y - (float)py;
-    const float yb[2] = { (float)(1.0 - (double)r), r };
-    r = p.z - (float)pz;
-    const float zb[2] = { (float)(1.0 - (double)r), r };
-    const float deriv[2] = { -1.f, 1.f };
-    for (int c = 0; c < 2; ++c) {
-        const int Z = pz + c;
-        float xx = 0.f, yy = 0.f, zz = 0.f;
-        for (int bb = 0; bb < 2; ++bb) {
-            const int Y = py + bb;
-            float xT = 0.f, yT = 0.f;
-            for (int aa = 0; aa < 2; ++aa) {
-                const int X = px + aa;
-                const float coeff = (X > -1 && X < fnx && Y > -1 && Y < fny && Z > -1 && Z < fnz)
-                                        ? ld(flo, ((size_t)Z * fny + Y) * fnx + X) : pad;
-                xT += coeff * deriv[aa];
-                yT += coeff * xb[aa];
-            }
-            xx += xT * yb[bb];
-            yy += yT * deriv[bb];
-            zz += yT * yb[bb];
-        }
-        g0 += xx * zb[c];
-        g1 += yy * zb[c];
-        g2 += zz * deriv[c];
-    }
-    if (g0 != g0) g0 = 0.f;
-    if (g1 != g1) g1 = 0.f;
-    if (g2 != g2) g2 = 0.f;
-    return make_float3(g0, g1, g2);
-}
-
-
-// One listed voxel idx of pair a: the histogram pass counts it (into h in LDS, or straight into
-// the pair's histogram when h is null), the gradient pass writes its voxel gradient.
-template<class TImg, bool GRAD>
-__device__ __forceinline__ void fixup_voxel(const FusedArgs& a, const uint32_t idx, uint32_t *h) {
-    const TImg *__restrict__ ref = (const TImg *)a.ref;
-    const TImg *__restrict__ flo = (const TImg *)a.flo;
-    const int x = (int)(idx % (uint32_t)a.nx), y = (int)((idx / (uint32_t)a.nx) % (uint32_t)a.ny);
-    const int z = (int)(idx / ((uint32_t)a.nx * a.ny));
-    const float4 bx = a.bX[x], by = a.bY[y], bz = a.bZ[z];
-    const float xb[4] = { bx.x, bx.y, bx.z, bx.w }, yb[4] = { by.x, by.y, by.z, by.w };
-    const float zb[4] = { bz.x, bz.y, bz.z, bz.w };
-    const float3 w = ref_spline_point(a.cpp, a.G, a.gnx, a.gny, a.preX[x], a.preY[y], a.preZ[z], xb, yb, zb,
-                                      a.fieldOrder, a.lut != 0);
-    const float3 p = affine_apply_d(a.floIjk, w.x, w.y, w.z);
-    const float war = sample_at(flo, a.fnx, a.fny, a.fnz, p, a.pad);
-    const float rv = ld(ref, idx);
-    if (!GRAD) {
-        if (rv == rv && war == war && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
-            atomicAdd(&(h ? h : a.hist)[(int)rv + (int)war * a.rb], 1u);
-    } else {
-        float d[3] = { 0.f, 0.f, 0.f };
-        if (rv == rv && war == war) {
-            const float3 g = gradient_at(flo, a.fnx, a.fny, a.fnz, p, a.pad);
-            const float gg[3] = { g.x, g.y, g.z };
-            nmi_voxel_term(rv, war, gg, a.logh, a.ent, a.rb, a.fb, 1.0, d);
-        }
-        const uint32_t N = (uint32_t)a.nx * a.ny * a.nz;
-        a.gvox[idx] = d[0];
-        a.gvox[idx + N] = d[1];
-        a.gvox[idx + 2 * N] = d[2];
-    }
-}
-
-// The voxels tile t of pair a listed, one workgroup per tile; a tile with many listed voxels (an
-// identity grid lists them all) counts them in LDS first.
-template<class TImg, bool GRAD>
-__device__ __forceinline__ void fixup_tile(const FusedArgs& a, const int t, uint32_t *h) {
-    const uint32_t n = a.ambCounts[t];
-    if (n == 0) return;  // uniform over the workgroup
-    const uint32_t *list = a.ambList + (size_t)t * kTileVoxels;
-    const int tid = threadIdx.x;
-    const bool useLds = !GRAD && n > 256;
-    if (useLds) {
-        for (int i = tid; i < a.rb * a.fb; i += blockDim.x) h[i] = 0u;
-        __syncthreads();
-    }
-    for (uint32_t i = tid; i < n; i += blockDim.x) fixup_voxel<TImg, GRAD>(a, list[i], useLds ? h : nullptr);
-    if (useLds) {
-        __syncthreads();
-        for (int i = tid; i < a.rb * a.fb; i += blockDim.x)
-            if (h[i]) atomicAdd(&a.hist[i], h[i]);
-    }
-}
-
-template<class TImg, bool GRAD>
-__global__ void __launch_bounds__(256) k_sweep_fixup(FusedArgs a) {
-    extern __shared__ __align__(16) float fixLds[];  // histogram counts (objective pass)
-    fixup_tile<TImg, GRAD>(a, blockIdx.x, (uint32_t *)fixLds);
-}
-
-// Batched: one workgroup per (pair, tile) item of the sweeps' work list.
-template<class TImg, bool GRAD>
-__global__ void __launch_bounds__(256)
-k_sweep_fixup_batch(const FusedArgs *__restrict__ A, const uint32_t *__restrict__ items, const int *active) {
-    extern __shared__ __align__(16) float fixLds[];
-    const uint32_t it = items[blockIdx.x];
-    const int pair = (int)(it >> 20);
-    if (active && !active[pair]) return;
-    fixup_tile<TImg, GRAD>(A[pair], (int)(it & 0xfffffu), (uint32_t *)fixLds);
-}
-
-// Per pair: zero the joint-histogram counts (block = pair).
-__global__ void k_clear_counts_batch(const FusedArgs *__restrict__ A, const int *active) {
-    if (active && !active[blockIdx.x]) return;
-    const FusedArgs& a = A[blockIdx.x];
-    for (int i = threadIdx.x; i < a.rb * a.fb; i += blockDim.x) a.hist[i] = 0u;
-}
 
 // Per pair (blockIdx.y): the NMI table K of the gradient sweep.
 __global__ void __launch_bounds__(256) k_nmi_table_batch(const FusedArgs *__restrict__ A, const int *active) {
@@ -1116,31 +1214,54 @@ static size_t sweep_lds(const FusedArgs& a) {
     return std::max<size_t>(((size_t)a.rb * a.fb + 3) / 4 * 4, (size_t)3 * (WZM + TZ) * WYM * WXM) * sizeof(float);
 }
 
-void fused_histogram(nrh_f3d *h, float *fieldOut) {
+void fused_histogram(nrh_f3d *h, float *fieldOut, bool beValue) {
     FusedPlan *p = plan_of(h);
     NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
     FusedArgs a = make_args(h, p);
     a.hist = h->histCount.p;
     a.fieldOut = fieldOut;
-    op_fill_u32(h->histCount.p, (size_t)a.rb * a.fb, 0u, h->stream);
+    // histCount is zero here: allocation zeroes it and every finalisation consumes it
     const size_t lds = sweep_lds(a);
-    const size_t ldsFix = (size_t)a.rb * a.fb * sizeof(uint32_t);
-    if (h->profile) NRH_HIP(hipEventRecord(h->ev[0], h->stream));
+    const bool timed = h->profile && h->profileSeen[0]++ % h->profilePeriod == 0;
+    if (timed) NRH_HIP(hipEventRecord(h->ev[0], h->stream));
     if (h->params.image_storage == 1)
         hipLaunchKernelGGL((k_sweep<__half, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     else
         hipLaunchKernelGGL((k_sweep<float, false>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     NRH_HIP(hipGetLastError());
-    if (h->profile) {
+    if (timed) {
         NRH_HIP(hipEventRecord(h->ev[1], h->stream));
         h->evPending[0] = true;
     }
-    if (h->params.image_storage == 1)
-        hipLaunchKernelGGL((k_sweep_fixup<__half, false>), dim3(p->numTiles), dim3(256), ldsFix, h->stream, a);
-    else
-        hipLaunchKernelGGL((k_sweep_fixup<float, false>), dim3(p->numTiles), dim3(256), ldsFix, h->stream, a);
+    const unsigned fixBlocks = (unsigned)((p->numTiles + kFixChunk - 1) / kFixChunk);
+    const size_t fixLds = (size_t)a.rb * a.fb * sizeof(uint32_t);
+    if (beValue) {
+        BeValueArgs be;
+        be.cpp = h->cppPos.p;
+        be.gnx = h->cpp.nx;
+        be.gny = h->cpp.ny;
+        be.gnz = h->cpp.nz;
+        be.bg = be_brick_grid(h->cpp.nx, h->cpp.ny, h->cpp.nz);
+        be.nb = be.bg.bx * be.bg.by * be.bg.bz;
+        h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)be.nb + 64));
+        be.partials = h->partials.p;
+        be.ticket = h->maxBits.p + 6;
+        be.scale = 1.0 / (double)(3 * h->G);
+        be.out = h->scalars.p + 0;
+        const dim3 grid(fixBlocks + (unsigned)be.nb);
+        if (h->params.image_storage == 1)
+            hipLaunchKernelGGL(k_sweep_fixup_be<__half>, grid, dim3(256), fixLds, h->stream, a, be, be_basis());
+        else
+            hipLaunchKernelGGL(k_sweep_fixup_be<float>, grid, dim3(256), fixLds, h->stream, a, be, be_basis());
+    } else if (h->params.image_storage == 1) {
+        hipLaunchKernelGGL((k_sweep_fixup<__half, false>), dim3(fixBlocks), dim3(256), fixLds, h->stream, a);
+    } else {
+        hipLaunchKernelGGL((k_sweep_fixup<float, false>), dim3(fixBlocks), dim3(256), fixLds, h->stream, a);
+    }
     NRH_HIP(hipGetLastError());
-    op_nmi_finalize(h);
+    // the finalisation also writes the gradient sweep's NMI table
+    p->kTab.alloc((size_t)(a.rb * a.fb + 3) / 4 * 4);
+    op_nmi_finalize(h, p->kTab.p);
     h->histVersion = h->cppVersion;
 }
 
@@ -1158,7 +1279,7 @@ static void projection_args(nrh_f3d *h, FusedPlan *p, const float *g, float weig
     px.nx = nx;
     px.gnx = gnx;
     px.rows = (size_t)ny * nz;
-    px.rowsPerBlock = std::max(1, 2048 / nx);  // rows: <= 24 KB of LDS for nx <= 2048
+    px.rowsPerBlock = std::max(1, 1024 / nx);  // rows: <= 12 KB of LDS for nx <= 1024 (six blocks per CU)
     px.pre0 = p->pre0X;
     px.cells = p->cellsX;
     px.cellX = p->cellX.p;
@@ -1187,26 +1308,43 @@ static void projection_args(nrh_f3d *h, FusedPlan *p, const float *g, float weig
     r.reorient = to_affine(h->flo.ijk);
     r.weight = node_weight_ratio(h, weight);
     r.nodeGrad = h->nodeGrad.p;
+    r.beGrad = nullptr;
+    r.maxBits = nullptr;
 }
 
 // The voxel gradient g (planar [3][N]) projected onto the nodes: x pass (rows staged in LDS,
 // cell sums), y pass, z pass with normalisation, reorientation and weight, into h->nodeGrad.
-static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float weight) {
+// beWeight > 0: the bending-energy sums the side stream computes (op_bending_energy_gradient_raw)
+// are joined in and added by the z pass, which also round-trips the points and, with maxBits,
+// tracks the final gradient's max length.
+static void project_to_nodes(nrh_f3d *h, FusedPlan *p, const float *g, float weight, float beWeight = 0.f,
+                             uint32_t *maxBits = nullptr) {
     ProjXArgs px;
     ProjYArgs py;
     NodeArgs r;
     size_t ldsX = 0;
     projection_args(h, p, g, weight, px, py, r, &ldsX);
+    if (beWeight > 0.f) {
+        // the bending-energy gradient sums on the side stream beside the x and y passes
+        fork_side(h);
+        op_bending_energy_gradient_raw(h, h->side);
+        r.beGrad = h->beGrad.p;
+        r.beRatio = beWeight / (float)h->G;
+        r.cpp = h->cppPos.p;
+        r.cppXyz = to_affine(h->cpp.xyz);
+        r.maxBits = maxBits;
+    }
     hipLaunchKernelGGL(k_project_x, dim3((unsigned)((px.rows + px.rowsPerBlock - 1) / px.rowsPerBlock)), dim3(256),
                        ldsX, h->stream, px);
     // xcd_tile needs a grid that is a multiple of 8 (extra blocks find no work)
     auto blocks8 = [](size_t n) { return dim3((unsigned)(((n + 255) / 256 + 7) / 8 * 8)); };
     hipLaunchKernelGGL(k_project_y, blocks8((size_t)py.nz * py.gny * py.gnx), dim3(256), 0, h->stream, py);
+    if (beWeight > 0.f) join_side(h);
     hipLaunchKernelGGL(k_project_nodes, blocks8(h->G), dim3(256), 0, h->stream, r);
     NRH_HIP(hipGetLastError());
 }
 
-void fused_gradient(nrh_f3d *h, float weight) {
+void fused_gradient(nrh_f3d *h, float weight, float beWeight, uint32_t *maxBits) {
     FusedPlan *p = plan_of(h);
     NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
     FusedArgs a = make_args(h, p);
@@ -1215,26 +1353,30 @@ void fused_gradient(nrh_f3d *h, float weight) {
     a.gvox = p->gvox.p;
     const int nbins = a.rb * a.fb;
     p->kTab.alloc((size_t)(nbins + 3) / 4 * 4);
-    hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, h->histLog.p, a.ent, a.rb,
-                       a.fb, p->kTab.p);
+    if (!h->kTabCurrent)  // the histogram came from another path: the table from its logs
+        hipLaunchKernelGGL(k_nmi_table, dim3((nbins + 255) / 256), dim3(256), 0, h->stream, h->histLog.p, a.ent, a.rb,
+                           a.fb, p->kTab.p);
     a.kTab = p->kTab.p;
     const size_t lds = sweep_lds(a);
-    if (h->profile) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
+    const bool timed = h->profile && h->profileSeen[1]++ % h->profilePeriod == 0;
+    if (timed) NRH_HIP(hipEventRecord(h->ev[2], h->stream));
     if (h->params.image_storage == 1)
         hipLaunchKernelGGL((k_sweep<__half, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     else
         hipLaunchKernelGGL((k_sweep<float, true>), dim3(grid_for(p->numTiles)), dim3(kThreads), lds, h->stream, a);
     NRH_HIP(hipGetLastError());
-    if (h->profile) {
+    if (timed) {
         NRH_HIP(hipEventRecord(h->ev[3], h->stream));
         h->evPending[1] = true;
     }
+    const dim3 fixGrid((unsigned)((p->numTiles + kFixChunk - 1) / kFixChunk));
     if (h->params.image_storage == 1)
-        hipLaunchKernelGGL((k_sweep_fixup<__half, true>), dim3(p->numTiles), dim3(256), 0, h->stream, a);
+        hipLaunchKernelGGL((k_sweep_fixup<__half, true>), fixGrid, dim3(256), 0, h->stream, a);
     else
-        hipLaunchKernelGGL((k_sweep_fixup<float, true>), dim3(p->numTiles), dim3(256), 0, h->stream, a);
+        hipLaunchKernelGGL((k_sweep_fixup<float, true>), fixGrid, dim3(256), 0, h->stream, a);
     NRH_HIP(hipGetLastError());
-    project_to_nodes(h, p, p->gvox.p, weight);
+    project_to_nodes(h, p, p->gvox.p, weight, beWeight, maxBits);
+    if (beWeight > 0.f) h->cppVersion++;  // the round trip may have changed the last bit of some points
 }
 
 // (a grid extension 0 puts the nodes off the voxel lattice: the convolution path handles it)
@@ -1255,7 +1397,8 @@ void batch_fused_setup(nrh_batch *b) {
     std::vector<NodeArgs> nds(b->P);
     std::vector<uint32_t> items;
     std::vector<int2> itX, itY, itN;
-    b->sweepLds = b->fixLds = b->projXLds = 0;
+    b->sweepLds = b->projXLds = b->fixLds = 0;
+    std::vector<int4> fixChunks;
     const int storage = b->pairs[0]->params.image_storage;
     for (int q = 0; q < b->P; ++q) {
         nrh_f3d *h = b->pairs[q];
@@ -1273,8 +1416,10 @@ void batch_fused_setup(nrh_batch *b) {
         a.kTab = p->kTab.p;
         args[q] = a;
         b->sweepLds = std::max(b->sweepLds, sweep_lds(a));
-        b->fixLds = std::max(b->fixLds, (size_t)nbins * sizeof(uint32_t));
         for (int t = 0; t < p->numTiles; ++t) items.push_back(((uint32_t)q << 20) | (uint32_t)t);
+        for (int t = 0; t < p->numTiles; t += kFixChunk)
+            fixChunks.push_back(make_int4(q, t, std::min(kFixChunk, p->numTiles - t), 0));
+        b->fixLds = std::max(b->fixLds, (size_t)nbins * sizeof(uint32_t));
         size_t ldsX = 0;
         projection_args(h, p, p->gvox.p, h->params.similarity_weight, pxs[q], pys[q], nds[q], &ldsX);
         b->projXLds = std::max(b->projXLds, ldsX);
@@ -1303,6 +1448,10 @@ void batch_fused_setup(nrh_batch *b) {
     uploadItems(b->itProjX, itX, b->nProjX);
     uploadItems(b->itProjY, itY, b->nProjY);
     uploadItems(b->itNodes, itN, b->nNodes);
+    b->fixChunks.alloc(fixChunks.size());
+    NRH_HIP(hipMemcpyAsync(b->fixChunks.p, fixChunks.data(), fixChunks.size() * sizeof(int4), hipMemcpyHostToDevice,
+                           b->stream));
+    b->nFixChunks = (int)fixChunks.size();
     b->sweepItems.alloc(items.size());
     NRH_HIP(hipMemcpyAsync(b->sweepItems.p, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                            b->stream));
@@ -1319,18 +1468,18 @@ void batch_set_sweep_blocks(nrh_batch *b, int blocksPerXcd) {
 
 void batch_histogram(nrh_batch *b, const int *active) {
     const FusedArgs *A = (const FusedArgs *)b->fusedArgs.p;
-    hipLaunchKernelGGL(k_clear_counts_batch, dim3(b->P), dim3(256), 0, b->stream, A, active);
+    // the pairs' counts are zero here: allocation zeroes them and every finalisation consumes them
     const bool half = b->pairs[0]->params.image_storage == 1;
     if (half) {
         hipLaunchKernelGGL((k_sweep_batch<__half, false>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, false>), dim3(b->nSweepItems), dim3(256), b->fixLds, b->stream,
-                           A, b->sweepItems.p, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, false>), dim3(b->nFixChunks), dim3(256), b->fixLds, b->stream,
+                           A, b->fixChunks.p, active);
     } else {
         hipLaunchKernelGGL((k_sweep_batch<float, false>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<float, false>), dim3(b->nSweepItems), dim3(256), b->fixLds, b->stream,
-                           A, b->sweepItems.p, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<float, false>), dim3(b->nFixChunks), dim3(256), b->fixLds, b->stream,
+                           A, b->fixChunks.p, active);
     }
     NRH_HIP(hipGetLastError());
     batch_finalize(b, active);
@@ -1343,13 +1492,13 @@ void batch_similarity_gradient(nrh_batch *b, const int *active) {
     if (half) {
         hipLaunchKernelGGL((k_sweep_batch<__half, true>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, true>), dim3(b->nSweepItems), dim3(256), 0, b->stream, A,
-                           b->sweepItems.p, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<__half, true>), dim3(b->nFixChunks), dim3(256), 0, b->stream, A,
+                           b->fixChunks.p, active);
     } else {
         hipLaunchKernelGGL((k_sweep_batch<float, true>), dim3(b->sweepBlocks), dim3(kThreads), b->sweepLds, b->stream,
                            A, b->sweepItems.p, b->nSweepItems, active);
-        hipLaunchKernelGGL((k_sweep_fixup_batch<float, true>), dim3(b->nSweepItems), dim3(256), 0, b->stream, A,
-                           b->sweepItems.p, active);
+        hipLaunchKernelGGL((k_sweep_fixup_batch<float, true>), dim3(b->nFixChunks), dim3(256), 0, b->stream, A,
+                           b->fixChunks.p, active);
     }
     hipLaunchKernelGGL(k_project_x_batch, dim3(b->nProjX), dim3(256), b->projXLds, b->stream,
                        (const ProjXArgs *)b->projXArgs.p, b->itProjX.p, active);

@@ -18,6 +18,12 @@ struct nrh_f3d {
 
     // control points (planar [3][G]) and optimiser arrays
     nrh::DevBuf<float> cppPos, best, nodeGrad, cgG, cgH;
+    nrh::DevBuf<float> beGrad;  // bending-energy gradient sums computed beside the gradient sweep [3][G]
+
+    // a second stream for work that overlaps the sweeps (the bending energy), forked from and
+    // joined back to `stream` with the two events
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
 
     // per-operation buffers, allocated on first use
     nrh::DevBuf<float> field, warped, warpedGrad, voxelGrad;
@@ -71,13 +77,17 @@ struct nrh_f3d {
 
     double *pinned = nullptr;  // host-pinned scalar read-back
     double *published = nullptr, *publishedDev = nullptr;  // coherent host memory the device writes results to
+    uint64_t publishSeq = 0;  // sequence number of the last publish (published[15] once it lands)
 
     uint64_t cppVersion = 1;   // bumped on every control-point change
     uint64_t histVersion = 0;  // cppVersion the histogram was computed at
+    bool kTabCurrent = false;  // the last finalisation also wrote the fused sweep's NMI table
     bool halfDirty = true;     // float16 copies of the images need refreshing
 
     // optional HIP-event timing of the fused sweeps (on this handle's stream)
     bool profile = false;
+    int profilePeriod = 1;        // time every profilePeriod-th sweep of each kind
+    long profileSeen[2] = { 0, 0 };
     hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
     bool evPending[2] = { false, false };
     double msSweep[2] = { 0.0, 0.0 };  // accumulated: histogram sweep, gradient sweep
@@ -99,11 +109,17 @@ void op_spline_field(nrh_f3d *h);
 void op_resample(nrh_f3d *h);
 void op_image_gradient(nrh_f3d *h);
 void op_joint_histogram(nrh_f3d *h, const float *warped);
-void op_nmi_finalize(nrh_f3d *h);
+// kTab (optional): also the fused gradient sweep's NMI table, when the histogram fits the LDS
+// finalisation (h->kTabCurrent says whether it was written)
+void op_nmi_finalize(nrh_f3d *h, float *kTab = nullptr);
 void op_nmi_voxel_gradient(nrh_f3d *h, double weight);
 void op_convolve_voxel_gradient(nrh_f3d *h);
 void op_voxel_to_node(nrh_f3d *h, float weight);
-void op_bending_energy(nrh_f3d *h, double *out_device);
+// stream: where the launch goes (null: the handle's stream)
+void op_bending_energy(nrh_f3d *h, double *out_device, hipStream_t stream = nullptr);
+// the bending-energy gradient's gathered sums alone into h->beGrad (no weight, no round trip),
+// for a projection that adds them (fused_gradient's BE overlap)
+void op_bending_energy_gradient_raw(nrh_f3d *h, hipStream_t stream);
 // trackMax: also leave the max length of the final gradient in maxBits[0] (float bits)
 void op_bending_energy_gradient(nrh_f3d *h, float weight, bool trackMax = false);
 void op_max_length(nrh_f3d *h);
@@ -119,10 +135,12 @@ void op_cg(nrh_f3d *h, int first);
 void op_cg_pair(nrh_f3d *fw, nrh_f3d *bw, int first);  // synchronous
 void op_update_dof(nrh_f3d *h, float scale);
 double read_scalar(nrh_f3d *h, const double *dev);
-// Copies n0, n1, n2 doubles from s0, s1, s2 (device) to h->published[0..] with one small kernel
-// (no copy engine); the caller synchronises the stream before reading.
+// Copies n0, n1, n2 doubles (at most 15 in all) from s0, s1, s2 (device) to h->published[0..]
+// with one small kernel (no copy engine), then a new sequence number to published[15];
+// wait_published returns once the host sees it (every earlier launch on the stream has finished).
 void op_publish(nrh_f3d *h, const double *s0, int n0, const double *s1 = nullptr, int n1 = 0,
                 const double *s2 = nullptr, int n2 = 0);
+void wait_published(nrh_f3d *h);
 
 // LNCC measure (lncc.hip); value = mean |local correlation| into out_device[0] (count in [1])
 void op_lncc_value(nrh_f3d *h, double *out_device);
@@ -145,8 +163,15 @@ void op_symmetrise_velocity(nrh_f3d *fw, nrh_f3d *bw);
 // fused evaluation (f3d_fused.hip)
 bool fused_supported(nrh_f3d *h);
 void fused_release(const nrh_f3d *h);
-void fused_histogram(nrh_f3d *h, float *fieldOut = nullptr);
-void fused_gradient(nrh_f3d *h, float weight);
+// beValue: also the bending energy (into scalars[0]), in the exact pass's launch
+void fused_histogram(nrh_f3d *h, float *fieldOut = nullptr, bool beValue = false);
+// beWeight > 0: the bending-energy gradient sums run on the side stream beside the x and y
+// projection passes (op_bending_energy_gradient_raw); the node projection joins and adds them,
+// round-trips the points and, with maxBits, tracks the final gradient's max length
+void fused_gradient(nrh_f3d *h, float weight, float beWeight = 0.f, uint32_t *maxBits = nullptr);
+// work on h->side after everything enqueued on h->stream so far / h->stream after h->side's work
+void fork_side(nrh_f3d *h);
+void join_side(nrh_f3d *h);
 void collect_sweep_times(nrh_f3d *h);  // after a stream synchronisation
 // The separable node projection of a voxel gradient (integer spacing, nodes on voxels): the
 // convolution + VoxelCentricToNodeCentric of the reference as three passes, into nodeGrad.

@@ -5,6 +5,7 @@
 // The hot loop of a registration uses the fused kernels of f3d_fused.hip instead.
 
 #include "device.hpp"
+#include "be_blocks.hpp"
 #include "m33.hpp"
 #include "m44.hpp"
 #include "host.hpp"
@@ -110,12 +111,16 @@ k_joint_histogram(const float *__restrict__ ref, const float *__restrict__ war, 
 // Smoothing, normalisation, marginals and entropies of reg_getNmiValue (reg-lib/cpu/_reg_nmi.cpp:249-344)
 // in FP64, one workgroup.  ent: H_r, H_f, H_rf, total, nmi.
 __global__ void __launch_bounds__(1024)
-k_nmi_finalize(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+k_nmi_finalize(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
                double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent) {
     __shared__ double red[16];
     const int nb = rb * fb, total = nb + rb + fb;
     const double k0 = 1.0 / 6.0, k1 = (double)(2.f / 3.f);  // GetBasisSplineValue(-1), (0)
-    for (int i = threadIdx.x; i < total; i += blockDim.x) pro[i] = i < nb ? (double)counts[i] : 0.0;
+    // the counts are consumed: zeroed as they are read, ready for the next histogram
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        pro[i] = i < nb ? (double)counts[i] : 0.0;
+        if (i < nb) counts[i] = 0u;
+    }
     __syncthreads();
     // along the reference axis into logh
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
@@ -184,20 +189,32 @@ k_nmi_finalize(const uint32_t *__restrict__ counts, int rb, int fb, double *__re
     }
 }
 
+// K[i] = log p_r + log p_w - NMI log p_rw for joint bin i = r + w rb, from the logs L (joint
+// [0, nb), reference marginal [nb, nb + rb), floating marginal after), rounded once to float.
+__device__ __forceinline__ float nmi_table_entry(const double *L, int nb, int rb, double nmi, int i) {
+#pragma clang fp contract(off)
+    return (float)((L[nb + i % rb] + L[nb + rb + i / rb]) - nmi * L[i]);
+}
+
 // The same finalisation with the working array in LDS (histograms of up to kFinLds entries):
 // identical arithmetic and thread partition, no global-memory round trips between phases.
 constexpr int kFinLds = 7936;
 constexpr int kFinPer = (kFinLds + 1023) / 1024;
 
 __device__ __forceinline__ void
-nmi_finalize_lds_block(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
-                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent) {
+nmi_finalize_lds_block(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
+                   float *__restrict__ kTab = nullptr) {
     __shared__ double P[kFinLds];
     __shared__ double red[16];
     __shared__ double totalS;
     const int nb = rb * fb, total = nb + rb + fb;
     const double k0 = 1.0 / 6.0, k1 = (double)(2.f / 3.f);
-    for (int i = threadIdx.x; i < total; i += blockDim.x) P[i] = i < nb ? (double)counts[i] : 0.0;
+    // the counts are consumed: zeroed as they are read, ready for the next histogram
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        P[i] = i < nb ? (double)counts[i] : 0.0;
+        if (i < nb) counts[i] = 0u;
+    }
     __syncthreads();
     double v[kFinPer];
     // along the reference axis
@@ -274,6 +291,7 @@ nmi_finalize_lds_block(const uint32_t *__restrict__ counts, int rb, int fb, doub
         pro[i] = p;
         logh[i] = l;
         logf[i] = (float)l;
+        P[i] = l;  // the logs stay in LDS for the table K
     }
     hr = block_sum_double(hr, red);
     __syncthreads();
@@ -286,6 +304,14 @@ nmi_finalize_lds_block(const uint32_t *__restrict__ counts, int rb, int fb, doub
         ent[2] = hj;
         ent[3] = act;
         ent[4] = (hr + hf) / hj;
+        totalS = (hr + hf) / hj;  // the NMI, for K
+    }
+    if (kTab) {
+        // the fused gradient sweep's table K[r + w rb] = log p_r + log p_w - NMI log p_rw (as
+        // k_nmi_table computes it, from the same FP64 logs)
+        __syncthreads();
+        const double nmi = totalS;
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) kTab[i] = nmi_table_entry(P, nb, rb, nmi, i);
     }
 }
 
@@ -324,7 +350,8 @@ k_conv_prepare(float *__restrict__ data, const int *__restrict__ mask, size_t N,
 // scalar path: kernel-weighted sums in double over the in-range taps).
 __global__ void __launch_bounds__(kBlock)
 k_conv_axis(const float *__restrict__ data, const float *__restrict__ dens, int nx, int ny, int nz, int axis,
-            const float *__restrict__ kernel, int radius, float *__restrict__ outData, float *__restrict__ outDens) {
+            const float *__restrict__ kernel, int radius, float *__restrict__ outData, float *__restrict__ outDens,
+            int order) {
 #pragma clang fp contract(off)
     const size_t N = (size_t)nx * ny * nz;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -339,7 +366,23 @@ k_conv_axis(const float *__restrict__ data, const float *__restrict__ dens, int 
     if (pre < 0) { k0 = -pre; pre = 0; }
     if (pst > L) pst = L;
     double is = 0, ds = 0;
-    for (int k = pre, kk = k0; k < pst; ++k, ++kk) {
+    int k = pre, kk = k0;
+    if (order == NRH_FIELD_ORDER_SSE) {
+        // the SSE build (reg-lib/cpu/_reg_tools.cpp:1018-1049): blocks of four taps summed in four
+        // float lanes (lane j takes the block's tap 3 - j), the lanes reduced as SSE3's two
+        // horizontal adds do, (l0 + l1) + (l2 + l3); the last < 4 taps added in double
+        float li[4] = { 0.f, 0.f, 0.f, 0.f }, ld[4] = { 0.f, 0.f, 0.f, 0.f };
+        for (; k < pst - 3; k += 4, kk += 4)
+            for (int j = 0; j < 4; ++j) {
+                const float kv = kernel[kk + 3 - j];
+                const size_t o = base + (size_t)(k + 3 - j) * stride;
+                li[j] = kv * data[o] + li[j];
+                ld[j] = kv * dens[o] + ld[j];
+            }
+        is = (double)((li[0] + li[1]) + (li[2] + li[3]));
+        ds = (double)((ld[0] + ld[1]) + (ld[2] + ld[3]));
+    }
+    for (; k < pst; ++k, ++kk) {
         const float kv = kernel[kk];
         is += (double)(kv * data[base + (size_t)k * stride]);
         ds += (double)(kv * dens[base + (size_t)k * stride]);
@@ -400,122 +443,16 @@ k_voxel_to_node(const float *__restrict__ vg, int nx, int ny, int nz, int gnx, i
 }
 
 // ------------------------------------------------------------------ bending energy
-struct BeBasis {
-    float b[6][27];  // xx, yy, zz, xy, yz, xz
-};
-
-// Node position in float, reg_getDisplacementFromDeformation_3D (reg-lib/cpu/_reg_tools.cpp:2004-2019)
-__device__ __forceinline__ float node_pos(const Affine& m, int r, int x, int y, int z) {
-#pragma clang fp contract(off)
-    return m.m[r * 4 + 0] * (float)x + m.m[r * 4 + 1] * (float)y + m.m[r * 4 + 2] * (float)z + m.m[r * 4 + 3];
-}
-
-// Bending energy on bricks of 8 x 8 x 4 nodes (one workgroup, one node per thread): the control
-// points of the brick and its halo are staged in LDS once, so every 27-neighbour stencil reads
-// LDS instead of global memory.
-constexpr int BBX = 8, BBY = 8, BBZ = 4;
-
-struct BrickGrid {
-    int bx, by, bz;  // bricks per axis
-};
-
-__device__ __forceinline__ void brick_origin(const BrickGrid& g, int b, int& x0, int& y0, int& z0) {
-    x0 = (b % g.bx) * BBX;
-    y0 = ((b / g.bx) % g.by) * BBY;
-    z0 = (b / (g.bx * g.by)) * BBZ;
-}
-
-// reg_spline_approxBendingEnergyValue3D (reg-lib/cpu/_reg_localTrans_regul.cpp:73-153) over the
-// interior nodes of the brick; FP64 block partials, summed in block order by the last block.
-__device__ __forceinline__ void
-be_value_block(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, BeBasis B,
-           double *__restrict__ partials, uint32_t *__restrict__ ticket, double scale, double *__restrict__ out, int bid, int nbk) {
-#pragma clang fp contract(off)
-    constexpr int HX = BBX + 2, HY = BBY + 2, HZ = BBZ + 2, HN = HX * HY * HZ;
-    __shared__ float sC[3][HN];  // deformation values of the brick and a one-node halo
-    __shared__ float sB[27][8];  // basis values per neighbour (xx, yy, zz, xy, yz, xz), read as broadcasts
-    __shared__ double red[16];
-    __shared__ bool last;
-    const size_t G = (size_t)gnx * gny * gnz;
-    int x0, y0, z0;
-    brick_origin(bg, bid, x0, y0, z0);
-    if (threadIdx.x < 27 * 6) sB[threadIdx.x / 6][threadIdx.x % 6] = B.b[threadIdx.x % 6][threadIdx.x / 6];
-    {
-        constexpr int kIter = (3 * HN + 255) / 256;
-        float v[kIter];
-#pragma unroll
-        for (int u = 0; u < kIter; ++u) {
-            // clamped, unconditional loads keep v in registers
-            const int i = min((int)threadIdx.x + u * 256, 3 * HN - 1);
-            const int c = i / HN, r = i % HN, hx = r % HX, hy = (r / HX) % HY, hz = r / (HX * HY);
-            const int X = min(max(x0 - 1 + hx, 0), gnx - 1), Y = min(max(y0 - 1 + hy, 0), gny - 1),
-                      Z = min(max(z0 - 1 + hz, 0), gnz - 1);
-            v[u] = cpp[c * G + ((size_t)Z * gny + Y) * gnx + X];
-        }
-#pragma unroll
-        for (int u = 0; u < kIter; ++u) {
-            const int i = threadIdx.x + u * 256;
-            if (i < 3 * HN) sC[i / HN][i % HN] = v[u];
-        }
-    }
-    __syncthreads();
-    const int tx = threadIdx.x % BBX, ty = (threadIdx.x / BBX) % BBY, tz = threadIdx.x / (BBX * BBY);
-    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
-    double s = 0;
-    if (x > 0 && y > 0 && z > 0 && x < gnx - 1 && y < gny - 1 && z < gnz - 1) {
-        float d[3][6] = {};
-        int i = 0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-#pragma unroll
-                for (int a = 0; a < 3; ++a, ++i) {
-                    const int h = ((tz + c) * HY + ty + b) * HX + tx + a;
-                    float bt[6];
-#pragma unroll
-                    for (int t = 0; t < 6; ++t) bt[t] = sB[i][t];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const float cv = sC[k][h];
-#pragma unroll
-                        for (int t = 0; t < 6; ++t) d[k][t] += bt[t] * cv;
-                    }
-                }
-        s = (double)(d[0][0] * d[0][0] + d[0][1] * d[0][1] + d[0][2] * d[0][2]) +
-            2.0 * (double)(d[0][3] * d[0][3] + d[0][4] * d[0][4] + d[0][5] * d[0][5]);
-        for (int k = 1; k < 3; ++k) {
-            s += (double)(d[k][0] * d[k][0]);
-            s += (double)(d[k][1] * d[k][1]);
-            s += (double)(d[k][2] * d[k][2]);
-            s += 2.0 * (double)(d[k][3] * d[k][3] + d[k][4] * d[k][4] + d[k][5] * d[k][5]);
-        }
-    }
-    s = block_sum_double(s, red);
-    // the last block to finish sums the per-block partials in block order
-    if (threadIdx.x == 0) {
-        partials[bid] = s;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == nbk - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double t = 0;
-    for (int i = threadIdx.x; i < (int)nbk; i += blockDim.x) t += partials[i];
-    t = block_sum_double(t, red);
-    if (threadIdx.x == 0) {
-        *out = t * scale;
-        *ticket = 0u;
-    }
-}
+// (the basis, brick grid and value block body are in be_blocks.hpp)
 
 // reg_spline_approxBendingEnergyGradient3D (regul.cpp:287-451) on a brick: the displacement
 // (cpp - node) of the brick with a two-node halo in LDS, the 18 second derivatives of every node
 // of the brick with a one-node halo (regul.cpp:317-379, the cross terms doubled in double), then
 // the adjoint 27-point gather grad += weight / G * value (regul.cpp:381-448) for the brick's
 // nodes.  Optionally tracks the max length of the final gradient.  The reference's in-place
-// (p - node) + node round trip of the points is k_round_trip, run after this kernel.
+// (p - node) + node round trip of the points is k_round_trip, run after this kernel.  RAW: the
+// gathered sums alone are written to grad (planar [3][G]) for a later grad += weight / G * value.
+template<bool RAW = false>
 __device__ __forceinline__ void
 be_gradient_block(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, Affine xyz, BeBasis B,
               float ratio, float *__restrict__ grad, uint32_t *__restrict__ maxBits, int bid) {
@@ -605,14 +542,20 @@ be_gradient_block(const float *__restrict__ cpp, int gnx, int gny, int gnz, Bric
 #pragma unroll
                         for (int k = 0; k < 3; ++k) gv[k] += sD[t * 3 + k][r] * bt[t];
                 }
-        g0 = grad[n] + ratio * gv[0];
-        g1 = grad[n + G] + ratio * gv[1];
-        g2 = grad[n + 2 * G] + ratio * gv[2];
-        grad[n] = g0;
-        grad[n + G] = g1;
-        grad[n + 2 * G] = g2;
+        if (RAW) {
+            grad[n] = gv[0];
+            grad[n + G] = gv[1];
+            grad[n + 2 * G] = gv[2];
+        } else {
+            g0 = grad[n] + ratio * gv[0];
+            g1 = grad[n + G] + ratio * gv[1];
+            g2 = grad[n + 2 * G] + ratio * gv[2];
+            grad[n] = g0;
+            grad[n + G] = g1;
+            grad[n + 2 * G] = g2;
+        }
     }
-    if (maxBits) {
+    if (!RAW && maxBits) {
         // reg_getMaximalLength of the final gradient (lengths >= 0 order like their bit patterns)
         uint32_t bm = valid ? __float_as_uint(sqrtf(g0 * g0 + g1 * g1 + g2 * g2)) : 0u;
         for (int off = 32; off > 0; off >>= 1) {
@@ -933,9 +876,10 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 // ------------------------------------------------------------------ single-pair kernels
 // Each wraps the block body above with the launch's own block index and count.
 __global__ void __launch_bounds__(1024)
-k_nmi_finalize_lds(const uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
-                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent) {
-    nmi_finalize_lds_block(counts, rb, fb, pro, logh, logf, ent);
+k_nmi_finalize_lds(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
+                   float *__restrict__ kTab) {
+    nmi_finalize_lds_block(counts, rb, fb, pro, logh, logf, ent, kTab);
 }
 
 __global__ void __launch_bounds__(256)
@@ -948,6 +892,12 @@ __global__ void __launch_bounds__(256)
 k_be_gradient(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, Affine xyz, BeBasis B,
               float ratio, float *__restrict__ grad, uint32_t *__restrict__ maxBits) {
     be_gradient_block(cpp, gnx, gny, gnz, bg, xyz, B, ratio, grad, maxBits, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256)
+k_be_gradient_raw(const float *__restrict__ cpp, int gnx, int gny, int gnz, BrickGrid bg, Affine xyz, BeBasis B,
+                  float *__restrict__ out) {
+    be_gradient_block<true>(cpp, gnx, gny, gnz, bg, xyz, B, 0.f, out, nullptr, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256) k_round_trip(float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz) {
@@ -1005,7 +955,7 @@ struct PairOps {
     uint32_t *maxBits;     // max gradient length as float bits
     double *maxLen;        // scalars[1]
     double *gam;           // scalars[2]
-    const uint32_t *counts;
+    uint32_t *counts;
     double *pro, *logh;
     float *logf;
     double *ent;
@@ -1141,6 +1091,11 @@ BeBasis make_be_basis() {
 
 }  // namespace
 
+BeBasis be_basis() {
+    static const BeBasis B = make_be_basis();
+    return B;
+}
+
 // ------------------------------------------------------------------ launches
 void op_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
     const unsigned b = std::min(blocks_for(n), 4096u);
@@ -1203,17 +1158,19 @@ void op_image_gradient(nrh_f3d *h) {
 void op_joint_histogram(nrh_f3d *h, const float *warped) {
     h->pairStateVersion = 0;
     const int rb = h->params.ref_bins, fb = h->params.flo_bins;
-    op_fill_u32(h->histCount.p, (size_t)rb * fb, 0u, h->stream);
+    // histCount is zero here: allocation zeroes it and every finalisation consumes it
     const unsigned b = std::min(blocks_for(h->N), 2048u);
     hipLaunchKernelGGL(k_joint_histogram, dim3(b), dim3(kBlock), (size_t)rb * fb * sizeof(uint32_t), h->stream,
                        ref_ptr(h), warped, h->mask.p, h->N, rb, fb, h->histCount.p);
     NRH_HIP(hipGetLastError());
 }
 
-void op_nmi_finalize(nrh_f3d *h) {
+void op_nmi_finalize(nrh_f3d *h, float *kTab) {
+    h->kTabCurrent = kTab != nullptr && h->totalBins() <= kFinLds;
     if (h->totalBins() <= kFinLds)
         hipLaunchKernelGGL(k_nmi_finalize_lds, dim3(1), dim3(1024), 0, h->stream, h->histCount.p,
-                           h->params.ref_bins, h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);
+                           h->params.ref_bins, h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p,
+                           h->kTabCurrent ? kTab : nullptr);
     else
         hipLaunchKernelGGL(k_nmi_finalize, dim3(1), dim3(1024), 0, h->stream, h->histCount.p, h->params.ref_bins,
                            h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);
@@ -1262,7 +1219,8 @@ void op_convolve_voxel_gradient(nrh_f3d *h) {
                                N, h->convDens.p, h->convNan.p);
             if (radius > 0 && dims[axis] > 1) {
                 hipLaunchKernelGGL(k_conv_axis, dim3(blocks_for(N)), dim3(kBlock), 0, h->stream, d, h->convDens.p,
-                                   dims[0], dims[1], dims[2], axis, h->convKernel.p, radius, h->convTmp.p, h->convDensTmp.p);
+                                   dims[0], dims[1], dims[2], axis, h->convKernel.p, radius, h->convTmp.p, h->convDensTmp.p,
+                                   h->params.field_order);
                 NRH_HIP(hipMemcpyAsync(d, h->convTmp.p, N * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
                 hipLaunchKernelGGL(k_conv_normalise, dim3(blocks_for(N)), dim3(kBlock), 0, h->stream, d, h->convDensTmp.p,
                                    h->convNan.p, N);
@@ -1337,13 +1295,38 @@ static BrickGrid brick_grid(const nrh_f3d *h) {
     return g;
 }
 
-void op_bending_energy(nrh_f3d *h, double *out_device) {
+void op_bending_energy(nrh_f3d *h, double *out_device, hipStream_t stream) {
     static const BeBasis B = make_be_basis();
     const BrickGrid bg = brick_grid(h);
     const unsigned nb = (unsigned)(bg.bx * bg.by * bg.bz);
     h->partials.alloc(std::max<size_t>(h->partials.n, 2 * (size_t)nb + 64));
-    hipLaunchKernelGGL(k_be_value, dim3(nb), dim3(256), 0, h->stream, h->cppPos.p, h->cpp.nx, h->cpp.ny, h->cpp.nz,
-                       bg, B, h->partials.p, h->maxBits.p + 6, 1.0 / (double)(3 * h->G), out_device);
+    hipLaunchKernelGGL(k_be_value, dim3(nb), dim3(256), 0, stream ? stream : h->stream, h->cppPos.p, h->cpp.nx,
+                       h->cpp.ny, h->cpp.nz, bg, B, h->partials.p, h->maxBits.p + 6, 1.0 / (double)(3 * h->G),
+                       out_device);
+    NRH_HIP(hipGetLastError());
+}
+
+void fork_side(nrh_f3d *h) {
+    if (!h->side) {
+        NRH_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+        NRH_HIP(hipEventCreateWithFlags(&h->evFork, hipEventDisableTiming));
+        NRH_HIP(hipEventCreateWithFlags(&h->evJoin, hipEventDisableTiming));
+    }
+    NRH_HIP(hipEventRecord(h->evFork, h->stream));
+    NRH_HIP(hipStreamWaitEvent(h->side, h->evFork, 0));
+}
+
+void join_side(nrh_f3d *h) {
+    NRH_HIP(hipEventRecord(h->evJoin, h->side));
+    NRH_HIP(hipStreamWaitEvent(h->stream, h->evJoin, 0));
+}
+
+void op_bending_energy_gradient_raw(nrh_f3d *h, hipStream_t stream) {
+    static const BeBasis B = make_be_basis();
+    const BrickGrid bg = brick_grid(h);
+    h->beGrad.alloc(3 * h->G);
+    hipLaunchKernelGGL(k_be_gradient_raw, dim3((unsigned)(bg.bx * bg.by * bg.bz)), dim3(256), 0, stream,
+                       h->cppPos.p, h->cpp.nx, h->cpp.ny, h->cpp.nz, bg, to_affine(h->cpp.xyz), B, h->beGrad.p);
     NRH_HIP(hipGetLastError());
 }
 
@@ -1562,24 +1545,50 @@ void op_update_dof(nrh_f3d *h, float scale) {
 }
 
 namespace {
+// The values, then (after a system-scope fence) the sequence number in slot 15, which the host
+// polls: a reader that sees the number sees the values.
 __global__ void k_publish(double *__restrict__ dst, const double *__restrict__ s0, int n0,
-                          const double *__restrict__ s1, int n1, const double *__restrict__ s2, int n2) {
+                          const double *__restrict__ s1, int n1, const double *__restrict__ s2, int n2,
+                          uint64_t seq) {
     const int i = threadIdx.x;
     if (i < n0) dst[i] = s0[i];
     else if (i < n0 + n1) dst[i] = s1[i - n0];
     else if (i < n0 + n1 + n2) dst[i] = s2[i - n0 - n1];
+    __threadfence_system();
+    __syncthreads();
+    if (i == 0) __hip_atomic_store((uint64_t *)(dst + 15), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 }  // namespace
 
 void op_publish(nrh_f3d *h, const double *s0, int n0, const double *s1, int n1, const double *s2, int n2) {
-    NRH_REQUIRE(n0 + n1 + n2 <= 16, NRH_ERR_ARG, "too many scalars to publish");
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, h->stream, h->publishedDev, s0, n0, s1, n1, s2, n2);
+    NRH_REQUIRE(n0 + n1 + n2 <= 15, NRH_ERR_ARG, "too many scalars to publish");
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, h->stream, h->publishedDev, s0, n0, s1, n1, s2, n2,
+                       ++h->publishSeq);
     NRH_HIP(hipGetLastError());
+}
+
+void wait_published(nrh_f3d *h) {
+    // poll the sequence number the publish kernel writes last (a driver synchronisation costs
+    // tens of microseconds of wake-up); every 256 polls ask the stream for an error, and once it
+    // reports completion the number must be there
+    const volatile uint64_t *seq = (const volatile uint64_t *)(h->published + 15);
+    for (unsigned spins = 1;; ++spins) {
+        if (__atomic_load_n((const uint64_t *)seq, __ATOMIC_ACQUIRE) == h->publishSeq) return;
+        if (spins % 256 == 0) {
+            const hipError_t e = hipStreamQuery(h->stream);
+            if (e == hipSuccess) {
+                NRH_REQUIRE(__atomic_load_n((const uint64_t *)seq, __ATOMIC_ACQUIRE) == h->publishSeq, NRH_ERR_HIP,
+                            "the stream completed without publishing its results");
+                return;
+            }
+            if (e != hipErrorNotReady) NRH_HIP(e);
+        }
+    }
 }
 
 double read_scalar(nrh_f3d *h, const double *dev) {
     op_publish(h, dev, 1);
-    NRH_HIP(hipStreamSynchronize(h->stream));
+    wait_published(h);
     return h->published[0];
 }
 

@@ -46,6 +46,7 @@ struct VelGrid {
     int gnx, gny, gnz;
     size_t G;
     Affine xyz, ijk;
+    int order;  // NRH_FIELD_ORDER_*: the summation order of the reference build reproduced
 };
 
 // Affine parts of a velocity grid on one field geometry.  pre: the grid's extension 0, applied to
@@ -153,7 +154,12 @@ __device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, co
     }
     const bool inside = pre[0] >= 0 && pre[0] + 3 < g.gnx && pre[1] >= 0 && pre[1] + 3 < g.gny && pre[2] >= 0 &&
                         pre[2] + 3 < g.gnz;
+    // scalar build: one sequential sum with weights (x y) z; SSE build (reg-lib/cpu/
+    // _reg_localTrans.cpp:1025-1052): four lanes (one per x tap) with weights (y z) x, reduced
+    // left to right
+    const bool sse = g.order == NRH_FIELD_ORDER_SSE;
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    float l0[4] = { 0.f, 0.f, 0.f, 0.f }, l1[4] = { 0.f, 0.f, 0.f, 0.f }, l2[4] = { 0.f, 0.f, 0.f, 0.f };
     if (__all(inside || !active)) {
         const size_t sy = g.gnx, sz = (size_t)g.gnx * g.gny;
         const size_t base = active ? ((size_t)pre[2] * g.gny + pre[1]) * g.gnx + pre[0] : 0;
@@ -167,12 +173,22 @@ __device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, co
                 cy[k] = g.cp[idx + g.G];
                 cz[k] = g.cp[idx + 2 * g.G];
             }
+            if (sse) {
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const float w = b[0][k & 3] * b[1][k >> 2] * b[2][c];
-                r0 += cx[k] * w;
-                r1 += cy[k] * w;
-                r2 += cz[k] * w;
+                for (int k = 0; k < 16; ++k) {
+                    const float w = (b[1][k >> 2] * b[2][c]) * b[0][k & 3];
+                    l0[k & 3] = w * cx[k] + l0[k & 3];
+                    l1[k & 3] = w * cy[k] + l1[k & 3];
+                    l2[k & 3] = w * cz[k] + l2[k & 3];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const float w = b[0][k & 3] * b[1][k >> 2] * b[2][c];
+                    r0 += cx[k] * w;
+                    r1 += cy[k] * w;
+                    r2 += cz[k] * w;
+                }
             }
         }
     } else if (active) {
@@ -196,13 +212,25 @@ __device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, co
                     } else {
                         cv = slided(g.cp, g.G, g.gnx, g.gny, g.gnz, g.xyz, X, Y, Z);
                     }
-                    const float w = b[0][a] * b[1][bb] * b[2][c];
-                    r0 += cv.x * w;
-                    r1 += cv.y * w;
-                    r2 += cv.z * w;
+                    if (sse) {
+                        const float w = (b[1][bb] * b[2][c]) * b[0][a];
+                        l0[a] = w * cv.x + l0[a];
+                        l1[a] = w * cv.y + l1[a];
+                        l2[a] = w * cv.z + l2[a];
+                    } else {
+                        const float w = b[0][a] * b[1][bb] * b[2][c];
+                        r0 += cv.x * w;
+                        r1 += cv.y * w;
+                        r2 += cv.z * w;
+                    }
                 }
             }
         }
+    }
+    if (sse) {
+        r0 = ((l0[0] + l0[1]) + l0[2]) + l0[3];
+        r1 = ((l1[0] + l1[1]) + l1[2]) + l1[3];
+        r2 = ((l2[0] + l2[1]) + l2[2]) + l2[3];
     }
     if (a.pre) {
         const float3 ao = mul_d(a.AS, (double)x, (double)y, (double)z);
@@ -606,6 +634,7 @@ VelGrid vel_grid(const nrh_f3d *h) {
     g.G = h->G;
     g.xyz = to_affine(h->cpp.xyz);
     g.ijk = to_affine(h->cpp.ijk);
+    g.order = h->params.field_order;
     return g;
 }
 

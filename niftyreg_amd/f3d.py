@@ -254,8 +254,9 @@ class F3dContent:
         self._op("nrh_f3d_fused_available", ctypes.byref(v))
         return bool(v.value)
 
-    def set_profiling(self, enable=True):
-        self._op("nrh_f3d_set_profiling", ctypes.c_int(1 if enable else 0))
+    def set_profiling(self, enable=True, period=1):
+        """HIP-event timing of every period-th fused sweep of each kind (enable=False stops it)."""
+        self._op("nrh_f3d_set_profiling", ctypes.c_int(int(period) if enable else 0))
 
     def sweep_times(self):
         ms = np.zeros(2, np.float64)

@@ -50,7 +50,9 @@ HipF3dContent::HipF3dContent(NiftiImage& referenceIn, NiftiImage& floatingIn, Ni
     DefContent(referenceIn, floatingIn, localWeightSimIn, referenceMaskIn, transformationMatrixIn, sizeof(float)),
     Content(referenceIn, floatingIn, referenceMaskIn, transformationMatrixIn, sizeof(float)) {
     if (bytesIn != sizeof(float)) NR_FATAL_ERROR("The HIP platform computes in single precision");
-    if (localWeightSimIn) NR_FATAL_ERROR("Local similarity weights are not supported on the HIP platform");
+    // reg_base passes its (usually empty) weight image; only a set image is a local weighting
+    if (localWeightSimIn && *localWeightSimIn)
+        NR_FATAL_ERROR("Local similarity weights are not supported on the HIP platform");
     params.ref_bins = params.flo_bins = 68;
     params.padding = std::numeric_limits<float>::quiet_NaN();
     params.similarity_weight = 1.f;

@@ -541,7 +541,15 @@ struct F3dHandle {
 };
 } // namespace
 
+// Runs print the reference's per-iteration objective lines when set (diagnostics).
+static bool g_verbose = false;
+
 extern "C" {
+
+int ref_set_verbose(int v) {
+    g_verbose = v != 0;
+    return 0;
+}
 
 typedef struct {
     int bins;            // user bin count (reference adds 4)
@@ -558,8 +566,10 @@ typedef struct {
     const float *affine; // input affine (reference -> floating), NULL for none
 } ref_f3d_params;
 
-static void configure(reg_f3d<float>& r, const ref_f3d_params *p) {
-    r.SetPlatformType(PlatformType::Cpu);
+// The platform is chosen first, as reg_f3d's command line does (reg-apps/reg_f3d.cpp:287): the
+// measures are created by the platform's measure creator when they are configured.
+static void configure(reg_f3d<float>& r, const ref_f3d_params *p, PlatformType platform = PlatformType::Cpu) {
+    r.SetPlatformType(platform);
     if (p->measure == 1) {
         r.UseLNCC(0, p->lncc_sigma);
     } else {
@@ -576,7 +586,7 @@ static void configure(reg_f3d<float>& r, const ref_f3d_params *p) {
     if (!p->use_pyramid) r.DoNotUsePyramidalApproach();
     r.SetWarpedPaddingValue(p->padding);
     if (p->affine) r.SetAffineTransformation(to_mat44(p->affine));
-    r.DoNotPrintOutInformation();
+    if (!g_verbose) r.DoNotPrintOutInformation();
 }
 
 // Create a reg_f3d<float>, run Initialise / InitCurrentLevel(level) / InitialiseSimilarity /
@@ -667,6 +677,31 @@ int ref_f3d_weights(void *hp, float *w) {
     F3dHandle *h = static_cast<F3dHandle*>(hp);
     w[0] = h->reg->SimWeight(); w[1] = h->reg->BEWeight(); w[2] = h->reg->LEWeight();
     return 0;
+}
+
+// Full registration through reg_base::Run (_reg_base.cpp:727-819) on the given platform
+// (PlatformType value; 3 is the HIP platform of the patched build).  Output: final-level grid.
+// Input images for Run(); named, so that a verbose run can print their names.
+static NiftiImage run_image(const nr_geom *g, const float *data, const char *name) {
+    NiftiImage img = make_image(g, data);
+    img->fname = nifti_strdup(name);
+    img->iname = nifti_strdup(name);
+    return img;
+}
+
+int ref_f3d_run_on(int platform, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                   const ref_f3d_params *p, nr_geom *cppOut, float *cppData, int cppCapacity) {
+    return guarded([&] {
+        RefF3d r;
+        configure(r, p, static_cast<PlatformType>(platform));
+        r.SetReferenceImage(run_image(ref, refData, "reference"));
+        r.SetFloatingImage(run_image(flo, floData, "floating"));
+        r.Run();
+        NiftiImage g = r.GetControlPointPositionImage();
+        geom_of(g, cppOut);
+        if (cppData && (int)g.nVoxels() <= cppCapacity)
+            std::memcpy(cppData, g->data, g.nVoxels() * sizeof(float));
+    });
 }
 
 // Full registration through reg_base::Run (_reg_base.cpp:727-819).  Output: final-level grid.
@@ -807,15 +842,16 @@ int ref_f3d2_gradient(void *hp, int normalise, double *maxLength, float *gradFw,
     });
 }
 
-// Full symmetric registration through reg_f3d2's run (reg_base::Run with the reg_f3d2 overrides).
-// Output: the final-level grid geometry, both velocity grids and their intent_p2.
-int ref_f3d2_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
-                 const ref_f3d_params *p, nr_geom *cppOut, float *fw, float *bw, int capacity, int *steps) {
+// Full symmetric registration through reg_f3d2's run (reg_base::Run with the reg_f3d2 overrides)
+// on the given platform.  Output: the final-level grid geometry, both velocity grids and their
+// intent_p2.
+int ref_f3d2_run_on(int platform, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                    const ref_f3d_params *p, nr_geom *cppOut, float *fw, float *bw, int capacity, int *steps) {
     return guarded([&] {
         RefF3d2 r;
-        configure(r, p);
-        r.SetReferenceImage(make_image(ref, refData));
-        r.SetFloatingImage(make_image(flo, floData));
+        configure(r, p, static_cast<PlatformType>(platform));
+        r.SetReferenceImage(run_image(ref, refData, "reference"));
+        r.SetFloatingImage(run_image(flo, floData, "floating"));
         r.Run();
         NiftiImage g = r.GetControlPointPositionImage();
         NiftiImage gb = r.GetBackwardControlPointPositionImage();
@@ -826,6 +862,11 @@ int ref_f3d2_run(const nr_geom *ref, const float *refData, const nr_geom *flo, c
         }
         if (steps) { steps[0] = (int)g->intent_p2; steps[1] = (int)gb->intent_p2; }
     });
+}
+
+int ref_f3d2_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                 const ref_f3d_params *p, nr_geom *cppOut, float *fw, float *bw, int capacity, int *steps) {
+    return ref_f3d2_run_on((int)PlatformType::Cpu, ref, refData, flo, floData, p, cppOut, fw, bw, capacity, steps);
 }
 
 int ref_f3d2_weights(void *hp, float *w) {
@@ -898,13 +939,14 @@ typedef struct {
     int symmetric;        // reg_aladin_sym (the reg_aladin default) instead of reg_aladin
 } ref_aladin_params;
 
-// reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364); matrix receives the final affine.
-int ref_aladin_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
-                   const ref_aladin_params *p, float *matrix) {
+// reg_aladin<float>::Run (reg-lib/_reg_aladin.cpp:315-364) on the given platform; matrix receives
+// the final affine.
+int ref_aladin_run_on(int platform, const nr_geom *ref, const float *refData, const nr_geom *flo,
+                      const float *floData, const ref_aladin_params *p, float *matrix) {
     return guarded([&] {
         std::unique_ptr<reg_aladin<float>> owner(p->symmetric ? new reg_aladin_sym<float>() : new reg_aladin<float>());
         reg_aladin<float>& a = *owner;
-        a.SetPlatformType(PlatformType::Cpu);
+        a.SetPlatformType(static_cast<PlatformType>(platform));
         a.SetInputReference(make_image(ref, refData));
         a.SetInputFloating(make_image(flo, floData));
         a.SetNumberOfLevels(p->levels);
@@ -919,6 +961,11 @@ int ref_aladin_run(const nr_geom *ref, const float *refData, const nr_geom *flo,
         a.Run();
         from_mat44(*a.GetTransformationMatrix(), matrix);
     });
+}
+
+int ref_aladin_run(const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                   const ref_aladin_params *p, float *matrix) {
+    return ref_aladin_run_on((int)PlatformType::Cpu, ref, refData, flo, floData, p, matrix);
 }
 
 // NiftyReg::Mat44Avg2 (reg-lib/cpu/Maths.cpp:479-491): expm((logm(A) + logm(B)) / 2) through Eigen.

@@ -394,9 +394,11 @@ def fp64_field(refg, cppg, cpp):
 
 
 def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing=-5.0, levels=3, maxit=150,
-                padding=float("nan"), measure=0, lncc_sigma=-5.0, affine=None, scalar=False, levels_to_perform=0):
-    """reg_f3d<float>::Run on the reference CPU build (scalar=True: the build without USE_SSE);
-    returns (grid geom, control points)."""
+                padding=float("nan"), measure=0, lncc_sigma=-5.0, affine=None, scalar=False, levels_to_perform=0,
+                platform=None):
+    """reg_f3d<float>::Run on the reference CPU build (scalar=True: the build without USE_SSE), or
+    with platform=PLATFORM_HIP on the reference build whose Platform selects libnrhip; returns
+    (grid geom, control points)."""
     p = ref_f3d_params()
     p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
     p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = (levels, levels_to_perform or levels, maxit, 1,
@@ -408,10 +410,14 @@ def ref_f3d_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.0, spacing
     sp = [abs(spacing) * d if spacing < 0 else spacing for d in refg.pixdim]
     cap = 3 * int(np.prod([int(np.ceil(n * d / s + 4)) for n, d, s in zip(refg.dims, refg.pixdim, sp)]))
     buf = np.zeros(cap, np.float32)
-    lib = ref_scalar() if scalar else ref()
-    _scheck(lib, lib.ref_f3d_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
-                             ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
-                             ctypes.byref(p), ctypes.byref(out), fp(buf), cap))
+    args = (ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)), ctypes.byref(rg(flog)),
+            fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)), ctypes.byref(p), ctypes.byref(out), fp(buf), cap)
+    if platform is not None:
+        lib = ref_hip()
+        _scheck(lib, lib.ref_f3d_run_on(int(platform), *args))
+    else:
+        lib = ref_scalar() if scalar else ref()
+        _scheck(lib, lib.ref_f3d_run(*args))
     n = 3 * out.nx * out.ny * out.nz
     return geom_from_nr(out), buf[:n].copy()
 
@@ -527,12 +533,19 @@ def ref_lts(rp, wp, n_defined, percent=50, affine=True, matrix=None):
 
 
 def ref_aladin_run(refg, ref_img, flog, flo_img, levels=3, maxit=5, block_percent=50, inlier=50, rigid=False,
-                   affine=True, align_centre=True, symmetric=False):
+                   affine=True, align_centre=True, symmetric=False, platform=None):
+    """reg_aladin<float>::Run (reg_aladin_sym with symmetric=True) on the reference CPU build, or with
+    platform=PLATFORM_HIP on the HIP-platform build; returns the 4x4 affine."""
     p = ref_aladin_params(levels, levels, maxit, block_percent, inlier, int(rigid), int(affine), int(align_centre),
                           int(symmetric))
     m = np.zeros(16, np.float32)
-    rcheck(ref().ref_aladin_run(ctypes.byref(rg(refg)), fp(ref_img.ravel()), ctypes.byref(rg(flog)),
-                                fp(flo_img.ravel()), ctypes.byref(p), fp(m)))
+    args = (ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).ravel()), ctypes.byref(rg(flog)),
+            fp(np.ascontiguousarray(flo_img, np.float32).ravel()), ctypes.byref(p), fp(m))
+    if platform is not None:
+        lib = ref_hip()
+        _scheck(lib, lib.ref_aladin_run_on(int(platform), *args))
+    else:
+        rcheck(ref().ref_aladin_run(*args))
     return m.reshape(4, 4)
 
 
@@ -697,6 +710,24 @@ def ref_affine_read(path):
 # ---------------------------------------------------------------- velocity fields (F3D2)
 REF_SCALAR = os.path.join(ROOT, "oracle", "_ref", "libnrref_nosse.so")
 _ref_scalar = None
+
+
+REF_HIP = os.path.join(ROOT, "oracle", "_ref", "libnrref_hip.so")
+_ref_hip = None
+PLATFORM_CPU, PLATFORM_HIP = 0, 3
+
+
+def ref_hip_available():
+    return os.path.exists(REF_HIP)
+
+
+def ref_hip():
+    """The default reference build with the HIP platform (PlatformType 3) in its Platform.cpp."""
+    global _ref_hip
+    if _ref_hip is None:
+        _ref_hip = ctypes.CDLL(REF_HIP)
+        _ref_hip.ref_last_error.restype = ctypes.c_char_p
+    return _ref_hip
 
 
 def ref_scalar():
@@ -946,9 +977,9 @@ def plugin_f3d2_eval(hip, refg, ref_img, flog, flo_img, cppg, fw, bw, bins=64, s
 
 
 def ref_f3d2_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.01, spacing=-5.0, levels=3, maxit=150,
-                 padding=float("nan"), affine=None, scalar=False):
-    """reg_f3d2<float>'s run on the reference CPU build (scalar=True: the build without USE_SSE);
-    returns (grid geom, fw, bw, steps)."""
+                 padding=float("nan"), affine=None, scalar=False, platform=None):
+    """reg_f3d2<float>'s run on the reference CPU build (scalar=True: the build without USE_SSE), or
+    with platform=PLATFORM_HIP on the HIP-platform build; returns (grid geom, fw, bw, steps)."""
     p = ref_f3d_params()
     p.bins, p.be, p.le, p.spacing = bins, be, le, spacing
     p.levels, p.levels_to_perform, p.maxit, p.use_pyramid, p.padding = levels, levels, maxit, 1, padding
@@ -960,10 +991,15 @@ def ref_f3d2_run(refg, ref_img, flog, flo_img, bins=64, be=0.001, le=0.01, spaci
     fw = np.zeros(cap, np.float32)
     bw = np.zeros(cap, np.float32)
     steps = np.zeros(2, np.int32)
-    lib = ref_scalar() if scalar else ref()
-    _scheck(lib, lib.ref_f3d2_run(ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)),
-                                  ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)),
-                                  ctypes.byref(p), ctypes.byref(out), fp(fw), fp(bw), cap, ip(steps)))
+    args = (ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).reshape(-1)), ctypes.byref(rg(flog)),
+            fp(np.ascontiguousarray(flo_img, np.float32).reshape(-1)), ctypes.byref(p), ctypes.byref(out), fp(fw),
+            fp(bw), cap, ip(steps))
+    if platform is not None:
+        lib = ref_hip()
+        _scheck(lib, lib.ref_f3d2_run_on(int(platform), *args))
+    else:
+        lib = ref_scalar() if scalar else ref()
+        _scheck(lib, lib.ref_f3d2_run(*args))
     n = 3 * out.nx * out.ny * out.nz
     return geom_from_nr(out), fw[:n].copy(), bw[:n].copy(), (int(steps[0]), int(steps[1]))
 

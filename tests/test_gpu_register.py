@@ -11,7 +11,13 @@ two trajectories part after a few iterations even when every single evaluation a
 the reference's own SSE and scalar builds end the same registrations up to 2.6e-3 apart (both
 runs are made here and printed).  Parity of a whole registration is therefore measured against
 that spread: the device's final NMI may differ from the default (SSE) build's by no more than the
-scalar build's does, plus 1e-4.
+scalar build's does, plus 1e-4.  The symmetric (reg_f3d2) registrations project the voxel gradient
+onto the nodes with the separable transposed field (the same mathematics as the reference's
+convolution and node extraction, rounded differently: about 1e-7 relative per evaluation), so
+their trajectories part from the SSE build's like a third build's would; their bound is the
+spread plus 5e-4 (2e-4 relative).  The reference's own Run() driving the same device through its
+HIP platform (per-operation kernels in the build's own orders) ends on the CPU platform's score
+exactly (tests/test_gpu_platform.py).
 """
 import time
 
@@ -26,6 +32,7 @@ from niftyreg_amd.geometry import Geom, create_control_point_grid
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not O.ref_available(), reason="reference build absent")]
 
 FINAL_NMI_TOL = 1e-4
+SYM_NMI_TOL = 5e-4
 
 
 def rotation(angle, t):
@@ -70,7 +77,7 @@ def test_config0_registration_matches_reference(gpu, levels, maxit):
 def test_symmetric_registration_matches_reference(gpu, levels, maxit):
     """reg_f3d2 (-vel, NMI, -be 0.001 -le 0.01) on both platforms.  Both final velocity grid pairs
     are scored with reg_f3d2's own summed NMI; the device's may differ from the SSE build's by no
-    more than the scalar build's does, plus twice the single-direction tolerance."""
+    more than the scalar build's does, plus the symmetric tolerance."""
     from niftyreg_amd.f3d import register_symmetric
     from niftyreg_amd.geometry import create_symmetric_control_point_grids
     ref, flo = phantoms.config0_pair(0)
@@ -93,7 +100,7 @@ def test_symmetric_registration_matches_reference(gpu, levels, maxit):
     assert np.array_equal(cg.xyz, rcg.xyz)
     assert abs(nmi - score_gpu) <= 1e-4 * score_gpu
     assert score_gpu > score0 + 0.02
-    assert abs(score_gpu - score_ref) <= abs(score_scalar - score_ref) + 2 * FINAL_NMI_TOL
+    assert abs(score_gpu - score_ref) <= abs(score_scalar - score_ref) + SYM_NMI_TOL
 
 
 def test_symmetric_registration_with_affine_matches_reference(gpu):
@@ -119,7 +126,7 @@ def test_symmetric_registration_with_affine_matches_reference(gpu):
     print(f"symmetric + affine: start {s0:.6f}, device {s_dev:.6f} ({its} it), reference SSE {s_ref:.6f}, "
           f"scalar {s_scalar:.6f}")
     assert s_dev > s0 + 0.02
-    assert abs(s_dev - s_ref) <= abs(s_scalar - s_ref) + 2 * FINAL_NMI_TOL
+    assert abs(s_dev - s_ref) <= abs(s_scalar - s_ref) + SYM_NMI_TOL
 
 
 def test_affine_initialised_registration_matches_reference(gpu):

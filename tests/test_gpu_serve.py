@@ -46,7 +46,7 @@ def _same(a, b):
 def test_results_same_alone_and_in_mix(gpu):
     reqs = [(make_request(d, s), config(m)) for d, s, m in REQUESTS]
     g = Geom.identity((48, 48, 48))
-    flat = np.full(g.nvox, 0.3, np.float32)
+    flat = np.full(g.nvox, 0.5, np.float32)
     reqs.insert(2, ((g, flat, g, flat), config(_lib.MEASURE_NMI)))
     alone = []
     srv = Server(budget_bytes=64 << 30, max_active=1)
