@@ -394,6 +394,16 @@ int nrh_aladin_download_positions(nrh_aladin *h, float *ref_pos, float *warp_pos
 int nrh_aladin_lts(nrh_aladin *h, int affine);
 /* The same LTS on host arrays (n correspondences, NaN warped x = undefined); m is in/out. */
 int nrh_lts(const float *ref_pos, const float *warp_pos, int n, int inlier_lts, int affine, float m[16]);
+/* No counterpart in the reference.  nrh_lts for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): ref_pos and
+ * warp_pos are device arrays of 3 n floats, matrix a device array of 16 floats holding the current
+ * matrix and receiving the fit, status a device int receiving 0 (fit), 1 (too few
+ * correspondences) or 2 (singular fit; the matrix is left unchanged).  n <= 262144.  The call
+ * only enqueues work on `stream` (a hipStream_t, NULL for the null stream) and allocates nothing:
+ * its scratch is a 7.6 MB static device array of the library, one per GPU, kept until the process
+ * ends, so calls that overlap on one GPU race on it. */
+int nrh_lts_static(const float *ref_pos, const float *warp_pos, int n, int inlier_lts, int affine, float *matrix,
+                   int *status, void *stream);
 
 /* The matrix step of reg_aladin_sym::UpdateTransformationMatrix (reg-lib/_reg_aladin_sym.cpp:
  * 140-153), run after both handles' LTS fits: forward <- Mat44Avg2(forward, inverse(backward)),

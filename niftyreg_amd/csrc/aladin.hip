@@ -1088,7 +1088,34 @@ void aladin_register_batch(int n, const nrh_geom *refs, const float *const *refD
     }
 }
 
+// ------------------------------------------------------------------ allocation-free LTS
+// Scratch of nrh_lts_static: the compacted correspondences (3 n floats each) and the residual
+// bits plus keep bytes (n + n / 4 words) of one fit of at most kLtsStaticMax correspondences
+// (7.6 MB on every GPU the library runs on, held until the process ends).  Host code never uses
+// the array's name as an address: each device's address comes from hipGetSymbolAddress.
+constexpr int kLtsStaticMax = 262144;
+constexpr size_t kLtsScratchWords = 6 * (size_t)kLtsStaticMax + kLtsStaticMax + (kLtsStaticMax + 3) / 4;
+static __device__ __align__(256) uint32_t scratch[kLtsScratchWords];
+
 }  // namespace
+
+// The address of `scratch` on the current device, looked up once per device (nrh_init does it for
+// the device it selects; nrh_lts_static for any other current device on its first call there).
+uint32_t *lts_static_scratch() {
+    static std::mutex mu;
+    static std::map<int, uint32_t *> ptrs;
+    int dev = 0;
+    NRH_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    uint32_t *ptr = nullptr;
+    NRH_HIP(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    NRH_REQUIRE(ptr != nullptr, NRH_ERR_HIP, "no device address for the LTS scratch");
+    ptrs[dev] = ptr;
+    return ptr;
+}
+
 }  // namespace nrh
 
 using namespace nrh;
@@ -1198,6 +1225,25 @@ int nrh_aladin_lts(nrh_aladin *h, int affine) {
         launch_lts(h, affine != 0);
         read_counters(h);
         raise_lts_status(h->pinned[1]);
+    });
+}
+
+int nrh_lts_static(const float *ref_pos, const float *warp_pos, int n, int inlier_lts, int affine, float *matrix,
+                   int *status, void *stream) {
+    return guard([&] {
+        NRH_REQUIRE(ref_pos && warp_pos && matrix && status, NRH_ERR_ARG, "null argument");
+        NRH_REQUIRE(n > 0, NRH_ERR_ARG, "no correspondences");
+        NRH_REQUIRE(n <= kLtsStaticMax, NRH_ERR_ARG,
+                    "nrh_lts_static takes at most 262144 correspondences (its static scratch); use nrh_lts");
+        NRH_REQUIRE(inlier_lts > 0 && inlier_lts <= 100, NRH_ERR_ARG, "LTS inlier percentage must be in (0, 100]");
+        uint32_t *base = lts_static_scratch();
+        float *P = (float *)base, *Q = P + 3 * (size_t)n;
+        uint32_t *res = (uint32_t *)(Q + 3 * (size_t)n);
+        hipStream_t st = (hipStream_t)stream;
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, status);
+        hipLaunchKernelGGL(k_lts, dim3(1), dim3(kLtsThreads), 0, st, ref_pos, warp_pos, n, matrix, P, Q, res,
+                           inlier_lts, affine ? 1 : 0, status);
+        NRH_HIP(hipGetLastError());
     });
 }
 

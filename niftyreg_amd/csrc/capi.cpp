@@ -148,6 +148,7 @@ int nrh_init(int device) {
         NRH_HIP(hipGetDeviceCount(&c));
         NRH_REQUIRE(device >= 0 && device < c, NRH_ERR_ARG, "invalid device index " + std::to_string(device));
         NRH_HIP(hipSetDevice(device));
+        lts_static_scratch();
     });
 }
 

@@ -108,4 +108,7 @@ inline size_t voxels(const nrh_geom& g) { return (size_t)g.nx * g.ny * g.nz; }
 
 void check_geom(const nrh_geom *g, const char *what);
 
+// Device address of nrh_lts_static's static scratch on the current device (aladin.hip).
+uint32_t *lts_static_scratch();
+
 }  // namespace nrh
