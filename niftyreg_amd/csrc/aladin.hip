@@ -342,7 +342,10 @@ __device__ void svd3(const double H[3][3], double U[3][3], double V[3][3]) {
         }
     for (int sweep = 0; sweep < 60; ++sweep) {
         double off = 0;
+        // unrolled: every index is a compile-time constant, so A and V stay in registers
+#pragma unroll
         for (int p = 0; p < 2; ++p)
+#pragma unroll
             for (int q = p + 1; q < 3; ++q) {
                 const double a = A[0][p] * A[0][p] + A[1][p] * A[1][p] + A[2][p] * A[2][p];
                 const double b = A[0][q] * A[0][q] + A[1][q] * A[1][q] + A[2][q] * A[2][q];
@@ -352,6 +355,7 @@ __device__ void svd3(const double H[3][3], double U[3][3], double V[3][3]) {
                 const double zeta = (b - a) / (2 * c);
                 const double tn = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
                 const double cs = 1 / sqrt(1 + tn * tn), sn = cs * tn;
+#pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     const double x = A[i][p], y = A[i][q];
                     A[i][p] = cs * x - sn * y;
@@ -430,23 +434,32 @@ __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q
                                { o[1], o[3], o[4], Spq[1][0], Spq[1][1], Spq[1][2] },
                                { o[2], o[4], o[5], Spq[2][0], Spq[2][1], Spq[2][2] } };
             const double scale = fmax(fabs(o[0]), fmax(fabs(o[3]), fabs(o[5])));
-            for (int col = 0; col < 3 && ok; ++col) {
+            // fully unrolled, the pivot row swapped in by selects: a stays in registers
+#pragma unroll
+            for (int col = 0; col < 3; ++col) {
                 int piv = col;
+                double best = fabs(a[col][col]);
+#pragma unroll
                 for (int r = col + 1; r < 3; ++r)
-                    if (fabs(a[r][col]) > fabs(a[piv][col])) piv = r;
-                if (!(fabs(a[piv][col]) > 1e-12 * scale)) {
-                    ok = 0;
-                    break;
-                }
-                if (piv != col)
-                    for (int k = 0; k < 6; ++k) {
-                        const double tmp = a[col][k];
-                        a[col][k] = a[piv][k];
-                        a[piv][k] = tmp;
+                    if (fabs(a[r][col]) > best) {
+                        best = fabs(a[r][col]);
+                        piv = r;
                     }
+                if (!(best > 1e-12 * scale)) ok = 0;
+#pragma unroll
+                for (int r = col + 1; r < 3; ++r)
+                    if (piv == r)
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) {
+                            const double tmp = a[col][k];
+                            a[col][k] = a[r][k];
+                            a[r][k] = tmp;
+                        }
+#pragma unroll
                 for (int r = 0; r < 3; ++r) {
                     if (r == col) continue;
                     const double f = a[r][col] / a[col][col];
+#pragma unroll
                     for (int k = col; k < 6; ++k) a[r][k] -= f * a[col][k];
                 }
             }
@@ -491,7 +504,7 @@ __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q
 __device__ __forceinline__ void
 lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, int nActive, float *__restrict__ matrix,
          float *__restrict__ P, float *__restrict__ Q, uint32_t *__restrict__ res, int percent, int affine,
-         int *__restrict__ status, LtsShared& s) {
+         int *__restrict__ status, LtsShared& s, uint8_t *keepIn = nullptr) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x;
     if (tid < 16) s.M[tid] = matrix[tid];
@@ -525,7 +538,8 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         if (tid == 0) *status = 2;
         return;
     }
-    uint8_t *keepFlag = reinterpret_cast<uint8_t *>(res + n);
+    // keep flags: after the residuals in global memory, or a separate array (LDS)
+    uint8_t *keepFlag = keepIn ? keepIn : reinterpret_cast<uint8_t *>(res + n);
     double lastDistance = DBL_MAX;
     if (tid < 16) s.last[tid] = 0.f;
     __syncthreads();
@@ -537,7 +551,7 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         }
         __syncthreads();
         // radix select of the keep-th smallest residual: digits of 11, 11 and 10 bits
-        uint32_t prefix = 0, prefMask = 0;
+        uint32_t prefix = 0, prefMask = 0, eqCount = 0;
         int k = (int)keep;
         const int shifts[3] = { 21, 10, 0 }, widths[3] = { 11, 11, 10 };
         for (int d = 0; d < 3; ++d) {
@@ -575,23 +589,34 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
                 }
             }
             __syncthreads();
+            if (d == 2) eqCount = s.hist[s.scal[1]];  // keys equal to the whole threshold
             prefix |= (uint32_t)s.scal[1] << sh;
             prefMask |= dm << sh;
             k = s.scal[2];
             __syncthreads();
         }
         // keep: residual below the threshold, or equal to it within the first k in index order
-        int seenEqual = 0;
+        // (when every key equal to the threshold is kept, the order plays no part: one pass)
         double dsum = 0;
-        for (int base = 0; base < n; base += kLtsThreads) {
-            const int i = base + tid;
-            const uint32_t key = i < n ? res[i] : 0xffffffffu;
-            int total;
-            const int eqPos = seenEqual + wg_scan(i < n && key == prefix, s, &total);
-            const bool kept = i < n && (key < prefix || (key == prefix && eqPos < k));
-            if (i < n) keepFlag[i] = kept ? 1 : 0;
-            if (kept) dsum += (double)__uint_as_float(key);
-            seenEqual += total;
+        if ((uint32_t)k == eqCount) {
+            for (int i = tid; i < n; i += kLtsThreads) {
+                const uint32_t key = res[i];
+                const bool kept = key <= prefix;
+                keepFlag[i] = kept ? 1 : 0;
+                if (kept) dsum += (double)__uint_as_float(key);
+            }
+        } else {
+            int seenEqual = 0;
+            for (int base = 0; base < n; base += kLtsThreads) {
+                const int i = base + tid;
+                const uint32_t key = i < n ? res[i] : 0xffffffffu;
+                int total;
+                const int eqPos = seenEqual + wg_scan(i < n && key == prefix, s, &total);
+                const bool kept = i < n && (key < prefix || (key == prefix && eqPos < k));
+                if (i < n) keepFlag[i] = kept ? 1 : 0;
+                if (kept) dsum += (double)__uint_as_float(key);
+                seenEqual += total;
+            }
         }
         double dv[1] = { dsum };
         wg_sum<1>(dv, s);
@@ -618,6 +643,31 @@ k_lts(const float *__restrict__ refPos, const float *__restrict__ warpPos, int n
       int *__restrict__ status) {
     __shared__ LtsShared s;
     lts_body(refPos, warpPos, nActive, matrix, P, Q, res, percent, affine, status, s);
+}
+
+// The same fit with the residuals and keep flags of up to kLtsLdsN correspondences in LDS (the
+// radix passes and the fits then read them from LDS instead of memory).
+constexpr int kLtsLdsN = 16384;
+
+__global__ void __launch_bounds__(kLtsThreads)
+k_lts_lds(const float *__restrict__ refPos, const float *__restrict__ warpPos, int nActive, float *__restrict__ matrix,
+          float *__restrict__ P, float *__restrict__ Q, int percent, int affine, int *__restrict__ status) {
+    __shared__ LtsShared s;
+    __shared__ uint32_t sRes[kLtsLdsN];
+    __shared__ uint8_t sKeep[kLtsLdsN];
+    lts_body(refPos, warpPos, nActive, matrix, P, Q, sRes, percent, affine, status, s, sKeep);
+}
+
+// One LTS fit of nActive correspondences on `stream`: in LDS when they fit, else with the
+// residual scratch `res` (nActive + nActive / 4 words) in memory.
+void launch_lts(hipStream_t stream, const float *refPos, const float *warpPos, int nActive, float *matrix, float *P,
+                float *Q, uint32_t *res, int percent, int affine, int *status) {
+    if (nActive <= kLtsLdsN)
+        hipLaunchKernelGGL(k_lts_lds, dim3(1), dim3(kLtsThreads), 0, stream, refPos, warpPos, nActive, matrix, P, Q,
+                           percent, affine, status);
+    else
+        hipLaunchKernelGGL(k_lts, dim3(1), dim3(kLtsThreads), 0, stream, refPos, warpPos, nActive, matrix, P, Q, res,
+                           percent, affine, status);
 }
 
 // The LTS fits of a batch of pairs in one launch: one workgroup per pair.
@@ -765,9 +815,8 @@ void launch_block_matching(nrh_aladin *h) {
 }
 
 void launch_lts(nrh_aladin *h, bool affine) {
-    hipLaunchKernelGGL(k_lts, dim3(1), dim3(kLtsThreads), 0, h->stream, h->refPos.p, h->warpPos.p, h->activeBlocks,
-                       h->matrix.p, h->ltsP.p, h->ltsQ.p, h->ltsRes.p, h->params.inlier_lts, affine ? 1 : 0,
-                       h->counters.p + 1);
+    launch_lts(h->stream, h->refPos.p, h->warpPos.p, h->activeBlocks, h->matrix.p, h->ltsP.p, h->ltsQ.p,
+               h->ltsRes.p, h->params.inlier_lts, affine ? 1 : 0, h->counters.p + 1);
     NRH_HIP(hipGetLastError());
 }
 
@@ -1241,8 +1290,7 @@ int nrh_lts_static(const float *ref_pos, const float *warp_pos, int n, int inlie
         uint32_t *res = (uint32_t *)(Q + 3 * (size_t)n);
         hipStream_t st = (hipStream_t)stream;
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, status);
-        hipLaunchKernelGGL(k_lts, dim3(1), dim3(kLtsThreads), 0, st, ref_pos, warp_pos, n, matrix, P, Q, res,
-                           inlier_lts, affine ? 1 : 0, status);
+        launch_lts(st, ref_pos, warp_pos, n, matrix, P, Q, res, inlier_lts, affine ? 1 : 0, status);
         NRH_HIP(hipGetLastError());
     });
 }
@@ -1273,8 +1321,7 @@ int nrh_lts(const float *ref_pos, const float *warp_pos, int n, int inlier_lts, 
         NRH_HIP(hipMemcpyAsync(wp.p, warp_pos, bytes, hipMemcpyHostToDevice, stream));
         NRH_HIP(hipMemcpyAsync(mat.p, m, 16 * sizeof(float), hipMemcpyHostToDevice, stream));
         NRH_HIP(hipMemsetAsync(status.p, 0, sizeof(int), stream));
-        hipLaunchKernelGGL(k_lts, dim3(1), dim3(kLtsThreads), 0, stream, rp.p, wp.p, n, mat.p, P.p, Q.p, res.p,
-                           inlier_lts, affine ? 1 : 0, status.p);
+        launch_lts(stream, rp.p, wp.p, n, mat.p, P.p, Q.p, res.p, inlier_lts, affine ? 1 : 0, status.p);
         NRH_HIP(hipGetLastError());
         int st = 0;
         float out[16];

@@ -330,6 +330,22 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Coordinates of linear index i on an nx x ny x (any) grid, x fastest: 32-bit divisions while the
+// index fits (a 64-bit division by a run-time value is a long emulated sequence on the GPU).
+__device__ __forceinline__ void index_xyz(size_t i, int nx, int ny, int& x, int& y, int& z) {
+    if (i <= 0xffffffffull) {
+        const uint32_t j = (uint32_t)i, row = j / (uint32_t)nx;
+        x = (int)(j - row * (uint32_t)nx);
+        y = (int)(row % (uint32_t)ny);
+        z = (int)(row / (uint32_t)ny);
+    } else {
+        const size_t row = i / (size_t)nx;
+        x = (int)(i - row * (size_t)nx);
+        y = (int)(row % (size_t)ny);
+        z = (int)(row / (size_t)ny);
+    }
+}
+
 // Node position in float, reg_getDisplacementFromDeformation_3D (reg-lib/cpu/_reg_tools.cpp:2004-2019)
 __device__ __forceinline__ float node_pos(const Affine& m, int r, int x, int y, int z) {
 #pragma clang fp contract(off)

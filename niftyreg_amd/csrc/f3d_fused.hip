@@ -776,7 +776,8 @@ constexpr int kTapBatch = 20;
 
 __device__ __forceinline__ void project_y_at(const ProjYArgs& p, const size_t t) {
     if (t >= (size_t)p.nz * p.gny * p.gnx) return;
-    const int ix = (int)(t % p.gnx), iy = (int)((t / p.gnx) % p.gny), z = (int)(t / ((size_t)p.gnx * p.gny));
+    int ix, iy, z;
+    index_xyz(t, p.gnx, p.gny, ix, iy, z);
     const size_t planeIn = (size_t)p.nz * p.ny * p.gnx, planeOut = (size_t)p.nz * p.gny * p.gnx;
     const int2 r = p.rangeY[iy];
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
@@ -834,7 +835,8 @@ __device__ __forceinline__ float length3(float x, float y, float z) {
 // Returns the float bits of the final gradient's length at node n (0 past the grid).
 __device__ __forceinline__ uint32_t project_nodes_at(const NodeArgs& r, const size_t n) {
     if (n >= r.G) return 0u;
-    const int i = (int)(n % r.gnx), j = (int)((n / r.gnx) % r.gny), k = (int)(n / ((size_t)r.gnx * r.gny));
+    int i, j, k;
+    index_xyz(n, r.gnx, r.gny, i, j, k);
     const float inv = r.invNx[i] * r.invNy[j] * r.invNz[k];
     float u0 = 0.f, u1 = 0.f, u2 = 0.f;
     if (inv != 0.f) {

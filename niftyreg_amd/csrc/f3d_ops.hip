@@ -332,6 +332,29 @@ k_nmi_voxel_gradient(const float *__restrict__ ref, const float *__restrict__ wa
     for (int k = 0; k < 3; ++k) grad[i + k * N] += d[k];
 }
 
+// The same voxel term with the log histogram staged in LDS once per workgroup (its 16 joint and 8
+// marginal look-ups per voxel then hit LDS): a persistent grid over the voxels, identical
+// arithmetic.  logLds: rb * fb + rb + fb doubles of dynamic LDS.
+__global__ void __launch_bounds__(kBlock)
+k_nmi_voxel_gradient_lds(const float *__restrict__ ref, const float *__restrict__ war, const float *__restrict__ wg,
+                         const int *__restrict__ mask, size_t N, int rb, int fb, const double *__restrict__ logh,
+                         const double *__restrict__ ent, double weight, float *__restrict__ grad) {
+#pragma clang fp contract(off)
+    extern __shared__ double logLds[];
+    const int total = rb * fb + rb + fb;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) logLds[i] = logh[i];
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+        if (mask && mask[i] < 0) continue;
+        const float rv = ref[i], wv = war[i];
+        if (!(rv == rv && wv == wv)) continue;
+        const float g[3] = { wg[i], wg[i + N], wg[i + 2 * N] };
+        float d[3];
+        nmi_voxel_term(rv, wv, g, logLds, ent, rb, fb, weight, d);
+        for (int k = 0; k < 3; ++k) grad[i + k * N] += d[k];
+    }
+}
+
 // ------------------------------------------------------------------ convolution
 // Density preparation of reg_tools_kernelConvolution (reg-lib/cpu/_reg_tools.cpp:860-866).
 __global__ void __launch_bounds__(kBlock)
@@ -356,7 +379,8 @@ k_conv_axis(const float *__restrict__ data, const float *__restrict__ dens, int 
     const size_t N = (size_t)nx * ny * nz;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    int x, y, z;
+    index_xyz(i, nx, ny, x, y, z);
     const int L = axis == 0 ? nx : axis == 1 ? ny : nz;
     const int l = axis == 0 ? x : axis == 1 ? y : z;
     const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)nx : (size_t)nx * ny;
@@ -407,7 +431,8 @@ k_voxel_to_node(const float *__restrict__ vg, int nx, int ny, int nz, int gnx, i
     const size_t G = (size_t)gnx * gny * gnz, N = (size_t)nx * ny * nz;
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+    int x, y, z;
+    index_xyz(n, gnx, gny, x, y, z);
     const float3 vc = affine_apply_d(T, (float)x, (float)y, (float)z);
     const int pre[3] = { floor_int(vc.x), floor_int(vc.y), floor_int(vc.z) };
     float bx[2], by[2], bz[2];
@@ -574,7 +599,8 @@ round_trip_block(float *__restrict__ cpp, int gnx, int gny, int gnz, Affine xyz,
     const size_t G = (size_t)gnx * gny * gnz;
     const size_t n = (size_t)bid * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+    int x, y, z;
+    index_xyz(n, gnx, gny, x, y, z);
     for (int k = 0; k < 3; ++k) {
         const float node = node_pos(xyz, k, x, y, z);
         cpp[n + k * G] = (cpp[n + k * G] - node) + node;
@@ -634,7 +660,8 @@ k_le_terms(const float *__restrict__ cpp, int gnx, int gny, int gnz, LeBasis B, 
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0;
     if (n < G) {
-        const int x = (int)(n % gnx), y = (int)((n / gnx) % gny), z = (int)(n / ((size_t)gnx * gny));
+        int x, y, z;
+    index_xyz(n, gnx, gny, x, y, z);
         float d0 = 0.f, d1 = 0.f, d2 = 0.f;
         if (x > 0 && y > 0 && z > 0 && x < gnx - 1 && y < gny - 1 && z < gnz - 1) {
             const M33 M = le_matrix(cpp, G, gnx, gny, x, y, z, sB, reo);
@@ -683,7 +710,8 @@ k_le_grad(const float *__restrict__ diag, int gnx, int gny, int gnz, LeBasis B, 
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = n < G;
     const size_t m = valid ? n : 0;
-    const int x = (int)(m % gnx), y = (int)((m / gnx) % gny), z = (int)(m / ((size_t)gnx * gny));
+    int x, y, z;
+    index_xyz(m, gnx, gny, x, y, z);
     float g0 = grad[m], g1 = grad[m + G], g2 = grad[m + 2 * G];
     for (int c = 1; c > -2; --c)
         for (int b = 1; b > -2; --b)
@@ -1181,9 +1209,18 @@ void op_nmi_voxel_gradient(nrh_f3d *h, double weight) {
     ensure_voxel_grad(h);
     NRH_REQUIRE(!h->warped.empty() && !h->warpedGrad.empty(), NRH_ERR_STATE,
                 "NMI voxel gradient needs the warped image and its gradient");
-    hipLaunchKernelGGL(k_nmi_voxel_gradient, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, ref_ptr(h),
-                       h->warped.p, h->warpedGrad.p, h->mask.p, h->N, h->params.ref_bins, h->params.flo_bins,
-                       h->histLog.p, h->ent.p, weight, h->voxelGrad.p);
+    const size_t logBytes = (size_t)h->totalBins() * sizeof(double);
+    if (logBytes <= 64 * 1024) {
+        // four workgroups per CU on 256 CUs, each staging the log histogram once
+        const unsigned grid = std::min(blocks_for(h->N), 1024u);
+        hipLaunchKernelGGL(k_nmi_voxel_gradient_lds, dim3(grid), dim3(kBlock), logBytes, h->stream, ref_ptr(h),
+                           h->warped.p, h->warpedGrad.p, h->mask.p, h->N, h->params.ref_bins, h->params.flo_bins,
+                           h->histLog.p, h->ent.p, weight, h->voxelGrad.p);
+    } else {
+        hipLaunchKernelGGL(k_nmi_voxel_gradient, dim3(blocks_for(h->N)), dim3(kBlock), 0, h->stream, ref_ptr(h),
+                           h->warped.p, h->warpedGrad.p, h->mask.p, h->N, h->params.ref_bins, h->params.flo_bins,
+                           h->histLog.p, h->ent.p, weight, h->voxelGrad.p);
+    }
     NRH_HIP(hipGetLastError());
 }
 

@@ -137,7 +137,8 @@ __device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, co
                                         float3& disp, float3& pos) {
 #pragma clang fp contract(off)
     const size_t ii = active ? i : 0;
-    const int x = (int)(ii % f.nx), y = (int)((ii / f.nx) % f.ny), z = (int)(ii / ((size_t)f.nx * f.ny));
+    int x, y, z;
+    index_xyz(ii, f.nx, f.ny, x, y, z);
     const float3 p0 = id_pos(f.xyz, x, y, z);
     const float3 p = a.pre ? mul_d(a.A, p0.x, p0.y, p0.z) : p0;
     const float3 v = to_vox(g.ijk, p);
@@ -306,7 +307,8 @@ __global__ void __launch_bounds__(kBlock) k_vel_scale(const float *disp, FieldGe
 #pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= f.n) return;
-    const int x = (int)(i % f.nx), y = (int)((i / f.nx) % f.ny), z = (int)(i / ((size_t)f.nx * f.ny));
+    int x, y, z;
+    index_xyz(i, f.nx, f.ny, x, y, z);
     const float3 p = id_pos(f.xyz, x, y, z);
     const double inv = 1.0 / scaling;  // exact: scaling is +-2^s
     out[i] = (float)((double)disp[i] * inv) + p.x;
@@ -323,7 +325,8 @@ __global__ void __launch_bounds__(kBlock) k_vel_affine_parts(float *fields, int 
 #pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= f.n) return;
-    const int x = (int)(i % f.nx), y = (int)((i / f.nx) % f.ny), z = (int)(i / ((size_t)f.nx * f.ny));
+    int x, y, z;
+    index_xyz(i, f.nx, f.ny, x, y, z);
     const float3 p0 = id_pos(f.xyz, x, y, z);
     const float3 ao = a.pre ? mul_d(a.AS, (double)x, (double)y, (double)z) : make_float3(0.f, 0.f, 0.f);
     float3 td = make_float3(0.f, 0.f, 0.f);
@@ -440,7 +443,8 @@ __global__ void __launch_bounds__(kBlock) k_resample_gradient(ResampleGradArgs r
 #pragma clang fp contract(off)
     const size_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= r.n) return;
-    const int x = (int)(i % r.nx), y = (int)((i / r.nx) % r.ny), z = (int)(i / ((size_t)r.nx * r.ny));
+    int x, y, z;
+    index_xyz(i, r.nx, r.ny, x, y, z);
     const float3 fc = to_vox(r.gradIjk, make_float3(r.def[i], r.def[i + r.n], r.def[i + 2 * r.n]));
     const float fcv[3] = { fc.x, fc.y, fc.z };
     int lo[3], hi[3];
@@ -600,7 +604,8 @@ __global__ void __launch_bounds__(kBlock) k_symmetrise(float *__restrict__ fw, f
 #pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= g.n) return;
-    const int x = (int)(i % g.nx), y = (int)((i / g.nx) % g.ny), z = (int)(i / ((size_t)g.nx * g.ny));
+    int x, y, z;
+    index_xyz(i, g.nx, g.ny, x, y, z);
     const float3 p = id_pos(g.xyz, x, y, z);
     const float pv[3] = { p.x, p.y, p.z };
 #pragma unroll
