@@ -237,6 +237,18 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]);
  * (reg-lib/_reg_f3d.cpp:395-411) and the maximal length returned; with max_length NULL the call
  * only enqueues the work on the handle's stream (the downloads wait for it). */
 int nrh_f3d_gradient(nrh_f3d *h, double *max_length);
+/* The similarity stages of the two calls above, for callers that assemble the objective
+ * themselves (the HIP platform plugin):
+ * nrh_f3d_similarity_value: reg_base::WarpFloatingImage + the measure value of
+ * ComputeSimilarityMeasure (reg-lib/_reg_base.cpp:469-495, :688-709) at the current control
+ * points, unweighted (NMI, or LNCC); NMI runs the fused histogram sweep when the geometry allows it
+ * (no field or warped image written).
+ * nrh_f3d_similarity_gradient: reg_f3d::GetSimilarityMeasureGradient (reg-lib/_reg_f3d.cpp:
+ * 352-360): the voxel-based measure gradient (time-point weight 1), convolved and extracted at the
+ * nodes with `weight` into the node gradient (overwritten); NMI runs the fused gradient sweep and
+ * the separable projection when the geometry allows it.  Only enqueues. */
+int nrh_f3d_similarity_value(nrh_f3d *h, double *value);
+int nrh_f3d_similarity_gradient(nrh_f3d *h, float weight);
 /* 1 when the handle's geometry takes the fused sweeps (integer grid spacing >= 4 voxels with
  * nodes on voxels), 0 when objective / gradient run the per-operation chain. */
 int nrh_f3d_fused_available(nrh_f3d *h, int *available);

@@ -39,14 +39,17 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
                 op_resample(h);
                 op_lncc_value(h, h->scalars.p + 4);
             } else if (beFused) {
-                fused_histogram(h, nullptr, true);
+                // with no linear energy to wait for, the finalisation publishes the results itself
+                if (p.le_weight > 0) fused_histogram(h, nullptr, true);
+                else fused_histogram(h, nullptr, true, h->scalars.p, 1);
             } else {
                 histogram_at_current_points(h);
             }
         }
+        const bool published = beFused && p.le_weight <= 0;
         if (p.be_weight > 0 && !beFused) op_bending_energy(h, h->scalars.p + 0);
         if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
-        op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
+        if (!published) op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
         wait_published(h);
         collect_sweep_times(h);
         const double *r = h->published;
@@ -58,6 +61,58 @@ int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
             terms[0] = wMeasure;
             terms[1] = wBE;
             terms[2] = wLE;
+        }
+    });
+}
+
+int nrh_f3d_similarity_value(nrh_f3d *h, double *value) {
+    return guard([&] {
+        NRH_REQUIRE(h && value, NRH_ERR_ARG, "null argument");
+        NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_objective");
+        const bool lncc = h->params.measure == NRH_MEASURE_LNCC;
+        if (lncc) {
+            // reg_f3d::WarpFloatingImage then reg_lncc::GetSimilarityMeasureValueFw
+            op_spline_field(h);
+            op_resample(h);
+            op_lncc_value(h, h->scalars.p + 4);
+        } else {
+            histogram_at_current_points(h);
+        }
+        op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, nullptr, 0, nullptr, 0);
+        wait_published(h);
+        collect_sweep_times(h);
+        *value = h->published[4];
+    });
+}
+
+int nrh_f3d_similarity_gradient(nrh_f3d *h, float weight) {
+    return guard([&] {
+        NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
+        NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_gradient");
+        if (h->params.measure == NRH_MEASURE_LNCC) {
+            op_spline_field(h);
+            op_resample(h);
+            op_image_gradient(h);
+            ensure_voxel_grad(h);
+            NRH_HIP(hipMemsetAsync(h->voxelGrad.p, 0, h->voxelGrad.bytes(), h->stream));
+            op_lncc_voxel_gradient(h, 1.0);
+            if (fused_projection_supported(h)) {
+                fused_project(h, h->voxelGrad.p, weight);
+            } else {
+                op_convolve_voxel_gradient(h);
+                op_voxel_to_node(h, weight);
+            }
+        } else if (fused_supported(h)) {
+            if (h->histVersion != h->cppVersion) fused_histogram(h);
+            fused_gradient(h, weight, 0.f, nullptr);
+        } else {
+            histogram_at_current_points(h);
+            op_image_gradient(h);
+            ensure_voxel_grad(h);
+            NRH_HIP(hipMemsetAsync(h->voxelGrad.p, 0, h->voxelGrad.bytes(), h->stream));
+            op_nmi_voxel_gradient(h, 1.0);
+            op_convolve_voxel_gradient(h);
+            op_voxel_to_node(h, weight);
         }
     });
 }
@@ -78,8 +133,14 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
             ensure_voxel_grad(h);
             NRH_HIP(hipMemsetAsync(h->voxelGrad.p, 0, h->voxelGrad.bytes(), h->stream));
             op_lncc_voxel_gradient(h, 1.0);
-            op_convolve_voxel_gradient(h);
-            op_voxel_to_node(h, p.similarity_weight);
+            // convolution + node extraction: the separable projection when the nodes lie on
+            // voxels at an integer spacing, else the reference-order convolution
+            if (fused_projection_supported(h)) {
+                fused_project(h, h->voxelGrad.p, p.similarity_weight);
+            } else {
+                op_convolve_voxel_gradient(h);
+                op_voxel_to_node(h, p.similarity_weight);
+            }
         } else if (p.similarity_weight > 0) {
             if (fused_supported(h)) {
                 // the bending-energy gradient sums run on the side stream beside the projection

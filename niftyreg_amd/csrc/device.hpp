@@ -280,6 +280,28 @@ __device__ __forceinline__ void nmi_voxel_term(float rv, float wv, const float g
     }
     // the joint log values are loaded together before the sums
     double jlog[4][4];
+    // every lane's 4 x 4 bins in range (the loop bounds r1, w1 give exactly four) and no NaN
+    // gradient component: the same sums without the per-pair checks
+    const bool full = r1 == r0 + 4 && r0 >= 0 && r0 + 3 < rb && w1 == w0 + 4 && w0 >= 0 && w0 + 3 < fb &&
+                      g[0] == g[0] && g[1] == g[1] && g[2] == g[2];
+    if (__all(full)) {
+#pragma unroll
+        for (int kr = 0; kr < 4; ++kr)
+#pragma unroll
+            for (int kw = 0; kw < 4; ++kw) jlog[kr][kw] = logh[(r0 + kr) + (w0 + kw) * rb];
+#pragma unroll
+        for (int kr = 0; kr < 4; ++kr)
+#pragma unroll
+            for (int kw = 0; kw < 4; ++kw) {
+                const double common = bR[kr] * dW[kw];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    rd[d] += common * (double)g[d] * rl[kr];
+                    wd[d] += common * (double)g[d] * wl[kw];
+                    jd[d] += common * (double)g[d] * jlog[kr][kw];
+                }
+            }
+    } else {
 #pragma unroll
     for (int kr = 0; kr < 4; ++kr)
 #pragma unroll
@@ -305,6 +327,7 @@ __device__ __forceinline__ void nmi_voxel_term(float rv, float wv, const float g
                 }
             }
         }
+    }
     }
     const double den = ent[2] * ent[3];
     for (int d = 0; d < 3; ++d) out[d] = (float)(weight * (rd[d] + wd[d] - nmi * jd[d]) / den);

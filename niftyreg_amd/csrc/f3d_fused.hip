@@ -1216,7 +1216,7 @@ static size_t sweep_lds(const FusedArgs& a) {
     return std::max<size_t>(((size_t)a.rb * a.fb + 3) / 4 * 4, (size_t)3 * (WZM + TZ) * WYM * WXM) * sizeof(float);
 }
 
-void fused_histogram(nrh_f3d *h, float *fieldOut, bool beValue) {
+void fused_histogram(nrh_f3d *h, float *fieldOut, bool beValue, const double *pubS1, int pubN1) {
     FusedPlan *p = plan_of(h);
     NRH_REQUIRE(fused_supported(h), NRH_ERR_UNSUPPORTED, "fused path unavailable for this geometry or bin count");
     FusedArgs a = make_args(h, p);
@@ -1263,7 +1263,7 @@ void fused_histogram(nrh_f3d *h, float *fieldOut, bool beValue) {
     NRH_HIP(hipGetLastError());
     // the finalisation also writes the gradient sweep's NMI table
     p->kTab.alloc((size_t)(a.rb * a.fb + 3) / 4 * 4);
-    op_nmi_finalize(h, p->kTab.p);
+    op_nmi_finalize(h, p->kTab.p, pubS1, pubN1);
     h->histVersion = h->cppVersion;
 }
 

@@ -34,6 +34,7 @@ struct nrh_f3d {
 
     // joint histogram: integer counts, reg_nmi's Pro / Log arrays, entropies
     nrh::DevBuf<uint32_t> histCount;
+    nrh::DevBuf<uint32_t> finTicket;  // the multi-workgroup finalisation's ticket (zero between launches)
     nrh::DevBuf<double> histPro, histLog;
     nrh::DevBuf<double> ent;       // H_r, H_f, H_rf, total, nmi
     nrh::DevBuf<float> histLogF;   // rb*fb joint log + rb + fb marginal logs, float
@@ -58,6 +59,7 @@ struct nrh_f3d {
     nrh::DevBuf<float> velTmp;      // flow / ping-pong field [3][N]
     nrh::DevBuf<float> velFields;   // intermediate fields of the gradient exponentiation
     nrh::DevBuf<uint32_t> velKeys;  // min / max keys of the flow displacement
+    nrh::DevBuf<uint8_t> velTileFlags;  // flow tiles left to the node-by-node pass
     hipEvent_t evPair = nullptr;    // cross-stream ordering of a forward / backward pair
     // the deformation field holds the exponentiation of the grid at velFieldVersion (0: it does
     // not) with velFieldSteps squarings; velFieldUpdated: the step number was chosen from the flow
@@ -111,7 +113,7 @@ void op_image_gradient(nrh_f3d *h);
 void op_joint_histogram(nrh_f3d *h, const float *warped);
 // kTab (optional): also the fused gradient sweep's NMI table, when the histogram fits the LDS
 // finalisation (h->kTabCurrent says whether it was written)
-void op_nmi_finalize(nrh_f3d *h, float *kTab = nullptr);
+void op_nmi_finalize(nrh_f3d *h, float *kTab = nullptr, const double *pubS1 = nullptr, int pubN1 = -1);
 void op_nmi_voxel_gradient(nrh_f3d *h, double weight);
 void op_convolve_voxel_gradient(nrh_f3d *h);
 void op_voxel_to_node(nrh_f3d *h, float weight);
@@ -164,7 +166,10 @@ void op_symmetrise_velocity(nrh_f3d *fw, nrh_f3d *bw);
 bool fused_supported(nrh_f3d *h);
 void fused_release(const nrh_f3d *h);
 // beValue: also the bending energy (into scalars[0]), in the exact pass's launch
-void fused_histogram(nrh_f3d *h, float *fieldOut = nullptr, bool beValue = false);
+// pubN1 >= 0: the finalisation also publishes the entropies and NMI followed by pubS1[0..pubN1)
+// (op_publish's layout, the sequence number advanced)
+void fused_histogram(nrh_f3d *h, float *fieldOut = nullptr, bool beValue = false, const double *pubS1 = nullptr,
+                     int pubN1 = -1);
 // beWeight > 0: the bending-energy gradient sums run on the side stream beside the x and y
 // projection passes (op_bending_energy_gradient_raw); the node projection joins and adds them,
 // round-trips the points and, with maxBits, tracks the final gradient's max length

@@ -134,7 +134,7 @@ k_nmi_finalize(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict
     __syncthreads();
     // along the floating axis back into pro
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-        const int r = i % rb, f = i / rb;
+        const int f = i / rb;
         double v = 0;
         if (f - 1 > -1) v += logh[i - rb] * k0;
         v += logh[i] * k1;
@@ -201,19 +201,16 @@ __device__ __forceinline__ float nmi_table_entry(const double *L, int nb, int rb
 constexpr int kFinLds = 7936;
 constexpr int kFinPer = (kFinLds + 1023) / 1024;
 
-__device__ __forceinline__ void
-nmi_finalize_lds_block(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
-                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
-                   float *__restrict__ kTab = nullptr) {
-    __shared__ double P[kFinLds];
-    __shared__ double red[16];
-    __shared__ double totalS;
+// Phase 1: P (LDS) = the smoothed joint histogram normalised by its total, then its marginals;
+// consume: the counts are zeroed as they are read, ready for the next histogram.
+__device__ __forceinline__ void fin_probabilities(uint32_t *__restrict__ counts, bool consume, int rb, int fb,
+                                                  double *P, double *red, double& totalS) {
     const int nb = rb * fb, total = nb + rb + fb;
     const double k0 = 1.0 / 6.0, k1 = (double)(2.f / 3.f);
     // the counts are consumed: zeroed as they are read, ready for the next histogram
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         P[i] = i < nb ? (double)counts[i] : 0.0;
-        if (i < nb) counts[i] = 0u;
+        if (consume && i < nb) counts[i] = 0u;
     }
     __syncthreads();
     double v[kFinPer];
@@ -278,19 +275,33 @@ nmi_finalize_lds_block(uint32_t *__restrict__ counts, int rb, int fb, double *__
         P[nb + i] = m;
     }
     __syncthreads();
+}
+
+// Phase 2: the entropies (each thread's strided terms, then block sums: H_r, H_f, H_rf), ent[] and
+// the table K.  COMPUTE: the logarithms are taken here and written with the probabilities;
+// otherwise other workgroups wrote them (logh) and they are read back.  P holds the logs after.
+template <bool COMPUTE>
+__device__ __forceinline__ void fin_entropies(int rb, int fb, double *P, double *red, double& totalS,
+                                              double *__restrict__ pro, double *__restrict__ logh,
+                                              float *__restrict__ logf, double *__restrict__ ent,
+                                              float *__restrict__ kTab) {
+    const int nb = rb * fb, total = nb + rb + fb;
+    const double act = totalS;
     double hr = 0, hf = 0, hj = 0;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const double p = P[i];
         double l = 0;
         if (p > 0) {
-            l = log(p);
+            l = COMPUTE ? log(p) : logh[i];
             if (i < nb) hj -= p * l;
             else if (i < nb + rb) hr -= p * l;
             else hf -= p * l;
         }
-        pro[i] = p;
-        logh[i] = l;
-        logf[i] = (float)l;
+        if (COMPUTE) {
+            pro[i] = p;
+            logh[i] = l;
+            logf[i] = (float)l;
+        }
         P[i] = l;  // the logs stay in LDS for the table K
     }
     hr = block_sum_double(hr, red);
@@ -313,6 +324,18 @@ nmi_finalize_lds_block(uint32_t *__restrict__ counts, int rb, int fb, double *__
         const double nmi = totalS;
         for (int i = threadIdx.x; i < nb; i += blockDim.x) kTab[i] = nmi_table_entry(P, nb, rb, nmi, i);
     }
+}
+
+// The whole finalisation in one workgroup.
+__device__ __forceinline__ void
+nmi_finalize_lds_block(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
+                   float *__restrict__ kTab = nullptr) {
+    __shared__ double P[kFinLds];
+    __shared__ double red[16];
+    __shared__ double totalS;
+    fin_probabilities(counts, true, rb, fb, P, red, totalS);
+    fin_entropies<true>(rb, fb, P, red, totalS, pro, logh, logf, ent, kTab);
 }
 
 // reg_getVoxelBasedNmiGradient3d<float> (reg-lib/cpu/_reg_nmi.cpp:501-593); grad +=
@@ -903,11 +926,52 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 
 // ------------------------------------------------------------------ single-pair kernels
 // Each wraps the block body above with the launch's own block index and count.
+// The finalisation over kFinGroups workgroups: each one forms the probabilities (phase 1, the
+// same arithmetic in every workgroup; the counts are only read) and takes the logarithms of its
+// share of the entries; the last one to finish (the ticket) zeroes the counts and runs phase 2 on
+// the logarithms the others wrote, so the results equal the one-workgroup finalisation's bit for
+// bit.  pub (may be null): the last workgroup then publishes ent[0..4] and s1[0..n1) to coherent
+// host memory followed by the sequence number seq (k_publish's layout).
+constexpr int kFinGroups = 8;
+
 __global__ void __launch_bounds__(1024)
-k_nmi_finalize_lds(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
-                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
-                   float *__restrict__ kTab) {
-    nmi_finalize_lds_block(counts, rb, fb, pro, logh, logf, ent, kTab);
+k_nmi_finalize_multi(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+                     double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
+                     float *__restrict__ kTab, uint32_t *__restrict__ ticket, double *__restrict__ pub,
+                     const double *__restrict__ s1, int n1, uint64_t seq) {
+    __shared__ double P[kFinLds];
+    __shared__ double red[16];
+    __shared__ double totalS;
+    __shared__ int sLast;
+    const int total = rb * fb + rb + fb;
+    fin_probabilities(counts, false, rb, fb, P, red, totalS);
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int lo = (int)blockIdx.x * per, hi = min(total, lo + per);
+    for (int i = lo + (int)threadIdx.x; i < hi; i += blockDim.x) {
+        const double p = P[i];
+        const double l = p > 0 ? log(p) : 0.0;
+        pro[i] = p;
+        logh[i] = l;
+        logf[i] = (float)l;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) sLast = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
+    __syncthreads();
+    if (!sLast) return;
+    __threadfence();
+    for (int i = threadIdx.x; i < rb * fb; i += blockDim.x) counts[i] = 0u;
+    fin_entropies<false>(rb, fb, P, red, totalS, pro, logh, logf, ent, kTab);
+    if (threadIdx.x == 0) *ticket = 0u;
+    if (pub) {
+        __syncthreads();
+        const int i = threadIdx.x;
+        if (i < 5) pub[i] = ent[i];
+        else if (i < 5 + n1) pub[i] = s1[i - 5];
+        __threadfence_system();
+        __syncthreads();
+        if (i == 0) __hip_atomic_store((uint64_t *)(pub + 15), seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 __global__ void __launch_bounds__(256)
@@ -1193,15 +1257,20 @@ void op_joint_histogram(nrh_f3d *h, const float *warped) {
     NRH_HIP(hipGetLastError());
 }
 
-void op_nmi_finalize(nrh_f3d *h, float *kTab) {
+void op_nmi_finalize(nrh_f3d *h, float *kTab, const double *pubS1, int pubN1) {
     h->kTabCurrent = kTab != nullptr && h->totalBins() <= kFinLds;
-    if (h->totalBins() <= kFinLds)
-        hipLaunchKernelGGL(k_nmi_finalize_lds, dim3(1), dim3(1024), 0, h->stream, h->histCount.p,
+    NRH_REQUIRE(pubN1 < 0 || (h->totalBins() <= kFinLds && pubN1 <= 10), NRH_ERR_ARG, "finalisation cannot publish");
+    if (h->totalBins() <= kFinLds) {
+        NRH_REQUIRE(!h->finTicket.empty(), NRH_ERR_STATE, "histogram buffers not allocated");
+        const bool pub = pubN1 >= 0;
+        hipLaunchKernelGGL(k_nmi_finalize_multi, dim3(kFinGroups), dim3(1024), 0, h->stream, h->histCount.p,
                            h->params.ref_bins, h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p,
-                           h->kTabCurrent ? kTab : nullptr);
-    else
+                           h->kTabCurrent ? kTab : nullptr, h->finTicket.p, pub ? h->publishedDev : nullptr, pubS1,
+                           pub ? pubN1 : 0, pub ? ++h->publishSeq : (uint64_t)0);
+    } else {
         hipLaunchKernelGGL(k_nmi_finalize, dim3(1), dim3(1024), 0, h->stream, h->histCount.p, h->params.ref_bins,
                            h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);
+    }
     NRH_HIP(hipGetLastError());
 }
 

@@ -40,14 +40,19 @@ k_lncc_prepare(const float *__restrict__ ref, const float *__restrict__ war, con
     A[5 * N + i] = ok ? 1.f : 0.f;
 }
 
-// One separable pass of reg_tools_kernelConvolution along `axis` for K planar arrays.
+// One separable pass of reg_tools_kernelConvolution along `axis` for K <= 6 planar arrays.  The K
+// sums of a voxel advance together tap by tap (K independent FP64 chains in flight, each in the
+// reference's tap order).
+constexpr int kConvArrays = 6;
+
 __global__ void __launch_bounds__(kB)
 k_conv_axis(const float *__restrict__ in, float *__restrict__ out, int K, size_t N, int nx, int ny, int nz, int axis,
             const float *__restrict__ kern, int radius) {
 #pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((size_t)nx * ny));
+    int x, y, z;
+    index_xyz(i, nx, ny, x, y, z);
     int l, L;
     size_t stride;
     if (axis == 0) { l = x; L = nx; stride = 1; }
@@ -57,11 +62,104 @@ k_conv_axis(const float *__restrict__ in, float *__restrict__ out, int K, size_t
     const int pst = min(l + radius + 1, L);
     if (pre < 0) { k0 = -pre; pre = 0; }
     const size_t base = i - (size_t)l * stride;
+    double acc[kConvArrays] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+    for (int k = pre, kk = k0; k < pst; ++k, ++kk) {
+        const float kv = kern[kk];
+        const float *src = in + base + (size_t)k * stride;
+#pragma unroll
+        for (int a = 0; a < kConvArrays; ++a)
+            if (a < K) acc[a] += (double)(kv * src[(size_t)a * N]);
+    }
+#pragma unroll
+    for (int a = 0; a < kConvArrays; ++a)
+        if (a < K) out[(size_t)a * N + i] = (float)acc[a];
+}
+
+// Line-tiled passes.  A workgroup stages a stretch of lines (outputs plus `radius` samples on each
+// side, zero outside the image) of one array at a time in LDS, so each input sample is fetched
+// from memory about once per pass instead of once per tap.  Every output is still summed tap by
+// tap in ascending order in double; the zero samples outside the line add +0 to a sum that starts
+// at +0 and cannot become -0, so each sum equals the reference's over the in-range taps only.
+constexpr int kLineOut = 64;      // outputs per workgroup along y or z (four waves of 16)
+constexpr int kRowsX = 32;        // rows per workgroup for the x pass (four waves of 8)
+constexpr int kMaxTileRadius = 96;
+
+size_t lds_yz(int radius) { return (size_t)(kLineOut + 2 * radius) * 64 * sizeof(float); }
+size_t lds_x(int radius) { return (size_t)kRowsX * (64 + 2 * radius) * sizeof(float); }
+
+// y or z pass: 64 neighbouring x columns of one plane (blockIdx.y) and kLineOut positions of the
+// line (blockIdx.z); LDS tile [kLineOut + 2 radius][64], lanes on x, conflict-free reads.
+__global__ void __launch_bounds__(kB)
+k_conv_lines_yz(const float *__restrict__ in, float *__restrict__ out, int K, size_t N, int nx, int L,
+                size_t outerStride, size_t lineStride, const float *__restrict__ kern, int radius) {
+#pragma clang fp contract(off)
+    extern __shared__ float tile[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
+    const bool col = x < nx;
+    const size_t base = (size_t)blockIdx.y * outerStride + (size_t)blockIdx.x * 64 + lane;
+    const int t0 = blockIdx.z * kLineOut;
+    const int rows = kLineOut + 2 * radius, taps = 2 * radius + 1;
     for (int a = 0; a < K; ++a) {
         const float *src = in + (size_t)a * N + base;
-        double s = 0.0;
-        for (int k = pre, kk = k0; k < pst; ++k, ++kk) s += (double)(kern[kk] * src[(size_t)k * stride]);
-        out[(size_t)a * N + i] = (float)s;
+        for (int p = w; p < rows; p += kB / 64) {
+            const int pos = t0 - radius + p;
+            tile[p * 64 + lane] = col && pos >= 0 && pos < L ? src[(size_t)pos * lineStride] : 0.f;
+        }
+        __syncthreads();
+        float *dst = out + (size_t)a * N + base;
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            const int o0 = w * 16 + g * 8;
+            const float *c = tile + o0 * 64 + lane;
+            double acc[8] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+            for (int kk = 0; kk < taps; ++kk) {
+                const float kv = kern[kk];
+#pragma unroll
+                for (int o = 0; o < 8; ++o) acc[o] += (double)(kv * c[(o + kk) * 64]);
+            }
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int pos = t0 + o0 + o;
+                if (col && pos < L) dst[(size_t)pos * lineStride] = (float)acc[o];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// x pass: kRowsX rows (row = y + ny z, blockIdx.x) and 64 outputs of each (blockIdx.y); LDS tile
+// [kRowsX][64 + 2 radius], lanes on x.
+__global__ void __launch_bounds__(kB)
+k_conv_lines_x(const float *__restrict__ in, float *__restrict__ out, int K, size_t N, int nx, int nrows,
+               const float *__restrict__ kern, int radius) {
+#pragma clang fp contract(off)
+    extern __shared__ float tile[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int W = 64 + 2 * radius, taps = 2 * radius + 1;
+    const int x0 = blockIdx.y * 64, row0 = blockIdx.x * kRowsX;
+    for (int a = 0; a < K; ++a) {
+        const float *src = in + (size_t)a * N;
+        for (int idx = threadIdx.x; idx < kRowsX * W; idx += kB) {
+            const int r = idx / W, p = idx - r * W;
+            const int xx = x0 - radius + p, row = row0 + r;
+            tile[idx] = xx >= 0 && xx < nx && row < nrows ? src[(size_t)row * nx + xx] : 0.f;
+        }
+        __syncthreads();
+        const int r0 = w * (kRowsX / 4);
+        const float *c = tile + r0 * W + lane;
+        double acc[kRowsX / 4] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+        for (int kk = 0; kk < taps; ++kk) {
+            const float kv = kern[kk];
+#pragma unroll
+            for (int r = 0; r < kRowsX / 4; ++r) acc[r] += (double)(kv * c[r * W + kk]);
+        }
+        float *dst = out + (size_t)a * N;
+        const int x = x0 + lane;
+#pragma unroll
+        for (int r = 0; r < kRowsX / 4; ++r)
+            if (x < nx && row0 + r0 + r < nrows) dst[(size_t)(row0 + r0 + r) * nx + x] = (float)acc[r];
+        __syncthreads();
     }
 }
 
@@ -114,15 +212,25 @@ k_lncc_value(const float *__restrict__ C, const uint8_t *__restrict__ comb, size
 // Sum of the block partials in block order; out[0] = sum / count, out[1] = count.
 __global__ void k_lncc_finish(const double *__restrict__ partSum, const unsigned long long *__restrict__ partCnt,
                               int nb, double *__restrict__ out) {
-    if (threadIdx.x != 0) return;
+    // one wave: lane l sums its run of partials in order, then a fixed tree over the lanes
+    const int l = threadIdx.x, per = (nb + 63) / 64;
     double s = 0.0;
     unsigned long long c = 0;
-    for (int b = 0; b < nb; ++b) {
-        s += partSum[b];
-        c += partCnt[b];
+    for (int k = 0; k < per; ++k) {
+        const int b = l * per + k;
+        if (b < nb) {
+            s += partSum[b];
+            c += partCnt[b];
+        }
     }
-    out[0] = s / (double)c;
-    out[1] = (double)c;
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    if (l == 0) {
+        out[0] = s / (double)c;
+        out[1] = (double)c;
+    }
 }
 
 // The three per-voxel derivative terms of reg_getVoxelBasedLnccGradient (:441-470), written as
@@ -177,10 +285,10 @@ k_lncc_terms(const float *__restrict__ C, const uint8_t *__restrict__ comb, size
 
 __global__ void k_lncc_weight(const unsigned long long *__restrict__ partCnt, int nb, double weight,
                               double *__restrict__ out) {
-    if (threadIdx.x != 0) return;
     unsigned long long c = 0;
-    for (int b = 0; b < nb; ++b) c += partCnt[b];
-    out[0] = weight / (double)c;
+    for (int b = threadIdx.x; b < nb; b += 64) c += partCnt[b];
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if (threadIdx.x == 0) out[0] = weight / (double)c;
 }
 
 // grad -= warped gradient * (T1 ref - T2 war + T3) * adjusted weight (:486-500), then non-finite
@@ -241,12 +349,29 @@ void ensure_lncc(nrh_f3d *h) {
 
 // Three axis passes over K + 1 planar arrays starting in `a`; returns the buffer holding the result.
 float *convolve_set(nrh_f3d *h, float *a, float *b, int K1) {
+    NRH_REQUIRE(K1 >= 1 && K1 <= kConvArrays, NRH_ERR_ARG, "at most six arrays per convolution pass");
     const int dims[3] = { h->ref.nx, h->ref.ny, h->ref.nz };
     float *src = a, *dst = b;
     for (int axis = 0; axis < 3; ++axis) {
-        if (dims[axis] <= 1 || h->lnccRadius[axis] <= 0) continue;
-        hipLaunchKernelGGL(k_conv_axis, dim3(blocks_of(h->N)), dim3(kB), 0, h->stream, src, dst, K1, h->N, dims[0],
-                           dims[1], dims[2], axis, h->lnccKern.p + h->lnccKernOff[axis], h->lnccRadius[axis]);
+        const int r = h->lnccRadius[axis];
+        if (dims[axis] <= 1 || r <= 0) continue;
+        const float *kern = h->lnccKern.p + h->lnccKernOff[axis];
+        if (r > kMaxTileRadius) {
+            hipLaunchKernelGGL(k_conv_axis, dim3(blocks_of(h->N)), dim3(kB), 0, h->stream, src, dst, K1, h->N,
+                               dims[0], dims[1], dims[2], axis, kern, r);
+        } else if (axis == 0) {
+            const int nrows = dims[1] * dims[2];
+            hipLaunchKernelGGL(k_conv_lines_x, dim3((nrows + kRowsX - 1) / kRowsX, (dims[0] + 63) / 64),
+                               dim3(kB), lds_x(r), h->stream, src, dst, K1, h->N, dims[0], nrows, kern, r);
+        } else {
+            const size_t plane = (size_t)dims[0] * dims[1];
+            const int L = dims[axis], outer = axis == 1 ? dims[2] : dims[1];
+            const size_t outerStride = axis == 1 ? plane : (size_t)dims[0];
+            const size_t lineStride = axis == 1 ? (size_t)dims[0] : plane;
+            hipLaunchKernelGGL(k_conv_lines_yz, dim3((dims[0] + 63) / 64, outer, (L + kLineOut - 1) / kLineOut),
+                               dim3(kB), lds_yz(r), h->stream, src, dst, K1, h->N, dims[0], L, outerStride,
+                               lineStride, kern, r);
+        }
         NRH_HIP(hipGetLastError());
         std::swap(src, dst);
     }

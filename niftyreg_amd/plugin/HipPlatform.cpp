@@ -1,6 +1,7 @@
 // HIP platform classes over libnrhip's C ABI; see HipPlatform.hpp.
 #include "HipPlatform.hpp"
 
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -70,7 +71,42 @@ HipF3dContent::HipF3dContent(NiftiImage& referenceIn, NiftiImage& floatingIn, Ni
     const nrh_geom r = GeomOf(reference), f = GeomOf(floating), c = GeomOf(controlPointGrid);
     Check(nrh_f3d_create(&r, static_cast<const float*>(reference->data), &f, static_cast<const float*>(floating->data),
                          referenceMask, &c, &params, &handle), "nrh_f3d_create");
+    const char *fused = std::getenv("NRH_PLUGIN_FUSED");
+    fusing = fused && std::strcmp(fused, "1") == 0;
     HipF3dContent::UpdateControlPointGrid();
+}
+
+void HipF3dContent::Materialise() {
+    const int p = pending;
+    if (!p) return;
+    pending = 0;
+    if (p & SField) {
+        Check(nrh_f3d_deformation_field(handle), "nrh_f3d_deformation_field");
+        onDevice |= Field;
+    }
+    if (p & SWarp) {
+        Check(nrh_f3d_resample(handle), "nrh_f3d_resample");
+        onDevice |= Warped;
+    }
+    if (p & SImageGradient) {
+        Check(nrh_f3d_image_gradient(handle), "nrh_f3d_image_gradient");
+        onDevice |= WarpedGradient;
+    }
+    if (p & SZeroVoxel) {
+        Check(nrh_f3d_zero_voxel_gradient(handle), "nrh_f3d_zero_voxel_gradient");
+        onDevice |= VoxelGradient;
+    }
+    if (p & SVoxelGradient) {
+        // the measure's voxel gradient at time-point weight 1 (the only weight the route defers)
+        if (params.measure == NRH_MEASURE_LNCC) {
+            Check(nrh_f3d_lncc_voxel_gradient(handle, 1.0), "nrh_f3d_lncc_voxel_gradient");
+        } else {
+            double nmi = 0, ent[4];
+            Check(nrh_f3d_nmi_value(handle, &nmi, ent), "nrh_f3d_nmi_value");
+            Check(nrh_f3d_nmi_voxel_gradient(handle, 1.0), "nrh_f3d_nmi_voxel_gradient");
+        }
+    }
+    if (p & SConvolved) Check(nrh_f3d_convolve_voxel_gradient(handle), "nrh_f3d_convolve_voxel_gradient");
 }
 
 HipF3dContent::~HipF3dContent() {
@@ -100,18 +136,21 @@ NiftiImage& HipF3dContent::GetTransformationGradient() {
 }
 
 NiftiImage& HipF3dContent::GetDeformationField() {
+    Materialise();
     if (onDevice & Field)
         Check(nrh_f3d_download_field(handle, static_cast<float*>(deformationField->data)), "nrh_f3d_download_field");
     return deformationField;
 }
 
 NiftiImage& HipF3dContent::GetWarped() {
+    Materialise();
     if (onDevice & Warped)
         Check(nrh_f3d_download_warped(handle, static_cast<float*>(warped->data)), "nrh_f3d_download_warped");
     return warped;
 }
 
 NiftiImage& HipF3dContent::GetWarpedGradient() {
+    Materialise();
     if (onDevice & WarpedGradient)
         Check(nrh_f3d_download_warped_gradient(handle, static_cast<float*>(warpedGradient->data)),
           "nrh_f3d_download_warped_gradient");
@@ -119,6 +158,7 @@ NiftiImage& HipF3dContent::GetWarpedGradient() {
 }
 
 NiftiImage& HipF3dContent::GetVoxelBasedMeasureGradient() {
+    Materialise();
     if (onDevice & VoxelGradient)
         Check(nrh_f3d_download_voxel_gradient(handle, static_cast<float*>(voxelBasedMeasureGradient->data)),
           "nrh_f3d_download_voxel_gradient");
@@ -140,33 +180,43 @@ void HipF3dContent::ZeroTransformationGradient() {
 }
 
 void HipF3dContent::ZeroVoxelBasedMeasureGradient() {
+    if ((pending & (SField | SWarp)) == (SField | SWarp) && !(pending & ~(SField | SWarp | SImageGradient))) {
+        Defer(SZeroVoxel);
+        return;
+    }
+    Materialise();
     Check(nrh_f3d_zero_voxel_gradient(handle), "nrh_f3d_zero_voxel_gradient");
     onDevice |= VoxelGradient;
 }
 
 void HipF3dContent::SetDeformationField(NiftiImage&& deformationFieldIn) {
+    Materialise();
     F3dContent::SetDeformationField(std::move(deformationFieldIn));
     onDevice &= ~Field;
     EnsureFieldOnDevice();
 }
 
 void HipF3dContent::UpdateDeformationField() {
+    Materialise();
     onDevice &= ~Field;
     EnsureFieldOnDevice();
 }
 
 void HipF3dContent::UpdateVoxelBasedMeasureGradient() {
+    Materialise();
     onDevice &= ~VoxelGradient;
     EnsureVoxelGradientOnDevice();
 }
 
 void HipF3dContent::EnsureFieldOnDevice() {
+    Materialise();
     if (onDevice & Field) return;
     Check(nrh_f3d_upload_field(handle, static_cast<const float*>(deformationField->data)), "nrh_f3d_upload_field");
     onDevice |= Field;
 }
 
 void HipF3dContent::EnsureVoxelGradientOnDevice() {
+    Materialise();
     if (onDevice & VoxelGradient) return;
     Check(nrh_f3d_upload_voxel_gradient(handle, static_cast<const float*>(voxelBasedMeasureGradient->data)),
           "nrh_f3d_upload_voxel_gradient");
@@ -252,8 +302,14 @@ nrh_f3d* HipCompute::Handle() {
 void HipCompute::ResampleImage(int interpolation, float paddingValue) {
     if (interpolation != 1) NR_FATAL_ERROR("Only linear interpolation is supported on the HIP platform");
     if (!(paddingValue != paddingValue)) NR_FATAL_ERROR("The HIP resampling pads with NaN");
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    if (c.Pending() == HipF3dContent::SField) {
+        c.Defer(HipF3dContent::SWarp);
+        return;
+    }
+    c.Materialise();
     Check(nrh_f3d_resample(Handle()), "nrh_f3d_resample");
-    dynamic_cast<HipF3dContent&>(con).MarkOnDevice(HipF3dContent::Warped);
+    c.MarkOnDevice(HipF3dContent::Warped);
 }
 
 double HipCompute::ApproxBendingEnergy() {
@@ -278,8 +334,16 @@ void HipCompute::ApproxLinearEnergyGradient(float weight) {
 
 void HipCompute::GetDeformationField(bool composition, bool bspline) {
     if (composition || !bspline) NR_FATAL_ERROR("Only the non-composed cubic B-spline field is supported on HIP");
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    if (c.Fusing()) {
+        // the stages recorded so far belong to the same control points and were not read
+        c.DropPending();
+        c.Defer(HipF3dContent::SField);
+        return;
+    }
+    c.Materialise();
     Check(nrh_f3d_deformation_field(Handle()), "nrh_f3d_deformation_field");
-    dynamic_cast<HipF3dContent&>(con).MarkOnDevice(HipF3dContent::Field);
+    c.MarkOnDevice(HipF3dContent::Field);
 }
 
 void HipCompute::UpdateControlPointPosition(float*, const float*, const float*, const float scale,
@@ -290,8 +354,15 @@ void HipCompute::UpdateControlPointPosition(float*, const float*, const float*, 
 
 void HipCompute::GetImageGradient(int interpolation, float, int activeTimePoint) {
     if (interpolation != 1 || activeTimePoint != 0) NR_FATAL_ERROR("Linear, single time point gradient only on HIP");
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    const int fw = HipF3dContent::SField | HipF3dContent::SWarp;
+    if (c.Pending() == fw || c.Pending() == (fw | HipF3dContent::SZeroVoxel)) {
+        c.Defer(HipF3dContent::SImageGradient);
+        return;
+    }
+    c.Materialise();
     Check(nrh_f3d_image_gradient(Handle()), "nrh_f3d_image_gradient");
-    dynamic_cast<HipF3dContent&>(con).MarkOnDevice(HipF3dContent::WarpedGradient);
+    c.MarkOnDevice(HipF3dContent::WarpedGradient);
 }
 
 double HipCompute::GetMaximalLength(bool optimiseX, bool optimiseY, bool optimiseZ) {
@@ -307,11 +378,22 @@ void HipCompute::NormaliseGradient(double maxGradLength, bool optimiseX, bool op
 }
 
 void HipCompute::ConvolveVoxelBasedMeasureGradient(float weight) {
+    HipF3dContent& c = dynamic_cast<HipF3dContent&>(con);
+    const int all = HipF3dContent::SField | HipF3dContent::SWarp | HipF3dContent::SImageGradient |
+                    HipF3dContent::SZeroVoxel | HipF3dContent::SVoxelGradient;
+    if (c.Pending() == all) {
+        // the whole similarity gradient in one call; the voxel gradient stays recorded (convolved)
+        Check(nrh_f3d_similarity_gradient(Handle(), weight), "nrh_f3d_similarity_gradient");
+        c.Defer(HipF3dContent::SConvolved);
+        return;
+    }
+    c.Materialise();
     Check(nrh_f3d_convolve_voxel_gradient(Handle()), "nrh_f3d_convolve_voxel_gradient");
     VoxelCentricToNodeCentric(weight);
 }
 
 void HipCompute::VoxelCentricToNodeCentric(float weight) {
+    dynamic_cast<HipF3dContent&>(con).Materialise();
     Check(nrh_f3d_voxel_to_node(Handle(), weight), "nrh_f3d_voxel_to_node");
 }
 
@@ -421,12 +503,18 @@ void reg_nmi_hip::InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, in
 
 double reg_nmi_hip::GetSimilarityMeasureValueFw() {
     double nmi = 0, ent[4];
+    if (!isSymmetric && content->Pending() == (HipF3dContent::SField | HipF3dContent::SWarp)) {
+        Check(nrh_f3d_similarity_value(content->GetHandle(), &nmi), "nrh_f3d_similarity_value");
+        return timePointWeights[0] * nmi;
+    }
+    content->Materialise();
     Check(nrh_f3d_nmi_value(content->GetHandle(), &nmi, ent), "nrh_f3d_nmi_value");
     return timePointWeights[0] * nmi;
 }
 
 double reg_nmi_hip::GetSimilarityMeasureValueBw() {
     double nmi = 0, ent[4];
+    contentBw->Materialise();
     Check(nrh_f3d_nmi_value(contentBw->GetHandle(), &nmi, ent), "nrh_f3d_nmi_value");
     return timePointWeights[0] * nmi;
 }
@@ -434,6 +522,12 @@ double reg_nmi_hip::GetSimilarityMeasureValueBw() {
 void reg_nmi_hip::GetVoxelBasedSimilarityMeasureGradientFw(int currentTimePoint) {
     // reg_nmi::GetVoxelBasedSimilarityMeasureGradientFw (reg-lib/cpu/_reg_nmi.cpp:622-637): the
     // joint histograms (both directions when symmetric) of the current warped images first
+    const int staged = HipF3dContent::SField | HipF3dContent::SWarp | HipF3dContent::SImageGradient |
+                       HipF3dContent::SZeroVoxel;
+    if (!isSymmetric && timePointWeights[currentTimePoint] == 1 && content->Pending() == staged) {
+        content->Defer(HipF3dContent::SVoxelGradient);
+        return;
+    }
     this->GetSimilarityMeasureValue();
     content->EnsureVoxelGradientOnDevice();
     Check(nrh_f3d_nmi_voxel_gradient(content->GetHandle(), timePointWeights[currentTimePoint]),
@@ -470,17 +564,29 @@ void reg_lncc_hip::InitialiseMeasure(nifti_image *refImg, nifti_image *floImg, i
 
 double reg_lncc_hip::GetSimilarityMeasureValueFw() {
     double v = 0;
+    if (!isSymmetric && content->Pending() == (HipF3dContent::SField | HipF3dContent::SWarp)) {
+        Check(nrh_f3d_similarity_value(content->GetHandle(), &v), "nrh_f3d_similarity_value");
+        return timePointWeights[0] * v;
+    }
+    content->Materialise();
     Check(nrh_f3d_lncc_value(content->GetHandle(), &v), "nrh_f3d_lncc_value");
     return timePointWeights[0] * v;
 }
 
 double reg_lncc_hip::GetSimilarityMeasureValueBw() {
     double v = 0;
+    contentBw->Materialise();
     Check(nrh_f3d_lncc_value(contentBw->GetHandle(), &v), "nrh_f3d_lncc_value");
     return timePointWeights[0] * v;
 }
 
 void reg_lncc_hip::GetVoxelBasedSimilarityMeasureGradientFw(int currentTimePoint) {
+    const int staged = HipF3dContent::SField | HipF3dContent::SWarp | HipF3dContent::SImageGradient |
+                       HipF3dContent::SZeroVoxel;
+    if (!isSymmetric && timePointWeights[currentTimePoint] == 1 && content->Pending() == staged) {
+        content->Defer(HipF3dContent::SVoxelGradient);
+        return;
+    }
     content->EnsureVoxelGradientOnDevice();
     Check(nrh_f3d_lncc_voxel_gradient(content->GetHandle(), timePointWeights[currentTimePoint]),
           "nrh_f3d_lncc_voxel_gradient");

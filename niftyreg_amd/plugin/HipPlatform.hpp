@@ -77,10 +77,30 @@ public:
     enum Buffer { Field = 1, Warped = 2, WarpedGradient = 4, VoxelGradient = 8 };
     void MarkOnDevice(int buffers) { onDevice |= buffers; }
 
+    // Fused route (environment variable NRH_PLUGIN_FUSED=1 at construction; F3D with a cubic
+    // B-spline grid).  reg_f3d's stages of one objective value (field, warp, measure value) or one
+    // similarity gradient (field, warp, image gradient, zeroed and measured voxel gradient,
+    // convolution and node extraction) are recorded instead of run, and the measure value / the
+    // convolution run them as one call (nrh_f3d_similarity_value / _gradient: the fused sweeps when
+    // the geometry allows it).  A reader of any recorded buffer runs the recorded stages first with
+    // the per-operation calls (Materialise) at the current control points: what the
+    // per-operation route computes, unless the points changed since the stages were recorded (the
+    // bending-energy gradient's round trip, an update), where the per-operation route would hold
+    // the buffers of the old points.  reg_f3d reads them only after recomputing them.
+    enum Stage { SField = 1, SWarp = 2, SImageGradient = 4, SZeroVoxel = 8, SVoxelGradient = 16, SConvolved = 32 };
+    bool Fusing() const { return fusing; }
+    int Pending() const { return pending; }
+    void Defer(int stage) { pending |= stage; }
+    void Materialise();
+    void DropPending() { pending = 0; }  // a new chain of stages starts
+    int Measure() const { return params.measure; }
+
 private:
     nrh_f3d *handle = nullptr;
     nrh_f3d_params params{};
     int onDevice = 0;
+    bool fusing = false;
+    int pending = 0;
 };
 
 class HipF3dContentCreator: public F3dContentCreator {

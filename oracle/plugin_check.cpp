@@ -10,6 +10,7 @@
 #include "HipPlatform.hpp"
 #include "ref_nifti.hpp"
 
+#include <chrono>
 #include <limits>
 #include <memory>
 
@@ -80,6 +81,52 @@ int plugin_f3d_eval(int hip, const nr_geom *ref, const float *refData, const nr_
 // convolved and projected, penalty gradients of both grids (NMI, or LNCC with lncc != 0).  cppFw / cppBw are updated in
 // place (the bending-energy gradient rounds them through displacements); steps[0..1] receive
 // the grids' intent_p2.  measure = NMI_fw + NMI_bw; be = BE_fw + BE_bw (+ 1000 (LE_fw + LE_bw)).
+// Wall time of `iters` evaluations (objective value, similarity gradient, bending-energy
+// gradient: the calls of one reg_f3d iteration without the line search) after one untimed
+// evaluation, on either platform; the content, compute and measure are built once.  With hip = 1
+// the device work is finished before the clock stops (the node gradient is downloaded).
+int plugin_f3d_time(int hip, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
+                    const nr_geom *cpp, const float *cppData, int bins, int iters, double *seconds) {
+    try {
+        NiftiImage r = make_image(ref, refData), f = make_image(flo, floData), g = make_grid(cpp, cppData);
+        std::unique_ptr<F3dContentCreator> cc(hip ? new HipF3dContentCreator() : new F3dContentCreator());
+        std::unique_ptr<F3dContent> con(cc->Create(r, f, g));
+        std::unique_ptr<ComputeFactory> cf(hip ? new HipComputeFactory() : new ComputeFactory());
+        std::unique_ptr<Compute> comp(cf->Produce(*con));
+        std::unique_ptr<MeasureCreator> mc(hip ? new HipMeasureCreator() : new MeasureCreator());
+        std::unique_ptr<reg_measure> m(mc->Create(MeasureType::Nmi));
+        dynamic_cast<reg_nmi&>(*m).SetReferenceBinNumber(bins + 4, 0);
+        dynamic_cast<reg_nmi&>(*m).SetFloatingBinNumber(bins + 4, 0);
+        m->SetTimePointWeight(0, 1.0);
+        mc->Initialise(*m, *con);
+        const float pad = std::numeric_limits<float>::quiet_NaN();
+        double sink = 0;
+        auto eval = [&] {
+            comp->GetDeformationField(false, true);
+            comp->ResampleImage(1, pad);
+            sink += m->GetSimilarityMeasureValue();
+            sink += comp->ApproxBendingEnergy();
+            comp->GetDeformationField(false, true);
+            comp->ResampleImage(1, pad);
+            con->ZeroVoxelBasedMeasureGradient();
+            comp->GetImageGradient(1, pad, 0);
+            m->GetVoxelBasedSimilarityMeasureGradient(0);
+            comp->ConvolveVoxelBasedMeasureGradient(0.999f);
+            comp->ApproxBendingEnergyGradient(0.001f);
+        };
+        eval();
+        con->GetTransformationGradient();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < iters; ++k) eval();
+        con->GetTransformationGradient();
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return sink == sink ? 0 : 1;
+    } catch (const std::exception& e) {
+        plugin_error = e.what();
+        return 1;
+    }
+}
+
 int plugin_f3d2_eval(int hip, const nr_geom *ref, const float *refData, const nr_geom *flo, const float *floData,
                      const nr_geom *cpp, float *cppFw, float *cppBw, int bins, int lncc, float simW, float beW,
                      float leW, double *measure, double *be, float *gradFw, float *gradBw, int *steps,

@@ -450,6 +450,20 @@ def ref_nmi_at(refg, ref_img, flog, flo_img, cpp, bins=64, spacing=-5.0, levels=
         r.close()
 
 
+def ref_lncc_at(refg, ref_img, flog, flo_img, cpp, spacing=-5.0, levels=1, lncc_sigma=-5.0, scalar=False):
+    """LNCC of reg_f3d<float> itself (the reference build, no penalty terms) at the control points
+    cpp of the finest level of a `levels`-level run."""
+    r = RefF3d(refg, ref_img, flog, flo_img, be=0.0, le=0.0, spacing=spacing, levels=levels, level=levels - 1,
+               measure=1, lncc_sigma=lncc_sigma, scalar=scalar)
+    try:
+        assert 3 * r.G == np.asarray(cpp).size, "grid of another geometry"
+        r.set_cpp(cpp)
+        _, terms = r.value()
+        return terms[0] / r.weights()[0]
+    finally:
+        r.close()
+
+
 def ref_nmi_sym_at(refg, ref_img, flog, flo_img, fw, bw, bins=64, spacing=-5.0, levels=1, affine=None,
                    scalar=False):
     """Summed forward / backward NMI of reg_f3d2<float> itself (no penalty terms) at the velocity
@@ -611,6 +625,23 @@ def plugin_f3d_eval(hip, refg, ref_img, flog, flo_img, cppg, cpp, bins=64, lncc=
     if rc != 0:
         raise RuntimeError(_plugin.plugin_last_error().decode())
     return m.value, be.value, grad, vg
+
+
+def plugin_f3d_time(hip, refg, ref_img, flog, flo_img, cppg, cpp, iters=5, bins=64):
+    """Seconds for `iters` objective + gradient evaluations through the reference's own calls on
+    the CPU classes (hip=False) or the HIP plugin (hip=True), after one untimed evaluation."""
+    global _plugin
+    if _plugin is None:
+        _plugin = ctypes.CDLL(PLUGIN)
+        _plugin.plugin_last_error.restype = ctypes.c_char_p
+    sec = ctypes.c_double()
+    rc = _plugin.plugin_f3d_time(int(hip), ctypes.byref(rg(refg)), fp(np.ascontiguousarray(ref_img, np.float32).ravel()),
+                                 ctypes.byref(rg(flog)), fp(np.ascontiguousarray(flo_img, np.float32).ravel()),
+                                 ctypes.byref(rg(cppg)), fp(np.ascontiguousarray(cpp, np.float32)), bins, int(iters),
+                                 ctypes.byref(sec))
+    if rc != 0:
+        raise RuntimeError(_plugin.plugin_last_error().decode())
+    return sec.value
 
 
 # ---------------------------------------------------------------- linear energy

@@ -31,13 +31,22 @@ def l2rel(a, b):
     return float(np.linalg.norm((a - b).astype(np.float64)) / np.linalg.norm(b.astype(np.float64)))
 
 
-def build(dims, spacing_vox=5.0, seed=0, amp=2.0, masked=False, be=0.001, flo_dims=None, storage=0):
+def _oblique(dims):
+    th = 0.2
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    m = np.eye(4, dtype=np.float32)
+    m[:3, :3] = (R @ np.diag([1.1, 0.9, 1.2])).astype(np.float32)
+    m[:3, 3] = [4.0, -6.0, 2.5]
+    return Geom(dims, m)
+
+
+def build(dims, spacing_vox=5.0, seed=0, amp=2.0, masked=False, be=0.001, flo_dims=None, storage=0, oblique=False):
     rng = np.random.default_rng(seed)
-    refg = Geom.identity(dims)
+    refg = _oblique(dims) if oblique else Geom.identity(dims)
     flog = Geom.identity(flo_dims) if flo_dims else refg
     ref = phantoms.blob_phantom(refg.dims, seed).ravel()
     flo = phantoms.blob_phantom(flog.dims, seed + 5).ravel()
-    cppg, cpp = create_control_point_grid(refg, [spacing_vox] * 3)
+    cppg, cpp = create_control_point_grid(refg, [spacing_vox * p for p in refg.pixdim])
     cpp = (cpp + rng.uniform(-amp, amp, cpp.size)).astype(np.float32)
     mask = (rng.uniform(size=refg.nvox) > 0.1).astype(np.int32) - 1 if masked else None
     params = default_params(similarity_weight=1.0 - be, be_weight=be, image_storage=storage)
@@ -55,6 +64,7 @@ FUSED_CASES = {
     "masked": dict(dims=(80, 72, 40), masked=True),
     "flo_other_size": dict(dims=(70, 66, 45), flo_dims=(75, 60, 49)),
     "large_warp": dict(dims=(96, 80, 64), amp=6.0),
+    "oblique": dict(dims=(66, 60, 44), oblique=True),
 }
 
 

@@ -12,6 +12,8 @@ convolution), so the whole registration follows the CPU platform's trajectory an
 scores agree to 1e-9 (relative).  Aladin matrices are held to the bounds of the device Aladin
 tests.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -34,6 +36,55 @@ def test_reg_f3d_run_on_hip_platform(gpu):
     n_hip, n_cpu, n_scalar = (O.ref_nmi_at(g, ref, g, flo, c, levels=3) for c in (hcpp, ccpp, scpp))
     print(f"reg_f3d Run: HIP platform {n_hip:.9f}, CPU platform {n_cpu:.9f}, scalar build {n_scalar:.9f}")
     assert abs(n_hip - n_cpu) <= 1e-9 * n_cpu
+
+
+@pytest.mark.parametrize("measure", [0, 1])
+def test_reg_f3d_run_on_hip_platform_fused_route(gpu, monkeypatch, measure):
+    """NRH_PLUGIN_FUSED=1: the plugin records reg_f3d's field / warp / gradient stages and runs
+    them as nrh_f3d_similarity_value / _gradient (the fused sweeps for NMI, the separable node
+    projection for both measures).  Their rounding differs from the per-operation kernels' at
+    1e-7, so the trajectory leaves the CPU platform's bit path: the final score (NMI, or LNCC) is
+    held to the registration bound of the device driver's tests (5e-4 relative) against both
+    reference builds' spread."""
+    ref, flo = phantoms.config0_pair(0)
+    g = Geom.identity((64, 64, 64))
+    kw = dict(levels=3, maxit=50, measure=measure)
+    monkeypatch.setenv("NRH_PLUGIN_FUSED", "1")
+    fcg, fcpp = O.ref_f3d_run(g, ref, g, flo, platform=O.PLATFORM_HIP, **kw)
+    monkeypatch.delenv("NRH_PLUGIN_FUSED")
+    ccg, ccpp = O.ref_f3d_run(g, ref, g, flo, **kw)
+    _, scpp = O.ref_f3d_run(g, ref, g, flo, scalar=True, **kw)
+    assert fcg.dims == ccg.dims
+    if measure == 0:
+        n_f, n_c, n_s = (O.ref_nmi_at(g, ref, g, flo, c, levels=3) for c in (fcpp, ccpp, scpp))
+    else:
+        n_f, n_c, n_s = (O.ref_lncc_at(g, ref, g, flo, c, levels=3) for c in (fcpp, ccpp, scpp))
+    print(f"reg_f3d Run (measure {measure}): HIP fused route {n_f:.9f}, CPU platform {n_c:.9f}, "
+          f"scalar build {n_s:.9f}")
+    assert abs(n_f - n_c) <= 5e-4 * abs(n_c) + abs(n_c - n_s)
+
+
+@pytest.mark.skipif(not O.plugin_available(), reason="plugin check library not built")
+def test_plugin_evaluation_timing(gpu, monkeypatch):
+    """Objective + gradient evaluations of reg_f3d's own calls at 256x256x160 (NMI, 5-voxel grid)
+    on the reference CPU classes (scalar build, OpenMP) and on the HIP platform's per-operation and
+    fused routes: evaluations per second of each (printed), the fused route faster than the
+    per-operation one."""
+    from niftyreg_amd.geometry import create_control_point_grid
+    dims = (256, 256, 160)
+    ref, flo = phantoms.config1_pair(0, dims)
+    g = Geom.identity(dims)
+    cg, cpp = create_control_point_grid(g, [5.0, 5.0, 5.0])
+    cpp = (cpp + np.random.default_rng(0).uniform(-1, 1, cpp.size)).astype(np.float32)
+    rates = {}
+    for name, hip, fused, iters in (("hip_fused", True, "1", 40), ("hip_ops", True, None, 20), ("cpu", False, None, 2)):
+        if fused:
+            monkeypatch.setenv("NRH_PLUGIN_FUSED", fused)
+        else:
+            monkeypatch.delenv("NRH_PLUGIN_FUSED", raising=False)
+        rates[name] = iters / O.plugin_f3d_time(hip, g, ref, g, flo, cg, cpp, iters=iters)
+    print("reg_f3d evaluation calls at 256x256x160 (evals/s): " + ", ".join(f"{k} {v:.2f}" for k, v in rates.items()))
+    assert rates["hip_fused"] > rates["hip_ops"] > rates["cpu"]
 
 
 def test_reg_f3d2_run_on_hip_platform(gpu):

@@ -64,6 +64,52 @@ def test_hip_platform_matches_reference_platform(gpu, lncc, which):
     assert np.array_equal(hgrad, cgrad)
 
 
+@needs_plugin
+@pytest.mark.gpu
+@pytest.mark.parametrize("lncc", [False, True])
+@pytest.mark.parametrize("which", ["cube", "oblique"])
+def test_hip_platform_fused_route_matches_reference_platform(gpu, monkeypatch, lncc, which):
+    """NRH_PLUGIN_FUSED=1: the plugin records the field / warp / gradient stages and runs them as
+    nrh_f3d_similarity_value / _gradient: the fused sweeps for NMI, the separable node projection for
+    both measures (integer 5-voxel spacing, nodes on voxels, also on the oblique geometry).  The
+    value within 1e-6, the node gradient within the fused tolerances (1e-3 L2, 2e-3 of the largest
+    element).  The convolved voxel gradient, read through the accessor after the bending-energy
+    gradient, runs the recorded per-operation stages at the current control points, which the
+    bending energy's round trip may have moved in the last bit (not at all on the cube's integer
+    node positions): bit-exact on the cube, within 1e-3 (L2) on the oblique grid."""
+    g, ref, flo, cg, cpp = _case(geom=_oblique((34, 30, 28)) if which == "oblique" else None)
+    cm, cbe, cgrad, cvg = O.plugin_f3d_eval(False, g, ref, g, flo, cg, cpp, lncc=lncc)
+    monkeypatch.setenv("NRH_PLUGIN_FUSED", "1")
+    hm, hbe, hgrad, hvg = O.plugin_f3d_eval(True, g, ref, g, flo, cg, cpp, lncc=lncc)
+    assert abs(hbe - cbe) <= 1e-12 * abs(cbe)
+    assert abs(hm - cm) <= 1e-6 * abs(cm)
+    rel = np.linalg.norm(hgrad - cgrad) / np.linalg.norm(cgrad)
+    if not lncc:
+        # the library's own fused gradient on the same inputs, for reference
+        from niftyreg_amd.f3d import F3dContent, default_params
+        h = F3dContent(g, ref, g, flo, cg, cpp, params=default_params(similarity_weight=0.999, be_weight=0.001,
+                                                                       le_weight=0.01))
+        h.rescale_intensities((2, 65), (2, 65))
+        h.objective()
+        h.gradient()
+        lib = h.node_gradient()
+        h.close()
+        print(f"{which}: plugin fused vs CPU {rel:.2e}, library fused vs CPU "
+              f"{np.linalg.norm(lib - cgrad) / np.linalg.norm(cgrad):.2e}, plugin vs library "
+              f"{np.linalg.norm(hgrad - lib) / np.linalg.norm(lib):.2e}")
+    assert rel <= 1e-3, rel
+    assert np.abs(hgrad - cgrad).max() <= 2e-3 * np.abs(cgrad).max()
+    if which == "cube":
+        assert np.array_equal(hvg, cvg, equal_nan=True)
+    else:
+        ok = np.isfinite(cvg) & np.isfinite(hvg)
+        assert np.array_equal(np.isfinite(cvg), np.isfinite(hvg))
+        vrel = np.linalg.norm(hvg[ok] - cvg[ok]) / np.linalg.norm(cvg[ok])
+        print(f"voxel gradient after the round trip: {np.count_nonzero(hvg[ok] != cvg[ok])} of {ok.sum()} differ, "
+              f"L2 {vrel:.2e}")
+        assert vrel <= 1e-3, vrel
+
+
 def _f3d2_case(which, seed=2, amplitude=2.0):
     from niftyreg_amd.geometry import create_symmetric_control_point_grids
     if which == "cube":
