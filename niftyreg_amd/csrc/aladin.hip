@@ -393,24 +393,32 @@ __device__ void svd3(const double H[3][3], double U[3][3], double V[3][3]) {
 // cross-covariance with the reflection guard (reg-lib/cpu/Maths.cpp:351-390).  Returns false
 // (uniformly) when the normal matrix is singular.
 __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q, const uint8_t *keep, int n,
-                        bool affine, LtsShared& s) {
-    double m[7] = { 0, 0, 0, 0, 0, 0, 0 };
-    for (int i = threadIdx.x; i < n; i += kLtsThreads) {
-        if (keep && !keep[i]) continue;
-        for (int k = 0; k < 3; ++k) {
-            m[k] += (double)P[3 * i + k];
-            m[3 + k] += (double)Q[3 * i + k];
-        }
-        m[6] += 1.0;
-    }
-    wg_sum<7>(m, s);
+                        bool affine, LtsShared& s, const double *sums = nullptr) {
     double mp[3], mq[3];
-    const double cnt = s.out[6];
-    for (int k = 0; k < 3; ++k) {
-        mp[k] = s.out[k] / cnt;
-        mq[k] = s.out[3 + k] / cnt;
+    if (sums) {
+        // the kept points' sums (p, q, count), accumulated by the caller in this pass's order
+        for (int k = 0; k < 3; ++k) {
+            mp[k] = sums[k] / sums[6];
+            mq[k] = sums[3 + k] / sums[6];
+        }
+    } else {
+        double m[7] = { 0, 0, 0, 0, 0, 0, 0 };
+        for (int i = threadIdx.x; i < n; i += kLtsThreads) {
+            if (keep && !keep[i]) continue;
+            for (int k = 0; k < 3; ++k) {
+                m[k] += (double)P[3 * i + k];
+                m[3 + k] += (double)Q[3 * i + k];
+            }
+            m[6] += 1.0;
+        }
+        wg_sum<7>(m, s);
+        const double cnt = s.out[6];
+        for (int k = 0; k < 3; ++k) {
+            mp[k] = s.out[k] / cnt;
+            mq[k] = s.out[3 + k] / cnt;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     double c[15] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
     for (int i = threadIdx.x; i < n; i += kLtsThreads) {
         if (keep && !keep[i]) continue;
@@ -544,10 +552,15 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
     if (tid < 16) s.last[tid] = 0.f;
     __syncthreads();
     for (int it = 0; it < 30; ++it) {
+        // residuals, counted into the first digit's histogram as they are formed
+        for (int b = tid; b < 2048; b += kLtsThreads) s.hist[b] = 0;
+        __syncthreads();
         for (int i = tid; i < n; i += kLtsThreads) {
             const float3 t = mat44_mul_f(s.M, P[3 * i], P[3 * i + 1], P[3 * i + 2]);
             const float dx = t.x - Q[3 * i], dy = t.y - Q[3 * i + 1], dz = t.z - Q[3 * i + 2];
-            res[i] = __float_as_uint(sqrtf(dx * dx + dy * dy + dz * dz));
+            const uint32_t key = __float_as_uint(sqrtf(dx * dx + dy * dy + dz * dz));
+            res[i] = key;
+            atomicAdd(&s.hist[key >> 21], 1u);
         }
         __syncthreads();
         // radix select of the keep-th smallest residual: digits of 11, 11 and 10 bits
@@ -557,13 +570,15 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         for (int d = 0; d < 3; ++d) {
             const int sh = shifts[d];
             const uint32_t dm = (1u << widths[d]) - 1u;
-            for (int b = tid; b < 2048; b += kLtsThreads) s.hist[b] = 0;
-            __syncthreads();
-            for (int i = tid; i < n; i += kLtsThreads) {
-                const uint32_t key = res[i];
-                if ((key & prefMask) == prefix) atomicAdd(&s.hist[(key >> sh) & dm], 1u);
+            if (d > 0) {
+                for (int b = tid; b < 2048; b += kLtsThreads) s.hist[b] = 0;
+                __syncthreads();
+                for (int i = tid; i < n; i += kLtsThreads) {
+                    const uint32_t key = res[i];
+                    if ((key & prefMask) == prefix) atomicAdd(&s.hist[(key >> sh) & dm], 1u);
+                }
+                __syncthreads();
             }
-            __syncthreads();
             if (tid < 64) {
                 uint32_t local = 0;
                 for (int b = 0; b < 32; ++b) local += s.hist[tid * 32 + b];
@@ -597,13 +612,23 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         }
         // keep: residual below the threshold, or equal to it within the first k in index order
         // (when every key equal to the threshold is kept, the order plays no part: one pass)
-        double dsum = 0;
+        // the distance and the kept points' sums (p, q, count) for the next fit, each thread over
+        // its indices in ascending order as lts_fit's own first pass
+        double dv[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        auto add_kept = [&](int i, uint32_t key) {
+            dv[0] += (double)__uint_as_float(key);
+            for (int c = 0; c < 3; ++c) {
+                dv[1 + c] += (double)P[3 * i + c];
+                dv[4 + c] += (double)Q[3 * i + c];
+            }
+            dv[7] += 1.0;
+        };
         if ((uint32_t)k == eqCount) {
             for (int i = tid; i < n; i += kLtsThreads) {
                 const uint32_t key = res[i];
                 const bool kept = key <= prefix;
                 keepFlag[i] = kept ? 1 : 0;
-                if (kept) dsum += (double)__uint_as_float(key);
+                if (kept) add_kept(i, key);
             }
         } else {
             int seenEqual = 0;
@@ -614,13 +639,14 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
                 const int eqPos = seenEqual + wg_scan(i < n && key == prefix, s, &total);
                 const bool kept = i < n && (key < prefix || (key == prefix && eqPos < k));
                 if (i < n) keepFlag[i] = kept ? 1 : 0;
-                if (kept) dsum += (double)__uint_as_float(key);
+                if (kept) add_kept(i, key);
                 seenEqual += total;
             }
         }
-        double dv[1] = { dsum };
-        wg_sum<1>(dv, s);
+        wg_sum<8>(dv, s);
         const double distance = s.out[0];
+        double sums[7];
+        for (int c = 0; c < 7; ++c) sums[c] = s.out[1 + c];
         if (distance > lastDistance || (lastDistance - distance) < 1e-3) {
             if (tid < 16) s.M[tid] = s.last[tid];
             __syncthreads();
@@ -629,7 +655,7 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         lastDistance = distance;
         if (tid < 16) s.last[tid] = s.M[tid];
         __syncthreads();
-        if (!lts_fit(P, Q, keepFlag, n, affine != 0, s)) {
+        if (!lts_fit(P, Q, keepFlag, n, affine != 0, s, sums)) {
             if (tid == 0) *status = 2;
             return;
         }

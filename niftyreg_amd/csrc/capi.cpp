@@ -105,9 +105,6 @@ void alloc_histogram(nrh_f3d *h) {
     h->histCount.alloc((size_t)rb * fb);
     // every histogram accumulates into zeroed counts; each finalisation zeroes them again
     NRH_HIP(hipMemsetAsync(h->histCount.p, 0, h->histCount.bytes(), h->stream));
-    // the multi-workgroup finalisation's ticket starts at zero and is reset by its last workgroup
-    h->finTicket.alloc(1);
-    NRH_HIP(hipMemsetAsync(h->finTicket.p, 0, sizeof(uint32_t), h->stream));
     NRH_HIP(hipStreamSynchronize(h->stream));
     h->histPro.alloc(h->totalBins());
     h->histLog.alloc(h->totalBins());

@@ -24,44 +24,62 @@ using namespace nrh;
 
 extern "C" {
 
+// The objective's measure, bending and linear energies published to host memory.
+static void publish_objective(nrh_f3d *h) {
+    const bool lncc = h->params.measure == NRH_MEASURE_LNCC;
+    op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
+}
+
+// reg_f3d::GetObjectiveFunctionValue (reg-lib/_reg_f3d.cpp:443-463) enqueued: the measure, the
+// bending and linear energies, and their publication to host memory.
+static void enqueue_objective(nrh_f3d *h) {
+    const nrh_f3d_params& p = h->params;
+    const bool lncc = p.measure == NRH_MEASURE_LNCC;
+    // the bending energy runs in the fused histogram's exact-pass launch
+    const bool beFused = p.be_weight > 0 && p.similarity_weight > 0 && !lncc && fused_supported(h);
+    if (p.similarity_weight > 0) {
+        if (lncc) {
+            // reg_f3d::WarpFloatingImage then reg_lncc::GetSimilarityMeasureValueFw
+            op_spline_field(h);
+            op_resample(h);
+            op_lncc_value(h, h->scalars.p + 4);
+        } else if (beFused) {
+            // with no linear energy to wait for, the finalisation publishes the results itself
+            if (p.le_weight > 0) fused_histogram(h, nullptr, true);
+            else fused_histogram(h, nullptr, true, h->scalars.p, 1);
+        } else {
+            histogram_at_current_points(h);
+        }
+    }
+    const bool published = beFused && p.le_weight <= 0;
+    if (p.be_weight > 0 && !beFused) op_bending_energy(h, h->scalars.p + 0);
+    if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
+    if (!published) publish_objective(h);
+}
+
+// The published objective: value = wMeasure - wBE - wLE, terms (may be null) the three terms.
+static void finish_objective(nrh_f3d *h, double *value, double terms[3]) {
+    const nrh_f3d_params& p = h->params;
+    wait_published(h);
+    collect_sweep_times(h);
+    const double *r = h->published;
+    const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * r[4] : 0.0;
+    const double wBE = p.be_weight > 0 ? (double)p.be_weight * r[5] : 0.0;
+    const double wLE = p.le_weight > 0 ? (double)p.le_weight * r[6] : 0.0;
+    *value = wMeasure - wBE - wLE;
+    if (terms) {
+        terms[0] = wMeasure;
+        terms[1] = wBE;
+        terms[2] = wLE;
+    }
+}
+
 int nrh_f3d_objective(nrh_f3d *h, double *value, double terms[3]) {
     return guard([&] {
         NRH_REQUIRE(h && value, NRH_ERR_ARG, "null argument");
         NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_objective");
-        const nrh_f3d_params& p = h->params;
-        const bool lncc = p.measure == NRH_MEASURE_LNCC;
-        // the bending energy runs in the fused histogram's exact-pass launch
-        const bool beFused = p.be_weight > 0 && p.similarity_weight > 0 && !lncc && fused_supported(h);
-        if (p.similarity_weight > 0) {
-            if (lncc) {
-                // reg_f3d::WarpFloatingImage then reg_lncc::GetSimilarityMeasureValueFw
-                op_spline_field(h);
-                op_resample(h);
-                op_lncc_value(h, h->scalars.p + 4);
-            } else if (beFused) {
-                // with no linear energy to wait for, the finalisation publishes the results itself
-                if (p.le_weight > 0) fused_histogram(h, nullptr, true);
-                else fused_histogram(h, nullptr, true, h->scalars.p, 1);
-            } else {
-                histogram_at_current_points(h);
-            }
-        }
-        const bool published = beFused && p.le_weight <= 0;
-        if (p.be_weight > 0 && !beFused) op_bending_energy(h, h->scalars.p + 0);
-        if (p.le_weight > 0) op_linear_energy(h, h->scalars.p + 9);
-        if (!published) op_publish(h, lncc ? h->scalars.p : h->ent.p, 5, h->scalars.p, 1, h->scalars.p + 9, 1);
-        wait_published(h);
-        collect_sweep_times(h);
-        const double *r = h->published;
-        const double wMeasure = p.similarity_weight > 0 ? (double)p.similarity_weight * r[4] : 0.0;
-        const double wBE = p.be_weight > 0 ? (double)p.be_weight * r[5] : 0.0;
-        const double wLE = p.le_weight > 0 ? (double)p.le_weight * r[6] : 0.0;
-        *value = wMeasure - wBE - wLE;
-        if (terms) {
-            terms[0] = wMeasure;
-            terms[1] = wBE;
-            terms[2] = wLE;
-        }
+        enqueue_objective(h);
+        finish_objective(h, value, terms);
     });
 }
 
@@ -117,10 +135,9 @@ int nrh_f3d_similarity_gradient(nrh_f3d *h, float weight) {
     });
 }
 
-int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
-    return guard([&] {
-        NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
-        NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_gradient");
+// reg_f3d::GetObjectiveFunctionGradient (reg-lib/_reg_f3d.cpp:553-577); with max_length, normalised
+// and the length returned (a synchronisation), else only enqueued.
+static void gradient_body(nrh_f3d *h, double *max_length) {
         const nrh_f3d_params& p = h->params;
         const bool be = p.be_weight > 0, le = p.le_weight > 0;
         bool beDone = false;
@@ -178,6 +195,13 @@ int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
         }
         // without a length to return the call only enqueues: the gradient stays on the device,
         // and the downloads (or the next call that returns a value) wait for it
+}
+
+int nrh_f3d_gradient(nrh_f3d *h, double *max_length) {
+    return guard([&] {
+        NRH_REQUIRE(h, NRH_ERR_ARG, "null handle");
+        NRH_REQUIRE(!h->velocity, NRH_ERR_UNSUPPORTED, "velocity grids are evaluated with nrh_f3d2_gradient");
+        gradient_body(h, max_length);
     });
 }
 

@@ -1155,6 +1155,10 @@ bool fused_supported(nrh_f3d *h) {
 void collect_sweep_times(nrh_f3d *h) {
     for (int k = 0; k < 2; ++k) {
         if (!h->evPending[k]) continue;
+        // a sweep still running (a gradient enqueued behind the objective) is collected later
+        const hipError_t q = hipEventQuery(h->ev[2 * k + 1]);
+        if (q == hipErrorNotReady) continue;
+        NRH_HIP(q);
         float ms = 0.f;
         NRH_HIP(hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]));
         h->msSweep[k] += ms;

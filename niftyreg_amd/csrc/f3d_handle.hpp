@@ -34,7 +34,6 @@ struct nrh_f3d {
 
     // joint histogram: integer counts, reg_nmi's Pro / Log arrays, entropies
     nrh::DevBuf<uint32_t> histCount;
-    nrh::DevBuf<uint32_t> finTicket;  // the multi-workgroup finalisation's ticket (zero between launches)
     nrh::DevBuf<double> histPro, histLog;
     nrh::DevBuf<double> ent;       // H_r, H_f, H_rf, total, nmi
     nrh::DevBuf<float> histLogF;   // rb*fb joint log + rb + fb marginal logs, float
@@ -59,7 +58,6 @@ struct nrh_f3d {
     nrh::DevBuf<float> velTmp;      // flow / ping-pong field [3][N]
     nrh::DevBuf<float> velFields;   // intermediate fields of the gradient exponentiation
     nrh::DevBuf<uint32_t> velKeys;  // min / max keys of the flow displacement
-    nrh::DevBuf<uint8_t> velTileFlags;  // flow tiles left to the node-by-node pass
     hipEvent_t evPair = nullptr;    // cross-stream ordering of a forward / backward pair
     // the deformation field holds the exponentiation of the grid at velFieldVersion (0: it does
     // not) with velFieldSteps squarings; velFieldUpdated: the step number was chosen from the flow

@@ -277,10 +277,8 @@ __device__ __forceinline__ void fin_probabilities(uint32_t *__restrict__ counts,
     __syncthreads();
 }
 
-// Phase 2: the entropies (each thread's strided terms, then block sums: H_r, H_f, H_rf), ent[] and
-// the table K.  COMPUTE: the logarithms are taken here and written with the probabilities;
-// otherwise other workgroups wrote them (logh) and they are read back.  P holds the logs after.
-template <bool COMPUTE>
+// Phase 2: the logarithms (written with the probabilities), the entropies (each thread's strided
+// terms, then block sums: H_r, H_f, H_rf), ent[] and the table K.  P holds the logs after.
 __device__ __forceinline__ void fin_entropies(int rb, int fb, double *P, double *red, double& totalS,
                                               double *__restrict__ pro, double *__restrict__ logh,
                                               float *__restrict__ logf, double *__restrict__ ent,
@@ -292,16 +290,14 @@ __device__ __forceinline__ void fin_entropies(int rb, int fb, double *P, double 
         const double p = P[i];
         double l = 0;
         if (p > 0) {
-            l = COMPUTE ? log(p) : logh[i];
+            l = log(p);
             if (i < nb) hj -= p * l;
             else if (i < nb + rb) hr -= p * l;
             else hf -= p * l;
         }
-        if (COMPUTE) {
-            pro[i] = p;
-            logh[i] = l;
-            logf[i] = (float)l;
-        }
+        pro[i] = p;
+        logh[i] = l;
+        logf[i] = (float)l;
         P[i] = l;  // the logs stay in LDS for the table K
     }
     hr = block_sum_double(hr, red);
@@ -335,7 +331,7 @@ nmi_finalize_lds_block(uint32_t *__restrict__ counts, int rb, int fb, double *__
     __shared__ double red[16];
     __shared__ double totalS;
     fin_probabilities(counts, true, rb, fb, P, red, totalS);
-    fin_entropies<true>(rb, fb, P, red, totalS, pro, logh, logf, ent, kTab);
+    fin_entropies(rb, fb, P, red, totalS, pro, logh, logf, ent, kTab);
 }
 
 // reg_getVoxelBasedNmiGradient3d<float> (reg-lib/cpu/_reg_nmi.cpp:501-593); grad +=
@@ -926,43 +922,15 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 
 // ------------------------------------------------------------------ single-pair kernels
 // Each wraps the block body above with the launch's own block index and count.
-// The finalisation over kFinGroups workgroups: each one forms the probabilities (phase 1, the
-// same arithmetic in every workgroup; the counts are only read) and takes the logarithms of its
-// share of the entries; the last one to finish (the ticket) zeroes the counts and runs phase 2 on
-// the logarithms the others wrote, so the results equal the one-workgroup finalisation's bit for
-// bit.  pub (may be null): the last workgroup then publishes ent[0..4] and s1[0..n1) to coherent
-// host memory followed by the sequence number seq (k_publish's layout).
-constexpr int kFinGroups = 8;
-
+// The single-pair finalisation (one workgroup).  pub (may be null): it then publishes ent[0..4]
+// and s1[0..n1) to coherent host memory followed by the sequence number seq (k_publish's
+// layout), so the objective needs no publishing kernel of its own.
 __global__ void __launch_bounds__(1024)
-k_nmi_finalize_multi(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
-                     double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
-                     float *__restrict__ kTab, uint32_t *__restrict__ ticket, double *__restrict__ pub,
-                     const double *__restrict__ s1, int n1, uint64_t seq) {
-    __shared__ double P[kFinLds];
-    __shared__ double red[16];
-    __shared__ double totalS;
-    __shared__ int sLast;
-    const int total = rb * fb + rb + fb;
-    fin_probabilities(counts, false, rb, fb, P, red, totalS);
-    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int lo = (int)blockIdx.x * per, hi = min(total, lo + per);
-    for (int i = lo + (int)threadIdx.x; i < hi; i += blockDim.x) {
-        const double p = P[i];
-        const double l = p > 0 ? log(p) : 0.0;
-        pro[i] = p;
-        logh[i] = l;
-        logf[i] = (float)l;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) sLast = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
-    __syncthreads();
-    if (!sLast) return;
-    __threadfence();
-    for (int i = threadIdx.x; i < rb * fb; i += blockDim.x) counts[i] = 0u;
-    fin_entropies<false>(rb, fb, P, red, totalS, pro, logh, logf, ent, kTab);
-    if (threadIdx.x == 0) *ticket = 0u;
+k_nmi_finalize_lds(uint32_t *__restrict__ counts, int rb, int fb, double *__restrict__ pro,
+                   double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
+                   float *__restrict__ kTab, double *__restrict__ pub, const double *__restrict__ s1, int n1,
+                   uint64_t seq) {
+    nmi_finalize_lds_block(counts, rb, fb, pro, logh, logf, ent, kTab);
     if (pub) {
         __syncthreads();
         const int i = threadIdx.x;
@@ -1261,12 +1229,11 @@ void op_nmi_finalize(nrh_f3d *h, float *kTab, const double *pubS1, int pubN1) {
     h->kTabCurrent = kTab != nullptr && h->totalBins() <= kFinLds;
     NRH_REQUIRE(pubN1 < 0 || (h->totalBins() <= kFinLds && pubN1 <= 10), NRH_ERR_ARG, "finalisation cannot publish");
     if (h->totalBins() <= kFinLds) {
-        NRH_REQUIRE(!h->finTicket.empty(), NRH_ERR_STATE, "histogram buffers not allocated");
         const bool pub = pubN1 >= 0;
-        hipLaunchKernelGGL(k_nmi_finalize_multi, dim3(kFinGroups), dim3(1024), 0, h->stream, h->histCount.p,
-                           h->params.ref_bins, h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p,
-                           h->kTabCurrent ? kTab : nullptr, h->finTicket.p, pub ? h->publishedDev : nullptr, pubS1,
-                           pub ? pubN1 : 0, pub ? ++h->publishSeq : (uint64_t)0);
+        hipLaunchKernelGGL(k_nmi_finalize_lds, dim3(1), dim3(1024), 0, h->stream, h->histCount.p, h->params.ref_bins,
+                           h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p,
+                           h->kTabCurrent ? kTab : nullptr, pub ? h->publishedDev : nullptr, pubS1, pub ? pubN1 : 0,
+                           pub ? ++h->publishSeq : (uint64_t)0);
     } else {
         hipLaunchKernelGGL(k_nmi_finalize, dim3(1), dim3(1024), 0, h->stream, h->histCount.p, h->params.ref_bins,
                            h->params.flo_bins, h->histPro.p, h->histLog.p, h->histLogF.p, h->ent.p);

@@ -18,7 +18,6 @@
 #include "f3d_handle.hpp"
 #include "host.hpp"
 #include "m33.hpp"
-#include <climits>
 #include <cmath>
 
 namespace nrh {
@@ -127,161 +126,167 @@ __device__ __forceinline__ uint32_t float_key(float v) {
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// Per-voxel set-up of the flow: identity position p0, the (extension-mapped) position p, the
-// spline support's first node pre and the basis values of the three axes.
-struct FlowPt {
-    float3 p0, p;
+// The velocity grid evaluated at voxel i's position: the composition branch of
+// reg_cubic_spline_getDeformationField3D (reg-lib/cpu/_reg_localTrans.cpp:903-1079, scalar
+// order) applied to the identity field, as reg_spline_getFlowFieldFromVelocityGrid (:3465-3491)
+// does.  Returns the displacement flow - x and the position x.  When every lane of the wave has
+// its 4x4x4 support inside the grid, each slab of 16 nodes is loaded at once (48 loads in flight)
+// before its terms are summed in the reference's order.  With an extension 0 the spline is
+// evaluated at A x and the displacement is taken from the affine-only field A S i in double.
+__device__ __forceinline__ void flow_at(const VelGrid& g, const FieldGeom& f, const VelAff& a, size_t i, bool active,
+                                        float3& disp, float3& pos) {
+#pragma clang fp contract(off)
+    const size_t ii = active ? i : 0;
+    int x, y, z;
+    index_xyz(ii, f.nx, f.ny, x, y, z);
+    const float3 p0 = id_pos(f.xyz, x, y, z);
+    const float3 p = a.pre ? mul_d(a.A, p0.x, p0.y, p0.z) : p0;
+    const float3 v = to_vox(g.ijk, p);
     int pre[3];
     float b[3][4];
-    bool inside;  // the 4x4x4 support lies in the grid
-};
-
-__device__ __forceinline__ void flow_point(const VelGrid& g, const FieldGeom& f, const VelAff& a, int x, int y, int z,
-                                           FlowPt& q) {
-#pragma clang fp contract(off)
-    q.p0 = id_pos(f.xyz, x, y, z);
-    q.p = a.pre ? mul_d(a.A, q.p0.x, q.p0.y, q.p0.z) : q.p0;
-    const float3 v = to_vox(g.ijk, q.p);
     const float vc[3] = { v.x, v.y, v.z };
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        q.pre[r] = floor_int(vc[r]);
-        float t = vc[r] - (float)q.pre[r];
-        q.pre[r]--;
+        pre[r] = floor_int(vc[r]);
+        float t = vc[r] - (float)pre[r];
+        pre[r]--;
         if (t < 0) t = 0;
-        bspline_basis(t, q.b[r]);
+        bspline_basis(t, b[r]);
     }
-    q.inside = q.pre[0] >= 0 && q.pre[0] + 3 < g.gnx && q.pre[1] >= 0 && q.pre[1] + 3 < g.gny && q.pre[2] >= 0 &&
-               q.pre[2] + 3 < g.gnz;
-}
-
-// The 64 terms of a support inside the grid, node (c, k >> 2, k & 3) of component m read as
-// ld(c, k, m).  Scalar build: one
-// sequential sum with weights (x y) z; SSE build (reg-lib/cpu/_reg_localTrans.cpp:1025-1052): four
-// lanes (one per x tap) with weights (y z) x, reduced left to right.
-template <class Load>
-__device__ __forceinline__ float3 flow_sums(const float (&b)[3][4], bool sse, Load ld) {
-#pragma clang fp contract(off)
+    const bool inside = pre[0] >= 0 && pre[0] + 3 < g.gnx && pre[1] >= 0 && pre[1] + 3 < g.gny && pre[2] >= 0 &&
+                        pre[2] + 3 < g.gnz;
+    // scalar build: one sequential sum with weights (x y) z; SSE build (reg-lib/cpu/
+    // _reg_localTrans.cpp:1025-1052): four lanes (one per x tap) with weights (y z) x, reduced
+    // left to right
+    const bool sse = g.order == NRH_FIELD_ORDER_SSE;
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
     float l0[4] = { 0.f, 0.f, 0.f, 0.f }, l1[4] = { 0.f, 0.f, 0.f, 0.f }, l2[4] = { 0.f, 0.f, 0.f, 0.f };
+    if (__all(inside || !active)) {
+        const size_t sy = g.gnx, sz = (size_t)g.gnx * g.gny;
+        const size_t base = active ? ((size_t)pre[2] * g.gny + pre[1]) * g.gnx + pre[0] : 0;
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < 4; ++c) {
+            float cx[16], cy[16], cz[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const float vx = ld(c, k, 0), vy = ld(c, k, 1), vz = ld(c, k, 2);
+            for (int k = 0; k < 16; ++k) {
+                const size_t idx = base + c * sz + (k >> 2) * sy + (k & 3);
+                cx[k] = g.cp[idx];
+                cy[k] = g.cp[idx + g.G];
+                cz[k] = g.cp[idx + 2 * g.G];
+            }
             if (sse) {
-                const float w = (b[1][k >> 2] * b[2][c]) * b[0][k & 3];
-                l0[k & 3] = w * vx + l0[k & 3];
-                l1[k & 3] = w * vy + l1[k & 3];
-                l2[k & 3] = w * vz + l2[k & 3];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const float w = (b[1][k >> 2] * b[2][c]) * b[0][k & 3];
+                    l0[k & 3] = w * cx[k] + l0[k & 3];
+                    l1[k & 3] = w * cy[k] + l1[k & 3];
+                    l2[k & 3] = w * cz[k] + l2[k & 3];
+                }
             } else {
-                const float w = b[0][k & 3] * b[1][k >> 2] * b[2][c];
-                r0 += vx * w;
-                r1 += vy * w;
-                r2 += vz * w;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const float w = b[0][k & 3] * b[1][k >> 2] * b[2][c];
+                    r0 += cx[k] * w;
+                    r1 += cy[k] * w;
+                    r2 += cz[k] * w;
+                }
             }
         }
+    } else if (active) {
+        for (int c = 0; c < 4; ++c) {
+            const int Z = pre[2] + c;
+            // get_GridValues (reg-lib/cpu/_reg_splineBasis.cpp:602-665): once a row of a slice
+            // falls outside the grid, the rest of that slice goes through the sliding path
+            bool out = !(Z > -1 && Z < g.gnz);
+            for (int bb = 0; bb < 4; ++bb) {
+                const int Y = pre[1] + bb;
+                if (!(Y > -1 && Y < g.gny && !out)) out = true;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int X = pre[0] + a;
+                    float3 cv;
+                    if (X > -1 && X < g.gnx && !out) {
+                        const size_t idx = ((size_t)Z * g.gny + Y) * g.gnx + X;
+                        cv.x = g.cp[idx];
+                        cv.y = g.cp[idx + g.G];
+                        cv.z = g.cp[idx + 2 * g.G];
+                    } else {
+                        cv = slided(g.cp, g.G, g.gnx, g.gny, g.gnz, g.xyz, X, Y, Z);
+                    }
+                    if (sse) {
+                        const float w = (b[1][bb] * b[2][c]) * b[0][a];
+                        l0[a] = w * cv.x + l0[a];
+                        l1[a] = w * cv.y + l1[a];
+                        l2[a] = w * cv.z + l2[a];
+                    } else {
+                        const float w = b[0][a] * b[1][bb] * b[2][c];
+                        r0 += cv.x * w;
+                        r1 += cv.y * w;
+                        r2 += cv.z * w;
+                    }
+                }
+            }
+        }
+    }
     if (sse) {
         r0 = ((l0[0] + l0[1]) + l0[2]) + l0[3];
         r1 = ((l1[0] + l1[1]) + l1[2]) + l1[3];
         r2 = ((l2[0] + l2[1]) + l2[2]) + l2[3];
     }
-    return make_float3(r0, r1, r2);
-}
-
-// The displacement flow - x from the spline sum s; with an extension 0 the affine-only field
-// A S i (double) is subtracted instead of the position.
-__device__ __forceinline__ float3 flow_displacement(const VelAff& a, int x, int y, int z, const FlowPt& q, float3 s) {
-#pragma clang fp contract(off)
     if (a.pre) {
         const float3 ao = mul_d(a.AS, (double)x, (double)y, (double)z);
-        return make_float3((float)((double)s.x - (double)ao.x), (float)((double)s.y - (double)ao.y),
-                           (float)((double)s.z - (double)ao.z));
+        disp = make_float3((float)((double)r0 - (double)ao.x), (float)((double)r1 - (double)ao.y),
+                           (float)((double)r2 - (double)ao.z));
+    } else {
+        disp = make_float3(r0 - p.x, r1 - p.y, r2 - p.z);
     }
-    return make_float3(s.x - q.p.x, s.y - q.p.y, s.z - q.p.z);
+    pos = p0;
 }
 
-// The 64 terms of any support, node by node: nodes outside the grid take the sliding values of
-// get_GridValues (reg-lib/cpu/_reg_splineBasis.cpp:602-665).
-__device__ __forceinline__ float3 flow_sums_sliding(const VelGrid& g, const FlowPt& q, bool sse) {
+// The scaled flow, the first field of the squaring (reg-lib/cpu/_reg_localTrans.cpp:3546-3561):
+// (flow - x) / scaling + x; a persistent grid-stride loop as k_vel_flow_disp.
+__global__ void __launch_bounds__(kBlock) k_vel_flow_scaled(VelGrid g, FieldGeom f, VelAff a, double scaling,
+                                                            float *__restrict__ out) {
 #pragma clang fp contract(off)
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-    float l0[4] = { 0.f, 0.f, 0.f, 0.f }, l1[4] = { 0.f, 0.f, 0.f, 0.f }, l2[4] = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int Z = q.pre[2] + c;
-        // get_GridValues (reg-lib/cpu/_reg_splineBasis.cpp:602-665): once a row of a slice
-        // falls outside the grid, the rest of that slice goes through the sliding path
-        bool out = !(Z > -1 && Z < g.gnz);
-#pragma unroll
-        for (int bb = 0; bb < 4; ++bb) {
-            const int Y = q.pre[1] + bb;
-            if (!(Y > -1 && Y < g.gny && !out)) out = true;
-#pragma unroll
-            for (int aa = 0; aa < 4; ++aa) {
-                const int X = q.pre[0] + aa;
-                float3 cv;
-                if (X > -1 && X < g.gnx && !out) {
-                    const size_t idx = ((size_t)Z * g.gny + Y) * g.gnx + X;
-                    cv.x = g.cp[idx];
-                    cv.y = g.cp[idx + g.G];
-                    cv.z = g.cp[idx + 2 * g.G];
-                } else {
-                    cv = slided(g.cp, g.G, g.gnx, g.gny, g.gnz, g.xyz, X, Y, Z);
-                }
-                if (sse) {
-                    const float w = (q.b[1][bb] * q.b[2][c]) * q.b[0][aa];
-                    l0[aa] = w * cv.x + l0[aa];
-                    l1[aa] = w * cv.y + l1[aa];
-                    l2[aa] = w * cv.z + l2[aa];
-                } else {
-                    const float w = q.b[0][aa] * q.b[1][bb] * q.b[2][c];
-                    r0 += cv.x * w;
-                    r1 += cv.y * w;
-                    r2 += cv.z * w;
-                }
-            }
+    // scaling is +-2^s: the double quotient equals the product with its exact reciprocal
+    const double inv = 1.0 / scaling;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const size_t rounds = (f.n + stride - 1) / stride;
+    for (size_t r = 0; r < rounds; ++r) {
+        const size_t i = r * stride + (size_t)blockIdx.x * kBlock + threadIdx.x;
+        const bool active = i < f.n;
+        float3 d, p;
+        flow_at(g, f, a, i, active, d, p);
+        if (active) {
+            out[i] = (float)((double)d.x * inv) + p.x;
+            out[i + f.n] = (float)((double)d.y * inv) + p.y;
+            out[i + 2 * f.n] = (float)((double)d.z * inv) + p.z;
         }
     }
-    if (sse) {
-        r0 = ((l0[0] + l0[1]) + l0[2]) + l0[3];
-        r1 = ((l1[0] + l1[1]) + l1[2]) + l1[3];
-        r2 = ((l2[0] + l2[1]) + l2[2]) + l2[3];
-    }
-    return make_float3(r0, r1, r2);
 }
 
-// Tiles of 64 x 4 voxels (one slice): the flow with the tile's control-point window staged in LDS.
-// The window is the bounding box of the tile's supports (block-wide min / max of the first
-// nodes); when every support lies in the grid and the window holds at most kFlowWin nodes, the 64
-// terms of each voxel are read from LDS (same terms, same order as from memory), else the tile
-// takes flow_at.  A persistent grid walks the tiles.  DISP: writes flow - x and folds its extremes
-// into keys[0] (min) and keys[1] (max); otherwise the scaled flow (flow - x) / scaling + x.
-constexpr int kFlowWin = 2048;
-constexpr int kFlowTileY = kBlock / 64;
-
-// DISP: flow - x and its running extremes; else the scaled flow (flow - x) / scaling + x with
-// the division as the product with the exact reciprocal of +-2^s.
-template <bool DISP>
-__device__ __forceinline__ void flow_store(const FieldGeom& f, size_t i, float3 d, float3 p0, double inv,
-                                           float *__restrict__ out, float& lo, float& hi) {
+// The flow's displacement flow - x, with its extremes folded into keys[0] (min) and keys[1]
+// (max): grid-stride loop, one block-wide reduction and one atomic pair per block.
+__global__ void __launch_bounds__(kBlock) k_vel_flow_disp(VelGrid g, FieldGeom f, VelAff a,
+                                                          float *__restrict__ out, uint32_t *__restrict__ keys) {
 #pragma clang fp contract(off)
-    if (DISP) {
-        out[i] = d.x;
-        out[i + f.n] = d.y;
-        out[i + 2 * f.n] = d.z;
-        lo = fminf(lo, fminf(d.x, fminf(d.y, d.z)));
-        hi = fmaxf(hi, fmaxf(d.x, fmaxf(d.y, d.z)));
-    } else {
-        out[i] = (float)((double)d.x * inv) + p0.x;
-        out[i + f.n] = (float)((double)d.y * inv) + p0.y;
-        out[i + 2 * f.n] = (float)((double)d.z * inv) + p0.z;
+    __shared__ float sLo[kBlock / 64], sHi[kBlock / 64];
+    float lo = INFINITY, hi = -INFINITY;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const size_t rounds = (f.n + stride - 1) / stride;
+    for (size_t r = 0; r < rounds; ++r) {
+        const size_t i = r * stride + (size_t)blockIdx.x * kBlock + threadIdx.x;
+        const bool active = i < f.n;
+        float3 d, p;
+        flow_at(g, f, a, i, active, d, p);
+        if (active) {
+            out[i] = d.x;
+            out[i + f.n] = d.y;
+            out[i + 2 * f.n] = d.z;
+            lo = fminf(lo, fminf(d.x, fminf(d.y, d.z)));
+            hi = fmaxf(hi, fmaxf(d.x, fmaxf(d.y, d.z)));
+        }
     }
-}
-
-// Block-wide min / max folded into keys[0] / keys[1] (one atomic pair per workgroup).
-__device__ __forceinline__ void fold_extremes(float lo, float hi, uint32_t *keys) {
-    __shared__ float sLo[kFlowTileY], sHi[kFlowTileY];
     for (int off = 32; off > 0; off >>= 1) {
         lo = fminf(lo, __shfl_xor(lo, off, 64));
         hi = fmaxf(hi, __shfl_xor(hi, off, 64));
@@ -292,114 +297,13 @@ __device__ __forceinline__ void fold_extremes(float lo, float hi, uint32_t *keys
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kFlowTileY; ++w) {
+        for (int w = 1; w < kBlock / 64; ++w) {
             lo = fminf(lo, sLo[w]);
             hi = fmaxf(hi, sHi[w]);
         }
         atomicMin(&keys[0], float_key(lo));
         atomicMax(&keys[1], float_key(hi));
     }
-}
-
-template <bool DISP>
-__global__ void __launch_bounds__(kBlock) k_vel_flow_tiles(VelGrid g, FieldGeom f, VelAff a, double scaling,
-                                                           float *__restrict__ out, uint32_t *__restrict__ keys,
-                                                           uint8_t *__restrict__ fallback, int tilesX, int tilesY,
-                                                           unsigned numTiles) {
-#pragma clang fp contract(off)
-    __shared__ float win[3 * kFlowWin];
-    __shared__ int sBox[kFlowTileY][8];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const bool sse = g.order == NRH_FIELD_ORDER_SSE;
-    const double inv = 1.0 / scaling;  // exact: scaling is +-2^s
-    float lo = INFINITY, hi = -INFINITY;
-    for (unsigned t = blockIdx.x; t < numTiles; t += gridDim.x) {
-        const int tx = (int)(t % (unsigned)tilesX);
-        const unsigned rest = t / (unsigned)tilesX;
-        const int ty = (int)(rest % (unsigned)tilesY), z = (int)(rest / (unsigned)tilesY);
-        const int x = tx * 64 + lane, y = ty * kFlowTileY + wv;
-        const bool active = x < f.nx && y < f.ny;
-        FlowPt q;
-        flow_point(g, f, a, active ? x : 0, active ? y : 0, z, q);
-        int mn[3], mx[3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            mn[r] = active ? q.pre[r] : INT_MAX;
-            mx[r] = active ? q.pre[r] : INT_MIN;
-            for (int off = 32; off > 0; off >>= 1) {
-                mn[r] = min(mn[r], __shfl_xor(mn[r], off, 64));
-                mx[r] = max(mx[r], __shfl_xor(mx[r], off, 64));
-            }
-        }
-        const int allIn = __all(q.inside || !active) ? 1 : 0;
-        if (lane == 0) {
-            sBox[wv][0] = mn[0]; sBox[wv][1] = mn[1]; sBox[wv][2] = mn[2];
-            sBox[wv][3] = mx[0]; sBox[wv][4] = mx[1]; sBox[wv][5] = mx[2];
-            sBox[wv][6] = allIn;
-        }
-        __syncthreads();
-        int o[3], e[3];
-        bool in = true;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            int l = INT_MAX, h = INT_MIN;
-            for (int w = 0; w < kFlowTileY; ++w) {
-                l = min(l, sBox[w][r]);
-                h = max(h, sBox[w][3 + r]);
-            }
-            o[r] = l;
-            e[r] = h - l + 4;  // wave 0 always holds an active voxel, so l <= h
-        }
-        for (int w = 0; w < kFlowTileY; ++w) in = in && sBox[w][6] != 0;
-        const int W = e[0] * e[1] * e[2];
-        const bool lds = in && W <= kFlowWin;
-        if (threadIdx.x == 0) fallback[t] = lds ? 0 : 1;
-        if (lds) {
-            for (int idx = threadIdx.x; idx < 3 * W; idx += kBlock) {
-                const int m = idx / W, rem = idx - m * W;
-                const int wx = rem % e[0], r2 = rem / e[0];
-                const int wy = r2 % e[1], wz = r2 / e[1];
-                win[idx] = g.cp[(size_t)m * g.G + ((size_t)(o[2] + wz) * g.gny + (o[1] + wy)) * g.gnx + (o[0] + wx)];
-            }
-            __syncthreads();
-            const int sy = e[0], sz = e[0] * e[1];
-            const int base = active ? ((q.pre[2] - o[2]) * e[1] + (q.pre[1] - o[1])) * e[0] + (q.pre[0] - o[0]) : 0;
-            const float3 s = flow_sums(q.b, sse, [&](int c, int k, int m) {
-                return win[m * W + base + c * sz + (k >> 2) * sy + (k & 3)];
-            });
-            const float3 d = flow_displacement(a, x, y, z, q, s);
-            if (active) flow_store<DISP>(f, ((size_t)z * f.ny + y) * f.nx + x, d, q.p0, inv, out, lo, hi);
-        }
-        __syncthreads();  // the window and the boxes are rewritten by the next tile
-    }
-    if (DISP) fold_extremes(lo, hi, keys);
-}
-
-// The tiles k_vel_flow_tiles flagged (a support leaving the grid, or a window larger than
-// kFlowWin), node by node from memory with the sliding values.
-template <bool DISP>
-__global__ void __launch_bounds__(kBlock) k_vel_flow_fallback(VelGrid g, FieldGeom f, VelAff a, double scaling,
-                                                              float *__restrict__ out, uint32_t *__restrict__ keys,
-                                                              const uint8_t *__restrict__ fallback, int tilesX,
-                                                              int tilesY, unsigned numTiles) {
-#pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const bool sse = g.order == NRH_FIELD_ORDER_SSE;
-    const double inv = 1.0 / scaling;
-    float lo = INFINITY, hi = -INFINITY;
-    for (unsigned t = blockIdx.x; t < numTiles; t += gridDim.x) {
-        if (!fallback[t]) continue;
-        const int tx = (int)(t % (unsigned)tilesX);
-        const unsigned rest = t / (unsigned)tilesX;
-        const int ty = (int)(rest % (unsigned)tilesY), z = (int)(rest / (unsigned)tilesY);
-        const int x = tx * 64 + lane, y = ty * kFlowTileY + wv;
-        if (x >= f.nx || y >= f.ny) continue;
-        FlowPt q;
-        flow_point(g, f, a, x, y, z, q);
-        const float3 d = flow_displacement(a, x, y, z, q, flow_sums_sliding(g, q, sse));
-        flow_store<DISP>(f, ((size_t)z * f.ny + y) * f.nx + x, d, q.p0, inv, out, lo, hi);
-    }
-    if (DISP) fold_extremes(lo, hi, keys);
 }
 
 // Displacement d -> d / scaling + x (reg_tools_divideValueToImage in double, then
@@ -518,77 +422,6 @@ __global__ void __launch_bounds__(kBlock) k_compose(const float *__restrict__ D,
     out[i] = a0;
     out[i + m] = a1;
     out[i + 2 * m] = a2;
-}
-
-// k_compose for arrays below 4 GiB: 32-bit byte offsets from the component planes' bases.
-__global__ void __launch_bounds__(kBlock) k_compose32(const float *__restrict__ D, FieldGeom d, const float *pos,
-                                                      size_t m, float *out) {
-#pragma clang fp contract(off)
-    const size_t ib = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    const bool active = ib < m;
-    const uint32_t io = active ? (uint32_t)ib * 4u : 0u;
-    const float *P1 = pos + m, *P2 = pos + 2 * m;
-    const float *D1 = D + d.n, *D2 = D + 2 * d.n;
-    auto ld = [](const float *b, uint32_t off) {
-        return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(b) + off);
-    };
-    const float3 real = make_float3(ld(pos, io), ld(P1, io), ld(P2, io));
-    const float3 v = to_vox(d.ijk, real);
-    const int px = floor_int(v.x), py = floor_int(v.y), pz = floor_int(v.z);
-    const float rx1 = v.x - (float)px, ry1 = v.y - (float)py, rz1 = v.z - (float)pz;
-    const float rx[2] = { 1.f - rx1, rx1 }, ry[2] = { 1.f - ry1, ry1 }, rz[2] = { 1.f - rz1, rz1 };
-    const bool inside = px > -1 && px + 1 < d.nx && py > -1 && py + 1 < d.ny && pz > -1 && pz + 1 < d.nz;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    if (__all(inside || !active) && d.nx > 1 && d.ny > 1 && d.nz > 1) {
-        const uint32_t sy = (uint32_t)d.nx * 4u, sz = (uint32_t)d.nx * (uint32_t)d.ny * 4u;
-        const uint32_t base = active ? (((uint32_t)pz * (uint32_t)d.ny + (uint32_t)py) * (uint32_t)d.nx + (uint32_t)px) * 4u : 0u;
-        float t0[8], t1[8], t2[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t o = base + (k >> 2) * sz + ((k >> 1) & 1) * sy + (k & 1) * 4u;
-            t0[k] = ld(D, o);
-            t1[k] = ld(D1, o);
-            t2[k] = ld(D2, o);
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const float w = rx[k & 1] * (ry[(k >> 1) & 1] * rz[k >> 2]);
-            a0 += t0[k] * w;
-            a1 += t1[k] * w;
-            a2 += t2[k] * w;
-        }
-    } else if (active) {
-        for (int c = 0; c < 2; ++c) {
-            const int Z = pz + c;
-            const bool inZ = Z > -1 && Z < d.nz;
-            for (int b = 0; b < 2; ++b) {
-                const int Y = py + b;
-                const float tb = ry[b] * rz[c];
-                const bool inY = Y > -1 && Y < d.ny;
-                for (int a = 0; a < 2; ++a) {
-                    const int X = px + a;
-                    float3 dv;
-                    if (X > -1 && X < d.nx && inY && inZ) {
-                        const uint32_t o = (((uint32_t)Z * (uint32_t)d.ny + (uint32_t)Y) * (uint32_t)d.nx + (uint32_t)X) * 4u;
-                        dv = make_float3(ld(D, o), ld(D1, o), ld(D2, o));
-                    } else {
-                        dv = slided(D, d.n, d.nx, d.ny, d.nz, d.xyz, X, Y, Z);
-                    }
-                    const float w = rx[a] * tb;
-                    a0 += dv.x * w;
-                    a1 += dv.y * w;
-                    a2 += dv.z * w;
-                }
-            }
-        }
-    }
-    if (!active) return;
-    auto st = [](float *b, uint32_t off, float v) {
-        *reinterpret_cast<float *>(reinterpret_cast<char *>(b) + off) = v;
-    };
-    st(out, io, a0);
-    st(out + m, io, a1);
-    st(out + 2 * m, io, a2);
 }
 
 struct ResampleGradArgs {
@@ -955,23 +788,11 @@ void launch_resample_gradient(ResampleGradArgs r, hipStream_t s) {
 }
 
 
-// The flow over persistent tiles (k_vel_flow_tiles): at most 2048 workgroups (8 per CU).
-// Then the flagged tiles (flags: one byte per tile, written by the first pass).
-size_t flow_tiles(const FieldGeom& f) {
-    return (size_t)((f.nx + 63) / 64) * ((f.ny + kFlowTileY - 1) / kFlowTileY) * f.nz;
-}
-
-template <bool DISP>
-void launch_flow(const VelGrid& g, const FieldGeom& f, const VelAff& a, double scaling, float *out, uint32_t *keys,
-                 uint8_t *flags, hipStream_t s) {
-    const int tilesX = (f.nx + 63) / 64, tilesY = (f.ny + kFlowTileY - 1) / kFlowTileY;
-    const size_t numTiles = flow_tiles(f);
-    NRH_REQUIRE(numTiles < (1ull << 31), NRH_ERR_UNSUPPORTED, "field too large for the flow tiles");
-    const unsigned grid = (unsigned)std::min<size_t>(numTiles, 2048);
-    hipLaunchKernelGGL(k_vel_flow_tiles<DISP>, dim3(grid), dim3(kBlock), 0, s, g, f, a, scaling, out, keys, flags,
-                       tilesX, tilesY, (unsigned)numTiles);
-    hipLaunchKernelGGL(k_vel_flow_fallback<DISP>, dim3(std::min<size_t>(numTiles, 512)), dim3(kBlock), 0, s, g, f, a,
-                       scaling, out, keys, flags, tilesX, tilesY, (unsigned)numTiles);
+// The flow kernels' persistent grid: enough workgroups to fill the device (8 per CU); the
+// extremes variant ends with one atomic pair per workgroup.
+inline unsigned flow_blocks(size_t n) {
+    const size_t b = (n + kBlock - 1) / kBlock;
+    return (unsigned)std::min<size_t>(b, 2048);
 }
 
 VelGrid vel_grid(const nrh_f3d *h) {
@@ -1033,11 +854,7 @@ double scaling_of(int steps) {
 }
 
 void launch_compose(const float *D, const FieldGeom& d, const float *pos, size_t m, float *out, hipStream_t s) {
-    const size_t lim = 0xffffffffull / 4;
-    if (3 * m <= lim && 3 * d.n <= lim)
-        hipLaunchKernelGGL(k_compose32, blocks8_for(m), dim3(kBlock), 0, s, D, d, pos, m, out);
-    else
-        hipLaunchKernelGGL(k_compose, blocks8_for(m), dim3(kBlock), 0, s, D, d, pos, m, out);
+    hipLaunchKernelGGL(k_compose, blocks8_for(m), dim3(kBlock), 0, s, D, d, pos, m, out);
 }
 
 // The rotation-free, spacing-normalised Jacobian factors of reg_trilinearResampleGradient for a
@@ -1106,8 +923,8 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
         const uint32_t init[2] = { 0xffffffffu, 0u };
         h->velKeys.alloc(2);
         NRH_HIP(hipMemcpyAsync(h->velKeys.p, init, sizeof init, hipMemcpyHostToDevice, h->stream));
-        h->velTileFlags.alloc(flow_tiles(f));
-        launch_flow<true>(g, f, a, 1.0, h->velTmp.p, h->velKeys.p, h->velTileFlags.p, h->stream);
+        hipLaunchKernelGGL(k_vel_flow_disp, dim3(flow_blocks(f.n)), dim3(kBlock), 0, h->stream, g, f, a,
+                           h->velTmp.p, h->velKeys.p);
         NRH_HIP(hipGetLastError());
         uint32_t keys[2];
         NRH_HIP(hipMemcpyAsync(keys, h->velKeys.p, sizeof keys, hipMemcpyDeviceToHost, h->stream));
@@ -1126,8 +943,8 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
         hipLaunchKernelGGL(k_vel_scale, blocks_for(f.n), dim3(kBlock), 0, h->stream, h->velTmp.p, f,
                            scaling_of(steps), field_k(0));
     } else {
-        h->velTileFlags.alloc(flow_tiles(f));
-        launch_flow<false>(g, f, a, scaling_of(steps), field_k(0), nullptr, h->velTileFlags.p, h->stream);
+        hipLaunchKernelGGL(k_vel_flow_scaled, dim3(flow_blocks(f.n)), dim3(kBlock), 0, h->stream, g, f, a,
+                           scaling_of(steps), field_k(0));
     }
     NRH_HIP(hipGetLastError());
     for (int k = 0; k < sq; ++k) launch_compose(field_k(k), f, field_k(k), f.n, field_k(k + 1), h->stream);
@@ -1145,8 +962,9 @@ void op_vel_def_field(nrh_f3d *h, bool update) {
 // grid `g` (steps squarings) on geometry f: fields[0..count-1] (count <= |steps| + 1).
 // The affine parts (extensions of the grid, the transformation removed) follow the squaring.
 static void intermediate_fields(const VelGrid& g, const FieldGeom& f, const VelAff& a, int steps, float *fields,
-                                int count, uint8_t *flags, hipStream_t s) {
-    launch_flow<false>(g, f, a, scaling_of(steps), fields, nullptr, flags, s);
+                                int count, hipStream_t s) {
+    hipLaunchKernelGGL(k_vel_flow_scaled, dim3(flow_blocks(f.n)), dim3(kBlock), 0, s, g, f, a, scaling_of(steps),
+                       fields);
     for (int k = 0; k + 1 < count; ++k)
         launch_compose(fields + (size_t)k * 3 * f.n, f, fields + (size_t)k * 3 * f.n, f.n,
                        fields + (size_t)(k + 1) * 3 * f.n, s);
@@ -1181,8 +999,7 @@ void op_exponentiate_gradient(nrh_f3d *h, nrh_f3d *bw, bool syncOther) {
         fields = bw->velChain.p;
     } else {
         h->velFields.alloc((size_t)sq * 3 * f.n);
-        h->velTileFlags.alloc(flow_tiles(f));
-        intermediate_fields(vel_grid(bw), f, a, steps, h->velFields.p, sq, h->velTileFlags.p, h->stream);
+        intermediate_fields(vel_grid(bw), f, a, steps, h->velFields.p, sq, h->stream);
         fields = h->velFields.p;
     }
     h->velTmp.alloc(3 * f.n);

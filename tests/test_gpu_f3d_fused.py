@@ -232,3 +232,4 @@ def test_full_size_deterministic(full_size):
     h.gradient()
     assert v1 == v2
     assert np.array_equal(g1, h.node_gradient())
+
