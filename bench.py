@@ -484,6 +484,11 @@ def measured_valu():
 
 def main():
     a = parse()
+    # stdout carries the one JSON line: whatever else the libraries print there (the reference
+    # build's warnings in the CPU baselines) goes to stderr
+    json_out = os.fdopen(os.dup(1), "w")
+    sys.stdout.flush()
+    os.dup2(2, 1)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}): every GPU box here has a single MI355X",
               file=sys.stderr)
@@ -513,7 +518,7 @@ def main():
     geom = Geom.identity(DIMS)
     ref, flo = phantoms.config1_pair(seed=0, dims=DIMS)
     if a.only_f3d2:
-        print(json.dumps(bench_f3d2(torch, ref, flo, geom, a.steps, not a.no_cpu_baseline)))
+        print(json.dumps(bench_f3d2(torch, ref, flo, geom, a.steps, not a.no_cpu_baseline)), file=json_out, flush=True)
         return
     if a.only:
         def lncc_line():
@@ -525,7 +530,7 @@ def main():
               "aladin": lambda: bench_aladin(torch, a.steps, not a.no_cpu_baseline),
               "lncc": lncc_line,
               "serve": lambda: bench_serving(torch, data["serve"])}[a.only]
-        print(json.dumps(fn()))
+        print(json.dumps(fn()), file=json_out, flush=True)
         return
     cppg, cpp = create_control_point_grid(geom, [5.0, 5.0, 5.0])
     rng = np.random.default_rng(7)
@@ -601,7 +606,7 @@ def main():
             out["secondary"] += [bench_batch_f3d(torch, data.pop("batch-f3d"), not a.no_cpu_baseline),
                                  bench_batch_aladin(torch, data.pop("batch-aladin"), not a.no_cpu_baseline),
                                  bench_serving(torch, data.pop("serve"))]
-    print(json.dumps(out))
+    print(json.dumps(out), file=json_out, flush=True)
 
 
 if __name__ == "__main__":
