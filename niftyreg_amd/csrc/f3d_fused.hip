@@ -190,9 +190,15 @@ __device__ __forceinline__ void load_taps(const FusedArgs& a, const TImg *__rest
     }
 }
 
+// GRAD = false: counts the voxel's (reference, warped) bin, or, when the warped value lies within
+// its error bound of an integer (where the reference's bin index (int)war, and the in-range test,
+// flip with the last bits), lists the voxel for the exact pass instead.  The bound: a position
+// error of up to ambTol floating voxels per axis moves the trilinear value by at most the taps'
+// spread per axis, plus eight ulps for the interpolation's own rounding.  Called by every lane.
 template<bool GRAD>
 __device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, float *dyn, double coef, float& g0,
-                                             float& g1, float& g2) {
+                                             float& g1, float& g2, uint32_t idx, uint32_t *ambList,
+                                             uint32_t *ambCount) {
     g0 = g1 = g2 = 0.f;
     if (t.rx < 0) t.rx = 0;
     if (t.ry < 0) t.ry = 0;
@@ -205,8 +211,24 @@ __device__ __forceinline__ void sweep_finish(const FusedArgs& a, VoxelTaps t, fl
     const float war = f0 * z0b + f1 * rz;
     const float rv = t.rv;
     if (!GRAD) {
-        if (t.active && rv == rv && war == war && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
-            atomicAdd(&((uint32_t *)dyn)[(int)rv + (int)war * a.rb], 1u);
+        bool edge = false;
+        if (t.active && rv == rv && war == war) {
+            float lo = t.v[0][0][0], hi = lo;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        lo = fminf(lo, t.v[c][b][e]);
+                        hi = fmaxf(hi, t.v[c][b][e]);
+                    }
+            const float tol = 3.f * (hi - lo) * a.ambTol + 8.f * 1.1920929e-7f * fabsf(war);
+            edge = fabsf(war - rintf(war)) <= tol;
+            if (!edge && rv >= 0.f && rv < (float)a.rb && war >= 0.f && war < (float)a.fb)
+                atomicAdd(&((uint32_t *)dyn)[(int)rv + (int)war * a.rb], 1u);
+        }
+        append_voxels(edge, idx, ambList, ambCount);
     } else if (t.active && rv == rv && war == war) {
         // image gradient in floating voxel units (TrilinearImageGradient)
         const float d00 = t.v[0][0][1] - t.v[0][0][0], d01 = t.v[0][1][1] - t.v[0][1][0];
@@ -627,7 +649,7 @@ __device__ __forceinline__ void sweep_tile(const FusedArgs& a, const int t, Swee
                 const int y = wave + k * (kThreads / 64);
                 if (y >= nY) continue;  // uniform over the wave
                 float g0, g1, g2;
-                sweep_finish<GRAD>(a, t[k], dyn, coef, g0, g1, g2);
+                sweep_finish<GRAD>(a, t[k], dyn, coef, g0, g1, g2, idx[k], ambList, &S.ambCount);
                 if (GRAD) {
                     a.gvox[idx[k]] = g0;
                     a.gvox[idx[k] + N] = g1;

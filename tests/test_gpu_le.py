@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracles as O
-from niftyreg_amd import phantoms
+from niftyreg_amd import _lib, phantoms
 from niftyreg_amd.f3d import F3dContent, default_params
 from niftyreg_amd.geometry import Geom, create_control_point_grid
 
@@ -92,8 +92,10 @@ def test_objective_and_gradient_with_linear_energy_match_reference_driver(gpu):
     cg = O.geom_from_nr(r.cpp_geom)
     r.close()
     assert lew > 0 and rterms[2] > 0
+    # the exact pass reproduces the build compared with (RefF3d: the default SSE build)
     h = F3dContent(g, ref.ravel(), g, flo.ravel(), cg, cpp,
-                   params=default_params(similarity_weight=simw, be_weight=bew, le_weight=lew))
+                   params=default_params(similarity_weight=simw, be_weight=bew, le_weight=lew,
+                                         field_order=_lib.FIELD_ORDER_SSE))
     h.rescale_intensities((2, 65), (2, 65))  # NMI, 64 bins: [2, bins + 1]
     v, terms = h.objective()
     assert abs(terms[2] - rterms[2]) <= 1e-6 * abs(rterms[2])
