@@ -286,8 +286,8 @@ k_block_match(const float *__restrict__ ref, const float *__restrict__ warped, c
 
 // ------------------------------------------------------------------ LTS
 struct LtsShared {
-    double red[kLtsThreads / 64][16];
-    double out[16];
+    double red[kLtsThreads / 64][24];
+    double out[24];
     uint32_t hist[2048];
     int wave[kLtsThreads / 64];
     float M[16], last[16];
@@ -395,12 +395,19 @@ __device__ void svd3(const double H[3][3], double U[3][3], double V[3][3]) {
 __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q, const uint8_t *keep, int n,
                         bool affine, LtsShared& s, const double *sums = nullptr) {
     double mp[3], mq[3];
+    double oc[15];
     if (sums) {
-        // the kept points' sums (p, q, count), accumulated by the caller in this pass's order
+        // the kept points' sums from the caller's pass: p, q, count, then the products p_a p_b
+        // (00 01 02 11 12 22) and p_a q_b; centred moments S = sum - count m m^T in FP64
+        const double cnt = sums[6];
         for (int k = 0; k < 3; ++k) {
-            mp[k] = sums[k] / sums[6];
-            mq[k] = sums[3 + k] / sums[6];
+            mp[k] = sums[k] / cnt;
+            mq[k] = sums[3 + k] / cnt;
         }
+        const int pa[6] = { 0, 0, 0, 1, 1, 2 }, pb[6] = { 0, 1, 2, 1, 2, 2 };
+        for (int k = 0; k < 6; ++k) oc[k] = sums[7 + k] - cnt * mp[pa[k]] * mp[pb[k]];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) oc[6 + a * 3 + b] = sums[13 + a * 3 + b] - cnt * mp[a] * mq[b];
     } else {
         double m[7] = { 0, 0, 0, 0, 0, 0, 0 };
         for (int i = threadIdx.x; i < n; i += kLtsThreads) {
@@ -418,20 +425,22 @@ __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q
             mq[k] = s.out[3 + k] / cnt;
         }
         __syncthreads();
+        double c[15] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+        for (int i = threadIdx.x; i < n; i += kLtsThreads) {
+            if (keep && !keep[i]) continue;
+            const double p0 = (double)P[3 * i] - mp[0], p1 = (double)P[3 * i + 1] - mp[1], p2 = (double)P[3 * i + 2] - mp[2];
+            const double q0 = (double)Q[3 * i] - mq[0], q1 = (double)Q[3 * i + 1] - mq[1], q2 = (double)Q[3 * i + 2] - mq[2];
+            c[0] += p0 * p0; c[1] += p0 * p1; c[2] += p0 * p2; c[3] += p1 * p1; c[4] += p1 * p2; c[5] += p2 * p2;
+            c[6] += p0 * q0; c[7] += p0 * q1; c[8] += p0 * q2;
+            c[9] += p1 * q0; c[10] += p1 * q1; c[11] += p1 * q2;
+            c[12] += p2 * q0; c[13] += p2 * q1; c[14] += p2 * q2;
+        }
+        wg_sum<15>(c, s);
+        for (int k = 0; k < 15; ++k) oc[k] = s.out[k];
+        __syncthreads();
     }
-    double c[15] = { 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0 };
-    for (int i = threadIdx.x; i < n; i += kLtsThreads) {
-        if (keep && !keep[i]) continue;
-        const double p0 = (double)P[3 * i] - mp[0], p1 = (double)P[3 * i + 1] - mp[1], p2 = (double)P[3 * i + 2] - mp[2];
-        const double q0 = (double)Q[3 * i] - mq[0], q1 = (double)Q[3 * i + 1] - mq[1], q2 = (double)Q[3 * i + 2] - mq[2];
-        c[0] += p0 * p0; c[1] += p0 * p1; c[2] += p0 * p2; c[3] += p1 * p1; c[4] += p1 * p2; c[5] += p2 * p2;
-        c[6] += p0 * q0; c[7] += p0 * q1; c[8] += p0 * q2;
-        c[9] += p1 * q0; c[10] += p1 * q1; c[11] += p1 * q2;
-        c[12] += p2 * q0; c[13] += p2 * q1; c[14] += p2 * q2;
-    }
-    wg_sum<15>(c, s);
     if (threadIdx.x == 0) {
-        const double *o = s.out;
+        const double *o = oc;
         // Spq[a][b] = sum (p_a - mp_a)(q_b - mq_b)
         const double Spq[3][3] = { { o[6], o[7], o[8] }, { o[9], o[10], o[11] }, { o[12], o[13], o[14] } };
         double L[3][3];
@@ -612,16 +621,23 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         }
         // keep: residual below the threshold, or equal to it within the first k in index order
         // (when every key equal to the threshold is kept, the order plays no part: one pass)
-        // the distance and the kept points' sums (p, q, count) for the next fit, each thread over
-        // its indices in ascending order as lts_fit's own first pass
-        double dv[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        // the distance and the kept points' sums for the next fit (p, q, count, p p^T, p q^T): the
+        // fit then needs no pass over the points of its own
+        double dv[23];
+        for (int c = 0; c < 23; ++c) dv[c] = 0.0;
         auto add_kept = [&](int i, uint32_t key) {
+            const double p[3] = { (double)P[3 * i], (double)P[3 * i + 1], (double)P[3 * i + 2] };
+            const double q[3] = { (double)Q[3 * i], (double)Q[3 * i + 1], (double)Q[3 * i + 2] };
             dv[0] += (double)__uint_as_float(key);
             for (int c = 0; c < 3; ++c) {
-                dv[1 + c] += (double)P[3 * i + c];
-                dv[4 + c] += (double)Q[3 * i + c];
+                dv[1 + c] += p[c];
+                dv[4 + c] += q[c];
             }
             dv[7] += 1.0;
+            dv[8] += p[0] * p[0]; dv[9] += p[0] * p[1]; dv[10] += p[0] * p[2];
+            dv[11] += p[1] * p[1]; dv[12] += p[1] * p[2]; dv[13] += p[2] * p[2];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) dv[14 + a * 3 + b] += p[a] * q[b];
         };
         if ((uint32_t)k == eqCount) {
             for (int i = tid; i < n; i += kLtsThreads) {
@@ -643,10 +659,10 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
                 seenEqual += total;
             }
         }
-        wg_sum<8>(dv, s);
+        wg_sum<23>(dv, s);
         const double distance = s.out[0];
-        double sums[7];
-        for (int c = 0; c < 7; ++c) sums[c] = s.out[1 + c];
+        double sums[22];
+        for (int c = 0; c < 22; ++c) sums[c] = s.out[1 + c];
         if (distance > lastDistance || (lastDistance - distance) < 1e-3) {
             if (tid < 16) s.M[tid] = s.last[tid];
             __syncthreads();
