@@ -55,6 +55,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMatchThreads = 384;  // 343 capture offsets rounded up to whole waves
 constexpr int kLtsThreads = 1024;
+constexpr int kLtsUnroll = 4;  // points per thread whose loads the LTS passes issue together
 
 struct Mat4 {
     float m[16];
@@ -285,9 +286,10 @@ k_block_match(const float *__restrict__ ref, const float *__restrict__ warped, c
 }
 
 // ------------------------------------------------------------------ LTS
-struct LtsShared {
+struct alignas(16) LtsShared {
     double red[kLtsThreads / 64][24];
     double out[24];
+    double fit[21];  // the fit's centred moments and means
     uint32_t hist[2048];
     int wave[kLtsThreads / 64];
     float M[16], last[16];
@@ -312,6 +314,29 @@ __device__ void wg_sum(double (&v)[K], LtsShared& s) {
         s.out[threadIdx.x] = r;
     }
     __syncthreads();
+}
+
+// Requires every value of a batch of loads in registers at this point, so that the batch's loads
+// are all issued before the first is used (the compiler otherwise sinks each load into the
+// conditional code that uses it, and the loads wait one after another).
+template<int U>
+__device__ __forceinline__ void batch_loaded(const float (&v)[U][6]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) asm volatile("" : : "v"(v[u][c]));
+}
+
+// Exclusive prefix sum over the 64 lanes of a wave (all lanes active).
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t up = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += up;
+    }
+    return incl - v;
 }
 
 // Exclusive prefix of a 0/1 flag over the workgroup (lane order = index order); returns the
@@ -387,28 +412,107 @@ __device__ void svd3(const double H[3][3], double U[3][3], double V[3][3]) {
         for (int j = 0; j < 3; ++j) V[i][j] = V2[i][j];
 }
 
-// Least-squares fit of the kept correspondences (keep == nullptr: all n) into s.M.
+// Thread 0's half of the fit: the 3x3 solve from s.fit (the centred moments Spp 00 01 02 11 12 22,
+// Spq row-major, then the means of p and q) into s.M and s.scal[0].  Out of line, so that its
+// registers do not add to those of the passes over the points.
+__device__ __noinline__ void lts_solve(LtsShared& s, bool affine) {
+    const double *o = s.fit, *mp = s.fit + 15, *mq = s.fit + 18;
+    // Spq[a][b] = sum (p_a - mp_a)(q_b - mq_b)
+    const double Spq[3][3] = { { o[6], o[7], o[8] }, { o[9], o[10], o[11] }, { o[12], o[13], o[14] } };
+    double L[3][3];
+    int ok = 1;
+    if (affine) {
+        // Spp X = Spq by Gaussian elimination with partial pivoting; L = X^T.
+        double a[3][6] = { { o[0], o[1], o[2], Spq[0][0], Spq[0][1], Spq[0][2] },
+                           { o[1], o[3], o[4], Spq[1][0], Spq[1][1], Spq[1][2] },
+                           { o[2], o[4], o[5], Spq[2][0], Spq[2][1], Spq[2][2] } };
+        const double scale = fmax(fabs(o[0]), fmax(fabs(o[3]), fabs(o[5])));
+        // fully unrolled, the pivot row swapped in by selects: a stays in registers
+#pragma unroll
+        for (int col = 0; col < 3; ++col) {
+            int piv = col;
+            double best = fabs(a[col][col]);
+#pragma unroll
+            for (int r = col + 1; r < 3; ++r)
+                if (fabs(a[r][col]) > best) {
+                    best = fabs(a[r][col]);
+                    piv = r;
+                }
+            if (!(best > 1e-12 * scale)) ok = 0;
+#pragma unroll
+            for (int r = col + 1; r < 3; ++r)
+                if (piv == r)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const double tmp = a[col][k];
+                        a[col][k] = a[r][k];
+                        a[r][k] = tmp;
+                    }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (r == col) continue;
+                const double f = a[r][col] / a[col][col];
+#pragma unroll
+                for (int k = col; k < 6; ++k) a[r][k] -= f * a[col][k];
+            }
+        }
+        if (ok)
+            for (int r = 0; r < 3; ++r)
+                for (int k = 0; k < 3; ++k) L[k][r] = a[r][3 + k] / a[r][r];
+    } else {
+        double U[3][3], V[3][3];
+        svd3(Spq, U, V);
+        auto vut = [&] {
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) L[a][b] = V[a][0] * U[b][0] + V[a][1] * U[b][1] + V[a][2] * U[b][2];
+        };
+        vut();
+        const double det = L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) -
+                           L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0]) +
+                           L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]);
+        if (det < 0) {
+            for (int i = 0; i < 3; ++i) V[i][2] = -V[i][2];
+            vut();
+        }
+    }
+    if (ok) {
+        for (int r = 0; r < 3; ++r) {
+            for (int k = 0; k < 3; ++k) s.M[r * 4 + k] = (float)L[r][k];
+            s.M[r * 4 + 3] = (float)(mq[r] - (L[r][0] * mp[0] + L[r][1] * mp[1] + L[r][2] * mp[2]));
+        }
+        s.M[12] = s.M[13] = s.M[14] = 0.f;
+        s.M[15] = 1.f;
+    }
+    s.scal[0] = ok;
+}
+
+// Least-squares fit of the kept correspondences (keep == nullptr: all n; fromSums:
+// from the sums the caller's keep pass left in s.out) into s.M.
 // Affine: the FP64 solution of [p 1] P = q through centred normal equations (the same minimiser
 // as EstimateAffineLeastSquares' QR, reg-lib/cpu/Maths.cpp:325-349).  Rigid: Kabsch on the centred
 // cross-covariance with the reflection guard (reg-lib/cpu/Maths.cpp:351-390).  Returns false
 // (uniformly) when the normal matrix is singular.
 __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q, const uint8_t *keep, int n,
-                        bool affine, LtsShared& s, const double *sums = nullptr) {
-    double mp[3], mq[3];
-    double oc[15];
-    if (sums) {
-        // the kept points' sums from the caller's pass: p, q, count, then the products p_a p_b
-        // (00 01 02 11 12 22) and p_a q_b; centred moments S = sum - count m m^T in FP64
-        const double cnt = sums[6];
-        for (int k = 0; k < 3; ++k) {
-            mp[k] = sums[k] / cnt;
-            mq[k] = sums[3 + k] / cnt;
+                        bool affine, LtsShared& s, bool fromSums = false) {
+    if (fromSums) {
+        // the kept points' sums from the caller's pass, in s.out[1..22]: p, q, count, then the
+        // products p_a p_b (00 01 02 11 12 22) and p_a q_b; centred moments S = sum - count m m^T
+        // in FP64
+        if (threadIdx.x == 0) {
+            const double *sums = s.out + 1;
+            double *oc = s.fit, *mp = s.fit + 15, *mq = s.fit + 18;
+            const double cnt = sums[6];
+            for (int k = 0; k < 3; ++k) {
+                mp[k] = sums[k] / cnt;
+                mq[k] = sums[3 + k] / cnt;
+            }
+            const int pa[6] = { 0, 0, 0, 1, 1, 2 }, pb[6] = { 0, 1, 2, 1, 2, 2 };
+            for (int k = 0; k < 6; ++k) oc[k] = sums[7 + k] - cnt * mp[pa[k]] * mp[pb[k]];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) oc[6 + a * 3 + b] = sums[13 + a * 3 + b] - cnt * mp[a] * mq[b];
         }
-        const int pa[6] = { 0, 0, 0, 1, 1, 2 }, pb[6] = { 0, 1, 2, 1, 2, 2 };
-        for (int k = 0; k < 6; ++k) oc[k] = sums[7 + k] - cnt * mp[pa[k]] * mp[pb[k]];
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) oc[6 + a * 3 + b] = sums[13 + a * 3 + b] - cnt * mp[a] * mq[b];
     } else {
+        double mp[3], mq[3];
         double m[7] = { 0, 0, 0, 0, 0, 0, 0 };
         for (int i = threadIdx.x; i < n; i += kLtsThreads) {
             if (keep && !keep[i]) continue;
@@ -436,79 +540,15 @@ __device__ bool lts_fit(const float *__restrict__ P, const float *__restrict__ Q
             c[12] += p2 * q0; c[13] += p2 * q1; c[14] += p2 * q2;
         }
         wg_sum<15>(c, s);
-        for (int k = 0; k < 15; ++k) oc[k] = s.out[k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double *o = oc;
-        // Spq[a][b] = sum (p_a - mp_a)(q_b - mq_b)
-        const double Spq[3][3] = { { o[6], o[7], o[8] }, { o[9], o[10], o[11] }, { o[12], o[13], o[14] } };
-        double L[3][3];
-        int ok = 1;
-        if (affine) {
-            // Spp X = Spq by Gaussian elimination with partial pivoting; L = X^T.
-            double a[3][6] = { { o[0], o[1], o[2], Spq[0][0], Spq[0][1], Spq[0][2] },
-                               { o[1], o[3], o[4], Spq[1][0], Spq[1][1], Spq[1][2] },
-                               { o[2], o[4], o[5], Spq[2][0], Spq[2][1], Spq[2][2] } };
-            const double scale = fmax(fabs(o[0]), fmax(fabs(o[3]), fabs(o[5])));
-            // fully unrolled, the pivot row swapped in by selects: a stays in registers
-#pragma unroll
-            for (int col = 0; col < 3; ++col) {
-                int piv = col;
-                double best = fabs(a[col][col]);
-#pragma unroll
-                for (int r = col + 1; r < 3; ++r)
-                    if (fabs(a[r][col]) > best) {
-                        best = fabs(a[r][col]);
-                        piv = r;
-                    }
-                if (!(best > 1e-12 * scale)) ok = 0;
-#pragma unroll
-                for (int r = col + 1; r < 3; ++r)
-                    if (piv == r)
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            const double tmp = a[col][k];
-                            a[col][k] = a[r][k];
-                            a[r][k] = tmp;
-                        }
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    if (r == col) continue;
-                    const double f = a[r][col] / a[col][col];
-#pragma unroll
-                    for (int k = col; k < 6; ++k) a[r][k] -= f * a[col][k];
-                }
-            }
-            if (ok)
-                for (int r = 0; r < 3; ++r)
-                    for (int k = 0; k < 3; ++k) L[k][r] = a[r][3 + k] / a[r][r];
-        } else {
-            double U[3][3], V[3][3];
-            svd3(Spq, U, V);
-            auto vut = [&] {
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) L[a][b] = V[a][0] * U[b][0] + V[a][1] * U[b][1] + V[a][2] * U[b][2];
-            };
-            vut();
-            const double det = L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) -
-                               L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0]) +
-                               L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]);
-            if (det < 0) {
-                for (int i = 0; i < 3; ++i) V[i][2] = -V[i][2];
-                vut();
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 15; ++k) s.fit[k] = s.out[k];
+            for (int k = 0; k < 3; ++k) {
+                s.fit[15 + k] = mp[k];
+                s.fit[18 + k] = mq[k];
             }
         }
-        if (ok) {
-            for (int r = 0; r < 3; ++r) {
-                for (int k = 0; k < 3; ++k) s.M[r * 4 + k] = (float)L[r][k];
-                s.M[r * 4 + 3] = (float)(mq[r] - (L[r][0] * mp[0] + L[r][1] * mp[1] + L[r][2] * mp[2]));
-            }
-            s.M[12] = s.M[13] = s.M[14] = 0.f;
-            s.M[15] = 1.f;
-        }
-        s.scal[0] = ok;
     }
+    if (threadIdx.x == 0) lts_solve(s, affine);
     __syncthreads();
     return s.scal[0] != 0;
 }
@@ -564,12 +604,29 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         // residuals, counted into the first digit's histogram as they are formed
         for (int b = tid; b < 2048; b += kLtsThreads) s.hist[b] = 0;
         __syncthreads();
-        for (int i = tid; i < n; i += kLtsThreads) {
-            const float3 t = mat44_mul_f(s.M, P[3 * i], P[3 * i + 1], P[3 * i + 2]);
-            const float dx = t.x - Q[3 * i], dy = t.y - Q[3 * i + 1], dz = t.z - Q[3 * i + 2];
-            const uint32_t key = __float_as_uint(sqrtf(dx * dx + dy * dy + dz * dz));
-            res[i] = key;
-            atomicAdd(&s.hist[key >> 21], 1u);
+        // kLtsUnroll points per thread per step: their loads are issued together (indexes
+        // clamped to n - 1, so every load stays in bounds), then the residuals formed
+        for (int i0 = tid; i0 < n; i0 += kLtsUnroll * kLtsThreads) {
+            float pq[kLtsUnroll][6];
+#pragma unroll
+            for (int u = 0; u < kLtsUnroll; ++u) {
+                const int i = min(i0 + u * kLtsThreads, n - 1);
+                for (int c = 0; c < 3; ++c) {
+                    pq[u][c] = P[3 * i + c];
+                    pq[u][3 + c] = Q[3 * i + c];
+                }
+            }
+            batch_loaded(pq);
+#pragma unroll
+            for (int u = 0; u < kLtsUnroll; ++u) {
+                const int i = i0 + u * kLtsThreads;
+                if (i >= n) break;
+                const float3 t = mat44_mul_f(s.M, pq[u][0], pq[u][1], pq[u][2]);
+                const float dx = t.x - pq[u][3], dy = t.y - pq[u][4], dz = t.z - pq[u][5];
+                const uint32_t key = __float_as_uint(sqrtf(dx * dx + dy * dy + dz * dz));
+                res[i] = key;
+                atomicAdd(&s.hist[key >> 21], 1u);
+            }
         }
         __syncthreads();
         // radix select of the keep-th smallest residual: digits of 11, 11 and 10 bits
@@ -589,27 +646,26 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
                 __syncthreads();
             }
             if (tid < 64) {
+                // the group of 32 bins holding the k-th key (prefix sums over the 64 groups),
+                // then the bin inside it (prefix sums over the group's 32 bins), both found
+                // by a ballot: the first bin whose inclusive count reaches k
+                const uint4 *grp = reinterpret_cast<const uint4 *>(&s.hist[tid * 32]);
                 uint32_t local = 0;
-                for (int b = 0; b < 32; ++b) local += s.hist[tid * 32 + b];
-                uint32_t incl = local;
-                for (int off = 1; off < 64; off <<= 1) {
-                    const uint32_t up = __shfl_up(incl, off, 64);
-                    if (tid >= off) incl += up;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const uint4 v = grp[b];
+                    local += v.x + v.y + v.z + v.w;
                 }
-                const uint32_t excl = incl - local;
-                if (excl < (uint32_t)k && (uint32_t)k <= incl) {
-                    uint32_t run = excl;
-                    int digit = tid * 32;
-                    for (int b = 0; b < 32; ++b) {
-                        const uint32_t h = s.hist[tid * 32 + b];
-                        if (run + h >= (uint32_t)k) {
-                            digit = tid * 32 + b;
-                            break;
-                        }
-                        run += h;
-                    }
-                    s.scal[1] = digit;
-                    s.scal[2] = k - (int)run;
+                const uint32_t excl = wave_excl_scan(local);
+                const unsigned long long inGroup = __ballot(excl < (uint32_t)k && (uint32_t)k <= excl + local);
+                const int g = inGroup ? __ffsll((long long)inGroup) - 1 : 0;
+                const uint32_t base = __shfl(excl, g, 64);
+                const uint32_t h = tid < 32 ? s.hist[g * 32 + tid] : 0u;
+                const uint32_t before = base + wave_excl_scan(h);
+                const unsigned long long inBin = __ballot(tid < 32 && before + h >= (uint32_t)k);
+                if (tid == (inBin ? __ffsll((long long)inBin) - 1 : 0)) {
+                    s.scal[1] = g * 32 + tid;
+                    s.scal[2] = k - (int)before;
                 }
             }
             __syncthreads();
@@ -625,9 +681,9 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         // fit then needs no pass over the points of its own
         double dv[23];
         for (int c = 0; c < 23; ++c) dv[c] = 0.0;
-        auto add_kept = [&](int i, uint32_t key) {
-            const double p[3] = { (double)P[3 * i], (double)P[3 * i + 1], (double)P[3 * i + 2] };
-            const double q[3] = { (double)Q[3 * i], (double)Q[3 * i + 1], (double)Q[3 * i + 2] };
+        auto add_point = [&](const float *pf, const float *qf, uint32_t key) {
+            const double p[3] = { (double)pf[0], (double)pf[1], (double)pf[2] };
+            const double q[3] = { (double)qf[0], (double)qf[1], (double)qf[2] };
             dv[0] += (double)__uint_as_float(key);
             for (int c = 0; c < 3; ++c) {
                 dv[1 + c] += p[c];
@@ -639,12 +695,28 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) dv[14 + a * 3 + b] += p[a] * q[b];
         };
+        auto add_kept = [&](int i, uint32_t key) { add_point(P + 3 * i, Q + 3 * i, key); };
         if ((uint32_t)k == eqCount) {
-            for (int i = tid; i < n; i += kLtsThreads) {
-                const uint32_t key = res[i];
-                const bool kept = key <= prefix;
-                keepFlag[i] = kept ? 1 : 0;
-                if (kept) add_kept(i, key);
+            for (int i0 = tid; i0 < n; i0 += kLtsUnroll * kLtsThreads) {
+                float pq[kLtsUnroll][6];
+#pragma unroll
+                for (int u = 0; u < kLtsUnroll; ++u) {
+                    const int i = min(i0 + u * kLtsThreads, n - 1);
+                    for (int c = 0; c < 3; ++c) {
+                        pq[u][c] = P[3 * i + c];
+                        pq[u][3 + c] = Q[3 * i + c];
+                    }
+                }
+                batch_loaded(pq);
+#pragma unroll
+                for (int u = 0; u < kLtsUnroll; ++u) {
+                    const int i = i0 + u * kLtsThreads;
+                    if (i >= n) break;
+                    const uint32_t key = res[i];
+                    const bool kept = key <= prefix;
+                    keepFlag[i] = kept ? 1 : 0;
+                    if (kept) add_point(pq[u], pq[u] + 3, key);
+                }
             }
         } else {
             int seenEqual = 0;
@@ -661,8 +733,6 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         }
         wg_sum<23>(dv, s);
         const double distance = s.out[0];
-        double sums[22];
-        for (int c = 0; c < 22; ++c) sums[c] = s.out[1 + c];
         if (distance > lastDistance || (lastDistance - distance) < 1e-3) {
             if (tid < 16) s.M[tid] = s.last[tid];
             __syncthreads();
@@ -671,7 +741,7 @@ lts_body(const float *__restrict__ refPos, const float *__restrict__ warpPos, in
         lastDistance = distance;
         if (tid < 16) s.last[tid] = s.M[tid];
         __syncthreads();
-        if (!lts_fit(P, Q, keepFlag, n, affine != 0, s, sums)) {
+        if (!lts_fit(P, Q, keepFlag, n, affine != 0, s, true)) {
             if (tid == 0) *status = 2;
             return;
         }
