@@ -298,15 +298,35 @@ struct alignas(16) LtsShared {
 };
 
 // Sum of K doubles over the workgroup in a fixed order; the result lands in s.out[0..K).
+// Inside a wave the values are halved at each exchange: the lanes whose exchange bit is clear keep
+// the lower half of the (zero-padded, power-of-two) set and send the upper half, the others the
+// reverse, so the wave makes W/2 + W/4 + ... + 1 exchanges instead of K per lane bit; after log2 W
+// of them lane l holds value (l >> (6 - log2 W)) summed over its group, and the remaining lane bits
+// are summed by plain exchanges.
 template<int K>
 __device__ void wg_sum(double (&v)[K], LtsShared& s) {
+    static_assert(K <= 32, "at most 32 values");
+    constexpr int W = K <= 8 ? 8 : K <= 16 ? 16 : 32;
+    constexpr int LW = W == 8 ? 3 : W == 16 ? 4 : 5;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double a[W];
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
-    if (lane == 0)
+    for (int k = 0; k < W; ++k) a[k] = k < K ? v[k] : 0.0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) s.red[wv][k] = v[k];
+    for (int st = 0; st < LW; ++st) {
+        const int off = 32 >> st, half = W >> (st + 1);
+        const bool upper = (lane & off) != 0;
+#pragma unroll
+        for (int j = 0; j < half; ++j) {
+            const double send = upper ? a[j] : a[j + half];
+            const double keep = upper ? a[j + half] : a[j];
+            a[j] = keep + __shfl_xor(send, off, 64);
+        }
+    }
+#pragma unroll
+    for (int off = 32 >> LW; off > 0; off >>= 1) a[0] += __shfl_xor(a[0], off, 64);
+    const int idx = (lane >> (6 - LW)) & (W - 1);
+    if ((lane & ((1 << (6 - LW)) - 1)) == 0 && idx < K) s.red[wv][idx] = a[0];
     __syncthreads();
     if (threadIdx.x < K) {
         double r = 0;
