@@ -347,16 +347,18 @@ __device__ __forceinline__ void batch_loaded(const float (&v)[U][6]) {
         for (int c = 0; c < 6; ++c) asm volatile("" : : "v"(v[u][c]));
 }
 
-// Exclusive prefix sum over the 64 lanes of a wave (all lanes active).
+// Exclusive prefix sum over the 64 lanes of a wave (all lanes active), by data-parallel
+// primitives (DPP): shifts right by 1, 2, 4 and 8 lanes inside each row of 16 (lanes with no source
+// add 0), then lane 15 of each row added to the next row and lane 31 to the upper half.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-    uint32_t incl = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t up = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += up;
-    }
-    return incl - v;
+    int incl = (int)v;
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, false);  // row_shr:1
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, false);  // row_shr:2
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, false);  // row_shr:4
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, false);  // row_shr:8
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)incl - v;
 }
 
 // Exclusive prefix of a 0/1 flag over the workgroup (lane order = index order); returns the
