@@ -217,28 +217,30 @@ k_block_match(const float *__restrict__ ref, const float *__restrict__ warped, c
     float cc = -1.f;
     if (t < 343) {
         const int l = t % 7, m = (t / 7) % 7, n = t / 49;
-        float w[64];
-#pragma unroll
-        for (int a = 0; a < 64; ++a) w[a] = sW[((a >> 4) + n) * 100 + (((a >> 2) & 3) + m) * 10 + (a & 3) + l];
+        // the lane's 64 warped values are read from LDS in each pass (constant offsets from one
+        // base) rather than held in registers: the kernel then fits several workgroups per CU
+        const float *wb = sW + n * 100 + m * 10 + l;
+        auto W = [&](int a) { return wb[(a >> 4) * 100 + ((a >> 2) & 3) * 10 + (a & 3)]; };
         float rm = 0.f, wm = 0.f, cnt = 0.f;
 #pragma unroll
         for (int a = 0; a < 64; ++a) {
             const float r = sR[a];
-            if (r == r && w[a] == w[a]) {
+            if (r == r && W(a) == W(a)) {
                 rm += r;
-                wm += w[a];
+                wm += W(a);
                 cnt += 1.f;
             }
         }
         if (cnt > 32.f) {
+            asm volatile("" ::: "memory");  // the second pass reads LDS again
             rm /= cnt;
             wm /= cnt;
             float rv = 0.f, wv = 0.f, c = 0.f;
 #pragma unroll
             for (int a = 0; a < 64; ++a) {
                 const float r = sR[a];
-                if (r == r && w[a] == w[a]) {
-                    const float rt = r - rm, wt = w[a] - wm;
+                if (r == r && W(a) == W(a)) {
+                    const float rt = r - rm, wt = W(a) - wm;
                     rv += rt * rt;
                     wv += wt * wt;
                     c += rt * wt;
