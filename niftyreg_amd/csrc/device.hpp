@@ -348,6 +348,36 @@ __device__ __forceinline__ double block_sum_double(double v, double *smem) {
     return r;  // valid on thread 0
 }
 
+// Three block sums at once, each in the same order as block_sum_double (smem: 48 doubles); the
+// results are valid on thread 0.
+__device__ __forceinline__ void block_sum_double3(double& a, double& b, double& c, double *smem) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        smem[wid] = a;
+        smem[16 + wid] = b;
+        smem[32 + wid] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double ra = 0, rb = 0, rc = 0;
+        for (int i = 0; i < nw; ++i) {
+            ra += smem[i];
+            rb += smem[16 + i];
+            rc += smem[32 + i];
+        }
+        a = ra;
+        b = rb;
+        c = rc;
+    }
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;

@@ -279,7 +279,7 @@ __device__ __forceinline__ void fin_probabilities(uint32_t *__restrict__ counts,
 
 // Phase 2: the logarithms (written with the probabilities), the entropies (each thread's strided
 // terms, then block sums: H_r, H_f, H_rf), ent[] and the table K.  P holds the logs after.
-__device__ __forceinline__ void fin_entropies(int rb, int fb, double *P, double *red, double& totalS,
+__device__ __forceinline__ void fin_entropies(int rb, int fb, double *P, double *red /* 48 */, double& totalS,
                                               double *__restrict__ pro, double *__restrict__ logh,
                                               float *__restrict__ logf, double *__restrict__ ent,
                                               float *__restrict__ kTab) {
@@ -300,11 +300,7 @@ __device__ __forceinline__ void fin_entropies(int rb, int fb, double *P, double 
         logf[i] = (float)l;
         P[i] = l;  // the logs stay in LDS for the table K
     }
-    hr = block_sum_double(hr, red);
-    __syncthreads();
-    hf = block_sum_double(hf, red);
-    __syncthreads();
-    hj = block_sum_double(hj, red);
+    block_sum_double3(hr, hf, hj, red);
     if (threadIdx.x == 0) {
         ent[0] = hr;
         ent[1] = hf;
@@ -328,7 +324,7 @@ nmi_finalize_lds_block(uint32_t *__restrict__ counts, int rb, int fb, double *__
                    double *__restrict__ logh, float *__restrict__ logf, double *__restrict__ ent,
                    float *__restrict__ kTab = nullptr) {
     __shared__ double P[kFinLds];
-    __shared__ double red[16];
+    __shared__ double red[48];
     __shared__ double totalS;
     fin_probabilities(counts, true, rb, fb, P, red, totalS);
     fin_entropies(rb, fb, P, red, totalS, pro, logh, logf, ent, kTab);
